@@ -1,0 +1,85 @@
+/*
+ * klt.h -- C ABI of the MI355X imaging library (libklt.so).
+ *
+ * The library is a drop-in for K-Litho's imaging hot path.  K-Litho has no
+ * library API of its own: its two tools call internal C++ functions that take
+ * std::vector buffers.  Each compute entry point below replaces one of those
+ * functions and is cited against it; the klitho_tcc / klitho_socs tools in
+ * this package call these entry points in place of the reference's ones.
+ *
+ * Conventions
+ *   - Every function returns KLT_OK (0) or a negative KLT_ERR_* code;
+ *     klt_last_error() gives the message of the calling thread's last error.
+ *   - Arrays are row-major.  Complex arrays are interleaved (re, im).
+ *   - Array arguments are device pointers unless KLT_HOST_PTRS is passed in
+ *     `flags`; then they are host pointers and the library stages them.
+ *   - A context owns one HIP stream (or a caller-provided one), device
+ *     buffers, twiddle tables and plans.  One context per host thread.
+ *   - Frequency index of a kernel / TCC entry: ID(nx,ny) =
+ *     (ny+Ny)*(2Nx+1) + (nx+Nx), |nx|<=Nx, |ny|<=Ny
+ *     (K-Litho-TCC/klitho_tcc.cpp:221,245).
+ */
+#ifndef KLT_H
+#define KLT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KLT_OK 0
+#define KLT_ERR_ARG -1      /* invalid argument (reference: exit 1 on bad argv) */
+#define KLT_ERR_SIZE -2     /* 4N+1 > L (reference: "ERROR in FI()", klitho_socs.cpp:166-169) */
+#define KLT_ERR_HIP -3      /* HIP runtime failure */
+#define KLT_ERR_NODEV -4    /* no usable GPU */
+#define KLT_ERR_IO -5       /* file could not be opened / read / written */
+#define KLT_ERR_NOMEM -6    /* allocation failure (reference: bad_alloc, klitho_tcc.cpp:368-371) */
+#define KLT_ERR_EIG -7      /* eigensolver did not converge (reference: zheevr info != 0) */
+
+#define KLT_HOST_PTRS 1u    /* array arguments are host memory */
+
+typedef struct klt_ctx klt_ctx;
+typedef struct klt_kernels klt_kernels;
+
+/* ---- context --------------------------------------------------------- */
+const char* klt_last_error(void);
+int klt_version(void);
+/* Number of visible GPUs (0 when none). */
+int klt_device_count(void);
+int klt_ctx_create(int device, klt_ctx** out);
+void klt_ctx_destroy(klt_ctx* ctx);
+/* Use `stream` (a hipStream_t) for all work of this context; NULL restores
+ * the context's own stream.  The caller keeps ownership. */
+int klt_ctx_set_stream(klt_ctx* ctx, void* stream);
+void* klt_ctx_stream(klt_ctx* ctx);
+int klt_ctx_sync(klt_ctx* ctx);
+/* Per-kernel timing (no reference counterpart; the reference prints only a
+ * whole-run wall clock, klitho_socs.cpp:355-361).  While on, every kernel
+ * launch is bracketed by HIP events on the context stream. */
+int klt_ctx_set_profiling(klt_ctx* ctx, int on);
+/* Writes one "name calls total_ms" line per kernel launched since the last
+ * report and clears the record; returns the byte count or an error. */
+int klt_ctx_profile_report(klt_ctx* ctx, char* buf, int len);
+
+/* ---- SOCS kernel bank --------------------------------------------------
+ * Replaces the kernel read loop of klitho_socs.cpp:297-318: nk kernels of
+ * sizeX x sizeY (odd) in the krn_i_{r,i}.bin layout, and their eigenvalues
+ * (kernel_info.txt).  The bank is packed on the device once. */
+int klt_kernels_create(klt_ctx* ctx, const float* krn_r, const float* krn_i,
+                       const float* scales, int nk, int sizeX, int sizeY,
+                       unsigned flags, klt_kernels** out);
+void klt_kernels_destroy(klt_kernels* kb);
+
+/* ---- SOCS imaging (klitho_socs.cpp:338-343: FT_r2c + calcSOCS) ---------
+ * image(x) = sum_k scales[k] |sum_n K_k[n] M^[n] e^{+2 pi i n.(x-c)/L}|^2 with
+ * M^ the mask spectrum of FT_r2c (klitho_socs.cpp:71-121).  mask and image
+ * are Ly x Lx float.  Errors with KLT_ERR_SIZE when 4N+1 exceeds Lx or Ly. */
+int klt_socs_image(klt_ctx* ctx, const klt_kernels* kb, const float* mask, int Lx,
+                   int Ly, float* image, unsigned flags);
+/* Same for nmask masks of one size, stored back to back. */
+int klt_socs_forward(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
+                     int nmask, int Lx, int Ly, float* images, unsigned flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KLT_H */

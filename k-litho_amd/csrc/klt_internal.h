@@ -1,0 +1,111 @@
+// klt_internal.h -- shared host-side plumbing of libklt.so: error state,
+// the context object, device buffer helpers and HIP error checks.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/klt.h"
+
+namespace klt {
+
+// Records `msg` as the calling thread's last error and returns `code`.
+int set_error(int code, const std::string& msg);
+
+#define KLT_HIP(expr)                                                         \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      return ::klt::set_error(KLT_ERR_HIP, std::string(#expr) + ": " +        \
+                                               hipGetErrorString(e_));        \
+  } while (0)
+
+#define KLT_TRY(expr)          \
+  do {                         \
+    int rc_ = (expr);          \
+    if (rc_ != KLT_OK) return rc_; \
+  } while (0)
+
+// Growable device buffer owned by a context.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need);
+  void release();
+  ~DevBuf() { release(); }
+  template <class T>
+  T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Twiddle tables of one SOCS geometry (Lx, Ly, Nx, Ny).
+// Rows of the input tables are padded with zeros to NP = round_up(max(Nx,Ny)+1, 8)
+// entries and the output-row table to G = round_up(2Ny+1, 8), so that the
+// register-blocked passes can run fixed-size loops.
+struct SocsPlan {
+  int Lx = 0, Ly = 0, Nx = 0, Ny = 0, M = 0, NP = 0, G = 0;
+  DevBuf twy_in;   // [Ly][NP]    e^{-2 pi i ny (y - floor(Ly/2))/Ly}, ny < NP
+  DevBuf twx_in;   // [Lx][NP]    e^{-2 pi i nx (x - floor(Lx/2))/Lx}, nx < NP
+  DevBuf twx_out;  // [Lx][2Nx+1] e^{+2 pi i gx (x - (Lx+1)/2)/Lx}
+  DevBuf twy_out;  // [Ly][G]     e^{+2 pi i gy (y - (Ly+1)/2)/Ly}, gy < G
+  DevBuf twM;      // [M]         e^{+2 pi i j/M}
+};
+
+}  // namespace klt
+
+struct klt_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  // per-kernel timing (klt_ctx_set_profiling): events bracketing launches
+  bool profiling = false;
+  struct TimedLaunch {
+    const char* name;
+    hipEvent_t start, stop;
+  };
+  std::vector<TimedLaunch> timed;
+  std::vector<hipEvent_t> event_pool;
+  hipEvent_t take_event();
+  // scratch reused across calls (grown on demand)
+  klt::DevBuf stage_in, stage_out, qpart, band, spec, ccols, misc;
+  std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::SocsPlan>> socs_plans;
+  int activate() const;  // hipSetDevice(device)
+};
+
+struct klt_kernels {
+  klt_ctx* ctx = nullptr;
+  int nk = 0, Nx = 0, Ny = 0;
+  klt::DevBuf kern;    // [nk][2Ny+1][2Nx+1] float2
+  klt::DevBuf scales;  // [nk] float
+  std::vector<float> h_scales;
+};
+
+// Launches `kernel` on the context stream; with profiling on, the launch is
+// bracketed by HIP events and recorded under `name`.
+#define KLT_LAUNCH(ctx, name, kernel, grid, block, lds, ...)                        \
+  do {                                                                              \
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;                                      \
+    if ((ctx)->profiling) {                                                         \
+      ev0_ = (ctx)->take_event();                                                   \
+      ev1_ = (ctx)->take_event();                                                   \
+      (void)hipEventRecord(ev0_, (ctx)->stream);                                    \
+    }                                                                               \
+    hipLaunchKernelGGL(kernel, grid, block, lds, (ctx)->stream, __VA_ARGS__);       \
+    if ((ctx)->profiling) {                                                         \
+      (void)hipEventRecord(ev1_, (ctx)->stream);                                    \
+      (ctx)->timed.push_back({name, ev0_, ev1_});                                   \
+    }                                                                               \
+  } while (0)
+
+namespace klt {
+// SOCS geometry plan (cached in the context).
+int get_socs_plan(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, SocsPlan** out);
+// Copy helpers honouring KLT_HOST_PTRS.
+int to_device(klt_ctx* ctx, const void* src, size_t bytes, unsigned flags, DevBuf& stage,
+              const void** dptr);
+}  // namespace klt

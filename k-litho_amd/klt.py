@@ -1,0 +1,195 @@
+"""ctypes binding of libklt.so (the C ABI declared in include/klt.h).
+
+Array arguments may be numpy arrays (host memory: passed with KLT_HOST_PTRS)
+or torch tensors on the GPU (device memory: passed as raw pointers).  Every
+call raises KltError with the library's message on a non-zero status.  The
+library must have been built (k-litho_amd/build.py); there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libklt.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "klt.h")
+
+KLT_OK = 0
+KLT_HOST_PTRS = 1
+
+ERRORS = {-1: "KLT_ERR_ARG", -2: "KLT_ERR_SIZE", -3: "KLT_ERR_HIP", -4: "KLT_ERR_NODEV",
+          -5: "KLT_ERR_IO", -6: "KLT_ERR_NOMEM", -7: "KLT_ERR_EIG"}
+
+
+class KltError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+        self.msg = msg
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libklt.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run k-litho_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        vp, i, u, fp = C.c_void_p, C.c_int, C.c_uint, C.c_void_p
+        L.klt_last_error.restype = C.c_char_p
+        L.klt_version.restype = i
+        L.klt_device_count.restype = i
+        L.klt_ctx_create.argtypes = [i, C.POINTER(vp)]
+        L.klt_ctx_destroy.argtypes = [vp]
+        L.klt_ctx_destroy.restype = None
+        L.klt_ctx_set_stream.argtypes = [vp, vp]
+        L.klt_ctx_stream.argtypes = [vp]
+        L.klt_ctx_stream.restype = vp
+        L.klt_ctx_sync.argtypes = [vp]
+        L.klt_ctx_set_profiling.argtypes = [vp, i]
+        L.klt_ctx_profile_report.argtypes = [vp, C.c_char_p, i]
+        L.klt_kernels_create.argtypes = [vp, fp, fp, fp, i, i, i, u, C.POINTER(vp)]
+        L.klt_kernels_destroy.argtypes = [vp]
+        L.klt_kernels_destroy.restype = None
+        L.klt_socs_image.argtypes = [vp, vp, fp, i, i, fp, u]
+        L.klt_socs_forward.argtypes = [vp, vp, fp, i, i, i, fp, u]
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != KLT_OK:
+        raise KltError(rc, lib().klt_last_error().decode(errors="replace"))
+
+
+def declared_symbols() -> list[str]:
+    """Names of the functions include/klt.h declares."""
+    import re
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(klt_[a-z0-9_]+)\s*\(", txt)))
+
+
+def _is_torch(a) -> bool:
+    return type(a).__module__.startswith("torch")
+
+
+def _ptr(a, dtype):
+    """(pointer, is_host) for a numpy array or a torch GPU tensor."""
+    if _is_torch(a):
+        import torch
+        want = {np.float32: torch.float32, np.float64: torch.float64,
+                np.int32: torch.int32}[dtype]
+        if not a.is_cuda or a.dtype != want or not a.is_contiguous():
+            raise TypeError(f"expected a contiguous {want} GPU tensor")
+        return C.c_void_p(a.data_ptr()), False
+    if not isinstance(a, np.ndarray) or a.dtype != dtype or not a.flags.c_contiguous:
+        raise TypeError(f"expected a C-contiguous numpy {np.dtype(dtype)} array")
+    return a.ctypes.data_as(C.c_void_p), True
+
+
+def _flags(*hosts) -> int:
+    if all(hosts):
+        return KLT_HOST_PTRS
+    if any(hosts):
+        raise TypeError("mix of host and device arrays in one call")
+    return 0
+
+
+class Context:
+    """One libklt context (device, stream, plans, scratch)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        check(lib().klt_ctx_create(device, C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().klt_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_stream(self, stream_ptr: int | None):
+        check(lib().klt_ctx_set_stream(self._h, C.c_void_p(stream_ptr or 0)))
+
+    def stream(self) -> int:
+        return lib().klt_ctx_stream(self._h) or 0
+
+    def sync(self):
+        check(lib().klt_ctx_sync(self._h))
+
+    def set_profiling(self, on: bool):
+        check(lib().klt_ctx_set_profiling(self._h, 1 if on else 0))
+
+    def profile_report(self) -> dict:
+        """{kernel name: (calls, total ms)} since the last report."""
+        buf = C.create_string_buffer(1 << 16)
+        n = lib().klt_ctx_profile_report(self._h, buf, len(buf))
+        if n < 0:
+            check(n)
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, calls, ms = line.split()
+            out[name] = (int(calls), float(ms))
+        return out
+
+    def kernels(self, kr, ki, scales, size_x: int, size_y: int) -> "KernelBank":
+        return KernelBank(self, kr, ki, scales, size_x, size_y)
+
+    def socs_forward(self, bank: "KernelBank", masks, images=None):
+        """SOCS images of masks [B, Ly, Lx] (or [Ly, Lx])."""
+        shape = tuple(masks.shape)
+        Ly, Lx = shape[-2:]
+        B = int(np.prod(shape[:-2])) if len(shape) > 2 else 1
+        if images is None:
+            if _is_torch(masks):
+                import torch
+                images = torch.empty_like(masks)
+            else:
+                images = np.empty(shape, np.float32)
+        pm, hm = _ptr(masks, np.float32)
+        pi, hi = _ptr(images, np.float32)
+        check(lib().klt_socs_forward(self._h, bank.handle, pm, B, Lx, Ly, pi, _flags(hm, hi)))
+        return images
+
+
+class KernelBank:
+    def __init__(self, ctx: Context, kr, ki, scales, size_x: int, size_y: int):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        pr, h1 = _ptr(kr, np.float32)
+        pim, h2 = _ptr(ki, np.float32)
+        ps, h3 = _ptr(scales, np.float32)
+        nk = int(np.prod(scales.shape))
+        check(lib().klt_kernels_create(ctx.handle, pr, pim, ps, nk, size_x, size_y,
+                                       _flags(h1, h2, h3), C.byref(self._h)))
+        self.nk, self.size_x, self.size_y = nk, size_x, size_y
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().klt_kernels_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

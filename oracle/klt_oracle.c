@@ -161,11 +161,14 @@ void or_sizing(float NA, int Lx, int Ly, float sigma, float lambda, int* Nx, int
 
 typedef struct {
   int sh, oSgm;
-  float dz, k, Lxn, Lyn, NA;
+  float dz, k, Lxn, Lyn, NA, nm2;
 } pupil_geom;
 
+/* nmed: refractive index of the image-side medium.  nmed = 1 is the
+ * reference's formula bit for bit; nmed > 1 extends it to immersion
+ * (NA > 1), where the reference's sqrt(1 - rho^2 NA^2) is NaN. */
 static void geom_init(pupil_geom* g, int srcSize, float outerSigma, float lambda,
-                      float NA, float defocus, int Lx, int Ly) {
+                      float NA, float defocus, int Lx, int Ly, float nmed) {
   g->sh = (int)((srcSize - 1) / 2.0);                  /* :189 */
   g->oSgm = (int)ceilf((float)g->sh * outerSigma);     /* :190 */
   float na2 = NA * NA;
@@ -174,6 +177,7 @@ static void geom_init(pupil_geom* g, int srcSize, float outerSigma, float lambda
   g->Lxn = (float)Lx * NA / lambda;                    /* :197 */
   g->Lyn = (float)Ly * NA / lambda;
   g->NA = NA;
+  g->nm2 = nmed * nmed;
 }
 
 /* Pupil value for source point (p,q) at frequency (nx,ny).  Returns 1 and
@@ -196,7 +200,12 @@ static int pupil_at(const pupil_geom* g, int p, int q, int nx, int ny, double* r
   float rn = rho2 * g->NA;
   rn = rn * g->NA;
   float dzk = g->dz * g->k;
-  float phi = (float)((double)dzk * sqrt(1.0 - (double)rn)); /* :220 */
+  if (g->dz == 0.0f) { /* zero defocus: unit pupil, no root evaluated */
+    *re = 1.0;
+    *im = 0.0;
+    return 1;
+  }
+  float phi = (float)((double)dzk * sqrt((double)g->nm2 - (double)rn)); /* :220 */
   *re = (double)cosf(phi);                             /* :221 */
   *im = (double)sinf(phi);
   return 1;
@@ -220,10 +229,10 @@ int or_count_source_points(const float* src, int srcSize, float outerSigma) {
  * (q outer, p inner).  pup: [nsrc][T] complex (zeros where outside);
  * sval: [nsrc] source weights; pq: [nsrc][2] (p,q).  Returns nsrc. */
 int or_pupil_matrix(const float* src, int srcSize, float outerSigma, float lambda,
-                    float NA, float defocus, int Lx, int Ly, int Nx, int Ny,
+                    float NA, float defocus, int Lx, int Ly, int Nx, int Ny, float nmed,
                     double* pup, double* sval, int* pq) {
   pupil_geom g;
-  geom_init(&g, srcSize, outerSigma, lambda, NA, defocus, Lx, Ly);
+  geom_init(&g, srcSize, outerSigma, lambda, NA, defocus, Lx, Ly, nmed);
   int W = 2 * Nx + 1, T = W * (2 * Ny + 1);
   int s = 0;
   for (int q = -g.oSgm; q <= g.oSgm; ++q)
@@ -253,14 +262,15 @@ int or_pupil_matrix(const float* src, int srcSize, float outerSigma, float lambd
  * accumulated in double over s in the reference's order; upper triangle
  * computed, lower triangle its conjugate.  tcc: T*T complex, zeroed here. */
 void or_tcc(double* tcc, const float* src, int srcSize, float outerSigma,
-            float lambda, float NA, float defocus, int Lx, int Ly, int Nx, int Ny) {
+            float lambda, float NA, float defocus, int Lx, int Ly, int Nx, int Ny,
+            float nmed) {
   int W = 2 * Nx + 1, T = W * (2 * Ny + 1);
   int ns = or_count_source_points(src, srcSize, outerSigma);
   double* pup = (double*)malloc(sizeof(double) * 2 * (size_t)T * (ns > 0 ? ns : 1));
   double* sv = (double*)malloc(sizeof(double) * (ns > 0 ? ns : 1));
   int* pq = (int*)malloc(sizeof(int) * 2 * (ns > 0 ? ns : 1));
-  or_pupil_matrix(src, srcSize, outerSigma, lambda, NA, defocus, Lx, Ly, Nx, Ny, pup,
-                  sv, pq);
+  or_pupil_matrix(src, srcSize, outerSigma, lambda, NA, defocus, Lx, Ly, Nx, Ny, nmed,
+                  pup, sv, pq);
   memset(tcc, 0, sizeof(double) * 2 * (size_t)T * T);
   for (int s = 0; s < ns; ++s) {
     const double* P = pup + (size_t)s * T * 2;
@@ -863,10 +873,10 @@ double or_socs_loss_grad(const double* mask, const float* tgt, int Lx, int Ly,
  * grid_grad: srcSize^2 (zero outside the disc).  Returns L. */
 double or_abbe_loss_grad(const double* mask, const float* tgt, int Lx, int Ly,
                          const float* src, int srcSize, float lambda, float NA,
-                         float defocus, int Nx, int Ny, double a, double thr,
+                         float defocus, int Nx, int Ny, float nmed, double a, double thr,
                          double* img_out, double* grid_grad) {
   pupil_geom g;
-  geom_init(&g, srcSize, 1.0f, lambda, NA, defocus, Lx, Ly);
+  geom_init(&g, srcSize, 1.0f, lambda, NA, defocus, Lx, Ly, nmed);
   int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
   size_t np = (size_t)Lx * Ly;
   double* band = (double*)malloc(sizeof(double) * 2 * T);
@@ -1001,10 +1011,11 @@ double or_abbe_loss_grad(const double* mask, const float* tgt, int Lx, int Ly,
 
 /* Abbe image only (no loss): I_A(x) = sum_s S_s |E_s(x)|^2 */
 void or_abbe_image(const double* mask, int Lx, int Ly, const float* src, int srcSize,
-                   float lambda, float NA, float defocus, int Nx, int Ny, double* img) {
+                   float lambda, float NA, float defocus, int Nx, int Ny, float nmed,
+                   double* img) {
   size_t np = (size_t)Lx * Ly;
   float* zt = (float*)calloc(np, sizeof(float));
-  or_abbe_loss_grad(mask, zt, Lx, Ly, src, srcSize, lambda, NA, defocus, Nx, Ny, 1.0,
-                    0.0, img, NULL);
+  or_abbe_loss_grad(mask, zt, Lx, Ly, src, srcSize, lambda, NA, defocus, Nx, Ny, nmed,
+                    1.0, 0.0, img, NULL);
   free(zt);
 }

@@ -49,8 +49,8 @@ def lib():
             "or_outer_sigma": (f, [_f32p, i]),
             "or_sizing": (None, [f, i, i, f, f, C.POINTER(i), C.POINTER(i)]),
             "or_count_source_points": (i, [_f32p, i, f]),
-            "or_pupil_matrix": (i, [_f32p, i, f, f, f, f, i, i, i, i, _f64p, _f64p, _i32p]),
-            "or_tcc": (None, [_f64p, _f32p, i, f, f, f, f, i, i, i, i]),
+            "or_pupil_matrix": (i, [_f32p, i, f, f, f, f, i, i, i, i, f, _f64p, _f64p, _i32p]),
+            "or_tcc": (None, [_f64p, _f32p, i, f, f, f, f, i, i, i, i, f]),
             "or_eigh_topk": (i, [_f64p, i, i, _f64p, _f64p]),
             "or_kernels_from_eig": (None, [_f64p, _f64p, i, i, _f32p, _f32p, _f32p]),
             "or_mask_band": (None, [_f32p, i, i, i, i, _f64p]),
@@ -59,9 +59,9 @@ def lib():
             "or_hopkins_image": (None, [_f32p, i, i, _f64p, i, i, _f64p]),
             "or_socs_loss_grad": (d, [_f64p, _f32p, i, i, _f32p, _f32p, _f32p, i, i, i,
                                       d, d, C.c_void_p, C.c_void_p]),
-            "or_abbe_loss_grad": (d, [_f64p, _f32p, i, i, _f32p, i, f, f, f, i, i, d, d,
+            "or_abbe_loss_grad": (d, [_f64p, _f32p, i, i, _f32p, i, f, f, f, i, i, f, d, d,
                                       C.c_void_p, C.c_void_p]),
-            "or_abbe_image": (None, [_f64p, i, i, _f32p, i, f, f, f, i, i, _f64p]),
+            "or_abbe_image": (None, [_f64p, i, i, _f32p, i, f, f, f, i, i, f, _f64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -122,7 +122,7 @@ def sizing(NA: float, Lx: int, Ly: int, sigma: float, lam: float = 193.0):
     return nx.value, ny.value
 
 
-def pupil_matrix(src, sigma, NA, defocus, Lx, Ly, Nx, Ny, lam=193.0):
+def pupil_matrix(src, sigma, NA, defocus, Lx, Ly, Nx, Ny, lam=193.0, nmed=1.0):
     s = np.ascontiguousarray(src, np.float32).ravel()
     n = int(round(np.sqrt(s.size)))
     ns = lib().or_count_source_points(s, n, sigma)
@@ -130,16 +130,16 @@ def pupil_matrix(src, sigma, NA, defocus, Lx, Ly, Nx, Ny, lam=193.0):
     pup = np.zeros((max(ns, 1), T, 2), np.float64)
     sv = np.zeros(max(ns, 1), np.float64)
     pq = np.zeros((max(ns, 1), 2), np.int32)
-    lib().or_pupil_matrix(s, n, sigma, lam, NA, defocus, Lx, Ly, Nx, Ny, pup, sv, pq)
+    lib().or_pupil_matrix(s, n, sigma, lam, NA, defocus, Lx, Ly, Nx, Ny, nmed, pup, sv, pq)
     return pup[:ns, :, 0] + 1j * pup[:ns, :, 1], sv[:ns], pq[:ns]
 
 
-def tcc(src, sigma, NA, defocus, Lx, Ly, Nx, Ny, lam=193.0) -> np.ndarray:
+def tcc(src, sigma, NA, defocus, Lx, Ly, Nx, Ny, lam=193.0, nmed=1.0) -> np.ndarray:
     s = np.ascontiguousarray(src, np.float32).ravel()
     n = int(round(np.sqrt(s.size)))
     T = (2 * Nx + 1) * (2 * Ny + 1)
     out = np.zeros((T, T, 2), np.float64)
-    lib().or_tcc(out, s, n, sigma, lam, NA, defocus, Lx, Ly, Nx, Ny)
+    lib().or_tcc(out, s, n, sigma, lam, NA, defocus, Lx, Ly, Nx, Ny, nmed)
     return out[..., 0] + 1j * out[..., 1]
 
 
@@ -220,7 +220,7 @@ def socs_loss_grad(mask, target, kr, ki, scales, Nx, Ny, a=50.0, thr=0.25, grad=
 
 
 def abbe_loss_grad(mask, target, src, NA, defocus, Nx, Ny, a=50.0, thr=0.25, lam=193.0,
-                   grad=True):
+                   grad=True, nmed=1.0):
     m = np.ascontiguousarray(mask, np.float64)
     Ly, Lx = m.shape
     s = np.ascontiguousarray(src, np.float32)
@@ -228,17 +228,17 @@ def abbe_loss_grad(mask, target, src, NA, defocus, Nx, Ny, a=50.0, thr=0.25, lam
     t = np.ascontiguousarray(target, np.float32)
     img = np.zeros((Ly, Lx), np.float64)
     g = np.zeros((n, n), np.float64) if grad else None
-    L = lib().or_abbe_loss_grad(m, t, Lx, Ly, s, n, lam, NA, defocus, Nx, Ny, a, thr,
+    L = lib().or_abbe_loss_grad(m, t, Lx, Ly, s, n, lam, NA, defocus, Nx, Ny, nmed, a, thr,
                                 img.ctypes.data, g.ctypes.data if grad else None)
     return L, img, g
 
 
-def abbe_image(mask, src, NA, defocus, Nx, Ny, lam=193.0) -> np.ndarray:
+def abbe_image(mask, src, NA, defocus, Nx, Ny, lam=193.0, nmed=1.0) -> np.ndarray:
     m = np.ascontiguousarray(mask, np.float64)
     Ly, Lx = m.shape
     s = np.ascontiguousarray(src, np.float32)
     img = np.zeros((Ly, Lx), np.float64)
-    lib().or_abbe_image(m, Lx, Ly, s, s.shape[0], lam, NA, defocus, Nx, Ny, img)
+    lib().or_abbe_image(m, Lx, Ly, s, s.shape[0], lam, NA, defocus, Nx, Ny, nmed, img)
     return img
 
 
