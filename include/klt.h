@@ -60,6 +60,48 @@ int klt_ctx_set_profiling(klt_ctx* ctx, int on);
  * report and clears the record; returns the byte count or an error. */
 int klt_ctx_profile_report(klt_ctx* ctx, char* buf, int len);
 
+/* ---- host boundary: sources, masks, sizing, file formats ---------------
+ * Host-only; bit-identical to the reference's float arithmetic. */
+/* K-Litho-TCC/source.cpp:105-142 (createSource): srcType is "Annular",
+ * "Dipole", "CrossQuadrupole", "Point" or "Import" (sourceFile: ".bin" raw
+ * float32, otherwise text); the result is normalised to sum 1. */
+int klt_source_create(float* src, int srcSize, const char* srcType, float innerR,
+                      float outerR, float radius, float offset, int onXAxis, float ptX,
+                      float ptY, const char* sourceFile);
+/* common/mask.cpp:31-60 (createMask): "LineSpace" or "Import", centred in the
+ * Lx x Ly period, times dose. */
+int klt_mask_create(float* msk, int Lx, int Ly, int maskSizeX, int maskSizeY,
+                    const char* maskType, int lineWidth, int spaceWidth, int isHorizontal,
+                    const char* maskFile, float dose);
+/* K-Litho-TCC/klitho_tcc.cpp:52-67 (getOuterSigma), not clamped. */
+float klt_outer_sigma(const float* src, int srcSize);
+/* K-Litho-TCC/klitho_tcc.cpp:479-486: clamps sigma to 1 and sets
+ * N = floor(f32(NA) * L * (1 + sigma) / lambda). */
+int klt_tcc_sizing(float NA, int Lx, int Ly, float outerSigma, float lambda, int* Nx, int* Ny);
+/* common/file_io.cpp:232-260 / 204-230 (readFloatArrayFrom{Binary,Txt}):
+ * reader chosen by extension; a size mismatch warns and zero-pads/truncates. */
+int klt_read_floats(const char* path, float* out, int size);
+/* common/file_io.cpp:137-150 (writeFloatArrayToBinary). */
+int klt_write_floats(const char* path, const float* data, int size);
+/* common/file_io.cpp:262-280 (writeKernelInfo). */
+int klt_write_kernel_info(const char* path, int sizeX, int sizeY, int nk, const float* scales);
+/* common/file_io.cpp:282-307 (readKernelInfo): returns nk (>= 0) or an error. */
+int klt_read_kernel_info(const char* path, int* sizeX, int* sizeY, float* scales, int cap);
+
+/* ---- TCC build (K-Litho-TCC/klitho_tcc.cpp:187-267, calcTCC) -----------
+ * TCC[i][j] = sum_s S_s P(i,s) conj(P(j,s)) over the source points the
+ * reference uses (S != 0, inside the unit disc, |p|,|q| <= ceil(sh*sigma)).
+ * Pupil membership and phase follow the reference's float32 arithmetic.
+ * nmed is the image-side immersion index: 1 reproduces the reference's
+ * sqrt(1 - rho^2 NA^2) defocus phase; NA may not exceed nmed.  Zero defocus
+ * gives a unit pupil (the reference's NaN for NA*rho > 1 is avoided).
+ * tcc: T x T complex double (T = (2Nx+1)(2Ny+1)), both triangles. */
+#define KLT_TCC_FP64 0       /* FP64 Gram, accumulated like the reference */
+#define KLT_TCC_SPLIT_F16 1  /* split hi+lo FP16 MFMA Gram, FP32 accumulation */
+int klt_tcc_build(klt_ctx* ctx, const float* src, int srcSize, float outerSigma, float lambda,
+                  float NA, float defocus, float nmed, int Lx, int Ly, int Nx, int Ny,
+                  int precision, double* tcc, unsigned flags);
+
 /* ---- SOCS kernel bank --------------------------------------------------
  * Replaces the kernel read loop of klitho_socs.cpp:297-318: nk kernels of
  * sizeX x sizeY (odd) in the krn_i_{r,i}.bin layout, and their eigenvalues

@@ -1,0 +1,409 @@
+// tcc.hip -- TCC build on MI355X (replaces calcTCC, klitho_tcc.cpp:187-267).
+//
+// TCC[i][j] = sum_s S_s P(i,s) conj(P(j,s)) over the source points the
+// reference uses.  Two stages:
+//   k_pupil: the pupil matrix P[s][i] with the reference's float32
+//     membership decisions and phase, evaluated with correctly rounded
+//     operations in the reference's order (this file is compiled with
+//     -ffp-contract=off);
+//   a complex Gram, either
+//     k_gram_split16 -- split-precision FP16 MFMA: X = 2^e S P and Y = P are
+//       split into hi + lo halves, X^H Y ~ Xh^H Yh + Xh^H Yl + Xl^H Yh with FP32
+//       accumulation (v_mfma_f32_16x16x32_f16), upper-triangle 64 x 64 tiles;
+//     k_gram_f64 -- the FP64 reference Gram, accumulated in double like the
+//       reference.
+// The TCC is written as T x T complex double, row-major, both triangles.
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "klt_internal.h"
+
+namespace {
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+struct PupilGeom {
+  int sh, Nx, Ny, T;
+  float dz, k, Lxn, Lyn, NA, nm2;
+};
+
+struct SrcPoint {
+  int p, q;
+  float S;
+};
+
+// Pupil value at frequency (nx, ny) for source point (p, q), as the loops at
+// klitho_tcc.cpp:202-222 compute it (returns false when the reference leaves
+// the entry zero).  Zero defocus gives a unit pupil without evaluating the
+// root, which extends the formula to NA > 1 (where the reference's
+// sqrt(1 - rho^2 NA^2) is NaN); nm2 is the squared immersion index (1 in the
+// reference).
+__device__ bool pupil_value(const PupilGeom& g, int p, int q, int nx, int ny, float2* out) {
+  const float sx = __fdiv_rn((float)p, (float)g.sh);
+  const float sy = __fdiv_rn((float)q, (float)g.sh);
+  const int nxMin = (int)__fmul_rn(__fsub_rn(-1.0f, sx), g.Lxn);
+  const int nxMax = (int)__fmul_rn(__fsub_rn(1.0f, sx), g.Lxn);
+  const int nyMin = (int)__fmul_rn(__fsub_rn(-1.0f, sy), g.Lyn);
+  const int nyMax = (int)__fmul_rn(__fsub_rn(1.0f, sy), g.Lyn);
+  if (nx < nxMin || nx > nxMax || ny < nyMin || ny > nyMax) return false;
+  const float fy = __fadd_rn(__fdiv_rn((float)ny, g.Lyn), sy);
+  if (!(__fmul_rn(fy, fy) <= 1.0f)) return false;
+  const float fx = __fadd_rn(__fdiv_rn((float)nx, g.Lxn), sx);
+  const float rho2 = __fadd_rn(__fmul_rn(fx, fx), __fmul_rn(fy, fy));
+  if (!(rho2 <= 1.0f)) return false;
+  if (g.dz == 0.0f) {
+    *out = make_float2(1.0f, 0.0f);
+    return true;
+  }
+  const float rn = __fmul_rn(__fmul_rn(rho2, g.NA), g.NA);
+  const float dzk = __fmul_rn(g.dz, g.k);
+  const double root = __dsqrt_rn(__dsub_rn((double)g.nm2, (double)rn));
+  const float phi = __double2float_rn(__dmul_rn((double)dzk, root));
+  *out = make_float2(cosf(phi), sinf(phi));
+  return true;
+}
+
+// P[s][i] for every used source point (blockIdx.y) and frequency i.
+__global__ __launch_bounds__(256) void k_pupil(PupilGeom g, const SrcPoint* __restrict__ pts,
+                                               float2* __restrict__ P) {
+  const int s = blockIdx.y;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.T) return;
+  const int W = 2 * g.Nx + 1;
+  const int ny = i / W - g.Ny, nx = i % W - g.Nx;
+  float2 v = make_float2(0.f, 0.f);
+  const SrcPoint sp = pts[s];
+  pupil_value(g, sp.p, sp.q, nx, ny, &v);
+  P[(size_t)s * g.T + i] = v;
+}
+
+// Split operands for the MFMA Gram, transposed to [Tpad][Kpad] (k contiguous):
+//   U  = [2^e S Pr ; 2^e S Pi],  V1 = [Pr ; Pi],  V2 = [-Pi ; Pr]
+// (k < ns: real parts, ns <= k < 2ns: imaginary parts), each as hi + lo fp16.
+__global__ __launch_bounds__(256) void k_split_operands(
+    const float2* __restrict__ P, const SrcPoint* __restrict__ pts, int ns, int T, int Kpad,
+    float scale, _Float16* __restrict__ Uh, _Float16* __restrict__ Ul,
+    _Float16* __restrict__ V1h, _Float16* __restrict__ V1l, _Float16* __restrict__ V2h,
+    _Float16* __restrict__ V2l) {
+  const int i = blockIdx.x;  // frequency row (< Tpad)
+  for (int k = threadIdx.x; k < Kpad; k += 256) {
+    float u = 0.f, v1 = 0.f, v2 = 0.f;
+    if (i < T && k < 2 * ns) {
+      const int s = k < ns ? k : k - ns;
+      const float2 p = P[(size_t)s * T + i];
+      const float su = scale * pts[s].S;
+      if (k < ns) {
+        u = su * p.x;
+        v1 = p.x;
+        v2 = -p.y;
+      } else {
+        u = su * p.y;
+        v1 = p.y;
+        v2 = p.x;
+      }
+    }
+    const size_t o = (size_t)i * Kpad + k;
+    const _Float16 uh = (_Float16)u, v1h = (_Float16)v1, v2h = (_Float16)v2;
+    Uh[o] = uh;
+    Ul[o] = (_Float16)(u - (float)uh);
+    V1h[o] = v1h;
+    V1l[o] = (_Float16)(v1 - (float)v1h);
+    V2h[o] = v2h;
+    V2l[o] = (_Float16)(v2 - (float)v2h);
+  }
+}
+
+// Split-FP16 MFMA Gram.  One block per upper-triangle pair of 64 x 64 tiles;
+// each of the 4 waves owns a 32 x 32 quadrant (2 x 2 MFMA 16 x 16 blocks, real
+// and imaginary parts).  v_mfma_f32_16x16x32_f16 operand layout: lane l holds
+// A[m = l%16][k = 8(l/16) .. +7] and B[k = 8(l/16) .. +7][n = l%16]; the result
+// D[m = 4(l/16) + r][n = l%16], r = 0..3.
+__global__ __launch_bounds__(256) void k_gram_split16(
+    const _Float16* __restrict__ Uh, const _Float16* __restrict__ Ul,
+    const _Float16* __restrict__ V1h, const _Float16* __restrict__ V1l,
+    const _Float16* __restrict__ V2h, const _Float16* __restrict__ V2l, int T, int Kpad,
+    const int2* __restrict__ tiles, float inv_scale, double* __restrict__ tcc) {
+  const int2 t = tiles[blockIdx.x];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i0 = t.x * 64 + (wave >> 1) * 32;
+  const int j0 = t.y * 64 + (wave & 1) * 32;
+  const int r16 = lane & 15, kq = (lane >> 4) * 8;
+  float4_t accR[2][2], accI[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      accR[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
+      accI[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
+    }
+  const size_t ra0 = (size_t)(i0 + r16) * Kpad + kq, ra1 = ra0 + (size_t)16 * Kpad;
+  const size_t rb0 = (size_t)(j0 + r16) * Kpad + kq, rb1 = rb0 + (size_t)16 * Kpad;
+  for (int k = 0; k < Kpad; k += 32) {
+    half8_t ah[2], al[2], bh1[2], bl1[2], bh2[2], bl2[2];
+    ah[0] = *reinterpret_cast<const half8_t*>(Uh + ra0 + k);
+    ah[1] = *reinterpret_cast<const half8_t*>(Uh + ra1 + k);
+    al[0] = *reinterpret_cast<const half8_t*>(Ul + ra0 + k);
+    al[1] = *reinterpret_cast<const half8_t*>(Ul + ra1 + k);
+    bh1[0] = *reinterpret_cast<const half8_t*>(V1h + rb0 + k);
+    bh1[1] = *reinterpret_cast<const half8_t*>(V1h + rb1 + k);
+    bl1[0] = *reinterpret_cast<const half8_t*>(V1l + rb0 + k);
+    bl1[1] = *reinterpret_cast<const half8_t*>(V1l + rb1 + k);
+    bh2[0] = *reinterpret_cast<const half8_t*>(V2h + rb0 + k);
+    bh2[1] = *reinterpret_cast<const half8_t*>(V2h + rb1 + k);
+    bl2[0] = *reinterpret_cast<const half8_t*>(V2l + rb0 + k);
+    bl2[1] = *reinterpret_cast<const half8_t*>(V2l + rb1 + k);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh1[b], accR[a][b], 0, 0, 0);
+        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bl1[b], accR[a][b], 0, 0, 0);
+        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], bh1[b], accR[a][b], 0, 0, 0);
+        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh2[b], accI[a][b], 0, 0, 0);
+        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bl2[b], accI[a][b], 0, 0, 0);
+        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], bh2[b], accI[a][b], 0, 0, 0);
+      }
+  }
+  const bool diag_tile = t.x == t.y;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + a * 16 + (lane >> 4) * 4 + r;
+        const int j = j0 + b * 16 + r16;
+        if (i >= T || j >= T || (diag_tile && i > j)) continue;
+        const double re = (double)accR[a][b][r] * inv_scale;
+        const double im = i == j ? 0.0 : (double)accI[a][b][r] * inv_scale;
+        tcc[2 * ((size_t)i * T + j)] = re;
+        tcc[2 * ((size_t)i * T + j) + 1] = im;
+        if (i != j) {
+          tcc[2 * ((size_t)j * T + i)] = re;
+          tcc[2 * ((size_t)j * T + i) + 1] = -im;
+        }
+      }
+}
+
+// FP64 reference Gram: 32 x 32 upper-triangle output tiles, source points
+// staged through LDS 16 at a time; thread (ty, tx) owns a 2 x 2 block.
+__global__ __launch_bounds__(256) void k_gram_f64(const float2* __restrict__ P,
+                                                  const SrcPoint* __restrict__ pts, int ns,
+                                                  int T, const int2* __restrict__ tiles,
+                                                  double* __restrict__ tcc) {
+  __shared__ double2 sx[16][32];
+  __shared__ double2 sy[16][32];
+  const int2 t = tiles[blockIdx.x];
+  const int i0 = t.x * 32, j0 = t.y * 32;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  double2 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = make_double2(0.0, 0.0);
+  for (int s0 = 0; s0 < ns; s0 += 16) {
+    for (int e = threadIdx.x; e < 16 * 32; e += 256) {
+      const int ss = e >> 5, c = e & 31;
+      const int s = s0 + ss;
+      double2 x = make_double2(0.0, 0.0), y = make_double2(0.0, 0.0);
+      if (s < ns) {
+        const double S = (double)pts[s].S;
+        if (i0 + c < T) {
+          const float2 p = P[(size_t)s * T + i0 + c];
+          x = make_double2((double)p.x * S, (double)p.y * S);
+        }
+        if (j0 + c < T) {
+          const float2 p = P[(size_t)s * T + j0 + c];
+          y = make_double2((double)p.x, (double)p.y);
+        }
+      }
+      sx[ss][c] = x;
+      sy[ss][c] = y;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int ss = 0; ss < 16; ++ss) {
+      const double2 x0 = sx[ss][ty * 2], x1 = sx[ss][ty * 2 + 1];
+      const double2 y0 = sy[ss][tx * 2], y1 = sy[ss][tx * 2 + 1];
+      // x * conj(y)
+      acc[0][0].x += x0.x * y0.x + x0.y * y0.y;
+      acc[0][0].y += x0.y * y0.x - x0.x * y0.y;
+      acc[0][1].x += x0.x * y1.x + x0.y * y1.y;
+      acc[0][1].y += x0.y * y1.x - x0.x * y1.y;
+      acc[1][0].x += x1.x * y0.x + x1.y * y0.y;
+      acc[1][0].y += x1.y * y0.x - x1.x * y0.y;
+      acc[1][1].x += x1.x * y1.x + x1.y * y1.y;
+      acc[1][1].y += x1.y * y1.x - x1.x * y1.y;
+    }
+    __syncthreads();
+  }
+  const bool diag_tile = t.x == t.y;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      const int i = i0 + ty * 2 + a, j = j0 + tx * 2 + b;
+      if (i >= T || j >= T || (diag_tile && i > j)) continue;
+      const double im = i == j ? 0.0 : acc[a][b].y;
+      tcc[2 * ((size_t)i * T + j)] = acc[a][b].x;
+      tcc[2 * ((size_t)i * T + j) + 1] = im;
+      if (i != j) {
+        tcc[2 * ((size_t)j * T + i)] = acc[a][b].x;
+        tcc[2 * ((size_t)j * T + i) + 1] = -im;
+      }
+    }
+}
+
+int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+}  // namespace
+
+namespace klt {
+
+// Geometry constants in the reference's float arithmetic (klitho_tcc.cpp:189-197).
+static PupilGeom make_geom(int srcSize, float lambda, float NA, float defocus, float nmed,
+                           int Lx, int Ly, int Nx, int Ny) {
+  PupilGeom g;
+  g.sh = (int)((srcSize - 1) / 2.0);
+  g.Nx = Nx;
+  g.Ny = Ny;
+  g.T = (2 * Nx + 1) * (2 * Ny + 1);
+  const float na2 = NA * NA;
+  g.dz = defocus / (na2 / lambda);
+  g.k = (float)(2 * M_PI / lambda);
+  g.Lxn = (float)Lx * NA / lambda;
+  g.Lyn = (float)Ly * NA / lambda;
+  g.NA = NA;
+  g.nm2 = nmed * nmed;
+  return g;
+}
+
+// Source points in the reference's order (q outer, p inner), klitho_tcc.cpp:202-205.
+static std::vector<SrcPoint> source_points(const float* src, int srcSize, float outerSigma) {
+  const int sh = (int)((srcSize - 1) / 2.0);
+  const int oS = (int)std::ceil((float)sh * outerSigma);
+  std::vector<SrcPoint> pts;
+  for (int q = -oS; q <= oS; ++q)
+    for (int p = -oS; p <= oS; ++p) {
+      if (q + sh < 0 || q + sh >= srcSize || p + sh < 0 || p + sh >= srcSize) continue;
+      const float S = src[(q + sh) * srcSize + p + sh];
+      if (S != 0.0f && p * p + q * q <= sh * sh) pts.push_back({p, q, S});
+    }
+  return pts;
+}
+
+// Pupil matrix of every used source point into ctx->misc ([ns][T] float2);
+// the points are left in ctx->stage_in.
+static int build_pupil(klt_ctx* ctx, const float* src_host, int srcSize, float outerSigma,
+                       const PupilGeom& g, int* ns_out, std::vector<SrcPoint>& pts) {
+  pts = source_points(src_host, srcSize, outerSigma);
+  const int ns = (int)pts.size();
+  *ns_out = ns;
+  if (ns == 0) return KLT_OK;
+  KLT_TRY(ctx->stage_in.ensure(sizeof(SrcPoint) * ns));
+  KLT_HIP(hipMemcpyAsync(ctx->stage_in.ptr, pts.data(), sizeof(SrcPoint) * ns,
+                         hipMemcpyHostToDevice, ctx->stream));
+  KLT_TRY(ctx->misc.ensure(sizeof(float2) * (size_t)ns * g.T));
+  KLT_LAUNCH(ctx, "k_pupil", k_pupil, dim3((g.T + 255) / 256, ns), dim3(256), 0, g,
+             ctx->stage_in.as<SrcPoint>(), ctx->misc.as<float2>());
+  KLT_HIP(hipGetLastError());
+  return KLT_OK;
+}
+
+int tcc_build_device(klt_ctx* ctx, const float* src_host, int srcSize, float outerSigma,
+                     float lambda, float NA, float defocus, float nmed, int Lx, int Ly,
+                     int Nx, int Ny, int precision, double* tcc_dev) {
+  const PupilGeom g = make_geom(srcSize, lambda, NA, defocus, nmed, Lx, Ly, Nx, Ny);
+  const int T = g.T;
+  std::vector<SrcPoint> pts;
+  int ns = 0;
+  KLT_TRY(build_pupil(ctx, src_host, srcSize, outerSigma, g, &ns, pts));
+  if (ns == 0) {
+    KLT_HIP(hipMemsetAsync(tcc_dev, 0, sizeof(double) * 2 * (size_t)T * T, ctx->stream));
+    return KLT_OK;
+  }
+  if (precision == KLT_TCC_FP64) {
+    const int nt = (T + 31) / 32;
+    std::vector<int2> tiles;
+    for (int a = 0; a < nt; ++a)
+      for (int b = a; b < nt; ++b) tiles.push_back(make_int2(a, b));
+    KLT_TRY(ctx->spec.ensure(sizeof(int2) * tiles.size()));
+    KLT_HIP(hipMemcpyAsync(ctx->spec.ptr, tiles.data(), sizeof(int2) * tiles.size(),
+                           hipMemcpyHostToDevice, ctx->stream));
+    KLT_LAUNCH(ctx, "k_gram_f64", k_gram_f64, dim3((unsigned)tiles.size()), dim3(256), 0,
+               ctx->misc.as<float2>(), ctx->stage_in.as<SrcPoint>(), ns, T,
+               ctx->spec.as<int2>(), tcc_dev);
+    KLT_HIP(hipGetLastError());
+    KLT_HIP(hipStreamSynchronize(ctx->stream));  // tiles / points stay valid until here
+    return KLT_OK;
+  }
+  if (precision != KLT_TCC_SPLIT_F16)
+    return set_error(KLT_ERR_ARG, "klt_tcc_build: unknown precision");
+  // power-of-two scale bringing max |S| into [0.5, 1)
+  float smax = 0.f;
+  for (auto& p : pts) smax = std::max(smax, std::fabs(p.S));
+  int e = 0;
+  (void)std::frexp(smax, &e);
+  const float scale = std::ldexp(1.0f, -e);
+  const int Kpad = round_up(2 * ns, 32), Tpad = round_up(T, 64);
+  const size_t nel = (size_t)Tpad * Kpad;
+  KLT_TRY(ctx->qpart.ensure(6 * nel * sizeof(_Float16)));
+  _Float16* base = ctx->qpart.as<_Float16>();
+  KLT_LAUNCH(ctx, "k_split_operands", k_split_operands, dim3(Tpad), dim3(256), 0,
+             ctx->misc.as<float2>(), ctx->stage_in.as<SrcPoint>(), ns, T, Kpad, scale, base,
+             base + nel, base + 2 * nel, base + 3 * nel, base + 4 * nel, base + 5 * nel);
+  const int nt = Tpad / 64;
+  std::vector<int2> tiles;
+  for (int a = 0; a < nt; ++a)
+    for (int b = a; b < nt; ++b) tiles.push_back(make_int2(a, b));
+  KLT_TRY(ctx->spec.ensure(sizeof(int2) * tiles.size()));
+  KLT_HIP(hipMemcpyAsync(ctx->spec.ptr, tiles.data(), sizeof(int2) * tiles.size(),
+                         hipMemcpyHostToDevice, ctx->stream));
+  KLT_LAUNCH(ctx, "k_gram_split16", k_gram_split16, dim3((unsigned)tiles.size()), dim3(256), 0,
+             base, base + nel, base + 2 * nel, base + 3 * nel, base + 4 * nel, base + 5 * nel,
+             T, Kpad, ctx->spec.as<int2>(), 1.0f / scale, tcc_dev);
+  KLT_HIP(hipGetLastError());
+  KLT_HIP(hipStreamSynchronize(ctx->stream));
+  return KLT_OK;
+}
+
+}  // namespace klt
+
+extern "C" {
+
+int klt_tcc_build(klt_ctx* ctx, const float* src, int srcSize, float outerSigma, float lambda,
+                  float NA, float defocus, float nmed, int Lx, int Ly, int Nx, int Ny,
+                  int precision, double* tcc, unsigned flags) {
+  if (!ctx || !src || !tcc) return klt::set_error(KLT_ERR_ARG, "klt_tcc_build: NULL argument");
+  if (srcSize <= 0 || Lx <= 0 || Ly <= 0 || Nx < 0 || Ny < 0 || lambda <= 0)
+    return klt::set_error(KLT_ERR_ARG, "Error: Invalid input parameters. Please check the values.");
+  if (NA < 0 || !(nmed >= 1.0f) || NA > nmed)
+    return klt::set_error(KLT_ERR_ARG, nmed == 1.0f ? "Error: NA must be in the range [0, 1]."
+                                                    : "Error: NA must not exceed the immersion index.");
+  KLT_TRY(ctx->activate());
+  std::vector<float> hsrc((size_t)srcSize * srcSize);
+  if (flags & KLT_HOST_PTRS) {
+    std::memcpy(hsrc.data(), src, sizeof(float) * hsrc.size());
+  } else {
+    KLT_HIP(hipMemcpyAsync(hsrc.data(), src, sizeof(float) * hsrc.size(), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    KLT_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  const int T = (2 * Nx + 1) * (2 * Ny + 1);
+  const size_t bytes = sizeof(double) * 2 * (size_t)T * T;
+  double* dst = tcc;
+  if (flags & KLT_HOST_PTRS) {
+    KLT_TRY(ctx->stage_out.ensure(bytes));
+    dst = ctx->stage_out.as<double>();
+  }
+  KLT_TRY(klt::tcc_build_device(ctx, hsrc.data(), srcSize, outerSigma, lambda, NA, defocus, nmed,
+                                Lx, Ly, Nx, Ny, precision, dst));
+  if (flags & KLT_HOST_PTRS) {
+    KLT_HIP(hipMemcpyAsync(tcc, dst, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    KLT_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return KLT_OK;
+}
+
+}  // extern "C"

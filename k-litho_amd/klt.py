@@ -53,6 +53,18 @@ def lib() -> C.CDLL:
         L.klt_ctx_sync.argtypes = [vp]
         L.klt_ctx_set_profiling.argtypes = [vp, i]
         L.klt_ctx_profile_report.argtypes = [vp, C.c_char_p, i]
+        f = C.c_float
+        cs = C.c_char_p
+        L.klt_source_create.argtypes = [fp, i, cs, f, f, f, f, i, f, f, cs]
+        L.klt_mask_create.argtypes = [fp, i, i, i, i, cs, i, i, i, cs, f]
+        L.klt_outer_sigma.argtypes = [fp, i]
+        L.klt_outer_sigma.restype = f
+        L.klt_tcc_sizing.argtypes = [f, i, i, f, f, C.POINTER(i), C.POINTER(i)]
+        L.klt_read_floats.argtypes = [cs, fp, i]
+        L.klt_write_floats.argtypes = [cs, fp, i]
+        L.klt_write_kernel_info.argtypes = [cs, i, i, i, fp]
+        L.klt_read_kernel_info.argtypes = [cs, C.POINTER(i), C.POINTER(i), fp, i]
+        L.klt_tcc_build.argtypes = [vp, fp, i, f, f, f, f, f, i, i, i, i, i, fp, u]
         L.klt_kernels_create.argtypes = [vp, fp, fp, fp, i, i, i, u, C.POINTER(vp)]
         L.klt_kernels_destroy.argtypes = [vp]
         L.klt_kernels_destroy.restype = None
@@ -101,6 +113,68 @@ def _flags(*hosts) -> int:
     return 0
 
 
+def _cstr(s) -> bytes:
+    return (s or "").encode() if isinstance(s, str) or s is None else s
+
+
+def source_create(src_size: int, src_type: str, inner=0.6, outer=0.9, radius=0.1, offset=0.5,
+                  on_x_axis=1, pt_x=0.6, pt_y=0.2, source_file="") -> np.ndarray:
+    """createSource (source.cpp:105-142) through the library."""
+    out = np.zeros((src_size, src_size), np.float32)
+    check(lib().klt_source_create(out.ctypes.data_as(C.c_void_p), src_size, _cstr(src_type),
+                                  inner, outer, radius, offset, int(on_x_axis), pt_x, pt_y,
+                                  _cstr(source_file)))
+    return out
+
+
+def mask_create(Lx: int, Ly: int, msx: int, msy: int, mask_type: str, line_width=128,
+                space_width=128, is_horizontal=0, mask_file="", dose=1.0) -> np.ndarray:
+    """createMask (mask.cpp:31-60) through the library."""
+    out = np.zeros((Ly, Lx), np.float32)
+    check(lib().klt_mask_create(out.ctypes.data_as(C.c_void_p), Lx, Ly, msx, msy,
+                                _cstr(mask_type), line_width, space_width, int(is_horizontal),
+                                _cstr(mask_file), dose))
+    return out
+
+
+def outer_sigma(src: np.ndarray) -> float:
+    s = np.ascontiguousarray(src, np.float32)
+    return float(lib().klt_outer_sigma(s.ctypes.data_as(C.c_void_p), s.shape[0]))
+
+
+def tcc_sizing(NA: float, Lx: int, Ly: int, sigma: float, lam: float = 193.0):
+    nx, ny = C.c_int(), C.c_int()
+    check(lib().klt_tcc_sizing(NA, Lx, Ly, sigma, lam, C.byref(nx), C.byref(ny)))
+    return nx.value, ny.value
+
+
+def read_floats(path: str, size: int) -> np.ndarray:
+    out = np.zeros(size, np.float32)
+    check(lib().klt_read_floats(_cstr(path), out.ctypes.data_as(C.c_void_p), size))
+    return out
+
+
+def write_floats(path: str, data: np.ndarray):
+    d = np.ascontiguousarray(data, np.float32).ravel()
+    check(lib().klt_write_floats(_cstr(path), d.ctypes.data_as(C.c_void_p), d.size))
+
+
+def write_kernel_info(path: str, size_x: int, size_y: int, scales: np.ndarray):
+    s = np.ascontiguousarray(scales, np.float32)
+    check(lib().klt_write_kernel_info(_cstr(path), size_x, size_y, s.size,
+                                      s.ctypes.data_as(C.c_void_p)))
+
+
+def read_kernel_info(path: str, cap: int = 4096):
+    sx, sy = C.c_int(), C.c_int()
+    sc = np.zeros(cap, np.float32)
+    nk = lib().klt_read_kernel_info(_cstr(path), C.byref(sx), C.byref(sy),
+                                    sc.ctypes.data_as(C.c_void_p), cap)
+    if nk < 0:
+        check(nk)
+    return sx.value, sy.value, sc[:nk].copy()
+
+
 class Context:
     """One libklt context (device, stream, plans, scratch)."""
 
@@ -145,6 +219,24 @@ class Context:
         for line in buf.value.decode().splitlines():
             name, calls, ms = line.split()
             out[name] = (int(calls), float(ms))
+        return out
+
+    def tcc_build(self, src, sigma: float, NA: float, defocus: float, Lx: int, Ly: int,
+                  Nx: int, Ny: int, precision: int = 0, nmed: float = 1.0, lam: float = 193.0,
+                  out=None):
+        """calcTCC on the GPU; precision 0 = FP64 Gram, 1 = split-FP16 MFMA."""
+        T = (2 * Nx + 1) * (2 * Ny + 1)
+        if out is None:
+            if _is_torch(src):
+                import torch
+                out = torch.empty((T, T, 2), dtype=torch.float64, device=src.device)
+            else:
+                out = np.empty((T, T, 2), np.float64)
+        ps, hs = _ptr(src, np.float32)
+        po, ho = _ptr(out, np.float64)
+        n = int(src.shape[0])
+        check(lib().klt_tcc_build(self._h, ps, n, sigma, lam, NA, defocus, nmed, Lx, Ly, Nx, Ny,
+                                  precision, po, _flags(hs, ho)))
         return out
 
     def kernels(self, kr, ki, scales, size_x: int, size_y: int) -> "KernelBank":

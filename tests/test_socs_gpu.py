@@ -35,8 +35,8 @@ def bank29():
 
 def _random_bank(rng, Nx, Ny, nk):
     T = (2 * Nx + 1) * (2 * Ny + 1)
-    kr = rng.standard_normal((nk, T)).astype(np.float32) / np.sqrt(T)
-    ki = rng.standard_normal((nk, T)).astype(np.float32) / np.sqrt(T)
+    kr = (rng.standard_normal((nk, T)) / np.sqrt(T)).astype(np.float32)
+    ki = (rng.standard_normal((nk, T)) / np.sqrt(T)).astype(np.float32)
     sc = np.sort(rng.random(nk).astype(np.float32))[::-1].copy()
     return kr, ki, sc
 

@@ -1,0 +1,74 @@
+"""TCC build on the GPU (klt_tcc_build) against the FP64 oracle.
+
+* FP64 Gram: entries within 1e-12 of the oracle at zero defocus (unit pupil,
+  same products, only the summation order differs) and within 1e-6 with
+  defocus (device cosf/sinf vs glibc's differ by at most an ulp or two).
+  The diagonal TCC[i][i] = sum_s S_s [i inside pupil(s)] pins every discrete
+  pupil decision: one flipped sample moves it by S_s >= 1e-4.
+* split-FP16 MFMA Gram: relative Frobenius error <= 1e-5 against FP64.
+"""
+import numpy as np
+import pytest
+
+import klt
+import oracle as O
+import workloads as W
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _annular():
+    src = O.source("Annular", 51, inner=0.6, outer=0.9)
+    return src, O.outer_sigma(src)
+
+
+def _cplx(a):
+    return a[..., 0] + 1j * a[..., 1]
+
+
+@pytest.mark.parametrize("L,NA,df,nmed", [(512, 1.35, 0.0, 1.44), (1088, 1.35, 0.0, 1.44),
+                                          (700, 0.95, 0.25, 1.0), (512, 1.35, -0.2361, 1.44)])
+def test_tcc_fp64_vs_oracle(ctx, L, NA, df, nmed):
+    print(f"case tcc fp64 L={L} NA={NA} df={df}", flush=True)
+    src, sig = _annular()
+    Nx, Ny = O.sizing(NA, L, L, sig)
+    ref = O.tcc(src, sig, NA, df, L, L, Nx, Ny, nmed=nmed)
+    got = _cplx(ctx.tcc_build(src, sig, NA, df, L, L, Nx, Ny, precision=0, nmed=nmed))
+    tol = 1e-12 if df == 0.0 else 1e-6
+    assert np.abs(got - ref).max() < tol * np.abs(ref).max()
+    assert np.abs(np.diag(got).real - np.diag(ref).real).max() < 1e-9
+    assert np.abs(got - got.conj().T).max() == 0.0
+
+
+@pytest.mark.parametrize("L,NA,df,nmed", [(512, 1.35, 0.0, 1.44), (1088, 1.35, 0.0, 1.44),
+                                          (1280, 1.35, 0.4722, 1.44)])
+def test_tcc_split16_vs_fp64(ctx, L, NA, df, nmed):
+    print(f"case tcc split16 L={L} df={df}", flush=True)
+    src, sig = _annular()
+    Nx, Ny = O.sizing(NA, L, L, sig)
+    a = _cplx(ctx.tcc_build(src, sig, NA, df, L, L, Nx, Ny, precision=0, nmed=nmed))
+    b = _cplx(ctx.tcc_build(src, sig, NA, df, L, L, Nx, Ny, precision=1, nmed=nmed))
+    assert relerr(b, a) < 1e-5
+    assert np.abs(b - b.conj().T).max() == 0.0
+
+
+def test_tcc_device_pointers_and_dipole(ctx):
+    import torch
+    print("case tcc dipole device", flush=True)
+    src = O.source("Dipole", 101, radius=0.2, offset=0.7, onx=1)
+    sig = O.outer_sigma(src)
+    Nx, Ny = O.sizing(0.9, 600, 600, sig)
+    ref = O.tcc(src, sig, 0.9, 0.1, 600, 600, Nx, Ny)
+    dsrc = torch.from_numpy(np.ascontiguousarray(src)).cuda()
+    got = ctx.tcc_build(dsrc, sig, 0.9, 0.1, 600, 600, Nx, Ny, precision=0)
+    torch.cuda.synchronize()
+    got = _cplx(got.cpu().numpy())
+    assert np.abs(got - ref).max() < 1e-6 * np.abs(ref).max()
+
+
+def test_tcc_rejects_na_above_medium(ctx):
+    src, sig = _annular()
+    with pytest.raises(klt.KltError) as e:
+        ctx.tcc_build(src, sig, 1.35, 0.3, 512, 512, 6, 6, precision=0, nmed=1.0)
+    assert "NA" in e.value.msg
