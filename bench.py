@@ -94,8 +94,9 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     ctx = klt.Context(0)
-    stream = torch.cuda.current_stream()
+    stream = torch.cuda.Stream()  # default priority; the library launches on it
     ctx.set_stream(stream.cuda_stream)
+    torch.cuda.set_stream(stream)
 
     kr, ki, sc = kernel_bank_arrays()
     bank = ctx.kernels(kr, ki, sc, KSIZE, KSIZE)

@@ -39,6 +39,13 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: run k-litho_amd/build.py")
+        try:
+            # torch ships its own libamdhip64.so.7; loading it first makes
+            # libklt bind to the same HIP runtime, so one process never holds
+            # two runtimes (torch then reports no GPU).
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp, i, u, fp = C.c_void_p, C.c_int, C.c_uint, C.c_void_p
         L.klt_last_error.restype = C.c_char_p

@@ -37,7 +37,8 @@ def test_tcc_fp64_vs_oracle(ctx, L, NA, df, nmed):
     got = _cplx(ctx.tcc_build(src, sig, NA, df, L, L, Nx, Ny, precision=0, nmed=nmed))
     tol = 1e-12 if df == 0.0 else 1e-6
     assert np.abs(got - ref).max() < tol * np.abs(ref).max()
-    assert np.abs(np.diag(got).real - np.diag(ref).real).max() < 1e-9
+    # one flipped pupil sample would move a diagonal entry by S_s >= 1e-4
+    assert np.abs(np.diag(got).real - np.diag(ref).real).max() < (1e-12 if df == 0 else 1e-6)
     assert np.abs(got - got.conj().T).max() == 0.0
 
 
