@@ -141,7 +141,8 @@ def main():
         kern[k]["share"] = kern[k]["ms_per_launch"] * kern[k]["launches"] / tot
     dom = max(kern, key=lambda k: kern[k]["ms_per_launch"] * kern[k]["launches"])
     # algorithmic HBM bytes per launch of each streaming kernel
-    alg = {"k_band_cols": 4.0 * L * L * B, "k_synth_rows": 4.0 * L * L * B}
+    alg = {"k_band_cols": 4.0 * L * L * B, "k_synth_rows": 4.0 * L * L * B,
+           "k_band_mfma": 4.0 * L * L * B, "k_synth_mfma": 4.0 * L * L * B}
     if dom in alg:
         ach = alg[dom] / (kern[dom]["ms_per_launch"] / 1e3) / 1e9
         roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
@@ -157,12 +158,12 @@ def main():
     one_m = masks[:1].contiguous()
     one_i = images[:1].contiguous()
     for _ in range(3):
-        ctx.socs_forward(bank, one_m, one_i)
+        ctx.socs_forward(bank, one_m, one_i, plain=True)
     torch.cuda.synchronize()
     n1 = 64
     e0.record(stream)
     for _ in range(n1):
-        ctx.socs_forward(bank, one_m, one_i)
+        ctx.socs_forward(bank, one_m, one_i, plain=True)
     e1.record(stream)
     torch.cuda.synchronize()
     plain = n1 / (e0.elapsed_time(e1) / 1e3)

@@ -114,10 +114,14 @@ void klt_kernels_destroy(klt_kernels* kb);
 /* ---- SOCS imaging (klitho_socs.cpp:338-343: FT_r2c + calcSOCS) ---------
  * image(x) = sum_k scales[k] |sum_n K_k[n] M^[n] e^{+2 pi i n.(x-c)/L}|^2 with
  * M^ the mask spectrum of FT_r2c (klitho_socs.cpp:71-121).  mask and image
- * are Ly x Lx float.  Errors with KLT_ERR_SIZE when 4N+1 exceeds Lx or Ly. */
+ * are Ly x Lx float.  Errors with KLT_ERR_SIZE when 4N+1 exceeds Lx or Ly.
+ * klt_socs_image is the plain single-image path: FP32 arithmetic throughout. */
 int klt_socs_image(klt_ctx* ctx, const klt_kernels* kb, const float* mask, int Lx,
                    int Ly, float* image, unsigned flags);
-/* Same for nmask masks of one size, stored back to back. */
+/* nmask masks of one size, stored back to back.  For N <= 15 the full-field
+ * DFT passes run as split hi+lo FP16 GEMMs on the matrix cores with FP32
+ * accumulation (image relative L2 error ~1e-6 against FP64); the coarse-grid
+ * FFTs are FP32. */
 int klt_socs_forward(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
                      int nmask, int Lx, int Ly, float* images, unsigned flags);
 

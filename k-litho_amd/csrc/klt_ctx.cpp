@@ -181,6 +181,7 @@ void klt_ctx_destroy(klt_ctx* ctx) {
   }
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
   ctx->socs_plans.clear();
+  ctx->fused_tables.clear();
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
 }

@@ -56,6 +56,15 @@ struct SocsPlan {
   DevBuf twM;      // [M]         e^{+2 pi i j/M}
 };
 
+// Half-precision operand tables of the fused SOCS path (socs_fused.hip):
+// W[n][x] (n = 2 nx: cos, 2 nx + 1: -sin; 32 rows, Kpad = round_up(Lx, 32)
+// columns), TY[y][j] (j = 2 gy: w cos, 2 gy + 1: -w sin; Lypad x 64) as
+// hi / lo fp16 pairs, and e^{+2 pi i j/64}.
+struct FusedTables {
+  int Kpad = 0, Lypad = 0;
+  DevBuf Wh, Wl, TYh, TYl, tw64;
+};
+
 }  // namespace klt
 
 struct klt_ctx {
@@ -74,6 +83,7 @@ struct klt_ctx {
   // scratch reused across calls (grown on demand)
   klt::DevBuf stage_in, stage_out, qpart, band, spec, ccols, misc;
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::SocsPlan>> socs_plans;
+  std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::FusedTables>> fused_tables;
   int activate() const;  // hipSetDevice(device)
 };
 
@@ -105,6 +115,13 @@ struct klt_kernels {
 namespace klt {
 // SOCS geometry plan (cached in the context).
 int get_socs_plan(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, SocsPlan** out);
+// SOCS forward passes: the FP32 path (any N <= 31) and the fused
+// split-precision path (N <= 15).
+int socs_forward_device(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int B,
+                        int Lx, int Ly, float* images);
+bool socs_fused_supported(const klt_kernels* kb);
+int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int B, int Lx,
+                       int Ly, float* images);
 // Copy helpers honouring KLT_HOST_PTRS.
 int to_device(klt_ctx* ctx, const void* src, size_t bytes, unsigned flags, DevBuf& stage,
               const void** dptr);

@@ -465,8 +465,8 @@ void klt_kernels_destroy(klt_kernels* kb) {
   delete kb;
 }
 
-int klt_socs_forward(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int nmask,
-                     int Lx, int Ly, float* images, unsigned flags) {
+static int socs_entry(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int nmask,
+                      int Lx, int Ly, float* images, unsigned flags, bool plain) {
   if (!ctx || !kb || !masks || !images)
     return klt::set_error(KLT_ERR_ARG, "klt_socs_forward: NULL argument");
   if (nmask <= 0 || Lx <= 0 || Ly <= 0)
@@ -482,7 +482,10 @@ int klt_socs_forward(klt_ctx* ctx, const klt_kernels* kb, const float* masks, in
     KLT_TRY(ctx->stage_out.ensure(bytes));
     dout = ctx->stage_out.as<float>();
   }
-  KLT_TRY(klt::socs_forward_device(ctx, kb, static_cast<const float*>(dm), nmask, Lx, Ly, dout));
+  if (plain || !klt::socs_fused_supported(kb))
+    KLT_TRY(klt::socs_forward_device(ctx, kb, static_cast<const float*>(dm), nmask, Lx, Ly, dout));
+  else
+    KLT_TRY(klt::socs_forward_fused(ctx, kb, static_cast<const float*>(dm), nmask, Lx, Ly, dout));
   if (flags & KLT_HOST_PTRS) {
     KLT_HIP(hipMemcpyAsync(images, dout, bytes, hipMemcpyDeviceToHost, ctx->stream));
     KLT_HIP(hipStreamSynchronize(ctx->stream));
@@ -490,9 +493,14 @@ int klt_socs_forward(klt_ctx* ctx, const klt_kernels* kb, const float* masks, in
   return KLT_OK;
 }
 
+int klt_socs_forward(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int nmask,
+                     int Lx, int Ly, float* images, unsigned flags) {
+  return socs_entry(ctx, kb, masks, nmask, Lx, Ly, images, flags, false);
+}
+
 int klt_socs_image(klt_ctx* ctx, const klt_kernels* kb, const float* mask, int Lx, int Ly,
                    float* image, unsigned flags) {
-  return klt_socs_forward(ctx, kb, mask, 1, Lx, Ly, image, flags);
+  return socs_entry(ctx, kb, mask, 1, Lx, Ly, image, flags, true);
 }
 
 }  // extern "C"

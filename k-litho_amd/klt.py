@@ -249,8 +249,10 @@ class Context:
     def kernels(self, kr, ki, scales, size_x: int, size_y: int) -> "KernelBank":
         return KernelBank(self, kr, ki, scales, size_x, size_y)
 
-    def socs_forward(self, bank: "KernelBank", masks, images=None):
-        """SOCS images of masks [B, Ly, Lx] (or [Ly, Lx])."""
+    def socs_forward(self, bank: "KernelBank", masks, images=None, plain: bool = False):
+        """SOCS images of masks [B, Ly, Lx] (or [Ly, Lx]).
+
+        plain=True runs klt_socs_image (the FP32 single-image path) per mask."""
         shape = tuple(masks.shape)
         Ly, Lx = shape[-2:]
         B = int(np.prod(shape[:-2])) if len(shape) > 2 else 1
@@ -262,7 +264,14 @@ class Context:
                 images = np.empty(shape, np.float32)
         pm, hm = _ptr(masks, np.float32)
         pi, hi = _ptr(images, np.float32)
-        check(lib().klt_socs_forward(self._h, bank.handle, pm, B, Lx, Ly, pi, _flags(hm, hi)))
+        fl = _flags(hm, hi)
+        if plain:
+            esz = 4 * Lx * Ly
+            for i in range(B):
+                check(lib().klt_socs_image(self._h, bank.handle, C.c_void_p(pm.value + i * esz),
+                                           Lx, Ly, C.c_void_p(pi.value + i * esz), fl))
+        else:
+            check(lib().klt_socs_forward(self._h, bank.handle, pm, B, Lx, Ly, pi, fl))
         return images
 
 

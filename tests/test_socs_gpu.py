@@ -41,16 +41,30 @@ def _random_bank(rng, Nx, Ny, nk):
     return kr, ki, sc
 
 
-@pytest.mark.parametrize("Lx,Ly,Nx,Ny", [(64, 64, 3, 3), (63, 65, 3, 2), (96, 80, 5, 6),
-                                        (128, 128, 14, 14), (57, 57, 14, 14), (200, 120, 7, 4)])
-def test_small_random_kernels(ctx, Lx, Ly, Nx, Ny):
-    print(f"case Lx={Lx} Ly={Ly} Nx={Nx} Ny={Ny}", flush=True)
+SMALL = [(64, 64, 3, 3), (63, 65, 3, 2), (96, 80, 5, 6), (128, 128, 14, 14),
+         (57, 57, 14, 14), (200, 120, 7, 4), (300, 260, 15, 15), (384, 256, 12, 9)]
+
+
+@pytest.mark.parametrize("plain", [True, False], ids=["plain", "fused"])
+@pytest.mark.parametrize("Lx,Ly,Nx,Ny", SMALL)
+def test_small_random_kernels(ctx, Lx, Ly, Nx, Ny, plain):
+    print(f"case Lx={Lx} Ly={Ly} Nx={Nx} Ny={Ny} plain={plain}", flush=True)
     rng = np.random.default_rng(Lx * 7 + Ly)
     kr, ki, sc = _random_bank(rng, Nx, Ny, 5)
     m = (rng.random((Ly, Lx)) > 0.5).astype(np.float32)
     bank = ctx.kernels(kr, ki, sc, 2 * Nx + 1, 2 * Ny + 1)
-    img = ctx.socs_forward(bank, m)
+    img = ctx.socs_forward(bank, m, plain=plain)
     assert relerr(img, O.socs_image(m, kr, ki, sc, Nx, Ny)) < TOL
+
+
+def test_continuous_mask_fused(ctx):
+    """Non-binary transmittance exercises the lo half of the split mask."""
+    print("case continuous mask", flush=True)
+    rng = np.random.default_rng(9)
+    kr, ki, sc = _random_bank(rng, 14, 14, 8)
+    m = W.relaxed(W.manhattan(512, 77), 5)
+    bank = ctx.kernels(kr, ki, sc, 29, 29)
+    assert relerr(ctx.socs_forward(bank, m), O.socs_image(m, kr, ki, sc, 14, 14)) < TOL
 
 
 def test_config0_linespace(ctx):
@@ -70,10 +84,11 @@ def test_config1_manhattan_1024(ctx, bank29):
     assert (Nx, Ny) == (14, 14)
     m = W.manhattan(1024, 1000)
     bank = ctx.kernels(kr, ki, sc, 29, 29)
-    img = ctx.socs_forward(bank, m)
     ref = O.socs_image(m, kr, ki, sc, Nx, Ny)
-    assert relerr(img, ref) < TOL
-    assert np.abs(img - ref).max() < 1e-5 * np.abs(ref).max() * 10
+    for plain in (True, False):
+        img = ctx.socs_forward(bank, m, plain=plain)
+        assert relerr(img, ref) < TOL
+        assert np.abs(img - ref).max() < 1e-4 * np.abs(ref).max()
 
 
 def test_properties_full_size(ctx, bank29):
