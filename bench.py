@@ -154,9 +154,10 @@ def main():
                 "note": "dominant kernel is compute-bound; whole-path bytes over path time"}
     path_gbs = 8.0 * L * L * B / (ms / 1e3) / 1e9
 
+    fused_img0 = images[0].cpu().numpy()
     # plain single-instance path: one call per image
-    one_m = masks[:1].contiguous()
-    one_i = images[:1].contiguous()
+    one_m = masks[:1].clone()
+    one_i = torch.empty_like(one_m)
     for _ in range(3):
         ctx.socs_forward(bank, one_m, one_i, plain=True)
     torch.cuda.synchronize()
@@ -172,8 +173,7 @@ def main():
     cpu = None
     err = None
     if not args.no_cpu:
-        gpu_img = images[0].cpu().numpy()
-        cpu, err = cpu_baseline(gpu_img, masks_h[0], kr, ki, sc, args.cpu_seconds)
+        cpu, err = cpu_baseline(fused_img0, masks_h[0], kr, ki, sc, args.cpu_seconds)
         log(f"cpu baseline: {cpu['value']:.2f} images/s; GPU rel-L2 vs FP64 oracle {err:.2e}")
 
     out = {

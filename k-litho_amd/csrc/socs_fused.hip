@@ -332,8 +332,9 @@ __global__ __launch_bounds__(256) void k_socs_mid(const float2* __restrict__ mpa
 
 // ---------------------------------------------------------------------------
 // Stage 3.  grid (ceil(Lx/128), B), 256 threads.
-// P[x][j] (j = 2 gy: Re, 2 gy + 1: Im) for the block's 128 columns, split to
-// hi/lo halves in LDS; then wave w computes rows y = 16 ((w>>1) + 2 i) ..
+// P[x][j] (j = 2 gy: Re, 2 gy + 1: Im) for the block's 128 columns, scaled by
+// a power of two and split to hi/lo halves in LDS; then wave w computes rows
+// y = 16 ((w>>1) + 2 i) ..
 // for the 64 columns (w & 1) of the strip, K = 64.
 // TY: [Lypad][64] hi/lo, TY[y][2gy] = w_gy cos, TY[y][2gy+1] = -w_gy sin.
 // ---------------------------------------------------------------------------
@@ -348,8 +349,23 @@ __global__ __launch_bounds__(256) void k_synth_mfma(const float2* __restrict__ s
   __shared__ __attribute__((aligned(16))) _Float16 sPl[kColsSynth * kPStride];
   const int b = blockIdx.y, x0 = blockIdx.x * kColsSynth, tid = threadIdx.x;
   const int Gx = 4 * Nx + 1, Hx = 2 * Nx + 1;
-  for (int i = tid; i < 32 * Gx; i += 256) sJ[i] = spec[(size_t)b * 32 * Gx + i];
+  __shared__ float red[4];
+  float part = 0.f;
+  for (int i = tid; i < 32 * Gx; i += 256) {
+    const float2 v = spec[(size_t)b * 32 * Gx + i];
+    sJ[i] = v;
+    part += fabsf(v.x) + fabsf(v.y);
+  }
+  // Power-of-two scale putting max |P| <= sum |J| near 2^12, so that the lo
+  // half of the split stays a normal fp16 number whatever the image's level.
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = part;
   __syncthreads();
+  const float bound = red[0] + red[1] + red[2] + red[3];
+  int ex = 0;
+  (void)frexpf(bound > 0.f ? bound : 1.f, &ex);
+  const float pscale = ldexpf(1.0f, 12 - ex), inv_pscale = ldexpf(1.0f, ex - 12);
   {
     const int c = tid & (kColsSynth - 1), half = tid >> 7;
     const int x = x0 + c;
@@ -372,6 +388,8 @@ __global__ __launch_bounds__(256) void k_synth_mfma(const float2* __restrict__ s
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int j = 2 * (half * 16 + q);
+      acc[q].x *= pscale;
+      acc[q].y *= pscale;
       const _Float16 hr = (_Float16)acc[q].x, hi = (_Float16)acc[q].y;
       sPh[c * kPStride + j] = hr;
       sPl[c * kPStride + j] = (_Float16)(acc[q].x - (float)hr);
@@ -409,7 +427,7 @@ __global__ __launch_bounds__(256) void k_synth_mfma(const float2* __restrict__ s
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int y = yb + (lane >> 4) * 4 + r;
-        if (y < Ly && x < Lx) img[(size_t)y * Lx + x] = d[r];
+        if (y < Ly && x < Lx) img[(size_t)y * Lx + x] = d[r] * inv_pscale;
       }
     }
   }
