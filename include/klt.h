@@ -102,6 +102,22 @@ int klt_tcc_build(klt_ctx* ctx, const float* src, int srcSize, float outerSigma,
                   float NA, float defocus, float nmed, int Lx, int Ly, int Nx, int Ny,
                   int precision, double* tcc, unsigned flags);
 
+/* ---- eigensolver (klitho_tcc.cpp:301-377, calcKernels / zheevr_) --------
+ * FP64 on the GPU: Householder tridiagonalisation, bisection for the k
+ * largest eigenvalues, inverse iteration with re-orthogonalisation inside
+ * clusters, back-transformation.  a: n x n complex double Hermitian (full
+ * storage; like zheevr_ with uplo='U' the result depends only on the upper
+ * triangle for an exactly Hermitian input).  w: k eigenvalues, descending;
+ * vecs: k x n complex, row q the unit eigenvector of w[q] (phase arbitrary,
+ * as in LAPACK). */
+int klt_eig_topk(klt_ctx* ctx, const double* a, int n, int k, double* w, double* vecs,
+                 unsigned flags);
+/* calcKernels: the nk (clamped to T) kernels of the largest eigenvalues in the
+ * krn_i_{r,i}.bin layout, krn[q][j] = z_q[T-1-j] as float (klitho_tcc.cpp:
+ * 358-362), and the eigenvalues as float (:344-347).  krn_r, krn_i: nk x T. */
+int klt_calc_kernels(klt_ctx* ctx, const double* tcc, int T, int nk, float* krn_r,
+                     float* krn_i, float* scales, unsigned flags);
+
 /* ---- SOCS kernel bank --------------------------------------------------
  * Replaces the kernel read loop of klitho_socs.cpp:297-318: nk kernels of
  * sizeX x sizeY (odd) in the krn_i_{r,i}.bin layout, and their eigenvalues

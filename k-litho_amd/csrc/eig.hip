@@ -1,0 +1,558 @@
+// eig.hip -- top-k eigenpairs of a Hermitian matrix in FP64 on MI355X
+// (replaces zheevr_ in calcKernels, K-Litho-TCC/klitho_tcc.cpp:301-377).
+//
+//   1. Householder reduction to tridiagonal form, one column per step:
+//        k_hh_symv : every block rebuilds the reflector v, tau of row j
+//                    (x = conj(A[j][j+1:])), then computes its rows of
+//                    p = tau A22 v and a partial sum of v^H p;
+//        k_hh_her2 : A22 -= v w^H + w v^H, w = p - (tau/2)(v^H p) v.
+//      The off-diagonal entries are complex; a diagonal unitary D maps the
+//      tridiagonal T to the real symmetric T_r = D^H T D with off-diagonal
+//      |e_j|.
+//   2. k_bisect: the k largest eigenvalues of T_r by bisection on Sturm
+//      counts (one thread per eigenvalue).
+//   3. k_inverse_iteration: eigenvectors of T_r by inverse iteration with a
+//      partially pivoted LU of T_r - lambda I; eigenvalues closer than
+//      1e-3 ||T|| form a cluster whose vectors are re-orthogonalised
+//      (modified Gram-Schmidt) at every iteration, one thread per cluster.
+//   4. k_back_transform: x = H_0 ... H_{n-3} D z, one block per vector.
+// Textbook algorithms (Golub & Van Loan, ch. 8; Wilkinson's inverse
+// iteration); all arithmetic FP64.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "klt_internal.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+
+struct cd {
+  double x, y;
+};
+
+__device__ __forceinline__ cd cmul(cd a, cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ cd cmulc(cd a, cd b) {  // conj(a) * b
+  return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block sum of (a, b) over 256 threads; valid in every thread.
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    red[2 * w] = a;
+    red[2 * w + 1] = b;
+  }
+  __syncthreads();
+  a = red[0] + red[2] + red[4] + red[6];
+  b = red[1] + red[3] + red[5] + red[7];
+}
+
+// Step j of the reduction.  A: n x n complex (row-major, full Hermitian
+// storage).  Each block rebuilds the reflector of x = conj(A[j][j+1:]) in
+// LDS, then computes p_i = tau sum_l A[j+1+i][j+1+l] v_l for rows
+// i in [blk*rows, ...) and the partial sum of conj(v_i) p_i.
+__global__ __launch_bounds__(kThreads) void k_hh_symv(double* __restrict__ A, int n, int j,
+                                                      int rows, cd* __restrict__ V,
+                                                      double* __restrict__ tau,
+                                                      cd* __restrict__ sub,
+                                                      double* __restrict__ diag,
+                                                      cd* __restrict__ p,
+                                                      cd* __restrict__ part) {
+  extern __shared__ cd sv[];  // m entries
+  __shared__ double red[8];
+  const int m = n - j - 1;
+  const cd* Aj = reinterpret_cast<const cd*>(A) + (size_t)j * n + j + 1;
+  double s2 = 0.0, dummy = 0.0;
+  for (int i = threadIdx.x; i < m; i += kThreads) {
+    const cd a = Aj[i];
+    const cd xi = {a.x, -a.y};
+    sv[i] = xi;
+    s2 += xi.x * xi.x + xi.y * xi.y;
+  }
+  block_sum2(s2, dummy, red);
+  const cd x0 = sv[0];
+  const double ax0 = hypot(x0.x, x0.y);
+  const double alpha = sqrt(s2);
+  const bool skip = (s2 - ax0 * ax0) <= 0.0;  // x already a multiple of e_1
+  cd ph = {1.0, 0.0};
+  if (ax0 > 0.0) ph = {x0.x / ax0, x0.y / ax0};
+  const double t = skip ? 0.0 : 1.0 / (alpha * (alpha + ax0));  // 2 / (v^H v)
+  __syncthreads();
+  if (threadIdx.x == 0 && !skip) {
+    sv[0].x += ph.x * alpha;
+    sv[0].y += ph.y * alpha;
+  }
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    cd* Vj = V + (size_t)j * n + j + 1;
+    for (int i = threadIdx.x; i < m; i += kThreads) Vj[i] = skip ? cd{0.0, 0.0} : sv[i];
+    if (threadIdx.x == 0) {
+      tau[j] = t;
+      sub[j] = skip ? x0 : cd{-ph.x * alpha, -ph.y * alpha};
+      diag[j] = A[2 * ((size_t)j * n + j)];
+    }
+  }
+  // rows of p, one wave per row
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r0 = blockIdx.x * rows, r1 = min(m, r0 + rows);
+  double pr = 0.0, pi = 0.0;  // partial conj(v) . p of this wave
+  for (int i = r0 + wave; i < r1; i += 4) {
+    const cd* Ai = reinterpret_cast<const cd*>(A) + (size_t)(j + 1 + i) * n + j + 1;
+    double sr = 0.0, si = 0.0;
+    if (!skip) {
+      for (int l = lane; l < m; l += 64) {
+        const cd a = Ai[l], v = sv[l];
+        sr += a.x * v.x - a.y * v.y;
+        si += a.x * v.y + a.y * v.x;
+      }
+    }
+    sr = wave_sum(sr) * t;
+    si = wave_sum(si) * t;
+    if (lane == 0) {
+      p[i] = {sr, si};
+      const cd vi = sv[i];
+      pr += vi.x * sr + vi.y * si;
+      pi += vi.x * si - vi.y * sr;
+    }
+  }
+  block_sum2(pr, pi, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = {pr, pi};
+}
+
+// A22 -= v w^H + w v^H on 64 x 64 tiles; w = p - (tau/2)(v^H p) v.
+__global__ __launch_bounds__(kThreads) void k_hh_her2(double* __restrict__ A, int n, int j,
+                                                      const cd* __restrict__ V,
+                                                      const double* __restrict__ tau,
+                                                      const cd* __restrict__ p,
+                                                      const cd* __restrict__ part, int nparts) {
+  __shared__ cd sv_i[64], sw_i[64], sv_l[64], sw_l[64];
+  __shared__ double red[8];
+  const int m = n - j - 1;
+  const double t = tau[j];
+  if (t == 0.0) return;  // no reflection this step (uniform over the grid)
+  double kr = 0.0, ki = 0.0;
+  for (int q = threadIdx.x; q < nparts; q += kThreads) {
+    kr += part[q].x;
+    ki += part[q].y;
+  }
+  block_sum2(kr, ki, red);
+  const cd K = {0.5 * t * kr, 0.5 * t * ki};
+  const cd* v = V + (size_t)j * n + j + 1;
+  const int i0 = blockIdx.y * 64, l0 = blockIdx.x * 64;
+  if (threadIdx.x < 64) {
+    const int i = i0 + threadIdx.x;
+    cd vi = {0.0, 0.0}, wi = {0.0, 0.0};
+    if (i < m) {
+      vi = v[i];
+      const cd kv = cmul(K, vi);
+      wi = {p[i].x - kv.x, p[i].y - kv.y};
+    }
+    sv_i[threadIdx.x] = vi;
+    sw_i[threadIdx.x] = wi;
+  } else if (threadIdx.x < 128) {
+    const int l = l0 + threadIdx.x - 64;
+    cd vl = {0.0, 0.0}, wl = {0.0, 0.0};
+    if (l < m) {
+      vl = v[l];
+      const cd kv = cmul(K, vl);
+      wl = {p[l].x - kv.x, p[l].y - kv.y};
+    }
+    sv_l[threadIdx.x - 64] = vl;
+    sw_l[threadIdx.x - 64] = wl;
+  }
+  __syncthreads();
+  const int tl = threadIdx.x & 63, tr = threadIdx.x >> 6;
+  const int l = l0 + tl;
+  if (l >= m) return;
+  const cd vl = sv_l[tl], wl = sw_l[tl];
+  for (int r = tr; r < 64; r += 4) {
+    const int i = i0 + r;
+    if (i >= m) break;
+    const cd vi = sv_i[r], wi = sw_i[r];
+    cd* a = reinterpret_cast<cd*>(A) + (size_t)(j + 1 + i) * n + j + 1 + l;
+    // v_i conj(w_l) + w_i conj(v_l)
+    const double ur = (vi.x * wl.x + vi.y * wl.y) + (wi.x * vl.x + wi.y * vl.y);
+    const double ui = (vi.y * wl.x - vi.x * wl.y) + (wi.y * vl.x - wi.x * vl.y);
+    cd av = *a;
+    av.x -= ur;
+    av.y -= ui;
+    *a = av;
+  }
+}
+
+// The last 2 x 2 block after the reduction.
+__global__ void k_hh_finish(const double* __restrict__ A, int n, cd* __restrict__ sub,
+                            double* __restrict__ diag) {
+  if (threadIdx.x != 0) return;
+  if (n >= 2) {
+    const cd a = reinterpret_cast<const cd*>(A)[(size_t)(n - 2) * n + n - 1];
+    sub[n - 2] = {a.x, -a.y};  // A[n-1][n-2] = conj(A[n-2][n-1])
+    diag[n - 2] = A[2 * ((size_t)(n - 2) * n + n - 2)];
+  }
+  diag[n - 1] = A[2 * ((size_t)(n - 1) * n + n - 1)];
+}
+
+// Number of eigenvalues of T_r below x (Sturm sequence of the LDL^T pivots).
+__device__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
+  int c = 0;
+  double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  if (q < 0) ++c;
+  for (int i = 1; i < n; ++i) {
+    q = d[i] - x - e2[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    if (q < 0) ++c;
+  }
+  return c;
+}
+
+// Thread idx finds the (idx+1)-th largest eigenvalue.
+__global__ void k_bisect(const double* __restrict__ d, const double* __restrict__ e2, int n,
+                         int k, double lo0, double hi0, double pivmin,
+                         double* __restrict__ w) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= k) return;
+  const int target = n - 1 - idx;  // ascending index
+  double lo = lo0, hi = hi0;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (hi - lo <= 2.2e-16 * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+    if (sturm_count(d, e2, n, mid, pivmin) > target) hi = mid;
+    else lo = mid;
+  }
+  w[idx] = 0.5 * (lo + hi);
+}
+
+// Inverse iteration, one thread per cluster of eigenvalues [first, last).
+// z: k x n (rows), work: per cluster 5n doubles.
+__global__ void k_inverse_iteration(const double* __restrict__ d, const double* __restrict__ e,
+                                    int n, const double* __restrict__ w,
+                                    const int* __restrict__ cl_begin, int ncl, double tnorm,
+                                    double* __restrict__ z, double* __restrict__ work) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncl) return;
+  const int first = cl_begin[c], last = cl_begin[c + 1];
+  double* lu_d = work + (size_t)c * 5 * n;  // U diagonal
+  double* lu_u1 = lu_d + n;                 // U first superdiagonal
+  double* lu_u2 = lu_u1 + n;                // U second superdiagonal (from swaps)
+  double* lu_l = lu_u2 + n;                 // multipliers
+  double* piv = lu_l + n;                   // 1 when rows i, i+1 were swapped
+  const double eps = 2.220446049250313e-16;
+  for (int q = first; q < last; ++q) {
+    // separate (numerically) equal eigenvalues of a cluster slightly
+    const double lam = w[q] + (double)(q - first) * 10.0 * eps * tnorm;
+    // LU of T - lam I with partial pivoting (rows i, i+1)
+    double a = d[0] - lam, b = n > 1 ? e[0] : 0.0;
+    for (int i = 0; i < n - 1; ++i) {
+      const double cdn = e[i], dn = d[i + 1] - lam, en = i + 1 < n - 1 ? e[i + 1] : 0.0;
+      if (fabs(a) >= fabs(cdn)) {
+        if (a == 0.0) a = eps * tnorm;
+        const double l = cdn / a;
+        lu_d[i] = a;
+        lu_u1[i] = b;
+        lu_u2[i] = 0.0;
+        lu_l[i] = l;
+        piv[i] = 0.0;
+        a = dn - l * b;
+        b = en;
+      } else {
+        const double l = a / cdn;
+        lu_d[i] = cdn;
+        lu_u1[i] = dn;
+        lu_u2[i] = en;
+        lu_l[i] = l;
+        piv[i] = 1.0;
+        a = b - l * dn;
+        b = -l * en;
+      }
+    }
+    if (a == 0.0) a = eps * tnorm;
+    lu_d[n - 1] = a;
+    double* x = z + (size_t)q * n;
+    // deterministic start vector
+    unsigned s = 2463534242u ^ (unsigned)(q * 2654435761u);
+    for (int i = 0; i < n; ++i) {
+      s ^= s << 13;
+      s ^= s >> 17;
+      s ^= s << 5;
+      x[i] = ((double)(s & 0xffffff) / 16777216.0) - 0.5;
+    }
+    for (int it = 0; it < 6; ++it) {
+      // forward: apply the row swaps and multipliers
+      for (int i = 0; i < n - 1; ++i) {
+        if (piv[i] != 0.0) {
+          const double tmp = x[i];
+          x[i] = x[i + 1];
+          x[i + 1] = tmp - lu_l[i] * x[i];
+        } else {
+          x[i + 1] -= lu_l[i] * x[i];
+        }
+      }
+      // back substitution with U (diag, super1, super2)
+      x[n - 1] /= lu_d[n - 1];
+      if (n >= 2) x[n - 2] = (x[n - 2] - lu_u1[n - 2] * x[n - 1]) / lu_d[n - 2];
+      for (int i = n - 3; i >= 0; --i)
+        x[i] = (x[i] - lu_u1[i] * x[i + 1] - lu_u2[i] * x[i + 2]) / lu_d[i];
+      // re-orthogonalise against the cluster's earlier vectors, normalise
+      for (int r = first; r < q; ++r) {
+        const double* y = z + (size_t)r * n;
+        double dot = 0.0;
+        for (int i = 0; i < n; ++i) dot += y[i] * x[i];
+        for (int i = 0; i < n; ++i) x[i] -= dot * y[i];
+      }
+      double nrm = 0.0;
+      for (int i = 0; i < n; ++i) nrm += x[i] * x[i];
+      nrm = 1.0 / sqrt(nrm);
+      for (int i = 0; i < n; ++i) x[i] *= nrm;
+    }
+  }
+}
+
+// x = H_0 ... H_{n-3} D z for vector blockIdx.x; D_i = prod_{r<i} sub[r]/|sub[r]|.
+__global__ __launch_bounds__(kThreads) void k_back_transform(const double* __restrict__ zr,
+                                                             int n, const cd* __restrict__ V,
+                                                             const double* __restrict__ tau,
+                                                             const cd* __restrict__ sub,
+                                                             cd* __restrict__ out) {
+  extern __shared__ cd sy[];  // n entries
+  __shared__ double red[8];
+  const int q = blockIdx.x;
+  const double* z = zr + (size_t)q * n;
+  if (threadIdx.x == 0) {
+    cd dl = {1.0, 0.0};
+    sy[0] = {z[0], 0.0};
+    for (int i = 0; i + 1 < n; ++i) {
+      const cd s = sub[i];
+      const double a = hypot(s.x, s.y);
+      const cd u = a > 0.0 ? cd{s.x / a, s.y / a} : cd{1.0, 0.0};
+      dl = cmul(dl, u);
+      sy[i + 1] = {dl.x * z[i + 1], dl.y * z[i + 1]};
+    }
+  }
+  __syncthreads();
+  for (int j = n - 3; j >= 0; --j) {
+    const double t = tau[j];
+    if (t == 0.0) continue;  // uniform
+    const cd* v = V + (size_t)j * n;
+    double sr = 0.0, si = 0.0;
+    for (int i = j + 1 + threadIdx.x; i < n; i += kThreads) {
+      const cd c = cmulc(v[i], sy[i]);
+      sr += c.x;
+      si += c.y;
+    }
+    block_sum2(sr, si, red);
+    const cd s = {t * sr, t * si};
+    for (int i = j + 1 + threadIdx.x; i < n; i += kThreads) {
+      const cd c = cmul(v[i], s);
+      sy[i].x -= c.x;
+      sy[i].y -= c.y;
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < n; i += kThreads) out[(size_t)q * n + i] = sy[i];
+}
+
+// Kernel files layout (klitho_tcc.cpp:344-362): eigenvalues as float, and
+// krn_q[j] = z_q[T-1-j] as float real / imaginary parts.
+__global__ void k_kernels_from_vectors(const cd* __restrict__ vec, const double* __restrict__ w,
+                                       int k, int T, float* __restrict__ kr,
+                                       float* __restrict__ ki, float* __restrict__ scales) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < k) scales[i] = (float)w[i];
+  if (i >= k * T) return;
+  const int q = i / T, jj = i - q * T;
+  const cd v = vec[(size_t)q * T + (T - 1 - jj)];
+  kr[i] = (float)v.x;
+  ki[i] = (float)v.y;
+}
+
+}  // namespace
+
+namespace klt {
+
+// Top-k eigenpairs of the Hermitian n x n matrix a_dev (device, complex
+// double, full storage; not modified).  w_dev: k eigenvalues (descending);
+// vec_dev: k x n complex eigenvectors (rows, unit norm).
+int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_dev,
+                    double* vec_dev) {
+  if (n <= 0 || k <= 0 || k > n) return set_error(KLT_ERR_ARG, "eig: bad n or k");
+  if (n > 8192) return set_error(KLT_ERR_ARG, "eig: n above 8192 is not supported");
+  const size_t nn = (size_t)n * n;
+  // workspace: A copy, V, tau, sub, diag, p, parts, e, e2, z, lu work, clusters
+  const int nparts_max = 256;
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t oA = carve(nn * 16), oV = carve(nn * 16), oTau = carve(n * 8),
+               oSub = carve(n * 16), oDiag = carve(n * 8), oP = carve(n * 16),
+               oPart = carve(nparts_max * 16), oE = carve(n * 8), oE2 = carve(n * 8),
+               oZ = carve((size_t)k * n * 8), oWork = carve((size_t)k * 5 * n * 8),
+               oCl = carve((k + 1) * 4);
+  KLT_TRY(ctx->eig_ws.ensure(off));
+  char* base = ctx->eig_ws.as<char>();
+  double* A = reinterpret_cast<double*>(base + oA);
+  cd* V = reinterpret_cast<cd*>(base + oV);
+  double* tau = reinterpret_cast<double*>(base + oTau);
+  cd* sub = reinterpret_cast<cd*>(base + oSub);
+  double* diag = reinterpret_cast<double*>(base + oDiag);
+  cd* p = reinterpret_cast<cd*>(base + oP);
+  cd* part = reinterpret_cast<cd*>(base + oPart);
+  double* e = reinterpret_cast<double*>(base + oE);
+  double* e2 = reinterpret_cast<double*>(base + oE2);
+  double* z = reinterpret_cast<double*>(base + oZ);
+  double* work = reinterpret_cast<double*>(base + oWork);
+  int* cl = reinterpret_cast<int*>(base + oCl);
+  hipStream_t st = ctx->stream;
+  KLT_HIP(hipMemcpyAsync(A, a_dev, nn * 16, hipMemcpyDeviceToDevice, st));
+  KLT_HIP(hipMemsetAsync(tau, 0, n * 8, st));
+  KLT_HIP(hipMemsetAsync(sub, 0, n * 16, st));
+  const size_t lds_v = std::max<size_t>(16, (size_t)n * 16);
+  if (lds_v > 64 * 1024) {
+    KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hh_symv),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_v));
+    KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_back_transform),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_v));
+  }
+  for (int j = 0; j + 2 < n; ++j) {
+    const int m = n - j - 1;
+    const int nblk = std::min(nparts_max, (m + 15) / 16);
+    const int rows = (m + nblk - 1) / nblk;
+    const int np = (m + rows - 1) / rows;
+    KLT_LAUNCH(ctx, "k_hh_symv", k_hh_symv, dim3(np), dim3(kThreads), (size_t)m * 16, A, n, j,
+               rows, V, tau, sub, diag, p, part);
+    const int tiles = (m + 63) / 64;
+    KLT_LAUNCH(ctx, "k_hh_her2", k_hh_her2, dim3(tiles, tiles), dim3(kThreads), 0, A, n, j, V,
+               tau, p, part, np);
+  }
+  KLT_LAUNCH(ctx, "k_hh_finish", k_hh_finish, dim3(1), dim3(64), 0, A, n, sub, diag);
+  KLT_HIP(hipGetLastError());
+  // real tridiagonal, Gershgorin bounds, pivot threshold (host: O(n))
+  std::vector<double> hd(n);
+  std::vector<cd> hs(n);
+  KLT_HIP(hipMemcpyAsync(hd.data(), diag, n * 8, hipMemcpyDeviceToHost, st));
+  KLT_HIP(hipMemcpyAsync(hs.data(), sub, n * 16, hipMemcpyDeviceToHost, st));
+  KLT_HIP(hipStreamSynchronize(st));
+  std::vector<double> he(n, 0.0), he2(n, 0.0);
+  double lo = 1e300, hi = -1e300, tnorm = 0.0;
+  for (int i = 0; i < n; ++i) {
+    if (i + 1 < n) {
+      he[i] = std::hypot(hs[i].x, hs[i].y);
+      he2[i] = he[i] * he[i];
+    }
+    const double r = (i > 0 ? he[i - 1] : 0.0) + (i + 1 < n ? he[i] : 0.0);
+    lo = std::min(lo, hd[i] - r);
+    hi = std::max(hi, hd[i] + r);
+    tnorm = std::max(tnorm, std::fabs(hd[i]) + r);
+  }
+  if (!std::isfinite(lo) || !std::isfinite(hi))
+    return set_error(KLT_ERR_EIG, "eig: non-finite tridiagonal (input has NaN/Inf)");
+  const double pivmin = std::max(1e-300, 2.2e-16 * 2.2e-16 * std::max(1.0, tnorm * tnorm)) ;
+  lo -= 2.2e-16 * tnorm + pivmin;
+  hi += 2.2e-16 * tnorm + pivmin;
+  KLT_HIP(hipMemcpyAsync(e, he.data(), n * 8, hipMemcpyHostToDevice, st));
+  KLT_HIP(hipMemcpyAsync(e2, he2.data(), n * 8, hipMemcpyHostToDevice, st));
+  KLT_LAUNCH(ctx, "k_bisect", k_bisect, dim3((k + 63) / 64), dim3(64), 0, diag, e2, n, k, lo, hi,
+             pivmin, w_dev);
+  std::vector<double> hw(k);
+  KLT_HIP(hipMemcpyAsync(hw.data(), w_dev, k * 8, hipMemcpyDeviceToHost, st));
+  KLT_HIP(hipStreamSynchronize(st));
+  // clusters of close eigenvalues (descending order)
+  std::vector<int> hcl;
+  hcl.push_back(0);
+  for (int q = 1; q < k; ++q)
+    if (std::fabs(hw[q - 1] - hw[q]) > 1e-3 * tnorm) hcl.push_back(q);
+  hcl.push_back(k);
+  const int ncl = (int)hcl.size() - 1;
+  KLT_HIP(hipMemcpyAsync(cl, hcl.data(), hcl.size() * 4, hipMemcpyHostToDevice, st));
+  KLT_LAUNCH(ctx, "k_inverse_iteration", k_inverse_iteration, dim3((ncl + 63) / 64), dim3(64), 0,
+             diag, e, n, w_dev, cl, ncl, tnorm, z, work);
+  KLT_LAUNCH(ctx, "k_back_transform", k_back_transform, dim3(k), dim3(kThreads), lds_v, z, n, V,
+             tau, sub, reinterpret_cast<cd*>(vec_dev));
+  KLT_HIP(hipGetLastError());
+  KLT_HIP(hipStreamSynchronize(st));  // host vectors above must outlive the copies
+  return KLT_OK;
+}
+
+}  // namespace klt
+
+extern "C" {
+
+int klt_eig_topk(klt_ctx* ctx, const double* a, int n, int k, double* w, double* vecs,
+                 unsigned flags) {
+  if (!ctx || !a || !w || !vecs) return klt::set_error(KLT_ERR_ARG, "klt_eig_topk: NULL argument");
+  if (n <= 0 || k <= 0) return klt::set_error(KLT_ERR_ARG, "klt_eig_topk: bad size");
+  if (k > n) k = n;
+  KLT_TRY(ctx->activate());
+  const size_t nn = (size_t)n * n;
+  const double* da = a;
+  double* dw = w;
+  double* dv = vecs;
+  if (flags & KLT_HOST_PTRS) {
+    KLT_TRY(ctx->stage_in.ensure(nn * 16));
+    KLT_TRY(ctx->stage_out.ensure((size_t)k * n * 16 + (size_t)k * 8 + 256));
+    KLT_HIP(hipMemcpyAsync(ctx->stage_in.ptr, a, nn * 16, hipMemcpyHostToDevice, ctx->stream));
+    da = ctx->stage_in.as<double>();
+    dv = ctx->stage_out.as<double>();
+    dw = dv + (size_t)k * n * 2;
+  }
+  KLT_TRY(klt::eig_topk_device(ctx, da, n, k, dw, dv));
+  if (flags & KLT_HOST_PTRS) {
+    KLT_HIP(hipMemcpyAsync(vecs, dv, (size_t)k * n * 16, hipMemcpyDeviceToHost, ctx->stream));
+    KLT_HIP(hipMemcpyAsync(w, dw, (size_t)k * 8, hipMemcpyDeviceToHost, ctx->stream));
+    KLT_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return KLT_OK;
+}
+
+int klt_calc_kernels(klt_ctx* ctx, const double* tcc, int T, int nk, float* krn_r,
+                     float* krn_i, float* scales, unsigned flags) {
+  if (!ctx || !tcc || !krn_r || !krn_i || !scales)
+    return klt::set_error(KLT_ERR_ARG, "klt_calc_kernels: NULL argument");
+  if (T <= 0 || nk <= 0) return klt::set_error(KLT_ERR_ARG, "klt_calc_kernels: bad size");
+  if (nk > T) nk = T;  // klitho_tcc.cpp:488-491
+  KLT_TRY(ctx->activate());
+  const size_t nn = (size_t)T * T;
+  const double* da = tcc;
+  if (flags & KLT_HOST_PTRS) {
+    KLT_TRY(ctx->stage_in.ensure(nn * 16));
+    KLT_HIP(hipMemcpyAsync(ctx->stage_in.ptr, tcc, nn * 16, hipMemcpyHostToDevice, ctx->stream));
+    da = ctx->stage_in.as<double>();
+  }
+  const size_t vb = (size_t)nk * T * 16, wb = (size_t)nk * 8, kb = (size_t)nk * T * 4;
+  KLT_TRY(ctx->stage_out.ensure(vb + wb + 2 * kb + nk * 4 + 1024));
+  char* o = ctx->stage_out.as<char>();
+  double* dv = reinterpret_cast<double*>(o);
+  double* dw = reinterpret_cast<double*>(o + vb);
+  float* fr = reinterpret_cast<float*>(o + vb + wb);
+  float* fi = fr + (size_t)nk * T;
+  float* fs = fi + (size_t)nk * T;
+  KLT_TRY(klt::eig_topk_device(ctx, da, T, nk, dw, dv));
+  const int tot = nk * T;
+  KLT_LAUNCH(ctx, "k_kernels_from_vectors", k_kernels_from_vectors, dim3((tot + 255) / 256),
+             dim3(256), 0, reinterpret_cast<const cd*>(dv), dw, nk, T, fr, fi, fs);
+  const hipMemcpyKind kind =
+      (flags & KLT_HOST_PTRS) ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  KLT_HIP(hipMemcpyAsync(krn_r, fr, kb, kind, ctx->stream));
+  KLT_HIP(hipMemcpyAsync(krn_i, fi, kb, kind, ctx->stream));
+  KLT_HIP(hipMemcpyAsync(scales, fs, nk * 4, kind, ctx->stream));
+  KLT_HIP(hipStreamSynchronize(ctx->stream));
+  return KLT_OK;
+}
+
+}  // extern "C"

@@ -81,7 +81,7 @@ struct klt_ctx {
   std::vector<hipEvent_t> event_pool;
   hipEvent_t take_event();
   // scratch reused across calls (grown on demand)
-  klt::DevBuf stage_in, stage_out, qpart, band, spec, ccols, misc;
+  klt::DevBuf stage_in, stage_out, qpart, band, spec, ccols, misc, eig_ws;
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::SocsPlan>> socs_plans;
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::FusedTables>> fused_tables;
   int activate() const;  // hipSetDevice(device)
@@ -122,6 +122,9 @@ int socs_forward_device(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
 bool socs_fused_supported(const klt_kernels* kb);
 int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int B, int Lx,
                        int Ly, float* images);
+// Top-k eigenpairs of a Hermitian matrix on the device (eig.hip).
+int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_dev,
+                    double* vec_dev);
 // Copy helpers honouring KLT_HOST_PTRS.
 int to_device(klt_ctx* ctx, const void* src, size_t bytes, unsigned flags, DevBuf& stage,
               const void** dptr);

@@ -72,6 +72,8 @@ def lib() -> C.CDLL:
         L.klt_write_kernel_info.argtypes = [cs, i, i, i, fp]
         L.klt_read_kernel_info.argtypes = [cs, C.POINTER(i), C.POINTER(i), fp, i]
         L.klt_tcc_build.argtypes = [vp, fp, i, f, f, f, f, f, i, i, i, i, i, fp, u]
+        L.klt_eig_topk.argtypes = [vp, fp, i, i, fp, fp, u]
+        L.klt_calc_kernels.argtypes = [vp, fp, i, i, fp, fp, fp, u]
         L.klt_kernels_create.argtypes = [vp, fp, fp, fp, i, i, i, u, C.POINTER(vp)]
         L.klt_kernels_destroy.argtypes = [vp]
         L.klt_kernels_destroy.restype = None
@@ -245,6 +247,45 @@ class Context:
         check(lib().klt_tcc_build(self._h, ps, n, sigma, lam, NA, defocus, nmed, Lx, Ly, Nx, Ny,
                                   precision, po, _flags(hs, ho)))
         return out
+
+    def eig_topk(self, a, k: int):
+        """Top-k eigenpairs of a Hermitian matrix a [n, n, 2] (re, im), FP64 on the GPU.
+
+        Returns (w [k], vecs [k, n, 2]) on the same side (host/device) as a."""
+        n = int(a.shape[0])
+        k = min(k, n)
+        if _is_torch(a):
+            import torch
+            w = torch.empty(k, dtype=torch.float64, device=a.device)
+            v = torch.empty((k, n, 2), dtype=torch.float64, device=a.device)
+        else:
+            w = np.empty(k, np.float64)
+            v = np.empty((k, n, 2), np.float64)
+        pa, ha = _ptr(a, np.float64)
+        pw, hw = _ptr(w, np.float64)
+        pv, hv = _ptr(v, np.float64)
+        check(lib().klt_eig_topk(self._h, pa, n, k, pw, pv, _flags(ha, hw, hv)))
+        return w, v
+
+    def calc_kernels(self, tcc, nk: int):
+        """calcKernels: (krn_r [nk, T], krn_i [nk, T], scales [nk]) float32."""
+        T = int(tcc.shape[0])
+        nk = min(nk, T)
+        if _is_torch(tcc):
+            import torch
+            kr = torch.empty((nk, T), dtype=torch.float32, device=tcc.device)
+            ki = torch.empty_like(kr)
+            sc = torch.empty(nk, dtype=torch.float32, device=tcc.device)
+        else:
+            kr = np.empty((nk, T), np.float32)
+            ki = np.empty((nk, T), np.float32)
+            sc = np.empty(nk, np.float32)
+        pt, ht = _ptr(tcc, np.float64)
+        p1, h1 = _ptr(kr, np.float32)
+        p2, h2 = _ptr(ki, np.float32)
+        p3, h3 = _ptr(sc, np.float32)
+        check(lib().klt_calc_kernels(self._h, pt, T, nk, p1, p2, p3, _flags(ht, h1, h2, h3)))
+        return kr, ki, sc
 
     def kernels(self, kr, ki, scales, size_x: int, size_y: int) -> "KernelBank":
         return KernelBank(self, kr, ki, scales, size_x, size_y)

@@ -1,0 +1,114 @@
+"""FP64 eigensolver on the GPU (klt_eig_topk / klt_calc_kernels) against the
+oracle's Householder + QL solver (itself pinned to LAPACK zheevr):
+
+* eigenvalues within 1e-12 relative of the oracle;
+* residual ||A v - lambda v|| <= 1e-11 ||A||, unit norm, orthogonality;
+* eigenvector phases are arbitrary and degenerate pairs are common (an
+  annular source gives exact pairs), so vectors are compared through cluster
+  projectors sum_{q in cluster} v_q v_q^H;
+* calcKernels: reversal krn[j] = z[T-1-j], float storage, and the SOCS image
+  from the GPU kernels equals the image from the oracle's kernels.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import workloads as W
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _c(a):
+    return a[..., 0] + 1j * a[..., 1]
+
+
+def _pack(a):
+    return np.ascontiguousarray(np.stack([a.real, a.imag], -1), np.float64)
+
+
+def _clusters(w, tol):
+    out, i = [], 0
+    while i < len(w):
+        j = i + 1
+        while j < len(w) and abs(w[j] - w[i]) < tol:
+            j += 1
+        out.append((i, j))
+        i = j
+    return out
+
+
+def _check(a, w, v, wo, vo, k):
+    nrm = np.abs(np.linalg.eigvalsh(a)).max()
+    assert np.abs(w - wo).max() < 1e-12 * nrm
+    for q in range(k):
+        assert np.linalg.norm(a @ v[q] - w[q] * v[q]) < 1e-11 * nrm
+    G = v @ v.conj().T
+    assert np.abs(G - np.eye(k)).max() < 1e-11
+    for (i, j) in _clusters(wo, 1e-9 * nrm):
+        if j == k and j < a.shape[0]:
+            continue  # cluster cut by k: its projector is not defined by k vectors
+        P1 = v[i:j].T @ v[i:j].conj()
+        P2 = vo[i:j].T @ vo[i:j].conj()
+        assert np.abs(P1 - P2).max() < 1e-9
+
+
+def _tcc(L, kind="Annular", NA=1.35, df=0.0, nmed=1.44, **kw):
+    src = O.source(kind, kw.pop("n", 51), **kw)
+    sig = O.outer_sigma(src)
+    Nx, Ny = O.sizing(NA, L, L, sig)
+    return O.tcc(src, sig, NA, df, L, L, Nx, Ny, nmed=nmed), Nx, Ny
+
+
+@pytest.mark.parametrize("case", ["cfg0", "cfg1", "dipole_defocus"])
+def test_eig_tcc_matches_oracle(ctx, case):
+    print(f"case eig {case}", flush=True)
+    if case == "cfg0":
+        a, _, _ = _tcc(512, inner=0.6, outer=0.9)
+        k = 24
+    elif case == "cfg1":
+        a, _, _ = _tcc(W.PERIOD_29, inner=0.6, outer=0.9)
+        k = 24
+    else:
+        a, _, _ = _tcc(900, "Dipole", NA=1.35, df=0.2361, n=101, radius=0.2, offset=0.7, onx=1)
+        k = 32
+    wo, vo = O.eigh_topk(a, k)
+    w, v = ctx.eig_topk(_pack(a), k)
+    _check(a, w, _c(v), wo, vo, k)
+
+
+@pytest.mark.parametrize("n,k,seed", [(1, 1, 0), (2, 2, 1), (3, 3, 2), (40, 40, 3), (257, 17, 4)])
+def test_eig_random_hermitian_with_degeneracies(ctx, n, k, seed):
+    print(f"case eig random n={n}", flush=True)
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    lam = np.sort(rng.random(n))[::-1] * 10
+    if n >= 6:
+        lam[1] = lam[2] = lam[1]      # exact pair
+        lam[3] = lam[4] = lam[5] = lam[3]  # exact triple
+    a = (Q * lam) @ Q.conj().T
+    a = 0.5 * (a + a.conj().T)
+    wo, vo = O.eigh_topk(a, k)
+    w, v = ctx.eig_topk(_pack(a), k)
+    _check(a, w, _c(v), wo, vo, k)
+
+
+def test_calc_kernels_layout_and_image(ctx):
+    print("case calc_kernels", flush=True)
+    a, Nx, Ny = _tcc(512, inner=0.6, outer=0.9)
+    kr, ki, sc = ctx.calc_kernels(_pack(a), 24)
+    wo, vo = O.eigh_topk(a, 24)
+    kro, kio, sco = O.kernels_from_eig(vo, wo)
+    assert np.abs(sc - sco).max() <= 2e-7 * sco[0]
+    # kernels equal up to phase inside each cluster: compare images
+    m = W.manhattan(512, 3)
+    assert relerr(O.socs_image(m, kr, ki, sc, Nx, Ny), O.socs_image(m, kro, kio, sco, Nx, Ny)) < 1e-6
+    T = a.shape[0]
+    z = kr[0].astype(np.float64) + 1j * ki[0]
+    v0 = z[::-1]  # undo the reversal: eigenvector of the largest eigenvalue
+    assert np.linalg.norm(a @ v0 - wo[0] * v0) < 1e-5 * wo[0]
+    assert abs(np.linalg.norm(v0) - 1) < 1e-6
+    # nk > T is clamped like the reference
+    kr2, _, sc2 = ctx.calc_kernels(_pack(a[:9, :9]), 20)
+    assert kr2.shape == (9, 9) and sc2.shape == (9,)
+    assert T == 169
