@@ -118,6 +118,13 @@ int klt_eig_topk(klt_ctx* ctx, const double* a, int n, int k, double* w, double*
 int klt_calc_kernels(klt_ctx* ctx, const double* tcc, int T, int nk, float* krn_r,
                      float* krn_i, float* scales, unsigned flags);
 
+/* ---- Hopkins image (klitho_tcc.cpp:534-544: calcImage + FT_c2r) ----------
+ * The image klitho_tcc writes to image.bin, from the full TCC:
+ * H[g] = sum_n1 M^[n1+g] conj(M^[n1]) TCC[ID(n1)][ID(n1+g)] for g_x >= 0, then
+ * the c2r synthesis.  tcc: T x T complex double; mask, image: Ly x Lx. */
+int klt_hopkins_image(klt_ctx* ctx, const float* mask, int Lx, int Ly, const double* tcc,
+                      int Nx, int Ny, float* image, unsigned flags);
+
 /* ---- SOCS kernel bank --------------------------------------------------
  * Replaces the kernel read loop of klitho_socs.cpp:297-318: nk kernels of
  * sizeX x sizeY (odd) in the krn_i_{r,i}.bin layout, and their eigenvalues

@@ -320,6 +320,59 @@ __global__ __launch_bounds__(kThreads) void k_synth_rows(const float2* __restric
   }
 }
 
+// Hopkins spectrum (calcImage, K-Litho-TCC/klitho_tcc.cpp:270-290):
+//   H[g] = sum_{n1} M^[n1+g] conj(M^[n1]) TCC[ID(n1)][ID(n1+g)], g_x >= 0,
+// then FT_c2r (:151-183) keeps the real part of the g_x = 0 column.  That
+// equals synthesising the Hermitian completion J (J[g] = H[g] for g_x > 0,
+// J[-g] = conj(H[g]), J[gy][0] = (H[gy][0] + conj(H[-gy][0])) / 2), written
+// here in the half-plane layout of the synthesis passes.  FP64 sums.
+__device__ double2 hopkins_term(const float2* __restrict__ band,
+                                const double* __restrict__ tcc, int Nx, int Ny, int gx,
+                                int gy) {
+  const int Wx = 2 * Nx + 1, T = Wx * (2 * Ny + 1);
+  double sr = 0.0, si = 0.0;
+  const int y0 = max(-Ny, -Ny - gy), y1 = min(Ny, Ny - gy);
+  const int x0 = max(-Nx, -Nx - gx), x1 = min(Nx, Nx - gx);
+  for (int ny = y0; ny <= y1; ++ny)
+    for (int nx = x0; nx <= x1; ++nx) {
+      const int i1 = (ny + Ny) * Wx + nx + Nx, i2 = (ny + gy + Ny) * Wx + nx + gx + Nx;
+      const float2 a = band[i2], b = band[i1];
+      // a conj(b)
+      const double pr = (double)a.x * b.x + (double)a.y * b.y;
+      const double pi = (double)a.y * b.x - (double)a.x * b.y;
+      const double tr = tcc[2 * ((size_t)i1 * T + i2)], ti = tcc[2 * ((size_t)i1 * T + i2) + 1];
+      sr += pr * tr - pi * ti;
+      si += pr * ti + pi * tr;
+    }
+  return make_double2(sr, si);
+}
+
+__global__ __launch_bounds__(kThreads) void k_hopkins_spec(const float2* __restrict__ band,
+                                                           const double* __restrict__ tcc,
+                                                           int Nx, int Ny, int G,
+                                                           float2* __restrict__ spec) {
+  const int Gx = 4 * Nx + 1;
+  const int o = blockIdx.x * kThreads + threadIdx.x;
+  if (o >= G * Gx) return;
+  const int gy = o / Gx, gx = o - gy * Gx - 2 * Nx;
+  float2 out = make_float2(0.f, 0.f);
+  if (gy <= 2 * Ny) {
+    double2 j;
+    if (gx > 0) {
+      j = hopkins_term(band, tcc, Nx, Ny, gx, gy);
+    } else if (gx < 0) {
+      const double2 h = hopkins_term(band, tcc, Nx, Ny, -gx, -gy);
+      j = make_double2(h.x, -h.y);
+    } else {
+      const double2 a = hopkins_term(band, tcc, Nx, Ny, 0, gy);
+      const double2 b = hopkins_term(band, tcc, Nx, Ny, 0, -gy);
+      j = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+    }
+    out = make_float2((float)j.x, (float)j.y);
+  }
+  spec[o] = out;
+}
+
 __global__ void k_pack_kernels(const float* __restrict__ kr, const float* __restrict__ ki,
                                int n, float2* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -414,9 +467,87 @@ int socs_forward_device(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
   return KLT_OK;
 }
 
+// Hopkins image of one mask from a device TCC (T x T complex double).
+int hopkins_image_device(klt_ctx* ctx, const float* mask, int Lx, int Ly, const double* tcc,
+                         int Nx, int Ny, float* image) {
+  SocsPlan* p = nullptr;
+  KLT_TRY(get_socs_plan(ctx, Lx, Ly, Nx, Ny, &p));
+  const int NP = p->NP, G = p->G;
+  const int T = (2 * Nx + 1) * (2 * Ny + 1), Gx = 4 * Nx + 1;
+  if (NP > 32 || G > 64)
+    return set_error(KLT_ERR_ARG, "Hopkins image: half-size N above 31 is not supported");
+  const int xblocks = (Lx + 2 * kThreads - 1) / (2 * kThreads);
+  int chunks = std::min(std::max(1, 2048 / xblocks), std::max(1, Ly / 8));
+  int rows = (Ly + chunks - 1) / chunks;
+  chunks = (Ly + rows - 1) / rows;
+  KLT_TRY(ctx->qpart.ensure((size_t)chunks * NP * Lx * sizeof(float2)));
+  KLT_TRY(ctx->band.ensure((size_t)T * sizeof(float2)));
+  KLT_TRY(ctx->spec.ensure((size_t)G * Gx * sizeof(float2)));
+  KLT_TRY(ctx->ccols.ensure((size_t)G * Lx * sizeof(float2)));
+  float2* qpart = ctx->qpart.as<float2>();
+  float2* band = ctx->band.as<float2>();
+  switch (NP) {
+    case 8: launch_band<8>(ctx, mask, 1, Lx, Ly, chunks, rows, p, qpart, band); break;
+    case 16: launch_band<16>(ctx, mask, 1, Lx, Ly, chunks, rows, p, qpart, band); break;
+    case 24: launch_band<24>(ctx, mask, 1, Lx, Ly, chunks, rows, p, qpart, band); break;
+    default: launch_band<32>(ctx, mask, 1, Lx, Ly, chunks, rows, p, qpart, band); break;
+  }
+  KLT_LAUNCH(ctx, "k_hopkins_spec", k_hopkins_spec, dim3((G * Gx + kThreads - 1) / kThreads),
+             dim3(kThreads), 0, band, tcc, Nx, Ny, G, ctx->spec.as<float2>());
+  KLT_LAUNCH(ctx, "k_synth_cols", k_synth_cols, dim3((Lx + kThreads - 1) / kThreads, 1),
+             dim3(kThreads), sizeof(float2) * G * Gx, ctx->spec.as<float2>(), G, Nx, Ny, Lx,
+             p->twx_out.as<float2>(), ctx->ccols.as<float2>());
+  float2* cc = ctx->ccols.as<float2>();
+  switch (G) {
+    case 8: launch_rows<8>(ctx, cc, 1, Lx, Ly, chunks, rows, p, image); break;
+    case 16: launch_rows<16>(ctx, cc, 1, Lx, Ly, chunks, rows, p, image); break;
+    case 24: launch_rows<24>(ctx, cc, 1, Lx, Ly, chunks, rows, p, image); break;
+    case 32: launch_rows<32>(ctx, cc, 1, Lx, Ly, chunks, rows, p, image); break;
+    case 40: launch_rows<40>(ctx, cc, 1, Lx, Ly, chunks, rows, p, image); break;
+    case 48: launch_rows<48>(ctx, cc, 1, Lx, Ly, chunks, rows, p, image); break;
+    case 56: launch_rows<56>(ctx, cc, 1, Lx, Ly, chunks, rows, p, image); break;
+    default: launch_rows<64>(ctx, cc, 1, Lx, Ly, chunks, rows, p, image); break;
+  }
+  KLT_HIP(hipGetLastError());
+  return KLT_OK;
+}
+
 }  // namespace klt
 
 extern "C" {
+
+int klt_hopkins_image(klt_ctx* ctx, const float* mask, int Lx, int Ly, const double* tcc,
+                      int Nx, int Ny, float* image, unsigned flags) {
+  if (!ctx || !mask || !tcc || !image)
+    return klt::set_error(KLT_ERR_ARG, "klt_hopkins_image: NULL argument");
+  if (Lx <= 0 || Ly <= 0 || Nx < 0 || Ny < 0)
+    return klt::set_error(KLT_ERR_ARG, "Error: Invalid input parameters. Please check the values.");
+  if (4 * Nx + 1 > Lx || 4 * Ny + 1 > Ly)
+    return klt::set_error(KLT_ERR_SIZE, "Hopkins image: 4N+1 exceeds the period");
+  KLT_TRY(ctx->activate());
+  const int T = (2 * Nx + 1) * (2 * Ny + 1);
+  const size_t mbytes = (size_t)Lx * Ly * sizeof(float), tbytes = (size_t)T * T * 16;
+  const float* dm = mask;
+  const double* dt = tcc;
+  float* di = image;
+  if (flags & KLT_HOST_PTRS) {
+    KLT_TRY(ctx->stage_in.ensure(mbytes + tbytes + 256));
+    char* b = ctx->stage_in.as<char>();
+    KLT_HIP(hipMemcpyAsync(b, tcc, tbytes, hipMemcpyHostToDevice, ctx->stream));
+    KLT_HIP(hipMemcpyAsync(b + tbytes, mask, mbytes, hipMemcpyHostToDevice, ctx->stream));
+    dt = reinterpret_cast<const double*>(b);
+    dm = reinterpret_cast<const float*>(b + tbytes);
+    KLT_TRY(ctx->stage_out.ensure(mbytes));
+    di = ctx->stage_out.as<float>();
+  }
+  KLT_TRY(klt::hopkins_image_device(ctx, dm, Lx, Ly, dt, Nx, Ny, di));
+  if (flags & KLT_HOST_PTRS) {
+    KLT_HIP(hipMemcpyAsync(image, di, mbytes, hipMemcpyDeviceToHost, ctx->stream));
+    KLT_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return KLT_OK;
+}
+
 
 int klt_kernels_create(klt_ctx* ctx, const float* krn_r, const float* krn_i,
                        const float* scales, int nk, int sizeX, int sizeY, unsigned flags,

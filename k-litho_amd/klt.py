@@ -74,6 +74,7 @@ def lib() -> C.CDLL:
         L.klt_tcc_build.argtypes = [vp, fp, i, f, f, f, f, f, i, i, i, i, i, fp, u]
         L.klt_eig_topk.argtypes = [vp, fp, i, i, fp, fp, u]
         L.klt_calc_kernels.argtypes = [vp, fp, i, i, fp, fp, fp, u]
+        L.klt_hopkins_image.argtypes = [vp, fp, i, i, fp, i, i, fp, u]
         L.klt_kernels_create.argtypes = [vp, fp, fp, fp, i, i, i, u, C.POINTER(vp)]
         L.klt_kernels_destroy.argtypes = [vp]
         L.klt_kernels_destroy.restype = None
@@ -286,6 +287,20 @@ class Context:
         p3, h3 = _ptr(sc, np.float32)
         check(lib().klt_calc_kernels(self._h, pt, T, nk, p1, p2, p3, _flags(ht, h1, h2, h3)))
         return kr, ki, sc
+
+    def hopkins_image(self, mask, tcc, Nx: int, Ny: int):
+        """Hopkins image (calcImage + FT_c2r) from a TCC [T, T, 2]."""
+        Ly, Lx = mask.shape
+        if _is_torch(mask):
+            import torch
+            img = torch.empty_like(mask)
+        else:
+            img = np.empty((Ly, Lx), np.float32)
+        pm, hm = _ptr(mask, np.float32)
+        pt, ht = _ptr(tcc, np.float64)
+        pi, hi = _ptr(img, np.float32)
+        check(lib().klt_hopkins_image(self._h, pm, Lx, Ly, pt, Nx, Ny, pi, _flags(hm, ht, hi)))
+        return img
 
     def kernels(self, kr, ki, scales, size_x: int, size_y: int) -> "KernelBank":
         return KernelBank(self, kr, ki, scales, size_x, size_y)

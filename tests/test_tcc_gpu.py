@@ -73,3 +73,15 @@ def test_tcc_rejects_na_above_medium(ctx):
     with pytest.raises(klt.KltError) as e:
         ctx.tcc_build(src, sig, 1.35, 0.3, 512, 512, 6, 6, precision=0, nmed=1.0)
     assert "NA" in e.value.msg
+
+
+@pytest.mark.parametrize("L,NA", [(256, 0.9), (255, 0.9), (512, 1.35)])
+def test_hopkins_image_vs_oracle(ctx, L, NA):
+    """klt_hopkins_image (calcImage + FT_c2r) against the oracle's Hopkins image."""
+    print(f"case hopkins L={L}", flush=True)
+    src, sig = _annular()
+    Nx, Ny = O.sizing(NA, L, L, sig)
+    t = O.tcc(src, sig, NA, 0.0, L, L, Nx, Ny, nmed=1.44)
+    m = W.manhattan(L, 42)
+    got = ctx.hopkins_image(m, np.ascontiguousarray(np.stack([t.real, t.imag], -1)), Nx, Ny)
+    assert relerr(got, O.hopkins_image(m, t, Nx, Ny)) < 1e-5
