@@ -148,6 +148,16 @@ int klt_socs_image(klt_ctx* ctx, const klt_kernels* kb, const float* mask, int L
 int klt_socs_forward(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
                      int nmask, int Lx, int Ly, float* images, unsigned flags);
 
+/* ---- adjoint (no reference counterpart) ---------------------------------
+ * Resist loss of each SOCS image, L = sum_x (sigmoid(a (I(x) - thr)) - T(x))^2,
+ * and its gradient with respect to the mask, dL/dm (Ly x Lx per mask), through
+ * the same band-limited passes as klt_socs_forward (kernel half-size <= 15).
+ * masks (continuous transmittance allowed), targets, grads: nmask x Ly x Lx;
+ * losses: nmask; images: optional (NULL) output of the forward images. */
+int klt_socs_backward(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
+                      const float* targets, int nmask, int Lx, int Ly, float a, float thr,
+                      float* grads, float* losses, float* images, unsigned flags);
+
 #ifdef __cplusplus
 }
 #endif

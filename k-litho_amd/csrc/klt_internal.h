@@ -63,6 +63,9 @@ struct SocsPlan {
 struct FusedTables {
   int Kpad = 0, Lypad = 0;
   DevBuf Wh, Wl, TYh, TYl, tw64;
+  // adjoint: resist-gradient band rows (64 x Kpad, centre (Lx+1)/2) and the
+  // column twiddles e^{-2 pi i gy (y - (Ly+1)/2)/Ly} (Ly x 32)
+  DevBuf Wgh, Wgl, twyg;
 };
 
 }  // namespace klt
@@ -82,6 +85,7 @@ struct klt_ctx {
   hipEvent_t take_event();
   // scratch reused across calls (grown on demand)
   klt::DevBuf stage_in, stage_out, qpart, band, spec, ccols, misc, eig_ws;
+  klt::DevBuf gpart, gspec, lpart, bwd_img;  // adjoint scratch
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::SocsPlan>> socs_plans;
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::FusedTables>> fused_tables;
   int activate() const;  // hipSetDevice(device)

@@ -73,80 +73,102 @@ __device__ __forceinline__ void load8(const float* row, int x, int Lx, bool vali
 }
 
 // ---------------------------------------------------------------------------
-// Stage 1.  grid (ceil(Ly/128), B), 256 threads.  Wave w owns rows
-// blk*128 + 32w .. +32 (two 16-row MFMA blocks) and all 32 output columns
-// (n = 2 nx: cos part, n = 2 nx + 1: -sin part, nx <= 15).
-// mpart: [B][nblk][2Ny+1][16] float2, entry (ny, nx) for nx in [0, Nx].
+// Stage 1 (and the adjoint's resist-gradient band).  grid (ceil(Ly/128), B),
+// 256 threads.  Wave w owns rows blk*128 + 32w .. +32 (two 16-row MFMA
+// blocks) and all 16*NCB output columns (n = 2 nx: cos part, n = 2 nx + 1:
+// -sin part).  The GEMM input is
+//   RESIST = false: the mask m[y][x];
+//   RESIST = true : G(x) = dL/dI = 2 (s - T) a s (1 - s), s = sigmoid(a (I - thr)),
+//                   from the image I and target T (the loss sum (s - T)^2 of
+//                   the block goes to lpart[b][blk]).
+// Epilogue: out[b][blk][ny + By][nx] = sum_{y in block} R[y][nx] tw(ny, y) for
+// |ny| <= By, 0 <= nx <= Bx; tw(ny, y) = twy[y][|ny|] (conjugated for ny < 0).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_band_mfma(const float* __restrict__ mask, int Lx,
-                                                   int Ly, int Kpad,
-                                                   const _Float16* __restrict__ Wh,
-                                                   const _Float16* __restrict__ Wl,
-                                                   const float2* __restrict__ twy, int NP,
-                                                   int Nx, int Ny,
-                                                   float2* __restrict__ mpart) {
-  __shared__ float sR[kRowsBand][33];
+template <int NCB, bool RESIST>
+__global__ __launch_bounds__(256) void k_band_mfma(
+    const float* __restrict__ in0, const float* __restrict__ in1, float a_sig, float thr,
+    int Lx, int Ly, int Kpad, const _Float16* __restrict__ Wh, const _Float16* __restrict__ Wl,
+    const float2* __restrict__ twy, int ldtw, int Bx, int By, int ldo,
+    float2* __restrict__ out_part, float* __restrict__ lpart) {
+  __shared__ float sR[kRowsBand][16 * NCB + 1];
+  __shared__ float red[4];
   const int blk = blockIdx.x, b = blockIdx.y, nblk = gridDim.x;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r16 = lane & 15, kq = (lane >> 4) * 8;
   const int ybase = blk * kRowsBand + wave * 32;
-  const float* img = mask + (size_t)b * Lx * Ly;
+  const size_t ib = (size_t)b * Lx * Ly;
   const int ya = ybase + r16, yb = ybase + 16 + r16;
   const bool va = ya < Ly, vb = yb < Ly;
-  const float* ra = img + (size_t)(va ? ya : 0) * Lx;
-  const float* rb = img + (size_t)(vb ? yb : 0) * Lx;
+  const size_t ra = ib + (size_t)(va ? ya : 0) * Lx, rb = ib + (size_t)(vb ? yb : 0) * Lx;
   const bool vec = (Lx & 3) == 0;
-  float4_t acc[2][2];
+  float4_t acc[2][NCB];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int c = 0; c < 2; ++c) acc[a][c] = float4_t{0.f, 0.f, 0.f, 0.f};
-  const _Float16* w0h = Wh + (size_t)r16 * Kpad + kq;
-  const _Float16* w0l = Wl + (size_t)r16 * Kpad + kq;
-  const _Float16* w1h = Wh + (size_t)(16 + r16) * Kpad + kq;
-  const _Float16* w1l = Wl + (size_t)(16 + r16) * Kpad + kq;
+    for (int c = 0; c < NCB; ++c) acc[a][c] = float4_t{0.f, 0.f, 0.f, 0.f};
+  float lsum = 0.f;
   for (int k = 0; k < Kpad; k += 32) {
     float va8[8], vb8[8];
-    load8(ra, k + kq, Lx, va, vec, va8);
-    load8(rb, k + kq, Lx, vb, vec, vb8);
+    load8(in0 + ra, k + kq, Lx, va, vec, va8);
+    load8(in0 + rb, k + kq, Lx, vb, vec, vb8);
+    if (RESIST) {
+      float ta8[8], tb8[8];
+      load8(in1 + ra, k + kq, Lx, va, vec, ta8);
+      load8(in1 + rb, k + kq, Lx, vb, vec, tb8);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const bool ina = va && k + kq + t < Lx, inb = vb && k + kq + t < Lx;
+        const float sa = 1.f / (1.f + __expf(-a_sig * (va8[t] - thr)));
+        const float sb = 1.f / (1.f + __expf(-a_sig * (vb8[t] - thr)));
+        const float da = sa - ta8[t], db = sb - tb8[t];
+        lsum += (ina ? da * da : 0.f) + (inb ? db * db : 0.f);
+        va8[t] = ina ? 2.f * da * a_sig * sa * (1.f - sa) : 0.f;
+        vb8[t] = inb ? 2.f * db * a_sig * sb * (1.f - sb) : 0.f;
+      }
+    }
     half8_t ah, al, bh, bl;
     split8(va8, ah, al);
     split8(vb8, bh, bl);
-    const half8_t c0h = *reinterpret_cast<const half8_t*>(w0h + k);
-    const half8_t c0l = *reinterpret_cast<const half8_t*>(w0l + k);
-    const half8_t c1h = *reinterpret_cast<const half8_t*>(w1h + k);
-    const half8_t c1l = *reinterpret_cast<const half8_t*>(w1l + k);
-    acc[0][0] = mfma3(ah, al, c0h, c0l, acc[0][0]);
-    acc[0][1] = mfma3(ah, al, c1h, c1l, acc[0][1]);
-    acc[1][0] = mfma3(bh, bl, c0h, c0l, acc[1][0]);
-    acc[1][1] = mfma3(bh, bl, c1h, c1l, acc[1][1]);
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) {
+      const size_t wo = (size_t)(c * 16 + r16) * Kpad + k + kq;
+      const half8_t ch = *reinterpret_cast<const half8_t*>(Wh + wo);
+      const half8_t cl = *reinterpret_cast<const half8_t*>(Wl + wo);
+      acc[0][c] = mfma3(ah, al, ch, cl, acc[0][c]);
+      acc[1][c] = mfma3(bh, bl, ch, cl, acc[1][c]);
+    }
   }
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+    for (int c = 0; c < NCB; ++c)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
         sR[wave * 32 + a * 16 + (lane >> 4) * 4 + r][c * 16 + r16] = acc[a][c][r];
+  if (RESIST) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
+    if (lane == 0) red[wave] = lsum;
+  }
   __syncthreads();
-  // partial band: sum over this block's rows of R[y][nx] e^{-2 pi i ny (y-cy)/Ly}
-  const int Wy = 2 * Ny + 1, nxp = Nx + 1;
+  if (RESIST && threadIdx.x == 0) lpart[(size_t)b * nblk + blk] = red[0] + red[1] + red[2] + red[3];
+  const int Wy = 2 * By + 1, nxp = Bx + 1;
   const int y0 = blk * kRowsBand;
   const int nrow = min(kRowsBand, Ly - y0);
-  float2* out = mpart + ((size_t)b * nblk + blk) * Wy * 16;
+  float2* out = out_part + ((size_t)b * nblk + blk) * Wy * ldo;
   for (int o = threadIdx.x; o < Wy * nxp; o += 256) {
     const int nyi = o / nxp, nx = o - nyi * nxp;
-    const int ny = nyi - Ny;
+    const int ny = nyi - By;
     const int any = ny < 0 ? -ny : ny;
     float sr = 0.f, si = 0.f;
     for (int yy = 0; yy < nrow; ++yy) {
-      float2 w = twy[(size_t)(y0 + yy) * NP + any];
+      float2 w = twy[(size_t)(y0 + yy) * ldtw + any];
       if (ny < 0) w.y = -w.y;
       const float rr = sR[yy][2 * nx], ri = sR[yy][2 * nx + 1];
       sr += rr * w.x - ri * w.y;
       si += rr * w.y + ri * w.x;
     }
-    out[nyi * 16 + nx] = make_float2(sr, si);
+    out[nyi * ldo + nx] = make_float2(sr, si);
   }
 }
 
@@ -331,6 +353,201 @@ __global__ __launch_bounds__(256) void k_socs_mid(const float2* __restrict__ mpa
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint mid stage.  grid (B), 256 threads.  With G = dL/dI and its band
+// G^[g] = 1/(Lx Ly) sum_x G(x) e^{-2 pi i g.(x - c_out)/L}, |g| <= 2N:
+//   g_c(u)   = sum_g G^[g] e^{+2 pi i g.u/64}              (coarse grid, registers)
+//   e_k(u)   = sum_n A_k[n] e^{+2 pi i n.u/64}, A_k = K_k M^
+//   F_k[-q]  = 1/64^2 sum_u conj(e_k(u)) g_c(u) e^{-2 pi i q.u/64}
+//            = sum_n' conj(A_k[n']) G^[n' + q]              (|n'+q| <= 3N < 64 - N)
+//   Gam[n]   = sum_k lambda_k K_k[n] F_k[n]
+// and dL/dm(x) = 2 Re sum_n Gam[n] e^{-2 pi i n.(x - c_in)/L}, written as the
+// Hermitian spectrum J[n] = Gam[-n] + conj(Gam[n]) (times e^{2 pi i n.(c_out -
+// c_in)/L}) in the synthesis layout [32][4Nx+1] (rows gy in [0, Ny]).
+// loss[b] = sum of the per-block loss partials.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_socs_mid_bwd(
+    const float2* __restrict__ mpart, const float2* __restrict__ gpart,
+    const float* __restrict__ lpart, int nblk, int Nx, int Ny, float inv_area,
+    const float2* __restrict__ kern, const float* __restrict__ scales, int nk,
+    const float2* __restrict__ tw64, int dcx, int dcy, int Lx, int Ly,
+    float2* __restrict__ gspec, float* __restrict__ loss) {
+  __shared__ float2 sM[31 * 31];
+  __shared__ float2 sB[61 * kM];
+  __shared__ float2 sX[32 * 64];
+  __shared__ float2 sGam[31 * 31];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int g = tid >> 3, t = tid & 7;
+  const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
+  const int Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
+  for (int i = tid; i < T; i += 256) {
+    const int nyi = i / Wx, nx = i - nyi * Wx - Nx;
+    const int ny = nyi - Ny;
+    const int r = nx >= 0 ? nyi : (-ny + Ny), c = nx >= 0 ? nx : -nx;
+    float2 s = make_float2(0.f, 0.f);
+    for (int k = 0; k < nblk; ++k) {
+      const float2 p = mpart[(((size_t)b * nblk + k) * Wy + r) * 16 + c];
+      s.x += p.x;
+      s.y += p.y;
+    }
+    if (nx < 0) s.y = -s.y;
+    sM[i] = make_float2(s.x * inv_area, s.y * inv_area);
+    sGam[i] = make_float2(0.f, 0.f);
+  }
+  if (tid == 0) {
+    float l = 0.f;
+    for (int k = 0; k < nblk; ++k) l += lpart[(size_t)b * nblk + k];
+    loss[b] = l;
+  }
+  float2 twi[8], twf[8];
+#pragma unroll
+  for (int k2 = 0; k2 < 8; ++k2) {
+    twi[k2] = tw64[(t * k2) & 63];
+    twf[k2] = make_float2(twi[k2].x, -twi[k2].y);
+  }
+  float2* xs = sX + g * 64;
+  __syncthreads();
+  // g_c rows: gy in [-2Ny, 2Ny] (row index gy + 2Ny), two rounds of 32 groups
+#pragma unroll
+  for (int rr = 0; rr < 2; ++rr) {
+    const int r = g + 32 * rr;
+    const bool act = r < Gy;
+    const int gy = r - 2 * Ny;
+    float2 v[8];
+    if (act) {
+#pragma unroll
+      for (int n2 = 0; n2 < 8; ++n2) {
+        const int n = t + 8 * n2;
+        const int gx = n < 32 ? n : n - 64;
+        float2 val = make_float2(0.f, 0.f);
+        if (gx >= -2 * Nx && gx <= 2 * Nx) {
+          const int row = gx >= 0 ? r : (-gy + 2 * Ny), col = gx >= 0 ? gx : -gx;
+          for (int k = 0; k < nblk; ++k) {
+            const float2 p = gpart[(((size_t)b * nblk + k) * Gy + row) * 32 + col];
+            val.x += p.x;
+            val.y += p.y;
+          }
+          val = make_float2(val.x * inv_area, gx >= 0 ? val.y * inv_area : -val.y * inv_area);
+        }
+        v[n2] = val;
+      }
+    }
+    fft64_group<1>(v, twi, xs, t, act);
+    if (act) {
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) sB[r * kM + t + 8 * k1] = v[k1];
+    }
+  }
+  __syncthreads();
+  float2 gc[2][8];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int ux = g + 32 * q;
+    float2 v[8];
+#pragma unroll
+    for (int n2 = 0; n2 < 8; ++n2) {
+      const int n = t + 8 * n2;
+      const int gy = n < 32 ? n : n - 64;
+      v[n2] = (gy >= -2 * Ny && gy <= 2 * Ny) ? sB[(gy + 2 * Ny) * kM + ux] : make_float2(0.f, 0.f);
+    }
+    fft64_group<1>(v, twi, xs, t, true);
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) gc[q][k1] = v[k1];
+  }
+  const float inv = 1.0f / (float)(kM * kM);
+  for (int k = 0; k < nk; ++k) {
+    const float2* K = kern + (size_t)k * T;
+    const float lam = scales[k];
+    {
+      const bool act = g < Wy;
+      float2 v[8];
+      if (act) {
+#pragma unroll
+        for (int n2 = 0; n2 < 8; ++n2) {
+          const int n = t + 8 * n2;
+          const int nx = n < 32 ? n : n - 64;
+          float2 val = make_float2(0.f, 0.f);
+          if (nx >= -Nx && nx <= Nx) val = cmulf(K[g * Wx + nx + Nx], sM[g * Wx + nx + Nx]);
+          v[n2] = val;
+        }
+      }
+      fft64_group<1>(v, twi, xs, t, act);
+      if (act) {
+#pragma unroll
+        for (int k1 = 0; k1 < 8; ++k1) sB[g * kM + t + 8 * k1] = v[k1];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int ux = g + 32 * q;
+      float2 v[8];
+#pragma unroll
+      for (int n2 = 0; n2 < 8; ++n2) {
+        const int n = t + 8 * n2;
+        const int ny = n < 32 ? n : n - 64;
+        v[n2] = (ny >= -Ny && ny <= Ny) ? sB[(ny + Ny) * kM + ux] : make_float2(0.f, 0.f);
+      }
+      fft64_group<1>(v, twi, xs, t, true);
+      // conj(e) g_c, then forward along uy
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) {
+        const float2 e = v[k1], c = gc[q][k1];
+        v[k1] = make_float2(e.x * c.x + e.y * c.y, e.x * c.y - e.y * c.x);
+      }
+      fft64_group<-1>(v, twf, xs, t, true);
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) {
+        const int qq = t + 8 * k1;
+        const int qy = qq < 32 ? qq : qq - 64;
+        if (qy >= -Ny && qy <= Ny) sB[(qy + Ny) * kM + ux] = v[k1];
+      }
+    }
+    __syncthreads();
+    {
+      const bool act = g < Wy;
+      const int qy = g - Ny;
+      float2 v[8];
+      if (act) {
+#pragma unroll
+        for (int n2 = 0; n2 < 8; ++n2) v[n2] = sB[g * kM + t + 8 * n2];
+      }
+      fft64_group<-1>(v, twf, xs, t, act);
+      if (act) {
+#pragma unroll
+        for (int k1 = 0; k1 < 8; ++k1) {
+          const int qq = t + 8 * k1;
+          const int qx = qq < 32 ? qq : qq - 64;
+          if (qx >= -Nx && qx <= Nx) {
+            const int idx = (-qy + Ny) * Wx + (-qx + Nx);
+            const float2 f = make_float2(v[k1].x * inv, v[k1].y * inv);
+            const float2 kf = cmulf(K[idx], f);
+            sGam[idx].x += lam * kf.x;
+            sGam[idx].y += lam * kf.y;
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  float2* out = gspec + (size_t)b * 32 * Gx;
+  for (int i = tid; i < 32 * Gx; i += 256) {
+    const int gy = i / Gx, gx = i - gy * Gx - 2 * Nx;
+    float2 j = make_float2(0.f, 0.f);
+    if (gy <= Ny && gx >= -Nx && gx <= Nx) {
+      const float2 gm = sGam[(-gy + Ny) * Wx + (-gx + Nx)];  // Gam[-n]
+      const float2 gp = sGam[(gy + Ny) * Wx + (gx + Nx)];    // Gam[n]
+      j = make_float2(gm.x + gp.x, gm.y - gp.y);
+      if (dcx | dcy) {
+        float sn, cs;
+        sincospif(2.0f * ((float)(gx * dcx) / (float)Lx + (float)(gy * dcy) / (float)Ly), &sn, &cs);
+        j = cmulf(j, make_float2(cs, sn));
+      }
+    }
+    out[i] = j;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Stage 3.  grid (ceil(Lx/128), B), 256 threads.
 // P[x][j] (j = 2 gy: Re, 2 gy + 1: Im) for the block's 128 columns, scaled by
 // a power of two and split to hi/lo halves in LDS; then wave w computes rows
@@ -490,6 +707,38 @@ static int get_fused_tables(klt_ctx* ctx, SocsPlan* p, FusedTables** out) {
   KLT_TRY(t->TYl.ensure(l.size() * sizeof(_Float16)));
   KLT_HIP(hipMemcpy(t->TYh.ptr, h.data(), h.size() * sizeof(_Float16), hipMemcpyHostToDevice));
   KLT_HIP(hipMemcpy(t->TYl.ptr, l.data(), l.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+  // resist-gradient band: rows n = 2 gx (cos), 2 gx + 1 (-sin), gx <= 2Nx,
+  // centre (Lx+1)/2; column twiddles e^{-2 pi i gy (y - (Ly+1)/2)/Ly}, gy <= 2Ny
+  {
+    const int cox = (Lx + 1) / 2;
+    std::vector<_Float16> gh((size_t)64 * t->Kpad), gl((size_t)64 * t->Kpad);
+    for (int n = 0; n < 64; ++n)
+      for (int x = 0; x < t->Kpad; ++x) {
+        double v = 0.0;
+        const int gx = n / 2;
+        if (gx <= 2 * Nx && x < Lx) {
+          long long r = ((long long)gx * (x - cox)) % Lx;
+          if (r < 0) r += Lx;
+          const double a = 2.0 * M_PI * (double)r / (double)Lx;
+          v = (n & 1) ? -std::sin(a) : std::cos(a);
+        }
+        split_half(v, &gh[(size_t)n * t->Kpad + x], &gl[(size_t)n * t->Kpad + x]);
+      }
+    KLT_TRY(t->Wgh.ensure(gh.size() * sizeof(_Float16)));
+    KLT_TRY(t->Wgl.ensure(gl.size() * sizeof(_Float16)));
+    KLT_HIP(hipMemcpy(t->Wgh.ptr, gh.data(), gh.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+    KLT_HIP(hipMemcpy(t->Wgl.ptr, gl.data(), gl.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+    std::vector<float2> ty((size_t)Ly * 32, make_float2(0.f, 0.f));
+    for (int y = 0; y < Ly; ++y)
+      for (int gy = 0; gy <= 2 * Ny; ++gy) {
+        long long r = ((long long)gy * (y - coy)) % Ly;
+        if (r < 0) r += Ly;
+        const double a = 2.0 * M_PI * (double)r / (double)Ly;
+        ty[(size_t)y * 32 + gy] = make_float2((float)std::cos(a), (float)-std::sin(a));
+      }
+    KLT_TRY(t->twyg.ensure(ty.size() * sizeof(float2)));
+    KLT_HIP(hipMemcpy(t->twyg.ptr, ty.data(), ty.size() * sizeof(float2), hipMemcpyHostToDevice));
+  }
   std::vector<float2> tw(kM);
   for (int j = 0; j < kM; ++j) {
     const double a = 2.0 * M_PI * j / kM;
@@ -520,9 +769,10 @@ int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, 
     const int nb = std::min(SB, B - b0);
     const float* m = masks + (size_t)b0 * Lx * Ly;
     float* out = images + (size_t)b0 * Lx * Ly;
-    KLT_LAUNCH(ctx, "k_band_mfma", k_band_mfma, dim3(nblk, nb), dim3(256), 0, m, Lx, Ly,
-               ft->Kpad, ft->Wh.as<_Float16>(), ft->Wl.as<_Float16>(),
-               p->twy_in.as<float2>(), p->NP, Nx, Ny, ctx->qpart.as<float2>());
+    KLT_LAUNCH(ctx, "k_band_mfma", (k_band_mfma<2, false>), dim3(nblk, nb), dim3(256), 0, m,
+               nullptr, 0.f, 0.f, Lx, Ly, ft->Kpad, ft->Wh.as<_Float16>(),
+               ft->Wl.as<_Float16>(), p->twy_in.as<float2>(), p->NP, Nx, Ny, 16,
+               ctx->qpart.as<float2>(), nullptr);
     KLT_LAUNCH(ctx, "k_socs_mid", k_socs_mid, dim3(nb), dim3(256), 0, ctx->qpart.as<float2>(),
                nblk, Nx, Ny, inv_area, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk,
                ft->tw64.as<float2>(), ctx->spec.as<float2>());
@@ -535,4 +785,99 @@ int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, 
   return KLT_OK;
 }
 
+int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
+                        const float* targets, int B, int Lx, int Ly, float a_sig, float thr,
+                        float* grads, float* losses, float* images) {
+  SocsPlan* p = nullptr;
+  KLT_TRY(get_socs_plan(ctx, Lx, Ly, kb->Nx, kb->Ny, &p));
+  FusedTables* ft = nullptr;
+  KLT_TRY(get_fused_tables(ctx, p, &ft));
+  const int Nx = kb->Nx, Ny = kb->Ny, Wy = 2 * Ny + 1, Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
+  const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
+  const int SB = std::min(B, 512);
+  KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk * Wy * 16 * sizeof(float2)));
+  KLT_TRY(ctx->spec.ensure((size_t)SB * 32 * Gx * sizeof(float2)));
+  KLT_TRY(ctx->gpart.ensure((size_t)SB * nblk * Gy * 32 * sizeof(float2)));
+  KLT_TRY(ctx->gspec.ensure((size_t)SB * 32 * Gx * sizeof(float2)));
+  KLT_TRY(ctx->lpart.ensure((size_t)SB * nblk * sizeof(float)));
+  float* img = images;
+  if (!img) {
+    KLT_TRY(ctx->bwd_img.ensure((size_t)SB * Lx * Ly * sizeof(float)));
+  }
+  const float inv_area = 1.0f / ((float)Lx * (float)Ly);
+  const int dcx = (Lx + 1) / 2 - Lx / 2, dcy = (Ly + 1) / 2 - Ly / 2;
+  for (int b0 = 0; b0 < B; b0 += SB) {
+    const int nb = std::min(SB, B - b0);
+    const size_t off = (size_t)b0 * Lx * Ly;
+    float* I = images ? images + off : ctx->bwd_img.as<float>();
+    KLT_LAUNCH(ctx, "k_band_mfma", (k_band_mfma<2, false>), dim3(nblk, nb), dim3(256), 0,
+               masks + off, nullptr, 0.f, 0.f, Lx, Ly, ft->Kpad, ft->Wh.as<_Float16>(),
+               ft->Wl.as<_Float16>(), p->twy_in.as<float2>(), p->NP, Nx, Ny, 16,
+               ctx->qpart.as<float2>(), nullptr);
+    KLT_LAUNCH(ctx, "k_socs_mid", k_socs_mid, dim3(nb), dim3(256), 0, ctx->qpart.as<float2>(),
+               nblk, Nx, Ny, inv_area, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk,
+               ft->tw64.as<float2>(), ctx->spec.as<float2>());
+    KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma,
+               dim3((Lx + kColsSynth - 1) / kColsSynth, nb), dim3(256), 0,
+               ctx->spec.as<float2>(), Nx, Lx, Ly, ft->Lypad, p->twx_out.as<float2>(),
+               ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), I);
+    KLT_LAUNCH(ctx, "k_band_resist", (k_band_mfma<4, true>), dim3(nblk, nb), dim3(256), 0, I,
+               targets + off, a_sig, thr, Lx, Ly, ft->Kpad, ft->Wgh.as<_Float16>(),
+               ft->Wgl.as<_Float16>(), ft->twyg.as<float2>(), 32, 2 * Nx, 2 * Ny, 32,
+               ctx->gpart.as<float2>(), ctx->lpart.as<float>());
+    KLT_LAUNCH(ctx, "k_socs_mid_bwd", k_socs_mid_bwd, dim3(nb), dim3(256), 0,
+               ctx->qpart.as<float2>(), ctx->gpart.as<float2>(), ctx->lpart.as<float>(), nblk,
+               Nx, Ny, inv_area, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk,
+               ft->tw64.as<float2>(), dcx, dcy, Lx, Ly, ctx->gspec.as<float2>(), losses + b0);
+    KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma,
+               dim3((Lx + kColsSynth - 1) / kColsSynth, nb), dim3(256), 0,
+               ctx->gspec.as<float2>(), Nx, Lx, Ly, ft->Lypad, p->twx_out.as<float2>(),
+               ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), grads + off);
+    KLT_HIP(hipGetLastError());
+  }
+  return KLT_OK;
+}
+
 }  // namespace klt
+
+extern "C" {
+
+int klt_socs_backward(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
+                      const float* targets, int nmask, int Lx, int Ly, float a_sig, float thr,
+                      float* grads, float* losses, float* images, unsigned flags) {
+  if (!ctx || !kb || !masks || !targets || !grads || !losses)
+    return klt::set_error(KLT_ERR_ARG, "klt_socs_backward: NULL argument");
+  if (nmask <= 0 || Lx <= 0 || Ly <= 0)
+    return klt::set_error(KLT_ERR_ARG, "klt_socs_backward: bad size");
+  if (4 * kb->Nx + 1 > Lx || 4 * kb->Ny + 1 > Ly)
+    return klt::set_error(KLT_ERR_SIZE, "ERROR in FI()");
+  if (!klt::socs_fused_supported(kb))
+    return klt::set_error(KLT_ERR_ARG, "klt_socs_backward: kernel half-size above 15");
+  KLT_TRY(ctx->activate());
+  const size_t bytes = (size_t)nmask * Lx * Ly * sizeof(float);
+  const float *dm = masks, *dt = targets;
+  float *dg = grads, *dl = losses, *di = images;
+  if (flags & KLT_HOST_PTRS) {
+    KLT_TRY(ctx->stage_in.ensure(2 * bytes));
+    char* s = ctx->stage_in.as<char>();
+    KLT_HIP(hipMemcpyAsync(s, masks, bytes, hipMemcpyHostToDevice, ctx->stream));
+    KLT_HIP(hipMemcpyAsync(s + bytes, targets, bytes, hipMemcpyHostToDevice, ctx->stream));
+    dm = reinterpret_cast<const float*>(s);
+    dt = reinterpret_cast<const float*>(s + bytes);
+    KLT_TRY(ctx->stage_out.ensure(2 * bytes + (size_t)nmask * 4 + 256));
+    char* o = ctx->stage_out.as<char>();
+    dg = reinterpret_cast<float*>(o);
+    di = images ? reinterpret_cast<float*>(o + bytes) : nullptr;
+    dl = reinterpret_cast<float*>(o + 2 * bytes);
+  }
+  KLT_TRY(klt::socs_backward_fused(ctx, kb, dm, dt, nmask, Lx, Ly, a_sig, thr, dg, dl, di));
+  if (flags & KLT_HOST_PTRS) {
+    KLT_HIP(hipMemcpyAsync(grads, dg, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    KLT_HIP(hipMemcpyAsync(losses, dl, (size_t)nmask * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (images) KLT_HIP(hipMemcpyAsync(images, di, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    KLT_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return KLT_OK;
+}
+
+}  // extern "C"

@@ -75,6 +75,8 @@ def lib() -> C.CDLL:
         L.klt_eig_topk.argtypes = [vp, fp, i, i, fp, fp, u]
         L.klt_calc_kernels.argtypes = [vp, fp, i, i, fp, fp, fp, u]
         L.klt_hopkins_image.argtypes = [vp, fp, i, i, fp, i, i, fp, u]
+        L.klt_socs_backward.argtypes = [vp, vp, fp, fp, i, i, i, C.c_float, C.c_float, fp, fp,
+                                        fp, u]
         L.klt_kernels_create.argtypes = [vp, fp, fp, fp, i, i, i, u, C.POINTER(vp)]
         L.klt_kernels_destroy.argtypes = [vp]
         L.klt_kernels_destroy.restype = None
@@ -329,6 +331,38 @@ class Context:
         else:
             check(lib().klt_socs_forward(self._h, bank.handle, pm, B, Lx, Ly, pi, fl))
         return images
+
+
+def _empty_like(a, shape=None, dtype=np.float32):
+    shape = tuple(a.shape) if shape is None else shape
+    if _is_torch(a):
+        import torch
+        tdt = {np.float32: torch.float32, np.float64: torch.float64}[dtype]
+        return torch.empty(shape, dtype=tdt, device=a.device)
+    return np.empty(shape, dtype)
+
+
+def socs_backward(ctx: "Context", bank: "KernelBank", masks, targets, a: float = 50.0,
+                  thr: float = 0.25, want_images: bool = False):
+    """(losses [B], grads [B, Ly, Lx], images or None) of the resist loss."""
+    shape = tuple(masks.shape)
+    Ly, Lx = shape[-2:]
+    B = int(np.prod(shape[:-2])) if len(shape) > 2 else 1
+    grads = _empty_like(masks)
+    losses = _empty_like(masks, (B,))
+    images = _empty_like(masks) if want_images else None
+    pm, hm = _ptr(masks, np.float32)
+    pt, ht = _ptr(targets, np.float32)
+    pg, hg = _ptr(grads, np.float32)
+    pl, hl = _ptr(losses, np.float32)
+    hs = [hm, ht, hg, hl]
+    pi = None
+    if images is not None:
+        pi, hi = _ptr(images, np.float32)
+        hs.append(hi)
+    check(lib().klt_socs_backward(ctx.handle, bank.handle, pm, pt, B, Lx, Ly, a, thr, pg, pl,
+                                  pi, _flags(*hs)))
+    return losses, grads, images
 
 
 class KernelBank:
