@@ -158,6 +158,18 @@ int klt_socs_backward(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
                       const float* targets, int nmask, int Lx, int Ly, float a, float thr,
                       float* grads, float* losses, float* images, unsigned flags);
 
+/* Gradient of the same loss with respect to the source, in the Abbe form:
+ * with I_A(x) = sum_s S_s |E_s(x)|^2 the untruncated image of the source src
+ * (equal to the Hopkins image of its TCC), dL/dS_s = sum_x G(x) |E_s(x)|^2,
+ * G = dL/dI_A, for every grid point inside the unit disc (zero elsewhere),
+ * with respect to the weights as given.  The optics are those of
+ * klt_tcc_build (FP64 Gram).  grad_src: nmask x srcSize^2; losses: nmask
+ * (loss of I_A); images: optional I_A (device pointers only, else NULL). */
+int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets, int nmask,
+                         int Lx, int Ly, const float* src, int srcSize, float lambda, float NA,
+                         float defocus, float nmed, int Nx, int Ny, float a, float thr,
+                         float* grad_src, float* losses, float* images, unsigned flags);
+
 #ifdef __cplusplus
 }
 #endif

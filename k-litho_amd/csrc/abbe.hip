@@ -1,0 +1,246 @@
+// abbe.hip -- gradient of the resist loss with respect to the source
+// (no reference counterpart), in the Abbe form of the partially coherent
+// image.
+//
+// With the kernels stored index-reversed (klitho_tcc.cpp:358-362), the image
+// of all SOCS kernels equals I_A(x) = sum_s S_s |E_s(x)|^2 with
+//   E_s(x) = sum_n A_s[n] e^{+2 pi i n.(x - c)/L},  A_s[n] = P(-n, s) M^[n],
+// which is also the Hopkins image of the TCC of the same source.  For the
+// loss L = sum_x (sigmoid(a (I_A - thr)) - T)^2, G = dL/dI_A and
+//   dL/dS_s = sum_x G(x) |E_s(x)|^2
+//           = Lx Ly sum_{n, n'} A_s[n] conj(A_s[n']) conj(G^[n - n'])
+// with G^ the |g| <= 2N band of G (the expression is exact: |E_s|^2 is band
+// limited to 2N).  The gradient is taken with respect to the source weights as
+// given (after createSource's normalisation), for every grid point inside the
+// unit disc (S_s = 0 included).
+//
+//   1. TCC of the source (FP64 Gram) and the Hopkins image I_A of each mask;
+//   2. mask band and resist band (G, loss) by the fused band GEMMs;
+//   3. k_abbe_assemble: per mask, M^ (2N+1)^2 and G^ (4N+1)^2 from the partials;
+//   4. k_abbe_grad: one block per (source point, mask): the pupil support is
+//      compacted into LDS and the double sum runs over the nonzero pairs, in
+//      FP64.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "klt_internal.h"
+#include "klt_pupil.h"
+
+namespace {
+
+using klt::PupilGeom;
+using klt::pupil_value;
+
+constexpr int kThreads = 256;
+
+__global__ __launch_bounds__(kThreads) void k_abbe_assemble(
+    const float2* __restrict__ mpart, const float2* __restrict__ gpart,
+    const float* __restrict__ lpart, int nblk, int Nx, int Ny, float inv_area,
+    float2* __restrict__ Mfull, float2* __restrict__ Gfull, float* __restrict__ loss) {
+  const int b = blockIdx.x;
+  const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
+  for (int i = threadIdx.x; i < Wx * Wy; i += kThreads) {
+    const int nyi = i / Wx, nx = i - nyi * Wx - Nx, ny = nyi - Ny;
+    const int r = nx >= 0 ? nyi : (-ny + Ny), c = nx >= 0 ? nx : -nx;
+    float2 s = make_float2(0.f, 0.f);
+    for (int k = 0; k < nblk; ++k) {
+      const float2 p = mpart[(((size_t)b * nblk + k) * Wy + r) * 16 + c];
+      s.x += p.x;
+      s.y += p.y;
+    }
+    Mfull[(size_t)b * Wx * Wy + i] = make_float2(s.x * inv_area, (nx < 0 ? -s.y : s.y) * inv_area);
+  }
+  for (int i = threadIdx.x; i < Gx * Gy; i += kThreads) {
+    const int gyi = i / Gx, gx = i - gyi * Gx - 2 * Nx, gy = gyi - 2 * Ny;
+    const int r = gx >= 0 ? gyi : (-gy + 2 * Ny), c = gx >= 0 ? gx : -gx;
+    float2 s = make_float2(0.f, 0.f);
+    for (int k = 0; k < nblk; ++k) {
+      const float2 p = gpart[(((size_t)b * nblk + k) * Gy + r) * 32 + c];
+      s.x += p.x;
+      s.y += p.y;
+    }
+    Gfull[(size_t)b * Gx * Gy + i] = make_float2(s.x * inv_area, (gx < 0 ? -s.y : s.y) * inv_area);
+  }
+  if (threadIdx.x == 0) {
+    float l = 0.f;
+    for (int k = 0; k < nblk; ++k) l += lpart[(size_t)b * nblk + k];
+    loss[b] = l;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_abbe_grad(PupilGeom g, const int2* __restrict__ pq,
+                                                        int srcSize,
+                                                        const float2* __restrict__ Mfull,
+                                                        const float2* __restrict__ Gfull,
+                                                        double area, float* __restrict__ grad) {
+  __shared__ float2 sG[61 * 61];
+  __shared__ float2 sA[31 * 31];
+  __shared__ short2 sN[31 * 31];
+  __shared__ int cnt[kThreads + 1];
+  __shared__ double red[4];
+  const int s = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  const int Nx = g.Nx, Ny = g.Ny, Wx = 2 * Nx + 1, T = g.T;
+  const int Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
+  for (int i = tid; i < Gx * Gy; i += kThreads) sG[i] = Gfull[(size_t)b * Gx * Gy + i];
+  const int2 p = pq[s];
+  // pupil support of this source point, compacted in frequency order
+  float2 av[4];
+  short2 an[4];
+  int c = 0;
+  for (int j = 0; j < 4; ++j) {
+    const int n = tid * 4 + j;
+    if (n >= T) break;
+    const int ny = n / Wx - Ny, nx = n % Wx - Nx;
+    float2 pv;
+    if (pupil_value(g, p.x, p.y, -nx, -ny, &pv)) {
+      const float2 m = Mfull[(size_t)b * T + n];
+      av[c] = make_float2(pv.x * m.x - pv.y * m.y, pv.x * m.y + pv.y * m.x);
+      an[c] = make_short2((short)nx, (short)ny);
+      ++c;
+    }
+  }
+  cnt[tid + 1] = c;
+  __syncthreads();
+  if (tid == 0) {
+    cnt[0] = 0;
+    for (int i = 1; i <= kThreads; ++i) cnt[i] += cnt[i - 1];
+  }
+  __syncthreads();
+  for (int j = 0; j < c; ++j) {
+    sA[cnt[tid] + j] = av[j];
+    sN[cnt[tid] + j] = an[j];
+  }
+  __syncthreads();
+  const int nnz = cnt[kThreads];
+  double acc = 0.0;
+  for (int u1 = tid; u1 < nnz; u1 += kThreads) {
+    const float2 a1 = sA[u1];
+    const short2 n1 = sN[u1];
+    for (int u2 = 0; u2 < nnz; ++u2) {
+      const float2 a2 = sA[u2];
+      const short2 n2 = sN[u2];
+      const float2 gg = sG[(n1.y - n2.y + 2 * Ny) * Gx + (n1.x - n2.x + 2 * Nx)];
+      // Re(a1 conj(a2) conj(gg))
+      const double pr = (double)a1.x * a2.x + (double)a1.y * a2.y;
+      const double pi = (double)a1.y * a2.x - (double)a1.x * a2.y;
+      acc += pr * gg.x + pi * gg.y;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = acc;
+  __syncthreads();
+  if (tid == 0) {
+    const double tot = red[0] + red[1] + red[2] + red[3];
+    grad[(size_t)b * srcSize * srcSize + (p.y + g.sh) * srcSize + p.x + g.sh] = (float)(area * tot);
+  }
+}
+
+__global__ void k_fill_zero(float* __restrict__ x, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.f;
+}
+
+}  // namespace
+
+extern "C" {
+
+int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets, int nmask,
+                         int Lx, int Ly, const float* src, int srcSize, float lambda, float NA,
+                         float defocus, float nmed, int Nx, int Ny, float a_sig, float thr,
+                         float* grad_src, float* losses, float* images, unsigned flags) {
+  if (!ctx || !masks || !targets || !src || !grad_src || !losses)
+    return klt::set_error(KLT_ERR_ARG, "klt_abbe_source_grad: NULL argument");
+  if (nmask <= 0 || Lx <= 0 || Ly <= 0 || srcSize <= 0 || Nx < 0 || Ny < 0)
+    return klt::set_error(KLT_ERR_ARG, "klt_abbe_source_grad: bad size");
+  if (Nx > 15 || Ny > 15)
+    return klt::set_error(KLT_ERR_ARG, "klt_abbe_source_grad: half-size N above 15");
+  if (4 * Nx + 1 > Lx || 4 * Ny + 1 > Ly) return klt::set_error(KLT_ERR_SIZE, "ERROR in FI()");
+  if (NA < 0 || !(nmed >= 1.0f) || NA > nmed)
+    return klt::set_error(KLT_ERR_ARG, "klt_abbe_source_grad: NA must not exceed the medium index");
+  KLT_TRY(ctx->activate());
+  hipStream_t st = ctx->stream;
+  const size_t npix = (size_t)Lx * Ly, bytes = (size_t)nmask * npix * sizeof(float);
+  const size_t ns2 = (size_t)srcSize * srcSize;
+  std::vector<float> hsrc(ns2);
+  const float *dm = masks, *dt = targets;
+  if (flags & KLT_HOST_PTRS) {
+    std::memcpy(hsrc.data(), src, ns2 * sizeof(float));
+    KLT_TRY(ctx->stage_in.ensure(2 * bytes));
+    char* s = ctx->stage_in.as<char>();
+    KLT_HIP(hipMemcpyAsync(s, masks, bytes, hipMemcpyHostToDevice, st));
+    KLT_HIP(hipMemcpyAsync(s + bytes, targets, bytes, hipMemcpyHostToDevice, st));
+    dm = reinterpret_cast<const float*>(s);
+    dt = reinterpret_cast<const float*>(s + bytes);
+  } else {
+    KLT_HIP(hipMemcpyAsync(hsrc.data(), src, ns2 * sizeof(float), hipMemcpyDeviceToHost, st));
+    KLT_HIP(hipStreamSynchronize(st));
+  }
+  float sigma = klt_outer_sigma(hsrc.data(), srcSize);
+  if (sigma > 1.0f) sigma = 1.0f;
+  const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
+  const int Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
+  const int nblk = (Ly + 127) / 128;
+  // outputs (device)
+  float *dg = grad_src, *dl = losses, *di = images;
+  if (flags & KLT_HOST_PTRS) {
+    KLT_TRY(ctx->stage_out.ensure((size_t)nmask * ns2 * 4 + (size_t)nmask * 4 + 512));
+    dg = ctx->stage_out.as<float>();
+    dl = dg + (size_t)nmask * ns2 + 64;
+    di = nullptr;
+  }
+  float* I = di;
+  if (!I) {
+    KLT_TRY(ctx->bwd_img.ensure(bytes));
+    I = ctx->bwd_img.as<float>();
+  }
+  // 1. TCC and Hopkins images
+  KLT_TRY(ctx->abbe_tcc.ensure((size_t)T * T * 16));
+  double* tcc = ctx->abbe_tcc.as<double>();
+  KLT_TRY(klt::tcc_build_device(ctx, hsrc.data(), srcSize, sigma, lambda, NA, defocus, nmed, Lx,
+                                Ly, Nx, Ny, KLT_TCC_FP64, tcc));
+  for (int b = 0; b < nmask; ++b)
+    KLT_TRY(klt::hopkins_image_device(ctx, dm + b * npix, Lx, Ly, tcc, Nx, Ny, I + b * npix));
+  // 2. bands
+  KLT_TRY(ctx->qpart.ensure((size_t)nmask * nblk * Wy * 16 * sizeof(float2)));
+  KLT_TRY(ctx->gpart.ensure((size_t)nmask * nblk * Gy * 32 * sizeof(float2)));
+  KLT_TRY(ctx->lpart.ensure((size_t)nmask * nblk * sizeof(float)));
+  KLT_TRY(klt::band_mask_partials(ctx, Lx, Ly, Nx, Ny, dm, nmask, ctx->qpart.as<float2>()));
+  KLT_TRY(klt::band_resist_partials(ctx, Lx, Ly, Nx, Ny, I, dt, nmask, a_sig, thr,
+                                    ctx->gpart.as<float2>(), ctx->lpart.as<float>()));
+  // 3. full bands per mask
+  KLT_TRY(ctx->abbe_band.ensure((size_t)nmask * (T + Gx * Gy) * sizeof(float2)));
+  float2* Mfull = ctx->abbe_band.as<float2>();
+  float2* Gfull = Mfull + (size_t)nmask * T;
+  const float inv_area = 1.0f / ((float)Lx * (float)Ly);
+  KLT_LAUNCH(ctx, "k_abbe_assemble", k_abbe_assemble, dim3(nmask), dim3(kThreads), 0,
+             ctx->qpart.as<float2>(), ctx->gpart.as<float2>(), ctx->lpart.as<float>(), nblk, Nx,
+             Ny, inv_area, Mfull, Gfull, dl);
+  // 4. per source point
+  const int sh = (int)((srcSize - 1) / 2.0);
+  std::vector<int2> pts;
+  for (int q = -sh; q <= sh; ++q)
+    for (int p = -sh; p <= sh; ++p)
+      if (p * p + q * q <= sh * sh) pts.push_back(make_int2(p, q));
+  KLT_TRY(ctx->misc.ensure(pts.size() * sizeof(int2)));
+  KLT_HIP(hipMemcpyAsync(ctx->misc.ptr, pts.data(), pts.size() * sizeof(int2),
+                         hipMemcpyHostToDevice, st));
+  const size_t ng = (size_t)nmask * ns2;
+  KLT_LAUNCH(ctx, "k_fill_zero", k_fill_zero, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0,
+             dg, ng);
+  const klt::PupilGeom g = klt::make_geom(srcSize, lambda, NA, defocus, nmed, Lx, Ly, Nx, Ny);
+  KLT_LAUNCH(ctx, "k_abbe_grad", k_abbe_grad, dim3((unsigned)pts.size(), nmask), dim3(kThreads),
+             0, g, ctx->misc.as<int2>(), srcSize, Mfull, Gfull, (double)Lx * (double)Ly, dg);
+  KLT_HIP(hipGetLastError());
+  if (flags & KLT_HOST_PTRS) {
+    KLT_HIP(hipMemcpyAsync(grad_src, dg, ng * 4, hipMemcpyDeviceToHost, st));
+    KLT_HIP(hipMemcpyAsync(losses, dl, (size_t)nmask * 4, hipMemcpyDeviceToHost, st));
+    if (images) KLT_HIP(hipMemcpyAsync(images, I, bytes, hipMemcpyDeviceToHost, st));
+  }
+  KLT_HIP(hipStreamSynchronize(st));  // the host point list must outlive the copy
+  return KLT_OK;
+}
+
+}  // extern "C"

@@ -86,6 +86,7 @@ struct klt_ctx {
   // scratch reused across calls (grown on demand)
   klt::DevBuf stage_in, stage_out, qpart, band, spec, ccols, misc, eig_ws;
   klt::DevBuf gpart, gspec, lpart, bwd_img;  // adjoint scratch
+  klt::DevBuf abbe_tcc, abbe_band;            // source-gradient scratch
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::SocsPlan>> socs_plans;
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::FusedTables>> fused_tables;
   int activate() const;  // hipSetDevice(device)
@@ -126,6 +127,21 @@ int socs_forward_device(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
 bool socs_fused_supported(const klt_kernels* kb);
 int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int B, int Lx,
                        int Ly, float* images);
+// Band partial sums of the fused passes (socs_fused.hip), nblk = ceil(Ly/128):
+// mask band  [B][nblk][2Ny+1][16]  (nx in [0, Nx], centre floor(L/2));
+// resist band [B][nblk][4Ny+1][32] (gx in [0, 2Nx], centre (L+1)/2), with the
+// loss partials lpart [B][nblk].  Kernel half-size <= 15.
+int band_mask_partials(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const float* masks, int B,
+                       float2* out);
+int band_resist_partials(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const float* images,
+                         const float* targets, int B, float a_sig, float thr, float2* out,
+                         float* lpart);
+// TCC on the device (tcc.hip) and the Hopkins image (socs.hip).
+int tcc_build_device(klt_ctx* ctx, const float* src_host, int srcSize, float outerSigma,
+                     float lambda, float NA, float defocus, float nmed, int Lx, int Ly,
+                     int Nx, int Ny, int precision, double* tcc_dev);
+int hopkins_image_device(klt_ctx* ctx, const float* mask, int Lx, int Ly, const double* tcc,
+                         int Nx, int Ny, float* image);
 // Top-k eigenpairs of a Hermitian matrix on the device (eig.hip).
 int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_dev,
                     double* vec_dev);

@@ -785,6 +785,35 @@ int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, 
   return KLT_OK;
 }
 
+int band_mask_partials(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const float* masks, int B,
+                       float2* out) {
+  SocsPlan* p = nullptr;
+  KLT_TRY(get_socs_plan(ctx, Lx, Ly, Nx, Ny, &p));
+  FusedTables* ft = nullptr;
+  KLT_TRY(get_fused_tables(ctx, p, &ft));
+  const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
+  KLT_LAUNCH(ctx, "k_band_mfma", (k_band_mfma<2, false>), dim3(nblk, B), dim3(256), 0, masks,
+             nullptr, 0.f, 0.f, Lx, Ly, ft->Kpad, ft->Wh.as<_Float16>(), ft->Wl.as<_Float16>(),
+             p->twy_in.as<float2>(), p->NP, Nx, Ny, 16, out, nullptr);
+  KLT_HIP(hipGetLastError());
+  return KLT_OK;
+}
+
+int band_resist_partials(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const float* images,
+                         const float* targets, int B, float a_sig, float thr, float2* out,
+                         float* lpart) {
+  SocsPlan* p = nullptr;
+  KLT_TRY(get_socs_plan(ctx, Lx, Ly, Nx, Ny, &p));
+  FusedTables* ft = nullptr;
+  KLT_TRY(get_fused_tables(ctx, p, &ft));
+  const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
+  KLT_LAUNCH(ctx, "k_band_resist", (k_band_mfma<4, true>), dim3(nblk, B), dim3(256), 0, images,
+             targets, a_sig, thr, Lx, Ly, ft->Kpad, ft->Wgh.as<_Float16>(),
+             ft->Wgl.as<_Float16>(), ft->twyg.as<float2>(), 32, 2 * Nx, 2 * Ny, 32, out, lpart);
+  KLT_HIP(hipGetLastError());
+  return KLT_OK;
+}
+
 int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
                         const float* targets, int B, int Lx, int Ly, float a_sig, float thr,
                         float* grads, float* losses, float* images) {

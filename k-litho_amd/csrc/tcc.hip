@@ -21,52 +21,16 @@
 #include <vector>
 
 #include "klt_internal.h"
+#include "klt_pupil.h"
 
 namespace {
 
+using klt::PupilGeom;
+using klt::SrcPoint;
+using klt::pupil_value;
+
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
-
-struct PupilGeom {
-  int sh, Nx, Ny, T;
-  float dz, k, Lxn, Lyn, NA, nm2;
-};
-
-struct SrcPoint {
-  int p, q;
-  float S;
-};
-
-// Pupil value at frequency (nx, ny) for source point (p, q), as the loops at
-// klitho_tcc.cpp:202-222 compute it (returns false when the reference leaves
-// the entry zero).  Zero defocus gives a unit pupil without evaluating the
-// root, which extends the formula to NA > 1 (where the reference's
-// sqrt(1 - rho^2 NA^2) is NaN); nm2 is the squared immersion index (1 in the
-// reference).
-__device__ bool pupil_value(const PupilGeom& g, int p, int q, int nx, int ny, float2* out) {
-  const float sx = __fdiv_rn((float)p, (float)g.sh);
-  const float sy = __fdiv_rn((float)q, (float)g.sh);
-  const int nxMin = (int)__fmul_rn(__fsub_rn(-1.0f, sx), g.Lxn);
-  const int nxMax = (int)__fmul_rn(__fsub_rn(1.0f, sx), g.Lxn);
-  const int nyMin = (int)__fmul_rn(__fsub_rn(-1.0f, sy), g.Lyn);
-  const int nyMax = (int)__fmul_rn(__fsub_rn(1.0f, sy), g.Lyn);
-  if (nx < nxMin || nx > nxMax || ny < nyMin || ny > nyMax) return false;
-  const float fy = __fadd_rn(__fdiv_rn((float)ny, g.Lyn), sy);
-  if (!(__fmul_rn(fy, fy) <= 1.0f)) return false;
-  const float fx = __fadd_rn(__fdiv_rn((float)nx, g.Lxn), sx);
-  const float rho2 = __fadd_rn(__fmul_rn(fx, fx), __fmul_rn(fy, fy));
-  if (!(rho2 <= 1.0f)) return false;
-  if (g.dz == 0.0f) {
-    *out = make_float2(1.0f, 0.0f);
-    return true;
-  }
-  const float rn = __fmul_rn(__fmul_rn(rho2, g.NA), g.NA);
-  const float dzk = __fmul_rn(g.dz, g.k);
-  const double root = __dsqrt_rn(__dsub_rn((double)g.nm2, (double)rn));
-  const float phi = __double2float_rn(__dmul_rn((double)dzk, root));
-  *out = make_float2(cosf(phi), sinf(phi));
-  return true;
-}
 
 // P[s][i] for every used source point (blockIdx.y) and frequency i.
 __global__ __launch_bounds__(256) void k_pupil(PupilGeom g, const SrcPoint* __restrict__ pts,
@@ -260,24 +224,6 @@ int round_up(int v, int m) { return (v + m - 1) / m * m; }
 }  // namespace
 
 namespace klt {
-
-// Geometry constants in the reference's float arithmetic (klitho_tcc.cpp:189-197).
-static PupilGeom make_geom(int srcSize, float lambda, float NA, float defocus, float nmed,
-                           int Lx, int Ly, int Nx, int Ny) {
-  PupilGeom g;
-  g.sh = (int)((srcSize - 1) / 2.0);
-  g.Nx = Nx;
-  g.Ny = Ny;
-  g.T = (2 * Nx + 1) * (2 * Ny + 1);
-  const float na2 = NA * NA;
-  g.dz = defocus / (na2 / lambda);
-  g.k = (float)(2 * M_PI / lambda);
-  g.Lxn = (float)Lx * NA / lambda;
-  g.Lyn = (float)Ly * NA / lambda;
-  g.NA = NA;
-  g.nm2 = nmed * nmed;
-  return g;
-}
 
 // Source points in the reference's order (q outer, p inner), klitho_tcc.cpp:202-205.
 static std::vector<SrcPoint> source_points(const float* src, int srcSize, float outerSigma) {

@@ -77,6 +77,8 @@ def lib() -> C.CDLL:
         L.klt_hopkins_image.argtypes = [vp, fp, i, i, fp, i, i, fp, u]
         L.klt_socs_backward.argtypes = [vp, vp, fp, fp, i, i, i, C.c_float, C.c_float, fp, fp,
                                         fp, u]
+        L.klt_abbe_source_grad.argtypes = [vp, fp, fp, i, i, i, fp, i, f, f, f, f, i, i, f, f,
+                                           fp, fp, fp, u]
         L.klt_kernels_create.argtypes = [vp, fp, fp, fp, i, i, i, u, C.POINTER(vp)]
         L.klt_kernels_destroy.argtypes = [vp]
         L.klt_kernels_destroy.restype = None
@@ -362,6 +364,34 @@ def socs_backward(ctx: "Context", bank: "KernelBank", masks, targets, a: float =
         hs.append(hi)
     check(lib().klt_socs_backward(ctx.handle, bank.handle, pm, pt, B, Lx, Ly, a, thr, pg, pl,
                                   pi, _flags(*hs)))
+    return losses, grads, images
+
+
+def abbe_source_grad(ctx: "Context", masks, targets, src, NA: float, defocus: float, Nx: int,
+                     Ny: int, a: float = 50.0, thr: float = 0.25, nmed: float = 1.0,
+                     lam: float = 193.0, want_images: bool = False):
+    """(losses [B], grad_src [B, n, n], Abbe images or None) of the resist loss."""
+    shape = tuple(masks.shape)
+    Ly, Lx = shape[-2:]
+    B = int(np.prod(shape[:-2])) if len(shape) > 2 else 1
+    n = int(src.shape[0])
+    grads = _empty_like(masks, (B, n, n))
+    losses = _empty_like(masks, (B,))
+    images = _empty_like(masks) if want_images else None
+    pm, hm = _ptr(masks, np.float32)
+    pt, ht = _ptr(targets, np.float32)
+    ps, hs_ = _ptr(src, np.float32)
+    pg, hg = _ptr(grads, np.float32)
+    pl, hl = _ptr(losses, np.float32)
+    hs = [hm, ht, hs_, hg, hl]
+    pi = None
+    if images is not None:
+        if not _is_torch(images):
+            raise TypeError("Abbe images are returned on the device only")
+        pi, hi = _ptr(images, np.float32)
+        hs.append(hi)
+    check(lib().klt_abbe_source_grad(ctx.handle, pm, pt, B, Lx, Ly, ps, n, lam, NA, defocus,
+                                     nmed, Nx, Ny, a, thr, pg, pl, pi, _flags(*hs)))
     return losses, grads, images
 
 
