@@ -168,8 +168,9 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
   const float *dm = masks, *dt = targets;
   if (flags & KLT_HOST_PTRS) {
     std::memcpy(hsrc.data(), src, ns2 * sizeof(float));
-    KLT_TRY(ctx->stage_in.ensure(2 * bytes));
-    char* s = ctx->stage_in.as<char>();
+    // own staging buffer: the TCC build below uses the context's stage_in
+    KLT_TRY(ctx->abbe_in.ensure(2 * bytes));
+    char* s = ctx->abbe_in.as<char>();
     KLT_HIP(hipMemcpyAsync(s, masks, bytes, hipMemcpyHostToDevice, st));
     KLT_HIP(hipMemcpyAsync(s + bytes, targets, bytes, hipMemcpyHostToDevice, st));
     dm = reinterpret_cast<const float*>(s);

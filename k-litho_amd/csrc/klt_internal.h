@@ -86,7 +86,8 @@ struct klt_ctx {
   // scratch reused across calls (grown on demand)
   klt::DevBuf stage_in, stage_out, qpart, band, spec, ccols, misc, eig_ws;
   klt::DevBuf gpart, gspec, lpart, bwd_img;  // adjoint scratch
-  klt::DevBuf abbe_tcc, abbe_band;            // source-gradient scratch
+  klt::DevBuf abbe_tcc, abbe_band, abbe_in;   // source-gradient scratch / staging
+  klt::DevBuf tcc_pts;                        // source points of the TCC build
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::SocsPlan>> socs_plans;
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::FusedTables>> fused_tables;
   int activate() const;  // hipSetDevice(device)

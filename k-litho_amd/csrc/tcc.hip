@@ -240,19 +240,19 @@ static std::vector<SrcPoint> source_points(const float* src, int srcSize, float 
 }
 
 // Pupil matrix of every used source point into ctx->misc ([ns][T] float2);
-// the points are left in ctx->stage_in.
+// the points are left in ctx->tcc_pts.
 static int build_pupil(klt_ctx* ctx, const float* src_host, int srcSize, float outerSigma,
                        const PupilGeom& g, int* ns_out, std::vector<SrcPoint>& pts) {
   pts = source_points(src_host, srcSize, outerSigma);
   const int ns = (int)pts.size();
   *ns_out = ns;
   if (ns == 0) return KLT_OK;
-  KLT_TRY(ctx->stage_in.ensure(sizeof(SrcPoint) * ns));
-  KLT_HIP(hipMemcpyAsync(ctx->stage_in.ptr, pts.data(), sizeof(SrcPoint) * ns,
+  KLT_TRY(ctx->tcc_pts.ensure(sizeof(SrcPoint) * ns));
+  KLT_HIP(hipMemcpyAsync(ctx->tcc_pts.ptr, pts.data(), sizeof(SrcPoint) * ns,
                          hipMemcpyHostToDevice, ctx->stream));
   KLT_TRY(ctx->misc.ensure(sizeof(float2) * (size_t)ns * g.T));
   KLT_LAUNCH(ctx, "k_pupil", k_pupil, dim3((g.T + 255) / 256, ns), dim3(256), 0, g,
-             ctx->stage_in.as<SrcPoint>(), ctx->misc.as<float2>());
+             ctx->tcc_pts.as<SrcPoint>(), ctx->misc.as<float2>());
   KLT_HIP(hipGetLastError());
   return KLT_OK;
 }
@@ -278,7 +278,7 @@ int tcc_build_device(klt_ctx* ctx, const float* src_host, int srcSize, float out
     KLT_HIP(hipMemcpyAsync(ctx->spec.ptr, tiles.data(), sizeof(int2) * tiles.size(),
                            hipMemcpyHostToDevice, ctx->stream));
     KLT_LAUNCH(ctx, "k_gram_f64", k_gram_f64, dim3((unsigned)tiles.size()), dim3(256), 0,
-               ctx->misc.as<float2>(), ctx->stage_in.as<SrcPoint>(), ns, T,
+               ctx->misc.as<float2>(), ctx->tcc_pts.as<SrcPoint>(), ns, T,
                ctx->spec.as<int2>(), tcc_dev);
     KLT_HIP(hipGetLastError());
     KLT_HIP(hipStreamSynchronize(ctx->stream));  // tiles / points stay valid until here
@@ -297,7 +297,7 @@ int tcc_build_device(klt_ctx* ctx, const float* src_host, int srcSize, float out
   KLT_TRY(ctx->qpart.ensure(6 * nel * sizeof(_Float16)));
   _Float16* base = ctx->qpart.as<_Float16>();
   KLT_LAUNCH(ctx, "k_split_operands", k_split_operands, dim3(Tpad), dim3(256), 0,
-             ctx->misc.as<float2>(), ctx->stage_in.as<SrcPoint>(), ns, T, Kpad, scale, base,
+             ctx->misc.as<float2>(), ctx->tcc_pts.as<SrcPoint>(), ns, T, Kpad, scale, base,
              base + nel, base + 2 * nel, base + 3 * nel, base + 4 * nel, base + 5 * nel);
   const int nt = Tpad / 64;
   std::vector<int2> tiles;
