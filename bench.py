@@ -1,16 +1,27 @@
 """Benchmark of the SOCS imaging hot path on one MI355X.
 
-Workload (1024^2 SOCS imaging, K = 24): seeded 1024 x 1024 Manhattan masks,
-24 kernels of 29 x 29.  A step is one klt_socs_forward call over a batch of
-masks already resident in HBM; `value` is images per second over the timed
-steps.  Also reported: the plain single-image path (one call per image), the
-per-kernel breakdown measured with HIP events on the library's stream, the
-roofline of the dominant kernel, and a CPU baseline: a numpy/pocketfft port
-of the reference's FFTW path (FT_r2c + calcSOCS + FI) timed on a bounded
-sample on this host, with the GPU image's relative L2 error against the
-FP64 oracle.
+Headline workload (1024^2 SOCS imaging, K = 24): seeded 1024 x 1024 Manhattan
+masks and the 24 kernels of 29 x 29 that the library itself derives on the
+GPU (TCC of the 51 x 51 annular source sigma 0.6/0.9 at NA 1.35, period
+1088 nm, then the FP64 eigensolver).  A step is one klt_socs_forward call
+over a batch of masks already resident in HBM; `value` is images per second
+over the timed steps.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--batch 256]
+Also reported (same JSON line):
+  plain        the FP32 single-image path (klt_socs_image, one call per image)
+  adjoint      klt_socs_backward on 16 relaxed 1024^2 masks (mask gradients/s)
+               and klt_abbe_source_grad (source gradient over the 51^2 grid)
+  tcc_eig      TCC T = 1225 (blurred dipole, period 1280, 5 defocus values),
+               split-FP16 MFMA Gram vs FP64 Gram, top-32 FP64 eigensolve
+  kernels      per-kernel HIP-event times of one step; `roofline` is the
+               dominant kernel's algorithmic HBM bytes over its duration
+  cpu_baseline a numpy/pocketfft port of the reference's FFTW path
+               (FT_r2c + calcSOCS + FI) timed on a bounded sample on this
+               host, with the GPU image's (and a 512^2 adjoint sample's)
+               relative L2 error vs the FP64 oracle
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--batch 512]
+    python bench.py --copy-calibration   # also times a 1 GiB torch copy
 """
 from __future__ import annotations
 
@@ -29,33 +40,122 @@ import klt  # noqa: E402
 import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+MFMA_F16_PEAK_TFLOPS = 2500.0
 L = 1024
 NK = 24
 KSIZE = 29
+NMED = 1.44  # water immersion index: NA 1.35 needs n >= NA; at zero defocus it has no effect
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kernel_bank_arrays():
-    """24 kernels of 29 x 29 with decaying eigenvalues (unit-norm, seeded).
+class Timer:
+    """HIP-event timing on one stream."""
 
-    The SOCS cost does not depend on kernel values; parity of the path is
-    checked against the oracle with these same kernels below."""
-    rng = np.random.Generator(np.random.PCG64(7))
-    T = KSIZE * KSIZE
-    kr = rng.standard_normal((NK, T))
-    ki = rng.standard_normal((NK, T))
-    nrm = np.sqrt((kr ** 2 + ki ** 2).sum(1, keepdims=True))
-    kr = (kr / nrm).astype(np.float32)
-    ki = (ki / nrm).astype(np.float32)
-    sc = (78.8 * 0.8 ** np.arange(NK)).astype(np.float32)
-    return kr, ki, sc
+    def __init__(self, torch, stream):
+        self.torch, self.stream = torch, stream
+
+    def __call__(self, fn, reps=1):
+        t = self.torch
+        e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        t.cuda.synchronize()
+        e0.record(self.stream)
+        for _ in range(reps):
+            fn()
+        e1.record(self.stream)
+        t.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps  # ms
 
 
-def cpu_baseline(gpu_img: np.ndarray, mask: np.ndarray, kr, ki, sc, budget_s: float):
-    """Reference-path port (numpy) on one core + FP64 oracle error."""
+def annular_source():
+    return klt.source_create(W.SRC_SIZE, "Annular", W.SIGMA_IN, W.SIGMA_OUT)
+
+
+def gpu_kernels(ctx):
+    """The 24 kernels of 29 x 29 of the 1024^2 workload, built on the GPU."""
+    src = annular_source()
+    sig = min(klt.outer_sigma(src), 1.0)
+    Nx, Ny = klt.tcc_sizing(W.NA, W.PERIOD_29, W.PERIOD_29, sig)
+    assert (Nx, Ny) == (14, 14)
+    tcc = ctx.tcc_build(src, sig, W.NA, 0.0, W.PERIOD_29, W.PERIOD_29, Nx, Ny, precision=0,
+                        nmed=NMED)
+    kr, ki, sc = ctx.calc_kernels(tcc, NK)
+    return src, kr, ki, sc
+
+
+def blurred_dipole():
+    """101 x 101 binary dipole (sigma_c 0.7, radius 0.2, on x) blurred by a
+    Gaussian of 1.5 cells (truncated at 4 sigma), normalised."""
+    d = klt.source_create(101, "Dipole", radius=0.2, offset=0.7, on_x_axis=1)
+    r = np.arange(-6, 7)
+    k = np.exp(-0.5 * (r / 1.5) ** 2)
+    k /= k.sum()
+    b = np.apply_along_axis(lambda v: np.convolve(v, k, mode="same"), 0, d.astype(np.float64))
+    b = np.apply_along_axis(lambda v: np.convolve(v, k, mode="same"), 1, b)
+    return np.ascontiguousarray((b / b.sum()).astype(np.float32))
+
+
+def bench_tcc_eig(ctx, timer):
+    src = blurred_dipole()
+    sig = min(klt.outer_sigma(src), 1.0)
+    Nx, Ny = klt.tcc_sizing(W.NA, W.PERIOD_35, W.PERIOD_35, sig)
+    T = (2 * Nx + 1) * (2 * Ny + 1)
+    sh = 50
+    yy, xx = np.mgrid[-sh:sh + 1, -sh:sh + 1]
+    used = int(np.count_nonzero((src != 0) & (xx * xx + yy * yy <= sh * sh)))
+    out = {"T": T, "source_points": used, "defocus_nm": [-50, -25, 0, 25, 50]}
+    rows = []
+    for df_nm in out["defocus_nm"]:
+        df = df_nm * W.NA ** 2 / W.LAMBDA
+
+        def build(prec):
+            return ctx.tcc_build(src, sig, W.NA, df, W.PERIOD_35, W.PERIOD_35, Nx, Ny, prec, NMED)
+
+        t64 = build(0)
+        t16 = build(1)
+        a = t64[..., 0] + 1j * t64[..., 1]
+        b = t16[..., 0] + 1j * t16[..., 1]
+        ms16 = timer(lambda: build(1), 3)
+        ms64 = timer(lambda: build(0), 3)
+        w64, _ = ctx.eig_topk(t64, 32)
+        w16, _ = ctx.eig_topk(t16, 32)
+        mse = timer(lambda: ctx.eig_topk(t64, 32), 1)
+        rows.append({"defocus_nm": df_nm, "tcc_split16_ms": ms16, "tcc_fp64_ms": ms64,
+                     "eig_top32_ms": mse,
+                     "tcc_rel_fro_err_split16_vs_fp64":
+                         float(np.linalg.norm(b - a) / np.linalg.norm(a)),
+                     "lambda_max_rel_err_split16": float(np.abs(w16 - w64).max() / w64[0]),
+                     "lambda0": float(w64[0])})
+    # algorithmic flops of the Gram: 8 n_src T^2 (complex MACs); split issues 3x
+    flops = 8.0 * used * T * T
+    best = min(r["tcc_split16_ms"] for r in rows)
+    out.update({"rows": rows, "gram_flops": flops,
+                "split16_tflops_algorithmic": flops / (best / 1e3) / 1e12,
+                "split16_mfma_utilisation": 3 * flops / (best / 1e3) / 1e12 / MFMA_F16_PEAK_TFLOPS,
+                "note": "times include the host round trip of the T x T result"})
+    return out
+
+
+def bench_adjoint(ctx, bank, timer, torch, dev, src):
+    B = 16
+    tg = np.stack([W.manhattan(L, 3000 + i) for i in range(B)])
+    mk = np.stack([W.relaxed(tg[i], 4000 + i) for i in range(B)])
+    dm = torch.from_numpy(mk).to(dev)
+    dt = torch.from_numpy(tg).to(dev)
+    ds = torch.from_numpy(np.ascontiguousarray(src)).to(dev)
+    klt.socs_backward(ctx, bank, dm, dt)
+    ms = timer(lambda: klt.socs_backward(ctx, bank, dm, dt), 5)
+    klt.abbe_source_grad(ctx, dm, dt, ds, W.NA, 0.0, 14, 14, nmed=NMED)
+    ms_src = timer(lambda: klt.abbe_source_grad(ctx, dm, dt, ds, W.NA, 0.0, 14, 14, nmed=NMED), 2)
+    return {"batch": B, "mask_grads_per_s": B / (ms / 1e3), "ms_per_batch": ms,
+            "source_grad_ms_per_batch": ms_src,
+            "hbm_bytes_per_mask_grad": 4 * 5 * L * L}
+
+
+def cpu_baseline(ctx, bank, fused_img, mask, kr, ki, sc, budget_s):
+    """Reference-path port (numpy) on one core, and FP64 oracle errors."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     import ref_pipeline as RP
@@ -69,10 +169,17 @@ def cpu_baseline(gpu_img: np.ndarray, mask: np.ndarray, kr, ki, sc, budget_s: fl
             break
     dt = time.perf_counter() - t0
     ref64 = O.socs_image(mask, kr, ki, sc, N, N)
-    err = float(np.linalg.norm(gpu_img - ref64) / np.linalg.norm(ref64))
-    return {"value": n / dt, "unit": "images/s", "cores": 1, "kind": "port",
-            "sample": f"{n} seeded 1024^2 Manhattan masks, K=24 29x29, numpy pocketfft "
-                      f"port of FT_r2c+calcSOCS+FI, {dt:.1f} s on 1 core"}, err
+    err = float(np.linalg.norm(fused_img - ref64) / np.linalg.norm(ref64))
+    # adjoint sample: one relaxed 512^2 mask
+    tg = W.manhattan(512, 77)
+    mk = W.relaxed(tg, 78)
+    _, g, _ = klt.socs_backward(ctx, bank, mk[None].copy(), tg[None].copy())
+    _, _, g64 = O.socs_loss_grad(mk.astype(np.float64), tg, kr, ki, sc, N, N)
+    gerr = float(np.linalg.norm(g[0] - g64) / np.linalg.norm(g64))
+    cpu = {"value": n / dt, "unit": "images/s", "cores": 1, "kind": "port",
+           "sample": f"{n} seeded 1024^2 Manhattan masks, K=24 29x29, numpy pocketfft port "
+                     f"of FT_r2c+calcSOCS+FI, {dt:.1f} s on 1 core"}
+    return cpu, err, gerr
 
 
 def main():
@@ -80,9 +187,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="headline and plain path only")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the SOCS "
@@ -97,11 +206,15 @@ def main():
     stream = torch.cuda.Stream()  # default priority; the library launches on it
     ctx.set_stream(stream.cuda_stream)
     torch.cuda.set_stream(stream)
+    timer = Timer(torch, stream)
 
-    kr, ki, sc = kernel_bank_arrays()
+    t0 = time.perf_counter()
+    src, kr, ki, sc = gpu_kernels(ctx)
+    log(f"kernels from the GPU TCC + eigensolver: lambda0 {sc[0]:.4f} "
+        f"({time.perf_counter() - t0:.2f} s)")
     bank = ctx.kernels(kr, ki, sc, KSIZE, KSIZE)
     B = args.batch
-    log(f"generating {B} masks of {L}^2")
+    log(f"generating masks of {L}^2 (batch {B})")
     masks_h = W.manhattan_batch(L, min(B, 64), 1000)
     reps = (B + masks_h.shape[0] - 1) // masks_h.shape[0]
     masks = torch.from_numpy(masks_h).to(dev).repeat(reps, 1, 1)[:B].contiguous()
@@ -112,22 +225,14 @@ def main():
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
     w0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        step()
-    e1.record(stream)
-    torch.cuda.synchronize()
+    ms = timer(step, args.steps)
     wall = time.perf_counter() - w0
-    ms = e0.elapsed_time(e1) / args.steps
     value = B / (ms / 1e3)
-    log(f"batched: {ms:.3f} ms/step, {value:.1f} images/s (wall {wall:.2f} s)")
+    log(f"batched fused: {ms:.3f} ms/step, {value:.1f} images/s (wall {wall:.2f} s)")
+    fused_img0 = images[0].cpu().numpy()
 
-    # per-kernel breakdown of one step (HIP events on the library stream)
+    # per-kernel breakdown (HIP events bracketing each launch on the stream)
     ctx.set_profiling(True)
     prof_steps = 3
     for _ in range(prof_steps):
@@ -139,60 +244,73 @@ def main():
     tot = sum(v[1] for v in rep.values()) / prof_steps
     for k in kern:
         kern[k]["share"] = kern[k]["ms_per_launch"] * kern[k]["launches"] / tot
-    dom = max(kern, key=lambda k: kern[k]["ms_per_launch"] * kern[k]["launches"])
-    # algorithmic HBM bytes per launch of each streaming kernel
-    alg = {"k_band_cols": 4.0 * L * L * B, "k_synth_rows": 4.0 * L * L * B,
-           "k_band_mfma": 4.0 * L * L * B, "k_synth_mfma": 4.0 * L * L * B}
-    if dom in alg:
-        ach = alg[dom] / (kern[dom]["ms_per_launch"] / 1e3) / 1e9
-        roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
-    else:
-        path_gbs = 8.0 * L * L * B / (tot / 1e3) / 1e9
-        roof = {"kernel": dom, "bound": "hbm", "achieved": path_gbs, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": path_gbs / HBM_PEAK_GBS, "traffic": None,
-                "note": "dominant kernel is compute-bound; whole-path bytes over path time"}
+    alg = {"k_band_mfma": 4.0 * L * L, "k_synth_mfma": 4.0 * L * L}  # bytes per image
+    for k, v in kern.items():
+        if k in alg:
+            per_launch = alg[k] * B / v["launches"]
+            v["algorithmic_bytes_per_launch"] = per_launch
+            v["achieved_gbs"] = per_launch / (v["ms_per_launch"] / 1e3) / 1e9
+    dom = max((k for k in kern if k in alg), key=lambda k: kern[k]["ms_per_launch"])
+    ach = kern[dom]["achieved_gbs"]
+    roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+            "algorithmic_bytes_per_launch": kern[dom]["algorithmic_bytes_per_launch"]}
     path_gbs = 8.0 * L * L * B / (ms / 1e3) / 1e9
 
-    fused_img0 = images[0].cpu().numpy()
     # plain single-instance path: one call per image
     one_m = masks[:1].clone()
     one_i = torch.empty_like(one_m)
     for _ in range(3):
         ctx.socs_forward(bank, one_m, one_i, plain=True)
-    torch.cuda.synchronize()
-    n1 = 64
-    e0.record(stream)
-    for _ in range(n1):
-        ctx.socs_forward(bank, one_m, one_i, plain=True)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    plain = n1 / (e0.elapsed_time(e1) / 1e3)
+    plain = 1.0 / (timer(lambda: ctx.socs_forward(bank, one_m, one_i, plain=True), 64) / 1e3)
     log(f"plain single-image path: {plain:.1f} images/s")
 
-    cpu = None
-    err = None
+    extras = {}
+    if not args.no_extras:
+        adj = bench_adjoint(ctx, bank, timer, torch, dev, src)
+        log(f"adjoint: {adj['mask_grads_per_s']:.1f} mask grads/s, source grad "
+            f"{adj['source_grad_ms_per_batch']:.2f} ms / 16 masks")
+        extras["adjoint"] = adj
+        tce = bench_tcc_eig(ctx, timer)
+        log(f"tcc T={tce['T']}: split16 {tce['split16_tflops_algorithmic']:.1f} TF/s "
+            f"algorithmic, eig top-32 {tce['rows'][2]['eig_top32_ms']:.1f} ms")
+        extras["tcc_eig"] = tce
+    if args.copy_calibration:
+        a = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev)
+        bcopy = torch.empty_like(a)
+        cms = timer(lambda: bcopy.copy_(a), 5)
+        extras["copy_calibration"] = {"bytes_read": a.numel() * 4, "bytes_written": a.numel() * 4,
+                                      "ms": cms, "gbs": 2 * a.numel() * 4 / (cms / 1e3) / 1e9}
+        log(f"copy calibration: {extras['copy_calibration']['gbs']:.0f} GB/s")
+
+    cpu = err = gerr = None
     if not args.no_cpu:
-        cpu, err = cpu_baseline(fused_img0, masks_h[0], kr, ki, sc, args.cpu_seconds)
-        log(f"cpu baseline: {cpu['value']:.2f} images/s; GPU rel-L2 vs FP64 oracle {err:.2e}")
+        cpu, err, gerr = cpu_baseline(ctx, bank, fused_img0, masks_h[0], kr, ki, sc,
+                                      args.cpu_seconds)
+        log(f"cpu baseline: {cpu['value']:.2f} images/s; GPU image rel-L2 vs FP64 {err:.2e}, "
+            f"mask gradient rel-L2 vs FP64 {gerr:.2e}")
 
     out = {
         "metric": "SOCS images/s at 1024^2 (K=24) + adjoint grads/s; rel-L2 err vs FP64 oracle",
         "value": value, "unit": "images/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "SOCS imaging 1024x1024 Manhattan masks, K=24 kernels 29x29",
+        "config": {"workload": "SOCS imaging, 1024x1024 seeded Manhattan masks, K=24 kernels "
+                               "29x29 (annular 0.6/0.9, NA 1.35, period 1088 nm)",
                    "batch": B, "L": L, "K": NK, "kernel_size": KSIZE,
-                   "kernels": "seeded unit-norm 29x29 bank (timing independent of values)",
+                   "precision": "split hi+lo FP16 matrix-core DFT passes, FP32 accumulation, "
+                                "FP32 coarse FFTs",
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "image_rel_l2_vs_fp64": err,
+        "grad_rel_l2_vs_fp64": gerr,
         "path_hbm_gbs": path_gbs,
         "plain_images_per_s": plain,
         "speedup_vs_plain": value / plain,
-        "image_rel_l2_vs_fp64": err,
         "kernels": kern,
     }
+    out.update(extras)
     print(json.dumps(out))
     bank.close()
     ctx.close()
