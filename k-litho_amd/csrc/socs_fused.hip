@@ -91,6 +91,7 @@ __global__ __launch_bounds__(256) void k_band_mfma(
     const float2* __restrict__ twy, int ldtw, int Bx, int By, int ldo,
     float2* __restrict__ out_part, float* __restrict__ lpart) {
   __shared__ float sR[kRowsBand][16 * NCB + 1];
+  __shared__ float2 sTw[kRowsBand][8 * NCB];
   __shared__ float red[4];
   const int blk = blockIdx.x, b = blockIdx.y, nblk = gridDim.x;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -107,14 +108,35 @@ __global__ __launch_bounds__(256) void k_band_mfma(
 #pragma unroll
     for (int c = 0; c < NCB; ++c) acc[a][c] = float4_t{0.f, 0.f, 0.f, 0.f};
   float lsum = 0.f;
+  // software pipeline: the loads of step k + 32 are issued before the
+  // conversions and matrix products of step k
+  float na8[8], nb8[8], nta8[8], ntb8[8];
+  load8(in0 + ra, kq, Lx, va, vec, na8);
+  load8(in0 + rb, kq, Lx, vb, vec, nb8);
+  if (RESIST) {
+    load8(in1 + ra, kq, Lx, va, vec, nta8);
+    load8(in1 + rb, kq, Lx, vb, vec, ntb8);
+  }
   for (int k = 0; k < Kpad; k += 32) {
-    float va8[8], vb8[8];
-    load8(in0 + ra, k + kq, Lx, va, vec, va8);
-    load8(in0 + rb, k + kq, Lx, vb, vec, vb8);
+    float va8[8], vb8[8], ta8[8], tb8[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      va8[t] = na8[t];
+      vb8[t] = nb8[t];
+      if (RESIST) {
+        ta8[t] = nta8[t];
+        tb8[t] = ntb8[t];
+      }
+    }
+    if (k + 32 < Kpad) {
+      load8(in0 + ra, k + 32 + kq, Lx, va, vec, na8);
+      load8(in0 + rb, k + 32 + kq, Lx, vb, vec, nb8);
+      if (RESIST) {
+        load8(in1 + ra, k + 32 + kq, Lx, va, vec, nta8);
+        load8(in1 + rb, k + 32 + kq, Lx, vb, vec, ntb8);
+      }
+    }
     if (RESIST) {
-      float ta8[8], tb8[8];
-      load8(in1 + ra, k + kq, Lx, va, vec, ta8);
-      load8(in1 + rb, k + kq, Lx, vb, vec, tb8);
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
         const bool ina = va && k + kq + t < Lx, inb = vb && k + kq + t < Lx;
@@ -155,6 +177,12 @@ __global__ __launch_bounds__(256) void k_band_mfma(
   const int Wy = 2 * By + 1, nxp = Bx + 1;
   const int y0 = blk * kRowsBand;
   const int nrow = min(kRowsBand, Ly - y0);
+  // this block's column twiddles, staged once (|ny| <= By < 8 NCB)
+  for (int e = threadIdx.x; e < kRowsBand * 8 * NCB; e += 256) {
+    const int yy = e / (8 * NCB), c = e - yy * (8 * NCB);
+    sTw[yy][c] = (yy < nrow && c <= By) ? twy[(size_t)(y0 + yy) * ldtw + c] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
   float2* out = out_part + ((size_t)b * nblk + blk) * Wy * ldo;
   for (int o = threadIdx.x; o < Wy * nxp; o += 256) {
     const int nyi = o / nxp, nx = o - nyi * nxp;
@@ -162,7 +190,7 @@ __global__ __launch_bounds__(256) void k_band_mfma(
     const int any = ny < 0 ? -ny : ny;
     float sr = 0.f, si = 0.f;
     for (int yy = 0; yy < nrow; ++yy) {
-      float2 w = twy[(size_t)(y0 + yy) * ldtw + any];
+      float2 w = sTw[yy][any];
       if (ny < 0) w.y = -w.y;
       const float rr = sR[yy][2 * nx], ri = sR[yy][2 * nx + 1];
       sr += rr * w.x - ri * w.y;
@@ -628,13 +656,28 @@ __global__ __launch_bounds__(256) void k_synth_mfma(const float2* __restrict__ s
       bl[cb][ks] = *reinterpret_cast<const half8_t*>(sPl + c * kPStride + ks * 32 + kq);
     }
   float* img = image + (size_t)b * Lx * Ly;
+  // software pipeline over row blocks: the next block's TY operands load
+  // while this block's products and stores run
+  half8_t nh0, nh1, nl0, nl1;
+  {
+    const int y1 = rp * 16;
+    const _Float16* th = TYh + (size_t)(y1 + r16) * 64 + kq;
+    const _Float16* tl = TYl + (size_t)(y1 + r16) * 64 + kq;
+    nh0 = *reinterpret_cast<const half8_t*>(th);
+    nh1 = *reinterpret_cast<const half8_t*>(th + 32);
+    nl0 = *reinterpret_cast<const half8_t*>(tl);
+    nl1 = *reinterpret_cast<const half8_t*>(tl + 32);
+  }
   for (int yb = rp * 16; yb < Lypad; yb += 32) {
-    const _Float16* th = TYh + (size_t)(yb + r16) * 64 + kq;
-    const _Float16* tl = TYl + (size_t)(yb + r16) * 64 + kq;
-    const half8_t ah0 = *reinterpret_cast<const half8_t*>(th);
-    const half8_t ah1 = *reinterpret_cast<const half8_t*>(th + 32);
-    const half8_t al0 = *reinterpret_cast<const half8_t*>(tl);
-    const half8_t al1 = *reinterpret_cast<const half8_t*>(tl + 32);
+    const half8_t ah0 = nh0, ah1 = nh1, al0 = nl0, al1 = nl1;
+    if (yb + 32 < Lypad) {
+      const _Float16* th = TYh + (size_t)(yb + 32 + r16) * 64 + kq;
+      const _Float16* tl = TYl + (size_t)(yb + 32 + r16) * 64 + kq;
+      nh0 = *reinterpret_cast<const half8_t*>(th);
+      nh1 = *reinterpret_cast<const half8_t*>(th + 32);
+      nl0 = *reinterpret_cast<const half8_t*>(tl);
+      nl1 = *reinterpret_cast<const half8_t*>(tl + 32);
+    }
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) {
       float4_t d = float4_t{0.f, 0.f, 0.f, 0.f};
