@@ -57,15 +57,16 @@ struct SocsPlan {
 };
 
 // Half-precision operand tables of the fused SOCS path (socs_fused.hip):
-// W[n][x] (n = 2 nx: cos, 2 nx + 1: -sin; 32 rows, Kpad = round_up(Lx, 32)
-// columns), TY[y][j] (j = 2 gy: w cos, 2 gy + 1: -w sin; Lypad x 64) as
-// hi / lo fp16 pairs, and e^{+2 pi i j/64}.
+// the band kernel's row twiddles Wf (rows n = 2 nx: cos, 2 nx + 1: -sin, 32
+// rows, Kpad = round_up(Lx, 128) columns, zero past Lx) as hi / lo fp16 pairs
+// in MFMA fragment order, TY[y][j] (j = 2 gy: w cos, 2 gy + 1: -w sin;
+// Lypad x 64) as hi / lo pairs, and e^{+2 pi i j/64}.
 struct FusedTables {
   int Kpad = 0, Lypad = 0;
-  DevBuf Wh, Wl, TYh, TYl, tw64;
-  // adjoint: resist-gradient band rows (64 x Kpad, centre (Lx+1)/2) and the
-  // column twiddles e^{-2 pi i gy (y - (Ly+1)/2)/Ly} (Ly x 32)
-  DevBuf Wgh, Wgl, twyg;
+  DevBuf Wf, TYh, TYl, tw64;
+  // adjoint: resist-gradient band rows Wgf (64 rows, centre (Lx+1)/2, fragment
+  // order) and the column twiddles e^{-2 pi i gy (y - (Ly+1)/2)/Ly} (Ly x 32)
+  DevBuf Wgf, twyg;
 };
 
 }  // namespace klt

@@ -35,6 +35,7 @@ constexpr int kM = 64;        // coarse grid
 constexpr int kRowsBand = 128;  // mask rows per band block
 constexpr int kColsSynth = 128; // image columns per synthesis block
 constexpr int kPStride = 72;    // LDS row stride (halves) of the P operand
+constexpr int kBandChunk = 4;   // 32-column steps per LDS twiddle chunk
 
 __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -81,85 +82,164 @@ __device__ __forceinline__ void load8(const float* row, int x, int Lx, bool vali
 //   RESIST = true : G(x) = dL/dI = 2 (s - T) a s (1 - s), s = sigmoid(a (I - thr)),
 //                   from the image I and target T (the loss sum (s - T)^2 of
 //                   the block goes to lpart[b][blk]).
+// The row twiddles come fragment-ordered (band_fragments): for step s (32
+// columns), column block c and half h (hi / lo), the 64 lanes' 16-byte MFMA
+// operands are contiguous.  The block copies them to LDS kBandChunk steps at a
+// time, double-buffered, so the four waves share one copy and each step's
+// operands are conflict-free 1 KB LDS reads.
 // Epilogue: out[b][blk][ny + By][nx] = sum_{y in block} R[y][nx] tw(ny, y) for
 // |ny| <= By, 0 <= nx <= Bx; tw(ny, y) = twy[y][|ny|] (conjugated for ny < 0).
 // ---------------------------------------------------------------------------
-template <int NCB, bool RESIST>
-__global__ __launch_bounds__(256) void k_band_mfma(
+template <int NCB, bool RESIST, bool FULL>
+__global__ __launch_bounds__(256, (RESIST || !FULL) ? 2 : 4) void k_band_mfma(
     const float* __restrict__ in0, const float* __restrict__ in1, float a_sig, float thr,
-    int Lx, int Ly, int Kpad, const _Float16* __restrict__ Wh, const _Float16* __restrict__ Wl,
-    const float2* __restrict__ twy, int ldtw, int Bx, int By, int ldo,
-    float2* __restrict__ out_part, float* __restrict__ lpart) {
-  __shared__ float sR[kRowsBand][16 * NCB + 1];
-  __shared__ float2 sTw[kRowsBand][8 * NCB];
+    int Lx, int Ly, int Kpad, const float4* Wf, const float2* __restrict__ twy,
+    int ldtw, int Bx, int By, int ldo, float2* __restrict__ out_part, float* __restrict__ lpart) {
+  constexpr int C = kBandChunk;
+  constexpr int STEP_F4 = NCB * 2 * 64;    // 16-byte fragments per step
+  constexpr int CHUNK_F4 = C * STEP_F4;    // per chunk
+  constexpr int PER_T = CHUNK_F4 / 256;    // per thread per chunk
+  constexpr int PS = PER_T / C;            // per thread per step
+  constexpr int R_BYTES = kRowsBand * (16 * NCB + 1) * 4;
+  constexpr int EPI_BYTES = R_BYTES + kRowsBand * 8 * NCB * 8;
+  constexpr int W_BYTES = 2 * CHUNK_F4 * 16;
+  // the twiddle chunks (main loop) and the epilogue's R / column twiddles
+  // share the LDS
+  __shared__ __attribute__((aligned(16))) char smem[EPI_BYTES > W_BYTES ? EPI_BYTES : W_BYTES];
   __shared__ float red[4];
+  auto sR = reinterpret_cast<float(*)[16 * NCB + 1]>(smem);
+  auto sTw = reinterpret_cast<float2(*)[8 * NCB]>(smem + R_BYTES);
+  float4* sW = reinterpret_cast<float4*>(smem);
+  const int tid = threadIdx.x;
   const int blk = blockIdx.x, b = blockIdx.y, nblk = gridDim.x;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = tid >> 6, lane = tid & 63;
   const int r16 = lane & 15, kq = (lane >> 4) * 8;
   const int ybase = blk * kRowsBand + wave * 32;
   const size_t ib = (size_t)b * Lx * Ly;
   const int ya = ybase + r16, yb = ybase + 16 + r16;
   const bool va = ya < Ly, vb = yb < Ly;
+  // rows past the image read row 0: their products land in rows of R that
+  // the epilogue never reads
   const size_t ra = ib + (size_t)(va ? ya : 0) * Lx, rb = ib + (size_t)(vb ? yb : 0) * Lx;
-  const bool vec = (Lx & 3) == 0;
   float4_t acc[2][NCB];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int c = 0; c < NCB; ++c) acc[a][c] = float4_t{0.f, 0.f, 0.f, 0.f};
   float lsum = 0.f;
-  // software pipeline: the loads of step k + 32 are issued before the
-  // conversions and matrix products of step k
-  float na8[8], nb8[8], nta8[8], ntb8[8];
-  load8(in0 + ra, kq, Lx, va, vec, na8);
-  load8(in0 + rb, kq, Lx, vb, vec, nb8);
-  if (RESIST) {
-    load8(in1 + ra, kq, Lx, va, vec, nta8);
-    load8(in1 + rb, kq, Lx, vb, vec, ntb8);
-  }
-  for (int k = 0; k < Kpad; k += 32) {
-    float va8[8], vb8[8], ta8[8], tb8[8];
+  // Rows stream DEPTH steps ahead of their use.  Each step converts its rows
+  // to split halves first, which frees their registers for the rows of step
+  // k + 32 DEPTH, then issues those rows, then runs the products.
+  constexpr int DEPTH = RESIST ? 1 : 2;
+  float pa[DEPTH][8], pb[DEPTH][8], pta[DEPTH][8], ptb[DEPTH][8];
+  auto fetch_rows = [&](int k, float* a8, float* b8, float* ta8, float* tb8) {
+    if constexpr (FULL) {
+      // whole 16-byte-aligned 32-column tiles: branch-free vector loads; the
+      // steps past the row (k >= Lx) re-read its last tile, against zero
+      // twiddle columns
+      k = min(k, Lx - 32);
+      const float4 a0 = *reinterpret_cast<const float4*>(in0 + ra + k + kq);
+      const float4 a1 = *reinterpret_cast<const float4*>(in0 + ra + k + kq + 4);
+      const float4 b0 = *reinterpret_cast<const float4*>(in0 + rb + k + kq);
+      const float4 b1 = *reinterpret_cast<const float4*>(in0 + rb + k + kq + 4);
+      a8[0] = a0.x; a8[1] = a0.y; a8[2] = a0.z; a8[3] = a0.w;
+      a8[4] = a1.x; a8[5] = a1.y; a8[6] = a1.z; a8[7] = a1.w;
+      b8[0] = b0.x; b8[1] = b0.y; b8[2] = b0.z; b8[3] = b0.w;
+      b8[4] = b1.x; b8[5] = b1.y; b8[6] = b1.z; b8[7] = b1.w;
+      if (RESIST) {
+        const float4 c0 = *reinterpret_cast<const float4*>(in1 + ra + k + kq);
+        const float4 c1 = *reinterpret_cast<const float4*>(in1 + ra + k + kq + 4);
+        const float4 d0 = *reinterpret_cast<const float4*>(in1 + rb + k + kq);
+        const float4 d1 = *reinterpret_cast<const float4*>(in1 + rb + k + kq + 4);
+        ta8[0] = c0.x; ta8[1] = c0.y; ta8[2] = c0.z; ta8[3] = c0.w;
+        ta8[4] = c1.x; ta8[5] = c1.y; ta8[6] = c1.z; ta8[7] = c1.w;
+        tb8[0] = d0.x; tb8[1] = d0.y; tb8[2] = d0.z; tb8[3] = d0.w;
+        tb8[4] = d1.x; tb8[5] = d1.y; tb8[6] = d1.z; tb8[7] = d1.w;
+      }
+    } else {
+      load8(in0 + ra, k + kq, Lx, true, false, a8);
+      load8(in0 + rb, k + kq, Lx, true, false, b8);
+      if (RESIST) {
+        load8(in1 + ra, k + kq, Lx, true, false, ta8);
+        load8(in1 + rb, k + kq, Lx, true, false, tb8);
+      }
+    }
+  };
+  const int nchunk = Kpad / (32 * C);
+  // Prologue: the first twiddle chunk straight to LDS, then the first DEPTH
+  // row steps.
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      va8[t] = na8[t];
-      vb8[t] = nb8[t];
-      if (RESIST) {
-        ta8[t] = nta8[t];
-        tb8[t] = ntb8[t];
+  for (int i = 0; i < PER_T; ++i) sW[i * 256 + tid] = Wf[i * 256 + tid];
+#pragma unroll
+  for (int d = 0; d < DEPTH; ++d) fetch_rows(32 * d, pa[d], pb[d], pta[d], ptb[d]);
+  __syncthreads();
+  // The next chunk's copy is spread over the steps: step s issues its piece s
+  // (PS fragments a thread) before the step's rows and stores piece s - 1,
+  // loaded a step earlier, so no wait for a piece drains the row pipeline.
+  // (Wf is not declared restrict: as a provably invariant load the compiler
+  // would rematerialize it next to its store.)
+  for (int ch = 0; ch < nchunk; ++ch) {
+    // the last chunk reloads itself into the idle buffer
+    const float4* wnext = Wf + (size_t)min(ch + 1, nchunk - 1) * CHUNK_F4;
+    const float4* wbuf = sW + (ch & 1) * CHUNK_F4;
+    float4* wdst = sW + ((ch + 1) & 1) * CHUNK_F4;
+    // the piece in flight (named registers: an array here may be moved to
+    // LDS or scratch by the compiler)
+    static_assert(PS == 1 || PS == 2, "one or two fragments a step");
+    float4 stp0, stp1;
+#pragma unroll
+    for (int s = 0; s < C; ++s) {
+      const int k = (ch * C + s) * 32, sl = DEPTH == 2 ? (s & 1) : 0;
+      half8_t wh[NCB], wl[NCB];
+#pragma unroll
+      for (int c = 0; c < NCB; ++c) {
+        wh[c] = __builtin_bit_cast(half8_t, wbuf[(s * NCB + c) * 128 + lane]);
+        wl[c] = __builtin_bit_cast(half8_t, wbuf[(s * NCB + c) * 128 + 64 + lane]);
       }
-    }
-    if (k + 32 < Kpad) {
-      load8(in0 + ra, k + 32 + kq, Lx, va, vec, na8);
-      load8(in0 + rb, k + 32 + kq, Lx, vb, vec, nb8);
-      if (RESIST) {
-        load8(in1 + ra, k + 32 + kq, Lx, va, vec, nta8);
-        load8(in1 + rb, k + 32 + kq, Lx, vb, vec, ntb8);
-      }
-    }
-    if (RESIST) {
+      float va8[8], vb8[8];
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
-        const bool ina = va && k + kq + t < Lx, inb = vb && k + kq + t < Lx;
-        const float sa = 1.f / (1.f + __expf(-a_sig * (va8[t] - thr)));
-        const float sb = 1.f / (1.f + __expf(-a_sig * (vb8[t] - thr)));
-        const float da = sa - ta8[t], db = sb - tb8[t];
-        lsum += (ina ? da * da : 0.f) + (inb ? db * db : 0.f);
-        va8[t] = ina ? 2.f * da * a_sig * sa * (1.f - sa) : 0.f;
-        vb8[t] = inb ? 2.f * db * a_sig * sb * (1.f - sb) : 0.f;
+        va8[t] = pa[sl][t];
+        vb8[t] = pb[sl][t];
+      }
+      if (RESIST) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const bool ina = va && k + kq + t < Lx, inb = vb && k + kq + t < Lx;
+          const float sa = 1.f / (1.f + __expf(-a_sig * (va8[t] - thr)));
+          const float sb = 1.f / (1.f + __expf(-a_sig * (vb8[t] - thr)));
+          const float da = sa - pta[sl][t], db = sb - ptb[sl][t];
+          lsum += (ina ? da * da : 0.f) + (inb ? db * db : 0.f);
+          va8[t] = ina ? 2.f * da * a_sig * sa * (1.f - sa) : 0.f;
+          vb8[t] = inb ? 2.f * db * a_sig * sb * (1.f - sb) : 0.f;
+        }
+      }
+      half8_t ah, al, bh, bl;
+      split8(va8, ah, al);
+      split8(vb8, bh, bl);
+      if (s > 0) {
+        wdst[((s - 1) * PS) * 256 + tid] = stp0;
+        if (PS == 2) wdst[((s - 1) * PS + 1) * 256 + tid] = stp1;
+      }
+      stp0 = wnext[(s * PS) * 256 + tid];
+      if (PS == 2) stp1 = wnext[(s * PS + 1) * 256 + tid];
+      __builtin_amdgcn_sched_barrier(0);
+      fetch_rows(min(k + 32 * DEPTH, Kpad - 32), pa[sl], pb[sl], pta[sl], ptb[sl]);
+      // keep the prefetch here: the scheduler would otherwise sink it to its
+      // uses and expose the full memory latency every step
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int c = 0; c < NCB; ++c) {
+        acc[0][c] = mfma3(ah, al, wh[c], wl[c], acc[0][c]);
+        acc[1][c] = mfma3(bh, bl, wh[c], wl[c], acc[1][c]);
       }
     }
-    half8_t ah, al, bh, bl;
-    split8(va8, ah, al);
-    split8(vb8, bh, bl);
-#pragma unroll
-    for (int c = 0; c < NCB; ++c) {
-      const size_t wo = (size_t)(c * 16 + r16) * Kpad + k + kq;
-      const half8_t ch = *reinterpret_cast<const half8_t*>(Wh + wo);
-      const half8_t cl = *reinterpret_cast<const half8_t*>(Wl + wo);
-      acc[0][c] = mfma3(ah, al, ch, cl, acc[0][c]);
-      acc[1][c] = mfma3(bh, bl, ch, cl, acc[1][c]);
-    }
+    wdst[((C - 1) * PS) * 256 + tid] = stp0;
+    if (PS == 2) wdst[((C - 1) * PS + 1) * 256 + tid] = stp1;
+    __syncthreads();
   }
+  // (the loop's last barrier also ends every read of the twiddle chunks,
+  // whose LDS R and the column twiddles now reuse)
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -172,19 +252,18 @@ __global__ __launch_bounds__(256) void k_band_mfma(
     for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
     if (lane == 0) red[wave] = lsum;
   }
-  __syncthreads();
-  if (RESIST && threadIdx.x == 0) lpart[(size_t)b * nblk + blk] = red[0] + red[1] + red[2] + red[3];
   const int Wy = 2 * By + 1, nxp = Bx + 1;
   const int y0 = blk * kRowsBand;
   const int nrow = min(kRowsBand, Ly - y0);
   // this block's column twiddles, staged once (|ny| <= By < 8 NCB)
-  for (int e = threadIdx.x; e < kRowsBand * 8 * NCB; e += 256) {
+  for (int e = tid; e < kRowsBand * 8 * NCB; e += 256) {
     const int yy = e / (8 * NCB), c = e - yy * (8 * NCB);
     sTw[yy][c] = (yy < nrow && c <= By) ? twy[(size_t)(y0 + yy) * ldtw + c] : make_float2(0.f, 0.f);
   }
   __syncthreads();
+  if (RESIST && tid == 0) lpart[(size_t)b * nblk + blk] = red[0] + red[1] + red[2] + red[3];
   float2* out = out_part + ((size_t)b * nblk + blk) * Wy * ldo;
-  for (int o = threadIdx.x; o < Wy * nxp; o += 256) {
+  for (int o = tid; o < Wy * nxp; o += 256) {
     const int nyi = o / nxp, nx = o - nyi * nxp;
     const int ny = nyi - By;
     const int any = ny < 0 ? -ny : ny;
@@ -589,9 +668,12 @@ __global__ __launch_bounds__(256) void k_synth_mfma(const float2* __restrict__ s
                                                     const _Float16* __restrict__ TYh,
                                                     const _Float16* __restrict__ TYl,
                                                     float* __restrict__ image) {
-  __shared__ float2 sJ[32 * 61];
-  __shared__ __attribute__((aligned(16))) _Float16 sPh[kColsSynth * kPStride];
-  __shared__ __attribute__((aligned(16))) _Float16 sPl[kColsSynth * kPStride];
+  // the spectrum (prologue) and the split P operand (main loop) share the LDS
+  constexpr int J_BYTES = 32 * 61 * 8, P_BYTES = 2 * kColsSynth * kPStride * 2;
+  __shared__ __attribute__((aligned(16))) char smem[J_BYTES > P_BYTES ? J_BYTES : P_BYTES];
+  float2* sJ = reinterpret_cast<float2*>(smem);
+  _Float16* sPh = reinterpret_cast<_Float16*>(smem);
+  _Float16* sPl = sPh + kColsSynth * kPStride;
   const int b = blockIdx.y, x0 = blockIdx.x * kColsSynth, tid = threadIdx.x;
   const int Gx = 4 * Nx + 1, Hx = 2 * Nx + 1;
   __shared__ float red[4];
@@ -618,18 +700,26 @@ __global__ __launch_bounds__(256) void k_synth_mfma(const float2* __restrict__ s
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[q] = make_float2(0.f, 0.f);
     if (x < Lx) {
+      // gx = +g and -g share the twiddle w (conjugated for -g):
+      // J[g] w + J[-g] conj(w); the next twiddle loads a step ahead
       const float2* tx = twx + (size_t)x * Hx;
-      for (int gx = -2 * Nx; gx <= 2 * Nx; ++gx) {
-        float2 w = tx[gx < 0 ? -gx : gx];
-        if (gx < 0) w.y = -w.y;
+      const float2* jr = sJ + half * 16 * Gx + 2 * Nx;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[q] = jr[q * Gx];
+      float2 wn = tx[Hx > 1 ? 1 : 0];
+      for (int g = 1; g <= 2 * Nx; ++g) {
+        const float2 w = wn;
+        wn = tx[g + 1 <= 2 * Nx ? g + 1 : g];
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
-          const float2 j = sJ[(half * 16 + q) * Gx + gx + 2 * Nx];
-          acc[q].x += j.x * w.x - j.y * w.y;
-          acc[q].y += j.x * w.y + j.y * w.x;
+          const float2 jp = jr[q * Gx + g], jm = jr[q * Gx - g];
+          acc[q].x += (jp.x + jm.x) * w.x - (jp.y - jm.y) * w.y;
+          acc[q].y += (jp.x - jm.x) * w.y + (jp.y + jm.y) * w.x;
         }
       }
     }
+    // every read of the spectrum ends before P overwrites it
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int j = 2 * (half * 16 + q);
@@ -678,6 +768,9 @@ __global__ __launch_bounds__(256) void k_synth_mfma(const float2* __restrict__ s
       nl0 = *reinterpret_cast<const half8_t*>(tl);
       nl1 = *reinterpret_cast<const half8_t*>(tl + 32);
     }
+    // lane l holds rows y = 4 (l / 16) + r of column x = l % 16: each store
+    // writes 64 contiguous bytes in each of four rows (measured faster here
+    // than the transposed orientation's 16-byte writes to sixteen rows)
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) {
       float4_t d = float4_t{0.f, 0.f, 0.f, 0.f};
@@ -705,6 +798,43 @@ void split_half(double v, _Float16* hi, _Float16* lo) {
 
 namespace klt {
 
+// Band launch: the branch-free vector-load variant needs 32-column tiles and
+// 16-byte-aligned inputs.
+template <int NCB, bool RESIST>
+static void launch_band(klt_ctx* ctx, const char* name, dim3 grid, const float* in0,
+                        const float* in1, float a_sig, float thr, int Lx, int Ly, int Kpad,
+                        const DevBuf& Wf, const float2* twy, int ldtw, int Bx, int By, int ldo,
+                        float2* out, float* lpart) {
+  const bool full = Lx % 32 == 0 && (reinterpret_cast<uintptr_t>(in0) & 15) == 0 &&
+                    (!RESIST || (reinterpret_cast<uintptr_t>(in1) & 15) == 0);
+  const float4* w = Wf.as<float4>();
+  if (full)
+    KLT_LAUNCH(ctx, name, (k_band_mfma<NCB, RESIST, true>), grid, dim3(256), 0, in0, in1, a_sig,
+               thr, Lx, Ly, Kpad, w, twy, ldtw, Bx, By, ldo, out, lpart);
+  else
+    KLT_LAUNCH(ctx, name, (k_band_mfma<NCB, RESIST, false>), grid, dim3(256), 0, in0, in1, a_sig,
+               thr, Lx, Ly, Kpad, w, twy, ldtw, Bx, By, ldo, out, lpart);
+}
+
+// Row twiddles w(n, x) (rows n < 16 ncb, columns x < Kpad) in the band
+// kernel's fragment order: for step s, column block c and half h (hi, lo),
+// lane l holds w(16 c + l % 16, 32 s + 8 (l / 16) + t), t < 8.
+template <typename F>
+static std::vector<_Float16> band_fragments(int ncb, int Kpad, F w) {
+  std::vector<_Float16> f((size_t)Kpad * 16 * ncb * 2);
+  for (int st = 0; st < Kpad / 32; ++st)
+    for (int c = 0; c < ncb; ++c)
+      for (int l = 0; l < 64; ++l)
+        for (int t = 0; t < 8; ++t) {
+          _Float16 h, lo;
+          split_half(w(16 * c + l % 16, 32 * st + 8 * (l / 16) + t), &h, &lo);
+          const size_t o = (((size_t)st * ncb + c) * 2 * 64 + l) * 8 + t;
+          f[o] = h;
+          f[o + 64 * 8] = lo;
+        }
+  return f;
+}
+
 static int get_fused_tables(klt_ctx* ctx, SocsPlan* p, FusedTables** out) {
   auto key = std::make_tuple(p->Lx, p->Ly, p->Nx, p->Ny);
   auto it = ctx->fused_tables.find(key);
@@ -714,28 +844,25 @@ static int get_fused_tables(klt_ctx* ctx, SocsPlan* p, FusedTables** out) {
   }
   auto t = std::make_unique<FusedTables>();
   const int Lx = p->Lx, Ly = p->Ly, Nx = p->Nx, Ny = p->Ny;
-  t->Kpad = round_up(Lx, 32);
+  t->Kpad = round_up(Lx, 32 * kBandChunk);
   t->Lypad = round_up(Ly, 32);
   const int cix = Lx / 2, coy = (Ly + 1) / 2;
-  std::vector<_Float16> h((size_t)32 * t->Kpad), l((size_t)32 * t->Kpad);
-  for (int n = 0; n < 32; ++n)
-    for (int x = 0; x < t->Kpad; ++x) {
-      double v = 0.0;
-      const int nx = n / 2;
-      if (nx <= Nx && x < Lx) {
-        long long r = ((long long)nx * (x - cix)) % Lx;
-        if (r < 0) r += Lx;
-        const double a = 2.0 * M_PI * (double)r / (double)Lx;
-        v = (n & 1) ? -std::sin(a) : std::cos(a);
-      }
-      split_half(v, &h[(size_t)n * t->Kpad + x], &l[(size_t)n * t->Kpad + x]);
-    }
-  KLT_TRY(t->Wh.ensure(h.size() * sizeof(_Float16)));
-  KLT_TRY(t->Wl.ensure(l.size() * sizeof(_Float16)));
-  KLT_HIP(hipMemcpy(t->Wh.ptr, h.data(), h.size() * sizeof(_Float16), hipMemcpyHostToDevice));
-  KLT_HIP(hipMemcpy(t->Wl.ptr, l.data(), l.size() * sizeof(_Float16), hipMemcpyHostToDevice));
-  h.assign((size_t)t->Lypad * 64, (_Float16)0.f);
-  l.assign((size_t)t->Lypad * 64, (_Float16)0.f);
+  // row twiddles of the mask band: n = 2 nx (cos), 2 nx + 1 (-sin), nx <= Nx,
+  // centre Lx/2, zero past Lx
+  auto row_twiddle = [Lx](int n, int x, int nmax, int centre) {
+    const int f = n / 2;
+    if (f > nmax || x >= Lx) return 0.0;
+    long long r = ((long long)f * (x - centre)) % Lx;
+    if (r < 0) r += Lx;
+    const double a = 2.0 * M_PI * (double)r / (double)Lx;
+    return (n & 1) ? -std::sin(a) : std::cos(a);
+  };
+  {
+    const auto f = band_fragments(2, t->Kpad, [&](int n, int x) { return row_twiddle(n, x, Nx, cix); });
+    KLT_TRY(t->Wf.ensure(f.size() * sizeof(_Float16)));
+    KLT_HIP(hipMemcpy(t->Wf.ptr, f.data(), f.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+  }
+  std::vector<_Float16> h((size_t)t->Lypad * 64, (_Float16)0.f), l((size_t)t->Lypad * 64, (_Float16)0.f);
   for (int y = 0; y < Ly; ++y)
     for (int gy = 0; gy <= 2 * Ny; ++gy) {
       long long r = ((long long)gy * (y - coy)) % Ly;
@@ -754,23 +881,10 @@ static int get_fused_tables(klt_ctx* ctx, SocsPlan* p, FusedTables** out) {
   // centre (Lx+1)/2; column twiddles e^{-2 pi i gy (y - (Ly+1)/2)/Ly}, gy <= 2Ny
   {
     const int cox = (Lx + 1) / 2;
-    std::vector<_Float16> gh((size_t)64 * t->Kpad), gl((size_t)64 * t->Kpad);
-    for (int n = 0; n < 64; ++n)
-      for (int x = 0; x < t->Kpad; ++x) {
-        double v = 0.0;
-        const int gx = n / 2;
-        if (gx <= 2 * Nx && x < Lx) {
-          long long r = ((long long)gx * (x - cox)) % Lx;
-          if (r < 0) r += Lx;
-          const double a = 2.0 * M_PI * (double)r / (double)Lx;
-          v = (n & 1) ? -std::sin(a) : std::cos(a);
-        }
-        split_half(v, &gh[(size_t)n * t->Kpad + x], &gl[(size_t)n * t->Kpad + x]);
-      }
-    KLT_TRY(t->Wgh.ensure(gh.size() * sizeof(_Float16)));
-    KLT_TRY(t->Wgl.ensure(gl.size() * sizeof(_Float16)));
-    KLT_HIP(hipMemcpy(t->Wgh.ptr, gh.data(), gh.size() * sizeof(_Float16), hipMemcpyHostToDevice));
-    KLT_HIP(hipMemcpy(t->Wgl.ptr, gl.data(), gl.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+    const auto f =
+        band_fragments(4, t->Kpad, [&](int n, int x) { return row_twiddle(n, x, 2 * Nx, cox); });
+    KLT_TRY(t->Wgf.ensure(f.size() * sizeof(_Float16)));
+    KLT_HIP(hipMemcpy(t->Wgf.ptr, f.data(), f.size() * sizeof(_Float16), hipMemcpyHostToDevice));
     std::vector<float2> ty((size_t)Ly * 32, make_float2(0.f, 0.f));
     for (int y = 0; y < Ly; ++y)
       for (int gy = 0; gy <= 2 * Ny; ++gy) {
@@ -812,10 +926,10 @@ int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, 
     const int nb = std::min(SB, B - b0);
     const float* m = masks + (size_t)b0 * Lx * Ly;
     float* out = images + (size_t)b0 * Lx * Ly;
-    KLT_LAUNCH(ctx, "k_band_mfma", (k_band_mfma<2, false>), dim3(nblk, nb), dim3(256), 0, m,
-               nullptr, 0.f, 0.f, Lx, Ly, ft->Kpad, ft->Wh.as<_Float16>(),
-               ft->Wl.as<_Float16>(), p->twy_in.as<float2>(), p->NP, Nx, Ny, 16,
-               ctx->qpart.as<float2>(), nullptr);
+    launch_band<2, false>(ctx, "k_band_mfma", dim3(nblk, nb), m, nullptr, 0.f, 0.f, Lx, Ly,
+                          ft->Kpad, ft->Wf,
+                          p->twy_in.as<float2>(), p->NP, Nx, Ny, 16, ctx->qpart.as<float2>(),
+                          nullptr);
     KLT_LAUNCH(ctx, "k_socs_mid", k_socs_mid, dim3(nb), dim3(256), 0, ctx->qpart.as<float2>(),
                nblk, Nx, Ny, inv_area, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk,
                ft->tw64.as<float2>(), ctx->spec.as<float2>());
@@ -835,9 +949,9 @@ int band_mask_partials(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const float
   FusedTables* ft = nullptr;
   KLT_TRY(get_fused_tables(ctx, p, &ft));
   const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
-  KLT_LAUNCH(ctx, "k_band_mfma", (k_band_mfma<2, false>), dim3(nblk, B), dim3(256), 0, masks,
-             nullptr, 0.f, 0.f, Lx, Ly, ft->Kpad, ft->Wh.as<_Float16>(), ft->Wl.as<_Float16>(),
-             p->twy_in.as<float2>(), p->NP, Nx, Ny, 16, out, nullptr);
+  launch_band<2, false>(ctx, "k_band_mfma", dim3(nblk, B), masks, nullptr, 0.f, 0.f, Lx, Ly,
+                        ft->Kpad, ft->Wf,
+                        p->twy_in.as<float2>(), p->NP, Nx, Ny, 16, out, nullptr);
   KLT_HIP(hipGetLastError());
   return KLT_OK;
 }
@@ -850,9 +964,9 @@ int band_resist_partials(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const flo
   FusedTables* ft = nullptr;
   KLT_TRY(get_fused_tables(ctx, p, &ft));
   const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
-  KLT_LAUNCH(ctx, "k_band_resist", (k_band_mfma<4, true>), dim3(nblk, B), dim3(256), 0, images,
-             targets, a_sig, thr, Lx, Ly, ft->Kpad, ft->Wgh.as<_Float16>(),
-             ft->Wgl.as<_Float16>(), ft->twyg.as<float2>(), 32, 2 * Nx, 2 * Ny, 32, out, lpart);
+  launch_band<4, true>(ctx, "k_band_resist", dim3(nblk, B), images, targets, a_sig, thr, Lx, Ly,
+                       ft->Kpad, ft->Wgf,
+                       ft->twyg.as<float2>(), 32, 2 * Nx, 2 * Ny, 32, out, lpart);
   KLT_HIP(hipGetLastError());
   return KLT_OK;
 }
@@ -882,10 +996,10 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
     const int nb = std::min(SB, B - b0);
     const size_t off = (size_t)b0 * Lx * Ly;
     float* I = images ? images + off : ctx->bwd_img.as<float>();
-    KLT_LAUNCH(ctx, "k_band_mfma", (k_band_mfma<2, false>), dim3(nblk, nb), dim3(256), 0,
-               masks + off, nullptr, 0.f, 0.f, Lx, Ly, ft->Kpad, ft->Wh.as<_Float16>(),
-               ft->Wl.as<_Float16>(), p->twy_in.as<float2>(), p->NP, Nx, Ny, 16,
-               ctx->qpart.as<float2>(), nullptr);
+    launch_band<2, false>(ctx, "k_band_mfma", dim3(nblk, nb), masks + off, nullptr, 0.f, 0.f,
+                          Lx, Ly, ft->Kpad, ft->Wf,
+                          p->twy_in.as<float2>(), p->NP, Nx, Ny, 16, ctx->qpart.as<float2>(),
+                          nullptr);
     KLT_LAUNCH(ctx, "k_socs_mid", k_socs_mid, dim3(nb), dim3(256), 0, ctx->qpart.as<float2>(),
                nblk, Nx, Ny, inv_area, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk,
                ft->tw64.as<float2>(), ctx->spec.as<float2>());
@@ -893,10 +1007,10 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
                dim3((Lx + kColsSynth - 1) / kColsSynth, nb), dim3(256), 0,
                ctx->spec.as<float2>(), Nx, Lx, Ly, ft->Lypad, p->twx_out.as<float2>(),
                ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), I);
-    KLT_LAUNCH(ctx, "k_band_resist", (k_band_mfma<4, true>), dim3(nblk, nb), dim3(256), 0, I,
-               targets + off, a_sig, thr, Lx, Ly, ft->Kpad, ft->Wgh.as<_Float16>(),
-               ft->Wgl.as<_Float16>(), ft->twyg.as<float2>(), 32, 2 * Nx, 2 * Ny, 32,
-               ctx->gpart.as<float2>(), ctx->lpart.as<float>());
+    launch_band<4, true>(ctx, "k_band_resist", dim3(nblk, nb), I, targets + off, a_sig, thr, Lx,
+                         Ly, ft->Kpad, ft->Wgf,
+                         ft->twyg.as<float2>(), 32, 2 * Nx, 2 * Ny, 32, ctx->gpart.as<float2>(),
+                         ctx->lpart.as<float>());
     KLT_LAUNCH(ctx, "k_socs_mid_bwd", k_socs_mid_bwd, dim3(nb), dim3(256), 0,
                ctx->qpart.as<float2>(), ctx->gpart.as<float2>(), ctx->lpart.as<float>(), nblk,
                Nx, Ny, inv_area, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk,
