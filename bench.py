@@ -13,6 +13,9 @@ Also reported (same JSON line):
                and klt_abbe_source_grad (source gradient over the 51^2 grid)
   tcc_eig      TCC T = 1225 (blurred dipole, period 1280, 5 defocus values),
                split-FP16 MFMA Gram vs FP64 Gram, top-32 FP64 eigensolve
+  batch_uneven 64 masks of sides {256..1024} x 3 kernel sets (defocus -25/0/25
+               nm) = 192 images per klt_socs_forward_batch graph launch, with
+               p50/p99 per-image latency and the same work as grouped calls
   kernels      per-kernel HIP-event times of one step; `roofline` is the
                dominant kernel's algorithmic HBM bytes over its duration
   cpu_baseline a numpy/pocketfft port of the reference's FFTW path
@@ -87,6 +90,11 @@ class Timer:
         return e0.elapsed_time(e1) / reps  # ms
 
 
+def rel_l2(got, ref):
+    ref = np.asarray(ref, np.float64)
+    return float(np.linalg.norm(np.asarray(got, np.float64) - ref) / np.linalg.norm(ref))
+
+
 def annular_source():
     return klt.source_create(W.SRC_SIZE, "Annular", W.SIGMA_IN, W.SIGMA_OUT)
 
@@ -101,6 +109,63 @@ def gpu_kernels(ctx):
                         nmed=NMED)
     kr, ki, sc = ctx.calc_kernels(tcc, NK)
     return src, kr, ki, sc
+
+
+def bench_batch_uneven(ctx, timer, torch, src):
+    """64 seeded Manhattan masks with sides from {256, 384, 512, 768, 1024}, each
+    under 3 kernel sets (annular source, defocus -25 / 0 / +25 nm, K = 24,
+    29 x 29): 192 images per klt_socs_forward_batch call (one graph launch)."""
+    rng = np.random.default_rng(3)
+    sides = [int(v) for v in rng.choice([256, 384, 512, 768, 1024], 64)]
+    masks = [torch.from_numpy(W.manhattan(n, 2000 + i)).cuda() for i, n in enumerate(sides)]
+    sig = min(klt.outer_sigma(src), 1.0)
+    banks, defocus = [], [-25, 0, 25]
+    for df_nm in defocus:
+        tcc = ctx.tcc_build(src, sig, W.NA, df_nm * W.NA ** 2 / W.LAMBDA, W.PERIOD_29,
+                            W.PERIOD_29, 14, 14, precision=0, nmed=NMED)
+        kr, ki, sc = ctx.calc_kernels(tcc, NK)
+        banks.append(ctx.kernels(kr, ki, sc, KSIZE, KSIZE))
+    images = [torch.empty_like(m) for m in masks for _ in banks]
+    for _ in range(3):  # the first call captures the graph
+        ctx.socs_forward_batch(banks, masks, images)
+    reps = 20
+    ms = timer(lambda: ctx.socs_forward_batch(banks, masks, images), reps)
+    lat = ctx.batch_latency()
+    # the same images as per-size, per-set klt_socs_forward calls
+    groups = {}
+    for i, n in enumerate(sides):
+        groups.setdefault(n, []).append(i)
+    stacked = {n: torch.stack([masks[i] for i in idx]) for n, idx in groups.items()}
+    outs = {n: torch.empty_like(t) for n, t in stacked.items()}
+
+    def grouped():
+        for b in banks:
+            for n, t in stacked.items():
+                ctx.socs_forward(b, t, outs[n])
+
+    grouped()
+    gms = timer(grouped, 5)
+    err = 0.0
+    for i in groups[sides[0]][:2]:
+        for s, b in enumerate(banks):
+            one = ctx.socs_forward(b, masks[i].clone())
+            ctx.sync()
+            err = max(err, rel_l2(images[i * 3 + s].cpu().numpy(), one.cpu().numpy()))
+    nbytes = sum(4 * n * n for n in sides) * (1 + len(banks))
+    for b in banks:
+        b.close()
+    return {"masks": len(sides), "kernel_sets": len(banks), "images": len(images),
+            "defocus_nm": defocus,
+            "sides": {str(n): len(groups.get(n, [])) for n in (256, 384, 512, 768, 1024)},
+            "ms_per_batch": ms, "images_per_s": len(images) / (ms / 1e3),
+            "p50_image_latency_us": float(np.percentile(lat, 50)),
+            "p99_image_latency_us": float(np.percentile(lat, 99)),
+            "algorithmic_hbm_bytes": nbytes, "hbm_gbs": nbytes / (ms / 1e3) / 1e9,
+            "ms_per_batch_grouped_calls": gms,
+            "max_rel_l2_vs_single_calls": err,
+            "note": "one mask read per mask (its band is shared by the 3 sets), one image "
+                    "write per image; latency = device wall clock from the batch start to "
+                    "the last strip of each image"}
 
 
 def blurred_dipole():
@@ -294,6 +359,11 @@ def main():
         log(f"tcc T={tce['T']}: split16 {tce['split16_tflops_algorithmic']:.1f} TF/s "
             f"algorithmic, eig top-32 {tce['rows'][2]['eig_top32_ms']:.1f} ms")
         extras["tcc_eig"] = tce
+        bu = bench_batch_uneven(ctx, timer, torch, src)
+        log(f"uneven batch: {bu['images_per_s']:.0f} images/s ({bu['ms_per_batch']:.3f} ms per "
+            f"192 images; grouped calls {bu['ms_per_batch_grouped_calls']:.3f} ms), p50 "
+            f"{bu['p50_image_latency_us']:.0f} us, p99 {bu['p99_image_latency_us']:.0f} us")
+        extras["batch_uneven"] = bu
     if args.copy_calibration:
         a = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev)
         bcopy = torch.empty_like(a)

@@ -148,6 +148,25 @@ int klt_socs_image(klt_ctx* ctx, const klt_kernels* kb, const float* mask, int L
 int klt_socs_forward(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
                      int nmask, int Lx, int Ly, float* images, unsigned flags);
 
+/* Masks of different sizes under several kernel sets in one call (no
+ * reference counterpart: klitho_socs.cpp:338-343 images one mask per run).
+ * masks[m] (Ly[m] x Lx[m]) under set s gives images[m * nbank + s].  Every
+ * set must have the same kernel size, half-size <= 15.  Each mask's band is
+ * computed once and shared by its sets.  The call's launches are captured in
+ * a hipGraph on first use and replayed when the same arguments (pointers,
+ * sizes, sets) come again, so a caller that refills the same buffers pays one
+ * graph launch per batch; a stream the caller is itself capturing receives
+ * the launches directly.  Device pointers only (the pointer arrays are host
+ * memory). */
+int klt_socs_forward_batch(klt_ctx* ctx, int nmask, const float* const* masks, const int* Lx,
+                           const int* Ly, int nbank, const klt_kernels* const* banks,
+                           float* const* images, unsigned flags);
+/* Completion time of each image of the context's last klt_socs_forward_batch
+ * call, in microseconds from the start of that batch on the device (device
+ * wall clock).  Waits for the batch; writes min(cap, n) values and returns the
+ * image count n. */
+int klt_socs_batch_latency(klt_ctx* ctx, float* us, int cap);
+
 /* ---- adjoint (no reference counterpart) ---------------------------------
  * Resist loss of each SOCS image, L = sum_x (sigmoid(a (I(x) - thr)) - T(x))^2,
  * and its gradient with respect to the mask, dL/dm (Ly x Lx per mask), through

@@ -69,6 +69,47 @@ struct FusedTables {
   DevBuf Wgf, twyg;
 };
 
+// Work descriptors of the mixed-size batch (klt_socs_forward_batch): one per
+// block of each stage, so that one launch covers masks of different sizes
+// and several kernel sets.
+struct BandJob {        // row block `blk` of one mask
+  const float* in;      // the mask, Ly x Lx
+  const float4* Wf;     // its row twiddles (fragment order)
+  const float2* twy;    // its column twiddles, [Ly][ldtw]
+  float2* out;          // its band partials, [nblk][2Ny+1][16]
+  int Lx, Ly, Kpad, ldtw, Nx, Ny, blk, pad_;
+};
+struct MidJob {         // one mask x one kernel set
+  const float2* mpart;  // the mask's band partials
+  const float2* kern;   // the set's kernels, [nk][2Ny+1][2Nx+1]
+  const float* scales;  // and eigenvalues
+  float2* spec;         // the image's intensity spectrum, [32][4Nx+1]
+  int nblk, Nx, Ny, nk;
+  float inv_area;
+  int pad_[3];
+};
+struct SynthJob {       // one 128-column strip of one image
+  const float2* spec;
+  const float2* twx;    // [Lx][2Nx+1]
+  const _Float16* TYh;
+  const _Float16* TYl;
+  float* img;
+  int Nx, Lx, Ly, Lypad, x0, image, nstrips, pad_;
+};
+
+// One captured mixed batch: its job tables, intermediates and the graph that
+// replays its three stages; `key` is the call's arguments.
+struct BatchGraph {
+  std::vector<uint64_t> key;
+  int nimg = 0, nband = 0, nmid = 0, nsynth = 0;
+  bool full = true;
+  DevBuf band_jobs, mid_jobs, synth_jobs, mpart, spec, strips_done, done_at, t0;
+  const float2* spec_tw = nullptr;  // e^{+2 pi i j/64} of the coarse FFTs
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~BatchGraph();
+};
+
 }  // namespace klt
 
 struct klt_ctx {
@@ -91,6 +132,9 @@ struct klt_ctx {
   klt::DevBuf tcc_pts;                        // source points of the TCC build
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::SocsPlan>> socs_plans;
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::FusedTables>> fused_tables;
+  // captured mixed batches, most recent last (a few kept), and the last one run
+  std::vector<std::unique_ptr<klt::BatchGraph>> batches;
+  klt::BatchGraph* last_batch = nullptr;
   int activate() const;  // hipSetDevice(device)
 };
 
@@ -147,6 +191,11 @@ int hopkins_image_device(klt_ctx* ctx, const float* mask, int Lx, int Ly, const 
 // Top-k eigenpairs of a Hermitian matrix on the device (eig.hip).
 int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_dev,
                     double* vec_dev);
+// Mixed-size, multi-set batch (socs_fused.hip): images[m * nbank + s] is mask
+// m under kernel set s.  Captured into a graph on first use, replayed after.
+int socs_forward_batch(klt_ctx* ctx, int nmask, const float* const* masks, const int* Lx,
+                       const int* Ly, int nbank, const klt_kernels* const* banks,
+                       float* const* images);
 // Copy helpers honouring KLT_HOST_PTRS.
 int to_device(klt_ctx* ctx, const void* src, size_t bytes, unsigned flags, DevBuf& stage,
               const void** dptr);

@@ -90,11 +90,14 @@ __device__ __forceinline__ void load8(const float* row, int x, int Lx, bool vali
 // Epilogue: out[b][blk][ny + By][nx] = sum_{y in block} R[y][nx] tw(ny, y) for
 // |ny| <= By, 0 <= nx <= Bx; tw(ny, y) = twy[y][|ny|] (conjugated for ny < 0).
 // ---------------------------------------------------------------------------
+// Body of one block: row block `blk` of one image (in0 / in1: that image;
+// out: its partials, nblk blocks x (2By+1) x ldo; lpart: its loss partials).
 template <int NCB, bool RESIST, bool FULL>
-__global__ __launch_bounds__(256, (RESIST || !FULL) ? 2 : 4) void k_band_mfma(
-    const float* __restrict__ in0, const float* __restrict__ in1, float a_sig, float thr,
-    int Lx, int Ly, int Kpad, const float4* Wf, const float2* __restrict__ twy,
-    int ldtw, int Bx, int By, int ldo, float2* __restrict__ out_part, float* __restrict__ lpart) {
+__device__ __forceinline__ void band_block(
+    int blk, const float* __restrict__ in0, const float* __restrict__ in1, float a_sig,
+    float thr, int Lx, int Ly, int Kpad, const float4* Wf, const float2* __restrict__ twy,
+    int ldtw, int Bx, int By, int ldo, float2* __restrict__ out_img,
+    float* __restrict__ lpart_img) {
   constexpr int C = kBandChunk;
   constexpr int STEP_F4 = NCB * 2 * 64;    // 16-byte fragments per step
   constexpr int CHUNK_F4 = C * STEP_F4;    // per chunk
@@ -111,11 +114,10 @@ __global__ __launch_bounds__(256, (RESIST || !FULL) ? 2 : 4) void k_band_mfma(
   auto sTw = reinterpret_cast<float2(*)[8 * NCB]>(smem + R_BYTES);
   float4* sW = reinterpret_cast<float4*>(smem);
   const int tid = threadIdx.x;
-  const int blk = blockIdx.x, b = blockIdx.y, nblk = gridDim.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int r16 = lane & 15, kq = (lane >> 4) * 8;
   const int ybase = blk * kRowsBand + wave * 32;
-  const size_t ib = (size_t)b * Lx * Ly;
+  const size_t ib = 0;
   const int ya = ybase + r16, yb = ybase + 16 + r16;
   const bool va = ya < Ly, vb = yb < Ly;
   // rows past the image read row 0: their products land in rows of R that
@@ -261,8 +263,8 @@ __global__ __launch_bounds__(256, (RESIST || !FULL) ? 2 : 4) void k_band_mfma(
     sTw[yy][c] = (yy < nrow && c <= By) ? twy[(size_t)(y0 + yy) * ldtw + c] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  if (RESIST && tid == 0) lpart[(size_t)b * nblk + blk] = red[0] + red[1] + red[2] + red[3];
-  float2* out = out_part + ((size_t)b * nblk + blk) * Wy * ldo;
+  if (RESIST && tid == 0) lpart_img[blk] = red[0] + red[1] + red[2] + red[3];
+  float2* out = out_img + (size_t)blk * Wy * ldo;
   for (int o = tid; o < Wy * nxp; o += 256) {
     const int nyi = o / nxp, nx = o - nyi * nxp;
     const int ny = nyi - By;
@@ -277,6 +279,29 @@ __global__ __launch_bounds__(256, (RESIST || !FULL) ? 2 : 4) void k_band_mfma(
     }
     out[nyi * ldo + nx] = make_float2(sr, si);
   }
+}
+
+// Uniform batch: grid (ceil(Ly/128), B), all images Lx x Ly.
+template <int NCB, bool RESIST, bool FULL>
+__global__ __launch_bounds__(256, (RESIST || !FULL) ? 2 : 4) void k_band_mfma(
+    const float* __restrict__ in0, const float* __restrict__ in1, float a_sig, float thr,
+    int Lx, int Ly, int Kpad, const float4* Wf, const float2* __restrict__ twy, int ldtw,
+    int Bx, int By, int ldo, float2* __restrict__ out_part, float* __restrict__ lpart) {
+  const int blk = blockIdx.x, b = blockIdx.y, nblk = gridDim.x;
+  const size_t ib = (size_t)b * Lx * Ly;
+  band_block<NCB, RESIST, FULL>(blk, in0 + ib, RESIST ? in1 + ib : nullptr, a_sig, thr, Lx, Ly,
+                                Kpad, Wf, twy, ldtw, Bx, By, ldo,
+                                out_part + (size_t)b * nblk * (2 * By + 1) * ldo,
+                                RESIST ? lpart + (size_t)b * nblk : nullptr);
+}
+
+// Mixed sizes: one block per BandJob (a row block of one mask).
+template <bool FULL>
+__global__ __launch_bounds__(256, FULL ? 4 : 2) void k_band_jobs(
+    const klt::BandJob* __restrict__ jobs) {
+  const klt::BandJob j = jobs[blockIdx.x];
+  band_block<2, false, FULL>(j.blk, j.in, nullptr, 0.f, 0.f, j.Lx, j.Ly, j.Kpad, j.Wf, j.twy,
+                             j.ldtw, j.Nx, j.Ny, 16, j.out, nullptr);
 }
 
 // 8-point DFT, X[k] = sum_n x[n] e^{s 2 pi i nk/8}, s = +1 (inverse) or -1.
@@ -338,16 +363,18 @@ __device__ __forceinline__ void fft64_group(float2* v, const float2* tw, float2*
 // Stage 2.  grid (B), 256 threads = 32 groups of 8.
 // spec: [B][32][4Nx+1], row gy in [0, 2Ny] (rows above zero), entry gx + 2Nx.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_socs_mid(const float2* __restrict__ mpart, int nblk,
-                                                  int Nx, int Ny, float inv_area,
-                                                  const float2* __restrict__ kern,
-                                                  const float* __restrict__ scales, int nk,
-                                                  const float2* __restrict__ twM,
-                                                  float2* __restrict__ spec) {
+// Body of one block: one image (mpart: its band partials, nblk blocks;
+// spec: its intensity spectrum).
+__device__ __forceinline__ void mid_image(const float2* __restrict__ mpart, int nblk, int Nx,
+                                          int Ny, float inv_area,
+                                          const float2* __restrict__ kern,
+                                          const float* __restrict__ scales, int nk,
+                                          const float2* __restrict__ twM,
+                                          float2* __restrict__ spec) {
   __shared__ float2 sM[31 * 31];
   __shared__ float2 sB[31 * kM];
   __shared__ float2 sX[32 * 64];
-  const int b = blockIdx.x, tid = threadIdx.x;
+  const int tid = threadIdx.x;
   const int g = tid >> 3, t = tid & 7;
   const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
   const int Gx = 4 * Nx + 1;
@@ -358,7 +385,7 @@ __global__ __launch_bounds__(256) void k_socs_mid(const float2* __restrict__ mpa
     const int r = nx >= 0 ? nyi : (-ny + Ny), c = nx >= 0 ? nx : -nx;
     float2 s = make_float2(0.f, 0.f);
     for (int k = 0; k < nblk; ++k) {
-      const float2 p = mpart[(((size_t)b * nblk + k) * Wy + r) * 16 + c];
+      const float2 p = mpart[((size_t)k * Wy + r) * 16 + c];
       s.x += p.x;
       s.y += p.y;
     }
@@ -437,7 +464,7 @@ __global__ __launch_bounds__(256) void k_socs_mid(const float2* __restrict__ mpa
   __syncthreads();
   // rows gy in [0, 2Ny]: J[gy][gx] = 1/64^2 sum_ux C[gy][ux] e^{-2 pi i gx ux/64}
   const float inv = 1.0f / (float)(kM * kM);
-  float2* out = spec + (size_t)b * 32 * Gx;
+  float2* out = spec;
   {
     const bool act = g < G2;
     float2 v[8];
@@ -457,6 +484,25 @@ __global__ __launch_bounds__(256) void k_socs_mid(const float2* __restrict__ mpa
     }
   }
   for (int i = G2 * Gx + tid; i < 32 * Gx; i += 256) out[i] = make_float2(0.f, 0.f);
+}
+
+// Uniform batch: one block per image.
+__global__ __launch_bounds__(256) void k_socs_mid(const float2* __restrict__ mpart, int nblk,
+                                                  int Nx, int Ny, float inv_area,
+                                                  const float2* __restrict__ kern,
+                                                  const float* __restrict__ scales, int nk,
+                                                  const float2* __restrict__ twM,
+                                                  float2* __restrict__ spec) {
+  const int b = blockIdx.x;
+  mid_image(mpart + (size_t)b * nblk * (2 * Ny + 1) * 16, nblk, Nx, Ny, inv_area, kern, scales,
+            nk, twM, spec + (size_t)b * 32 * (4 * Nx + 1));
+}
+
+// Mixed sizes and kernel sets: one block per MidJob (one mask x one set).
+__global__ __launch_bounds__(256) void k_socs_mid_jobs(const klt::MidJob* __restrict__ jobs,
+                                                       const float2* __restrict__ twM) {
+  const klt::MidJob j = jobs[blockIdx.x];
+  mid_image(j.mpart, j.nblk, j.Nx, j.Ny, j.inv_area, j.kern, j.scales, j.nk, twM, j.spec);
 }
 
 // ---------------------------------------------------------------------------
@@ -662,24 +708,26 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd(
 // for the 64 columns (w & 1) of the strip, K = 64.
 // TY: [Lypad][64] hi/lo, TY[y][2gy] = w_gy cos, TY[y][2gy+1] = -w_gy sin.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_synth_mfma(const float2* __restrict__ spec, int Nx,
-                                                    int Lx, int Ly, int Lypad,
-                                                    const float2* __restrict__ twx,
-                                                    const _Float16* __restrict__ TYh,
-                                                    const _Float16* __restrict__ TYl,
-                                                    float* __restrict__ image) {
+// Body of one block: the 128-column strip at x0 of one image (spec: its
+// intensity spectrum; img: the image).
+__device__ __forceinline__ void synth_strip(int x0, const float2* __restrict__ spec, int Nx,
+                                            int Lx, int Ly, int Lypad,
+                                            const float2* __restrict__ twx,
+                                            const _Float16* __restrict__ TYh,
+                                            const _Float16* __restrict__ TYl,
+                                            float* __restrict__ img) {
   // the spectrum (prologue) and the split P operand (main loop) share the LDS
   constexpr int J_BYTES = 32 * 61 * 8, P_BYTES = 2 * kColsSynth * kPStride * 2;
   __shared__ __attribute__((aligned(16))) char smem[J_BYTES > P_BYTES ? J_BYTES : P_BYTES];
   float2* sJ = reinterpret_cast<float2*>(smem);
   _Float16* sPh = reinterpret_cast<_Float16*>(smem);
   _Float16* sPl = sPh + kColsSynth * kPStride;
-  const int b = blockIdx.y, x0 = blockIdx.x * kColsSynth, tid = threadIdx.x;
+  const int tid = threadIdx.x;
   const int Gx = 4 * Nx + 1, Hx = 2 * Nx + 1;
   __shared__ float red[4];
   float part = 0.f;
   for (int i = tid; i < 32 * Gx; i += 256) {
-    const float2 v = spec[(size_t)b * 32 * Gx + i];
+    const float2 v = spec[i];
     sJ[i] = v;
     part += fabsf(v.x) + fabsf(v.y);
   }
@@ -745,7 +793,6 @@ __global__ __launch_bounds__(256) void k_synth_mfma(const float2* __restrict__ s
       bh[cb][ks] = *reinterpret_cast<const half8_t*>(sPh + c * kPStride + ks * 32 + kq);
       bl[cb][ks] = *reinterpret_cast<const half8_t*>(sPl + c * kPStride + ks * 32 + kq);
     }
-  float* img = image + (size_t)b * Lx * Ly;
   // software pipeline over row blocks: the next block's TY operands load
   // while this block's products and stores run
   half8_t nh0, nh1, nl0, nl1;
@@ -784,6 +831,43 @@ __global__ __launch_bounds__(256) void k_synth_mfma(const float2* __restrict__ s
       }
     }
   }
+}
+
+// Uniform batch: grid (ceil(Lx/128), B).
+__global__ __launch_bounds__(256, 4) void k_synth_mfma(const float2* __restrict__ spec, int Nx,
+                                                    int Lx, int Ly, int Lypad,
+                                                    const float2* __restrict__ twx,
+                                                    const _Float16* __restrict__ TYh,
+                                                    const _Float16* __restrict__ TYl,
+                                                    float* __restrict__ image) {
+  const int b = blockIdx.y;
+  synth_strip(blockIdx.x * kColsSynth, spec + (size_t)b * 32 * (4 * Nx + 1), Nx, Lx, Ly, Lypad,
+              twx, TYh, TYl, image + (size_t)b * Lx * Ly);
+}
+
+// Mixed sizes and kernel sets: one block per SynthJob (one strip of one
+// image).  The block that completes an image's last strip records the
+// device wall clock in done_at[image].
+__global__ __launch_bounds__(256) void k_synth_jobs(const klt::SynthJob* __restrict__ jobs,
+                                                    unsigned* __restrict__ strips_done,
+                                                    unsigned long long* __restrict__ done_at) {
+  const klt::SynthJob j = jobs[blockIdx.x];
+  synth_strip(j.x0, j.spec, j.Nx, j.Lx, j.Ly, j.Lypad, j.twx, j.TYh, j.TYl, j.img);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&strips_done[j.image], 1u) + 1u == (unsigned)j.nstrips)
+      done_at[j.image] = wall_clock64();
+  }
+}
+
+// Graph head of a mixed batch: clears the strip counters (a fill kernel, so
+// that graph replays start from zero) and records the start time.
+__global__ void k_batch_begin(unsigned* __restrict__ strips_done, int n,
+                              unsigned long long* __restrict__ t0) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    strips_done[i] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *t0 = wall_clock64();
 }
 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
@@ -942,6 +1026,154 @@ int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, 
   return KLT_OK;
 }
 
+BatchGraph::~BatchGraph() {
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+}
+
+// Builds the job tables of a mixed batch and records its four launches on the
+// context stream (into a graph while capturing).
+static int build_batch(klt_ctx* ctx, BatchGraph* bg, int nmask, const float* const* masks,
+                       const int* Lx, const int* Ly, int nbank,
+                       const klt_kernels* const* banks, float* const* images) {
+  const int Nx = banks[0]->Nx, Ny = banks[0]->Ny, Wy = 2 * Ny + 1, Gx = 4 * Nx + 1;
+  std::vector<size_t> moff(nmask);
+  size_t mtot = 0;
+  for (int m = 0; m < nmask; ++m) {
+    moff[m] = mtot;
+    mtot += (size_t)((Ly[m] + kRowsBand - 1) / kRowsBand) * Wy * 16;
+  }
+  bg->nimg = nmask * nbank;
+  KLT_TRY(bg->mpart.ensure(mtot * sizeof(float2)));
+  KLT_TRY(bg->spec.ensure((size_t)bg->nimg * 32 * Gx * sizeof(float2)));
+  KLT_TRY(bg->strips_done.ensure((size_t)bg->nimg * sizeof(unsigned)));
+  KLT_TRY(bg->done_at.ensure((size_t)bg->nimg * sizeof(unsigned long long)));
+  KLT_TRY(bg->t0.ensure(sizeof(unsigned long long)));
+  std::vector<BandJob> bj;
+  std::vector<MidJob> mj;
+  std::vector<SynthJob> sj;
+  const float2* tw64 = nullptr;
+  bg->full = true;
+  for (int m = 0; m < nmask; ++m) {
+    SocsPlan* p = nullptr;
+    KLT_TRY(get_socs_plan(ctx, Lx[m], Ly[m], Nx, Ny, &p));
+    FusedTables* ft = nullptr;
+    KLT_TRY(get_fused_tables(ctx, p, &ft));
+    tw64 = ft->tw64.as<float2>();
+    bg->full = bg->full && Lx[m] % 32 == 0 && (reinterpret_cast<uintptr_t>(masks[m]) & 15) == 0;
+    const int nblk = (Ly[m] + kRowsBand - 1) / kRowsBand;
+    float2* mp = bg->mpart.as<float2>() + moff[m];
+    for (int blk = 0; blk < nblk; ++blk)
+      bj.push_back(BandJob{masks[m], ft->Wf.as<float4>(), p->twy_in.as<float2>(), mp, Lx[m],
+                           Ly[m], ft->Kpad, p->NP, Nx, Ny, blk, 0});
+    const int nstrips = (Lx[m] + kColsSynth - 1) / kColsSynth;
+    for (int s = 0; s < nbank; ++s) {
+      const int img = m * nbank + s;
+      float2* sp = bg->spec.as<float2>() + (size_t)img * 32 * Gx;
+      mj.push_back(MidJob{mp, banks[s]->kern.as<float2>(), banks[s]->scales.as<float>(), sp,
+                          nblk, Nx, Ny, banks[s]->nk,
+                          1.0f / ((float)Lx[m] * (float)Ly[m]), {0, 0, 0}});
+      for (int st = 0; st < nstrips; ++st)
+        sj.push_back(SynthJob{sp, p->twx_out.as<float2>(), ft->TYh.as<_Float16>(),
+                              ft->TYl.as<_Float16>(), images[img], Nx, Lx[m], Ly[m], ft->Lypad,
+                              st * kColsSynth, img, nstrips, 0});
+    }
+  }
+  bg->nband = (int)bj.size();
+  bg->nmid = (int)mj.size();
+  bg->nsynth = (int)sj.size();
+  KLT_TRY(bg->band_jobs.ensure(bj.size() * sizeof(BandJob)));
+  KLT_TRY(bg->mid_jobs.ensure(mj.size() * sizeof(MidJob)));
+  KLT_TRY(bg->synth_jobs.ensure(sj.size() * sizeof(SynthJob)));
+  KLT_HIP(hipMemcpy(bg->band_jobs.ptr, bj.data(), bj.size() * sizeof(BandJob),
+                    hipMemcpyHostToDevice));
+  KLT_HIP(hipMemcpy(bg->mid_jobs.ptr, mj.data(), mj.size() * sizeof(MidJob),
+                    hipMemcpyHostToDevice));
+  KLT_HIP(hipMemcpy(bg->synth_jobs.ptr, sj.data(), sj.size() * sizeof(SynthJob),
+                    hipMemcpyHostToDevice));
+  bg->spec_tw = tw64;
+  return KLT_OK;
+}
+
+// The batch's launches, in stream order.
+static void launch_batch(klt_ctx* ctx, const BatchGraph* bg) {
+  hipLaunchKernelGGL(k_batch_begin, dim3(std::max(1, std::min(64, (bg->nimg + 255) / 256))),
+                     dim3(256), 0, ctx->stream, bg->strips_done.as<unsigned>(), bg->nimg,
+                     bg->t0.as<unsigned long long>());
+  if (bg->full)
+    hipLaunchKernelGGL(k_band_jobs<true>, dim3(bg->nband), dim3(256), 0, ctx->stream,
+                       bg->band_jobs.as<BandJob>());
+  else
+    hipLaunchKernelGGL(k_band_jobs<false>, dim3(bg->nband), dim3(256), 0, ctx->stream,
+                       bg->band_jobs.as<BandJob>());
+  hipLaunchKernelGGL(k_socs_mid_jobs, dim3(bg->nmid), dim3(256), 0, ctx->stream,
+                     bg->mid_jobs.as<MidJob>(), bg->spec_tw);
+  hipLaunchKernelGGL(k_synth_jobs, dim3(bg->nsynth), dim3(256), 0, ctx->stream,
+                     bg->synth_jobs.as<SynthJob>(), bg->strips_done.as<unsigned>(),
+                     bg->done_at.as<unsigned long long>());
+}
+
+int socs_forward_batch(klt_ctx* ctx, int nmask, const float* const* masks, const int* Lx,
+                       const int* Ly, int nbank, const klt_kernels* const* banks,
+                       float* const* images) {
+  std::vector<uint64_t> key;
+  key.push_back(((uint64_t)nmask << 32) | (uint32_t)nbank);
+  for (int m = 0; m < nmask; ++m) {
+    key.push_back(reinterpret_cast<uintptr_t>(masks[m]));
+    key.push_back(((uint64_t)(uint32_t)Lx[m] << 32) | (uint32_t)Ly[m]);
+  }
+  for (int s = 0; s < nbank; ++s) {
+    key.push_back(reinterpret_cast<uintptr_t>(banks[s]->kern.ptr));
+    key.push_back(reinterpret_cast<uintptr_t>(banks[s]->scales.ptr));
+    key.push_back(((uint64_t)(uint32_t)banks[s]->nk << 32) |
+                  (uint32_t)(banks[s]->Nx * 64 + banks[s]->Ny));
+  }
+  for (int i = 0; i < nmask * nbank; ++i) key.push_back(reinterpret_cast<uintptr_t>(images[i]));
+  // A stream that the caller is capturing records the launches into the
+  // caller's graph directly.
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  KLT_HIP(hipStreamIsCapturing(ctx->stream, &cap));
+  for (size_t i = 0; i < ctx->batches.size(); ++i)
+    if (ctx->batches[i]->key == key) {
+      std::unique_ptr<BatchGraph> bg = std::move(ctx->batches[i]);
+      ctx->batches.erase(ctx->batches.begin() + i);
+      ctx->batches.push_back(std::move(bg));
+      BatchGraph* b = ctx->batches.back().get();
+      ctx->last_batch = b;
+      if (cap != hipStreamCaptureStatusNone || !b->exec) {
+        launch_batch(ctx, b);
+        KLT_HIP(hipGetLastError());
+      } else {
+        KLT_HIP(hipGraphLaunch(b->exec, ctx->stream));
+      }
+      return KLT_OK;
+    }
+  auto bg = std::make_unique<BatchGraph>();
+  bg->key = key;
+  KLT_TRY(build_batch(ctx, bg.get(), nmask, masks, Lx, Ly, nbank, banks, images));
+  if (cap != hipStreamCaptureStatusNone) {
+    launch_batch(ctx, bg.get());
+    KLT_HIP(hipGetLastError());
+  } else {
+    KLT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    launch_batch(ctx, bg.get());
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(ctx->stream, &bg->graph);
+    KLT_HIP(le);
+    KLT_HIP(ce);
+    KLT_HIP(hipGraphInstantiate(&bg->exec, bg->graph, nullptr, nullptr, 0));
+    KLT_HIP(hipGraphLaunch(bg->exec, ctx->stream));
+  }
+  ctx->batches.push_back(std::move(bg));
+  if (ctx->batches.size() > 4) {
+    // the evicted batch's buffers may still be in use by its last launch
+    KLT_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->batches.erase(ctx->batches.begin());
+  }
+  ctx->last_batch = ctx->batches.back().get();
+  return KLT_OK;
+}
+
 int band_mask_partials(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const float* masks, int B,
                        float2* out) {
   SocsPlan* p = nullptr;
@@ -1064,6 +1296,55 @@ int klt_socs_backward(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
     KLT_HIP(hipStreamSynchronize(ctx->stream));
   }
   return KLT_OK;
+}
+
+int klt_socs_forward_batch(klt_ctx* ctx, int nmask, const float* const* masks, const int* Lx,
+                           const int* Ly, int nbank, const klt_kernels* const* banks,
+                           float* const* images, unsigned flags) {
+  if (!ctx || !masks || !Lx || !Ly || !banks || !images || nmask <= 0 || nbank <= 0)
+    return klt::set_error(KLT_ERR_ARG, "klt_socs_forward_batch: NULL argument or empty batch");
+  if (flags & KLT_HOST_PTRS)
+    return klt::set_error(KLT_ERR_ARG, "klt_socs_forward_batch: device pointers only");
+  for (int s = 0; s < nbank; ++s) {
+    if (!banks[s])
+      return klt::set_error(KLT_ERR_ARG, "klt_socs_forward_batch: NULL kernel set");
+    if (banks[s]->Nx != banks[0]->Nx || banks[s]->Ny != banks[0]->Ny)
+      return klt::set_error(KLT_ERR_ARG,
+                            "klt_socs_forward_batch: kernel sets of different sizes");
+  }
+  if (!klt::socs_fused_supported(banks[0]))
+    return klt::set_error(KLT_ERR_ARG, "klt_socs_forward_batch: kernel half-size above 15");
+  for (int m = 0; m < nmask; ++m) {
+    if (!masks[m] || Lx[m] <= 0 || Ly[m] <= 0)
+      return klt::set_error(KLT_ERR_ARG, "klt_socs_forward_batch: invalid mask " +
+                                             std::to_string(m));
+    if (4 * banks[0]->Nx + 1 > Lx[m] || 4 * banks[0]->Ny + 1 > Ly[m])  // klitho_socs.cpp:164-169
+      return klt::set_error(KLT_ERR_SIZE, "ERROR in FI()");
+  }
+  for (int i = 0; i < nmask * nbank; ++i)
+    if (!images[i]) return klt::set_error(KLT_ERR_ARG, "klt_socs_forward_batch: NULL image");
+  KLT_TRY(ctx->activate());
+  return klt::socs_forward_batch(ctx, nmask, masks, Lx, Ly, nbank, banks, images);
+}
+
+int klt_socs_batch_latency(klt_ctx* ctx, float* us, int cap) {
+  if (!ctx || (!us && cap > 0))
+    return klt::set_error(KLT_ERR_ARG, "klt_socs_batch_latency: NULL argument");
+  const klt::BatchGraph* bg = ctx->last_batch;
+  if (!bg) return klt::set_error(KLT_ERR_ARG, "klt_socs_batch_latency: no batch has run");
+  KLT_TRY(ctx->activate());
+  KLT_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<unsigned long long> done(bg->nimg);
+  unsigned long long t0 = 0;
+  KLT_HIP(hipMemcpy(done.data(), bg->done_at.ptr, done.size() * sizeof(unsigned long long),
+                    hipMemcpyDeviceToHost));
+  KLT_HIP(hipMemcpy(&t0, bg->t0.ptr, sizeof(t0), hipMemcpyDeviceToHost));
+  int khz = 0;
+  KLT_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+  if (khz <= 0) return klt::set_error(KLT_ERR_HIP, "klt_socs_batch_latency: no wall clock rate");
+  const int n = std::min(cap, bg->nimg);
+  for (int i = 0; i < n; ++i) us[i] = (float)((double)(done[i] - t0) * 1000.0 / (double)khz);
+  return bg->nimg;
 }
 
 }  // extern "C"

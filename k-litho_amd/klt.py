@@ -84,6 +84,8 @@ def lib() -> C.CDLL:
         L.klt_kernels_destroy.restype = None
         L.klt_socs_image.argtypes = [vp, vp, fp, i, i, fp, u]
         L.klt_socs_forward.argtypes = [vp, vp, fp, i, i, i, fp, u]
+        L.klt_socs_forward_batch.argtypes = [vp, i, vp, vp, vp, i, vp, vp, u]
+        L.klt_socs_batch_latency.argtypes = [vp, fp, i]
         _lib = L
     return _lib
 
@@ -333,6 +335,35 @@ class Context:
         else:
             check(lib().klt_socs_forward(self._h, bank.handle, pm, B, Lx, Ly, pi, fl))
         return images
+
+    def socs_forward_batch(self, banks, masks, images=None):
+        """SOCS images of masks of different sizes under several kernel sets
+        in one klt_socs_forward_batch call.  masks: list of [Ly, Lx] device
+        (torch) tensors; banks: list of KernelBank.  Returns the list of
+        images, images[m * len(banks) + s] = mask m under bank s."""
+        import torch
+        nm, nb = len(masks), len(banks)
+        if images is None:
+            images = [torch.empty_like(m) for m in masks for _ in banks]
+        mp = (C.c_void_p * nm)(*[m.data_ptr() for m in masks])
+        lx = (C.c_int * nm)(*[int(m.shape[-1]) for m in masks])
+        ly = (C.c_int * nm)(*[int(m.shape[-2]) for m in masks])
+        bp = (C.c_void_p * nb)(*[b.handle.value for b in banks])
+        ip = (C.c_void_p * (nm * nb))(*[im.data_ptr() for im in images])
+        check(lib().klt_socs_forward_batch(self._h, nm, mp, lx, ly, nb, bp, ip, 0))
+        return images
+
+    def batch_latency(self) -> np.ndarray:
+        """Per-image completion times (us from the batch start) of the last
+        socs_forward_batch call."""
+        n = lib().klt_socs_batch_latency(self._h, None, 0)
+        if n < 0:
+            check(n)
+        out = np.empty(n, np.float32)
+        rc = lib().klt_socs_batch_latency(self._h, out.ctypes.data_as(C.c_void_p), n)
+        if rc < 0:
+            check(rc)
+        return out
 
 
 def _empty_like(a, shape=None, dtype=np.float32):
