@@ -9,12 +9,13 @@
 //      The off-diagonal entries are complex; a diagonal unitary D maps the
 //      tridiagonal T to the real symmetric T_r = D^H T D with off-diagonal
 //      |e_j|.
-//   2. k_bisect: the k largest eigenvalues of T_r by bisection on Sturm
-//      counts (one thread per eigenvalue).
+//   2. k_bisect: the k largest eigenvalues of T_r by multisection on Sturm
+//      counts (one wave per eigenvalue, 64 points per pass).
 //   3. k_inverse_iteration: eigenvectors of T_r by inverse iteration with a
-//      partially pivoted LU of T_r - lambda I; eigenvalues closer than
-//      1e-4 ||T|| form a cluster whose vectors are re-orthogonalised
-//      (modified Gram-Schmidt) at every iteration, one thread per cluster.
+//      partially pivoted LU of T_r - lambda I (one thread per vector);
+//      eigenvalues closer than 1e-4 ||T|| form a cluster whose vectors are
+//      re-orthogonalised in eigenvalue order at every iteration (Gram-Schmidt
+//      twice, parallel over the vector length), one block per cluster.
 //   4. k_back_transform: x = H_0 ... H_{n-3} D z, one block per vector.
 // Textbook algorithms (Golub & Van Loan, ch. 8; Wilkinson's inverse
 // iteration); all arithmetic FP64.
@@ -218,71 +219,126 @@ __device__ int sturm_count(const double* d, const double* e2, int n, double x, d
   return c;
 }
 
-// Thread idx finds the (idx+1)-th largest eigenvalue.
-__global__ void k_bisect(const double* __restrict__ d, const double* __restrict__ e2, int n,
-                         int k, double lo0, double hi0, double pivmin,
-                         double* __restrict__ w) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+// Wave idx finds the (idx+1)-th largest eigenvalue by multisection: its 64
+// lanes count the eigenvalues below 64 equally spaced interior points of the
+// bracket at once (Sturm sequences), and the bracket shrinks to the gap
+// between the two points where the count passes the target, 65x per pass.
+__global__ __launch_bounds__(64) void k_bisect(const double* __restrict__ d,
+                                               const double* __restrict__ e2, int n, int k,
+                                               double lo0, double hi0, double pivmin,
+                                               double* __restrict__ w) {
+  const int idx = blockIdx.x, lane = threadIdx.x;
   if (idx >= k) return;
   const int target = n - 1 - idx;  // ascending index
   double lo = lo0, hi = hi0;
-  for (int it = 0; it < 200; ++it) {
-    const double mid = 0.5 * (lo + hi);
-    if (mid <= lo || mid >= hi) break;
+  for (int it = 0; it < 64; ++it) {
     if (hi - lo <= 2.2e-16 * fmax(fabs(lo), fabs(hi)) + pivmin) break;
-    if (sturm_count(d, e2, n, mid, pivmin) > target) hi = mid;
-    else lo = mid;
+    const double h = hi - lo;
+    const double x = lo + h * (double)(lane + 1) / 65.0;
+    const bool above = sturm_count(d, e2, n, x, pivmin) > target;
+    const unsigned long long mask = __ballot(above);
+    const int f = mask ? __ffsll(mask) - 1 : 64;  // first point past the eigenvalue
+    const double nlo = f == 0 ? lo : lo + h * (double)f / 65.0;
+    const double nhi = f == 64 ? hi : lo + h * (double)(f + 1) / 65.0;
+    if (!(nlo > lo || nhi < hi)) break;  // no progress left in double precision
+    lo = nlo;
+    hi = nhi;
   }
-  w[idx] = 0.5 * (lo + hi);
+  if (lane == 0) w[idx] = 0.5 * (lo + hi);
 }
 
-// Inverse iteration, one thread per cluster of eigenvalues [first, last).
-// z: k x n (rows), work: per cluster 5n doubles.
-__global__ void k_inverse_iteration(const double* __restrict__ d, const double* __restrict__ e,
-                                    int n, const double* __restrict__ w,
-                                    const int* __restrict__ cl_begin, int ncl, double tnorm,
-                                    double* __restrict__ z, double* __restrict__ work) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncl) return;
-  const int first = cl_begin[c], last = cl_begin[c + 1];
-  double* lu_d = work + (size_t)c * 5 * n;  // U diagonal
-  double* lu_u1 = lu_d + n;                 // U first superdiagonal
-  double* lu_u2 = lu_u1 + n;                // U second superdiagonal (from swaps)
-  double* lu_l = lu_u2 + n;                 // multipliers
-  double* piv = lu_l + n;                   // 1 when rows i, i+1 were swapped
+// LU of T_r - lam I with partial pivoting between rows i and i+1, into the
+// 5n doubles at lu: U diagonal, first and second superdiagonals, multipliers,
+// and the swap flags.
+__device__ void tri_lu(const double* __restrict__ d, const double* __restrict__ e, int n,
+                       double lam, double tnorm, double* __restrict__ lu) {
   const double eps = 2.220446049250313e-16;
-  for (int q = first; q < last; ++q) {
-    // separate (numerically) equal eigenvalues of a cluster slightly
-    const double lam = w[q] + (double)(q - first) * 10.0 * eps * tnorm;
-    // LU of T - lam I with partial pivoting (rows i, i+1)
-    double a = d[0] - lam, b = n > 1 ? e[0] : 0.0;
-    for (int i = 0; i < n - 1; ++i) {
-      const double cdn = e[i], dn = d[i + 1] - lam, en = i + 1 < n - 1 ? e[i + 1] : 0.0;
-      if (fabs(a) >= fabs(cdn)) {
-        if (a == 0.0) a = eps * tnorm;
-        const double l = cdn / a;
-        lu_d[i] = a;
-        lu_u1[i] = b;
-        lu_u2[i] = 0.0;
-        lu_l[i] = l;
-        piv[i] = 0.0;
-        a = dn - l * b;
-        b = en;
-      } else {
-        const double l = a / cdn;
-        lu_d[i] = cdn;
-        lu_u1[i] = dn;
-        lu_u2[i] = en;
-        lu_l[i] = l;
-        piv[i] = 1.0;
-        a = b - l * dn;
-        b = -l * en;
-      }
+  double* lu_d = lu;
+  double* lu_u1 = lu + n;
+  double* lu_u2 = lu + 2 * n;
+  double* lu_l = lu + 3 * n;
+  double* piv = lu + 4 * n;
+  double a = d[0] - lam, b = n > 1 ? e[0] : 0.0;
+  for (int i = 0; i < n - 1; ++i) {
+    const double cdn = e[i], dn = d[i + 1] - lam, en = i + 1 < n - 1 ? e[i + 1] : 0.0;
+    if (fabs(a) >= fabs(cdn)) {
+      if (a == 0.0) a = eps * tnorm;
+      const double l = cdn / a;
+      lu_d[i] = a;
+      lu_u1[i] = b;
+      lu_u2[i] = 0.0;
+      lu_l[i] = l;
+      piv[i] = 0.0;
+      a = dn - l * b;
+      b = en;
+    } else {
+      const double l = a / cdn;
+      lu_d[i] = cdn;
+      lu_u1[i] = dn;
+      lu_u2[i] = en;
+      lu_l[i] = l;
+      piv[i] = 1.0;
+      a = b - l * dn;
+      b = -l * en;
     }
-    if (a == 0.0) a = eps * tnorm;
-    lu_d[n - 1] = a;
-    double* x = z + (size_t)q * n;
+  }
+  if (a == 0.0) a = eps * tnorm;
+  lu_d[n - 1] = a;
+}
+
+// x <- (T_r - lam I)^{-1} x with the factors of tri_lu.
+__device__ void tri_solve(const double* __restrict__ lu, int n, double* __restrict__ x) {
+  const double* lu_d = lu;
+  const double* lu_u1 = lu + n;
+  const double* lu_u2 = lu + 2 * n;
+  const double* lu_l = lu + 3 * n;
+  const double* piv = lu + 4 * n;
+  double xi = x[0];
+  for (int i = 0; i < n - 1; ++i) {
+    const double xn = x[i + 1];
+    if (piv[i] != 0.0) {
+      x[i] = xn;
+      xi = xi - lu_l[i] * xn;
+    } else {
+      x[i] = xi;
+      xi = xn - lu_l[i] * xi;
+    }
+  }
+  x[n - 1] = xi;
+  double x1 = x[n - 1] / lu_d[n - 1];
+  x[n - 1] = x1;
+  if (n < 2) return;
+  double x0 = (x[n - 2] - lu_u1[n - 2] * x1) / lu_d[n - 2];
+  x[n - 2] = x0;
+  for (int i = n - 3; i >= 0; --i) {
+    const double v = (x[i] - lu_u1[i] * x0 - lu_u2[i] * x1) / lu_d[i];
+    x[i] = v;
+    x1 = x0;
+    x0 = v;
+  }
+}
+
+// Eigenvectors of T_r by inverse iteration, one block per cluster of close
+// eigenvalues (cl_begin[c] .. cl_begin[c+1]).  Thread t of the block factors
+// T_r - lam_q I and solves with it for vector q = first + t; after every
+// round of solves the block re-orthogonalises the cluster's vectors in
+// eigenvalue order (classical Gram-Schmidt applied twice, against 32 earlier
+// vectors at a time, parallel over the vector length) and normalises them.
+// Equal eigenvalues of a cluster are separated slightly in the shifts.
+__global__ __launch_bounds__(256) void k_inverse_iteration(
+    const double* __restrict__ d, const double* __restrict__ e, int n,
+    const double* __restrict__ w, const int* __restrict__ cl_begin, double tnorm,
+    double* __restrict__ z, double* __restrict__ work) {
+  __shared__ double red[4][33];
+  __shared__ double dots[32];
+  const int c = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int first = cl_begin[c], last = cl_begin[c + 1];
+  const double eps = 2.220446049250313e-16;
+  for (int q = first + tid; q < last; q += 256) {
+    const double lam = w[q] + (double)(q - first) * 10.0 * eps * tnorm;
+    tri_lu(d, e, n, lam, tnorm, work + (size_t)q * 5 * n);
     // deterministic start vector
+    double* x = z + (size_t)q * n;
     unsigned s = 2463534242u ^ (unsigned)(q * 2654435761u);
     for (int i = 0; i < n; ++i) {
       s ^= s << 13;
@@ -290,33 +346,50 @@ __global__ void k_inverse_iteration(const double* __restrict__ d, const double* 
       s ^= s << 5;
       x[i] = ((double)(s & 0xffffff) / 16777216.0) - 0.5;
     }
-    for (int it = 0; it < 6; ++it) {
-      // forward: apply the row swaps and multipliers
-      for (int i = 0; i < n - 1; ++i) {
-        if (piv[i] != 0.0) {
-          const double tmp = x[i];
-          x[i] = x[i + 1];
-          x[i + 1] = tmp - lu_l[i] * x[i];
-        } else {
-          x[i + 1] -= lu_l[i] * x[i];
+  }
+  for (int it = 0; it < 6; ++it) {
+    for (int q = first + tid; q < last; q += 256)
+      tri_solve(work + (size_t)q * 5 * n, n, z + (size_t)q * n);
+    __syncthreads();
+    for (int q = first; q < last; ++q) {
+      double* x = z + (size_t)q * n;
+      for (int pass = 0; pass < 2; ++pass)
+        for (int r0 = first; r0 < q; r0 += 32) {
+          const int nr = min(32, q - r0);
+          double acc[32];
+#pragma unroll
+          for (int j = 0; j < 32; ++j) acc[j] = 0.0;
+          for (int i = tid; i < n; i += 256) {
+            const double xi = x[i];
+#pragma unroll
+            for (int j = 0; j < 32; ++j)
+              if (j < nr) acc[j] += z[(size_t)(r0 + j) * n + i] * xi;
+          }
+#pragma unroll
+          for (int j = 0; j < 32; ++j) {
+            const double v = wave_sum(acc[j]);
+            if (lane == 0) red[wv][j] = v;
+          }
+          __syncthreads();
+          if (tid < 32) dots[tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+          __syncthreads();
+          for (int i = tid; i < n; i += 256) {
+            double v = x[i];
+#pragma unroll
+            for (int j = 0; j < 32; ++j)
+              if (j < nr) v -= dots[j] * z[(size_t)(r0 + j) * n + i];
+            x[i] = v;
+          }
+          __syncthreads();
         }
-      }
-      // back substitution with U (diag, super1, super2)
-      x[n - 1] /= lu_d[n - 1];
-      if (n >= 2) x[n - 2] = (x[n - 2] - lu_u1[n - 2] * x[n - 1]) / lu_d[n - 2];
-      for (int i = n - 3; i >= 0; --i)
-        x[i] = (x[i] - lu_u1[i] * x[i + 1] - lu_u2[i] * x[i + 2]) / lu_d[i];
-      // re-orthogonalise against the cluster's earlier vectors, normalise
-      for (int r = first; r < q; ++r) {
-        const double* y = z + (size_t)r * n;
-        double dot = 0.0;
-        for (int i = 0; i < n; ++i) dot += y[i] * x[i];
-        for (int i = 0; i < n; ++i) x[i] -= dot * y[i];
-      }
-      double nrm = 0.0;
-      for (int i = 0; i < n; ++i) nrm += x[i] * x[i];
-      nrm = 1.0 / sqrt(nrm);
-      for (int i = 0; i < n; ++i) x[i] *= nrm;
+      double s2 = 0.0;
+      for (int i = tid; i < n; i += 256) s2 += x[i] * x[i];
+      s2 = wave_sum(s2);
+      if (lane == 0) red[wv][32] = s2;
+      __syncthreads();
+      const double inv = 1.0 / sqrt(red[0][32] + red[1][32] + red[2][32] + red[3][32]);
+      for (int i = tid; i < n; i += 256) x[i] *= inv;
+      __syncthreads();
     }
   }
 }
@@ -467,8 +540,8 @@ int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_d
   hi += 2.2e-16 * tnorm + pivmin;
   KLT_HIP(hipMemcpyAsync(e, he.data(), n * 8, hipMemcpyHostToDevice, st));
   KLT_HIP(hipMemcpyAsync(e2, he2.data(), n * 8, hipMemcpyHostToDevice, st));
-  KLT_LAUNCH(ctx, "k_bisect", k_bisect, dim3((k + 63) / 64), dim3(64), 0, diag, e2, n, k, lo, hi,
-             pivmin, w_dev);
+  KLT_LAUNCH(ctx, "k_bisect", k_bisect, dim3(k), dim3(64), 0, diag, e2, n, k, lo, hi, pivmin,
+             w_dev);
   std::vector<double> hw(k);
   KLT_HIP(hipMemcpyAsync(hw.data(), w_dev, k * 8, hipMemcpyDeviceToHost, st));
   KLT_HIP(hipStreamSynchronize(st));
@@ -483,8 +556,8 @@ int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_d
   hcl.push_back(k);
   const int ncl = (int)hcl.size() - 1;
   KLT_HIP(hipMemcpyAsync(cl, hcl.data(), hcl.size() * 4, hipMemcpyHostToDevice, st));
-  KLT_LAUNCH(ctx, "k_inverse_iteration", k_inverse_iteration, dim3((ncl + 63) / 64), dim3(64), 0,
-             diag, e, n, w_dev, cl, ncl, tnorm, z, work);
+  KLT_LAUNCH(ctx, "k_inverse_iteration", k_inverse_iteration, dim3(ncl), dim3(256), 0, diag,
+             e, n, w_dev, cl, tnorm, z, work);
   KLT_LAUNCH(ctx, "k_back_transform", k_back_transform, dim3(k), dim3(kThreads), lds_v, z, n, V,
              tau, sub, reinterpret_cast<cd*>(vec_dev));
   KLT_HIP(hipGetLastError());

@@ -130,6 +130,8 @@ struct klt_ctx {
   klt::DevBuf gpart, gspec, lpart, bwd_img;  // adjoint scratch
   klt::DevBuf abbe_tcc, abbe_band, abbe_in;   // source-gradient scratch / staging
   klt::DevBuf tcc_pts;                        // source points of the TCC build
+  klt::DevBuf tcc_part;                       // split-K partial sums of the TCC Gram
+  klt::DevBuf tw64;                           // e^{+2 pi i j/64}, j < 64
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::SocsPlan>> socs_plans;
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::FusedTables>> fused_tables;
   // captured mixed batches, most recent last (a few kept), and the last one run
@@ -171,6 +173,11 @@ int get_socs_plan(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, SocsPlan** out);
 int socs_forward_device(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int B,
                         int Lx, int Ly, float* images);
 bool socs_fused_supported(const klt_kernels* kb);
+// Coarse-grid stage (64-point FFTs, FP32) of the single-image path, kernel
+// half-size <= 15: band [B][2Ny+1][2Nx+1] (normalised) to the intensity
+// spectra spec [B][G][4Nx+1].
+int socs_mid_from_band(klt_ctx* ctx, const klt_kernels* kb, const float2* band, int B,
+                       float2* spec, int G);
 int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int B, int Lx,
                        int Ly, float* images);
 // Band partial sums of the fused passes (socs_fused.hip), nblk = ceil(Ly/128):

@@ -7,10 +7,12 @@
 // limited to |g| <= 2N, so this implementation:
 //   1. k_band_cols / k_band_rows: computes just that band of the mask
 //      spectrum with two separable DFT passes (one read of the mask);
-//   2. k_socs_coarse: per image, forms A_k = K_k * M^ on an M x M grid
-//      (M = power of two >= 4N+1, which is exact: |g| <= 2N never aliases),
-//      inverse-transforms it, accumulates lambda_k |E_k|^2, and forward-
-//      transforms the coarse intensity to its |g| <= 2N spectrum;
+//   2. per image, forms A_k = K_k * M^ on an M x M grid (M = power of two
+//      >= 4N+1, which is exact: |g| <= 2N never aliases), inverse-transforms
+//      it, accumulates lambda_k |E_k|^2, and forward-transforms the coarse
+//      intensity to its |g| <= 2N spectrum: for N <= 15 with the 64-point
+//      LDS FFTs of socs_fused.hip (k_socs_mid_band), above that with direct
+//      transforms (k_socs_coarse);
 //   3. k_synth_cols / k_synth_rows: evaluates the band-limited intensity at
 //      every pixel with two separable passes (one write of the image).
 // HBM traffic per image is one mask read and one image write plus small
@@ -446,9 +448,13 @@ int socs_forward_device(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
       case 24: launch_band<24>(ctx, m, nb, Lx, Ly, chunks, rows, p, qpart, band); break;
       default: launch_band<32>(ctx, m, nb, Lx, Ly, chunks, rows, p, qpart, band); break;
     }
-    KLT_LAUNCH(ctx, "k_socs_coarse", k_socs_coarse, dim3(nb), dim3(kThreads), lds_coarse, band,
-               kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk, Nx, Ny, M,
-               p->twM.as<float2>(), ctx->spec.as<float2>(), G);
+    if (socs_fused_supported(kb)) {
+      KLT_TRY(socs_mid_from_band(ctx, kb, band, nb, ctx->spec.as<float2>(), G));
+    } else {
+      KLT_LAUNCH(ctx, "k_socs_coarse", k_socs_coarse, dim3(nb), dim3(kThreads), lds_coarse,
+                 band, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk, Nx, Ny, M,
+                 p->twM.as<float2>(), ctx->spec.as<float2>(), G);
+    }
     KLT_LAUNCH(ctx, "k_synth_cols", k_synth_cols, dim3((Lx + kThreads - 1) / kThreads, nb),
                dim3(kThreads), sizeof(float2) * G * Gx, ctx->spec.as<float2>(), G, Nx, Ny, Lx,
                p->twx_out.as<float2>(), ctx->ccols.as<float2>());

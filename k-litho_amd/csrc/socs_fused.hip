@@ -363,14 +363,17 @@ __device__ __forceinline__ void fft64_group(float2* v, const float2* tw, float2*
 // Stage 2.  grid (B), 256 threads = 32 groups of 8.
 // spec: [B][32][4Nx+1], row gy in [0, 2Ny] (rows above zero), entry gx + 2Nx.
 // ---------------------------------------------------------------------------
-// Body of one block: one image (mpart: its band partials, nblk blocks;
-// spec: its intensity spectrum).
+// Body of one block: one image.  FULL_BAND = false: mpart holds its band
+// partials (nblk blocks of the band kernel); true: mpart is its whole
+// normalised band M^[ny+Ny][nx+Nx] (the FP32 single-image path).  spec: its
+// intensity spectrum, spec_rows rows of 4Nx+1 (rows past 2Ny zero).
+template <bool FULL_BAND>
 __device__ __forceinline__ void mid_image(const float2* __restrict__ mpart, int nblk, int Nx,
                                           int Ny, float inv_area,
                                           const float2* __restrict__ kern,
                                           const float* __restrict__ scales, int nk,
                                           const float2* __restrict__ twM,
-                                          float2* __restrict__ spec) {
+                                          float2* __restrict__ spec, int spec_rows) {
   __shared__ float2 sM[31 * 31];
   __shared__ float2 sB[31 * kM];
   __shared__ float2 sX[32 * 64];
@@ -380,6 +383,10 @@ __device__ __forceinline__ void mid_image(const float2* __restrict__ mpart, int 
   const int Gx = 4 * Nx + 1;
   // mask-spectrum band from the per-block partials (Hermitian for nx < 0)
   for (int i = tid; i < T; i += 256) {
+    if (FULL_BAND) {
+      sM[i] = mpart[i];
+      continue;
+    }
     const int nyi = i / Wx, nx = i - nyi * Wx - Nx;
     const int ny = nyi - Ny;
     const int r = nx >= 0 ? nyi : (-ny + Ny), c = nx >= 0 ? nx : -nx;
@@ -483,7 +490,7 @@ __device__ __forceinline__ void mid_image(const float2* __restrict__ mpart, int 
       }
     }
   }
-  for (int i = G2 * Gx + tid; i < 32 * Gx; i += 256) out[i] = make_float2(0.f, 0.f);
+  for (int i = G2 * Gx + tid; i < spec_rows * Gx; i += 256) out[i] = make_float2(0.f, 0.f);
 }
 
 // Uniform batch: one block per image.
@@ -494,15 +501,28 @@ __global__ __launch_bounds__(256) void k_socs_mid(const float2* __restrict__ mpa
                                                   const float2* __restrict__ twM,
                                                   float2* __restrict__ spec) {
   const int b = blockIdx.x;
-  mid_image(mpart + (size_t)b * nblk * (2 * Ny + 1) * 16, nblk, Nx, Ny, inv_area, kern, scales,
-            nk, twM, spec + (size_t)b * 32 * (4 * Nx + 1));
+  mid_image<false>(mpart + (size_t)b * nblk * (2 * Ny + 1) * 16, nblk, Nx, Ny, inv_area, kern,
+                   scales, nk, twM, spec + (size_t)b * 32 * (4 * Nx + 1), 32);
+}
+
+// FP32 single-image path: one block per image from its whole band
+// [B][2Ny+1][2Nx+1]; spec: [B][G][4Nx+1].
+__global__ __launch_bounds__(256) void k_socs_mid_band(const float2* __restrict__ band, int Nx,
+                                                       int Ny, const float2* __restrict__ kern,
+                                                       const float* __restrict__ scales, int nk,
+                                                       const float2* __restrict__ twM,
+                                                       float2* __restrict__ spec, int G) {
+  const int b = blockIdx.x;
+  mid_image<true>(band + (size_t)b * (2 * Nx + 1) * (2 * Ny + 1), 0, Nx, Ny, 1.f, kern, scales,
+                  nk, twM, spec + (size_t)b * G * (4 * Nx + 1), G);
 }
 
 // Mixed sizes and kernel sets: one block per MidJob (one mask x one set).
 __global__ __launch_bounds__(256) void k_socs_mid_jobs(const klt::MidJob* __restrict__ jobs,
                                                        const float2* __restrict__ twM) {
   const klt::MidJob j = jobs[blockIdx.x];
-  mid_image(j.mpart, j.nblk, j.Nx, j.Ny, j.inv_area, j.kern, j.scales, j.nk, twM, j.spec);
+  mid_image<false>(j.mpart, j.nblk, j.Nx, j.Ny, j.inv_area, j.kern, j.scales, j.nk, twM, j.spec,
+                   32);
 }
 
 // ---------------------------------------------------------------------------
@@ -1171,6 +1191,24 @@ int socs_forward_batch(klt_ctx* ctx, int nmask, const float* const* masks, const
     ctx->batches.erase(ctx->batches.begin());
   }
   ctx->last_batch = ctx->batches.back().get();
+  return KLT_OK;
+}
+
+int socs_mid_from_band(klt_ctx* ctx, const klt_kernels* kb, const float2* band, int B,
+                       float2* spec, int G) {
+  if (!ctx->tw64.ptr) {
+    std::vector<float2> tw(kM);
+    for (int j = 0; j < kM; ++j) {
+      const double a = 2.0 * M_PI * j / kM;
+      tw[j] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    KLT_TRY(ctx->tw64.ensure(kM * sizeof(float2)));
+    KLT_HIP(hipMemcpy(ctx->tw64.ptr, tw.data(), kM * sizeof(float2), hipMemcpyHostToDevice));
+  }
+  KLT_LAUNCH(ctx, "k_socs_mid_band", k_socs_mid_band, dim3(B), dim3(256), 0, band, kb->Nx,
+             kb->Ny, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk,
+             ctx->tw64.as<float2>(), spec, G);
+  KLT_HIP(hipGetLastError());
   return KLT_OK;
 }
 

@@ -82,20 +82,26 @@ __global__ __launch_bounds__(256) void k_split_operands(
   }
 }
 
-// Split-FP16 MFMA Gram.  One block per upper-triangle pair of 64 x 64 tiles;
-// each of the 4 waves owns a 32 x 32 quadrant (2 x 2 MFMA 16 x 16 blocks, real
-// and imaginary parts).  v_mfma_f32_16x16x32_f16 operand layout: lane l holds
-// A[m = l%16][k = 8(l/16) .. +7] and B[k = 8(l/16) .. +7][n = l%16]; the result
-// D[m = 4(l/16) + r][n = l%16], r = 0..3.
+// Split-FP16 MFMA Gram, split over the source points.  Block (tile, ks) takes
+// one upper-triangle pair of 64 x 64 tiles and the ks-th chunk of kc columns
+// of K; each of the 4 waves owns a 32 x 32 quadrant (2 x 2 MFMA 16 x 16
+// blocks, real and imaginary parts) and writes its FP32 partial sums to
+// part[ks][tile][64][64] (re, im).  v_mfma_f32_16x16x32_f16 operand layout:
+// lane l holds A[m = l%16][k = 8(l/16) .. +7] and B[k = 8(l/16) .. +7][n = l%16];
+// the result D[m = 4(l/16) + r][n = l%16], r = 0..3.  Operands load two
+// steps ahead into two register sets (the loop runs two steps per pass; Kpad
+// and kc are multiples of 64, and the loads past the chunk's end re-read its
+// last step).
 __global__ __launch_bounds__(256) void k_gram_split16(
     const _Float16* __restrict__ Uh, const _Float16* __restrict__ Ul,
     const _Float16* __restrict__ V1h, const _Float16* __restrict__ V1l,
-    const _Float16* __restrict__ V2h, const _Float16* __restrict__ V2l, int T, int Kpad,
-    const int2* __restrict__ tiles, float inv_scale, double* __restrict__ tcc) {
-  const int2 t = tiles[blockIdx.x];
+    const _Float16* __restrict__ V2h, const _Float16* __restrict__ V2l, int Kpad, int kc,
+    const int2* __restrict__ tiles, float2* __restrict__ part) {
+  const int tile = blockIdx.x, ks = blockIdx.y, ntiles = gridDim.x;
+  const int2 t = tiles[tile];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i0 = t.x * 64 + (wave >> 1) * 32;
-  const int j0 = t.y * 64 + (wave & 1) * 32;
+  const int qi = (wave >> 1) * 32, qj = (wave & 1) * 32;
+  const int i0 = t.x * 64 + qi, j0 = t.y * 64 + qj;
   const int r16 = lane & 15, kq = (lane >> 4) * 8;
   float4_t accR[2][2], accI[2][2];
 #pragma unroll
@@ -107,51 +113,93 @@ __global__ __launch_bounds__(256) void k_gram_split16(
     }
   const size_t ra0 = (size_t)(i0 + r16) * Kpad + kq, ra1 = ra0 + (size_t)16 * Kpad;
   const size_t rb0 = (size_t)(j0 + r16) * Kpad + kq, rb1 = rb0 + (size_t)16 * Kpad;
-  for (int k = 0; k < Kpad; k += 32) {
-    half8_t ah[2], al[2], bh1[2], bl1[2], bh2[2], bl2[2];
-    ah[0] = *reinterpret_cast<const half8_t*>(Uh + ra0 + k);
-    ah[1] = *reinterpret_cast<const half8_t*>(Uh + ra1 + k);
-    al[0] = *reinterpret_cast<const half8_t*>(Ul + ra0 + k);
-    al[1] = *reinterpret_cast<const half8_t*>(Ul + ra1 + k);
-    bh1[0] = *reinterpret_cast<const half8_t*>(V1h + rb0 + k);
-    bh1[1] = *reinterpret_cast<const half8_t*>(V1h + rb1 + k);
-    bl1[0] = *reinterpret_cast<const half8_t*>(V1l + rb0 + k);
-    bl1[1] = *reinterpret_cast<const half8_t*>(V1l + rb1 + k);
-    bh2[0] = *reinterpret_cast<const half8_t*>(V2h + rb0 + k);
-    bh2[1] = *reinterpret_cast<const half8_t*>(V2h + rb1 + k);
-    bl2[0] = *reinterpret_cast<const half8_t*>(V2l + rb0 + k);
-    bl2[1] = *reinterpret_cast<const half8_t*>(V2l + rb1 + k);
+  const int k0 = ks * kc, k1 = min(Kpad, k0 + kc);
+  // operands of one step: [0..1] U hi, [2..3] U lo, [4..5] V1 hi, [6..7] V1 lo,
+  // [8..9] V2 hi, [10..11] V2 lo
+  half8_t s0[12], s1[12];
+  auto load = [&](int k, half8_t* o) {
+    k = min(k, k1 - 32);
+    o[0] = *reinterpret_cast<const half8_t*>(Uh + ra0 + k);
+    o[1] = *reinterpret_cast<const half8_t*>(Uh + ra1 + k);
+    o[2] = *reinterpret_cast<const half8_t*>(Ul + ra0 + k);
+    o[3] = *reinterpret_cast<const half8_t*>(Ul + ra1 + k);
+    o[4] = *reinterpret_cast<const half8_t*>(V1h + rb0 + k);
+    o[5] = *reinterpret_cast<const half8_t*>(V1h + rb1 + k);
+    o[6] = *reinterpret_cast<const half8_t*>(V1l + rb0 + k);
+    o[7] = *reinterpret_cast<const half8_t*>(V1l + rb1 + k);
+    o[8] = *reinterpret_cast<const half8_t*>(V2h + rb0 + k);
+    o[9] = *reinterpret_cast<const half8_t*>(V2h + rb1 + k);
+    o[10] = *reinterpret_cast<const half8_t*>(V2l + rb0 + k);
+    o[11] = *reinterpret_cast<const half8_t*>(V2l + rb1 + k);
+  };
+  auto products = [&](const half8_t* c) {
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
-        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh1[b], accR[a][b], 0, 0, 0);
-        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bl1[b], accR[a][b], 0, 0, 0);
-        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], bh1[b], accR[a][b], 0, 0, 0);
-        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh2[b], accI[a][b], 0, 0, 0);
-        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bl2[b], accI[a][b], 0, 0, 0);
-        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], bh2[b], accI[a][b], 0, 0, 0);
+        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(c[a], c[4 + b], accR[a][b], 0, 0, 0);
+        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(c[a], c[6 + b], accR[a][b], 0, 0, 0);
+        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(c[2 + a], c[4 + b], accR[a][b], 0, 0, 0);
+        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(c[a], c[8 + b], accI[a][b], 0, 0, 0);
+        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(c[a], c[10 + b], accI[a][b], 0, 0, 0);
+        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(c[2 + a], c[8 + b], accI[a][b], 0, 0, 0);
       }
+  };
+  if (k0 < k1) {
+    load(k0, s0);
+    load(k0 + 32, s1);
   }
-  const bool diag_tile = t.x == t.y;
+  for (int k = k0; k < k1; k += 64) {
+    // the products read their operands at issue, so a set reloads right after
+    products(s0);
+    __builtin_amdgcn_sched_barrier(0);
+    load(k + 64, s0);
+    __builtin_amdgcn_sched_barrier(0);
+    products(s1);
+    __builtin_amdgcn_sched_barrier(0);
+    load(k + 96, s1);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  float2* out = part + ((size_t)ks * ntiles + tile) * 64 * 64;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = i0 + a * 16 + (lane >> 4) * 4 + r;
-        const int j = j0 + b * 16 + r16;
-        if (i >= T || j >= T || (diag_tile && i > j)) continue;
-        const double re = (double)accR[a][b][r] * inv_scale;
-        const double im = i == j ? 0.0 : (double)accI[a][b][r] * inv_scale;
-        tcc[2 * ((size_t)i * T + j)] = re;
-        tcc[2 * ((size_t)i * T + j) + 1] = im;
-        if (i != j) {
-          tcc[2 * ((size_t)j * T + i)] = re;
-          tcc[2 * ((size_t)j * T + i) + 1] = -im;
-        }
+        const int ii = qi + a * 16 + (lane >> 4) * 4 + r, jj = qj + b * 16 + r16;
+        out[ii * 64 + jj] = make_float2(accR[a][b][r], accI[a][b][r]);
       }
+}
+
+// Sums the nks partials of each tile in a fixed order, scales them and writes
+// both triangles of the T x T complex double TCC (imaginary diagonal zero).
+__global__ __launch_bounds__(256) void k_gram_reduce(const float2* __restrict__ part, int nks,
+                                                     const int2* __restrict__ tiles, int T,
+                                                     double inv_scale,
+                                                     double* __restrict__ tcc) {
+  const int tile = blockIdx.x, ntiles = gridDim.x;
+  const int2 t = tiles[tile];
+  const bool diag_tile = t.x == t.y;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int ii = e >> 6, jj = e & 63;
+    const int i = t.x * 64 + ii, j = t.y * 64 + jj;
+    if (i >= T || j >= T || (diag_tile && i > j)) continue;
+    double re = 0.0, im = 0.0;
+    for (int ks = 0; ks < nks; ++ks) {
+      const float2 v = part[((size_t)ks * ntiles + tile) * 64 * 64 + e];
+      re += (double)v.x;
+      im += (double)v.y;
+    }
+    re *= inv_scale;
+    im = i == j ? 0.0 : im * inv_scale;
+    tcc[2 * ((size_t)i * T + j)] = re;
+    tcc[2 * ((size_t)i * T + j) + 1] = im;
+    if (i != j) {
+      tcc[2 * ((size_t)j * T + i)] = re;
+      tcc[2 * ((size_t)j * T + i) + 1] = -im;
+    }
+  }
 }
 
 // FP64 reference Gram: 32 x 32 upper-triangle output tiles, source points
@@ -292,7 +340,7 @@ int tcc_build_device(klt_ctx* ctx, const float* src_host, int srcSize, float out
   int e = 0;
   (void)std::frexp(smax, &e);
   const float scale = std::ldexp(1.0f, -e);
-  const int Kpad = round_up(2 * ns, 32), Tpad = round_up(T, 64);
+  const int Kpad = round_up(2 * ns, 64), Tpad = round_up(T, 64);
   const size_t nel = (size_t)Tpad * Kpad;
   KLT_TRY(ctx->qpart.ensure(6 * nel * sizeof(_Float16)));
   _Float16* base = ctx->qpart.as<_Float16>();
@@ -306,9 +354,17 @@ int tcc_build_device(klt_ctx* ctx, const float* src_host, int srcSize, float out
   KLT_TRY(ctx->spec.ensure(sizeof(int2) * tiles.size()));
   KLT_HIP(hipMemcpyAsync(ctx->spec.ptr, tiles.data(), sizeof(int2) * tiles.size(),
                          hipMemcpyHostToDevice, ctx->stream));
-  KLT_LAUNCH(ctx, "k_gram_split16", k_gram_split16, dim3((unsigned)tiles.size()), dim3(256), 0,
-             base, base + nel, base + 2 * nel, base + 3 * nel, base + 4 * nel, base + 5 * nel,
-             T, Kpad, ctx->spec.as<int2>(), 1.0f / scale, tcc_dev);
+  // split K so that the grid holds at least two blocks per CU
+  const int pairs = Kpad / 64, ntl = (int)tiles.size();
+  const int nks = std::max(1, std::min(pairs, (512 + ntl - 1) / ntl));
+  const int kc = (pairs + nks - 1) / nks * 64;
+  KLT_TRY(ctx->tcc_part.ensure((size_t)nks * ntl * 64 * 64 * sizeof(float2)));
+  KLT_LAUNCH(ctx, "k_gram_split16", k_gram_split16, dim3((unsigned)ntl, (unsigned)nks),
+             dim3(256), 0, base, base + nel, base + 2 * nel, base + 3 * nel, base + 4 * nel,
+             base + 5 * nel, Kpad, kc, ctx->spec.as<int2>(), ctx->tcc_part.as<float2>());
+  KLT_LAUNCH(ctx, "k_gram_reduce", k_gram_reduce, dim3((unsigned)ntl), dim3(256), 0,
+             ctx->tcc_part.as<float2>(), nks, ctx->spec.as<int2>(), T, 1.0 / (double)scale,
+             tcc_dev);
   KLT_HIP(hipGetLastError());
   KLT_HIP(hipStreamSynchronize(ctx->stream));
   return KLT_OK;
