@@ -60,6 +60,19 @@ int klt_ctx_set_profiling(klt_ctx* ctx, int on);
  * report and clears the record; returns the byte count or an error. */
 int klt_ctx_profile_report(klt_ctx* ctx, char* buf, int len);
 
+/* Launch-shape tuning of the batched SOCS passes (no reference counterpart).
+ * klt_socs_forward and klt_socs_backward split their kernels over more blocks
+ * when a batch is too small to fill the GPU: column ranges of the band pass,
+ * row ranges of the synthesis pass, groups of kernels in the coarse-grid
+ * stage.  With tuning on (the default) the first call of each (batch, size,
+ * kernel set) runs the candidate splits once each, timed with HIP events,
+ * and keeps the fastest for later calls; off, a fixed rule decides. */
+int klt_ctx_set_autotune(klt_ctx* ctx, int on);
+/* One line per tuned shape: "socs_forward|socs_backward B Lx Ly Nx Ny nk
+ * column_splits row_splits kernel_groups ms"; returns the byte count or an
+ * error. */
+int klt_ctx_tune_report(klt_ctx* ctx, char* buf, int len);
+
 /* ---- host boundary: sources, masks, sizing, file formats ---------------
  * Host-only; bit-identical to the reference's float arithmetic. */
 /* K-Litho-TCC/source.cpp:105-142 (createSource): srcType is "Annular",

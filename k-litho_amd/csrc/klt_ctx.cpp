@@ -207,6 +207,30 @@ int klt_ctx_set_profiling(klt_ctx* ctx, int on) {
   return KLT_OK;
 }
 
+int klt_ctx_set_autotune(klt_ctx* ctx, int on) {
+  if (!ctx) return klt::set_error(KLT_ERR_ARG, "klt_ctx_set_autotune: ctx is NULL");
+  ctx->autotune = on != 0;
+  return KLT_OK;
+}
+
+int klt_ctx_tune_report(klt_ctx* ctx, char* buf, int len) {
+  if (!ctx || !buf || len <= 0)
+    return klt::set_error(KLT_ERR_ARG, "klt_ctx_tune_report: bad argument");
+  std::string out;
+  for (auto& kv : ctx->fused_tune) {
+    char line[256];
+    const auto& k = kv.first;
+    snprintf(line, sizeof line, "%s %d %d %d %d %d %d %d %d %d %.6f\n",
+             k[0] == 0 ? "socs_forward" : "socs_backward", k[1], k[2], k[3], k[4], k[5], k[6],
+             kv.second.ks, kv.second.rs, kv.second.kg, kv.second.ms);
+    out += line;
+  }
+  if ((int)out.size() + 1 > len)
+    return klt::set_error(KLT_ERR_ARG, "klt_ctx_tune_report: buffer too small");
+  std::memcpy(buf, out.c_str(), out.size() + 1);
+  return (int)out.size();
+}
+
 int klt_ctx_profile_report(klt_ctx* ctx, char* buf, int len) {
   if (!ctx || !buf || len <= 0)
     return klt::set_error(KLT_ERR_ARG, "klt_ctx_profile_report: bad argument");

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstddef>
 #include <cstdint>
 #include <map>
@@ -132,8 +133,17 @@ struct klt_ctx {
   klt::DevBuf tcc_pts;                        // source points of the TCC build
   klt::DevBuf tcc_part;                       // split-K partial sums of the TCC Gram
   klt::DevBuf tw64;                           // e^{+2 pi i j/64}, j < 64
+  klt::DevBuf ipart, gam_part;                // kernel-group partials of the coarse stages
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::SocsPlan>> socs_plans;
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::FusedTables>> fused_tables;
+  // split factors of the fused passes per launch shape (kind, batch, Lx, Ly,
+  // Nx, Ny, nk), chosen by timing on first use while `autotune` is on
+  struct Tuned {
+    int ks, rs, kg;  // band column ranges, synthesis row ranges, kernel groups
+    float ms;
+  };
+  bool autotune = true;
+  std::map<std::array<int, 7>, Tuned> fused_tune;
   // captured mixed batches, most recent last (a few kept), and the last one run
   std::vector<std::unique_ptr<klt::BatchGraph>> batches;
   klt::BatchGraph* last_batch = nullptr;

@@ -20,6 +20,8 @@
 // 64 >= 4N+1 for N <= 15, so the coarse grid is exact (no aliasing of the
 // |g| <= 2N intensity band).
 #include <algorithm>
+#include <array>
+#include <functional>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -36,6 +38,7 @@ constexpr int kRowsBand = 128;  // mask rows per band block
 constexpr int kColsSynth = 128; // image columns per synthesis block
 constexpr int kPStride = 72;    // LDS row stride (halves) of the P operand
 constexpr int kBandChunk = 4;   // 32-column steps per LDS twiddle chunk
+constexpr int kGroupSlots = 2048;  // (image, kernel group) partials of one launch
 
 __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -90,14 +93,17 @@ __device__ __forceinline__ void load8(const float* row, int x, int Lx, bool vali
 // Epilogue: out[b][blk][ny + By][nx] = sum_{y in block} R[y][nx] tw(ny, y) for
 // |ny| <= By, 0 <= nx <= Bx; tw(ny, y) = twy[y][|ny|] (conjugated for ny < 0).
 // ---------------------------------------------------------------------------
-// Body of one block: row block `blk` of one image (in0 / in1: that image;
-// out: its partials, nblk blocks x (2By+1) x ldo; lpart: its loss partials).
+// Body of one block: row block `blk` of one image over the twiddle chunks
+// [ch0, ch1) (128 columns each; the epilogue is linear in R, so blocks over
+// different column ranges give partials that simply add).  in0 / in1: that
+// image; the block writes partial `pidx` of its partials (out_img:
+// [parts][2By+1][ldo]) and of its loss partials (lpart_img).
 template <int NCB, bool RESIST, bool FULL>
 __device__ __forceinline__ void band_block(
-    int blk, const float* __restrict__ in0, const float* __restrict__ in1, float a_sig,
-    float thr, int Lx, int Ly, int Kpad, const float4* Wf, const float2* __restrict__ twy,
-    int ldtw, int Bx, int By, int ldo, float2* __restrict__ out_img,
-    float* __restrict__ lpart_img) {
+    int blk, int ch0, int ch1, int pidx, const float* __restrict__ in0,
+    const float* __restrict__ in1, float a_sig, float thr, int Lx, int Ly, int Kpad,
+    const float4* Wf, const float2* __restrict__ twy, int ldtw, int Bx, int By, int ldo,
+    float2* __restrict__ out_img, float* __restrict__ lpart_img) {
   constexpr int C = kBandChunk;
   constexpr int STEP_F4 = NCB * 2 * 64;    // 16-byte fragments per step
   constexpr int CHUNK_F4 = C * STEP_F4;    // per chunk
@@ -167,24 +173,28 @@ __device__ __forceinline__ void band_block(
       }
     }
   };
-  const int nchunk = Kpad / (32 * C);
+  const int kend = ch1 * C * 32;  // the block's last column (exclusive)
   // Prologue: the first twiddle chunk straight to LDS, then the first DEPTH
   // row steps.
+  if (ch0 < ch1) {  // (an empty range leaves zero partials)
 #pragma unroll
-  for (int i = 0; i < PER_T; ++i) sW[i * 256 + tid] = Wf[i * 256 + tid];
+    for (int i = 0; i < PER_T; ++i)
+      sW[i * 256 + tid] = Wf[(size_t)ch0 * CHUNK_F4 + i * 256 + tid];
 #pragma unroll
-  for (int d = 0; d < DEPTH; ++d) fetch_rows(32 * d, pa[d], pb[d], pta[d], ptb[d]);
+    for (int d = 0; d < DEPTH; ++d)
+      fetch_rows(ch0 * C * 32 + 32 * d, pa[d], pb[d], pta[d], ptb[d]);
+  }
   __syncthreads();
   // The next chunk's copy is spread over the steps: step s issues its piece s
   // (PS fragments a thread) before the step's rows and stores piece s - 1,
   // loaded a step earlier, so no wait for a piece drains the row pipeline.
   // (Wf is not declared restrict: as a provably invariant load the compiler
   // would rematerialize it next to its store.)
-  for (int ch = 0; ch < nchunk; ++ch) {
+  for (int ch = ch0; ch < ch1; ++ch) {
     // the last chunk reloads itself into the idle buffer
-    const float4* wnext = Wf + (size_t)min(ch + 1, nchunk - 1) * CHUNK_F4;
-    const float4* wbuf = sW + (ch & 1) * CHUNK_F4;
-    float4* wdst = sW + ((ch + 1) & 1) * CHUNK_F4;
+    const float4* wnext = Wf + (size_t)min(ch + 1, ch1 - 1) * CHUNK_F4;
+    const float4* wbuf = sW + ((ch - ch0) & 1) * CHUNK_F4;
+    float4* wdst = sW + ((ch - ch0 + 1) & 1) * CHUNK_F4;
     // the piece in flight (named registers: an array here may be moved to
     // LDS or scratch by the compiler)
     static_assert(PS == 1 || PS == 2, "one or two fragments a step");
@@ -226,7 +236,7 @@ __device__ __forceinline__ void band_block(
       stp0 = wnext[(s * PS) * 256 + tid];
       if (PS == 2) stp1 = wnext[(s * PS + 1) * 256 + tid];
       __builtin_amdgcn_sched_barrier(0);
-      fetch_rows(min(k + 32 * DEPTH, Kpad - 32), pa[sl], pb[sl], pta[sl], ptb[sl]);
+      fetch_rows(min(k + 32 * DEPTH, kend - 32), pa[sl], pb[sl], pta[sl], ptb[sl]);
       // keep the prefetch here: the scheduler would otherwise sink it to its
       // uses and expose the full memory latency every step
       __builtin_amdgcn_sched_barrier(0);
@@ -263,8 +273,8 @@ __device__ __forceinline__ void band_block(
     sTw[yy][c] = (yy < nrow && c <= By) ? twy[(size_t)(y0 + yy) * ldtw + c] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  if (RESIST && tid == 0) lpart_img[blk] = red[0] + red[1] + red[2] + red[3];
-  float2* out = out_img + (size_t)blk * Wy * ldo;
+  if (RESIST && tid == 0) lpart_img[pidx] = red[0] + red[1] + red[2] + red[3];
+  float2* out = out_img + (size_t)pidx * Wy * ldo;
   for (int o = tid; o < Wy * nxp; o += 256) {
     const int nyi = o / nxp, nx = o - nyi * nxp;
     const int ny = nyi - By;
@@ -281,18 +291,25 @@ __device__ __forceinline__ void band_block(
   }
 }
 
-// Uniform batch: grid (ceil(Ly/128), B), all images Lx x Ly.
+// Uniform batch: grid (ceil(Ly/128) * ks, B), all images Lx x Ly; block x
+// takes row block x / ks and the (x % ks)-th of ks column ranges, and writes
+// partial x of its image (ceil(Ly/128) * ks partials per image).
 template <int NCB, bool RESIST, bool FULL>
 __global__ __launch_bounds__(256, (RESIST || !FULL) ? 2 : 4) void k_band_mfma(
     const float* __restrict__ in0, const float* __restrict__ in1, float a_sig, float thr,
-    int Lx, int Ly, int Kpad, const float4* Wf, const float2* __restrict__ twy, int ldtw,
-    int Bx, int By, int ldo, float2* __restrict__ out_part, float* __restrict__ lpart) {
-  const int blk = blockIdx.x, b = blockIdx.y, nblk = gridDim.x;
+    int Lx, int Ly, int Kpad, int ks, const float4* Wf, const float2* __restrict__ twy,
+    int ldtw, int Bx, int By, int ldo, float2* __restrict__ out_part,
+    float* __restrict__ lpart) {
+  const int b = blockIdx.y, parts = gridDim.x;
+  const int blk = blockIdx.x / ks, kh = blockIdx.x - blk * ks;
+  const int nchunk = Kpad / (kBandChunk * 32), per = (nchunk + ks - 1) / ks;
+  const int ch0 = min(nchunk, kh * per), ch1 = min(nchunk, ch0 + per);
   const size_t ib = (size_t)b * Lx * Ly;
-  band_block<NCB, RESIST, FULL>(blk, in0 + ib, RESIST ? in1 + ib : nullptr, a_sig, thr, Lx, Ly,
-                                Kpad, Wf, twy, ldtw, Bx, By, ldo,
-                                out_part + (size_t)b * nblk * (2 * By + 1) * ldo,
-                                RESIST ? lpart + (size_t)b * nblk : nullptr);
+  band_block<NCB, RESIST, FULL>(blk, ch0, ch1, blockIdx.x, in0 + ib,
+                                RESIST ? in1 + ib : nullptr, a_sig, thr, Lx, Ly, Kpad, Wf, twy,
+                                ldtw, Bx, By, ldo,
+                                out_part + (size_t)b * parts * (2 * By + 1) * ldo,
+                                RESIST ? lpart + (size_t)b * parts : nullptr);
 }
 
 // Mixed sizes: one block per BandJob (a row block of one mask).
@@ -300,8 +317,9 @@ template <bool FULL>
 __global__ __launch_bounds__(256, FULL ? 4 : 2) void k_band_jobs(
     const klt::BandJob* __restrict__ jobs) {
   const klt::BandJob j = jobs[blockIdx.x];
-  band_block<2, false, FULL>(j.blk, j.in, nullptr, 0.f, 0.f, j.Lx, j.Ly, j.Kpad, j.Wf, j.twy,
-                             j.ldtw, j.Nx, j.Ny, 16, j.out, nullptr);
+  band_block<2, false, FULL>(j.blk, 0, j.Kpad / (kBandChunk * 32), j.blk, j.in, nullptr, 0.f,
+                             0.f, j.Lx, j.Ly, j.Kpad, j.Wf, j.twy, j.ldtw, j.Nx, j.Ny, 16, j.out,
+                             nullptr);
 }
 
 // 8-point DFT, X[k] = sum_n x[n] e^{s 2 pi i nk/8}, s = +1 (inverse) or -1.
@@ -363,97 +381,15 @@ __device__ __forceinline__ void fft64_group(float2* v, const float2* tw, float2*
 // Stage 2.  grid (B), 256 threads = 32 groups of 8.
 // spec: [B][32][4Nx+1], row gy in [0, 2Ny] (rows above zero), entry gx + 2Nx.
 // ---------------------------------------------------------------------------
-// Body of one block: one image.  FULL_BAND = false: mpart holds its band
-// partials (nblk blocks of the band kernel); true: mpart is its whole
-// normalised band M^[ny+Ny][nx+Nx] (the FP32 single-image path).  spec: its
-// intensity spectrum, spec_rows rows of 4Nx+1 (rows past 2Ny zero).
-template <bool FULL_BAND>
-__device__ __forceinline__ void mid_image(const float2* __restrict__ mpart, int nblk, int Nx,
-                                          int Ny, float inv_area,
-                                          const float2* __restrict__ kern,
-                                          const float* __restrict__ scales, int nk,
-                                          const float2* __restrict__ twM,
-                                          float2* __restrict__ spec, int spec_rows) {
-  __shared__ float2 sM[31 * 31];
-  __shared__ float2 sB[31 * kM];
-  __shared__ float2 sX[32 * 64];
-  const int tid = threadIdx.x;
-  const int g = tid >> 3, t = tid & 7;
-  const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
+// Forward transform of a coarse intensity held in registers (thread (g, t)
+// holds column ux = g + 32 q at rows uy = t + 8 k1, the input layout of a
+// column FFT) to its |g| <= 2N spectrum, spec_rows rows of 4Nx+1 (rows past
+// 2Ny zero).  sB: >= 31 x 64 float2 of LDS; xs: the group's FFT exchange.
+__device__ __forceinline__ void coarse_to_spec(const float (&acc)[2][8], float2* sB, float2* xs,
+                                               const float2 (&twf)[8], int Nx, int Ny,
+                                               float2* __restrict__ spec, int spec_rows) {
+  const int tid = threadIdx.x, g = tid >> 3, t = tid & 7;
   const int Gx = 4 * Nx + 1;
-  // mask-spectrum band from the per-block partials (Hermitian for nx < 0)
-  for (int i = tid; i < T; i += 256) {
-    if (FULL_BAND) {
-      sM[i] = mpart[i];
-      continue;
-    }
-    const int nyi = i / Wx, nx = i - nyi * Wx - Nx;
-    const int ny = nyi - Ny;
-    const int r = nx >= 0 ? nyi : (-ny + Ny), c = nx >= 0 ? nx : -nx;
-    float2 s = make_float2(0.f, 0.f);
-    for (int k = 0; k < nblk; ++k) {
-      const float2 p = mpart[((size_t)k * Wy + r) * 16 + c];
-      s.x += p.x;
-      s.y += p.y;
-    }
-    if (nx < 0) s.y = -s.y;
-    sM[i] = make_float2(s.x * inv_area, s.y * inv_area);
-  }
-  float2 twi[8], twf[8];
-#pragma unroll
-  for (int k2 = 0; k2 < 8; ++k2) {
-    twi[k2] = twM[(t * k2) & 63];
-    twf[k2] = make_float2(twi[k2].x, -twi[k2].y);
-  }
-  float acc[2][8];
-#pragma unroll
-  for (int q = 0; q < 2; ++q)
-#pragma unroll
-    for (int k = 0; k < 8; ++k) acc[q][k] = 0.f;
-  float2* xs = sX + g * 64;
-  __syncthreads();
-  for (int k = 0; k < nk; ++k) {
-    const float2* K = kern + (size_t)k * T;
-    const float lam = scales[k];
-    // rows: B[r][u] = sum_nx A[r][nx] e^{+2 pi i nx u / 64}
-    {
-      const bool act = g < Wy;
-      float2 v[8];
-      if (act) {
-#pragma unroll
-        for (int n2 = 0; n2 < 8; ++n2) {
-          const int n = t + 8 * n2;
-          const int nx = n < 32 ? n : n - 64;
-          float2 val = make_float2(0.f, 0.f);
-          if (nx >= -Nx && nx <= Nx) val = cmulf(K[g * Wx + nx + Nx], sM[g * Wx + nx + Nx]);
-          v[n2] = val;
-        }
-      }
-      fft64_group<1>(v, twi, xs, t, act);
-      if (act) {
-#pragma unroll
-        for (int k1 = 0; k1 < 8; ++k1) sB[g * kM + t + 8 * k1] = v[k1];
-      }
-    }
-    __syncthreads();
-    // columns: E[uy][ux] = sum_ny B[ny][ux] e^{+2 pi i ny uy / 64}
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int ux = g + 32 * q;
-      float2 v[8];
-#pragma unroll
-      for (int n2 = 0; n2 < 8; ++n2) {
-        const int n = t + 8 * n2;
-        const int ny = n < 32 ? n : n - 64;
-        v[n2] = (ny >= -Ny && ny <= Ny) ? sB[(ny + Ny) * kM + ux] : make_float2(0.f, 0.f);
-      }
-      fft64_group<1>(v, twi, xs, t, true);
-#pragma unroll
-      for (int k1 = 0; k1 < 8; ++k1) acc[q][k1] += lam * (v[k1].x * v[k1].x + v[k1].y * v[k1].y);
-    }
-  }
-  // forward transform of the coarse intensity.  Thread (g, t) holds column
-  // ux = g + 32 q at rows uy = t + 8 k1: the input layout of a column FFT.
   const int G2 = 2 * Ny + 1;
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
@@ -493,16 +429,152 @@ __device__ __forceinline__ void mid_image(const float2* __restrict__ mpart, int 
   for (int i = G2 * Gx + tid; i < spec_rows * Gx; i += 256) out[i] = make_float2(0.f, 0.f);
 }
 
-// Uniform batch: one block per image.
+// Body of one block: one image over kernels [k0, k1).  FULL_BAND = false:
+// mpart holds its band partials (nblk blocks of the band kernel); true: mpart
+// is its whole normalised band M^[ny+Ny][nx+Nx] (the FP32 single-image path).
+// With ipart, the block stores its partial coarse intensity there (64 x 64
+// floats, thread-major) for k_socs_mid_fin; else spec receives the intensity
+// spectrum, spec_rows rows of 4Nx+1 (rows past 2Ny zero).
+template <bool FULL_BAND>
+__device__ __forceinline__ void mid_image(const float2* __restrict__ mpart, int nblk, int Nx,
+                                          int Ny, float inv_area,
+                                          const float2* __restrict__ kern,
+                                          const float* __restrict__ scales, int k0, int k1,
+                                          const float2* __restrict__ twM,
+                                          float2* __restrict__ spec, int spec_rows,
+                                          float* __restrict__ ipart) {
+  __shared__ float2 sM[31 * 31];
+  __shared__ float2 sB[31 * kM];
+  __shared__ float2 sX[32 * 64];
+  const int tid = threadIdx.x;
+  const int g = tid >> 3, t = tid & 7;
+  const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
+  // mask-spectrum band from the per-block partials (Hermitian for nx < 0)
+  for (int i = tid; i < T; i += 256) {
+    if (FULL_BAND) {
+      sM[i] = mpart[i];
+      continue;
+    }
+    const int nyi = i / Wx, nx = i - nyi * Wx - Nx;
+    const int ny = nyi - Ny;
+    const int r = nx >= 0 ? nyi : (-ny + Ny), c = nx >= 0 ? nx : -nx;
+    float2 s = make_float2(0.f, 0.f);
+    for (int k = 0; k < nblk; ++k) {
+      const float2 p = mpart[((size_t)k * Wy + r) * 16 + c];
+      s.x += p.x;
+      s.y += p.y;
+    }
+    if (nx < 0) s.y = -s.y;
+    sM[i] = make_float2(s.x * inv_area, s.y * inv_area);
+  }
+  float2 twi[8], twf[8];
+#pragma unroll
+  for (int k2 = 0; k2 < 8; ++k2) {
+    twi[k2] = twM[(t * k2) & 63];
+    twf[k2] = make_float2(twi[k2].x, -twi[k2].y);
+  }
+  float acc[2][8];
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[q][k] = 0.f;
+  float2* xs = sX + g * 64;
+  __syncthreads();
+  for (int k = k0; k < k1; ++k) {
+    const float2* K = kern + (size_t)k * T;
+    const float lam = scales[k];
+    // rows: B[r][u] = sum_nx A[r][nx] e^{+2 pi i nx u / 64}
+    {
+      const bool act = g < Wy;
+      float2 v[8];
+      if (act) {
+#pragma unroll
+        for (int n2 = 0; n2 < 8; ++n2) {
+          const int n = t + 8 * n2;
+          const int nx = n < 32 ? n : n - 64;
+          float2 val = make_float2(0.f, 0.f);
+          if (nx >= -Nx && nx <= Nx) val = cmulf(K[g * Wx + nx + Nx], sM[g * Wx + nx + Nx]);
+          v[n2] = val;
+        }
+      }
+      fft64_group<1>(v, twi, xs, t, act);
+      if (act) {
+#pragma unroll
+        for (int k1 = 0; k1 < 8; ++k1) sB[g * kM + t + 8 * k1] = v[k1];
+      }
+    }
+    __syncthreads();
+    // columns: E[uy][ux] = sum_ny B[ny][ux] e^{+2 pi i ny uy / 64}
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int ux = g + 32 * q;
+      float2 v[8];
+#pragma unroll
+      for (int n2 = 0; n2 < 8; ++n2) {
+        const int n = t + 8 * n2;
+        const int ny = n < 32 ? n : n - 64;
+        v[n2] = (ny >= -Ny && ny <= Ny) ? sB[(ny + Ny) * kM + ux] : make_float2(0.f, 0.f);
+      }
+      fft64_group<1>(v, twi, xs, t, true);
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) acc[q][k1] += lam * (v[k1].x * v[k1].x + v[k1].y * v[k1].y);
+    }
+  }
+  if (ipart) {  // a kernel group's partial coarse intensity
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) ipart[tid * 16 + q * 8 + k1] = acc[q][k1];
+    return;
+  }
+  coarse_to_spec(acc, sB, xs, twf, Nx, Ny, spec, spec_rows);
+}
+
+// Uniform batch: grid (kg, B); block (j, b) takes the j-th of kg groups of
+// the kernels.  kg = 1 writes the spectrum; kg > 1 writes partial coarse
+// intensities to ipart [B][kg][4096] for k_socs_mid_fin.
 __global__ __launch_bounds__(256) void k_socs_mid(const float2* __restrict__ mpart, int nblk,
                                                   int Nx, int Ny, float inv_area,
                                                   const float2* __restrict__ kern,
                                                   const float* __restrict__ scales, int nk,
                                                   const float2* __restrict__ twM,
-                                                  float2* __restrict__ spec) {
-  const int b = blockIdx.x;
+                                                  float2* __restrict__ spec,
+                                                  float* __restrict__ ipart) {
+  const int grp = blockIdx.x, kg = gridDim.x, b = blockIdx.y;
+  const int per = (nk + kg - 1) / kg, k0 = min(nk, grp * per), k1 = min(nk, k0 + per);
   mid_image<false>(mpart + (size_t)b * nblk * (2 * Ny + 1) * 16, nblk, Nx, Ny, inv_area, kern,
-                   scales, nk, twM, spec + (size_t)b * 32 * (4 * Nx + 1), 32);
+                   scales, k0, k1, twM, spec + (size_t)b * 32 * (4 * Nx + 1), 32,
+                   kg > 1 ? ipart + ((size_t)b * kg + grp) * kM * kM : nullptr);
+}
+
+// Sums the kg partial coarse intensities of each image and transforms them
+// to the intensity spectrum [B][32][4Nx+1].
+__global__ __launch_bounds__(256) void k_socs_mid_fin(const float* __restrict__ ipart, int kg,
+                                                      int Nx, int Ny,
+                                                      const float2* __restrict__ twM,
+                                                      float2* __restrict__ spec) {
+  __shared__ float2 sB[31 * kM];
+  __shared__ float2 sX[32 * 64];
+  const int b = blockIdx.x, tid = threadIdx.x, g = tid >> 3, t = tid & 7;
+  float acc[2][8];
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) acc[q][k1] = 0.f;
+  for (int j = 0; j < kg; ++j) {
+    const float* src = ipart + ((size_t)b * kg + j) * kM * kM + tid * 16;
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) acc[q][k1] += src[q * 8 + k1];
+  }
+  float2 twf[8];
+#pragma unroll
+  for (int k2 = 0; k2 < 8; ++k2) {
+    const float2 w = twM[(t * k2) & 63];
+    twf[k2] = make_float2(w.x, -w.y);
+  }
+  coarse_to_spec(acc, sB, sX + g * 64, twf, Nx, Ny, spec + (size_t)b * 32 * (4 * Nx + 1), 32);
 }
 
 // FP32 single-image path: one block per image from its whole band
@@ -514,15 +586,15 @@ __global__ __launch_bounds__(256) void k_socs_mid_band(const float2* __restrict_
                                                        float2* __restrict__ spec, int G) {
   const int b = blockIdx.x;
   mid_image<true>(band + (size_t)b * (2 * Nx + 1) * (2 * Ny + 1), 0, Nx, Ny, 1.f, kern, scales,
-                  nk, twM, spec + (size_t)b * G * (4 * Nx + 1), G);
+                  0, nk, twM, spec + (size_t)b * G * (4 * Nx + 1), G, nullptr);
 }
 
 // Mixed sizes and kernel sets: one block per MidJob (one mask x one set).
 __global__ __launch_bounds__(256) void k_socs_mid_jobs(const klt::MidJob* __restrict__ jobs,
                                                        const float2* __restrict__ twM) {
   const klt::MidJob j = jobs[blockIdx.x];
-  mid_image<false>(j.mpart, j.nblk, j.Nx, j.Ny, j.inv_area, j.kern, j.scales, j.nk, twM, j.spec,
-                   32);
+  mid_image<false>(j.mpart, j.nblk, j.Nx, j.Ny, j.inv_area, j.kern, j.scales, 0, j.nk, twM,
+                   j.spec, 32, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -538,17 +610,43 @@ __global__ __launch_bounds__(256) void k_socs_mid_jobs(const klt::MidJob* __rest
 // c_in)/L}) in the synthesis layout [32][4Nx+1] (rows gy in [0, Ny]).
 // loss[b] = sum of the per-block loss partials.
 // ---------------------------------------------------------------------------
+// Gradient spectrum J[n] = Gam[-n] + conj(Gam[n]) (times the odd-size phase)
+// in the synthesis layout [32][4Nx+1] (rows gy in [0, Ny]).
+__device__ __forceinline__ void gam_to_spec(const float2* sGam, int Nx, int Ny, int dcx, int dcy,
+                                            int Lx, int Ly, float2* __restrict__ out) {
+  const int Wx = 2 * Nx + 1, Gx = 4 * Nx + 1;
+  for (int i = threadIdx.x; i < 32 * Gx; i += 256) {
+    const int gy = i / Gx, gx = i - gy * Gx - 2 * Nx;
+    float2 j = make_float2(0.f, 0.f);
+    if (gy <= Ny && gx >= -Nx && gx <= Nx) {
+      const float2 gm = sGam[(-gy + Ny) * Wx + (-gx + Nx)];  // Gam[-n]
+      const float2 gp = sGam[(gy + Ny) * Wx + (gx + Nx)];    // Gam[n]
+      j = make_float2(gm.x + gp.x, gm.y - gp.y);
+      if (dcx | dcy) {
+        float sn, cs;
+        sincospif(2.0f * ((float)(gx * dcx) / (float)Lx + (float)(gy * dcy) / (float)Ly), &sn, &cs);
+        j = cmulf(j, make_float2(cs, sn));
+      }
+    }
+    out[i] = j;
+  }
+}
+
+// grid (kg, B): block (j, b) takes the j-th of kg groups of the kernels;
+// kg = 1 writes the gradient spectrum, kg > 1 its partial Gam to
+// gam_part [B][kg][(2Ny+1)(2Nx+1)] for k_socs_mid_bwd_fin.
 __global__ __launch_bounds__(256) void k_socs_mid_bwd(
     const float2* __restrict__ mpart, const float2* __restrict__ gpart,
     const float* __restrict__ lpart, int nblk, int Nx, int Ny, float inv_area,
     const float2* __restrict__ kern, const float* __restrict__ scales, int nk,
     const float2* __restrict__ tw64, int dcx, int dcy, int Lx, int Ly,
-    float2* __restrict__ gspec, float* __restrict__ loss) {
+    float2* __restrict__ gspec, float* __restrict__ loss, float2* __restrict__ gam_part) {
   __shared__ float2 sM[31 * 31];
   __shared__ float2 sB[61 * kM];
   __shared__ float2 sX[32 * 64];
   __shared__ float2 sGam[31 * 31];
-  const int b = blockIdx.x, tid = threadIdx.x;
+  const int b = blockIdx.y, grp = blockIdx.x, kg = gridDim.x, tid = threadIdx.x;
+  const int per = (nk + kg - 1) / kg, kbeg = min(nk, grp * per), kend = min(nk, kbeg + per);
   const int g = tid >> 3, t = tid & 7;
   const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
   const int Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
@@ -566,7 +664,7 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd(
     sM[i] = make_float2(s.x * inv_area, s.y * inv_area);
     sGam[i] = make_float2(0.f, 0.f);
   }
-  if (tid == 0) {
+  if (tid == 0 && grp == 0) {
     float l = 0.f;
     for (int k = 0; k < nblk; ++k) l += lpart[(size_t)b * nblk + k];
     loss[b] = l;
@@ -627,7 +725,7 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd(
     for (int k1 = 0; k1 < 8; ++k1) gc[q][k1] = v[k1];
   }
   const float inv = 1.0f / (float)(kM * kM);
-  for (int k = 0; k < nk; ++k) {
+  for (int k = kbeg; k < kend; ++k) {
     const float2* K = kern + (size_t)k * T;
     const float lam = scales[k];
     {
@@ -702,22 +800,32 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd(
     }
   }
   __syncthreads();
-  float2* out = gspec + (size_t)b * 32 * Gx;
-  for (int i = tid; i < 32 * Gx; i += 256) {
-    const int gy = i / Gx, gx = i - gy * Gx - 2 * Nx;
-    float2 j = make_float2(0.f, 0.f);
-    if (gy <= Ny && gx >= -Nx && gx <= Nx) {
-      const float2 gm = sGam[(-gy + Ny) * Wx + (-gx + Nx)];  // Gam[-n]
-      const float2 gp = sGam[(gy + Ny) * Wx + (gx + Nx)];    // Gam[n]
-      j = make_float2(gm.x + gp.x, gm.y - gp.y);
-      if (dcx | dcy) {
-        float sn, cs;
-        sincospif(2.0f * ((float)(gx * dcx) / (float)Lx + (float)(gy * dcy) / (float)Ly), &sn, &cs);
-        j = cmulf(j, make_float2(cs, sn));
-      }
-    }
-    out[i] = j;
+  if (kg > 1) {
+    float2* dst = gam_part + ((size_t)b * kg + grp) * T;
+    for (int i = tid; i < T; i += 256) dst[i] = sGam[i];
+    return;
   }
+  gam_to_spec(sGam, Nx, Ny, dcx, dcy, Lx, Ly, gspec + (size_t)b * 32 * Gx);
+}
+
+// Sums the kg partial Gam of each image and writes its gradient spectrum.
+__global__ __launch_bounds__(256) void k_socs_mid_bwd_fin(const float2* __restrict__ gam_part,
+                                                          int kg, int Nx, int Ny, int dcx,
+                                                          int dcy, int Lx, int Ly,
+                                                          float2* __restrict__ gspec) {
+  __shared__ float2 sGam[31 * 31];
+  const int b = blockIdx.x, T = (2 * Nx + 1) * (2 * Ny + 1);
+  for (int i = threadIdx.x; i < T; i += 256) {
+    float2 v = make_float2(0.f, 0.f);
+    for (int j = 0; j < kg; ++j) {
+      const float2 p = gam_part[((size_t)b * kg + j) * T + i];
+      v.x += p.x;
+      v.y += p.y;
+    }
+    sGam[i] = v;
+  }
+  __syncthreads();
+  gam_to_spec(sGam, Nx, Ny, dcx, dcy, Lx, Ly, gspec + (size_t)b * 32 * (4 * Nx + 1));
 }
 
 // ---------------------------------------------------------------------------
@@ -728,9 +836,10 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd(
 // for the 64 columns (w & 1) of the strip, K = 64.
 // TY: [Lypad][64] hi/lo, TY[y][2gy] = w_gy cos, TY[y][2gy+1] = -w_gy sin.
 // ---------------------------------------------------------------------------
-// Body of one block: the 128-column strip at x0 of one image (spec: its
-// intensity spectrum; img: the image).
-__device__ __forceinline__ void synth_strip(int x0, const float2* __restrict__ spec, int Nx,
+// Body of one block: rows [y0, y1) (multiples of 32) of the 128-column strip
+// at x0 of one image (spec: its intensity spectrum; img: the image).
+__device__ __forceinline__ void synth_strip(int x0, int y0, int y1,
+                                            const float2* __restrict__ spec, int Nx,
                                             int Lx, int Ly, int Lypad,
                                             const float2* __restrict__ twx,
                                             const _Float16* __restrict__ TYh,
@@ -816,18 +925,18 @@ __device__ __forceinline__ void synth_strip(int x0, const float2* __restrict__ s
   // software pipeline over row blocks: the next block's TY operands load
   // while this block's products and stores run
   half8_t nh0, nh1, nl0, nl1;
-  {
-    const int y1 = rp * 16;
-    const _Float16* th = TYh + (size_t)(y1 + r16) * 64 + kq;
-    const _Float16* tl = TYl + (size_t)(y1 + r16) * 64 + kq;
+  if (y0 + rp * 16 < y1) {
+    const int yf = y0 + rp * 16;
+    const _Float16* th = TYh + (size_t)(yf + r16) * 64 + kq;
+    const _Float16* tl = TYl + (size_t)(yf + r16) * 64 + kq;
     nh0 = *reinterpret_cast<const half8_t*>(th);
     nh1 = *reinterpret_cast<const half8_t*>(th + 32);
     nl0 = *reinterpret_cast<const half8_t*>(tl);
     nl1 = *reinterpret_cast<const half8_t*>(tl + 32);
   }
-  for (int yb = rp * 16; yb < Lypad; yb += 32) {
+  for (int yb = y0 + rp * 16; yb < y1; yb += 32) {
     const half8_t ah0 = nh0, ah1 = nh1, al0 = nl0, al1 = nl1;
-    if (yb + 32 < Lypad) {
+    if (yb + 32 < y1) {
       const _Float16* th = TYh + (size_t)(yb + 32 + r16) * 64 + kq;
       const _Float16* tl = TYl + (size_t)(yb + 32 + r16) * 64 + kq;
       nh0 = *reinterpret_cast<const half8_t*>(th);
@@ -853,16 +962,19 @@ __device__ __forceinline__ void synth_strip(int x0, const float2* __restrict__ s
   }
 }
 
-// Uniform batch: grid (ceil(Lx/128), B).
+// Uniform batch: grid (ceil(Lx/128) * rs, B); block x takes strip x / rs and
+// the (x % rs)-th of rs row ranges.
 __global__ __launch_bounds__(256, 4) void k_synth_mfma(const float2* __restrict__ spec, int Nx,
-                                                    int Lx, int Ly, int Lypad,
+                                                    int Lx, int Ly, int Lypad, int rs,
                                                     const float2* __restrict__ twx,
                                                     const _Float16* __restrict__ TYh,
                                                     const _Float16* __restrict__ TYl,
                                                     float* __restrict__ image) {
-  const int b = blockIdx.y;
-  synth_strip(blockIdx.x * kColsSynth, spec + (size_t)b * 32 * (4 * Nx + 1), Nx, Lx, Ly, Lypad,
-              twx, TYh, TYl, image + (size_t)b * Lx * Ly);
+  const int b = blockIdx.y, strip = blockIdx.x / rs, rh = blockIdx.x - strip * rs;
+  const int per = (Lypad / 32 + rs - 1) / rs * 32;
+  const int y0 = min(Lypad, rh * per), y1 = min(Lypad, y0 + per);
+  synth_strip(strip * kColsSynth, y0, y1, spec + (size_t)b * 32 * (4 * Nx + 1), Nx, Lx, Ly,
+              Lypad, twx, TYh, TYl, image + (size_t)b * Lx * Ly);
 }
 
 // Mixed sizes and kernel sets: one block per SynthJob (one strip of one
@@ -872,7 +984,7 @@ __global__ __launch_bounds__(256) void k_synth_jobs(const klt::SynthJob* __restr
                                                     unsigned* __restrict__ strips_done,
                                                     unsigned long long* __restrict__ done_at) {
   const klt::SynthJob j = jobs[blockIdx.x];
-  synth_strip(j.x0, j.spec, j.Nx, j.Lx, j.Ly, j.Lypad, j.twx, j.TYh, j.TYl, j.img);
+  synth_strip(j.x0, 0, j.Lypad, j.spec, j.Nx, j.Lx, j.Ly, j.Lypad, j.twx, j.TYh, j.TYl, j.img);
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
@@ -905,19 +1017,20 @@ namespace klt {
 // Band launch: the branch-free vector-load variant needs 32-column tiles and
 // 16-byte-aligned inputs.
 template <int NCB, bool RESIST>
-static void launch_band(klt_ctx* ctx, const char* name, dim3 grid, const float* in0,
-                        const float* in1, float a_sig, float thr, int Lx, int Ly, int Kpad,
-                        const DevBuf& Wf, const float2* twy, int ldtw, int Bx, int By, int ldo,
-                        float2* out, float* lpart) {
+static void launch_band(klt_ctx* ctx, const char* name, int nblk, int ks, int B,
+                        const float* in0, const float* in1, float a_sig, float thr, int Lx,
+                        int Ly, int Kpad, const DevBuf& Wf, const float2* twy, int ldtw, int Bx,
+                        int By, int ldo, float2* out, float* lpart) {
   const bool full = Lx % 32 == 0 && (reinterpret_cast<uintptr_t>(in0) & 15) == 0 &&
                     (!RESIST || (reinterpret_cast<uintptr_t>(in1) & 15) == 0);
   const float4* w = Wf.as<float4>();
+  const dim3 grid(nblk * ks, B);
   if (full)
     KLT_LAUNCH(ctx, name, (k_band_mfma<NCB, RESIST, true>), grid, dim3(256), 0, in0, in1, a_sig,
-               thr, Lx, Ly, Kpad, w, twy, ldtw, Bx, By, ldo, out, lpart);
+               thr, Lx, Ly, Kpad, ks, w, twy, ldtw, Bx, By, ldo, out, lpart);
   else
     KLT_LAUNCH(ctx, name, (k_band_mfma<NCB, RESIST, false>), grid, dim3(256), 0, in0, in1, a_sig,
-               thr, Lx, Ly, Kpad, w, twy, ldtw, Bx, By, ldo, out, lpart);
+               thr, Lx, Ly, Kpad, ks, w, twy, ldtw, Bx, By, ldo, out, lpart);
 }
 
 // Row twiddles w(n, x) (rows n < 16 ncb, columns x < Kpad) in the band
@@ -1014,6 +1127,81 @@ static int get_fused_tables(klt_ctx* ctx, SocsPlan* p, FusedTables** out) {
 
 bool socs_fused_supported(const klt_kernels* kb) { return kb->Nx <= 15 && kb->Ny <= 15; }
 
+// Split factors of the streaming kernels: ks column ranges per band row
+// block, rs row ranges per synthesis strip (more blocks for small batches).
+static bool split_ok(int units, int parts) {
+  return parts >= 1 && (parts - 1) * ((units + parts - 1) / parts) < units;
+}
+
+// Picks the split factors for one launch shape: ks column ranges per band
+// row block, rs row ranges per synthesis strip, kg kernel groups per image in
+// the coarse-grid stage.  With tuning on (the default) the first call of a
+// shape runs `run` once per candidate on the context stream, timed with HIP
+// events, one factor at a time, and keeps the fastest; with tuning off, or
+// while the stream is being captured or launches are being profiled, the
+// smallest factors that give each grid two blocks per CU (one for kg).
+static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, int Ny, int nk,
+                      int nchunk, int nblk, int nstrips, int nrow32,
+                      const std::function<int(const klt_ctx::Tuned&)>& run,
+                      klt_ctx::Tuned* out) {
+  const std::array<int, 7> key{kind, nb, Lx, Ly, Nx, Ny, nk};
+  auto it = ctx->fused_tune.find(key);
+  if (it != ctx->fused_tune.end()) {
+    *out = it->second;
+    return KLT_OK;
+  }
+  klt_ctx::Tuned h{1, 1, 1, 0.f};
+  while (h.ks < 4 && nblk * h.ks * nb < 512 && split_ok(nchunk, 2 * h.ks)) h.ks *= 2;
+  while (h.rs < 4 && nstrips * h.rs * nb < 512 && split_ok(nrow32, 2 * h.rs)) h.rs *= 2;
+  auto kg_ok = [&](int v) { return v <= nk && (v == 1 || v * nb <= kGroupSlots); };
+  while (h.kg < 8 && h.kg * nb < 256 && kg_ok(2 * h.kg)) h.kg *= 2;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  KLT_HIP(hipStreamIsCapturing(ctx->stream, &cap));
+  if (!ctx->autotune || ctx->profiling || cap != hipStreamCaptureStatusNone) {
+    *out = h;
+    return KLT_OK;
+  }
+  hipEvent_t e0 = ctx->take_event(), e1 = ctx->take_event();
+  auto time_run = [&](const klt_ctx::Tuned& c, float* ms) -> int {
+    KLT_HIP(hipEventRecord(e0, ctx->stream));
+    KLT_TRY(run(c));
+    KLT_HIP(hipEventRecord(e1, ctx->stream));
+    KLT_HIP(hipEventSynchronize(e1));
+    KLT_HIP(hipEventElapsedTime(ms, e0, e1));
+    return KLT_OK;
+  };
+  KLT_TRY(run(h));  // warm-up (first launches of this shape)
+  klt_ctx::Tuned best = h;
+  KLT_TRY(time_run(best, &best.ms));
+  float ms = 0.f;
+  for (int v = 1; v <= 4; v *= 2) {
+    if (v == best.ks || !split_ok(nchunk, v)) continue;
+    klt_ctx::Tuned c = best;
+    c.ks = v;
+    KLT_TRY(time_run(c, &ms));
+    if (ms < best.ms) best = c, best.ms = ms;
+  }
+  for (int v = 1; v <= 4; v *= 2) {
+    if (v == best.rs || !split_ok(nrow32, v)) continue;
+    klt_ctx::Tuned c = best;
+    c.rs = v;
+    KLT_TRY(time_run(c, &ms));
+    if (ms < best.ms) best = c, best.ms = ms;
+  }
+  for (int v = 1; v <= 8; v *= 2) {
+    if (v == best.kg || !kg_ok(v)) continue;
+    klt_ctx::Tuned c = best;
+    c.kg = v;
+    KLT_TRY(time_run(c, &ms));
+    if (ms < best.ms) best = c, best.ms = ms;
+  }
+  ctx->event_pool.push_back(e0);
+  ctx->event_pool.push_back(e1);
+  ctx->fused_tune[key] = best;
+  *out = best;
+  return KLT_OK;
+}
+
 int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int B, int Lx,
                        int Ly, float* images) {
   SocsPlan* p = nullptr;
@@ -1022,26 +1210,38 @@ int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, 
   KLT_TRY(get_fused_tables(ctx, p, &ft));
   const int Nx = kb->Nx, Ny = kb->Ny, Wy = 2 * Ny + 1, Gx = 4 * Nx + 1;
   const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
+  const int nstrips = (Lx + kColsSynth - 1) / kColsSynth;
   const int SB = std::min(B, 1024);
-  KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk * Wy * 16 * sizeof(float2)));
+  KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk * 4 * Wy * 16 * sizeof(float2)));
   KLT_TRY(ctx->spec.ensure((size_t)SB * 32 * Gx * sizeof(float2)));
+  KLT_TRY(ctx->ipart.ensure((size_t)kGroupSlots * kM * kM * sizeof(float)));
   const float inv_area = 1.0f / ((float)Lx * (float)Ly);
   for (int b0 = 0; b0 < B; b0 += SB) {
     const int nb = std::min(SB, B - b0);
     const float* m = masks + (size_t)b0 * Lx * Ly;
     float* out = images + (size_t)b0 * Lx * Ly;
-    launch_band<2, false>(ctx, "k_band_mfma", dim3(nblk, nb), m, nullptr, 0.f, 0.f, Lx, Ly,
-                          ft->Kpad, ft->Wf,
-                          p->twy_in.as<float2>(), p->NP, Nx, Ny, 16, ctx->qpart.as<float2>(),
-                          nullptr);
-    KLT_LAUNCH(ctx, "k_socs_mid", k_socs_mid, dim3(nb), dim3(256), 0, ctx->qpart.as<float2>(),
-               nblk, Nx, Ny, inv_area, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk,
-               ft->tw64.as<float2>(), ctx->spec.as<float2>());
-    KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma,
-               dim3((Lx + kColsSynth - 1) / kColsSynth, nb), dim3(256), 0,
-               ctx->spec.as<float2>(), Nx, Lx, Ly, ft->Lypad, p->twx_out.as<float2>(),
-               ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), out);
-    KLT_HIP(hipGetLastError());
+    auto run = [&](const klt_ctx::Tuned& c) -> int {
+      launch_band<2, false>(ctx, "k_band_mfma", nblk, c.ks, nb, m, nullptr, 0.f, 0.f, Lx, Ly,
+                            ft->Kpad, ft->Wf, p->twy_in.as<float2>(), p->NP, Nx, Ny, 16,
+                            ctx->qpart.as<float2>(), nullptr);
+      KLT_LAUNCH(ctx, "k_socs_mid", k_socs_mid, dim3(c.kg, nb), dim3(256), 0,
+                 ctx->qpart.as<float2>(), nblk * c.ks, Nx, Ny, inv_area, kb->kern.as<float2>(),
+                 kb->scales.as<float>(), kb->nk, ft->tw64.as<float2>(), ctx->spec.as<float2>(),
+                 ctx->ipart.as<float>());
+      if (c.kg > 1)
+        KLT_LAUNCH(ctx, "k_socs_mid_fin", k_socs_mid_fin, dim3(nb), dim3(256), 0,
+                   ctx->ipart.as<float>(), c.kg, Nx, Ny, ft->tw64.as<float2>(),
+                   ctx->spec.as<float2>());
+      KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma, dim3(nstrips * c.rs, nb), dim3(256), 0,
+                 ctx->spec.as<float2>(), Nx, Lx, Ly, ft->Lypad, c.rs, p->twx_out.as<float2>(),
+                 ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), out);
+      KLT_HIP(hipGetLastError());
+      return KLT_OK;
+    };
+    klt_ctx::Tuned c{};
+    KLT_TRY(pick_split(ctx, 0, nb, Lx, Ly, Nx, Ny, kb->nk, ft->Kpad / (kBandChunk * 32), nblk,
+                       nstrips, ft->Lypad / 32, run, &c));
+    KLT_TRY(run(c));
   }
   return KLT_OK;
 }
@@ -1219,7 +1419,7 @@ int band_mask_partials(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const float
   FusedTables* ft = nullptr;
   KLT_TRY(get_fused_tables(ctx, p, &ft));
   const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
-  launch_band<2, false>(ctx, "k_band_mfma", dim3(nblk, B), masks, nullptr, 0.f, 0.f, Lx, Ly,
+  launch_band<2, false>(ctx, "k_band_mfma", nblk, 1, B, masks, nullptr, 0.f, 0.f, Lx, Ly,
                         ft->Kpad, ft->Wf,
                         p->twy_in.as<float2>(), p->NP, Nx, Ny, 16, out, nullptr);
   KLT_HIP(hipGetLastError());
@@ -1234,7 +1434,7 @@ int band_resist_partials(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const flo
   FusedTables* ft = nullptr;
   KLT_TRY(get_fused_tables(ctx, p, &ft));
   const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
-  launch_band<4, true>(ctx, "k_band_resist", dim3(nblk, B), images, targets, a_sig, thr, Lx, Ly,
+  launch_band<4, true>(ctx, "k_band_resist", nblk, 1, B, images, targets, a_sig, thr, Lx, Ly,
                        ft->Kpad, ft->Wgf,
                        ft->twyg.as<float2>(), 32, 2 * Nx, 2 * Ny, 32, out, lpart);
   KLT_HIP(hipGetLastError());
@@ -1250,14 +1450,16 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
   KLT_TRY(get_fused_tables(ctx, p, &ft));
   const int Nx = kb->Nx, Ny = kb->Ny, Wy = 2 * Ny + 1, Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
   const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
+  const int nstrips = (Lx + kColsSynth - 1) / kColsSynth;
   const int SB = std::min(B, 512);
-  KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk * Wy * 16 * sizeof(float2)));
+  KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk * 4 * Wy * 16 * sizeof(float2)));
   KLT_TRY(ctx->spec.ensure((size_t)SB * 32 * Gx * sizeof(float2)));
-  KLT_TRY(ctx->gpart.ensure((size_t)SB * nblk * Gy * 32 * sizeof(float2)));
+  KLT_TRY(ctx->gpart.ensure((size_t)SB * nblk * 4 * Gy * 32 * sizeof(float2)));
   KLT_TRY(ctx->gspec.ensure((size_t)SB * 32 * Gx * sizeof(float2)));
-  KLT_TRY(ctx->lpart.ensure((size_t)SB * nblk * sizeof(float)));
-  float* img = images;
-  if (!img) {
+  KLT_TRY(ctx->lpart.ensure((size_t)SB * nblk * 4 * sizeof(float)));
+  KLT_TRY(ctx->ipart.ensure((size_t)kGroupSlots * kM * kM * sizeof(float)));
+  KLT_TRY(ctx->gam_part.ensure((size_t)kGroupSlots * (2 * Nx + 1) * (2 * Ny + 1) * sizeof(float2)));
+  if (!images) {
     KLT_TRY(ctx->bwd_img.ensure((size_t)SB * Lx * Ly * sizeof(float)));
   }
   const float inv_area = 1.0f / ((float)Lx * (float)Ly);
@@ -1266,30 +1468,44 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
     const int nb = std::min(SB, B - b0);
     const size_t off = (size_t)b0 * Lx * Ly;
     float* I = images ? images + off : ctx->bwd_img.as<float>();
-    launch_band<2, false>(ctx, "k_band_mfma", dim3(nblk, nb), masks + off, nullptr, 0.f, 0.f,
-                          Lx, Ly, ft->Kpad, ft->Wf,
-                          p->twy_in.as<float2>(), p->NP, Nx, Ny, 16, ctx->qpart.as<float2>(),
-                          nullptr);
-    KLT_LAUNCH(ctx, "k_socs_mid", k_socs_mid, dim3(nb), dim3(256), 0, ctx->qpart.as<float2>(),
-               nblk, Nx, Ny, inv_area, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk,
-               ft->tw64.as<float2>(), ctx->spec.as<float2>());
-    KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma,
-               dim3((Lx + kColsSynth - 1) / kColsSynth, nb), dim3(256), 0,
-               ctx->spec.as<float2>(), Nx, Lx, Ly, ft->Lypad, p->twx_out.as<float2>(),
-               ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), I);
-    launch_band<4, true>(ctx, "k_band_resist", dim3(nblk, nb), I, targets + off, a_sig, thr, Lx,
-                         Ly, ft->Kpad, ft->Wgf,
-                         ft->twyg.as<float2>(), 32, 2 * Nx, 2 * Ny, 32, ctx->gpart.as<float2>(),
-                         ctx->lpart.as<float>());
-    KLT_LAUNCH(ctx, "k_socs_mid_bwd", k_socs_mid_bwd, dim3(nb), dim3(256), 0,
-               ctx->qpart.as<float2>(), ctx->gpart.as<float2>(), ctx->lpart.as<float>(), nblk,
-               Nx, Ny, inv_area, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk,
-               ft->tw64.as<float2>(), dcx, dcy, Lx, Ly, ctx->gspec.as<float2>(), losses + b0);
-    KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma,
-               dim3((Lx + kColsSynth - 1) / kColsSynth, nb), dim3(256), 0,
-               ctx->gspec.as<float2>(), Nx, Lx, Ly, ft->Lypad, p->twx_out.as<float2>(),
-               ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), grads + off);
-    KLT_HIP(hipGetLastError());
+    auto run = [&](const klt_ctx::Tuned& c) -> int {
+      const int ks = c.ks, rs = c.rs, parts = nblk * ks;
+      launch_band<2, false>(ctx, "k_band_mfma", nblk, ks, nb, masks + off, nullptr, 0.f, 0.f,
+                            Lx, Ly, ft->Kpad, ft->Wf, p->twy_in.as<float2>(), p->NP, Nx, Ny,
+                            16, ctx->qpart.as<float2>(), nullptr);
+      KLT_LAUNCH(ctx, "k_socs_mid", k_socs_mid, dim3(c.kg, nb), dim3(256), 0,
+                 ctx->qpart.as<float2>(), parts, Nx, Ny, inv_area, kb->kern.as<float2>(),
+                 kb->scales.as<float>(), kb->nk, ft->tw64.as<float2>(), ctx->spec.as<float2>(),
+                 ctx->ipart.as<float>());
+      if (c.kg > 1)
+        KLT_LAUNCH(ctx, "k_socs_mid_fin", k_socs_mid_fin, dim3(nb), dim3(256), 0,
+                   ctx->ipart.as<float>(), c.kg, Nx, Ny, ft->tw64.as<float2>(),
+                   ctx->spec.as<float2>());
+      KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma, dim3(nstrips * rs, nb), dim3(256), 0,
+                 ctx->spec.as<float2>(), Nx, Lx, Ly, ft->Lypad, rs, p->twx_out.as<float2>(),
+                 ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), I);
+      launch_band<4, true>(ctx, "k_band_resist", nblk, ks, nb, I, targets + off, a_sig, thr, Lx,
+                           Ly, ft->Kpad, ft->Wgf, ft->twyg.as<float2>(), 32, 2 * Nx, 2 * Ny, 32,
+                           ctx->gpart.as<float2>(), ctx->lpart.as<float>());
+      KLT_LAUNCH(ctx, "k_socs_mid_bwd", k_socs_mid_bwd, dim3(c.kg, nb), dim3(256), 0,
+                 ctx->qpart.as<float2>(), ctx->gpart.as<float2>(), ctx->lpart.as<float>(),
+                 parts, Nx, Ny, inv_area, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk,
+                 ft->tw64.as<float2>(), dcx, dcy, Lx, Ly, ctx->gspec.as<float2>(),
+                 losses + b0, ctx->gam_part.as<float2>());
+      if (c.kg > 1)
+        KLT_LAUNCH(ctx, "k_socs_mid_bwd_fin", k_socs_mid_bwd_fin, dim3(nb), dim3(256), 0,
+                   ctx->gam_part.as<float2>(), c.kg, Nx, Ny, dcx, dcy, Lx, Ly,
+                   ctx->gspec.as<float2>());
+      KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma, dim3(nstrips * rs, nb), dim3(256), 0,
+                 ctx->gspec.as<float2>(), Nx, Lx, Ly, ft->Lypad, rs, p->twx_out.as<float2>(),
+                 ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), grads + off);
+      KLT_HIP(hipGetLastError());
+      return KLT_OK;
+    };
+    klt_ctx::Tuned c{};
+    KLT_TRY(pick_split(ctx, 1, nb, Lx, Ly, Nx, Ny, kb->nk, ft->Kpad / (kBandChunk * 32), nblk,
+                       nstrips, ft->Lypad / 32, run, &c));
+    KLT_TRY(run(c));
   }
   return KLT_OK;
 }

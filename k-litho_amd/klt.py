@@ -60,6 +60,8 @@ def lib() -> C.CDLL:
         L.klt_ctx_sync.argtypes = [vp]
         L.klt_ctx_set_profiling.argtypes = [vp, i]
         L.klt_ctx_profile_report.argtypes = [vp, C.c_char_p, i]
+        L.klt_ctx_set_autotune.argtypes = [vp, i]
+        L.klt_ctx_tune_report.argtypes = [vp, C.c_char_p, i]
         f = C.c_float
         cs = C.c_char_p
         L.klt_source_create.argtypes = [fp, i, cs, f, f, f, f, i, f, f, cs]
@@ -224,6 +226,22 @@ class Context:
 
     def set_profiling(self, on: bool):
         check(lib().klt_ctx_set_profiling(self._h, 1 if on else 0))
+
+    def set_autotune(self, on: bool):
+        check(lib().klt_ctx_set_autotune(self._h, 1 if on else 0))
+
+    def tune_report(self) -> list:
+        """[(kind, B, Lx, Ly, Nx, Ny, nk, column_splits, row_splits, kernel_groups, ms)]
+        of the tuned shapes."""
+        buf = C.create_string_buffer(1 << 16)
+        n = lib().klt_ctx_tune_report(self._h, buf, len(buf))
+        if n < 0:
+            check(n)
+        out = []
+        for line in buf.value.decode().splitlines():
+            f = line.split()
+            out.append((f[0], *map(int, f[1:10]), float(f[10])))
+        return out
 
     def profile_report(self) -> dict:
         """{kernel name: (calls, total ms)} since the last report."""

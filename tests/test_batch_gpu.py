@@ -2,8 +2,9 @@
 
 Each image of a batch must equal the same mask imaged alone through
 klt_socs_forward with the same kernel set: the same per-image arithmetic runs
-in both. The batch tests therefore check:
-  * equality with the single-size calls (relative L2 <= 1e-7), and against
+in both, up to the order of the FP32 band sums when the single call splits
+its band pass over column ranges. The batch tests therefore check:
+  * agreement with the single-size calls (relative L2 <= 1e-6), and against
     the FP64 oracle (image relative L2 <= 1e-5);
   * graph replays with the program's own eager work between them: the same
     buffers refilled with new masks give the new images on every replay;
@@ -55,7 +56,7 @@ def test_batch_equals_single_calls_and_oracle(ctx):
             got = imgs[m * 3 + s].cpu().numpy()
             single = ctx.socs_forward(bank, mk.clone())
             ctx.sync()
-            assert relerr(got, single.cpu().numpy()) < 1e-7, (SIZES[m], s)
+            assert relerr(got, single.cpu().numpy()) < 1e-6, (SIZES[m], s)
             if m in (0, 4):  # one even, one odd size against FP64
                 ref = O.socs_image(mk.cpu().numpy(), kr, ki, sc, N, N)
                 assert relerr(got, ref) < 1e-5, (SIZES[m], s)
@@ -81,7 +82,7 @@ def test_batch_graph_replays_follow_new_data(ctx):
             for s, (bank, _) in enumerate(banks):
                 single = ctx.socs_forward(bank, mk.clone())
                 ctx.sync()
-                assert relerr(imgs[m * 2 + s].cpu().numpy(), single.cpu().numpy()) < 1e-7, rep
+                assert relerr(imgs[m * 2 + s].cpu().numpy(), single.cpu().numpy()) < 1e-6, rep
         lat = ctx.batch_latency()
         assert lat.shape == (len(masks) * len(banks),)
         assert np.all(np.isfinite(lat)) and np.all(lat > 0) and lat.max() < 1e6
