@@ -14,9 +14,11 @@
 // given (after createSource's normalisation), for every grid point inside the
 // unit disc (S_s = 0 included).
 //
-//   1. TCC of the source (FP64 Gram) and the Hopkins image I_A of each mask;
-//   2. mask band and resist band (G, loss) by the fused band GEMMs;
-//   3. k_abbe_assemble: per mask, M^ (2N+1)^2 and G^ (4N+1)^2 from the partials;
+//   1. TCC of the source (FP64 Gram);
+//   2. mask bands by the matrix-core band pass, M^ (2N+1)^2 per mask
+//      (k_abbe_assemble), and the Hopkins image I_A of each mask from it (one
+//      wave per spectrum entry, then the matrix-core synthesis);
+//   3. resist bands (G, loss) by the band pass and G^ (4N+1)^2 per mask;
 //   4. k_abbe_grad: one block per (source point, mask): the pupil support is
 //      compacted into LDS and the double sum runs over the nonzero pairs, in
 //      FP64.
@@ -35,16 +37,19 @@ using klt::pupil_value;
 
 constexpr int kThreads = 256;
 
+// Per mask: with mpart, M^ (2N+1)^2 from the mask-band partials; with gpart,
+// G^ (4N+1)^2 from the resist-band partials and the loss from lpart.
 __global__ __launch_bounds__(kThreads) void k_abbe_assemble(
     const float2* __restrict__ mpart, const float2* __restrict__ gpart,
     const float* __restrict__ lpart, int nblk, int Nx, int Ny, float inv_area,
     float2* __restrict__ Mfull, float2* __restrict__ Gfull, float* __restrict__ loss) {
   const int b = blockIdx.x;
   const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
-  for (int i = threadIdx.x; i < Wx * Wy; i += kThreads) {
+  for (int i = threadIdx.x; mpart && i < Wx * Wy; i += kThreads) {
     const int nyi = i / Wx, nx = i - nyi * Wx - Nx, ny = nyi - Ny;
     const int r = nx >= 0 ? nyi : (-ny + Ny), c = nx >= 0 ? nx : -nx;
     float2 s = make_float2(0.f, 0.f);
+#pragma unroll 8
     for (int k = 0; k < nblk; ++k) {
       const float2 p = mpart[(((size_t)b * nblk + k) * Wy + r) * 16 + c];
       s.x += p.x;
@@ -52,10 +57,12 @@ __global__ __launch_bounds__(kThreads) void k_abbe_assemble(
     }
     Mfull[(size_t)b * Wx * Wy + i] = make_float2(s.x * inv_area, (nx < 0 ? -s.y : s.y) * inv_area);
   }
+  if (!gpart) return;
   for (int i = threadIdx.x; i < Gx * Gy; i += kThreads) {
     const int gyi = i / Gx, gx = i - gyi * Gx - 2 * Nx, gy = gyi - 2 * Ny;
     const int r = gx >= 0 ? gyi : (-gy + 2 * Ny), c = gx >= 0 ? gx : -gx;
     float2 s = make_float2(0.f, 0.f);
+#pragma unroll 8
     for (int k = 0; k < nblk; ++k) {
       const float2 p = gpart[(((size_t)b * nblk + k) * Gy + r) * 32 + c];
       s.x += p.x;
@@ -197,28 +204,30 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
     KLT_TRY(ctx->bwd_img.ensure(bytes));
     I = ctx->bwd_img.as<float>();
   }
-  // 1. TCC and Hopkins images
+  // 1. TCC of the source
   KLT_TRY(ctx->abbe_tcc.ensure((size_t)T * T * 16));
   double* tcc = ctx->abbe_tcc.as<double>();
   KLT_TRY(klt::tcc_build_device(ctx, hsrc.data(), srcSize, sigma, lambda, NA, defocus, nmed, Lx,
                                 Ly, Nx, Ny, KLT_TCC_FP64, tcc));
-  for (int b = 0; b < nmask; ++b)
-    KLT_TRY(klt::hopkins_image_device(ctx, dm + b * npix, Lx, Ly, tcc, Nx, Ny, I + b * npix));
-  // 2. bands
+  // 2. mask bands (matrix-core band pass), then the Hopkins images from them
   KLT_TRY(ctx->qpart.ensure((size_t)nmask * nblk * Wy * 16 * sizeof(float2)));
   KLT_TRY(ctx->gpart.ensure((size_t)nmask * nblk * Gy * 32 * sizeof(float2)));
   KLT_TRY(ctx->lpart.ensure((size_t)nmask * nblk * sizeof(float)));
-  KLT_TRY(klt::band_mask_partials(ctx, Lx, Ly, Nx, Ny, dm, nmask, ctx->qpart.as<float2>()));
-  KLT_TRY(klt::band_resist_partials(ctx, Lx, Ly, Nx, Ny, I, dt, nmask, a_sig, thr,
-                                    ctx->gpart.as<float2>(), ctx->lpart.as<float>()));
-  // 3. full bands per mask
   KLT_TRY(ctx->abbe_band.ensure((size_t)nmask * (T + Gx * Gy) * sizeof(float2)));
   float2* Mfull = ctx->abbe_band.as<float2>();
   float2* Gfull = Mfull + (size_t)nmask * T;
   const float inv_area = 1.0f / ((float)Lx * (float)Ly);
+  KLT_TRY(klt::band_mask_partials(ctx, Lx, Ly, Nx, Ny, dm, nmask, ctx->qpart.as<float2>()));
   KLT_LAUNCH(ctx, "k_abbe_assemble", k_abbe_assemble, dim3(nmask), dim3(kThreads), 0,
-             ctx->qpart.as<float2>(), ctx->gpart.as<float2>(), ctx->lpart.as<float>(), nblk, Nx,
-             Ny, inv_area, Mfull, Gfull, dl);
+             ctx->qpart.as<float2>(), nullptr, nullptr, nblk, Nx, Ny, inv_area, Mfull, Gfull,
+             dl);
+  KLT_TRY(klt::hopkins_images_fused(ctx, Mfull, nmask, Lx, Ly, tcc, Nx, Ny, I));
+  // 3. resist bands and the full G^ per mask
+  KLT_TRY(klt::band_resist_partials(ctx, Lx, Ly, Nx, Ny, I, dt, nmask, a_sig, thr,
+                                    ctx->gpart.as<float2>(), ctx->lpart.as<float>()));
+  KLT_LAUNCH(ctx, "k_abbe_assemble", k_abbe_assemble, dim3(nmask), dim3(kThreads), 0, nullptr,
+             ctx->gpart.as<float2>(), ctx->lpart.as<float>(), nblk, Nx, Ny, inv_area, Mfull,
+             Gfull, dl);
   // 4. per source point
   const int sh = (int)((srcSize - 1) / 2.0);
   std::vector<int2> pts;

@@ -134,6 +134,7 @@ struct klt_ctx {
   klt::DevBuf tcc_part;                       // split-K partial sums of the TCC Gram
   klt::DevBuf tw64;                           // e^{+2 pi i j/64}, j < 64
   klt::DevBuf ipart, gam_part;                // kernel-group partials of the coarse stages
+  klt::DevBuf hspec;                          // Hopkins spectra of the source gradient
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::SocsPlan>> socs_plans;
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::FusedTables>> fused_tables;
   // split factors of the fused passes per launch shape (kind, batch, Lx, Ly,
@@ -205,6 +206,14 @@ int tcc_build_device(klt_ctx* ctx, const float* src_host, int srcSize, float out
                      int Nx, int Ny, int precision, double* tcc_dev);
 int hopkins_image_device(klt_ctx* ctx, const float* mask, int Lx, int Ly, const double* tcc,
                          int Nx, int Ny, float* image);
+// Hopkins intensity spectra of B images from their bands [B][(2Ny+1)(2Nx+1)]
+// (normalised, centred): spec [B][rows][4Nx+1] (socs.hip).
+int hopkins_spec_device(klt_ctx* ctx, const float2* band, int B, const double* tcc, int Nx,
+                        int Ny, int rows, float2* spec);
+// Hopkins images of B masks of Lx x Ly from their bands, through the batched
+// matrix-core synthesis (kernel half-size <= 15; socs_fused.hip).
+int hopkins_images_fused(klt_ctx* ctx, const float2* band, int B, int Lx, int Ly,
+                         const double* tcc, int Nx, int Ny, float* images);
 // Top-k eigenpairs of a Hermitian matrix on the device (eig.hip).
 int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_dev,
                     double* vec_dev);

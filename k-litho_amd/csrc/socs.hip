@@ -328,51 +328,63 @@ __global__ __launch_bounds__(kThreads) void k_synth_rows(const float2* __restric
 // equals synthesising the Hermitian completion J (J[g] = H[g] for g_x > 0,
 // J[-g] = conj(H[g]), J[gy][0] = (H[gy][0] + conj(H[-gy][0])) / 2), written
 // here in the half-plane layout of the synthesis passes.  FP64 sums.
+// One wave's share of H[g] = sum_n1 M^[n1 + g] conj(M^[n1]) TCC[ID(n1)][ID(n1 + g)]
+// (lanes stride over the n1 rectangle; FP64), summed over the wave.
 __device__ double2 hopkins_term(const float2* __restrict__ band,
                                 const double* __restrict__ tcc, int Nx, int Ny, int gx,
                                 int gy) {
   const int Wx = 2 * Nx + 1, T = Wx * (2 * Ny + 1);
-  double sr = 0.0, si = 0.0;
   const int y0 = max(-Ny, -Ny - gy), y1 = min(Ny, Ny - gy);
   const int x0 = max(-Nx, -Nx - gx), x1 = min(Nx, Nx - gx);
-  for (int ny = y0; ny <= y1; ++ny)
-    for (int nx = x0; nx <= x1; ++nx) {
-      const int i1 = (ny + Ny) * Wx + nx + Nx, i2 = (ny + gy + Ny) * Wx + nx + gx + Nx;
-      const float2 a = band[i2], b = band[i1];
-      // a conj(b)
-      const double pr = (double)a.x * b.x + (double)a.y * b.y;
-      const double pi = (double)a.y * b.x - (double)a.x * b.y;
-      const double tr = tcc[2 * ((size_t)i1 * T + i2)], ti = tcc[2 * ((size_t)i1 * T + i2) + 1];
-      sr += pr * tr - pi * ti;
-      si += pr * ti + pi * tr;
-    }
+  const int nxr = x1 - x0 + 1, total = (y1 - y0 + 1) * nxr;
+  double sr = 0.0, si = 0.0;
+  for (int j = threadIdx.x & 63; j < total; j += 64) {
+    const int ny = y0 + j / nxr, nx = x0 + j % nxr;
+    const int i1 = (ny + Ny) * Wx + nx + Nx, i2 = (ny + gy + Ny) * Wx + nx + gx + Nx;
+    const float2 a = band[i2], b = band[i1];
+    // a conj(b)
+    const double pr = (double)a.x * b.x + (double)a.y * b.y;
+    const double pi = (double)a.y * b.x - (double)a.x * b.y;
+    const double tr = tcc[2 * ((size_t)i1 * T + i2)], ti = tcc[2 * ((size_t)i1 * T + i2) + 1];
+    sr += pr * tr - pi * ti;
+    si += pr * ti + pi * tr;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    sr += __shfl_xor(sr, o, 64);
+    si += __shfl_xor(si, o, 64);
+  }
   return make_double2(sr, si);
 }
 
+// Hopkins intensity spectrum J[gy][gx] (gy in [0, 2Ny], gx in [-2Nx, 2Nx],
+// rows up to `rows` zero) of image blockIdx.y from its band [T]; one wave per
+// entry.  spec: [B][rows][4Nx+1].
 __global__ __launch_bounds__(kThreads) void k_hopkins_spec(const float2* __restrict__ band,
                                                            const double* __restrict__ tcc,
-                                                           int Nx, int Ny, int G,
+                                                           int Nx, int Ny, int rows,
                                                            float2* __restrict__ spec) {
-  const int Gx = 4 * Nx + 1;
-  const int o = blockIdx.x * kThreads + threadIdx.x;
-  if (o >= G * Gx) return;
+  const int Gx = 4 * Nx + 1, T = (2 * Nx + 1) * (2 * Ny + 1);
+  const int o = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6), b = blockIdx.y;
+  if (o >= rows * Gx) return;  // whole waves
   const int gy = o / Gx, gx = o - gy * Gx - 2 * Nx;
+  const float2* bd = band + (size_t)b * T;
   float2 out = make_float2(0.f, 0.f);
   if (gy <= 2 * Ny) {
     double2 j;
     if (gx > 0) {
-      j = hopkins_term(band, tcc, Nx, Ny, gx, gy);
+      j = hopkins_term(bd, tcc, Nx, Ny, gx, gy);
     } else if (gx < 0) {
-      const double2 h = hopkins_term(band, tcc, Nx, Ny, -gx, -gy);
+      const double2 h = hopkins_term(bd, tcc, Nx, Ny, -gx, -gy);
       j = make_double2(h.x, -h.y);
     } else {
-      const double2 a = hopkins_term(band, tcc, Nx, Ny, 0, gy);
-      const double2 b = hopkins_term(band, tcc, Nx, Ny, 0, -gy);
-      j = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+      const double2 a = hopkins_term(bd, tcc, Nx, Ny, 0, gy);
+      const double2 c = hopkins_term(bd, tcc, Nx, Ny, 0, -gy);
+      j = make_double2(0.5 * (a.x + c.x), 0.5 * (a.y - c.y));
     }
     out = make_float2((float)j.x, (float)j.y);
   }
-  spec[o] = out;
+  if ((threadIdx.x & 63) == 0) spec[(size_t)b * rows * Gx + o] = out;
 }
 
 __global__ void k_pack_kernels(const float* __restrict__ kr, const float* __restrict__ ki,
@@ -474,6 +486,15 @@ int socs_forward_device(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
 }
 
 // Hopkins image of one mask from a device TCC (T x T complex double).
+int hopkins_spec_device(klt_ctx* ctx, const float2* band, int B, const double* tcc, int Nx,
+                        int Ny, int rows, float2* spec) {
+  const int waves = kThreads / 64, n = rows * (4 * Nx + 1);
+  KLT_LAUNCH(ctx, "k_hopkins_spec", k_hopkins_spec, dim3((n + waves - 1) / waves, B),
+             dim3(kThreads), 0, band, tcc, Nx, Ny, rows, spec);
+  KLT_HIP(hipGetLastError());
+  return KLT_OK;
+}
+
 int hopkins_image_device(klt_ctx* ctx, const float* mask, int Lx, int Ly, const double* tcc,
                          int Nx, int Ny, float* image) {
   SocsPlan* p = nullptr;
@@ -498,8 +519,7 @@ int hopkins_image_device(klt_ctx* ctx, const float* mask, int Lx, int Ly, const 
     case 24: launch_band<24>(ctx, mask, 1, Lx, Ly, chunks, rows, p, qpart, band); break;
     default: launch_band<32>(ctx, mask, 1, Lx, Ly, chunks, rows, p, qpart, band); break;
   }
-  KLT_LAUNCH(ctx, "k_hopkins_spec", k_hopkins_spec, dim3((G * Gx + kThreads - 1) / kThreads),
-             dim3(kThreads), 0, band, tcc, Nx, Ny, G, ctx->spec.as<float2>());
+  KLT_TRY(hopkins_spec_device(ctx, band, 1, tcc, Nx, Ny, G, ctx->spec.as<float2>()));
   KLT_LAUNCH(ctx, "k_synth_cols", k_synth_cols, dim3((Lx + kThreads - 1) / kThreads, 1),
              dim3(kThreads), sizeof(float2) * G * Gx, ctx->spec.as<float2>(), G, Nx, Ny, Lx,
              p->twx_out.as<float2>(), ctx->ccols.as<float2>());

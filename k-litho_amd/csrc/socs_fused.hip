@@ -1412,6 +1412,23 @@ int socs_mid_from_band(klt_ctx* ctx, const klt_kernels* kb, const float2* band, 
   return KLT_OK;
 }
 
+int hopkins_images_fused(klt_ctx* ctx, const float2* band, int B, int Lx, int Ly,
+                         const double* tcc, int Nx, int Ny, float* images) {
+  SocsPlan* p = nullptr;
+  KLT_TRY(get_socs_plan(ctx, Lx, Ly, Nx, Ny, &p));
+  FusedTables* ft = nullptr;
+  KLT_TRY(get_fused_tables(ctx, p, &ft));
+  const int Gx = 4 * Nx + 1;
+  KLT_TRY(ctx->hspec.ensure((size_t)B * 32 * Gx * sizeof(float2)));
+  KLT_TRY(hopkins_spec_device(ctx, band, B, tcc, Nx, Ny, 32, ctx->hspec.as<float2>()));
+  KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma,
+             dim3((Lx + kColsSynth - 1) / kColsSynth, B), dim3(256), 0, ctx->hspec.as<float2>(),
+             Nx, Lx, Ly, ft->Lypad, 1, p->twx_out.as<float2>(), ft->TYh.as<_Float16>(),
+             ft->TYl.as<_Float16>(), images);
+  KLT_HIP(hipGetLastError());
+  return KLT_OK;
+}
+
 int band_mask_partials(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const float* masks, int B,
                        float2* out) {
   SocsPlan* p = nullptr;
