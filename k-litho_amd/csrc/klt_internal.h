@@ -65,6 +65,7 @@ struct SocsPlan {
 struct FusedTables {
   int Kpad = 0, Lypad = 0;
   DevBuf Wf, TYh, TYl, tw64;
+  DevBuf Ef;  // synthesis column twiddles, hi / lo fp16 in MFMA fragment order
   // adjoint: resist-gradient band rows Wgf (64 rows, centre (Lx+1)/2, fragment
   // order) and the column twiddles e^{-2 pi i gy (y - (Ly+1)/2)/Ly} (Ly x 32)
   DevBuf Wgf, twyg;
@@ -91,7 +92,7 @@ struct MidJob {         // one mask x one kernel set
 };
 struct SynthJob {       // one 128-column strip of one image
   const float2* spec;
-  const float2* twx;    // [Lx][2Nx+1]
+  const float4* Ef;     // synthesis column twiddles (fragment order)
   const _Float16* TYh;
   const _Float16* TYl;
   float* img;

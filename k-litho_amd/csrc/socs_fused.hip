@@ -841,23 +841,24 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd_fin(const float2* __restri
 __device__ __forceinline__ void synth_strip(int x0, int y0, int y1,
                                             const float2* __restrict__ spec, int Nx,
                                             int Lx, int Ly, int Lypad,
-                                            const float2* __restrict__ twx,
+                                            const float4* __restrict__ Ef,
                                             const _Float16* __restrict__ TYh,
                                             const _Float16* __restrict__ TYl,
                                             float* __restrict__ img) {
-  // the spectrum (prologue) and the split P operand (main loop) share the LDS
-  constexpr int J_BYTES = 32 * 61 * 8, P_BYTES = 2 * kColsSynth * kPStride * 2;
-  __shared__ __attribute__((aligned(16))) char smem[J_BYTES > P_BYTES ? J_BYTES : P_BYTES];
-  float2* sJ = reinterpret_cast<float2*>(smem);
+  // the split spectrum operand (prologue) and the split P operand (main
+  // loop) share the LDS
+  constexpr int B_BYTES = 2 * 16 * 64 * 16, P_BYTES = 2 * kColsSynth * kPStride * 2;
+  __shared__ __attribute__((aligned(16))) char smem[B_BYTES > P_BYTES ? B_BYTES : P_BYTES];
+  half8_t* sBh = reinterpret_cast<half8_t*>(smem);  // [4 steps][4 column tiles][64 lanes]
+  half8_t* sBl = sBh + 16 * 64;
   _Float16* sPh = reinterpret_cast<_Float16*>(smem);
   _Float16* sPl = sPh + kColsSynth * kPStride;
   const int tid = threadIdx.x;
-  const int Gx = 4 * Nx + 1, Hx = 2 * Nx + 1;
+  const int Gx = 4 * Nx + 1;
   __shared__ float red[4];
   float part = 0.f;
   for (int i = tid; i < 32 * Gx; i += 256) {
     const float2 v = spec[i];
-    sJ[i] = v;
     part += fabsf(v.x) + fabsf(v.y);
   }
   // Power-of-two scale putting max |P| <= sum |J| near 2^12, so that the lo
@@ -870,48 +871,72 @@ __device__ __forceinline__ void synth_strip(int x0, int y0, int y1,
   int ex = 0;
   (void)frexpf(bound > 0.f ? bound : 1.f, &ex);
   const float pscale = ldexpf(1.0f, 12 - ex), inv_pscale = ldexpf(1.0f, ex - 12);
+  const int wave = tid >> 6, lane = tid & 63;
+  const int r16 = lane & 15, kq = (lane >> 4) * 8;
+  // P[x][j] = sum_k E[x][k] B[k][j] on the matrix cores, K = 128: row
+  // k = 2 gxi + c of E is the cos (c = 0) / sin (c = 1) part of
+  // e^{+2 pi i gx (x - (Lx+1)/2)/Lx}, gxi = gx + 2Nx (a hi / lo table in
+  // fragment order); column j = 2 gy + d of P its real (d = 0) / imaginary
+  // part, so B[2gxi][2gy] = Jr, B[2gxi+1][2gy] = -Ji, B[2gxi][2gy+1] = Ji,
+  // B[2gxi+1][2gy+1] = Jr, scaled by pscale and split here.  Fragment
+  // (s, nt, lane) of B holds rows 32 s + 8 (lane/16) + t of column
+  // 16 nt + lane % 16.
   {
-    const int c = tid & (kColsSynth - 1), half = tid >> 7;
-    const int x = x0 + c;
-    float2 acc[16];
+    const int j = 16 * wave + r16, gy = j >> 1, d = j & 1;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) acc[q] = make_float2(0.f, 0.f);
-    if (x < Lx) {
-      // gx = +g and -g share the twiddle w (conjugated for -g):
-      // J[g] w + J[-g] conj(w); the next twiddle loads a step ahead
-      const float2* tx = twx + (size_t)x * Hx;
-      const float2* jr = sJ + half * 16 * Gx + 2 * Nx;
+    for (int st = 0; st < 4; ++st) {
+      float v[8];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) acc[q] = jr[q * Gx];
-      float2 wn = tx[Hx > 1 ? 1 : 0];
-      for (int g = 1; g <= 2 * Nx; ++g) {
-        const float2 w = wn;
-        wn = tx[g + 1 <= 2 * Nx ? g + 1 : g];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          const float2 jp = jr[q * Gx + g], jm = jr[q * Gx - g];
-          acc[q].x += (jp.x + jm.x) * w.x - (jp.y - jm.y) * w.y;
-          acc[q].y += (jp.x - jm.x) * w.y + (jp.y + jm.y) * w.x;
-        }
+      for (int t = 0; t < 8; ++t) {
+        const int k = 32 * st + kq + t, gxi = k >> 1, c = k & 1;
+        const float2 J = gxi < Gx ? spec[gy * Gx + gxi] : make_float2(0.f, 0.f);
+        v[t] = pscale * (d == 0 ? (c == 0 ? J.x : -J.y) : (c == 0 ? J.y : J.x));
       }
-    }
-    // every read of the spectrum ends before P overwrites it
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int j = 2 * (half * 16 + q);
-      acc[q].x *= pscale;
-      acc[q].y *= pscale;
-      const _Float16 hr = (_Float16)acc[q].x, hi = (_Float16)acc[q].y;
-      sPh[c * kPStride + j] = hr;
-      sPl[c * kPStride + j] = (_Float16)(acc[q].x - (float)hr);
-      sPh[c * kPStride + j + 1] = hi;
-      sPl[c * kPStride + j + 1] = (_Float16)(acc[q].y - (float)hi);
+      half8_t h, l;
+      split8(v, h, l);
+      sBh[(st * 4 + wave) * 64 + lane] = h;
+      sBl[(st * 4 + wave) * 64 + lane] = l;
     }
   }
   __syncthreads();
-  const int wave = tid >> 6, lane = tid & 63;
-  const int r16 = lane & 15, kq = (lane >> 4) * 8;
+  {
+    // wave w: columns 32 w .. +32 (two 16-column tiles), all 64 j
+    float4_t pacc[2][4];
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) pacc[m][nt] = float4_t{0.f, 0.f, 0.f, 0.f};
+    const float4* ef = Ef + (size_t)(x0 / 16 + 2 * wave) * 512;
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      half8_t ah[2], al[2];
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        ah[m] = __builtin_bit_cast(half8_t, ef[((m * 4 + st) * 2 + 0) * 64 + lane]);
+        al[m] = __builtin_bit_cast(half8_t, ef[((m * 4 + st) * 2 + 1) * 64 + lane]);
+      }
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        const half8_t bh = sBh[(st * 4 + nt) * 64 + lane], bl = sBl[(st * 4 + nt) * 64 + lane];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) pacc[m][nt] = mfma3(ah[m], al[m], bh, bl, pacc[m][nt]);
+      }
+    }
+    __syncthreads();  // every read of the B operand ends before P overwrites it
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int c = 32 * wave + 16 * m + 4 * (lane >> 4) + r, j = 16 * nt + r16;
+          const float v = pacc[m][nt][r];
+          const _Float16 h = (_Float16)v;
+          sPh[c * kPStride + j] = h;
+          sPl[c * kPStride + j] = (_Float16)(v - (float)h);
+        }
+  }
+  __syncthreads();
   const int ch = wave & 1, rp = wave >> 1;
   half8_t bh[4][2], bl[4][2];
 #pragma unroll
@@ -966,7 +991,7 @@ __device__ __forceinline__ void synth_strip(int x0, int y0, int y1,
 // the (x % rs)-th of rs row ranges.
 __global__ __launch_bounds__(256, 4) void k_synth_mfma(const float2* __restrict__ spec, int Nx,
                                                     int Lx, int Ly, int Lypad, int rs,
-                                                    const float2* __restrict__ twx,
+                                                    const float4* __restrict__ Ef,
                                                     const _Float16* __restrict__ TYh,
                                                     const _Float16* __restrict__ TYl,
                                                     float* __restrict__ image) {
@@ -974,7 +999,7 @@ __global__ __launch_bounds__(256, 4) void k_synth_mfma(const float2* __restrict_
   const int per = (Lypad / 32 + rs - 1) / rs * 32;
   const int y0 = min(Lypad, rh * per), y1 = min(Lypad, y0 + per);
   synth_strip(strip * kColsSynth, y0, y1, spec + (size_t)b * 32 * (4 * Nx + 1), Nx, Lx, Ly,
-              Lypad, twx, TYh, TYl, image + (size_t)b * Lx * Ly);
+              Lypad, Ef, TYh, TYl, image + (size_t)b * Lx * Ly);
 }
 
 // Mixed sizes and kernel sets: one block per SynthJob (one strip of one
@@ -984,7 +1009,7 @@ __global__ __launch_bounds__(256) void k_synth_jobs(const klt::SynthJob* __restr
                                                     unsigned* __restrict__ strips_done,
                                                     unsigned long long* __restrict__ done_at) {
   const klt::SynthJob j = jobs[blockIdx.x];
-  synth_strip(j.x0, 0, j.Lypad, j.spec, j.Nx, j.Lx, j.Ly, j.Lypad, j.twx, j.TYh, j.TYl, j.img);
+  synth_strip(j.x0, 0, j.Lypad, j.spec, j.Nx, j.Lx, j.Ly, j.Lypad, j.Ef, j.TYh, j.TYl, j.img);
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
@@ -1094,6 +1119,36 @@ static int get_fused_tables(klt_ctx* ctx, SocsPlan* p, FusedTables** out) {
   KLT_TRY(t->TYl.ensure(l.size() * sizeof(_Float16)));
   KLT_HIP(hipMemcpy(t->TYh.ptr, h.data(), h.size() * sizeof(_Float16), hipMemcpyHostToDevice));
   KLT_HIP(hipMemcpy(t->TYl.ptr, l.data(), l.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+  // synthesis column twiddles E[x][k] (k = 2 gxi + c: cos / sin of
+  // 2 pi gx (x - (Lx+1)/2)/Lx, gx = gxi - 2Nx; zero past Lx and 4Nx+1) as hi /
+  // lo fp16 in the A-fragment order of synth_strip: tile xt of 16 columns,
+  // step s, half h, lane l: x = 16 xt + l % 16, k = 32 s + 8 (l/16) + t
+  {
+    const int Lxp = round_up(Lx, kColsSynth), Gx = 4 * Nx + 1, cox = (Lx + 1) / 2;
+    std::vector<_Float16> ef((size_t)Lxp / 16 * 512 * 8);
+    for (int xt = 0; xt < Lxp / 16; ++xt)
+      for (int st = 0; st < 4; ++st)
+        for (int ln = 0; ln < 64; ++ln)
+          for (int tt = 0; tt < 8; ++tt) {
+            const int x = 16 * xt + ln % 16, k = 32 * st + 8 * (ln / 16) + tt;
+            const int gxi = k / 2, c = k % 2;
+            double v = 0.0;
+            if (x < Lx && gxi < Gx) {
+              long long r = ((long long)(gxi - 2 * Nx) * (x - cox)) % Lx;
+              if (r < 0) r += Lx;
+              const double ang = 2.0 * M_PI * (double)r / (double)Lx;
+              v = c ? std::sin(ang) : std::cos(ang);
+            }
+            _Float16 vh, vl;
+            split_half(v, &vh, &vl);
+            const size_t o = ((((size_t)xt * 4 + st) * 2) * 64 + ln) * 8 + tt;
+            ef[o] = vh;
+            ef[o + 64 * 8] = vl;
+          }
+    KLT_TRY(t->Ef.ensure(ef.size() * sizeof(_Float16)));
+    KLT_HIP(hipMemcpy(t->Ef.ptr, ef.data(), ef.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+  }
+
   // resist-gradient band: rows n = 2 gx (cos), 2 gx + 1 (-sin), gx <= 2Nx,
   // centre (Lx+1)/2; column twiddles e^{-2 pi i gy (y - (Ly+1)/2)/Ly}, gy <= 2Ny
   {
@@ -1233,7 +1288,7 @@ int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, 
                    ctx->ipart.as<float>(), c.kg, Nx, Ny, ft->tw64.as<float2>(),
                    ctx->spec.as<float2>());
       KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma, dim3(nstrips * c.rs, nb), dim3(256), 0,
-                 ctx->spec.as<float2>(), Nx, Lx, Ly, ft->Lypad, c.rs, p->twx_out.as<float2>(),
+                 ctx->spec.as<float2>(), Nx, Lx, Ly, ft->Lypad, c.rs, ft->Ef.as<float4>(),
                  ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), out);
       KLT_HIP(hipGetLastError());
       return KLT_OK;
@@ -1294,7 +1349,7 @@ static int build_batch(klt_ctx* ctx, BatchGraph* bg, int nmask, const float* con
                           nblk, Nx, Ny, banks[s]->nk,
                           1.0f / ((float)Lx[m] * (float)Ly[m]), {0, 0, 0}});
       for (int st = 0; st < nstrips; ++st)
-        sj.push_back(SynthJob{sp, p->twx_out.as<float2>(), ft->TYh.as<_Float16>(),
+        sj.push_back(SynthJob{sp, ft->Ef.as<float4>(), ft->TYh.as<_Float16>(),
                               ft->TYl.as<_Float16>(), images[img], Nx, Lx[m], Ly[m], ft->Lypad,
                               st * kColsSynth, img, nstrips, 0});
     }
@@ -1423,7 +1478,7 @@ int hopkins_images_fused(klt_ctx* ctx, const float2* band, int B, int Lx, int Ly
   KLT_TRY(hopkins_spec_device(ctx, band, B, tcc, Nx, Ny, 32, ctx->hspec.as<float2>()));
   KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma,
              dim3((Lx + kColsSynth - 1) / kColsSynth, B), dim3(256), 0, ctx->hspec.as<float2>(),
-             Nx, Lx, Ly, ft->Lypad, 1, p->twx_out.as<float2>(), ft->TYh.as<_Float16>(),
+             Nx, Lx, Ly, ft->Lypad, 1, ft->Ef.as<float4>(), ft->TYh.as<_Float16>(),
              ft->TYl.as<_Float16>(), images);
   KLT_HIP(hipGetLastError());
   return KLT_OK;
@@ -1499,7 +1554,7 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
                    ctx->ipart.as<float>(), c.kg, Nx, Ny, ft->tw64.as<float2>(),
                    ctx->spec.as<float2>());
       KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma, dim3(nstrips * rs, nb), dim3(256), 0,
-                 ctx->spec.as<float2>(), Nx, Lx, Ly, ft->Lypad, rs, p->twx_out.as<float2>(),
+                 ctx->spec.as<float2>(), Nx, Lx, Ly, ft->Lypad, rs, ft->Ef.as<float4>(),
                  ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), I);
       launch_band<4, true>(ctx, "k_band_resist", nblk, ks, nb, I, targets + off, a_sig, thr, Lx,
                            Ly, ft->Kpad, ft->Wgf, ft->twyg.as<float2>(), 32, 2 * Nx, 2 * Ny, 32,
@@ -1514,7 +1569,7 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
                    ctx->gam_part.as<float2>(), c.kg, Nx, Ny, dcx, dcy, Lx, Ly,
                    ctx->gspec.as<float2>());
       KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma, dim3(nstrips * rs, nb), dim3(256), 0,
-                 ctx->gspec.as<float2>(), Nx, Lx, Ly, ft->Lypad, rs, p->twx_out.as<float2>(),
+                 ctx->gspec.as<float2>(), Nx, Lx, Ly, ft->Lypad, rs, ft->Ef.as<float4>(),
                  ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), grads + off);
       KLT_HIP(hipGetLastError());
       return KLT_OK;
