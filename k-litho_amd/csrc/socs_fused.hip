@@ -947,6 +947,14 @@ __device__ __forceinline__ void synth_strip(int x0, int y0, int y1,
       bh[cb][ks] = *reinterpret_cast<const half8_t*>(sPh + c * kPStride + ks * 32 + kq);
       bl[cb][ks] = *reinterpret_cast<const half8_t*>(sPl + c * kPStride + ks * 32 + kq);
     }
+  // P now lives in registers: its LDS becomes each wave's output staging
+  // (16 rows x 64 columns, rows padded to 68 floats), so that the image is
+  // written 4 rows x 256 contiguous bytes per store
+  __syncthreads();
+  constexpr int kStg = 68;
+  float* stg = reinterpret_cast<float*>(smem) + wave * 16 * kStg;
+  const int xs0 = x0 + ch * 64;
+  const bool vec = (Lx & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 15) == 0;
   // software pipeline over row blocks: the next block's TY operands load
   // while this block's products and stores run
   half8_t nh0, nh1, nl0, nl1;
@@ -969,19 +977,32 @@ __device__ __forceinline__ void synth_strip(int x0, int y0, int y1,
       nl0 = *reinterpret_cast<const half8_t*>(tl);
       nl1 = *reinterpret_cast<const half8_t*>(tl + 32);
     }
-    // lane l holds rows y = 4 (l / 16) + r of column x = l % 16: each store
-    // writes 64 contiguous bytes in each of four rows (measured faster here
-    // than the transposed orientation's 16-byte writes to sixteen rows)
+    // lane l holds rows 4 (l / 16) + r of column cb * 16 + l % 16 of the tile
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) {
       float4_t d = float4_t{0.f, 0.f, 0.f, 0.f};
       d = mfma3(ah0, al0, bh[cb][0], bl[cb][0], d);
       d = mfma3(ah1, al1, bh[cb][1], bl[cb][1], d);
-      const int x = x0 + ch * 64 + cb * 16 + r16;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int y = yb + (lane >> 4) * 4 + r;
-        if (y < Ly && x < Lx) img[(size_t)y * Lx + x] = d[r] * inv_pscale;
+      for (int r = 0; r < 4; ++r)
+        stg[((lane >> 4) * 4 + r) * kStg + cb * 16 + r16] = d[r] * inv_pscale;
+    }
+    // (the wave's LDS accesses complete in order: the reads below see the
+    // writes above, and the next block's writes follow these reads)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int row = q * 4 + (lane >> 4), col = r16 * 4;
+      const float4 v = *reinterpret_cast<const float4*>(stg + row * kStg + col);
+      const int y = yb + row, x = xs0 + col;
+      if (y >= Ly) continue;
+      float* dst = img + (size_t)y * Lx + x;
+      if (vec) {
+        if (x < Lx) *reinterpret_cast<float4*>(dst) = v;
+      } else {
+        if (x < Lx) dst[0] = v.x;
+        if (x + 1 < Lx) dst[1] = v.y;
+        if (x + 2 < Lx) dst[2] = v.z;
+        if (x + 3 < Lx) dst[3] = v.w;
       }
     }
   }
