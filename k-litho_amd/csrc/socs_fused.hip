@@ -31,6 +31,7 @@
 namespace {
 
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kM = 64;        // coarse grid
@@ -39,6 +40,13 @@ constexpr int kColsSynth = 128; // image columns per synthesis block
 constexpr int kPStride = 72;    // LDS row stride (halves) of the P operand
 constexpr int kBandChunk = 4;   // 32-column steps per LDS twiddle chunk
 constexpr int kGroupSlots = 2048;  // (image, kernel group) partials of one launch
+// LDS strides of the coarse-grid stage, in float2: 72 = 8 (mod 16) puts the
+// 64 lanes of a wave on distinct banks (two dwords each) for both the row
+// (sB[g][t + 8 k]) and the column (sB[t + 8 n][g]) accesses of a 64 x 64 grid;
+// the FFT scratch of a group holds its 8 x 8 exchange with row stride 9, the
+// groups 72 apart.
+constexpr int kSB = 72;
+constexpr int kXS = 72;
 
 __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -58,6 +66,25 @@ __device__ __forceinline__ float4_t mfma3(half8_t ah, half8_t al, half8_t bh, ha
   c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, c, 0, 0, 0);
   c = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, c, 0, 0, 0);
   c = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, c, 0, 0, 0);
+  return c;
+}
+
+__device__ __forceinline__ void split4(const float* v, half4_t& hi, half4_t& lo) {
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const _Float16 h = (_Float16)v[t];
+    hi[t] = h;
+    lo[t] = (_Float16)(v[t] - (float)h);
+  }
+}
+
+// the same three products on v_mfma_f32_16x16x16_f16 (K = 16; lane l holds
+// A[l % 16][4 (l / 16) + t] and B[4 (l / 16) + t][l % 16])
+__device__ __forceinline__ float4_t mfma3k16(half4_t ah, half4_t al, half4_t bh, half4_t bl,
+                                             float4_t c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, c, 0, 0, 0);
   return c;
 }
 
@@ -91,7 +118,12 @@ __device__ __forceinline__ void load8(const float* row, int x, int Lx, bool vali
 // time, double-buffered, so the four waves share one copy and each step's
 // operands are conflict-free 1 KB LDS reads.
 // Epilogue: out[b][blk][ny + By][nx] = sum_{y in block} R[y][nx] tw(ny, y) for
-// |ny| <= By, 0 <= nx <= Bx; tw(ny, y) = twy[y][|ny|] (conjugated for ny < 0).
+// |ny| <= By, 0 <= nx <= Bx; tw(ny, y) = twy[y][|ny|] (conjugated for ny < 0),
+// By < 8 NCB.  It runs on the matrix cores too: the accumulator tiles are
+// already the B operand of v_mfma_f32_16x16x16_f16 (lane l holds rows
+// 4 (l / 16) + r of column l % 16), so each wave contracts its 32 rows with the
+// real and imaginary twiddle parts (A operand: |ny| x rows), split hi + lo like
+// the main loop, and the block sums the four waves' P tiles in LDS.
 // ---------------------------------------------------------------------------
 // Body of one block: row block `blk` of one image over the twiddle chunks
 // [ch0, ch1) (128 columns each; the epilogue is linear in R, so blocks over
@@ -109,15 +141,15 @@ __device__ __forceinline__ void band_block(
   constexpr int CHUNK_F4 = C * STEP_F4;    // per chunk
   constexpr int PER_T = CHUNK_F4 / 256;    // per thread per chunk
   constexpr int PS = PER_T / C;            // per thread per step
-  constexpr int R_BYTES = kRowsBand * (16 * NCB + 1) * 4;
-  constexpr int EPI_BYTES = R_BYTES + kRowsBand * 8 * NCB * 8;
+  constexpr int NT = NCB / 2;        // 16-row tiles of |ny| in the epilogue
+  constexpr int PST = 16 * NCB + 1;  // LDS row stride of the epilogue's P tiles
+  constexpr int EPI_BYTES = 4 * 2 * 16 * NT * PST * 4;
   constexpr int W_BYTES = 2 * CHUNK_F4 * 16;
-  // the twiddle chunks (main loop) and the epilogue's R / column twiddles
-  // share the LDS
+  // the twiddle chunks (main loop) and the epilogue's P tiles share the LDS
   __shared__ __attribute__((aligned(16))) char smem[EPI_BYTES > W_BYTES ? EPI_BYTES : W_BYTES];
   __shared__ float red[4];
-  auto sR = reinterpret_cast<float(*)[16 * NCB + 1]>(smem);
-  auto sTw = reinterpret_cast<float2(*)[8 * NCB]>(smem + R_BYTES);
+  // [wave][re / im twiddle part][|ny|][n]
+  auto sP = reinterpret_cast<float(*)[2][16 * NT][PST]>(smem);
   float4* sW = reinterpret_cast<float4*>(smem);
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -251,49 +283,100 @@ __device__ __forceinline__ void band_block(
     __syncthreads();
   }
   // (the loop's last barrier also ends every read of the twiddle chunks,
-  // whose LDS R and the column twiddles now reuse)
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int c = 0; c < NCB; ++c)
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        sR[wave * 32 + a * 16 + (lane >> 4) * 4 + r][c * 16 + r16] = acc[a][c][r];
+  // whose LDS the P tiles now reuse)
   if (RESIST) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
     if (lane == 0) red[wave] = lsum;
   }
   const int Wy = 2 * By + 1, nxp = Bx + 1;
-  const int y0 = blk * kRowsBand;
-  const int nrow = min(kRowsBand, Ly - y0);
-  // this block's column twiddles, staged once (|ny| <= By < 8 NCB)
-  for (int e = tid; e < kRowsBand * 8 * NCB; e += 256) {
-    const int yy = e / (8 * NCB), c = e - yy * (8 * NCB);
-    sTw[yy][c] = (yy < nrow && c <= By) ? twy[(size_t)(y0 + yy) * ldtw + c] : make_float2(0.f, 0.f);
+  // this lane's column twiddles: A[|ny| = 16 nt + r16][row 4 (lane / 16) + t]
+  // of row tile a (zero for rows past the image and |ny| > By)
+  float2 tw[NT][2][4];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int y = ybase + a * 16 + (lane >> 4) * 4 + t, ny = nt * 16 + r16;
+        tw[nt][a][t] = (y < Ly && ny <= By) ? twy[(size_t)y * ldtw + ny] : make_float2(0.f, 0.f);
+      }
+  // R * 2^sh has its largest magnitude (over the wave) in [2^13, 2^14) and the
+  // twiddles are scaled by 2^12: neither hi + lo half overflows FP16, and
+  // what falls below its normal range is < 2^-36 of the largest value
+  float mx = 0.f;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < NCB; ++c)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) mx = fmaxf(mx, fabsf(acc[a][c][r]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+  int ex;
+  (void)frexpf(mx, &ex);
+  const int sh = 14 - ex;
+  half4_t rh[2][NCB], rl[2][NCB];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) {
+      float v[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) v[r] = ldexpf(acc[a][c][r], sh);
+      split4(v, rh[a][c], rl[a][c]);
+    }
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    half4_t th[2][2], tl[2][2];  // [row tile][re / im]
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      float re[4], im[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        re[t] = tw[nt][a][t].x * 4096.f;
+        im[t] = tw[nt][a][t].y * 4096.f;
+      }
+      split4(re, th[a][0], tl[a][0]);
+      split4(im, th[a][1], tl[a][1]);
+    }
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) {
+      float4_t p0 = {0.f, 0.f, 0.f, 0.f}, p1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        p0 = mfma3k16(th[a][0], tl[a][0], rh[a][c], rl[a][c], p0);
+        p1 = mfma3k16(th[a][1], tl[a][1], rh[a][c], rl[a][c], p1);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = nt * 16 + (lane >> 4) * 4 + r;
+        sP[wave][0][row][c * 16 + r16] = ldexpf(p0[r], -sh - 12);
+        sP[wave][1][row][c * 16 + r16] = ldexpf(p1[r], -sh - 12);
+      }
+    }
   }
   __syncthreads();
   if (RESIST && tid == 0) lpart_img[pidx] = red[0] + red[1] + red[2] + red[3];
+  // sum the waves (fixed order): with q0 = sum Re(tw) R and q1 = sum Im(tw) R,
+  // tw R = (q0.re - q1.im) + i (q0.im + q1.re) for ny >= 0, and the conjugate
+  // twiddle of -ny gives (q0.re + q1.im) + i (q0.im - q1.re)
   float2* out = out_img + (size_t)pidx * Wy * ldo;
-  for (int o = tid; o < Wy * nxp; o += 256) {
-    const int nyi = o / nxp, nx = o - nyi * nxp;
-    const int ny = nyi - By;
-    const int any = ny < 0 ? -ny : ny;
-    float sr = 0.f, si = 0.f;
-    for (int yy = 0; yy < nrow; ++yy) {
-      float2 w = sTw[yy][any];
-      if (ny < 0) w.y = -w.y;
-      const float rr = sR[yy][2 * nx], ri = sR[yy][2 * nx + 1];
-      sr += rr * w.x - ri * w.y;
-      si += rr * w.y + ri * w.x;
+  for (int o = tid; o < (By + 1) * nxp; o += 256) {
+    const int ny = o / nxp, nx = o - ny * nxp;
+    float q0r = 0.f, q0i = 0.f, q1r = 0.f, q1i = 0.f;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      q0r += sP[w][0][ny][2 * nx];
+      q0i += sP[w][0][ny][2 * nx + 1];
+      q1r += sP[w][1][ny][2 * nx];
+      q1i += sP[w][1][ny][2 * nx + 1];
     }
-    out[nyi * ldo + nx] = make_float2(sr, si);
+    out[(By + ny) * ldo + nx] = make_float2(q0r - q1i, q0i + q1r);
+    if (ny > 0) out[(By - ny) * ldo + nx] = make_float2(q0r + q1i, q0i - q1r);
   }
 }
-
-// Uniform batch: grid (ceil(Ly/128) * ks, B), all images Lx x Ly; block x
-// takes row block x / ks and the (x % ks)-th of ks column ranges, and writes
-// partial x of its image (ceil(Ly/128) * ks partials per image).
 template <int NCB, bool RESIST, bool FULL>
 __global__ __launch_bounds__(256, (RESIST || !FULL) ? 2 : 4) void k_band_mfma(
     const float* __restrict__ in0, const float* __restrict__ in1, float a_sig, float thr,
@@ -356,25 +439,36 @@ __device__ __forceinline__ void dft8(float2* x) {
   }
 }
 
+// Orders the LDS accesses of one wave: a wave's LDS operations complete in
+// issue order, so only the compiler has to be kept from moving them across.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One 64-point transform per group of 8 threads (64 = 8 x 8).  On entry
 // thread t holds x[t + 8 n2] in v[n2]; on exit it holds X[t + 8 k1] in v[k1].
-// tw[k2] = e^{S 2 pi i t k2 / 64}.  xs: the group's 64-entry LDS scratch.
-// Every thread of the block must call it (it synchronises the block).
+// tw[k2] = e^{S 2 pi i t k2 / 64}.  xs: the group's LDS scratch (8 rows of
+// stride 9).
+// The 8 threads of a group lie in one wave, so the exchange synchronises that
+// wave only: callers order LDS data shared between waves themselves.  Every
+// thread of the wave calls it.
 template <int S>
 __device__ __forceinline__ void fft64_group(float2* v, const float2* tw, float2* xs, int t,
                                             bool active) {
   if (active) {
     dft8<S>(v);
 #pragma unroll
-    for (int k2 = 0; k2 < 8; ++k2) xs[t * 8 + k2] = cmulf(v[k2], tw[k2]);
+    for (int k2 = 0; k2 < 8; ++k2) xs[t * 9 + k2] = cmulf(v[k2], tw[k2]);
   }
-  __syncthreads();
+  wave_sync();
   if (active) {
 #pragma unroll
-    for (int n1 = 0; n1 < 8; ++n1) v[n1] = xs[n1 * 8 + t];
+    for (int n1 = 0; n1 < 8; ++n1) v[n1] = xs[n1 * 9 + t];
     dft8<S>(v);
   }
-  __syncthreads();
+  wave_sync();
 }
 
 // ---------------------------------------------------------------------------
@@ -384,13 +478,14 @@ __device__ __forceinline__ void fft64_group(float2* v, const float2* tw, float2*
 // Forward transform of a coarse intensity held in registers (thread (g, t)
 // holds column ux = g + 32 q at rows uy = t + 8 k1, the input layout of a
 // column FFT) to its |g| <= 2N spectrum, spec_rows rows of 4Nx+1 (rows past
-// 2Ny zero).  sB: >= 31 x 64 float2 of LDS; xs: the group's FFT exchange.
+// 2Ny zero).  sB: >= 31 rows of kSB float2 of LDS; xs: the group's FFT exchange.
 __device__ __forceinline__ void coarse_to_spec(const float (&acc)[2][8], float2* sB, float2* xs,
                                                const float2 (&twf)[8], int Nx, int Ny,
                                                float2* __restrict__ spec, int spec_rows) {
   const int tid = threadIdx.x, g = tid >> 3, t = tid & 7;
   const int Gx = 4 * Nx + 1;
   const int G2 = 2 * Ny + 1;
+  __syncthreads();  // every wave is done with the caller's use of sB
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     const int ux = g + 32 * q;
@@ -401,7 +496,7 @@ __device__ __forceinline__ void coarse_to_spec(const float (&acc)[2][8], float2*
 #pragma unroll
     for (int k1 = 0; k1 < 8; ++k1) {
       const int gy = t + 8 * k1;
-      if (gy < G2) sB[gy * kM + ux] = v[k1];
+      if (gy < G2) sB[gy * kSB + ux] = v[k1];
     }
   }
   __syncthreads();
@@ -413,7 +508,7 @@ __device__ __forceinline__ void coarse_to_spec(const float (&acc)[2][8], float2*
     float2 v[8];
     if (act) {
 #pragma unroll
-      for (int n2 = 0; n2 < 8; ++n2) v[n2] = sB[g * kM + t + 8 * n2];
+      for (int n2 = 0; n2 < 8; ++n2) v[n2] = sB[g * kSB + t + 8 * n2];
     }
     fft64_group<-1>(v, twf, xs, t, act);
     if (act) {
@@ -444,8 +539,8 @@ __device__ __forceinline__ void mid_image(const float2* __restrict__ mpart, int 
                                           float2* __restrict__ spec, int spec_rows,
                                           float* __restrict__ ipart) {
   __shared__ float2 sM[31 * 31];
-  __shared__ float2 sB[31 * kM];
-  __shared__ float2 sX[32 * 64];
+  __shared__ float2 sB[31 * kSB];
+  __shared__ float2 sX[32 * kXS];
   const int tid = threadIdx.x;
   const int g = tid >> 3, t = tid & 7;
   const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
@@ -478,7 +573,7 @@ __device__ __forceinline__ void mid_image(const float2* __restrict__ mpart, int 
   for (int q = 0; q < 2; ++q)
 #pragma unroll
     for (int k = 0; k < 8; ++k) acc[q][k] = 0.f;
-  float2* xs = sX + g * 64;
+  float2* xs = sX + g * kXS;
   __syncthreads();
   for (int k = k0; k < k1; ++k) {
     const float2* K = kern + (size_t)k * T;
@@ -500,7 +595,7 @@ __device__ __forceinline__ void mid_image(const float2* __restrict__ mpart, int 
       fft64_group<1>(v, twi, xs, t, act);
       if (act) {
 #pragma unroll
-        for (int k1 = 0; k1 < 8; ++k1) sB[g * kM + t + 8 * k1] = v[k1];
+        for (int k1 = 0; k1 < 8; ++k1) sB[g * kSB + t + 8 * k1] = v[k1];
       }
     }
     __syncthreads();
@@ -513,12 +608,13 @@ __device__ __forceinline__ void mid_image(const float2* __restrict__ mpart, int 
       for (int n2 = 0; n2 < 8; ++n2) {
         const int n = t + 8 * n2;
         const int ny = n < 32 ? n : n - 64;
-        v[n2] = (ny >= -Ny && ny <= Ny) ? sB[(ny + Ny) * kM + ux] : make_float2(0.f, 0.f);
+        v[n2] = (ny >= -Ny && ny <= Ny) ? sB[(ny + Ny) * kSB + ux] : make_float2(0.f, 0.f);
       }
       fft64_group<1>(v, twi, xs, t, true);
 #pragma unroll
       for (int k1 = 0; k1 < 8; ++k1) acc[q][k1] += lam * (v[k1].x * v[k1].x + v[k1].y * v[k1].y);
     }
+    __syncthreads();  // the column reads of sB end before the next kernel's rows
   }
   if (ipart) {  // a kernel group's partial coarse intensity
 #pragma unroll
@@ -553,8 +649,8 @@ __global__ __launch_bounds__(256) void k_socs_mid_fin(const float* __restrict__ 
                                                       int Nx, int Ny,
                                                       const float2* __restrict__ twM,
                                                       float2* __restrict__ spec) {
-  __shared__ float2 sB[31 * kM];
-  __shared__ float2 sX[32 * 64];
+  __shared__ float2 sB[31 * kSB];
+  __shared__ float2 sX[32 * kXS];
   const int b = blockIdx.x, tid = threadIdx.x, g = tid >> 3, t = tid & 7;
   float acc[2][8];
 #pragma unroll
@@ -574,7 +670,7 @@ __global__ __launch_bounds__(256) void k_socs_mid_fin(const float* __restrict__ 
     const float2 w = twM[(t * k2) & 63];
     twf[k2] = make_float2(w.x, -w.y);
   }
-  coarse_to_spec(acc, sB, sX + g * 64, twf, Nx, Ny, spec + (size_t)b * 32 * (4 * Nx + 1), 32);
+  coarse_to_spec(acc, sB, sX + g * kXS, twf, Nx, Ny, spec + (size_t)b * 32 * (4 * Nx + 1), 32);
 }
 
 // FP32 single-image path: one block per image from its whole band
@@ -642,8 +738,8 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd(
     const float2* __restrict__ tw64, int dcx, int dcy, int Lx, int Ly,
     float2* __restrict__ gspec, float* __restrict__ loss, float2* __restrict__ gam_part) {
   __shared__ float2 sM[31 * 31];
-  __shared__ float2 sB[61 * kM];
-  __shared__ float2 sX[32 * 64];
+  __shared__ float2 sB[61 * kSB];
+  __shared__ float2 sX[32 * kXS];
   __shared__ float2 sGam[31 * 31];
   const int b = blockIdx.y, grp = blockIdx.x, kg = gridDim.x, tid = threadIdx.x;
   const int per = (nk + kg - 1) / kg, kbeg = min(nk, grp * per), kend = min(nk, kbeg + per);
@@ -675,7 +771,7 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd(
     twi[k2] = tw64[(t * k2) & 63];
     twf[k2] = make_float2(twi[k2].x, -twi[k2].y);
   }
-  float2* xs = sX + g * 64;
+  float2* xs = sX + g * kXS;
   __syncthreads();
   // g_c rows: gy in [-2Ny, 2Ny] (row index gy + 2Ny), two rounds of 32 groups
 #pragma unroll
@@ -705,7 +801,7 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd(
     fft64_group<1>(v, twi, xs, t, act);
     if (act) {
 #pragma unroll
-      for (int k1 = 0; k1 < 8; ++k1) sB[r * kM + t + 8 * k1] = v[k1];
+      for (int k1 = 0; k1 < 8; ++k1) sB[r * kSB + t + 8 * k1] = v[k1];
     }
   }
   __syncthreads();
@@ -718,12 +814,14 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd(
     for (int n2 = 0; n2 < 8; ++n2) {
       const int n = t + 8 * n2;
       const int gy = n < 32 ? n : n - 64;
-      v[n2] = (gy >= -2 * Ny && gy <= 2 * Ny) ? sB[(gy + 2 * Ny) * kM + ux] : make_float2(0.f, 0.f);
+      v[n2] = (gy >= -2 * Ny && gy <= 2 * Ny) ? sB[(gy + 2 * Ny) * kSB + ux] : make_float2(0.f, 0.f);
     }
     fft64_group<1>(v, twi, xs, t, true);
 #pragma unroll
     for (int k1 = 0; k1 < 8; ++k1) gc[q][k1] = v[k1];
   }
+  __syncthreads();  // the column reads of g_c end before the kernels' rows
+
   const float inv = 1.0f / (float)(kM * kM);
   for (int k = kbeg; k < kend; ++k) {
     const float2* K = kern + (size_t)k * T;
@@ -744,7 +842,7 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd(
       fft64_group<1>(v, twi, xs, t, act);
       if (act) {
 #pragma unroll
-        for (int k1 = 0; k1 < 8; ++k1) sB[g * kM + t + 8 * k1] = v[k1];
+        for (int k1 = 0; k1 < 8; ++k1) sB[g * kSB + t + 8 * k1] = v[k1];
       }
     }
     __syncthreads();
@@ -756,7 +854,7 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd(
       for (int n2 = 0; n2 < 8; ++n2) {
         const int n = t + 8 * n2;
         const int ny = n < 32 ? n : n - 64;
-        v[n2] = (ny >= -Ny && ny <= Ny) ? sB[(ny + Ny) * kM + ux] : make_float2(0.f, 0.f);
+        v[n2] = (ny >= -Ny && ny <= Ny) ? sB[(ny + Ny) * kSB + ux] : make_float2(0.f, 0.f);
       }
       fft64_group<1>(v, twi, xs, t, true);
       // conj(e) g_c, then forward along uy
@@ -770,7 +868,7 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd(
       for (int k1 = 0; k1 < 8; ++k1) {
         const int qq = t + 8 * k1;
         const int qy = qq < 32 ? qq : qq - 64;
-        if (qy >= -Ny && qy <= Ny) sB[(qy + Ny) * kM + ux] = v[k1];
+        if (qy >= -Ny && qy <= Ny) sB[(qy + Ny) * kSB + ux] = v[k1];
       }
     }
     __syncthreads();
@@ -780,7 +878,7 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd(
       float2 v[8];
       if (act) {
 #pragma unroll
-        for (int n2 = 0; n2 < 8; ++n2) v[n2] = sB[g * kM + t + 8 * n2];
+        for (int n2 = 0; n2 < 8; ++n2) v[n2] = sB[g * kSB + t + 8 * n2];
       }
       fft64_group<-1>(v, twf, xs, t, act);
       if (act) {
