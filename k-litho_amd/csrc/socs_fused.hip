@@ -361,36 +361,43 @@ __device__ __forceinline__ void band_block(
   if (RESIST && tid == 0) lpart_img[pidx] = red[0] + red[1] + red[2] + red[3];
   // sum the waves (fixed order): with q0 = sum Re(tw) R and q1 = sum Im(tw) R,
   // tw R = (q0.re - q1.im) + i (q0.im + q1.re) for ny >= 0, and the conjugate
-  // twiddle of -ny gives (q0.re + q1.im) + i (q0.im - q1.re)
+  // twiddle of -ny gives (q0.re + q1.im) + i (q0.im - q1.re).  Whole rows of
+  // ldo = 8 NCB entries are written (zero past Bx): a partly written 128-byte
+  // line costs a fill from HBM, which measurably slows the pass.
   float2* out = out_img + (size_t)pidx * Wy * ldo;
-  for (int o = tid; o < (By + 1) * nxp; o += 256) {
-    const int ny = o / nxp, nx = o - ny * nxp;
-    float q0r = 0.f, q0i = 0.f, q1r = 0.f, q1i = 0.f;
+  for (int o = tid; o < Wy * ldo; o += 256) {
+    const int nyi = o / ldo, nx = o - nyi * ldo;
+    const int ny = nyi - By, any = ny < 0 ? -ny : ny;
+    float2 val = make_float2(0.f, 0.f);
+    if (nx <= Bx) {
+      float q0r = 0.f, q0i = 0.f, q1r = 0.f, q1i = 0.f;
 #pragma unroll
-    for (int w = 0; w < 4; ++w) {
-      q0r += sP[w][0][ny][2 * nx];
-      q0i += sP[w][0][ny][2 * nx + 1];
-      q1r += sP[w][1][ny][2 * nx];
-      q1i += sP[w][1][ny][2 * nx + 1];
+      for (int w = 0; w < 4; ++w) {
+        q0r += sP[w][0][any][2 * nx];
+        q0i += sP[w][0][any][2 * nx + 1];
+        q1r += sP[w][1][any][2 * nx];
+        q1i += sP[w][1][any][2 * nx + 1];
+      }
+      val = ny >= 0 ? make_float2(q0r - q1i, q0i + q1r) : make_float2(q0r + q1i, q0i - q1r);
     }
-    out[(By + ny) * ldo + nx] = make_float2(q0r - q1i, q0i + q1r);
-    if (ny > 0) out[(By - ny) * ldo + nx] = make_float2(q0r + q1i, q0i - q1r);
+    out[o] = val;
   }
 }
+// grid (parts * nimg): block w takes part w % parts (= row block * ks +
+// column range) of image w / parts.
 template <int NCB, bool RESIST, bool FULL>
 __global__ __launch_bounds__(256, (RESIST || !FULL) ? 2 : 4) void k_band_mfma(
     const float* __restrict__ in0, const float* __restrict__ in1, float a_sig, float thr,
-    int Lx, int Ly, int Kpad, int ks, const float4* Wf, const float2* __restrict__ twy,
-    int ldtw, int Bx, int By, int ldo, float2* __restrict__ out_part,
-    float* __restrict__ lpart) {
-  const int b = blockIdx.y, parts = gridDim.x;
-  const int blk = blockIdx.x / ks, kh = blockIdx.x - blk * ks;
+    int Lx, int Ly, int Kpad, int ks, int parts, const float4* Wf,
+    const float2* __restrict__ twy, int ldtw, int Bx, int By, int ldo,
+    float2* __restrict__ out_part, float* __restrict__ lpart) {
   const int nchunk = Kpad / (kBandChunk * 32), per = (nchunk + ks - 1) / ks;
+  const int b = blockIdx.x / parts, px = blockIdx.x - b * parts;
+  const int blk = px / ks, kh = px - blk * ks;
   const int ch0 = min(nchunk, kh * per), ch1 = min(nchunk, ch0 + per);
   const size_t ib = (size_t)b * Lx * Ly;
-  band_block<NCB, RESIST, FULL>(blk, ch0, ch1, blockIdx.x, in0 + ib,
-                                RESIST ? in1 + ib : nullptr, a_sig, thr, Lx, Ly, Kpad, Wf, twy,
-                                ldtw, Bx, By, ldo,
+  band_block<NCB, RESIST, FULL>(blk, ch0, ch1, px, in0 + ib, RESIST ? in1 + ib : nullptr, a_sig,
+                                thr, Lx, Ly, Kpad, Wf, twy, ldtw, Bx, By, ldo,
                                 out_part + (size_t)b * parts * (2 * By + 1) * ldo,
                                 RESIST ? lpart + (size_t)b * parts : nullptr);
 }
@@ -1173,13 +1180,14 @@ static void launch_band(klt_ctx* ctx, const char* name, int nblk, int ks, int B,
   const bool full = Lx % 32 == 0 && (reinterpret_cast<uintptr_t>(in0) & 15) == 0 &&
                     (!RESIST || (reinterpret_cast<uintptr_t>(in1) & 15) == 0);
   const float4* w = Wf.as<float4>();
-  const dim3 grid(nblk * ks, B);
+  const int parts = nblk * ks;
+  const dim3 grid(parts * B);
   if (full)
     KLT_LAUNCH(ctx, name, (k_band_mfma<NCB, RESIST, true>), grid, dim3(256), 0, in0, in1, a_sig,
-               thr, Lx, Ly, Kpad, ks, w, twy, ldtw, Bx, By, ldo, out, lpart);
+               thr, Lx, Ly, Kpad, ks, parts, w, twy, ldtw, Bx, By, ldo, out, lpart);
   else
     KLT_LAUNCH(ctx, name, (k_band_mfma<NCB, RESIST, false>), grid, dim3(256), 0, in0, in1, a_sig,
-               thr, Lx, Ly, Kpad, ks, w, twy, ldtw, Bx, By, ldo, out, lpart);
+               thr, Lx, Ly, Kpad, ks, parts, w, twy, ldtw, Bx, By, ldo, out, lpart);
 }
 
 // Row twiddles w(n, x) (rows n < 16 ncb, columns x < Kpad) in the band
