@@ -259,14 +259,15 @@ def bench_adjoint(ctx, bank, timer, torch, dev, src):
     ds = torch.from_numpy(np.ascontiguousarray(src)).to(dev)
     klt.socs_backward(ctx, bank, dm, dt)
     ms = timer(lambda: klt.socs_backward(ctx, bank, dm, dt), 5)
+    # first source-gradient call outside the profiled one (buffers, first launches)
+    klt.abbe_source_grad(ctx, dm, dt, ds, W.NA, 0.0, 14, 14, nmed=NMED)
     ctx.set_profiling(True)
     klt.socs_backward(ctx, bank, dm, dt)
     kern = {k: v[1] for k, v in ctx.profile_report().items()}
     klt.abbe_source_grad(ctx, dm, dt, ds, W.NA, 0.0, 14, 14, nmed=NMED)
     kern_src = {k: v[1] for k, v in ctx.profile_report().items()}
     ctx.set_profiling(False)
-    klt.abbe_source_grad(ctx, dm, dt, ds, W.NA, 0.0, 14, 14, nmed=NMED)
-    ms_src = timer(lambda: klt.abbe_source_grad(ctx, dm, dt, ds, W.NA, 0.0, 14, 14, nmed=NMED), 2)
+    ms_src = timer(lambda: klt.abbe_source_grad(ctx, dm, dt, ds, W.NA, 0.0, 14, 14, nmed=NMED), 3)
     return {"batch": B, "mask_grads_per_s": B / (ms / 1e3), "ms_per_batch": ms,
             "kernels_ms": kern, "source_grad_ms_per_batch": ms_src,
             "source_grad_kernels_ms": kern_src,

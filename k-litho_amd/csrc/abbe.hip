@@ -18,10 +18,11 @@
 //   2. mask bands by the matrix-core band pass, M^ (2N+1)^2 per mask
 //      (k_abbe_assemble), and the Hopkins image I_A of each mask from it (one
 //      wave per spectrum entry, then the matrix-core synthesis);
-//   3. resist bands (G, loss) by the band pass and G^ (4N+1)^2 per mask;
-//   4. k_abbe_grad: one block per (source point, mask): the pupil support is
-//      compacted into LDS and the double sum runs over the nonzero pairs, in
-//      FP64.
+//   3. resist bands (G, loss partials) by the band pass;
+//   4. the double sum on the 64 x 64 coarse grid (abbe_grad_coarse in
+//      socs_fused.hip): Lx Ly sum_{n, n'} A_s[n] conj(A_s[n']) conj(G^[n - n'])
+//      = Lx Ly / 64^2 sum_u g_c(u) |e_s(u)|^2 with e_s, g_c the 64-point
+//      transforms of A_s and G^, one 2D transform per source point.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -32,20 +33,16 @@
 
 namespace {
 
-using klt::PupilGeom;
-using klt::pupil_value;
-
 constexpr int kThreads = 256;
 
-// Per mask: with mpart, M^ (2N+1)^2 from the mask-band partials; with gpart,
-// G^ (4N+1)^2 from the resist-band partials and the loss from lpart.
-__global__ __launch_bounds__(kThreads) void k_abbe_assemble(
-    const float2* __restrict__ mpart, const float2* __restrict__ gpart,
-    const float* __restrict__ lpart, int nblk, int Nx, int Ny, float inv_area,
-    float2* __restrict__ Mfull, float2* __restrict__ Gfull, float* __restrict__ loss) {
+// Per mask: M^ (2N+1)^2 from the mask-band partials.
+__global__ __launch_bounds__(kThreads) void k_abbe_assemble(const float2* __restrict__ mpart,
+                                                            int nblk, int Nx, int Ny,
+                                                            float inv_area,
+                                                            float2* __restrict__ Mfull) {
   const int b = blockIdx.x;
-  const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
-  for (int i = threadIdx.x; mpart && i < Wx * Wy; i += kThreads) {
+  const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1;
+  for (int i = threadIdx.x; i < Wx * Wy; i += kThreads) {
     const int nyi = i / Wx, nx = i - nyi * Wx - Nx, ny = nyi - Ny;
     const int r = nx >= 0 ? nyi : (-ny + Ny), c = nx >= 0 ? nx : -nx;
     float2 s = make_float2(0.f, 0.f);
@@ -56,92 +53,6 @@ __global__ __launch_bounds__(kThreads) void k_abbe_assemble(
       s.y += p.y;
     }
     Mfull[(size_t)b * Wx * Wy + i] = make_float2(s.x * inv_area, (nx < 0 ? -s.y : s.y) * inv_area);
-  }
-  if (!gpart) return;
-  for (int i = threadIdx.x; i < Gx * Gy; i += kThreads) {
-    const int gyi = i / Gx, gx = i - gyi * Gx - 2 * Nx, gy = gyi - 2 * Ny;
-    const int r = gx >= 0 ? gyi : (-gy + 2 * Ny), c = gx >= 0 ? gx : -gx;
-    float2 s = make_float2(0.f, 0.f);
-#pragma unroll 8
-    for (int k = 0; k < nblk; ++k) {
-      const float2 p = gpart[(((size_t)b * nblk + k) * Gy + r) * 32 + c];
-      s.x += p.x;
-      s.y += p.y;
-    }
-    Gfull[(size_t)b * Gx * Gy + i] = make_float2(s.x * inv_area, (gx < 0 ? -s.y : s.y) * inv_area);
-  }
-  if (threadIdx.x == 0) {
-    float l = 0.f;
-    for (int k = 0; k < nblk; ++k) l += lpart[(size_t)b * nblk + k];
-    loss[b] = l;
-  }
-}
-
-__global__ __launch_bounds__(kThreads) void k_abbe_grad(PupilGeom g, const int2* __restrict__ pq,
-                                                        int srcSize,
-                                                        const float2* __restrict__ Mfull,
-                                                        const float2* __restrict__ Gfull,
-                                                        double area, float* __restrict__ grad) {
-  __shared__ float2 sG[61 * 61];
-  __shared__ float2 sA[31 * 31];
-  __shared__ short2 sN[31 * 31];
-  __shared__ int cnt[kThreads + 1];
-  __shared__ double red[4];
-  const int s = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
-  const int Nx = g.Nx, Ny = g.Ny, Wx = 2 * Nx + 1, T = g.T;
-  const int Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
-  for (int i = tid; i < Gx * Gy; i += kThreads) sG[i] = Gfull[(size_t)b * Gx * Gy + i];
-  const int2 p = pq[s];
-  // pupil support of this source point, compacted in frequency order
-  float2 av[4];
-  short2 an[4];
-  int c = 0;
-  for (int j = 0; j < 4; ++j) {
-    const int n = tid * 4 + j;
-    if (n >= T) break;
-    const int ny = n / Wx - Ny, nx = n % Wx - Nx;
-    float2 pv;
-    if (pupil_value(g, p.x, p.y, -nx, -ny, &pv)) {
-      const float2 m = Mfull[(size_t)b * T + n];
-      av[c] = make_float2(pv.x * m.x - pv.y * m.y, pv.x * m.y + pv.y * m.x);
-      an[c] = make_short2((short)nx, (short)ny);
-      ++c;
-    }
-  }
-  cnt[tid + 1] = c;
-  __syncthreads();
-  if (tid == 0) {
-    cnt[0] = 0;
-    for (int i = 1; i <= kThreads; ++i) cnt[i] += cnt[i - 1];
-  }
-  __syncthreads();
-  for (int j = 0; j < c; ++j) {
-    sA[cnt[tid] + j] = av[j];
-    sN[cnt[tid] + j] = an[j];
-  }
-  __syncthreads();
-  const int nnz = cnt[kThreads];
-  double acc = 0.0;
-  for (int u1 = tid; u1 < nnz; u1 += kThreads) {
-    const float2 a1 = sA[u1];
-    const short2 n1 = sN[u1];
-    for (int u2 = 0; u2 < nnz; ++u2) {
-      const float2 a2 = sA[u2];
-      const short2 n2 = sN[u2];
-      const float2 gg = sG[(n1.y - n2.y + 2 * Ny) * Gx + (n1.x - n2.x + 2 * Nx)];
-      // Re(a1 conj(a2) conj(gg))
-      const double pr = (double)a1.x * a2.x + (double)a1.y * a2.y;
-      const double pi = (double)a1.y * a2.x - (double)a1.x * a2.y;
-      acc += pr * gg.x + pi * gg.y;
-    }
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-  if ((tid & 63) == 0) red[tid >> 6] = acc;
-  __syncthreads();
-  if (tid == 0) {
-    const double tot = red[0] + red[1] + red[2] + red[3];
-    grad[(size_t)b * srcSize * srcSize + (p.y + g.sh) * srcSize + p.x + g.sh] = (float)(area * tot);
   }
 }
 
@@ -189,7 +100,7 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
   float sigma = klt_outer_sigma(hsrc.data(), srcSize);
   if (sigma > 1.0f) sigma = 1.0f;
   const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
-  const int Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
+  const int Gy = 4 * Ny + 1;
   const int nblk = (Ly + 127) / 128;
   // outputs (device)
   float *dg = grad_src, *dl = losses, *di = images;
@@ -213,21 +124,16 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
   KLT_TRY(ctx->qpart.ensure((size_t)nmask * nblk * Wy * 16 * sizeof(float2)));
   KLT_TRY(ctx->gpart.ensure((size_t)nmask * nblk * Gy * 32 * sizeof(float2)));
   KLT_TRY(ctx->lpart.ensure((size_t)nmask * nblk * sizeof(float)));
-  KLT_TRY(ctx->abbe_band.ensure((size_t)nmask * (T + Gx * Gy) * sizeof(float2)));
+  KLT_TRY(ctx->abbe_band.ensure((size_t)nmask * T * sizeof(float2)));
   float2* Mfull = ctx->abbe_band.as<float2>();
-  float2* Gfull = Mfull + (size_t)nmask * T;
   const float inv_area = 1.0f / ((float)Lx * (float)Ly);
   KLT_TRY(klt::band_mask_partials(ctx, Lx, Ly, Nx, Ny, dm, nmask, ctx->qpart.as<float2>()));
   KLT_LAUNCH(ctx, "k_abbe_assemble", k_abbe_assemble, dim3(nmask), dim3(kThreads), 0,
-             ctx->qpart.as<float2>(), nullptr, nullptr, nblk, Nx, Ny, inv_area, Mfull, Gfull,
-             dl);
+             ctx->qpart.as<float2>(), nblk, Nx, Ny, inv_area, Mfull);
   KLT_TRY(klt::hopkins_images_fused(ctx, Mfull, nmask, Lx, Ly, tcc, Nx, Ny, I));
-  // 3. resist bands and the full G^ per mask
+  // 3. resist bands and loss partials
   KLT_TRY(klt::band_resist_partials(ctx, Lx, Ly, Nx, Ny, I, dt, nmask, a_sig, thr,
                                     ctx->gpart.as<float2>(), ctx->lpart.as<float>()));
-  KLT_LAUNCH(ctx, "k_abbe_assemble", k_abbe_assemble, dim3(nmask), dim3(kThreads), 0, nullptr,
-             ctx->gpart.as<float2>(), ctx->lpart.as<float>(), nblk, Nx, Ny, inv_area, Mfull,
-             Gfull, dl);
   // 4. per source point
   const int sh = (int)((srcSize - 1) / 2.0);
   std::vector<int2> pts;
@@ -241,9 +147,9 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
   KLT_LAUNCH(ctx, "k_fill_zero", k_fill_zero, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0,
              dg, ng);
   const klt::PupilGeom g = klt::make_geom(srcSize, lambda, NA, defocus, nmed, Lx, Ly, Nx, Ny);
-  KLT_LAUNCH(ctx, "k_abbe_grad", k_abbe_grad, dim3((unsigned)pts.size(), nmask), dim3(kThreads),
-             0, g, ctx->misc.as<int2>(), srcSize, Mfull, Gfull, (double)Lx * (double)Ly, dg);
-  KLT_HIP(hipGetLastError());
+  KLT_TRY(klt::abbe_grad_coarse(ctx, ctx->qpart.as<float2>(), ctx->gpart.as<float2>(),
+                                ctx->lpart.as<float>(), nmask, Lx, Ly, g, ctx->misc.as<int2>(),
+                                (int)pts.size(), srcSize, dg, dl));
   if (flags & KLT_HOST_PTRS) {
     KLT_HIP(hipMemcpyAsync(grad_src, dg, ng * 4, hipMemcpyDeviceToHost, st));
     KLT_HIP(hipMemcpyAsync(losses, dl, (size_t)nmask * 4, hipMemcpyDeviceToHost, st));

@@ -215,6 +215,15 @@ int hopkins_spec_device(klt_ctx* ctx, const float2* band, int B, const double* t
 // matrix-core synthesis (kernel half-size <= 15; socs_fused.hip).
 int hopkins_images_fused(klt_ctx* ctx, const float2* band, int B, int Lx, int Ly,
                          const double* tcc, int Nx, int Ny, float* images);
+struct PupilGeom;  // klt_pupil.h
+// Source gradient of the resist loss for B masks (socs_fused.hip): from the
+// mask-band partials mpart, the resist-band partials gpart and loss partials
+// lpart (band_mask_partials / band_resist_partials), dL/dS at the npts source
+// points pts (grid offsets from the centre) into grad [B][srcSize^2], and the
+// losses.  Kernel half-size <= 15.
+int abbe_grad_coarse(klt_ctx* ctx, const float2* mpart, const float2* gpart,
+                     const float* lpart, int B, int Lx, int Ly, const PupilGeom& geom,
+                     const int2* pts, int npts, int srcSize, float* grad, float* loss);
 // Top-k eigenpairs of a Hermitian matrix on the device (eig.hip).
 int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_dev,
                     double* vec_dev);

@@ -1,6 +1,7 @@
 // klt_pupil.h -- the pupil function of the reference, shared by the TCC build
-// (tcc.hip) and the Abbe source gradient (abbe.hip).  Translation units that
-// include it are compiled with -ffp-contract=off.
+// (tcc.hip) and the Abbe source gradient (socs_fused.hip).  pupil_value turns
+// floating-point contraction off for its own body, so its float arithmetic
+// rounds like the reference's whatever the including file's flags.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -25,26 +26,28 @@ struct SrcPoint {
 // sqrt(1 - rho^2 NA^2) is NaN); nm2 is the squared immersion index (1 in the
 // reference).
 __device__ inline bool pupil_value(const PupilGeom& g, int p, int q, int nx, int ny, float2* out) {
-  const float sx = __fdiv_rn((float)p, (float)g.sh);
-  const float sy = __fdiv_rn((float)q, (float)g.sh);
-  const int nxMin = (int)__fmul_rn(__fsub_rn(-1.0f, sx), g.Lxn);
-  const int nxMax = (int)__fmul_rn(__fsub_rn(1.0f, sx), g.Lxn);
-  const int nyMin = (int)__fmul_rn(__fsub_rn(-1.0f, sy), g.Lyn);
-  const int nyMax = (int)__fmul_rn(__fsub_rn(1.0f, sy), g.Lyn);
+  // (plain operators: the pragma governs only the expressions written here)
+#pragma clang fp contract(off)
+  const float sx = (float)p / (float)g.sh;
+  const float sy = (float)q / (float)g.sh;
+  const int nxMin = (int)((-1.0f - sx) * g.Lxn);
+  const int nxMax = (int)((1.0f - sx) * g.Lxn);
+  const int nyMin = (int)((-1.0f - sy) * g.Lyn);
+  const int nyMax = (int)((1.0f - sy) * g.Lyn);
   if (nx < nxMin || nx > nxMax || ny < nyMin || ny > nyMax) return false;
-  const float fy = __fadd_rn(__fdiv_rn((float)ny, g.Lyn), sy);
-  if (!(__fmul_rn(fy, fy) <= 1.0f)) return false;
-  const float fx = __fadd_rn(__fdiv_rn((float)nx, g.Lxn), sx);
-  const float rho2 = __fadd_rn(__fmul_rn(fx, fx), __fmul_rn(fy, fy));
+  const float fy = (float)ny / g.Lyn + sy;
+  if (!(fy * fy <= 1.0f)) return false;
+  const float fx = (float)nx / g.Lxn + sx;
+  const float rho2 = fx * fx + fy * fy;
   if (!(rho2 <= 1.0f)) return false;
   if (g.dz == 0.0f) {
     *out = make_float2(1.0f, 0.0f);
     return true;
   }
-  const float rn = __fmul_rn(__fmul_rn(rho2, g.NA), g.NA);
-  const float dzk = __fmul_rn(g.dz, g.k);
-  const double root = __dsqrt_rn(__dsub_rn((double)g.nm2, (double)rn));
-  const float phi = __double2float_rn(__dmul_rn((double)dzk, root));
+  const float rn = rho2 * g.NA * g.NA;
+  const float dzk = g.dz * g.k;
+  const double root = __dsqrt_rn((double)g.nm2 - (double)rn);
+  const float phi = (float)((double)dzk * root);
   *out = make_float2(cosf(phi), sinf(phi));
   return true;
 }

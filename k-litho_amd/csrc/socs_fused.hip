@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "klt_internal.h"
+#include "klt_pupil.h"
 
 namespace {
 
@@ -934,6 +935,156 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd_fin(const float2* __restri
 }
 
 // ---------------------------------------------------------------------------
+// Source gradient of the resist loss on the coarse grid.  grid (ngrp, B):
+// block (j, b) takes the j-th of ngrp groups of the source points pq[0, npts)
+// for mask b.  For source point s, with A_s[n] = P(-n, s) M^[n],
+//   e_s(u) = sum_n A_s[n] e^{+2 pi i n.u/64},  g_c(u) = sum_g G^[g] e^{+2 pi i g.u/64}
+// (G^ the |g| <= 2N band of G = dL/dI, centred like the image), and
+//   dL/dS_s = sum_x G(x) |E_s(x)|^2 = Lx Ly / 64^2 sum_u g_c(u) |e_s(u)|^2,
+// exact because |e_s|^2 g_c holds frequencies up to 4N < 64.  The per-point
+// 2D transform is the coarse-grid stage's: rows of A_s (pupil evaluated in
+// place), then columns, then the dot product with g_c (registers) reduced over
+// the block.  loss[b] = sum of the resist-band loss partials.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_abbe_grad_coarse(
+    const float2* __restrict__ mpart, const float2* __restrict__ gpart,
+    const float* __restrict__ lpart, int nblk, klt::PupilGeom pg, const int2* __restrict__ pq,
+    int npts, int srcSize, float inv_area, double scale, const float2* __restrict__ tw64,
+    float* __restrict__ grad, float* __restrict__ loss) {
+  __shared__ float2 sM[31 * 31];
+  __shared__ float2 sB[61 * kSB];
+  __shared__ float2 sX[32 * kXS];
+  __shared__ float red[4];
+  const int b = blockIdx.y, grp = blockIdx.x, ngrp = gridDim.x, tid = threadIdx.x;
+  const int per = (npts + ngrp - 1) / ngrp, s0 = min(npts, grp * per), s1 = min(npts, s0 + per);
+  const int g = tid >> 3, t = tid & 7, wave = tid >> 6, lane = tid & 63;
+  const int Nx = pg.Nx, Ny = pg.Ny, Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
+  const int Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
+  for (int i = tid; i < T; i += 256) {
+    const int nyi = i / Wx, nx = i - nyi * Wx - Nx;
+    const int ny = nyi - Ny;
+    const int r = nx >= 0 ? nyi : (-ny + Ny), c = nx >= 0 ? nx : -nx;
+    float2 s = make_float2(0.f, 0.f);
+    for (int k = 0; k < nblk; ++k) {
+      const float2 p = mpart[(((size_t)b * nblk + k) * Wy + r) * 16 + c];
+      s.x += p.x;
+      s.y += p.y;
+    }
+    if (nx < 0) s.y = -s.y;
+    sM[i] = make_float2(s.x * inv_area, s.y * inv_area);
+  }
+  if (tid == 0 && grp == 0) {
+    float l = 0.f;
+    for (int k = 0; k < nblk; ++k) l += lpart[(size_t)b * nblk + k];
+    loss[b] = l;
+  }
+  float2 twi[8];
+#pragma unroll
+  for (int k2 = 0; k2 < 8; ++k2) twi[k2] = tw64[(t * k2) & 63];
+  float2* xs = sX + g * kXS;
+  // g_c rows: gy in [-2Ny, 2Ny] (row index gy + 2Ny), two rounds of 32 groups
+#pragma unroll
+  for (int rr = 0; rr < 2; ++rr) {
+    const int r = g + 32 * rr;
+    const bool act = r < Gy;
+    const int gy = r - 2 * Ny;
+    float2 v[8];
+    if (act) {
+#pragma unroll
+      for (int n2 = 0; n2 < 8; ++n2) {
+        const int n = t + 8 * n2;
+        const int gx = n < 32 ? n : n - 64;
+        float2 val = make_float2(0.f, 0.f);
+        if (gx >= -2 * Nx && gx <= 2 * Nx) {
+          const int row = gx >= 0 ? r : (-gy + 2 * Ny), col = gx >= 0 ? gx : -gx;
+          for (int k = 0; k < nblk; ++k) {
+            const float2 p = gpart[(((size_t)b * nblk + k) * Gy + row) * 32 + col];
+            val.x += p.x;
+            val.y += p.y;
+          }
+          val = make_float2(val.x * inv_area, gx >= 0 ? val.y * inv_area : -val.y * inv_area);
+        }
+        v[n2] = val;
+      }
+    }
+    fft64_group<1>(v, twi, xs, t, act);
+    if (act) {
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) sB[r * kSB + t + 8 * k1] = v[k1];
+    }
+  }
+  __syncthreads();
+  // g_c columns (real: G^ is Hermitian); thread (g, t) holds column g + 32 q
+  // at rows t + 8 k1, the layout of the points' column transforms below
+  float gc[2][8];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int ux = g + 32 * q;
+    float2 v[8];
+#pragma unroll
+    for (int n2 = 0; n2 < 8; ++n2) {
+      const int n = t + 8 * n2;
+      const int gy = n < 32 ? n : n - 64;
+      v[n2] = (gy >= -2 * Ny && gy <= 2 * Ny) ? sB[(gy + 2 * Ny) * kSB + ux] : make_float2(0.f, 0.f);
+    }
+    fft64_group<1>(v, twi, xs, t, true);
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) gc[q][k1] = v[k1].x;
+  }
+  __syncthreads();  // the column reads of g_c end before the points' rows
+  for (int s = s0; s < s1; ++s) {
+    const int2 p = pq[s];
+    {
+      const bool act = g < Wy;
+      const int ny = g - Ny;
+      float2 v[8];
+      if (act) {
+#pragma unroll
+        for (int n2 = 0; n2 < 8; ++n2) {
+          const int n = t + 8 * n2;
+          const int nx = n < 32 ? n : n - 64;
+          float2 val = make_float2(0.f, 0.f), pv;
+          if (nx >= -Nx && nx <= Nx && klt::pupil_value(pg, p.x, p.y, -nx, -ny, &pv))
+            val = cmulf(pv, sM[g * Wx + nx + Nx]);
+          v[n2] = val;
+        }
+      }
+      fft64_group<1>(v, twi, xs, t, act);
+      if (act) {
+#pragma unroll
+        for (int k1 = 0; k1 < 8; ++k1) sB[g * kSB + t + 8 * k1] = v[k1];
+      }
+    }
+    __syncthreads();
+    float part = 0.f;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int ux = g + 32 * q;
+      float2 v[8];
+#pragma unroll
+      for (int n2 = 0; n2 < 8; ++n2) {
+        const int n = t + 8 * n2;
+        const int ny = n < 32 ? n : n - 64;
+        v[n2] = (ny >= -Ny && ny <= Ny) ? sB[(ny + Ny) * kSB + ux] : make_float2(0.f, 0.f);
+      }
+      fft64_group<1>(v, twi, xs, t, true);
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) part += gc[q][k1] * (v[k1].x * v[k1].x + v[k1].y * v[k1].y);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+    if (lane == 0) red[wave] = part;
+    // (also ends the column reads of sB before the next point's rows)
+    __syncthreads();
+    if (tid == 0) {
+      const double tot = (double)red[0] + (double)red[1] + (double)red[2] + (double)red[3];
+      grad[(size_t)b * srcSize * srcSize + (size_t)(p.y + pg.sh) * srcSize + p.x + pg.sh] =
+          (float)(scale * tot);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Stage 3.  grid (ceil(Lx/128), B), 256 threads.
 // P[x][j] (j = 2 gy: Re, 2 gy + 1: Im) for the block's 128 columns, scaled by
 // a power of two and split to hi/lo halves in LDS; then wave w computes rows
@@ -1581,8 +1732,8 @@ int socs_forward_batch(klt_ctx* ctx, int nmask, const float* const* masks, const
   return KLT_OK;
 }
 
-int socs_mid_from_band(klt_ctx* ctx, const klt_kernels* kb, const float2* band, int B,
-                       float2* spec, int G) {
+// The context's e^{+2 pi i j/64} table (j < 64), made on first use.
+static int ctx_tw64(klt_ctx* ctx) {
   if (!ctx->tw64.ptr) {
     std::vector<float2> tw(kM);
     for (int j = 0; j < kM; ++j) {
@@ -1592,9 +1743,31 @@ int socs_mid_from_band(klt_ctx* ctx, const klt_kernels* kb, const float2* band, 
     KLT_TRY(ctx->tw64.ensure(kM * sizeof(float2)));
     KLT_HIP(hipMemcpy(ctx->tw64.ptr, tw.data(), kM * sizeof(float2), hipMemcpyHostToDevice));
   }
+  return KLT_OK;
+}
+
+int socs_mid_from_band(klt_ctx* ctx, const klt_kernels* kb, const float2* band, int B,
+                       float2* spec, int G) {
+  KLT_TRY(ctx_tw64(ctx));
   KLT_LAUNCH(ctx, "k_socs_mid_band", k_socs_mid_band, dim3(B), dim3(256), 0, band, kb->Nx,
              kb->Ny, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk,
              ctx->tw64.as<float2>(), spec, G);
+  KLT_HIP(hipGetLastError());
+  return KLT_OK;
+}
+
+int abbe_grad_coarse(klt_ctx* ctx, const float2* mpart, const float2* gpart,
+                     const float* lpart, int B, int Lx, int Ly, const PupilGeom& geom,
+                     const int2* pts, int npts, int srcSize, float* grad, float* loss) {
+  if (geom.Nx > 15 || geom.Ny > 15)
+    return set_error(KLT_ERR_ARG, "abbe_grad_coarse: half-size N above 15");
+  KLT_TRY(ctx_tw64(ctx));
+  const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
+  // about two blocks per CU over all masks
+  const int ngrp = std::max(1, std::min(npts, (512 + B - 1) / B));
+  KLT_LAUNCH(ctx, "k_abbe_grad_coarse", k_abbe_grad_coarse, dim3(ngrp, B), dim3(256), 0, mpart,
+             gpart, lpart, nblk, geom, pts, npts, srcSize, 1.0f / ((float)Lx * (float)Ly),
+             (double)Lx * (double)Ly / (double)(kM * kM), ctx->tw64.as<float2>(), grad, loss);
   KLT_HIP(hipGetLastError());
   return KLT_OK;
 }

@@ -21,7 +21,8 @@ def _annular():
     return src, O.outer_sigma(src)
 
 
-@pytest.mark.parametrize("L,df,N", [(256, 0.0, None), (512, 0.2361, None), (300, 0.0, 6)])
+@pytest.mark.parametrize("L,df,N", [(256, 0.0, None), (512, 0.2361, None), (300, 0.0, 6),
+                                    (255, 0.2361, 6)])
 def test_source_gradient_vs_fp64(ctx, L, df, N):
     print(f"case abbe L={L} df={df}", flush=True)
     src, sig = _annular()
