@@ -479,6 +479,34 @@ __device__ __forceinline__ void fft64_group(float2* v, const float2* tw, float2*
   wave_sync();
 }
 
+// Two independent 64-point transforms per group (scratch xa, xb): the same
+// steps as fft64_group, interleaved so each wave has twice the independent
+// work between its LDS round trips.
+template <int S>
+__device__ __forceinline__ void fft64_group2(float2* va, float2* vb, const float2* tw,
+                                             float2* xa, float2* xb, int t, bool active) {
+  if (active) {
+    dft8<S>(va);
+    dft8<S>(vb);
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) {
+      xa[t * 9 + k2] = cmulf(va[k2], tw[k2]);
+      xb[t * 9 + k2] = cmulf(vb[k2], tw[k2]);
+    }
+  }
+  wave_sync();
+  if (active) {
+#pragma unroll
+    for (int n1 = 0; n1 < 8; ++n1) {
+      va[n1] = xa[n1 * 9 + t];
+      vb[n1] = xb[n1 * 9 + t];
+    }
+    dft8<S>(va);
+    dft8<S>(vb);
+  }
+  wave_sync();
+}
+
 // ---------------------------------------------------------------------------
 // Stage 2.  grid (B), 256 threads = 32 groups of 8.
 // spec: [B][32][4Nx+1], row gy in [0, 2Ny] (rows above zero), entry gx + 2Nx.
@@ -546,9 +574,11 @@ __device__ __forceinline__ void mid_image(const float2* __restrict__ mpart, int 
                                           const float2* __restrict__ twM,
                                           float2* __restrict__ spec, int spec_rows,
                                           float* __restrict__ ipart) {
+  // two kernels per pass: their row transforms (sB, sB + 31 kSB) and FFT
+  // scratch (sX, sX + 32 kXS)
   __shared__ float2 sM[31 * 31];
-  __shared__ float2 sB[31 * kSB];
-  __shared__ float2 sX[32 * kXS];
+  __shared__ float2 sB[2 * 31 * kSB];
+  __shared__ float2 sX[2 * 32 * kXS];
   const int tid = threadIdx.x;
   const int g = tid >> 3, t = tid & 7;
   const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
@@ -582,28 +612,45 @@ __device__ __forceinline__ void mid_image(const float2* __restrict__ mpart, int 
 #pragma unroll
     for (int k = 0; k < 8; ++k) acc[q][k] = 0.f;
   float2* xs = sX + g * kXS;
+  float2* xs2 = sX + (32 + g) * kXS;
+  float2* sB2 = sB + 31 * kSB;
   __syncthreads();
-  for (int k = k0; k < k1; ++k) {
-    const float2* K = kern + (size_t)k * T;
-    const float lam = scales[k];
+  // Kernels in pairs (an odd count ends with a pair whose second kernel is
+  // the first again, weighted 0).  With |n| <= 15, inputs t + 8 n2 of the
+  // transforms are zero for n2 = 2..5 (|n| >= 16): written as constants so
+  // the first radix-8 stage drops them.
+  for (int k = k0; k < k1; k += 2) {
+    const bool two = k + 1 < k1;
+    const float2* Ka = kern + (size_t)k * T;
+    const float2* Kb = two ? Ka + T : Ka;
+    const float lama = scales[k], lamb = two ? scales[k + 1] : 0.f;
     // rows: B[r][u] = sum_nx A[r][nx] e^{+2 pi i nx u / 64}
     {
       const bool act = g < Wy;
-      float2 v[8];
+      float2 va[8], vb[8];
+#pragma unroll
+      for (int n2 = 0; n2 < 8; ++n2) va[n2] = vb[n2] = make_float2(0.f, 0.f);
       if (act) {
 #pragma unroll
         for (int n2 = 0; n2 < 8; ++n2) {
+          if (n2 >= 2 && n2 <= 5) continue;
           const int n = t + 8 * n2;
           const int nx = n < 32 ? n : n - 64;
-          float2 val = make_float2(0.f, 0.f);
-          if (nx >= -Nx && nx <= Nx) val = cmulf(K[g * Wx + nx + Nx], sM[g * Wx + nx + Nx]);
-          v[n2] = val;
+          if (nx >= -Nx && nx <= Nx) {
+            const int i = g * Wx + nx + Nx;
+            const float2 m = sM[i];
+            va[n2] = cmulf(Ka[i], m);
+            vb[n2] = cmulf(Kb[i], m);
+          }
         }
       }
-      fft64_group<1>(v, twi, xs, t, act);
+      fft64_group2<1>(va, vb, twi, xs, xs2, t, act);
       if (act) {
 #pragma unroll
-        for (int k1 = 0; k1 < 8; ++k1) sB[g * kSB + t + 8 * k1] = v[k1];
+        for (int k1 = 0; k1 < 8; ++k1) {
+          sB[g * kSB + t + 8 * k1] = va[k1];
+          sB2[g * kSB + t + 8 * k1] = vb[k1];
+        }
       }
     }
     __syncthreads();
@@ -611,18 +658,25 @@ __device__ __forceinline__ void mid_image(const float2* __restrict__ mpart, int 
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int ux = g + 32 * q;
-      float2 v[8];
+      float2 va[8], vb[8];
 #pragma unroll
       for (int n2 = 0; n2 < 8; ++n2) {
+        va[n2] = vb[n2] = make_float2(0.f, 0.f);
+        if (n2 >= 2 && n2 <= 5) continue;
         const int n = t + 8 * n2;
         const int ny = n < 32 ? n : n - 64;
-        v[n2] = (ny >= -Ny && ny <= Ny) ? sB[(ny + Ny) * kSB + ux] : make_float2(0.f, 0.f);
+        if (ny >= -Ny && ny <= Ny) {
+          va[n2] = sB[(ny + Ny) * kSB + ux];
+          vb[n2] = sB2[(ny + Ny) * kSB + ux];
+        }
       }
-      fft64_group<1>(v, twi, xs, t, true);
+      fft64_group2<1>(va, vb, twi, xs, xs2, t, true);
 #pragma unroll
-      for (int k1 = 0; k1 < 8; ++k1) acc[q][k1] += lam * (v[k1].x * v[k1].x + v[k1].y * v[k1].y);
+      for (int k1 = 0; k1 < 8; ++k1)
+        acc[q][k1] += lama * (va[k1].x * va[k1].x + va[k1].y * va[k1].y) +
+                      lamb * (vb[k1].x * vb[k1].x + vb[k1].y * vb[k1].y);
     }
-    __syncthreads();  // the column reads of sB end before the next kernel's rows
+    __syncthreads();  // the column reads of sB end before the next pair's rows
   }
   if (ipart) {  // a kernel group's partial coarse intensity
 #pragma unroll
@@ -1526,7 +1580,7 @@ static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, in
     KLT_TRY(time_run(c, &ms));
     if (ms < best.ms) best = c, best.ms = ms;
   }
-  for (int v = 1; v <= 8; v *= 2) {
+  for (int v : {1, 2, 4, 8, 12, 16, 24}) {
     if (v == best.kg || !kg_ok(v)) continue;
     klt_ctx::Tuned c = best;
     c.kg = v;

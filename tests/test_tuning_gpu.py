@@ -56,7 +56,7 @@ def test_forward_splits_agree(ctx, L):
     shapes = [r for r in ctx.tune_report()
               if r[0] == "socs_forward" and r[1:6] == (2, L, L, 14, 14)]
     assert len(shapes) == 1 and shapes[0][7] in (1, 2, 4) and shapes[0][8] in (1, 2, 4)
-    assert shapes[0][9] in (1, 2, 4, 8)
+    assert shapes[0][9] in (1, 2, 4)  # kernel groups (candidates up to the 6 kernels)
 
 
 def test_backward_splits_agree(ctx):
