@@ -6,6 +6,9 @@
 //                    (x = conj(A[j][j+1:])), then computes its rows of
 //                    p = tau A22 v and a partial sum of v^H p;
 //        k_hh_her2 : A22 -= v w^H + w v^H, w = p - (tau/2)(v^H p) v.
+//      For n <= 3400 one launch per step (k_hh_step) does both: it applies
+//      step j-1's rank-2 update to row j (for the reflector) and to the
+//      block's rows of the trailing matrix while multiplying them by v.
 //      The off-diagonal entries are complex; a diagonal unitary D maps the
 //      tridiagonal T to the real symmetric T_r = D^H T D with off-diagonal
 //      |e_j|.
@@ -133,6 +136,125 @@ __global__ __launch_bounds__(kThreads) void k_hh_symv(double* __restrict__ A, in
 }
 
 // A22 -= v w^H + w v^H on 64 x 64 tiles; w = p - (tau/2)(v^H p) v.
+// Step j with step j-1's update folded in.  A holds the matrix with the
+// updates of steps < j-1 applied.  Step j-1's update A[j.., j..] -= u,
+// u(i, l) = v'_i conj(w'_l) + w'_i conj(v'_l) with v' = V[j-1][j..],
+// w' = p' - K' v', K' = (tau'/2) sum part', is applied
+//   - to row j, by every block, before it builds the reflector x = conj(A[j][j+1:]);
+//   - to the block's rows of the trailing matrix A[j+1.., j+1..], written back
+//     as they are multiplied by v (p_i = tau sum_l A[j+1+i][j+1+l] v_l).
+// Rows j, columns j and the entries left of them are finished and not
+// updated.  Dynamic LDS: (3m + 2) complex values, m = n - j - 1.
+__global__ __launch_bounds__(kThreads) void k_hh_step(double* __restrict__ A, int n, int j,
+                                                      int rows, cd* __restrict__ V,
+                                                      double* __restrict__ tau,
+                                                      cd* __restrict__ sub,
+                                                      double* __restrict__ diag,
+                                                      const cd* __restrict__ p_prev,
+                                                      const cd* __restrict__ part_prev,
+                                                      int nparts_prev, cd* __restrict__ p,
+                                                      cd* __restrict__ part) {
+  extern __shared__ cd sh[];
+  __shared__ double red[8];
+  const int m = n - j - 1;
+  cd* sv = sh;              // row j (raw), then this step's reflector; m entries
+  cd* svp = sh + m;         // v', m + 1 entries (global index j + i)
+  cd* spp = svp + (m + 1);  // p'; w'_i = p'_i - K' v'_i is formed where used
+  const double tp = j > 0 ? tau[j - 1] : 0.0;
+  const bool upd = tp != 0.0;  // uniform over the grid
+  // every load of the prologue at once: step j-1's partial sums, v', p' and
+  // row j
+  const cd* Aj = reinterpret_cast<const cd*>(A) + (size_t)j * n + j + 1;
+  double kr = 0.0, ki = 0.0;
+  if (upd) {
+    for (int q = threadIdx.x; q < nparts_prev; q += kThreads) {
+      kr += part_prev[q].x;
+      ki += part_prev[q].y;
+    }
+    const cd* vp = V + (size_t)(j - 1) * n + j;
+    for (int i = threadIdx.x; i <= m; i += kThreads) {
+      svp[i] = vp[i];
+      spp[i] = p_prev[i];
+    }
+  }
+  for (int i = threadIdx.x; i < m; i += kThreads) sv[i] = Aj[i];
+  block_sum2(kr, ki, red);  // (its barriers also publish the LDS writes)
+  const cd K = {0.5 * tp * kr, 0.5 * tp * ki};
+  auto wprev = [&](int i) {
+    const cd v = svp[i], kv = cmul(K, v), pp = spp[i];
+    return cd{pp.x - kv.x, pp.y - kv.y};
+  };
+  // row j after the update: entries l + 1 of step j-1's block, l < m
+  const cd v0 = upd ? svp[0] : cd{0.0, 0.0}, w0 = upd ? wprev(0) : cd{0.0, 0.0};
+  double s2 = 0.0, dummy = 0.0;
+  for (int i = threadIdx.x; i < m; i += kThreads) {
+    cd a = sv[i];
+    if (upd) {
+      const cd vl = svp[i + 1], wl = wprev(i + 1);
+      a.x -= (v0.x * wl.x + v0.y * wl.y) + (w0.x * vl.x + w0.y * vl.y);
+      a.y -= (v0.y * wl.x - v0.x * wl.y) + (w0.y * vl.x - w0.x * vl.y);
+    }
+    const cd xi = {a.x, -a.y};
+    sv[i] = xi;  // (each thread rewrites only the entries it read)
+    s2 += xi.x * xi.x + xi.y * xi.y;
+  }
+  block_sum2(s2, dummy, red);
+  const cd x0 = sv[0];
+  const double ax0 = hypot(x0.x, x0.y);
+  const double alpha = sqrt(s2);
+  const bool skip = (s2 - ax0 * ax0) <= 0.0;  // x already a multiple of e_1
+  cd ph = {1.0, 0.0};
+  if (ax0 > 0.0) ph = {x0.x / ax0, x0.y / ax0};
+  const double t = skip ? 0.0 : 1.0 / (alpha * (alpha + ax0));  // 2 / (v^H v)
+  __syncthreads();
+  if (threadIdx.x == 0 && !skip) {
+    sv[0].x += ph.x * alpha;
+    sv[0].y += ph.y * alpha;
+  }
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    cd* Vj = V + (size_t)j * n + j + 1;
+    for (int i = threadIdx.x; i < m; i += kThreads) Vj[i] = skip ? cd{0.0, 0.0} : sv[i];
+    if (threadIdx.x == 0) {
+      tau[j] = t;
+      sub[j] = skip ? x0 : cd{-ph.x * alpha, -ph.y * alpha};
+      // u(0, 0) = 2 Re(v'_0 conj(w'_0))
+      diag[j] = A[2 * ((size_t)j * n + j)] - 2.0 * (v0.x * w0.x + v0.y * w0.y);
+    }
+  }
+  // the block's rows: update, write back, multiply by v (one wave per row)
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r0 = blockIdx.x * rows, r1 = min(m, r0 + rows);
+  double pr = 0.0, pi = 0.0;  // partial conj(v) . p of this wave
+  for (int i = r0 + wave; i < r1; i += 4) {
+    cd* Ai = reinterpret_cast<cd*>(A) + (size_t)(j + 1 + i) * n + j + 1;
+    const cd vi = upd ? svp[i + 1] : cd{0.0, 0.0}, wi = upd ? wprev(i + 1) : cd{0.0, 0.0};
+    double sr = 0.0, si = 0.0;
+    for (int l = lane; l < m; l += 64) {
+      cd a = Ai[l];
+      if (upd) {
+        const cd vl = svp[l + 1], wl = wprev(l + 1);
+        a.x -= (vi.x * wl.x + vi.y * wl.y) + (wi.x * vl.x + wi.y * vl.y);
+        a.y -= (vi.y * wl.x - vi.x * wl.y) + (wi.y * vl.x - wi.x * vl.y);
+        Ai[l] = a;
+      }
+      const cd v = sv[l];
+      sr += a.x * v.x - a.y * v.y;
+      si += a.x * v.y + a.y * v.x;
+    }
+    sr = wave_sum(sr) * t;
+    si = wave_sum(si) * t;
+    if (lane == 0) {
+      p[i] = {sr, si};
+      const cd v = sv[i];
+      pr += v.x * sr + v.y * si;
+      pi += v.x * si - v.y * sr;
+    }
+  }
+  block_sum2(pr, pi, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = {pr, pi};
+}
+
 __global__ __launch_bounds__(kThreads) void k_hh_her2(double* __restrict__ A, int n, int j,
                                                       const cd* __restrict__ V,
                                                       const double* __restrict__ tau,
@@ -465,7 +587,7 @@ int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_d
   if (n > 8192) return set_error(KLT_ERR_ARG, "eig: n above 8192 is not supported");
   const size_t nn = (size_t)n * n;
   // workspace: A copy, V, tau, sub, diag, p, parts, e, e2, z, lu work, clusters
-  const int nparts_max = 256;
+  const int nparts_max = 1024;
   size_t off = 0;
   auto carve = [&](size_t bytes) {
     size_t o = off;
@@ -473,8 +595,8 @@ int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_d
     return o;
   };
   const size_t oA = carve(nn * 16), oV = carve(nn * 16), oTau = carve(n * 8),
-               oSub = carve(n * 16), oDiag = carve(n * 8), oP = carve(n * 16),
-               oPart = carve(nparts_max * 16), oE = carve(n * 8), oE2 = carve(n * 8),
+               oSub = carve(n * 16), oDiag = carve(n * 8), oP = carve(2 * (size_t)n * 16),
+               oPart = carve(2 * nparts_max * 16), oE = carve(n * 8), oE2 = carve(n * 8),
                oZ = carve((size_t)k * n * 8), oWork = carve((size_t)k * 5 * n * 8),
                oCl = carve((k + 1) * 4);
   KLT_TRY(ctx->eig_ws.ensure(off));
@@ -502,16 +624,40 @@ int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_d
     KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_back_transform),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_v));
   }
+  // one launch per step while v, v' and w' fit in LDS (3 n complex values)
+  const size_t lds_step = (size_t)(3 * n + 2) * 16;
+  const bool fused = lds_step <= 160 * 1024;
+  if (fused && lds_step > 64 * 1024)
+    KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hh_step),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_step));
+  int np_prev = 0;
   for (int j = 0; j + 2 < n; ++j) {
     const int m = n - j - 1;
-    const int nblk = std::min(nparts_max, (m + 15) / 16);
+    // the fused step: one row per wave; the two-kernel path: 16 rows a block
+    const int nblk = std::min(nparts_max, fused ? (m + 3) / 4 : (m + 15) / 16);
     const int rows = (m + nblk - 1) / nblk;
     const int np = (m + rows - 1) / rows;
-    KLT_LAUNCH(ctx, "k_hh_symv", k_hh_symv, dim3(np), dim3(kThreads), (size_t)m * 16, A, n, j,
-               rows, V, tau, sub, diag, p, part);
-    const int tiles = (m + 63) / 64;
-    KLT_LAUNCH(ctx, "k_hh_her2", k_hh_her2, dim3(tiles, tiles), dim3(kThreads), 0, A, n, j, V,
-               tau, p, part, np);
+    // step j writes p / part set j & 1 and reads step j-1's from the other
+    cd* pj = p + (size_t)(j & 1) * n;
+    cd* partj = part + (size_t)(j & 1) * nparts_max;
+    if (fused) {
+      const cd* pp = p + (size_t)((j + 1) & 1) * n;
+      const cd* partp = part + (size_t)((j + 1) & 1) * nparts_max;
+      KLT_LAUNCH(ctx, "k_hh_step", k_hh_step, dim3(np), dim3(kThreads),
+                 (size_t)(3 * m + 2) * 16, A, n, j, rows, V, tau, sub, diag, pp, partp, np_prev,
+                 pj, partj);
+      if (j + 3 == n) {  // the last step's update, for the final 2 x 2 block
+        KLT_LAUNCH(ctx, "k_hh_her2", k_hh_her2, dim3(1, 1), dim3(kThreads), 0, A, n, j, V, tau,
+                   pj, partj, np);
+      }
+    } else {
+      KLT_LAUNCH(ctx, "k_hh_symv", k_hh_symv, dim3(np), dim3(kThreads), (size_t)m * 16, A, n,
+                 j, rows, V, tau, sub, diag, pj, partj);
+      const int tiles = (m + 63) / 64;
+      KLT_LAUNCH(ctx, "k_hh_her2", k_hh_her2, dim3(tiles, tiles), dim3(kThreads), 0, A, n, j,
+                 V, tau, pj, partj, np);
+    }
+    np_prev = np;
   }
   KLT_LAUNCH(ctx, "k_hh_finish", k_hh_finish, dim3(1), dim3(64), 0, A, n, sub, diag);
   KLT_HIP(hipGetLastError());
