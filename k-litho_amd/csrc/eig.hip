@@ -489,11 +489,12 @@ __global__ __launch_bounds__(256) void k_inverse_iteration(
           }
 #pragma unroll
           for (int j = 0; j < 32; ++j) {
+            if (j >= nr) break;  // (uniform)
             const double v = wave_sum(acc[j]);
             if (lane == 0) red[wv][j] = v;
           }
           __syncthreads();
-          if (tid < 32) dots[tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+          if (tid < nr) dots[tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
           __syncthreads();
           for (int i = tid; i < n; i += 256) {
             double v = x[i];
