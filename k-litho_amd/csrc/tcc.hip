@@ -54,13 +54,25 @@ __global__ __launch_bounds__(256) void k_split_operands(
     float scale, _Float16* __restrict__ Uh, _Float16* __restrict__ Ul,
     _Float16* __restrict__ V1h, _Float16* __restrict__ V1l, _Float16* __restrict__ V2h,
     _Float16* __restrict__ V2l) {
-  const int i = blockIdx.x;  // frequency row (< Tpad)
-  for (int k = threadIdx.x; k < Kpad; k += 256) {
+  // grid (Tpad / 64, Kpad / 64): a 64 x 64 tile of (row i, column k), read
+  // along i from P[s][i] and written along k through LDS
+  __shared__ float2 sp[64][65];
+  const int i0 = blockIdx.x * 64, kb = blockIdx.y * 64, tid = threadIdx.x;
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int kk = e >> 6, ii = e & 63;
+    const int i = i0 + ii, k = kb + kk;
+    float2 p = make_float2(0.f, 0.f);
+    if (i < T && k < 2 * ns) p = P[(size_t)(k < ns ? k : k - ns) * T + i];
+    sp[kk][ii] = p;
+  }
+  __syncthreads();
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int ii = e >> 6, kk = e & 63;
+    const int i = i0 + ii, k = kb + kk;
     float u = 0.f, v1 = 0.f, v2 = 0.f;
     if (i < T && k < 2 * ns) {
-      const int s = k < ns ? k : k - ns;
-      const float2 p = P[(size_t)s * T + i];
-      const float su = scale * pts[s].S;
+      const float2 p = sp[kk][ii];
+      const float su = scale * pts[k < ns ? k : k - ns].S;
       if (k < ns) {
         u = su * p.x;
         v1 = p.x;
@@ -92,113 +104,149 @@ __global__ __launch_bounds__(256) void k_split_operands(
 // steps ahead into two register sets (the loop runs two steps per pass; Kpad
 // and kc are multiples of 64, and the loads past the chunk's end re-read its
 // last step).
-__global__ __launch_bounds__(256) void k_gram_split16(
+// Split-precision Gram, one 128 x 128 output tile (tile list entry blockIdx.x)
+// over the K range ks * kc .. + kc (blockIdx.y).  Waves 2 x 2, each a 64 x 64
+// sub-tile of 4 x 4 MFMA blocks (TCC_R and TCC_I).  Per 32-wide K step the
+// block stages the step's U, V1, V2 hi / lo rows (6 x 128 rows x 32 halves)
+// in LDS, and issues the next step's global loads before its products.
+constexpr int kGT = 128;     // output tile
+constexpr int kGLd = 40;     // LDS row stride (halves): 80 bytes
+__global__ __launch_bounds__(256, 1) void k_gram_split16(
     const _Float16* __restrict__ Uh, const _Float16* __restrict__ Ul,
     const _Float16* __restrict__ V1h, const _Float16* __restrict__ V1l,
     const _Float16* __restrict__ V2h, const _Float16* __restrict__ V2l, int Kpad, int kc,
     const int2* __restrict__ tiles, float2* __restrict__ part) {
+  __shared__ __attribute__((aligned(16))) _Float16 sop[6][kGT * kGLd];
   const int tile = blockIdx.x, ks = blockIdx.y, ntiles = gridDim.x;
   const int2 t = tiles[tile];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int qi = (wave >> 1) * 32, qj = (wave & 1) * 32;
-  const int i0 = t.x * 64 + qi, j0 = t.y * 64 + qj;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wr = wave >> 1, wc = wave & 1;
   const int r16 = lane & 15, kq = (lane >> 4) * 8;
-  float4_t accR[2][2], accI[2][2];
+  const int k0 = ks * kc, k1 = min(Kpad, k0 + kc);
+  // staging: thread tid moves 16-byte chunks tid and tid + 256 of each array
+  // (chunk c: row c / 4, halves 8 (c % 4) .. + 8 of the step); offsets of
+  // its two chunks in the U rows (ou) and the V rows (ov) at k = 0
+  const int row0 = tid >> 2, col0 = (tid & 3) * 8;  // chunk tid + 256: row + 64
+  const size_t ou0 = (size_t)(t.x * kGT + row0) * Kpad + col0, ou1 = ou0 + (size_t)64 * Kpad;
+  const size_t ov0 = (size_t)(t.y * kGT + row0) * Kpad + col0, ov1 = ov0 + (size_t)64 * Kpad;
+  uint4 g0, g1, g2, g3, g4, g5, g6, g7, g8, g9, g10, g11;
+#define KLT_GRAM_FETCH(k)                                                   \
+  do {                                                                      \
+    g0 = *reinterpret_cast<const uint4*>(Uh + ou0 + (k));                   \
+    g1 = *reinterpret_cast<const uint4*>(Uh + ou1 + (k));                   \
+    g2 = *reinterpret_cast<const uint4*>(Ul + ou0 + (k));                   \
+    g3 = *reinterpret_cast<const uint4*>(Ul + ou1 + (k));                   \
+    g4 = *reinterpret_cast<const uint4*>(V1h + ov0 + (k));                  \
+    g5 = *reinterpret_cast<const uint4*>(V1h + ov1 + (k));                  \
+    g6 = *reinterpret_cast<const uint4*>(V1l + ov0 + (k));                  \
+    g7 = *reinterpret_cast<const uint4*>(V1l + ov1 + (k));                  \
+    g8 = *reinterpret_cast<const uint4*>(V2h + ov0 + (k));                  \
+    g9 = *reinterpret_cast<const uint4*>(V2h + ov1 + (k));                  \
+    g10 = *reinterpret_cast<const uint4*>(V2l + ov0 + (k));                 \
+    g11 = *reinterpret_cast<const uint4*>(V2l + ov1 + (k));                 \
+  } while (0)
+  float4_t accR[4][4], accI[4][4];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < 4; ++b) {
       accR[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
       accI[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
     }
-  const size_t ra0 = (size_t)(i0 + r16) * Kpad + kq, ra1 = ra0 + (size_t)16 * Kpad;
-  const size_t rb0 = (size_t)(j0 + r16) * Kpad + kq, rb1 = rb0 + (size_t)16 * Kpad;
-  const int k0 = ks * kc, k1 = min(Kpad, k0 + kc);
-  // operands of one step: [0..1] U hi, [2..3] U lo, [4..5] V1 hi, [6..7] V1 lo,
-  // [8..9] V2 hi, [10..11] V2 lo
-  half8_t s0[12], s1[12];
-  auto load = [&](int k, half8_t* o) {
-    k = min(k, k1 - 32);
-    o[0] = *reinterpret_cast<const half8_t*>(Uh + ra0 + k);
-    o[1] = *reinterpret_cast<const half8_t*>(Uh + ra1 + k);
-    o[2] = *reinterpret_cast<const half8_t*>(Ul + ra0 + k);
-    o[3] = *reinterpret_cast<const half8_t*>(Ul + ra1 + k);
-    o[4] = *reinterpret_cast<const half8_t*>(V1h + rb0 + k);
-    o[5] = *reinterpret_cast<const half8_t*>(V1h + rb1 + k);
-    o[6] = *reinterpret_cast<const half8_t*>(V1l + rb0 + k);
-    o[7] = *reinterpret_cast<const half8_t*>(V1l + rb1 + k);
-    o[8] = *reinterpret_cast<const half8_t*>(V2h + rb0 + k);
-    o[9] = *reinterpret_cast<const half8_t*>(V2h + rb1 + k);
-    o[10] = *reinterpret_cast<const half8_t*>(V2l + rb0 + k);
-    o[11] = *reinterpret_cast<const half8_t*>(V2l + rb1 + k);
-  };
-  auto products = [&](const half8_t* c) {
+  if (k0 < k1) KLT_GRAM_FETCH(k0);
+  const int l0 = row0 * kGLd + col0, l1 = l0 + 64 * kGLd;
+  for (int k = k0; k < k1; k += 32) {
+    __syncthreads();  // the previous step's operand reads are done
+    *reinterpret_cast<uint4*>(&sop[0][l0]) = g0;
+    *reinterpret_cast<uint4*>(&sop[0][l1]) = g1;
+    *reinterpret_cast<uint4*>(&sop[1][l0]) = g2;
+    *reinterpret_cast<uint4*>(&sop[1][l1]) = g3;
+    *reinterpret_cast<uint4*>(&sop[2][l0]) = g4;
+    *reinterpret_cast<uint4*>(&sop[2][l1]) = g5;
+    *reinterpret_cast<uint4*>(&sop[3][l0]) = g6;
+    *reinterpret_cast<uint4*>(&sop[3][l1]) = g7;
+    *reinterpret_cast<uint4*>(&sop[4][l0]) = g8;
+    *reinterpret_cast<uint4*>(&sop[4][l1]) = g9;
+    *reinterpret_cast<uint4*>(&sop[5][l0]) = g10;
+    *reinterpret_cast<uint4*>(&sop[5][l1]) = g11;
+    __syncthreads();
+    if (k + 32 < k1) KLT_GRAM_FETCH(k + 32);
+    // A fragments of the wave's 4 row blocks (hi, lo), kept over the columns
+    half8_t ah[4], al[4];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 4; ++a) {
+      const int row = wr * 64 + a * 16 + r16;
+      ah[a] = *reinterpret_cast<const half8_t*>(&sop[0][row * kGLd + kq]);
+      al[a] = *reinterpret_cast<const half8_t*>(&sop[1][row * kGLd + kq]);
+    }
 #pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(c[a], c[4 + b], accR[a][b], 0, 0, 0);
-        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(c[a], c[6 + b], accR[a][b], 0, 0, 0);
-        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(c[2 + a], c[4 + b], accR[a][b], 0, 0, 0);
-        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(c[a], c[8 + b], accI[a][b], 0, 0, 0);
-        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(c[a], c[10 + b], accI[a][b], 0, 0, 0);
-        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(c[2 + a], c[8 + b], accI[a][b], 0, 0, 0);
+    for (int b = 0; b < 4; ++b) {
+      const int col = wc * 64 + b * 16 + r16;
+      const half8_t b1h = *reinterpret_cast<const half8_t*>(&sop[2][col * kGLd + kq]);
+      const half8_t b1l = *reinterpret_cast<const half8_t*>(&sop[3][col * kGLd + kq]);
+      const half8_t b2h = *reinterpret_cast<const half8_t*>(&sop[4][col * kGLd + kq]);
+      const half8_t b2l = *reinterpret_cast<const half8_t*>(&sop[5][col * kGLd + kq]);
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], b1h, accR[a][b], 0, 0, 0);
+        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], b1l, accR[a][b], 0, 0, 0);
+        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], b1h, accR[a][b], 0, 0, 0);
+        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], b2h, accI[a][b], 0, 0, 0);
+        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], b2l, accI[a][b], 0, 0, 0);
+        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], b2h, accI[a][b], 0, 0, 0);
       }
-  };
-  if (k0 < k1) {
-    load(k0, s0);
-    load(k0 + 32, s1);
+    }
   }
-  for (int k = k0; k < k1; k += 64) {
-    // the products read their operands at issue, so a set reloads right after
-    products(s0);
-    __builtin_amdgcn_sched_barrier(0);
-    load(k + 64, s0);
-    __builtin_amdgcn_sched_barrier(0);
-    products(s1);
-    __builtin_amdgcn_sched_barrier(0);
-    load(k + 96, s1);
-    __builtin_amdgcn_sched_barrier(0);
-  }
-  float2* out = part + ((size_t)ks * ntiles + tile) * 64 * 64;
+#undef KLT_GRAM_FETCH
+  float2* out = part + ((size_t)ks * ntiles + tile) * kGT * kGT;
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < 4; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int ii = qi + a * 16 + (lane >> 4) * 4 + r, jj = qj + b * 16 + r16;
-        out[ii * 64 + jj] = make_float2(accR[a][b][r], accI[a][b][r]);
+        const int ii = wr * 64 + a * 16 + (lane >> 4) * 4 + r, jj = wc * 64 + b * 16 + r16;
+        out[ii * kGT + jj] = make_float2(accR[a][b][r], accI[a][b][r]);
       }
 }
-
-// Sums the nks partials of each tile in a fixed order, scales them and writes
-// both triangles of the T x T complex double TCC (imaginary diagonal zero).
+// Sums the nks partials of each 128 x 128 tile in a fixed order, scales them
+// and writes both triangles.  grid (ntiles, kGT / 16): block (tile, c) takes
+// the 16-column strip c of the tile; the strip passes through LDS so that the
+// mirrored entries tcc[j][i] are written along rows as well.
 __global__ __launch_bounds__(256) void k_gram_reduce(const float2* __restrict__ part, int nks,
                                                      const int2* __restrict__ tiles, int T,
                                                      double inv_scale,
                                                      double* __restrict__ tcc) {
-  const int tile = blockIdx.x, ntiles = gridDim.x;
+  __shared__ double2 sv[kGT][17];
+  const int tile = blockIdx.x, ntiles = gridDim.x, c0 = blockIdx.y * 16;
   const int2 t = tiles[tile];
   const bool diag_tile = t.x == t.y;
-  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-    const int ii = e >> 6, jj = e & 63;
-    const int i = t.x * 64 + ii, j = t.y * 64 + jj;
-    if (i >= T || j >= T || (diag_tile && i > j)) continue;
+  for (int e = threadIdx.x; e < kGT * 16; e += 256) {
+    const int ii = e >> 4, jj = e & 15;
+    const int i = t.x * kGT + ii, j = t.y * kGT + c0 + jj;
     double re = 0.0, im = 0.0;
     for (int ks = 0; ks < nks; ++ks) {
-      const float2 v = part[((size_t)ks * ntiles + tile) * 64 * 64 + e];
+      const float2 v = part[((size_t)ks * ntiles + tile) * kGT * kGT + ii * kGT + c0 + jj];
       re += (double)v.x;
       im += (double)v.y;
     }
     re *= inv_scale;
     im = i == j ? 0.0 : im * inv_scale;
-    tcc[2 * ((size_t)i * T + j)] = re;
-    tcc[2 * ((size_t)i * T + j) + 1] = im;
-    if (i != j) {
-      tcc[2 * ((size_t)j * T + i)] = re;
-      tcc[2 * ((size_t)j * T + i) + 1] = -im;
+    sv[ii][jj] = make_double2(re, im);
+    if (i < T && j < T && !(diag_tile && i > j)) {
+      tcc[2 * ((size_t)i * T + j)] = re;
+      tcc[2 * ((size_t)i * T + j) + 1] = im;
     }
+  }
+  __syncthreads();
+  // mirror: row j of the TCC gets conj of column j of the strip
+  for (int e = threadIdx.x; e < kGT * 16; e += 256) {
+    const int jj = e >> 7, ii = e & (kGT - 1);
+    const int i = t.x * kGT + ii, j = t.y * kGT + c0 + jj;
+    if (i >= T || j >= T || i >= j) continue;
+    const double2 v = sv[ii][jj];
+    tcc[2 * ((size_t)j * T + i)] = v.x;
+    tcc[2 * ((size_t)j * T + i) + 1] = -v.y;
   }
 }
 
@@ -340,29 +388,29 @@ int tcc_build_device(klt_ctx* ctx, const float* src_host, int srcSize, float out
   int e = 0;
   (void)std::frexp(smax, &e);
   const float scale = std::ldexp(1.0f, -e);
-  const int Kpad = round_up(2 * ns, 64), Tpad = round_up(T, 64);
+  const int Kpad = round_up(2 * ns, 64), Tpad = round_up(T, kGT);
   const size_t nel = (size_t)Tpad * Kpad;
   KLT_TRY(ctx->qpart.ensure(6 * nel * sizeof(_Float16)));
   _Float16* base = ctx->qpart.as<_Float16>();
-  KLT_LAUNCH(ctx, "k_split_operands", k_split_operands, dim3(Tpad), dim3(256), 0,
+  KLT_LAUNCH(ctx, "k_split_operands", k_split_operands, dim3(Tpad / 64, Kpad / 64), dim3(256), 0,
              ctx->misc.as<float2>(), ctx->tcc_pts.as<SrcPoint>(), ns, T, Kpad, scale, base,
              base + nel, base + 2 * nel, base + 3 * nel, base + 4 * nel, base + 5 * nel);
-  const int nt = Tpad / 64;
+  const int nt = Tpad / kGT;
   std::vector<int2> tiles;
   for (int a = 0; a < nt; ++a)
     for (int b = a; b < nt; ++b) tiles.push_back(make_int2(a, b));
   KLT_TRY(ctx->spec.ensure(sizeof(int2) * tiles.size()));
   KLT_HIP(hipMemcpyAsync(ctx->spec.ptr, tiles.data(), sizeof(int2) * tiles.size(),
                          hipMemcpyHostToDevice, ctx->stream));
-  // split K so that the grid holds at least two blocks per CU
-  const int pairs = Kpad / 64, ntl = (int)tiles.size();
-  const int nks = std::max(1, std::min(pairs, (512 + ntl - 1) / ntl));
-  const int kc = (pairs + nks - 1) / nks * 64;
-  KLT_TRY(ctx->tcc_part.ensure((size_t)nks * ntl * 64 * 64 * sizeof(float2)));
+  // split K so that the grid fills the CUs once (one block per CU)
+  const int steps = Kpad / 32, ntl = (int)tiles.size();
+  const int nks = std::max(1, std::min(steps, 256 / ntl));
+  const int kc = (steps + nks - 1) / nks * 32;
+  KLT_TRY(ctx->tcc_part.ensure((size_t)nks * ntl * kGT * kGT * sizeof(float2)));
   KLT_LAUNCH(ctx, "k_gram_split16", k_gram_split16, dim3((unsigned)ntl, (unsigned)nks),
              dim3(256), 0, base, base + nel, base + 2 * nel, base + 3 * nel, base + 4 * nel,
              base + 5 * nel, Kpad, kc, ctx->spec.as<int2>(), ctx->tcc_part.as<float2>());
-  KLT_LAUNCH(ctx, "k_gram_reduce", k_gram_reduce, dim3((unsigned)ntl), dim3(256), 0,
+  KLT_LAUNCH(ctx, "k_gram_reduce", k_gram_reduce, dim3((unsigned)ntl, kGT / 16), dim3(256), 0,
              ctx->tcc_part.as<float2>(), nks, ctx->spec.as<int2>(), T, 1.0 / (double)scale,
              tcc_dev);
   KLT_HIP(hipGetLastError());
