@@ -52,6 +52,15 @@ void klt_ctx_destroy(klt_ctx* ctx);
 int klt_ctx_set_stream(klt_ctx* ctx, void* stream);
 void* klt_ctx_stream(klt_ctx* ctx);
 int klt_ctx_sync(klt_ctx* ctx);
+/* Stream ordering with a caller's stream (a hipStream_t; NULL is the null
+ * stream), without a host wait (no reference counterpart: the reference is
+ * single-threaded host code).  klt_ctx_wait_stream: later work of the context
+ * starts after everything queued so far on `stream`, e.g. the kernels that
+ * produced its device inputs.  klt_ctx_join_stream: later work on `stream`
+ * starts after everything queued so far on the context's stream, e.g. the
+ * consumers of its device outputs. */
+int klt_ctx_wait_stream(klt_ctx* ctx, void* stream);
+int klt_ctx_join_stream(klt_ctx* ctx, void* stream);
 /* Per-kernel timing (no reference counterpart; the reference prints only a
  * whole-run wall clock, klitho_socs.cpp:355-361).  While on, every kernel
  * launch is bracketed by HIP events on the context stream. */

@@ -201,6 +201,32 @@ int klt_ctx_sync(klt_ctx* ctx) {
   return KLT_OK;
 }
 
+int klt_ctx_wait_stream(klt_ctx* ctx, void* stream) {
+  if (!ctx) return klt::set_error(KLT_ERR_ARG, "klt_ctx_wait_stream: ctx is NULL");
+  const hipStream_t other = static_cast<hipStream_t>(stream);
+  if (other == ctx->stream) return KLT_OK;
+  KLT_TRY(ctx->activate());
+  hipEvent_t e = ctx->take_event();
+  const hipError_t r = hipEventRecord(e, other);
+  const hipError_t w = r == hipSuccess ? hipStreamWaitEvent(ctx->stream, e, 0) : r;
+  ctx->event_pool.push_back(e);  // (a wait holds the event's state at the call)
+  KLT_HIP(w);
+  return KLT_OK;
+}
+
+int klt_ctx_join_stream(klt_ctx* ctx, void* stream) {
+  if (!ctx) return klt::set_error(KLT_ERR_ARG, "klt_ctx_join_stream: ctx is NULL");
+  const hipStream_t other = static_cast<hipStream_t>(stream);
+  if (other == ctx->stream) return KLT_OK;
+  KLT_TRY(ctx->activate());
+  hipEvent_t e = ctx->take_event();
+  const hipError_t r = hipEventRecord(e, ctx->stream);
+  const hipError_t w = r == hipSuccess ? hipStreamWaitEvent(other, e, 0) : r;
+  ctx->event_pool.push_back(e);
+  KLT_HIP(w);
+  return KLT_OK;
+}
+
 int klt_ctx_set_profiling(klt_ctx* ctx, int on) {
   if (!ctx) return klt::set_error(KLT_ERR_ARG, "klt_ctx_set_profiling: ctx is NULL");
   ctx->profiling = on != 0;

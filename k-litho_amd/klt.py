@@ -58,6 +58,8 @@ def lib() -> C.CDLL:
         L.klt_ctx_stream.argtypes = [vp]
         L.klt_ctx_stream.restype = vp
         L.klt_ctx_sync.argtypes = [vp]
+        L.klt_ctx_wait_stream.argtypes = [vp, vp]
+        L.klt_ctx_join_stream.argtypes = [vp, vp]
         L.klt_ctx_set_profiling.argtypes = [vp, i]
         L.klt_ctx_profile_report.argtypes = [vp, C.c_char_p, i]
         L.klt_ctx_set_autotune.argtypes = [vp, i]
@@ -121,6 +123,29 @@ def _ptr(a, dtype):
     if not isinstance(a, np.ndarray) or a.dtype != dtype or not a.flags.c_contiguous:
         raise TypeError(f"expected a C-contiguous numpy {np.dtype(dtype)} array")
     return a.ctypes.data_as(C.c_void_p), True
+
+
+class _TorchOrder:
+    """Stream order around a call that takes torch tensors.  The library works
+    on its own stream; this makes the call start after the work queued so far
+    on torch's current stream (the kernels that produced its inputs) and
+    torch's later work start after the call (the consumers of its outputs)."""
+
+    def __init__(self, handle, *arrays):
+        self.h, self.s = handle, None
+        if any(_is_torch(a) for a in arrays if a is not None):
+            import torch
+            self.s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def __enter__(self):
+        if self.s is not None:
+            check(lib().klt_ctx_wait_stream(self.h, self.s))
+        return self
+
+    def __exit__(self, et, ev, tb):
+        if self.s is not None and et is None:
+            check(lib().klt_ctx_join_stream(self.h, self.s))
+        return False
 
 
 def _flags(*hosts) -> int:
@@ -269,8 +294,9 @@ class Context:
         ps, hs = _ptr(src, np.float32)
         po, ho = _ptr(out, np.float64)
         n = int(src.shape[0])
-        check(lib().klt_tcc_build(self._h, ps, n, sigma, lam, NA, defocus, nmed, Lx, Ly, Nx, Ny,
-                                  precision, po, _flags(hs, ho)))
+        with _TorchOrder(self._h, src, out):
+            check(lib().klt_tcc_build(self._h, ps, n, sigma, lam, NA, defocus, nmed, Lx, Ly, Nx,
+                                      Ny, precision, po, _flags(hs, ho)))
         return out
 
     def eig_topk(self, a, k: int):
@@ -289,7 +315,8 @@ class Context:
         pa, ha = _ptr(a, np.float64)
         pw, hw = _ptr(w, np.float64)
         pv, hv = _ptr(v, np.float64)
-        check(lib().klt_eig_topk(self._h, pa, n, k, pw, pv, _flags(ha, hw, hv)))
+        with _TorchOrder(self._h, a):
+            check(lib().klt_eig_topk(self._h, pa, n, k, pw, pv, _flags(ha, hw, hv)))
         return w, v
 
     def calc_kernels(self, tcc, nk: int):
@@ -309,7 +336,9 @@ class Context:
         p1, h1 = _ptr(kr, np.float32)
         p2, h2 = _ptr(ki, np.float32)
         p3, h3 = _ptr(sc, np.float32)
-        check(lib().klt_calc_kernels(self._h, pt, T, nk, p1, p2, p3, _flags(ht, h1, h2, h3)))
+        with _TorchOrder(self._h, tcc):
+            check(lib().klt_calc_kernels(self._h, pt, T, nk, p1, p2, p3,
+                                         _flags(ht, h1, h2, h3)))
         return kr, ki, sc
 
     def hopkins_image(self, mask, tcc, Nx: int, Ny: int):
@@ -323,7 +352,9 @@ class Context:
         pm, hm = _ptr(mask, np.float32)
         pt, ht = _ptr(tcc, np.float64)
         pi, hi = _ptr(img, np.float32)
-        check(lib().klt_hopkins_image(self._h, pm, Lx, Ly, pt, Nx, Ny, pi, _flags(hm, ht, hi)))
+        with _TorchOrder(self._h, mask, tcc):
+            check(lib().klt_hopkins_image(self._h, pm, Lx, Ly, pt, Nx, Ny, pi,
+                                          _flags(hm, ht, hi)))
         return img
 
     def kernels(self, kr, ki, scales, size_x: int, size_y: int) -> "KernelBank":
@@ -345,13 +376,15 @@ class Context:
         pm, hm = _ptr(masks, np.float32)
         pi, hi = _ptr(images, np.float32)
         fl = _flags(hm, hi)
-        if plain:
-            esz = 4 * Lx * Ly
-            for i in range(B):
-                check(lib().klt_socs_image(self._h, bank.handle, C.c_void_p(pm.value + i * esz),
-                                           Lx, Ly, C.c_void_p(pi.value + i * esz), fl))
-        else:
-            check(lib().klt_socs_forward(self._h, bank.handle, pm, B, Lx, Ly, pi, fl))
+        with _TorchOrder(self._h, masks, images):
+            if plain:
+                esz = 4 * Lx * Ly
+                for i in range(B):
+                    check(lib().klt_socs_image(self._h, bank.handle,
+                                               C.c_void_p(pm.value + i * esz), Lx, Ly,
+                                               C.c_void_p(pi.value + i * esz), fl))
+            else:
+                check(lib().klt_socs_forward(self._h, bank.handle, pm, B, Lx, Ly, pi, fl))
         return images
 
     def socs_forward_batch(self, banks, masks, images=None):
@@ -368,7 +401,8 @@ class Context:
         ly = (C.c_int * nm)(*[int(m.shape[-2]) for m in masks])
         bp = (C.c_void_p * nb)(*[b.handle.value for b in banks])
         ip = (C.c_void_p * (nm * nb))(*[im.data_ptr() for im in images])
-        check(lib().klt_socs_forward_batch(self._h, nm, mp, lx, ly, nb, bp, ip, 0))
+        with _TorchOrder(self._h, *masks):
+            check(lib().klt_socs_forward_batch(self._h, nm, mp, lx, ly, nb, bp, ip, 0))
         return images
 
     def batch_latency(self) -> np.ndarray:
@@ -411,8 +445,9 @@ def socs_backward(ctx: "Context", bank: "KernelBank", masks, targets, a: float =
     if images is not None:
         pi, hi = _ptr(images, np.float32)
         hs.append(hi)
-    check(lib().klt_socs_backward(ctx.handle, bank.handle, pm, pt, B, Lx, Ly, a, thr, pg, pl,
-                                  pi, _flags(*hs)))
+    with _TorchOrder(ctx.handle, masks, targets):
+        check(lib().klt_socs_backward(ctx.handle, bank.handle, pm, pt, B, Lx, Ly, a, thr, pg,
+                                      pl, pi, _flags(*hs)))
     return losses, grads, images
 
 
@@ -439,8 +474,9 @@ def abbe_source_grad(ctx: "Context", masks, targets, src, NA: float, defocus: fl
             raise TypeError("Abbe images are returned on the device only")
         pi, hi = _ptr(images, np.float32)
         hs.append(hi)
-    check(lib().klt_abbe_source_grad(ctx.handle, pm, pt, B, Lx, Ly, ps, n, lam, NA, defocus,
-                                     nmed, Nx, Ny, a, thr, pg, pl, pi, _flags(*hs)))
+    with _TorchOrder(ctx.handle, masks, targets, src):
+        check(lib().klt_abbe_source_grad(ctx.handle, pm, pt, B, Lx, Ly, ps, n, lam, NA,
+                                         defocus, nmed, Nx, Ny, a, thr, pg, pl, pi, _flags(*hs)))
     return losses, grads, images
 
 
@@ -452,8 +488,9 @@ class KernelBank:
         pim, h2 = _ptr(ki, np.float32)
         ps, h3 = _ptr(scales, np.float32)
         nk = int(np.prod(scales.shape))
-        check(lib().klt_kernels_create(ctx.handle, pr, pim, ps, nk, size_x, size_y,
-                                       _flags(h1, h2, h3), C.byref(self._h)))
+        with _TorchOrder(ctx.handle, kr, ki, scales):
+            check(lib().klt_kernels_create(ctx.handle, pr, pim, ps, nk, size_x, size_y,
+                                           _flags(h1, h2, h3), C.byref(self._h)))
         self.nk, self.size_x, self.size_y = nk, size_x, size_y
 
     @property
