@@ -14,7 +14,9 @@
 // given (after createSource's normalisation), for every grid point inside the
 // unit disc (S_s = 0 included).
 //
-//   1. TCC of the source (FP64 Gram);
+//   1. TCC of the source (split-FP16 matrix-core Gram: within 2e-7 of the
+//      FP64 Gram in relative Frobenius norm, far inside the gradient's
+//      tolerance);
 //   2. mask bands by the matrix-core band pass, M^ (2N+1)^2 per mask
 //      (k_abbe_assemble), and the Hopkins image I_A of each mask from it (one
 //      wave per spectrum entry, then the matrix-core synthesis);
@@ -119,7 +121,7 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
   KLT_TRY(ctx->abbe_tcc.ensure((size_t)T * T * 16));
   double* tcc = ctx->abbe_tcc.as<double>();
   KLT_TRY(klt::tcc_build_device(ctx, hsrc.data(), srcSize, sigma, lambda, NA, defocus, nmed, Lx,
-                                Ly, Nx, Ny, KLT_TCC_FP64, tcc));
+                                Ly, Nx, Ny, KLT_TCC_SPLIT_F16, tcc));
   // 2. mask bands (matrix-core band pass), then the Hopkins images from them
   KLT_TRY(ctx->qpart.ensure((size_t)nmask * nblk * Wy * 16 * sizeof(float2)));
   KLT_TRY(ctx->gpart.ensure((size_t)nmask * nblk * Gy * 32 * sizeof(float2)));
