@@ -162,6 +162,21 @@ __global__ __launch_bounds__(kThreads) void k_hh_step(double* __restrict__ A, in
   cd* spp = svp + (m + 1);  // p'; w'_i = p'_i - K' v'_i is formed where used
   const double tp = j > 0 ? tau[j - 1] : 0.0;
   const bool upd = tp != 0.0;  // uniform over the grid
+  // the wave's first trailing row (entries lane + 64 c, c < kPre) is loaded
+  // first: it does not depend on the reflector, so its latency overlaps the
+  // prologue
+  constexpr int kPre = 24;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r0 = blockIdx.x * rows, r1 = min(m, r0 + rows);
+  cd pre[kPre];
+  if (r0 + wave < r1) {
+    const cd* Af = reinterpret_cast<const cd*>(A) + (size_t)(j + 1 + r0 + wave) * n + j + 1;
+#pragma unroll
+    for (int c = 0; c < kPre; ++c) {
+      const int l = lane + 64 * c;
+      pre[c] = l < m ? Af[l] : cd{0.0, 0.0};
+    }
+  }
   // every load of the prologue at once: step j-1's partial sums, v', p' and
   // row j
   const cd* Aj = reinterpret_cast<const cd*>(A) + (size_t)j * n + j + 1;
@@ -223,15 +238,12 @@ __global__ __launch_bounds__(kThreads) void k_hh_step(double* __restrict__ A, in
     }
   }
   // the block's rows: update, write back, multiply by v (one wave per row)
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int r0 = blockIdx.x * rows, r1 = min(m, r0 + rows);
   double pr = 0.0, pi = 0.0;  // partial conj(v) . p of this wave
   for (int i = r0 + wave; i < r1; i += 4) {
     cd* Ai = reinterpret_cast<cd*>(A) + (size_t)(j + 1 + i) * n + j + 1;
     const cd vi = upd ? svp[i + 1] : cd{0.0, 0.0}, wi = upd ? wprev(i + 1) : cd{0.0, 0.0};
     double sr = 0.0, si = 0.0;
-    for (int l = lane; l < m; l += 64) {
-      cd a = Ai[l];
+    auto entry = [&](int l, cd a) {
       if (upd) {
         const cd vl = svp[l + 1], wl = wprev(l + 1);
         a.x -= (vi.x * wl.x + vi.y * wl.y) + (wi.x * vl.x + wi.y * vl.y);
@@ -241,7 +253,17 @@ __global__ __launch_bounds__(kThreads) void k_hh_step(double* __restrict__ A, in
       const cd v = sv[l];
       sr += a.x * v.x - a.y * v.y;
       si += a.x * v.y + a.y * v.x;
+    };
+    int lstart = lane;
+    if (i == r0 + wave) {  // the prefetched row
+#pragma unroll
+      for (int c = 0; c < kPre; ++c) {
+        const int l = lane + 64 * c;
+        if (l < m) entry(l, pre[c]);
+      }
+      lstart = lane + 64 * kPre;
     }
+    for (int l = lstart; l < m; l += 64) entry(l, Ai[l]);
     sr = wave_sum(sr) * t;
     si = wave_sum(si) * t;
     if (lane == 0) {
