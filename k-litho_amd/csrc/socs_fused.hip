@@ -790,74 +790,89 @@ __device__ __forceinline__ void gam_to_spec(const float2* sGam, int Nx, int Ny, 
   }
 }
 
-// grid (kg, B): block (j, b) takes the j-th of kg groups of the kernels;
-// kg = 1 writes the gradient spectrum, kg > 1 its partial Gam to
-// gam_part [B][kg][(2Ny+1)(2Nx+1)] for k_socs_mid_bwd_fin.
-__global__ __launch_bounds__(256) void k_socs_mid_bwd(
-    const float2* __restrict__ mpart, const float2* __restrict__ gpart,
-    const float* __restrict__ lpart, int nblk, int Nx, int Ny, float inv_area,
-    const float2* __restrict__ kern, const float* __restrict__ scales, int nk,
-    const float2* __restrict__ tw64, int dcx, int dcy, int Lx, int Ly,
-    float2* __restrict__ gspec, float* __restrict__ loss, float2* __restrict__ gam_part) {
-  __shared__ float2 sM[31 * 31];
-  __shared__ float2 sB[61 * kSB];
-  __shared__ float2 sX[32 * kXS];
-  __shared__ float2 sGam[31 * 31];
-  const int b = blockIdx.y, grp = blockIdx.x, kg = gridDim.x, tid = threadIdx.x;
-  const int per = (nk + kg - 1) / kg, kbeg = min(nk, grp * per), kend = min(nk, kbeg + per);
-  const int g = tid >> 3, t = tid & 7;
+// grid (ceil(T / 256), B), one thread per entry: the normalised mask band M^
+// [B][(2Ny+1)(2Nx+1)] from the band partials (Hermitian for nx < 0), shared
+// by the coarse stages that follow.
+__global__ __launch_bounds__(256) void k_band_assemble(const float2* __restrict__ mpart,
+                                                       int nblk, int Nx, int Ny,
+                                                       float inv_area,
+                                                       float2* __restrict__ Mfull) {
+  const int b = blockIdx.y;
   const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
-  const int Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
-  for (int i = tid; i < T; i += 256) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < T) {
     const int nyi = i / Wx, nx = i - nyi * Wx - Nx;
     const int ny = nyi - Ny;
     const int r = nx >= 0 ? nyi : (-ny + Ny), c = nx >= 0 ? nx : -nx;
     float2 s = make_float2(0.f, 0.f);
+#pragma unroll 8
     for (int k = 0; k < nblk; ++k) {
       const float2 p = mpart[(((size_t)b * nblk + k) * Wy + r) * 16 + c];
       s.x += p.x;
       s.y += p.y;
     }
     if (nx < 0) s.y = -s.y;
-    sM[i] = make_float2(s.x * inv_area, s.y * inv_area);
-    sGam[i] = make_float2(0.f, 0.f);
+    Mfull[(size_t)b * T + i] = make_float2(s.x * inv_area, s.y * inv_area);
   }
-  if (tid == 0 && grp == 0) {
+}
+
+// grid (ceil(Gy Gx / 256), B), one thread per entry: the resist band G^
+// [B][4Ny+1][4Nx+1] (centred, Hermitian for gx < 0) from the resist-band
+// partials, and (block 0) the loss of each image.
+__global__ __launch_bounds__(256) void k_resist_sum(const float2* __restrict__ gpart,
+                                                    const float* __restrict__ lpart, int nblk,
+                                                    int Nx, int Ny, float inv_area,
+                                                    float2* __restrict__ Gfull,
+                                                    float* __restrict__ loss) {
+  const int b = blockIdx.y;
+  const int Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     float l = 0.f;
     for (int k = 0; k < nblk; ++k) l += lpart[(size_t)b * nblk + k];
     loss[b] = l;
   }
-  float2 twi[8], twf[8];
-#pragma unroll
-  for (int k2 = 0; k2 < 8; ++k2) {
-    twi[k2] = tw64[(t * k2) & 63];
-    twf[k2] = make_float2(twi[k2].x, -twi[k2].y);
+  if (i >= Gx * Gy) return;
+  const int gyi = i / Gx, gx = i - gyi * Gx - 2 * Nx, gy = gyi - 2 * Ny;
+  const int row = gx >= 0 ? gyi : (-gy + 2 * Ny), col = gx >= 0 ? gx : -gx;
+  float2 s = make_float2(0.f, 0.f);
+#pragma unroll 8
+  for (int k = 0; k < nblk; ++k) {
+    const float2 p = gpart[(((size_t)b * nblk + k) * Gy + row) * 32 + col];
+    s.x += p.x;
+    s.y += p.y;
   }
+  Gfull[(size_t)b * Gx * Gy + i] = make_float2(s.x * inv_area, gx >= 0 ? s.y * inv_area : -s.y * inv_area);
+}
+
+// Per image (grid B): g_c(u) = sum_g G^[g] e^{+2 pi i g.u/64} on the coarse grid
+// (real: G^ is Hermitian), stored thread-major ([B][256 threads][16]: thread
+// (g, t) holds column g + 32 q at rows t + 8 k1, the layout of the coarse-stage
+// column transforms).
+__global__ __launch_bounds__(256) void k_resist_coarse(const float2* __restrict__ Gfull, int Nx,
+                                                       int Ny, const float2* __restrict__ tw64,
+                                                       float* __restrict__ gcout) {
+  __shared__ float2 sB[61 * kSB];
+  __shared__ float2 sX[32 * kXS];
+  const int b = blockIdx.x, tid = threadIdx.x, g = tid >> 3, t = tid & 7;
+  const int Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
+  const float2* G = Gfull + (size_t)b * Gx * Gy;
+  float2 twi[8];
+#pragma unroll
+  for (int k2 = 0; k2 < 8; ++k2) twi[k2] = tw64[(t * k2) & 63];
   float2* xs = sX + g * kXS;
-  __syncthreads();
-  // g_c rows: gy in [-2Ny, 2Ny] (row index gy + 2Ny), two rounds of 32 groups
+  // rows: gy in [-2Ny, 2Ny] (row index gy + 2Ny), two rounds of 32 groups
 #pragma unroll
   for (int rr = 0; rr < 2; ++rr) {
     const int r = g + 32 * rr;
     const bool act = r < Gy;
-    const int gy = r - 2 * Ny;
     float2 v[8];
     if (act) {
 #pragma unroll
       for (int n2 = 0; n2 < 8; ++n2) {
         const int n = t + 8 * n2;
         const int gx = n < 32 ? n : n - 64;
-        float2 val = make_float2(0.f, 0.f);
-        if (gx >= -2 * Nx && gx <= 2 * Nx) {
-          const int row = gx >= 0 ? r : (-gy + 2 * Ny), col = gx >= 0 ? gx : -gx;
-          for (int k = 0; k < nblk; ++k) {
-            const float2 p = gpart[(((size_t)b * nblk + k) * Gy + row) * 32 + col];
-            val.x += p.x;
-            val.y += p.y;
-          }
-          val = make_float2(val.x * inv_area, gx >= 0 ? val.y * inv_area : -val.y * inv_area);
-        }
-        v[n2] = val;
+        v[n2] = (gx >= -2 * Nx && gx <= 2 * Nx) ? G[r * Gx + gx + 2 * Nx] : make_float2(0.f, 0.f);
       }
     }
     fft64_group<1>(v, twi, xs, t, act);
@@ -867,7 +882,7 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd(
     }
   }
   __syncthreads();
-  float2 gc[2][8];
+  float* out = gcout + (size_t)b * kM * kM + tid * 16;
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     const int ux = g + 32 * q;
@@ -880,10 +895,48 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd(
     }
     fft64_group<1>(v, twi, xs, t, true);
 #pragma unroll
-    for (int k1 = 0; k1 < 8; ++k1) gc[q][k1] = v[k1];
+    for (int k1 = 0; k1 < 8; ++k1) out[q * 8 + k1] = v[k1].x;
   }
-  __syncthreads();  // the column reads of g_c end before the kernels' rows
+}
 
+// grid (kg, B): block (j, b) takes the j-th of kg groups of the kernels, from
+// the image's M^ (k_band_assemble) and g_c (k_resist_assemble); kg = 1 writes
+// the gradient spectrum, kg > 1 its partial Gam to
+// gam_part [B][kg][(2Ny+1)(2Nx+1)] for k_socs_mid_bwd_fin.
+__global__ __launch_bounds__(256) void k_socs_mid_bwd(
+    const float2* __restrict__ Mfull, const float* __restrict__ gcin, int Nx, int Ny,
+    const float2* __restrict__ kern, const float* __restrict__ scales, int nk,
+    const float2* __restrict__ tw64, int dcx, int dcy, int Lx, int Ly,
+    float2* __restrict__ gspec, float2* __restrict__ gam_part) {
+  __shared__ float2 sM[31 * 31];
+  __shared__ float2 sB[31 * kSB];
+  __shared__ float2 sX[32 * kXS];
+  __shared__ float2 sGam[31 * 31];
+  const int b = blockIdx.y, grp = blockIdx.x, kg = gridDim.x, tid = threadIdx.x;
+  const int per = (nk + kg - 1) / kg, kbeg = min(nk, grp * per), kend = min(nk, kbeg + per);
+  const int g = tid >> 3, t = tid & 7;
+  const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
+  const int Gx = 4 * Nx + 1;
+  for (int i = tid; i < T; i += 256) {
+    sM[i] = Mfull[(size_t)b * T + i];
+    sGam[i] = make_float2(0.f, 0.f);
+  }
+  float2 twi[8], twf[8];
+#pragma unroll
+  for (int k2 = 0; k2 < 8; ++k2) {
+    twi[k2] = tw64[(t * k2) & 63];
+    twf[k2] = make_float2(twi[k2].x, -twi[k2].y);
+  }
+  float gc[2][8];
+  {
+    const float* src = gcin + (size_t)b * kM * kM + tid * 16;
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) gc[q][k1] = src[q * 8 + k1];
+  }
+  float2* xs = sX + g * kXS;
+  __syncthreads();
   const float inv = 1.0f / (float)(kM * kM);
   for (int k = kbeg; k < kend; ++k) {
     const float2* K = kern + (size_t)k * T;
@@ -919,11 +972,12 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd(
         v[n2] = (ny >= -Ny && ny <= Ny) ? sB[(ny + Ny) * kSB + ux] : make_float2(0.f, 0.f);
       }
       fft64_group<1>(v, twi, xs, t, true);
-      // conj(e) g_c, then forward along uy
+      // conj(e) g_c (g_c real), then forward along uy
 #pragma unroll
       for (int k1 = 0; k1 < 8; ++k1) {
-        const float2 e = v[k1], c = gc[q][k1];
-        v[k1] = make_float2(e.x * c.x + e.y * c.y, e.x * c.y - e.y * c.x);
+        const float2 e = v[k1];
+        const float c = gc[q][k1];
+        v[k1] = make_float2(e.x * c, -e.y * c);
       }
       fft64_group<-1>(v, twf, xs, t, true);
 #pragma unroll
@@ -966,6 +1020,21 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd(
     return;
   }
   gam_to_spec(sGam, Nx, Ny, dcx, dcy, Lx, Ly, gspec + (size_t)b * 32 * Gx);
+}
+
+// The forward coarse stage from each image's M^ (k_band_assemble): grid
+// (kg, B) as k_socs_mid.
+__global__ __launch_bounds__(256) void k_socs_mid_full(const float2* __restrict__ Mfull, int Nx,
+                                                       int Ny, const float2* __restrict__ kern,
+                                                       const float* __restrict__ scales,
+                                                       int nk, const float2* __restrict__ twM,
+                                                       float2* __restrict__ spec,
+                                                       float* __restrict__ ipart) {
+  const int grp = blockIdx.x, kg = gridDim.x, b = blockIdx.y;
+  const int per = (nk + kg - 1) / kg, k0 = min(nk, grp * per), k1 = min(nk, k0 + per);
+  mid_image<true>(Mfull + (size_t)b * (2 * Nx + 1) * (2 * Ny + 1), 0, Nx, Ny, 1.f, kern, scales,
+                  k0, k1, twM, spec + (size_t)b * 32 * (4 * Nx + 1), 32,
+                  kg > 1 ? ipart + ((size_t)b * kg + grp) * kM * kM : nullptr);
 }
 
 // Sums the kg partial Gam of each image and writes its gradient spectrum.
@@ -1890,6 +1959,9 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
   KLT_TRY(ctx->lpart.ensure((size_t)SB * nblk * 4 * sizeof(float)));
   KLT_TRY(ctx->ipart.ensure((size_t)kGroupSlots * kM * kM * sizeof(float)));
   KLT_TRY(ctx->gam_part.ensure((size_t)kGroupSlots * (2 * Nx + 1) * (2 * Ny + 1) * sizeof(float2)));
+  KLT_TRY(ctx->mhat.ensure((size_t)SB * (2 * Nx + 1) * Wy * sizeof(float2)));
+  KLT_TRY(ctx->gcoarse.ensure((size_t)SB * kM * kM * sizeof(float)));
+  KLT_TRY(ctx->gfull.ensure((size_t)SB * Gx * Gy * sizeof(float2)));
   if (!images) {
     KLT_TRY(ctx->bwd_img.ensure((size_t)SB * Lx * Ly * sizeof(float)));
   }
@@ -1904,10 +1976,13 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
       launch_band<2, false>(ctx, "k_band_mfma", nblk, ks, nb, masks + off, nullptr, 0.f, 0.f,
                             Lx, Ly, ft->Kpad, ft->Wf, p->twy_in.as<float2>(), p->NP, Nx, Ny,
                             16, ctx->qpart.as<float2>(), nullptr);
-      KLT_LAUNCH(ctx, "k_socs_mid", k_socs_mid, dim3(c.kg, nb), dim3(256), 0,
-                 ctx->qpart.as<float2>(), parts, Nx, Ny, inv_area, kb->kern.as<float2>(),
-                 kb->scales.as<float>(), kb->nk, ft->tw64.as<float2>(), ctx->spec.as<float2>(),
-                 ctx->ipart.as<float>());
+      // each image's M^ once, for the forward and the adjoint coarse stages
+      KLT_LAUNCH(ctx, "k_band_assemble", k_band_assemble,
+                 dim3((unsigned)((2 * Nx + 1) * Wy + 255) / 256, nb), dim3(256), 0,
+                 ctx->qpart.as<float2>(), parts, Nx, Ny, inv_area, ctx->mhat.as<float2>());
+      KLT_LAUNCH(ctx, "k_socs_mid", k_socs_mid_full, dim3(c.kg, nb), dim3(256), 0,
+                 ctx->mhat.as<float2>(), Nx, Ny, kb->kern.as<float2>(), kb->scales.as<float>(),
+                 kb->nk, ft->tw64.as<float2>(), ctx->spec.as<float2>(), ctx->ipart.as<float>());
       if (c.kg > 1)
         KLT_LAUNCH(ctx, "k_socs_mid_fin", k_socs_mid_fin, dim3(nb), dim3(256), 0,
                    ctx->ipart.as<float>(), c.kg, Nx, Ny, ft->tw64.as<float2>(),
@@ -1918,11 +1993,18 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
       launch_band<4, true>(ctx, "k_band_resist", nblk, ks, nb, I, targets + off, a_sig, thr, Lx,
                            Ly, ft->Kpad, ft->Wgf, ft->twyg.as<float2>(), 32, 2 * Nx, 2 * Ny, 32,
                            ctx->gpart.as<float2>(), ctx->lpart.as<float>());
+      // each image's G^, coarse-grid g_c and loss once, for all kernel groups
+      KLT_LAUNCH(ctx, "k_resist_sum", k_resist_sum,
+                 dim3((unsigned)(Gx * Gy + 255) / 256, nb), dim3(256), 0,
+                 ctx->gpart.as<float2>(), ctx->lpart.as<float>(), parts, Nx, Ny, inv_area,
+                 ctx->gfull.as<float2>(), losses + b0);
+      KLT_LAUNCH(ctx, "k_resist_coarse", k_resist_coarse, dim3(nb), dim3(256), 0,
+                 ctx->gfull.as<float2>(), Nx, Ny, ft->tw64.as<float2>(),
+                 ctx->gcoarse.as<float>());
       KLT_LAUNCH(ctx, "k_socs_mid_bwd", k_socs_mid_bwd, dim3(c.kg, nb), dim3(256), 0,
-                 ctx->qpart.as<float2>(), ctx->gpart.as<float2>(), ctx->lpart.as<float>(),
-                 parts, Nx, Ny, inv_area, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk,
-                 ft->tw64.as<float2>(), dcx, dcy, Lx, Ly, ctx->gspec.as<float2>(),
-                 losses + b0, ctx->gam_part.as<float2>());
+                 ctx->mhat.as<float2>(), ctx->gcoarse.as<float>(), Nx, Ny,
+                 kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk, ft->tw64.as<float2>(),
+                 dcx, dcy, Lx, Ly, ctx->gspec.as<float2>(), ctx->gam_part.as<float2>());
       if (c.kg > 1)
         KLT_LAUNCH(ctx, "k_socs_mid_bwd_fin", k_socs_mid_bwd_fin, dim3(nb), dim3(256), 0,
                    ctx->gam_part.as<float2>(), c.kg, Nx, Ny, dcx, dcy, Lx, Ly,
