@@ -136,6 +136,7 @@ struct klt_ctx {
   klt::DevBuf tw64;                           // e^{+2 pi i j/64}, j < 64
   klt::DevBuf ipart, gam_part;                // kernel-group partials of the coarse stages
   klt::DevBuf mhat, gfull, gcoarse;           // adjoint: per-image M^, G^ and coarse-grid g_c
+  klt::DevBuf gsum;                           // kernel-group partials summed per image
   klt::DevBuf hspec;                          // Hopkins spectra of the source gradient
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::SocsPlan>> socs_plans;
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::FusedTables>> fused_tables;

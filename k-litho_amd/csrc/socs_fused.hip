@@ -705,6 +705,19 @@ __global__ __launch_bounds__(256) void k_socs_mid(const float2* __restrict__ mpa
                    kg > 1 ? ipart + ((size_t)b * kg + grp) * kM * kM : nullptr);
 }
 
+// grid (ceil(n / 256), B), one thread per element: out[b][e] = sum over
+// j < kg (in order) of part[b][j][e], the kernel-group partials of the coarse
+// stages, ahead of their per-image finishing transforms.
+__global__ __launch_bounds__(256) void k_sum_groups(const float* __restrict__ part, int kg,
+                                                    int n, float* __restrict__ out) {
+  const int b = blockIdx.y, e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= n) return;
+  const float* p = part + (size_t)b * kg * n + e;
+  float s = 0.f;
+  for (int j = 0; j < kg; ++j) s += p[(size_t)j * n];
+  out[(size_t)b * n + e] = s;
+}
+
 // Sums the kg partial coarse intensities of each image and transforms them
 // to the intensity spectrum [B][32][4Nx+1].
 __global__ __launch_bounds__(256) void k_socs_mid_fin(const float* __restrict__ ipart, int kg,
@@ -1676,6 +1689,7 @@ int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, 
   KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk * 4 * Wy * 16 * sizeof(float2)));
   KLT_TRY(ctx->spec.ensure((size_t)SB * 32 * Gx * sizeof(float2)));
   KLT_TRY(ctx->ipart.ensure((size_t)kGroupSlots * kM * kM * sizeof(float)));
+  KLT_TRY(ctx->gsum.ensure((size_t)SB * kM * kM * sizeof(float)));
   const float inv_area = 1.0f / ((float)Lx * (float)Ly);
   for (int b0 = 0; b0 < B; b0 += SB) {
     const int nb = std::min(SB, B - b0);
@@ -1689,10 +1703,13 @@ int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, 
                  ctx->qpart.as<float2>(), nblk * c.ks, Nx, Ny, inv_area, kb->kern.as<float2>(),
                  kb->scales.as<float>(), kb->nk, ft->tw64.as<float2>(), ctx->spec.as<float2>(),
                  ctx->ipart.as<float>());
-      if (c.kg > 1)
+      if (c.kg > 1) {
+        KLT_LAUNCH(ctx, "k_sum_groups", k_sum_groups, dim3(kM * kM / 256, nb), dim3(256), 0,
+                   ctx->ipart.as<float>(), c.kg, kM * kM, ctx->gsum.as<float>());
         KLT_LAUNCH(ctx, "k_socs_mid_fin", k_socs_mid_fin, dim3(nb), dim3(256), 0,
-                   ctx->ipart.as<float>(), c.kg, Nx, Ny, ft->tw64.as<float2>(),
+                   ctx->gsum.as<float>(), 1, Nx, Ny, ft->tw64.as<float2>(),
                    ctx->spec.as<float2>());
+      }
       KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma, dim3(nstrips * c.rs, nb), dim3(256), 0,
                  ctx->spec.as<float2>(), Nx, Lx, Ly, ft->Lypad, c.rs, ft->Ef.as<float4>(),
                  ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), out);
@@ -1962,6 +1979,7 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
   KLT_TRY(ctx->mhat.ensure((size_t)SB * (2 * Nx + 1) * Wy * sizeof(float2)));
   KLT_TRY(ctx->gcoarse.ensure((size_t)SB * kM * kM * sizeof(float)));
   KLT_TRY(ctx->gfull.ensure((size_t)SB * Gx * Gy * sizeof(float2)));
+  KLT_TRY(ctx->gsum.ensure((size_t)SB * std::max(kM * kM, 2 * (2 * Nx + 1) * Wy) * sizeof(float)));
   if (!images) {
     KLT_TRY(ctx->bwd_img.ensure((size_t)SB * Lx * Ly * sizeof(float)));
   }
@@ -1983,10 +2001,13 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
       KLT_LAUNCH(ctx, "k_socs_mid", k_socs_mid_full, dim3(c.kg, nb), dim3(256), 0,
                  ctx->mhat.as<float2>(), Nx, Ny, kb->kern.as<float2>(), kb->scales.as<float>(),
                  kb->nk, ft->tw64.as<float2>(), ctx->spec.as<float2>(), ctx->ipart.as<float>());
-      if (c.kg > 1)
+      if (c.kg > 1) {
+        KLT_LAUNCH(ctx, "k_sum_groups", k_sum_groups, dim3(kM * kM / 256, nb), dim3(256), 0,
+                   ctx->ipart.as<float>(), c.kg, kM * kM, ctx->gsum.as<float>());
         KLT_LAUNCH(ctx, "k_socs_mid_fin", k_socs_mid_fin, dim3(nb), dim3(256), 0,
-                   ctx->ipart.as<float>(), c.kg, Nx, Ny, ft->tw64.as<float2>(),
+                   ctx->gsum.as<float>(), 1, Nx, Ny, ft->tw64.as<float2>(),
                    ctx->spec.as<float2>());
+      }
       KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma, dim3(nstrips * rs, nb), dim3(256), 0,
                  ctx->spec.as<float2>(), Nx, Lx, Ly, ft->Lypad, rs, ft->Ef.as<float4>(),
                  ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), I);
@@ -2005,10 +2026,15 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
                  ctx->mhat.as<float2>(), ctx->gcoarse.as<float>(), Nx, Ny,
                  kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk, ft->tw64.as<float2>(),
                  dcx, dcy, Lx, Ly, ctx->gspec.as<float2>(), ctx->gam_part.as<float2>());
-      if (c.kg > 1)
+      if (c.kg > 1) {
+        const int n2 = 2 * (2 * Nx + 1) * Wy;  // Gam entries as floats
+        KLT_LAUNCH(ctx, "k_sum_groups", k_sum_groups, dim3((n2 + 255) / 256, nb), dim3(256), 0,
+                   reinterpret_cast<const float*>(ctx->gam_part.as<float2>()), c.kg, n2,
+                   ctx->gsum.as<float>());
         KLT_LAUNCH(ctx, "k_socs_mid_bwd_fin", k_socs_mid_bwd_fin, dim3(nb), dim3(256), 0,
-                   ctx->gam_part.as<float2>(), c.kg, Nx, Ny, dcx, dcy, Lx, Ly,
-                   ctx->gspec.as<float2>());
+                   reinterpret_cast<const float2*>(ctx->gsum.as<float>()), 1, Nx, Ny, dcx,
+                   dcy, Lx, Ly, ctx->gspec.as<float2>());
+      }
       KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma, dim3(nstrips * rs, nb), dim3(256), 0,
                  ctx->gspec.as<float2>(), Nx, Lx, Ly, ft->Lypad, rs, ft->Ef.as<float4>(),
                  ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), grads + off);
