@@ -393,8 +393,6 @@ __global__ void k_pack_kernels(const float* __restrict__ kr, const float* __rest
   if (i < n) out[i] = make_float2(kr[i], ki[i]);
 }
 
-int round_up(int v, int m) { return (v + m - 1) / m * m; }
-
 }  // namespace
 
 namespace klt {

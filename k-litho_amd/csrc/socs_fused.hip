@@ -290,7 +290,7 @@ __device__ __forceinline__ void band_block(
     for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
     if (lane == 0) red[wave] = lsum;
   }
-  const int Wy = 2 * By + 1, nxp = Bx + 1;
+  const int Wy = 2 * By + 1;
   // this lane's column twiddles: A[|ny| = 16 nt + r16][row 4 (lane / 16) + t]
   // of row tile a (zero for rows past the image and |ny| > By)
   float2 tw[NT][2][4];
@@ -1095,7 +1095,7 @@ __global__ __launch_bounds__(256) void k_abbe_grad_coarse(
   const int per = (npts + ngrp - 1) / ngrp, s0 = min(npts, grp * per), s1 = min(npts, s0 + per);
   const int g = tid >> 3, t = tid & 7, wave = tid >> 6, lane = tid & 63;
   const int Nx = pg.Nx, Ny = pg.Ny, Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
-  const int Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
+  const int Gy = 4 * Ny + 1;
   for (int i = tid; i < T; i += 256) {
     const int nyi = i / Wx, nx = i - nyi * Wx - Nx;
     const int ny = nyi - Ny;
