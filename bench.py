@@ -378,6 +378,23 @@ def main():
             "algorithmic_bytes_per_launch": kern[dom]["algorithmic_bytes_per_launch"]}
     path_gbs = 8.0 * L * L * B / (ms / 1e3) / 1e9
 
+    # what plain streaming kernels reach on the same buffers: a read-only
+    # reduction of the masks, a fill and a copy of the images (2 GiB each)
+    nbytes = 4.0 * masks.numel()
+    rates = {}
+    for name, fn in (("read", lambda: masks.sum()), ("write", lambda: images.fill_(0.5)),
+                     ("copy", lambda: images.copy_(masks))):
+        fn()  # first launch loads the kernel
+        rates[name] = timer(fn, 5)
+    r_ms, w_ms, c_ms = rates["read"], rates["write"], rates["copy"]
+    stream_rates = {"read_gbs": nbytes / (r_ms / 1e3) / 1e9,
+                    "write_gbs": nbytes / (w_ms / 1e3) / 1e9,
+                    "copy_gbs": 2 * nbytes / (c_ms / 1e3) / 1e9,
+                    "bytes": nbytes,
+                    "note": "torch sum over the masks, fill_ and copy_ into the images"}
+    log(f"streaming reference rates: read {stream_rates['read_gbs']:.0f}, write "
+        f"{stream_rates['write_gbs']:.0f}, copy {stream_rates['copy_gbs']:.0f} GB/s")
+
     # plain single-instance path: one call per image
     one_m = masks[:1].clone()
     one_i = torch.empty_like(one_m)
@@ -437,6 +454,7 @@ def main():
         "plain_images_per_s": plain,
         "speedup_vs_plain": value / plain,
         "kernels": kern,
+        "hbm_stream_rates": stream_rates,
     }
     out.update(extras)
     print(json.dumps(out))

@@ -77,6 +77,13 @@ int klt_ctx_profile_report(klt_ctx* ctx, char* buf, int len);
  * kernel set) runs the candidate splits once each, timed with HIP events,
  * and keeps the fastest for later calls; off, a fixed rule decides. */
 int klt_ctx_set_autotune(klt_ctx* ctx, int on);
+/* Pins the split factors of later klt_socs_forward / klt_socs_backward calls
+ * (no reference counterpart): ks column ranges of the band pass (at most 8),
+ * rs row ranges of the synthesis pass (at most 16), kg kernel groups of the
+ * coarse-grid stage; 0 leaves a factor to the tuner or the fixed rule.  A
+ * factor that does not fit a launch shape is lowered to the largest that
+ * does.  Pinned shapes are not timed and not recorded in the tune report. */
+int klt_ctx_set_splits(klt_ctx* ctx, int ks, int rs, int kg);
 /* One line per tuned shape: "socs_forward|socs_backward B Lx Ly Nx Ny nk
  * column_splits row_splits kernel_groups ms"; returns the byte count or an
  * error. */

@@ -239,6 +239,16 @@ int klt_ctx_set_autotune(klt_ctx* ctx, int on) {
   return KLT_OK;
 }
 
+int klt_ctx_set_splits(klt_ctx* ctx, int ks, int rs, int kg) {
+  if (!ctx) return klt::set_error(KLT_ERR_ARG, "klt_ctx_set_splits: ctx is NULL");
+  if (ks < 0 || rs < 0 || kg < 0)
+    return klt::set_error(KLT_ERR_ARG, "klt_ctx_set_splits: negative split factor");
+  ctx->pin_ks = ks;
+  ctx->pin_rs = rs;
+  ctx->pin_kg = kg;
+  return KLT_OK;
+}
+
 int klt_ctx_tune_report(klt_ctx* ctx, char* buf, int len) {
   if (!ctx || !buf || len <= 0)
     return klt::set_error(KLT_ERR_ARG, "klt_ctx_tune_report: bad argument");

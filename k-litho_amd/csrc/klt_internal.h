@@ -148,6 +148,8 @@ struct klt_ctx {
   };
   bool autotune = true;
   std::map<std::array<int, 7>, Tuned> fused_tune;
+  // split factors pinned by klt_ctx_set_splits (0: tuned or fixed rule)
+  int pin_ks = 0, pin_rs = 0, pin_kg = 0;
   // captured mixed batches, most recent last (a few kept), and the last one run
   std::vector<std::unique_ptr<klt::BatchGraph>> batches;
   klt::BatchGraph* last_batch = nullptr;

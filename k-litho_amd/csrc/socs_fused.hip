@@ -41,6 +41,8 @@ constexpr int kColsSynth = 128; // image columns per synthesis block
 constexpr int kPStride = 72;    // LDS row stride (halves) of the P operand
 constexpr int kBandChunk = 4;   // 32-column steps per LDS twiddle chunk
 constexpr int kGroupSlots = 2048;  // (image, kernel group) partials of one launch
+constexpr int kMaxKs = 8;        // band column ranges per row block, at most
+constexpr int kMaxRs = 16;       // synthesis row ranges per strip, at most
 // LDS strides of the coarse-grid stage, in float2: 72 = 8 (mod 16) puts the
 // 64 lanes of a wave on distinct banks (two dwords each) for both the row
 // (sB[g][t + 8 k]) and the column (sB[t + 8 n][g]) accesses of a 64 x 64 grid;
@@ -1629,6 +1631,18 @@ static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, in
   while (h.rs < 4 && nstrips * h.rs * nb < 512 && split_ok(nrow32, 2 * h.rs)) h.rs *= 2;
   auto kg_ok = [&](int v) { return v <= nk && (v == 1 || v * nb <= kGroupSlots); };
   while (h.kg < 8 && h.kg * nb < 256 && kg_ok(2 * h.kg)) h.kg *= 2;
+  if (ctx->pin_ks || ctx->pin_rs || ctx->pin_kg) {
+    auto fit = [](int v, int hi, const std::function<bool(int)>& ok) {
+      v = std::min(v, hi);
+      while (v > 1 && !ok(v)) --v;
+      return std::max(v, 1);
+    };
+    if (ctx->pin_ks) h.ks = fit(ctx->pin_ks, kMaxKs, [&](int v) { return split_ok(nchunk, v); });
+    if (ctx->pin_rs) h.rs = fit(ctx->pin_rs, kMaxRs, [&](int v) { return split_ok(nrow32, v); });
+    if (ctx->pin_kg) h.kg = fit(ctx->pin_kg, nk, kg_ok);
+    *out = h;
+    return KLT_OK;
+  }
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   KLT_HIP(hipStreamIsCapturing(ctx->stream, &cap));
   if (!ctx->autotune || ctx->profiling || cap != hipStreamCaptureStatusNone) {
@@ -1648,14 +1662,14 @@ static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, in
   klt_ctx::Tuned best = h;
   KLT_TRY(time_run(best, &best.ms));
   float ms = 0.f;
-  for (int v = 1; v <= 4; v *= 2) {
+  for (int v = 1; v <= kMaxKs; v *= 2) {
     if (v == best.ks || !split_ok(nchunk, v)) continue;
     klt_ctx::Tuned c = best;
     c.ks = v;
     KLT_TRY(time_run(c, &ms));
     if (ms < best.ms) best = c, best.ms = ms;
   }
-  for (int v = 1; v <= 4; v *= 2) {
+  for (int v = 1; v <= kMaxRs; v *= 2) {
     if (v == best.rs || !split_ok(nrow32, v)) continue;
     klt_ctx::Tuned c = best;
     c.rs = v;
@@ -1686,7 +1700,7 @@ int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, 
   const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
   const int nstrips = (Lx + kColsSynth - 1) / kColsSynth;
   const int SB = std::min(B, 1024);
-  KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk * 4 * Wy * 16 * sizeof(float2)));
+  KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk * kMaxKs * Wy * 16 * sizeof(float2)));
   KLT_TRY(ctx->spec.ensure((size_t)SB * 32 * Gx * sizeof(float2)));
   KLT_TRY(ctx->ipart.ensure((size_t)kGroupSlots * kM * kM * sizeof(float)));
   KLT_TRY(ctx->gsum.ensure((size_t)SB * kM * kM * sizeof(float)));
@@ -1969,11 +1983,11 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
   const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
   const int nstrips = (Lx + kColsSynth - 1) / kColsSynth;
   const int SB = std::min(B, 512);
-  KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk * 4 * Wy * 16 * sizeof(float2)));
+  KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk * kMaxKs * Wy * 16 * sizeof(float2)));
   KLT_TRY(ctx->spec.ensure((size_t)SB * 32 * Gx * sizeof(float2)));
-  KLT_TRY(ctx->gpart.ensure((size_t)SB * nblk * 4 * Gy * 32 * sizeof(float2)));
+  KLT_TRY(ctx->gpart.ensure((size_t)SB * nblk * kMaxKs * Gy * 32 * sizeof(float2)));
   KLT_TRY(ctx->gspec.ensure((size_t)SB * 32 * Gx * sizeof(float2)));
-  KLT_TRY(ctx->lpart.ensure((size_t)SB * nblk * 4 * sizeof(float)));
+  KLT_TRY(ctx->lpart.ensure((size_t)SB * nblk * kMaxKs * sizeof(float)));
   KLT_TRY(ctx->ipart.ensure((size_t)kGroupSlots * kM * kM * sizeof(float)));
   KLT_TRY(ctx->gam_part.ensure((size_t)kGroupSlots * (2 * Nx + 1) * (2 * Ny + 1) * sizeof(float2)));
   KLT_TRY(ctx->mhat.ensure((size_t)SB * (2 * Nx + 1) * Wy * sizeof(float2)));

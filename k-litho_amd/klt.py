@@ -63,6 +63,7 @@ def lib() -> C.CDLL:
         L.klt_ctx_set_profiling.argtypes = [vp, i]
         L.klt_ctx_profile_report.argtypes = [vp, C.c_char_p, i]
         L.klt_ctx_set_autotune.argtypes = [vp, i]
+        L.klt_ctx_set_splits.argtypes = [vp, i, i, i]
         L.klt_ctx_tune_report.argtypes = [vp, C.c_char_p, i]
         f = C.c_float
         cs = C.c_char_p
@@ -254,6 +255,11 @@ class Context:
 
     def set_autotune(self, on: bool):
         check(lib().klt_ctx_set_autotune(self._h, 1 if on else 0))
+
+    def set_splits(self, ks: int = 0, rs: int = 0, kg: int = 0):
+        """Pins the band column splits, synthesis row splits and kernel groups
+        of later batched calls (0: automatic)."""
+        check(lib().klt_ctx_set_splits(self._h, ks, rs, kg))
 
     def tune_report(self) -> list:
         """[(kind, B, Lx, Ly, Nx, Ny, nk, column_splits, row_splits, kernel_groups, ms)]
