@@ -133,6 +133,7 @@ struct klt_ctx {
   klt::DevBuf abbe_tcc, abbe_band, abbe_in;   // source-gradient scratch / staging
   klt::DevBuf tcc_pts;                        // source points of the TCC build
   klt::DevBuf tcc_part;                       // split-K partial sums of the TCC Gram
+  klt::DevBuf tcc_jobs;                       // per-block jobs of the TCC Gram
   klt::DevBuf tw64;                           // e^{+2 pi i j/64}, j < 64
   klt::DevBuf ipart, gam_part;                // kernel-group partials of the coarse stages
   klt::DevBuf mhat, gfull, gcoarse;           // adjoint: per-image M^, G^ and coarse-grid g_c

@@ -104,8 +104,13 @@ __global__ __launch_bounds__(256) void k_split_operands(
 // steps ahead into two register sets (the loop runs two steps per pass; Kpad
 // and kc are multiples of 64, and the loads past the chunk's end re-read its
 // last step).
-// Split-precision Gram, one 128 x 128 output tile (tile list entry blockIdx.x)
-// over the K range ks * kc .. + kc (blockIdx.y).  Waves 2 x 2, each a 64 x 64
+// Split-precision Gram, one 128 x 128 output tile over one K range per block:
+// jobs[blockIdx.x] = (row panel, column panel, K range ks * kc .. + kc, tile
+// list index), or x < 0 for an idle block.  The host lays the jobs out so
+// that the blocks of one XCD (block ids equal mod 8) take consecutive jobs of
+// one K range: they share operand panels, which then come from that XCD's L2
+// instead of each block reading its own from the memory-side cache.
+// Waves 2 x 2, each a 64 x 64
 // sub-tile of 4 x 4 MFMA blocks (TCC_R and TCC_I).  Per 32-wide K step the
 // block stages the step's U, V1, V2 hi / lo rows (6 x 128 rows x 32 halves)
 // in LDS, and issues the next step's global loads before its products.
@@ -115,10 +120,12 @@ __global__ __launch_bounds__(256, 1) void k_gram_split16(
     const _Float16* __restrict__ Uh, const _Float16* __restrict__ Ul,
     const _Float16* __restrict__ V1h, const _Float16* __restrict__ V1l,
     const _Float16* __restrict__ V2h, const _Float16* __restrict__ V2l, int Kpad, int kc,
-    const int2* __restrict__ tiles, float2* __restrict__ part) {
+    const int4* __restrict__ jobs, int ntiles, float2* __restrict__ part) {
   __shared__ __attribute__((aligned(16))) _Float16 sop[6][kGT * kGLd];
-  const int tile = blockIdx.x, ks = blockIdx.y, ntiles = gridDim.x;
-  const int2 t = tiles[tile];
+  const int4 job = jobs[blockIdx.x];
+  if (job.x < 0) return;
+  const int2 t = make_int2(job.x, job.y);
+  const int ks = job.z, tile = job.w;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wr = wave >> 1, wc = wave & 1;
   const int r16 = lane & 15, kq = (lane >> 4) * 8;
@@ -406,10 +413,26 @@ int tcc_build_device(klt_ctx* ctx, const float* src_host, int srcSize, float out
   const int steps = Kpad / 32, ntl = (int)tiles.size();
   const int nks = std::max(1, std::min(steps, 256 / ntl));
   const int kc = (steps + nks - 1) / nks * 32;
+  // jobs in (K range, tile) order, dealt to the 8 XCDs in consecutive runs:
+  // block id 8 i + x (XCD x under the round-robin dispatch) takes job
+  // x * per + i
+  constexpr int kXcds = 8;
+  const int njobs = ntl * nks, per = (njobs + kXcds - 1) / kXcds;
+  std::vector<int4> jobs((size_t)per * kXcds, make_int4(-1, -1, 0, 0));
+  for (int x = 0; x < kXcds; ++x)
+    for (int i = 0; i < per; ++i) {
+      const int jb = x * per + i;
+      if (jb >= njobs) continue;
+      const int ks = jb / ntl, tl = jb - ks * ntl;
+      jobs[(size_t)i * kXcds + x] = make_int4(tiles[tl].x, tiles[tl].y, ks, tl);
+    }
+  KLT_TRY(ctx->tcc_jobs.ensure(sizeof(int4) * jobs.size()));
+  KLT_HIP(hipMemcpyAsync(ctx->tcc_jobs.ptr, jobs.data(), sizeof(int4) * jobs.size(),
+                         hipMemcpyHostToDevice, ctx->stream));
   KLT_TRY(ctx->tcc_part.ensure((size_t)nks * ntl * kGT * kGT * sizeof(float2)));
-  KLT_LAUNCH(ctx, "k_gram_split16", k_gram_split16, dim3((unsigned)ntl, (unsigned)nks),
-             dim3(256), 0, base, base + nel, base + 2 * nel, base + 3 * nel, base + 4 * nel,
-             base + 5 * nel, Kpad, kc, ctx->spec.as<int2>(), ctx->tcc_part.as<float2>());
+  KLT_LAUNCH(ctx, "k_gram_split16", k_gram_split16, dim3((unsigned)jobs.size()), dim3(256), 0,
+             base, base + nel, base + 2 * nel, base + 3 * nel, base + 4 * nel, base + 5 * nel,
+             Kpad, kc, ctx->tcc_jobs.as<int4>(), ntl, ctx->tcc_part.as<float2>());
   KLT_LAUNCH(ctx, "k_gram_reduce", k_gram_reduce, dim3((unsigned)ntl, kGT / 16), dim3(256), 0,
              ctx->tcc_part.as<float2>(), nks, ctx->spec.as<int2>(), T, 1.0 / (double)scale,
              tcc_dev);
