@@ -19,7 +19,8 @@
 //      eigenvalues closer than 1e-4 ||T|| form a cluster whose vectors are
 //      re-orthogonalised in eigenvalue order at every iteration (Gram-Schmidt
 //      twice, parallel over the vector length), one block per cluster.
-//   4. k_back_transform: x = H_0 ... H_{n-3} D z, one block per vector.
+//   4. k_back_transform: x = H_0 ... H_{n-3} D z, one block per vector (D by a
+//      block scan of the unit phases).
 // Textbook algorithms (Golub & Van Loan, ch. 8; Wilkinson's inverse
 // iteration); all arithmetic FP64.
 #include <algorithm>
@@ -350,16 +351,30 @@ __global__ void k_hh_finish(const double* __restrict__ A, int n, cd* __restrict_
 }
 
 // Number of eigenvalues of T_r below x (Sturm sequence of the LDL^T pivots).
+// d and e2 are in LDS; the loads of 8 steps are issued ahead of their
+// recurrence steps.
 __device__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
   int c = 0;
   double q = d[0] - x;
   if (fabs(q) < pivmin) q = -pivmin;
   if (q < 0) ++c;
-  for (int i = 1; i < n; ++i) {
-    q = d[i] - x - e2[i - 1] / q;
+  auto step = [&](double di, double ei) {
+    q = di - x - ei / q;
     if (fabs(q) < pivmin) q = -pivmin;
     if (q < 0) ++c;
+  };
+  int i = 1;
+  for (; i + 8 <= n; i += 8) {
+    double dv[8], ev[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      dv[u] = d[i + u];
+      ev[u] = e2[i - 1 + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) step(dv[u], ev[u]);
   }
+  for (; i < n; ++i) step(d[i], e2[i - 1]);
   return c;
 }
 
@@ -371,15 +386,23 @@ __global__ __launch_bounds__(64) void k_bisect(const double* __restrict__ d,
                                                const double* __restrict__ e2, int n, int k,
                                                double lo0, double hi0, double pivmin,
                                                double* __restrict__ w) {
+  extern __shared__ double sde[];  // d, then e2 (n entries each)
   const int idx = blockIdx.x, lane = threadIdx.x;
   if (idx >= k) return;
+  for (int i = lane; i < n; i += 64) {
+    sde[i] = d[i];
+    sde[n + i] = e2[i];
+  }
+  __syncthreads();
+  const double* ld = sde;
+  const double* le2 = sde + n;
   const int target = n - 1 - idx;  // ascending index
   double lo = lo0, hi = hi0;
   for (int it = 0; it < 64; ++it) {
     if (hi - lo <= 2.2e-16 * fmax(fabs(lo), fabs(hi)) + pivmin) break;
     const double h = hi - lo;
     const double x = lo + h * (double)(lane + 1) / 65.0;
-    const bool above = sturm_count(d, e2, n, x, pivmin) > target;
+    const bool above = sturm_count(ld, le2, n, x, pivmin) > target;
     const unsigned long long mask = __ballot(above);
     const int f = mask ? __ffsll(mask) - 1 : 64;  // first point past the eigenvalue
     const double nlo = f == 0 ? lo : lo + h * (double)f / 65.0;
@@ -393,7 +416,13 @@ __global__ __launch_bounds__(64) void k_bisect(const double* __restrict__ d,
 
 // LU of T_r - lam I with partial pivoting between rows i and i+1, into the
 // 5n doubles at lu: U diagonal, first and second superdiagonals, multipliers,
-// and the swap flags.
+// and the swap flags.  The sweeps below are serial in i; each runs in chunks
+// of kSweep steps whose loads are issued together ahead of the chunk's
+// recurrence, so that the memory latency is paid once per chunk instead of
+// once per step.
+constexpr int kSweep = 8;
+constexpr int kGs = 8;  // earlier vectors per Gram-Schmidt round (registers)
+
 __device__ void tri_lu(const double* __restrict__ d, const double* __restrict__ e, int n,
                        double lam, double tnorm, double* __restrict__ lu) {
   const double eps = 2.220446049250313e-16;
@@ -403,28 +432,54 @@ __device__ void tri_lu(const double* __restrict__ d, const double* __restrict__ 
   double* lu_l = lu + 3 * n;
   double* piv = lu + 4 * n;
   double a = d[0] - lam, b = n > 1 ? e[0] : 0.0;
-  for (int i = 0; i < n - 1; ++i) {
-    const double cdn = e[i], dn = d[i + 1] - lam, en = i + 1 < n - 1 ? e[i + 1] : 0.0;
+  // one elimination step: row i (a, b, 0) against row i + 1 (cdn, dn, en)
+  auto step = [&](double cdn, double dn, double en, double& od, double& ou1, double& ou2,
+                  double& ol, double& op) {
     if (fabs(a) >= fabs(cdn)) {
       if (a == 0.0) a = eps * tnorm;
       const double l = cdn / a;
-      lu_d[i] = a;
-      lu_u1[i] = b;
-      lu_u2[i] = 0.0;
-      lu_l[i] = l;
-      piv[i] = 0.0;
+      od = a;
+      ou1 = b;
+      ou2 = 0.0;
+      ol = l;
+      op = 0.0;
       a = dn - l * b;
       b = en;
     } else {
       const double l = a / cdn;
-      lu_d[i] = cdn;
-      lu_u1[i] = dn;
-      lu_u2[i] = en;
-      lu_l[i] = l;
-      piv[i] = 1.0;
+      od = cdn;
+      ou1 = dn;
+      ou2 = en;
+      ol = l;
+      op = 1.0;
       a = b - l * dn;
       b = -l * en;
     }
+  };
+  int i = 0;
+  for (; i + kSweep < n - 1; i += kSweep) {  // steps i .. i + kSweep - 1, all with i + 1 < n - 1
+    double cv[kSweep], dv[kSweep], ev[kSweep];
+#pragma unroll
+    for (int u = 0; u < kSweep; ++u) {
+      cv[u] = e[i + u];
+      dv[u] = d[i + 1 + u] - lam;
+      ev[u] = e[i + 1 + u];
+    }
+    double od[kSweep], ou1[kSweep], ou2[kSweep], ol[kSweep], op[kSweep];
+#pragma unroll
+    for (int u = 0; u < kSweep; ++u) step(cv[u], dv[u], ev[u], od[u], ou1[u], ou2[u], ol[u], op[u]);
+#pragma unroll
+    for (int u = 0; u < kSweep; ++u) {
+      lu_d[i + u] = od[u];
+      lu_u1[i + u] = ou1[u];
+      lu_u2[i + u] = ou2[u];
+      lu_l[i + u] = ol[u];
+      piv[i + u] = op[u];
+    }
+  }
+  for (; i < n - 1; ++i) {
+    const double cdn = e[i], dn = d[i + 1] - lam, en = i + 1 < n - 1 ? e[i + 1] : 0.0;
+    step(cdn, dn, en, lu_d[i], lu_u1[i], lu_u2[i], lu_l[i], piv[i]);
   }
   if (a == 0.0) a = eps * tnorm;
   lu_d[n - 1] = a;
@@ -437,8 +492,31 @@ __device__ void tri_solve(const double* __restrict__ lu, int n, double* __restri
   const double* lu_u2 = lu + 2 * n;
   const double* lu_l = lu + 3 * n;
   const double* piv = lu + 4 * n;
+  // L (with the row swaps): forward
   double xi = x[0];
-  for (int i = 0; i < n - 1; ++i) {
+  int i = 0;
+  for (; i + kSweep <= n - 1; i += kSweep) {
+    double pv[kSweep], lv[kSweep], xn[kSweep], xo[kSweep];
+#pragma unroll
+    for (int u = 0; u < kSweep; ++u) {
+      pv[u] = piv[i + u];
+      lv[u] = lu_l[i + u];
+      xn[u] = x[i + 1 + u];
+    }
+#pragma unroll
+    for (int u = 0; u < kSweep; ++u) {
+      if (pv[u] != 0.0) {
+        xo[u] = xn[u];
+        xi = xi - lv[u] * xn[u];
+      } else {
+        xo[u] = xi;
+        xi = xn[u] - lv[u] * xi;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kSweep; ++u) x[i + u] = xo[u];
+  }
+  for (; i < n - 1; ++i) {
     const double xn = x[i + 1];
     if (piv[i] != 0.0) {
       x[i] = xn;
@@ -449,12 +527,33 @@ __device__ void tri_solve(const double* __restrict__ lu, int n, double* __restri
     }
   }
   x[n - 1] = xi;
+  // U: backward
   double x1 = x[n - 1] / lu_d[n - 1];
   x[n - 1] = x1;
   if (n < 2) return;
   double x0 = (x[n - 2] - lu_u1[n - 2] * x1) / lu_d[n - 2];
   x[n - 2] = x0;
-  for (int i = n - 3; i >= 0; --i) {
+  i = n - 3;
+  for (; i - (kSweep - 1) >= 0; i -= kSweep) {
+    double xv[kSweep], u1v[kSweep], u2v[kSweep], dv[kSweep];
+#pragma unroll
+    for (int u = 0; u < kSweep; ++u) {
+      xv[u] = x[i - u];
+      u1v[u] = lu_u1[i - u];
+      u2v[u] = lu_u2[i - u];
+      dv[u] = lu_d[i - u];
+    }
+#pragma unroll
+    for (int u = 0; u < kSweep; ++u) {
+      const double v = (xv[u] - u1v[u] * x0 - u2v[u] * x1) / dv[u];
+      xv[u] = v;
+      x1 = x0;
+      x0 = v;
+    }
+#pragma unroll
+    for (int u = 0; u < kSweep; ++u) x[i - u] = xv[u];
+  }
+  for (; i >= 0; --i) {
     const double v = (x[i] - lu_u1[i] * x0 - lu_u2[i] * x1) / lu_d[i];
     x[i] = v;
     x1 = x0;
@@ -466,8 +565,8 @@ __device__ void tri_solve(const double* __restrict__ lu, int n, double* __restri
 // eigenvalues (cl_begin[c] .. cl_begin[c+1]).  Thread t of the block factors
 // T_r - lam_q I and solves with it for vector q = first + t; after every
 // round of solves the block re-orthogonalises the cluster's vectors in
-// eigenvalue order (classical Gram-Schmidt applied twice, against 32 earlier
-// vectors at a time, parallel over the vector length) and normalises them.
+// eigenvalue order (Gram-Schmidt applied twice, against kGs earlier vectors
+// at a time, parallel over the vector length) and normalises them.
 // Equal eigenvalues of a cluster are separated slightly in the shifts.
 __global__ __launch_bounds__(256) void k_inverse_iteration(
     const double* __restrict__ d, const double* __restrict__ e, int n,
@@ -498,20 +597,19 @@ __global__ __launch_bounds__(256) void k_inverse_iteration(
     for (int q = first; q < last; ++q) {
       double* x = z + (size_t)q * n;
       for (int pass = 0; pass < 2; ++pass)
-        for (int r0 = first; r0 < q; r0 += 32) {
-          const int nr = min(32, q - r0);
-          double acc[32];
+        for (int r0 = first; r0 < q; r0 += kGs) {
+          const int nr = min(kGs, q - r0);
+          double acc[kGs];
 #pragma unroll
-          for (int j = 0; j < 32; ++j) acc[j] = 0.0;
+          for (int j = 0; j < kGs; ++j) acc[j] = 0.0;
           for (int i = tid; i < n; i += 256) {
             const double xi = x[i];
 #pragma unroll
-            for (int j = 0; j < 32; ++j)
+            for (int j = 0; j < kGs; ++j)
               if (j < nr) acc[j] += z[(size_t)(r0 + j) * n + i] * xi;
           }
 #pragma unroll
-          for (int j = 0; j < 32; ++j) {
-            if (j >= nr) break;  // (uniform)
+          for (int j = 0; j < kGs; ++j) {
             const double v = wave_sum(acc[j]);
             if (lane == 0) red[wv][j] = v;
           }
@@ -521,7 +619,7 @@ __global__ __launch_bounds__(256) void k_inverse_iteration(
           for (int i = tid; i < n; i += 256) {
             double v = x[i];
 #pragma unroll
-            for (int j = 0; j < 32; ++j)
+            for (int j = 0; j < kGs; ++j)
               if (j < nr) v -= dots[j] * z[(size_t)(r0 + j) * n + i];
             x[i] = v;
           }
@@ -539,6 +637,39 @@ __global__ __launch_bounds__(256) void k_inverse_iteration(
   }
 }
 
+// sy[i] = D_i z[i], D_i = prod_{r < i} sub[r] / |sub[r]| (unit phases; 1 for a
+// zero sub[r]), for a block of kThreads threads: each thread forms the
+// running products over its own run of indices, a block scan combines the
+// runs' totals.  scan: kThreads complex values of LDS.
+__device__ void phase_scale(const double* __restrict__ z, int n, const cd* __restrict__ sub,
+                            cd* __restrict__ sy, cd* __restrict__ scan) {
+  const int tid = threadIdx.x;
+  const int run = (n + kThreads - 1) / kThreads, i0 = tid * run, i1 = min(n, i0 + run);
+  auto phase = [&](int r) {  // factor between D_r and D_{r+1}
+    const cd s = sub[r];
+    const double a = hypot(s.x, s.y);
+    return a > 0.0 ? cd{s.x / a, s.y / a} : cd{1.0, 0.0};
+  };
+  // this run's product of the factors r in [i0, i1) (D_{i1} / D_{i0})
+  cd tot = {1.0, 0.0};
+  for (int r = i0; r < i1 && r + 1 < n; ++r) tot = cmul(tot, phase(r));
+  scan[tid] = tot;
+  __syncthreads();
+  // inclusive scan (Hillis-Steele) of the runs' products
+  for (int o = 1; o < kThreads; o <<= 1) {
+    const cd prev = tid >= o ? scan[tid - o] : cd{1.0, 0.0};
+    __syncthreads();
+    if (tid >= o) scan[tid] = cmul(prev, scan[tid]);
+    __syncthreads();
+  }
+  cd dl = tid > 0 ? scan[tid - 1] : cd{1.0, 0.0};  // D_{i0}
+  for (int i = i0; i < i1; ++i) {
+    sy[i] = {dl.x * z[i], dl.y * z[i]};
+    if (i + 1 < n) dl = cmul(dl, phase(i));
+  }
+  __syncthreads();
+}
+
 // x = H_0 ... H_{n-3} D z for vector blockIdx.x; D_i = prod_{r<i} sub[r]/|sub[r]|.
 __global__ __launch_bounds__(kThreads) void k_back_transform(const double* __restrict__ zr,
                                                              int n, const cd* __restrict__ V,
@@ -549,18 +680,8 @@ __global__ __launch_bounds__(kThreads) void k_back_transform(const double* __res
   __shared__ double red[8];
   const int q = blockIdx.x;
   const double* z = zr + (size_t)q * n;
-  if (threadIdx.x == 0) {
-    cd dl = {1.0, 0.0};
-    sy[0] = {z[0], 0.0};
-    for (int i = 0; i + 1 < n; ++i) {
-      const cd s = sub[i];
-      const double a = hypot(s.x, s.y);
-      const cd u = a > 0.0 ? cd{s.x / a, s.y / a} : cd{1.0, 0.0};
-      dl = cmul(dl, u);
-      sy[i + 1] = {dl.x * z[i + 1], dl.y * z[i + 1]};
-    }
-  }
-  __syncthreads();
+  __shared__ cd scan[kThreads];
+  phase_scale(z, n, sub, sy, scan);
   for (int j = n - 3; j >= 0; --j) {
     const double t = tau[j];
     if (t == 0.0) continue;  // uniform
@@ -709,8 +830,12 @@ int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_d
   hi += 2.2e-16 * tnorm + pivmin;
   KLT_HIP(hipMemcpyAsync(e, he.data(), n * 8, hipMemcpyHostToDevice, st));
   KLT_HIP(hipMemcpyAsync(e2, he2.data(), n * 8, hipMemcpyHostToDevice, st));
-  KLT_LAUNCH(ctx, "k_bisect", k_bisect, dim3(k), dim3(64), 0, diag, e2, n, k, lo, hi, pivmin,
-             w_dev);
+  const size_t lds_de = (size_t)2 * n * 8;
+  if (lds_de > 64 * 1024)
+    KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bisect),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_de));
+  KLT_LAUNCH(ctx, "k_bisect", k_bisect, dim3(k), dim3(64), lds_de, diag, e2, n, k, lo, hi,
+             pivmin, w_dev);
   std::vector<double> hw(k);
   KLT_HIP(hipMemcpyAsync(hw.data(), w_dev, k * 8, hipMemcpyDeviceToHost, st));
   KLT_HIP(hipStreamSynchronize(st));
