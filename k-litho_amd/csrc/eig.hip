@@ -704,6 +704,73 @@ __global__ __launch_bounds__(kThreads) void k_back_transform(const double* __res
   for (int i = threadIdx.x; i < n; i += kThreads) out[(size_t)q * n + i] = sy[i];
 }
 
+// The same for n <= 256 C: thread t keeps y[t + 256 c] (c < C) in registers
+// for the whole sweep (so a step needs no barrier beyond its reduction) and
+// loads reflector j - 1 while it applies reflector j.
+template <int C>
+__global__ __launch_bounds__(kThreads) void k_back_transform_reg(const double* __restrict__ zr,
+                                                                 int n,
+                                                                 const cd* __restrict__ V,
+                                                                 const double* __restrict__ tau,
+                                                                 const cd* __restrict__ sub,
+                                                                 cd* __restrict__ out) {
+  extern __shared__ cd sy[];  // n entries
+  __shared__ double red[8];
+  __shared__ cd scan[kThreads];
+  const int q = blockIdx.x, tid = threadIdx.x;
+  phase_scale(zr + (size_t)q * n, n, sub, sy, scan);
+  const cd zero = {0.0, 0.0};
+  cd y[C], vc[C], vn[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const int i = tid + kThreads * c;
+    y[c] = i < n ? sy[i] : zero;
+  }
+  // row j of V holds the reflector's entries i >= j + 1 (the rest is unset)
+  auto load_row = [&](int j, cd(&dst)[C]) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int i = tid + kThreads * c;
+      dst[c] = (i > j && i < n) ? V[(size_t)j * n + i] : zero;
+    }
+  };
+  int j = n - 3;
+  if (j >= 0) load_row(j, vc);
+  double t = j >= 0 ? tau[j] : 0.0;
+  for (; j >= 0; --j) {
+    double tn = 0.0;
+    if (j > 0) {
+      load_row(j - 1, vn);
+      tn = tau[j - 1];
+    }
+    if (t != 0.0) {  // (uniform)
+      double sr = 0.0, si = 0.0;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const cd m = cmulc(vc[c], y[c]);
+        sr += m.x;
+        si += m.y;
+      }
+      block_sum2(sr, si, red);
+      const cd s = {t * sr, t * si};
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const cd m = cmul(vc[c], s);
+        y[c].x -= m.x;
+        y[c].y -= m.y;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) vc[c] = vn[c];
+    t = tn;
+  }
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const int i = tid + kThreads * c;
+    if (i < n) out[(size_t)q * n + i] = y[c];
+  }
+}
+
 // Kernel files layout (klitho_tcc.cpp:344-362): eigenvalues as float, and
 // krn_q[j] = z_q[T-1-j] as float real / imaginary parts.
 __global__ void k_kernels_from_vectors(const cd* __restrict__ vec, const double* __restrict__ w,
@@ -852,8 +919,19 @@ int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_d
   KLT_HIP(hipMemcpyAsync(cl, hcl.data(), hcl.size() * 4, hipMemcpyHostToDevice, st));
   KLT_LAUNCH(ctx, "k_inverse_iteration", k_inverse_iteration, dim3(ncl), dim3(256), 0, diag,
              e, n, w_dev, cl, tnorm, z, work);
-  KLT_LAUNCH(ctx, "k_back_transform", k_back_transform, dim3(k), dim3(kThreads), lds_v, z, n, V,
-             tau, sub, reinterpret_cast<cd*>(vec_dev));
+  if (n <= kThreads * 5) {
+    KLT_LAUNCH(ctx, "k_back_transform", k_back_transform_reg<5>, dim3(k), dim3(kThreads), lds_v,
+               z, n, V, tau, sub, reinterpret_cast<cd*>(vec_dev));
+  } else if (n <= kThreads * 8) {
+    if (lds_v > 64 * 1024)
+      KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_back_transform_reg<8>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_v));
+    KLT_LAUNCH(ctx, "k_back_transform", k_back_transform_reg<8>, dim3(k), dim3(kThreads), lds_v,
+               z, n, V, tau, sub, reinterpret_cast<cd*>(vec_dev));
+  } else {
+    KLT_LAUNCH(ctx, "k_back_transform", k_back_transform, dim3(k), dim3(kThreads), lds_v, z, n,
+               V, tau, sub, reinterpret_cast<cd*>(vec_dev));
+  }
   KLT_HIP(hipGetLastError());
   KLT_HIP(hipStreamSynchronize(st));  // host vectors above must outlive the copies
   return KLT_OK;
