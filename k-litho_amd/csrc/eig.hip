@@ -43,10 +43,30 @@ __device__ __forceinline__ cd cmulc(cd a, cd b) {  // conj(a) * b
   return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
 }
 
+// One DPP move of a double across lanes (both 32-bit halves with the same
+// pattern); lanes without a source read 0.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes, valid in every lane: pairs and quads (quad_perm),
+// rows of 16 (row_shr 4, 8: lane 15 of each row holds the row's sum), then
+// rows 0..3 into lane 63 (row_bcast 15 and 31), read back from lane 63.
+// Register-to-register moves, about a third of the latency of a
+// __shfl_xor butterfly.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp_move<0xb1>(v);       // quad_perm [1, 0, 3, 2]
+  v += dpp_move<0x4e>(v);       // quad_perm [2, 3, 0, 1]
+  v += dpp_move<0x114>(v);      // row_shr:4
+  v += dpp_move<0x118>(v);      // row_shr:8
+  v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
 }
 
 // Block sum of (a, b) over 256 threads; valid in every thread.
