@@ -20,11 +20,13 @@
 //   2. mask bands by the matrix-core band pass, M^ (2N+1)^2 per mask
 //      (k_abbe_assemble), and the Hopkins image I_A of each mask from it (one
 //      wave per spectrum entry, then the matrix-core synthesis);
-//   3. resist bands (G, loss partials) by the band pass;
+//   3. resist bands (G, loss partials) by the band pass, then per mask the
+//      loss and the coarse-grid g_c of G^ (resist_coarse_grid);
 //   4. the double sum on the 64 x 64 coarse grid (abbe_grad_coarse in
 //      socs_fused.hip): Lx Ly sum_{n, n'} A_s[n] conj(A_s[n']) conj(G^[n - n'])
 //      = Lx Ly / 64^2 sum_u g_c(u) |e_s(u)|^2 with e_s, g_c the 64-point
-//      transforms of A_s and G^, one 2D transform per source point.
+//      transforms of A_s and G^, one 2D transform per source point (two
+//      points per pass, the pupil values from a table built once per call).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -103,7 +105,9 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
   if (sigma > 1.0f) sigma = 1.0f;
   const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
   const int Gy = 4 * Ny + 1;
-  const int nblk = (Ly + 127) / 128;
+  // band partials: row blocks of 128 rows, each split into ks column ranges
+  const int ks = klt::band_split(Lx, Ly, nmask);
+  const int nblk = (Ly + 127) / 128 * ks;
   // outputs (device)
   float *dg = grad_src, *dl = losses, *di = images;
   if (flags & KLT_HOST_PTRS) {
@@ -129,12 +133,12 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
   KLT_TRY(ctx->abbe_band.ensure((size_t)nmask * T * sizeof(float2)));
   float2* Mfull = ctx->abbe_band.as<float2>();
   const float inv_area = 1.0f / ((float)Lx * (float)Ly);
-  KLT_TRY(klt::band_mask_partials(ctx, Lx, Ly, Nx, Ny, dm, nmask, ctx->qpart.as<float2>()));
+  KLT_TRY(klt::band_mask_partials(ctx, Lx, Ly, Nx, Ny, dm, nmask, ks, ctx->qpart.as<float2>()));
   KLT_LAUNCH(ctx, "k_abbe_assemble", k_abbe_assemble, dim3(nmask), dim3(kThreads), 0,
              ctx->qpart.as<float2>(), nblk, Nx, Ny, inv_area, Mfull);
   KLT_TRY(klt::hopkins_images_fused(ctx, Mfull, nmask, Lx, Ly, tcc, Nx, Ny, I));
   // 3. resist bands and loss partials
-  KLT_TRY(klt::band_resist_partials(ctx, Lx, Ly, Nx, Ny, I, dt, nmask, a_sig, thr,
+  KLT_TRY(klt::band_resist_partials(ctx, Lx, Ly, Nx, Ny, I, dt, nmask, ks, a_sig, thr,
                                     ctx->gpart.as<float2>(), ctx->lpart.as<float>()));
   // 4. per source point
   const int sh = (int)((srcSize - 1) / 2.0);
@@ -149,9 +153,12 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
   KLT_LAUNCH(ctx, "k_fill_zero", k_fill_zero, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0,
              dg, ng);
   const klt::PupilGeom g = klt::make_geom(srcSize, lambda, NA, defocus, nmed, Lx, Ly, Nx, Ny);
-  KLT_TRY(klt::abbe_grad_coarse(ctx, ctx->qpart.as<float2>(), ctx->gpart.as<float2>(),
-                                ctx->lpart.as<float>(), nmask, Lx, Ly, g, ctx->misc.as<int2>(),
-                                (int)pts.size(), srcSize, dg, dl));
+  KLT_TRY(ctx->gcoarse.ensure((size_t)nmask * 64 * 64 * sizeof(float)));
+  KLT_TRY(klt::resist_coarse_grid(ctx, ctx->gpart.as<float2>(), ctx->lpart.as<float>(), nblk,
+                                  nmask, Nx, Ny, inv_area, dl, ctx->gcoarse.as<float>()));
+  KLT_TRY(klt::abbe_grad_coarse(ctx, Mfull, ctx->gcoarse.as<float>(), nmask, Lx, Ly, g,
+                                ctx->misc.as<int2>(),
+                                (int)pts.size(), srcSize, dg));
   if (flags & KLT_HOST_PTRS) {
     KLT_HIP(hipMemcpyAsync(grad_src, dg, ng * 4, hipMemcpyDeviceToHost, st));
     KLT_HIP(hipMemcpyAsync(losses, dl, (size_t)nmask * 4, hipMemcpyDeviceToHost, st));

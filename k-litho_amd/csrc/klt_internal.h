@@ -131,6 +131,7 @@ struct klt_ctx {
   klt::DevBuf stage_in, stage_out, qpart, band, spec, ccols, misc, eig_ws;
   klt::DevBuf gpart, gspec, lpart, bwd_img;  // adjoint scratch
   klt::DevBuf abbe_tcc, abbe_band, abbe_in;   // source-gradient scratch / staging
+  klt::DevBuf abbe_pupil;                     // pupil table of the source-gradient points
   klt::DevBuf tcc_pts;                        // source points of the TCC build
   klt::DevBuf tcc_part;                       // split-K partial sums of the TCC Gram
   klt::DevBuf tcc_jobs;                       // per-block jobs of the TCC Gram
@@ -197,15 +198,18 @@ int socs_mid_from_band(klt_ctx* ctx, const klt_kernels* kb, const float2* band, 
                        float2* spec, int G);
 int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int B, int Lx,
                        int Ly, float* images);
-// Band partial sums of the fused passes (socs_fused.hip), nblk = ceil(Ly/128):
-// mask band  [B][nblk][2Ny+1][16]  (nx in [0, Nx], centre floor(L/2));
-// resist band [B][nblk][4Ny+1][32] (gx in [0, 2Nx], centre (L+1)/2), with the
-// loss partials lpart [B][nblk].  Kernel half-size <= 15.
+// Band partial sums of the fused passes (socs_fused.hip), nblk = ceil(Ly/128)
+// row blocks split into ks column ranges (band_split: the fixed rule of the
+// batched passes, parts = nblk * ks):
+// mask band  [B][parts][2Ny+1][16]  (nx in [0, Nx], centre floor(L/2));
+// resist band [B][parts][4Ny+1][32] (gx in [0, 2Nx], centre (L+1)/2), with the
+// loss partials lpart [B][parts].  Kernel half-size <= 15.
+int band_split(int Lx, int Ly, int B);
 int band_mask_partials(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const float* masks, int B,
-                       float2* out);
+                       int ks, float2* out);
 int band_resist_partials(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const float* images,
-                         const float* targets, int B, float a_sig, float thr, float2* out,
-                         float* lpart);
+                         const float* targets, int B, int ks, float a_sig, float thr,
+                         float2* out, float* lpart);
 // TCC on the device (tcc.hip) and the Hopkins image (socs.hip).
 int tcc_build_device(klt_ctx* ctx, const float* src_host, int srcSize, float outerSigma,
                      float lambda, float NA, float defocus, float nmed, int Lx, int Ly,
@@ -221,14 +225,19 @@ int hopkins_spec_device(klt_ctx* ctx, const float2* band, int B, const double* t
 int hopkins_images_fused(klt_ctx* ctx, const float2* band, int B, int Lx, int Ly,
                          const double* tcc, int Nx, int Ny, float* images);
 struct PupilGeom;  // klt_pupil.h
-// Source gradient of the resist loss for B masks (socs_fused.hip): from the
-// mask-band partials mpart, the resist-band partials gpart and loss partials
-// lpart (band_mask_partials / band_resist_partials), dL/dS at the npts source
-// points pts (grid offsets from the centre) into grad [B][srcSize^2], and the
-// losses.  Kernel half-size <= 15.
-int abbe_grad_coarse(klt_ctx* ctx, const float2* mpart, const float2* gpart,
-                     const float* lpart, int B, int Lx, int Ly, const PupilGeom& geom,
-                     const int2* pts, int npts, int srcSize, float* grad, float* loss);
+// From the resist-band partials gpart and loss partials lpart
+// (band_resist_partials, `parts` per image) of B images: the losses, and
+// g_c(u) = sum_g G^[g] e^{+2 pi i g.u/64} on the 64 x 64 coarse grid, thread-major
+// [B][256][16] floats (socs_fused.hip).  Kernel half-size <= 15.
+int resist_coarse_grid(klt_ctx* ctx, const float2* gpart, const float* lpart, int parts, int B,
+                       int Nx, int Ny, float inv_area, float* loss, float* gc);
+// Source gradient of the resist loss for B masks (socs_fused.hip): from each
+// mask's normalised band M^ [B][(2Ny+1)(2Nx+1)] and its coarse-grid g_c
+// (resist_coarse_grid), dL/dS at the npts source points pts (grid offsets from
+// the centre) into grad [B][srcSize^2].  Kernel half-size <= 15.
+int abbe_grad_coarse(klt_ctx* ctx, const float2* Mfull, const float* gc, int B, int Lx, int Ly,
+                     const PupilGeom& geom, const int2* pts, int npts, int srcSize,
+                     float* grad);
 // Top-k eigenpairs of a Hermitian matrix on the device (eig.hip).
 int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_dev,
                     double* vec_dev);

@@ -415,6 +415,24 @@ __global__ __launch_bounds__(256, FULL ? 4 : 2) void k_band_jobs(
                              nullptr);
 }
 
+// Sum over the 64 lanes, valid in every lane, by DPP moves: pairs and quads
+// (quad_perm), rows of 16 (row_shr 4, 8), rows into lane 63 (row_bcast 15,
+// 31), read back from lane 63.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ float dpp_move_f(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, true));
+}
+__device__ __forceinline__ float wave_sum_f(float v) {
+  v += dpp_move_f<0xb1>(v);       // quad_perm [1, 0, 3, 2]
+  v += dpp_move_f<0x4e>(v);       // quad_perm [2, 3, 0, 1]
+  v += dpp_move_f<0x114>(v);      // row_shr:4
+  v += dpp_move_f<0x118>(v);      // row_shr:8
+  v += dpp_move_f<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move_f<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 // 8-point DFT, X[k] = sum_n x[n] e^{s 2 pi i nk/8}, s = +1 (inverse) or -1.
 template <int S>
 __device__ __forceinline__ void dft8(float2* x) {
@@ -1084,138 +1102,120 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd_fin(const float2* __restri
 // place), then columns, then the dot product with g_c (registers) reduced over
 // the block.  loss[b] = sum of the resist-band loss partials.
 // ---------------------------------------------------------------------------
+// grid (ceil(T / 256), npts): ptab[s][i] = P(n, s) for frequency i = (ny + Ny)
+// (2Nx + 1) + nx + Nx of source point pq[s] (0 outside the pupil).
+__global__ __launch_bounds__(256) void k_abbe_pupil(klt::PupilGeom pg,
+                                                    const int2* __restrict__ pq,
+                                                    float2* __restrict__ ptab) {
+  const int s = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= pg.T) return;
+  const int Wx = 2 * pg.Nx + 1;
+  const int ny = i / Wx - pg.Ny, nx = i % Wx - pg.Nx;
+  const int2 p = pq[s];
+  float2 v = make_float2(0.f, 0.f);
+  (void)klt::pupil_value(pg, p.x, p.y, nx, ny, &v);
+  ptab[(size_t)s * pg.T + i] = v;
+}
+
 __global__ __launch_bounds__(256) void k_abbe_grad_coarse(
-    const float2* __restrict__ mpart, const float2* __restrict__ gpart,
-    const float* __restrict__ lpart, int nblk, klt::PupilGeom pg, const int2* __restrict__ pq,
-    int npts, int srcSize, float inv_area, double scale, const float2* __restrict__ tw64,
-    float* __restrict__ grad, float* __restrict__ loss) {
+    const float2* __restrict__ Mfull, const float* __restrict__ gcin, klt::PupilGeom pg,
+    const int2* __restrict__ pq, const float2* __restrict__ ptab, int npts, int srcSize,
+    double scale, const float2* __restrict__ tw64, float* __restrict__ grad) {
   __shared__ float2 sM[31 * 31];
-  __shared__ float2 sB[61 * kSB];
-  __shared__ float2 sX[32 * kXS];
-  __shared__ float red[4];
+  __shared__ float2 sB[62 * kSB];      // two points' rows (31 each)
+  __shared__ float2 sX[2 * 32 * kXS];  // FFT scratch of the two points
+  __shared__ float red[4][2];
   const int b = blockIdx.y, grp = blockIdx.x, ngrp = gridDim.x, tid = threadIdx.x;
   const int per = (npts + ngrp - 1) / ngrp, s0 = min(npts, grp * per), s1 = min(npts, s0 + per);
   const int g = tid >> 3, t = tid & 7, wave = tid >> 6, lane = tid & 63;
   const int Nx = pg.Nx, Ny = pg.Ny, Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
-  const int Gy = 4 * Ny + 1;
-  for (int i = tid; i < T; i += 256) {
-    const int nyi = i / Wx, nx = i - nyi * Wx - Nx;
-    const int ny = nyi - Ny;
-    const int r = nx >= 0 ? nyi : (-ny + Ny), c = nx >= 0 ? nx : -nx;
-    float2 s = make_float2(0.f, 0.f);
-    for (int k = 0; k < nblk; ++k) {
-      const float2 p = mpart[(((size_t)b * nblk + k) * Wy + r) * 16 + c];
-      s.x += p.x;
-      s.y += p.y;
-    }
-    if (nx < 0) s.y = -s.y;
-    sM[i] = make_float2(s.x * inv_area, s.y * inv_area);
-  }
-  if (tid == 0 && grp == 0) {
-    float l = 0.f;
-    for (int k = 0; k < nblk; ++k) l += lpart[(size_t)b * nblk + k];
-    loss[b] = l;
-  }
+  for (int i = tid; i < T; i += 256) sM[i] = Mfull[(size_t)b * T + i];
   float2 twi[8];
 #pragma unroll
   for (int k2 = 0; k2 < 8; ++k2) twi[k2] = tw64[(t * k2) & 63];
   float2* xs = sX + g * kXS;
-  // g_c rows: gy in [-2Ny, 2Ny] (row index gy + 2Ny), two rounds of 32 groups
-#pragma unroll
-  for (int rr = 0; rr < 2; ++rr) {
-    const int r = g + 32 * rr;
-    const bool act = r < Gy;
-    const int gy = r - 2 * Ny;
-    float2 v[8];
-    if (act) {
-#pragma unroll
-      for (int n2 = 0; n2 < 8; ++n2) {
-        const int n = t + 8 * n2;
-        const int gx = n < 32 ? n : n - 64;
-        float2 val = make_float2(0.f, 0.f);
-        if (gx >= -2 * Nx && gx <= 2 * Nx) {
-          const int row = gx >= 0 ? r : (-gy + 2 * Ny), col = gx >= 0 ? gx : -gx;
-          for (int k = 0; k < nblk; ++k) {
-            const float2 p = gpart[(((size_t)b * nblk + k) * Gy + row) * 32 + col];
-            val.x += p.x;
-            val.y += p.y;
-          }
-          val = make_float2(val.x * inv_area, gx >= 0 ? val.y * inv_area : -val.y * inv_area);
-        }
-        v[n2] = val;
-      }
-    }
-    fft64_group<1>(v, twi, xs, t, act);
-    if (act) {
-#pragma unroll
-      for (int k1 = 0; k1 < 8; ++k1) sB[r * kSB + t + 8 * k1] = v[k1];
-    }
-  }
-  __syncthreads();
-  // g_c columns (real: G^ is Hermitian); thread (g, t) holds column g + 32 q
-  // at rows t + 8 k1, the layout of the points' column transforms below
+  // g_c in the layout of the points' column transforms: thread (g, t) holds
+  // column g + 32 q at rows t + 8 k1
   float gc[2][8];
+  {
+    const float* src = gcin + (size_t)b * kM * kM + tid * 16;
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int ux = g + 32 * q;
-    float2 v[8];
+    for (int q = 0; q < 2; ++q)
 #pragma unroll
-    for (int n2 = 0; n2 < 8; ++n2) {
-      const int n = t + 8 * n2;
-      const int gy = n < 32 ? n : n - 64;
-      v[n2] = (gy >= -2 * Ny && gy <= 2 * Ny) ? sB[(gy + 2 * Ny) * kSB + ux] : make_float2(0.f, 0.f);
-    }
-    fft64_group<1>(v, twi, xs, t, true);
-#pragma unroll
-    for (int k1 = 0; k1 < 8; ++k1) gc[q][k1] = v[k1].x;
+      for (int k1 = 0; k1 < 8; ++k1) gc[q][k1] = src[q * 8 + k1];
   }
-  __syncthreads();  // the column reads of g_c end before the points' rows
-  for (int s = s0; s < s1; ++s) {
-    const int2 p = pq[s];
+  __syncthreads();  // sM is complete before the points' rows
+  // two source points per pass (an odd count ends with a pair whose second
+  // point repeats the first, not stored): interleaved transforms with their
+  // own scratch and half the barriers
+  float2* xs2 = sX + (32 + g) * kXS;
+  float2* sB2 = sB + 31 * kSB;
+  for (int s = s0; s < s1; s += 2) {
+    const bool two = s + 1 < s1;
+    const int2 pa = pq[s], pb = two ? pq[s + 1] : pa;
     {
       const bool act = g < Wy;
-      const int ny = g - Ny;
-      float2 v[8];
+      // P(-n, s) of the two points: entry T - 1 - (index of n)
+      const float2* Pa = ptab + (size_t)s * T + (T - 1 - g * Wx - Nx);
+      const float2* Pb = two ? Pa + T : Pa;
+      float2 va[8], vb[8];
       if (act) {
 #pragma unroll
         for (int n2 = 0; n2 < 8; ++n2) {
           const int n = t + 8 * n2;
           const int nx = n < 32 ? n : n - 64;
-          float2 val = make_float2(0.f, 0.f), pv;
-          if (nx >= -Nx && nx <= Nx && klt::pupil_value(pg, p.x, p.y, -nx, -ny, &pv))
-            val = cmulf(pv, sM[g * Wx + nx + Nx]);
-          v[n2] = val;
+          float2 a = make_float2(0.f, 0.f), c = make_float2(0.f, 0.f);
+          if (nx >= -Nx && nx <= Nx) {
+            const float2 m = sM[g * Wx + nx + Nx];
+            a = cmulf(Pa[-nx], m);
+            c = cmulf(Pb[-nx], m);
+          }
+          va[n2] = a;
+          vb[n2] = c;
         }
       }
-      fft64_group<1>(v, twi, xs, t, act);
+      fft64_group2<1>(va, vb, twi, xs, xs2, t, act);
       if (act) {
 #pragma unroll
-        for (int k1 = 0; k1 < 8; ++k1) sB[g * kSB + t + 8 * k1] = v[k1];
+        for (int k1 = 0; k1 < 8; ++k1) {
+          sB[g * kSB + t + 8 * k1] = va[k1];
+          sB2[g * kSB + t + 8 * k1] = vb[k1];
+        }
       }
     }
     __syncthreads();
-    float part = 0.f;
+    float parta = 0.f, partb = 0.f;
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int ux = g + 32 * q;
-      float2 v[8];
+      float2 va[8], vb[8];
 #pragma unroll
       for (int n2 = 0; n2 < 8; ++n2) {
         const int n = t + 8 * n2;
         const int ny = n < 32 ? n : n - 64;
-        v[n2] = (ny >= -Ny && ny <= Ny) ? sB[(ny + Ny) * kSB + ux] : make_float2(0.f, 0.f);
+        const bool in = ny >= -Ny && ny <= Ny;
+        va[n2] = in ? sB[(ny + Ny) * kSB + ux] : make_float2(0.f, 0.f);
+        vb[n2] = in ? sB2[(ny + Ny) * kSB + ux] : make_float2(0.f, 0.f);
       }
-      fft64_group<1>(v, twi, xs, t, true);
+      fft64_group2<1>(va, vb, twi, xs, xs2, t, true);
 #pragma unroll
-      for (int k1 = 0; k1 < 8; ++k1) part += gc[q][k1] * (v[k1].x * v[k1].x + v[k1].y * v[k1].y);
+      for (int k1 = 0; k1 < 8; ++k1) {
+        parta += gc[q][k1] * (va[k1].x * va[k1].x + va[k1].y * va[k1].y);
+        partb += gc[q][k1] * (vb[k1].x * vb[k1].x + vb[k1].y * vb[k1].y);
+      }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
-    if (lane == 0) red[wave] = part;
-    // (also ends the column reads of sB before the next point's rows)
+    parta = wave_sum_f(parta);
+    partb = wave_sum_f(partb);
+    if (lane == 0) {
+      red[wave][0] = parta;
+      red[wave][1] = partb;
+    }
+    // (also ends the column reads of sB before the next pair's rows)
     __syncthreads();
-    if (tid == 0) {
-      const double tot = (double)red[0] + (double)red[1] + (double)red[2] + (double)red[3];
+    if (tid < 2 && (tid == 0 || two)) {
+      const int2 p = tid == 0 ? pa : pb;
+      const double tot = (double)red[0][tid] + (double)red[1][tid] + (double)red[2][tid] +
+                         (double)red[3][tid];
       grad[(size_t)b * srcSize * srcSize + (size_t)(p.y + pg.sh) * srcSize + p.x + pg.sh] =
           (float)(scale * tot);
     }
@@ -1910,18 +1910,35 @@ int socs_mid_from_band(klt_ctx* ctx, const klt_kernels* kb, const float2* band, 
   return KLT_OK;
 }
 
-int abbe_grad_coarse(klt_ctx* ctx, const float2* mpart, const float2* gpart,
-                     const float* lpart, int B, int Lx, int Ly, const PupilGeom& geom,
-                     const int2* pts, int npts, int srcSize, float* grad, float* loss) {
+int resist_coarse_grid(klt_ctx* ctx, const float2* gpart, const float* lpart, int parts, int B,
+                       int Nx, int Ny, float inv_area, float* loss, float* gc) {
+  if (Nx > 15 || Ny > 15) return set_error(KLT_ERR_ARG, "resist_coarse_grid: half-size N above 15");
+  KLT_TRY(ctx_tw64(ctx));
+  const int Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
+  KLT_TRY(ctx->gfull.ensure((size_t)B * Gx * Gy * sizeof(float2)));
+  KLT_LAUNCH(ctx, "k_resist_sum", k_resist_sum, dim3((unsigned)(Gx * Gy + 255) / 256, B),
+             dim3(256), 0, gpart, lpart, parts, Nx, Ny, inv_area, ctx->gfull.as<float2>(), loss);
+  KLT_LAUNCH(ctx, "k_resist_coarse", k_resist_coarse, dim3(B), dim3(256), 0,
+             ctx->gfull.as<float2>(), Nx, Ny, ctx->tw64.as<float2>(), gc);
+  KLT_HIP(hipGetLastError());
+  return KLT_OK;
+}
+
+int abbe_grad_coarse(klt_ctx* ctx, const float2* Mfull, const float* gc, int B, int Lx, int Ly,
+                     const PupilGeom& geom, const int2* pts, int npts, int srcSize,
+                     float* grad) {
   if (geom.Nx > 15 || geom.Ny > 15)
     return set_error(KLT_ERR_ARG, "abbe_grad_coarse: half-size N above 15");
   KLT_TRY(ctx_tw64(ctx));
-  const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
+  // the pupil at every (point, frequency), once for all masks
+  KLT_TRY(ctx->abbe_pupil.ensure(sizeof(float2) * (size_t)npts * geom.T));
+  KLT_LAUNCH(ctx, "k_abbe_pupil", k_abbe_pupil, dim3((geom.T + 255) / 256, npts), dim3(256), 0,
+             geom, pts, ctx->abbe_pupil.as<float2>());
   // about two blocks per CU over all masks
   const int ngrp = std::max(1, std::min(npts, (512 + B - 1) / B));
-  KLT_LAUNCH(ctx, "k_abbe_grad_coarse", k_abbe_grad_coarse, dim3(ngrp, B), dim3(256), 0, mpart,
-             gpart, lpart, nblk, geom, pts, npts, srcSize, 1.0f / ((float)Lx * (float)Ly),
-             (double)Lx * (double)Ly / (double)(kM * kM), ctx->tw64.as<float2>(), grad, loss);
+  KLT_LAUNCH(ctx, "k_abbe_grad_coarse", k_abbe_grad_coarse, dim3(ngrp, B), dim3(256), 0, Mfull,
+             gc, geom, pts, ctx->abbe_pupil.as<float2>(), npts, srcSize,
+             (double)Lx * (double)Ly / (double)(kM * kM), ctx->tw64.as<float2>(), grad);
   KLT_HIP(hipGetLastError());
   return KLT_OK;
 }
@@ -1935,22 +1952,35 @@ int hopkins_images_fused(klt_ctx* ctx, const float2* band, int B, int Lx, int Ly
   const int Gx = 4 * Nx + 1;
   KLT_TRY(ctx->hspec.ensure((size_t)B * 32 * Gx * sizeof(float2)));
   KLT_TRY(hopkins_spec_device(ctx, band, B, tcc, Nx, Ny, 32, ctx->hspec.as<float2>()));
-  KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma,
-             dim3((Lx + kColsSynth - 1) / kColsSynth, B), dim3(256), 0, ctx->hspec.as<float2>(),
-             Nx, Lx, Ly, ft->Lypad, 1, ft->Ef.as<float4>(), ft->TYh.as<_Float16>(),
-             ft->TYl.as<_Float16>(), images);
+  // row ranges per strip by the fixed rule of the batched passes
+  const int nstrips = (Lx + kColsSynth - 1) / kColsSynth, nrow32 = ft->Lypad / 32;
+  int rs = 1;
+  while (rs < 4 && nstrips * rs * B < 512 && split_ok(nrow32, 2 * rs)) rs *= 2;
+  KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma, dim3(nstrips * rs, B), dim3(256), 0,
+             ctx->hspec.as<float2>(), Nx, Lx, Ly, ft->Lypad, rs, ft->Ef.as<float4>(),
+             ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), images);
   KLT_HIP(hipGetLastError());
   return KLT_OK;
 }
 
+int band_split(int Lx, int Ly, int B) {
+  const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
+  const int nchunk = round_up(Lx, 32 * kBandChunk) / (32 * kBandChunk);
+  int ks = 1;
+  while (ks < 4 && nblk * ks * B < 512 && split_ok(nchunk, 2 * ks)) ks *= 2;
+  return ks;
+}
+
 int band_mask_partials(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const float* masks, int B,
-                       float2* out) {
+                       int ks, float2* out) {
   SocsPlan* p = nullptr;
   KLT_TRY(get_socs_plan(ctx, Lx, Ly, Nx, Ny, &p));
   FusedTables* ft = nullptr;
   KLT_TRY(get_fused_tables(ctx, p, &ft));
   const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
-  launch_band<2, false>(ctx, "k_band_mfma", nblk, 1, B, masks, nullptr, 0.f, 0.f, Lx, Ly,
+  if (ks < 1 || !split_ok(ft->Kpad / (32 * kBandChunk), ks))
+    return set_error(KLT_ERR_ARG, "band_mask_partials: column split does not fit");
+  launch_band<2, false>(ctx, "k_band_mfma", nblk, ks, B, masks, nullptr, 0.f, 0.f, Lx, Ly,
                         ft->Kpad, ft->Wf,
                         p->twy_in.as<float2>(), p->NP, Nx, Ny, 16, out, nullptr);
   KLT_HIP(hipGetLastError());
@@ -1958,14 +1988,16 @@ int band_mask_partials(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const float
 }
 
 int band_resist_partials(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const float* images,
-                         const float* targets, int B, float a_sig, float thr, float2* out,
-                         float* lpart) {
+                         const float* targets, int B, int ks, float a_sig, float thr,
+                         float2* out, float* lpart) {
   SocsPlan* p = nullptr;
   KLT_TRY(get_socs_plan(ctx, Lx, Ly, Nx, Ny, &p));
   FusedTables* ft = nullptr;
   KLT_TRY(get_fused_tables(ctx, p, &ft));
   const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
-  launch_band<4, true>(ctx, "k_band_resist", nblk, 1, B, images, targets, a_sig, thr, Lx, Ly,
+  if (ks < 1 || !split_ok(ft->Kpad / (32 * kBandChunk), ks))
+    return set_error(KLT_ERR_ARG, "band_resist_partials: column split does not fit");
+  launch_band<4, true>(ctx, "k_band_resist", nblk, ks, B, images, targets, a_sig, thr, Lx, Ly,
                        ft->Kpad, ft->Wgf,
                        ft->twyg.as<float2>(), 32, 2 * Nx, 2 * Ny, 32, out, lpart);
   KLT_HIP(hipGetLastError());
