@@ -93,6 +93,61 @@ def test_eig_random_hermitian_with_degeneracies(ctx, n, k, seed):
     _check(a, w, _c(v), wo, vo, k)
 
 
+def _reflected(n, lam, rng, nref=3):
+    """Q diag(lam) Q^H with Q = H_nref ... H_1 a product of random Householder
+    reflectors H = I - 2 u u^H, built in O(n^2); its eigenvectors are Q e_i."""
+    a = np.diag(lam.astype(np.complex128))
+    us = []
+    for _ in range(nref):
+        u = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        u /= np.linalg.norm(u)
+        us.append(u)
+        au, uha = a @ u, u.conj() @ a
+        uau = u.conj() @ au
+        a = a - 2 * np.outer(u, uha) - 2 * np.outer(au, u.conj()) + 4 * uau * np.outer(u, u.conj())
+    a = 0.5 * (a + a.conj().T)
+
+    def vec(i):
+        v = np.zeros(n, np.complex128)
+        v[i] = 1.0
+        for u in us:
+            v = v - 2 * u * (u.conj() @ v)
+        return v
+    return a, vec
+
+
+# Sizes past the other tests' paths: n = 1500 and 2100 take the
+# back-transform's 8-column register path and its LDS path, n = 3500 the
+# two-launch Householder steps (rows too long for the fused step's LDS).  The
+# matrices are built with known eigenpairs (no O(n^3) reference solve).
+@pytest.mark.parametrize("n,k,seed", [(1500, 12, 5), (2100, 8, 6), (3500, 6, 7)])
+def test_eig_large_known_spectrum(ctx, n, k, seed):
+    print(f"case eig known spectrum n={n}", flush=True)
+    rng = np.random.default_rng(seed)
+    lam = rng.random(n) * 10
+    order = np.argsort(lam)[::-1]
+    lam[order[2]] = lam[order[1]]                        # exact pair
+    lam[order[4]] = lam[order[5]] = lam[order[3]]        # exact triple
+    order = np.argsort(lam, kind="stable")[::-1]
+    a, vec = _reflected(n, lam, rng)
+    w, v = ctx.eig_topk(_pack(a), k)
+    v = _c(v)
+    nrm = np.abs(lam).max()
+    assert np.abs(w - lam[order[:k]]).max() < 1e-12 * nrm
+    for q in range(k):
+        assert np.linalg.norm(a @ v[q] - w[q] * v[q]) < 1e-11 * nrm
+    assert np.abs(v @ v.conj().T - np.eye(k)).max() < 1e-11
+    # the top eigenvalue is simple: its vector matches Q e_i up to phase; the
+    # pair and the triple through their projectors
+    for (i, j) in [(0, 1), (1, 3), (3, 6)]:
+        if j > k:
+            continue
+        P1 = v[i:j].T @ v[i:j].conj()
+        ref = np.stack([vec(order[q]) for q in range(i, j)])
+        P2 = ref.T @ ref.conj()
+        assert np.abs(P1 - P2).max() < 1e-9
+
+
 def test_calc_kernels_layout_and_image(ctx):
     print("case calc_kernels", flush=True)
     a, Nx, Ny = _tcc(512, inner=0.6, outer=0.9)
