@@ -147,8 +147,7 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
     for (int p = -sh; p <= sh; ++p)
       if (p * p + q * q <= sh * sh) pts.push_back(make_int2(p, q));
   KLT_TRY(ctx->misc.ensure(pts.size() * sizeof(int2)));
-  KLT_HIP(hipMemcpyAsync(ctx->misc.ptr, pts.data(), pts.size() * sizeof(int2),
-                         hipMemcpyHostToDevice, st));
+  KLT_TRY(ctx->upload(ctx->misc.ptr, pts.data(), pts.size() * sizeof(int2)));
   const size_t ng = (size_t)nmask * ns2;
   KLT_LAUNCH(ctx, "k_fill_zero", k_fill_zero, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0,
              dg, ng);
@@ -164,7 +163,7 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
     KLT_HIP(hipMemcpyAsync(losses, dl, (size_t)nmask * 4, hipMemcpyDeviceToHost, st));
     if (images) KLT_HIP(hipMemcpyAsync(images, I, bytes, hipMemcpyDeviceToHost, st));
   }
-  KLT_HIP(hipStreamSynchronize(st));  // the host point list must outlive the copy
+  if (flags & KLT_HOST_PTRS) KLT_HIP(hipStreamSynchronize(st));  // the host outputs are complete
   return KLT_OK;
 }
 

@@ -117,6 +117,32 @@ int get_socs_plan(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, SocsPlan** out) 
 
 }  // namespace klt
 
+int klt_ctx::upload(void* dev, const void* host, size_t bytes) {
+  constexpr size_t kCap = 4u << 20;
+  const size_t need = (bytes + 255) / 256 * 256;
+  if (need > kCap) {  // larger than the area: a synchronous copy
+    KLT_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, stream));
+    KLT_HIP(hipStreamSynchronize(stream));
+    return KLT_OK;
+  }
+  if (!up_base) {
+    KLT_HIP(hipHostMalloc(reinterpret_cast<void**>(&up_base), kCap, hipHostMallocDefault));
+    up_cap = kCap;
+    up_used = 0;
+  }
+  if (up_used + need > up_cap) {  // reuse from the start once the queued copies are done
+    if (up_last) KLT_HIP(hipEventSynchronize(up_last));
+    up_used = 0;
+  }
+  char* piece = up_base + up_used;
+  up_used += need;
+  std::memcpy(piece, host, bytes);
+  KLT_HIP(hipMemcpyAsync(dev, piece, bytes, hipMemcpyHostToDevice, stream));
+  if (!up_last) KLT_HIP(hipEventCreateWithFlags(&up_last, hipEventDisableTiming));
+  KLT_HIP(hipEventRecord(up_last, stream));
+  return KLT_OK;
+}
+
 hipEvent_t klt_ctx::take_event() {
   if (!event_pool.empty()) {
     hipEvent_t e = event_pool.back();
@@ -180,6 +206,8 @@ void klt_ctx_destroy(klt_ctx* ctx) {
     (void)hipEventDestroy(t.stop);
   }
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+  if (ctx->up_last) (void)hipEventDestroy(ctx->up_last);
+  if (ctx->up_base) (void)hipHostFree(ctx->up_base);
   ctx->socs_plans.clear();
   ctx->fused_tables.clear();
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);

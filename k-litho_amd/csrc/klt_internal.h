@@ -127,6 +127,16 @@ struct klt_ctx {
   std::vector<TimedLaunch> timed;
   std::vector<hipEvent_t> event_pool;
   hipEvent_t take_event();
+  // Pinned host staging for the small host-to-device uploads of a call (source
+  // points, tile and job lists): upload() copies `bytes` from host memory into
+  // the next free piece of the pinned area and queues an asynchronous copy to
+  // `dev` on the context stream, so the call need not wait for the copy before
+  // its host data goes out of scope.  The area is reused from its start once
+  // full, after the last queued copy has completed.
+  char* up_base = nullptr;
+  size_t up_cap = 0, up_used = 0;
+  hipEvent_t up_last = nullptr;
+  int upload(void* dev, const void* host, size_t bytes);
   // scratch reused across calls (grown on demand)
   klt::DevBuf stage_in, stage_out, qpart, band, spec, ccols, misc, eig_ws;
   klt::DevBuf gpart, gspec, lpart, bwd_img;  // adjoint scratch

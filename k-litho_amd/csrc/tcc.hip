@@ -351,8 +351,7 @@ static int build_pupil(klt_ctx* ctx, const float* src_host, int srcSize, float o
   *ns_out = ns;
   if (ns == 0) return KLT_OK;
   KLT_TRY(ctx->tcc_pts.ensure(sizeof(SrcPoint) * ns));
-  KLT_HIP(hipMemcpyAsync(ctx->tcc_pts.ptr, pts.data(), sizeof(SrcPoint) * ns,
-                         hipMemcpyHostToDevice, ctx->stream));
+  KLT_TRY(ctx->upload(ctx->tcc_pts.ptr, pts.data(), sizeof(SrcPoint) * ns));
   KLT_TRY(ctx->misc.ensure(sizeof(float2) * (size_t)ns * g.T));
   KLT_LAUNCH(ctx, "k_pupil", k_pupil, dim3((g.T + 255) / 256, ns), dim3(256), 0, g,
              ctx->tcc_pts.as<SrcPoint>(), ctx->misc.as<float2>());
@@ -378,13 +377,11 @@ int tcc_build_device(klt_ctx* ctx, const float* src_host, int srcSize, float out
     for (int a = 0; a < nt; ++a)
       for (int b = a; b < nt; ++b) tiles.push_back(make_int2(a, b));
     KLT_TRY(ctx->spec.ensure(sizeof(int2) * tiles.size()));
-    KLT_HIP(hipMemcpyAsync(ctx->spec.ptr, tiles.data(), sizeof(int2) * tiles.size(),
-                           hipMemcpyHostToDevice, ctx->stream));
+    KLT_TRY(ctx->upload(ctx->spec.ptr, tiles.data(), sizeof(int2) * tiles.size()));
     KLT_LAUNCH(ctx, "k_gram_f64", k_gram_f64, dim3((unsigned)tiles.size()), dim3(256), 0,
                ctx->misc.as<float2>(), ctx->tcc_pts.as<SrcPoint>(), ns, T,
                ctx->spec.as<int2>(), tcc_dev);
     KLT_HIP(hipGetLastError());
-    KLT_HIP(hipStreamSynchronize(ctx->stream));  // tiles / points stay valid until here
     return KLT_OK;
   }
   if (precision != KLT_TCC_SPLIT_F16)
@@ -407,8 +404,7 @@ int tcc_build_device(klt_ctx* ctx, const float* src_host, int srcSize, float out
   for (int a = 0; a < nt; ++a)
     for (int b = a; b < nt; ++b) tiles.push_back(make_int2(a, b));
   KLT_TRY(ctx->spec.ensure(sizeof(int2) * tiles.size()));
-  KLT_HIP(hipMemcpyAsync(ctx->spec.ptr, tiles.data(), sizeof(int2) * tiles.size(),
-                         hipMemcpyHostToDevice, ctx->stream));
+  KLT_TRY(ctx->upload(ctx->spec.ptr, tiles.data(), sizeof(int2) * tiles.size()));
   // split K so that the grid fills the CUs once (one block per CU)
   const int steps = Kpad / 32, ntl = (int)tiles.size();
   const int nks = std::max(1, std::min(steps, 256 / ntl));
@@ -427,8 +423,7 @@ int tcc_build_device(klt_ctx* ctx, const float* src_host, int srcSize, float out
       jobs[(size_t)i * kXcds + x] = make_int4(tiles[tl].x, tiles[tl].y, ks, tl);
     }
   KLT_TRY(ctx->tcc_jobs.ensure(sizeof(int4) * jobs.size()));
-  KLT_HIP(hipMemcpyAsync(ctx->tcc_jobs.ptr, jobs.data(), sizeof(int4) * jobs.size(),
-                         hipMemcpyHostToDevice, ctx->stream));
+  KLT_TRY(ctx->upload(ctx->tcc_jobs.ptr, jobs.data(), sizeof(int4) * jobs.size()));
   KLT_TRY(ctx->tcc_part.ensure((size_t)nks * ntl * kGT * kGT * sizeof(float2)));
   KLT_LAUNCH(ctx, "k_gram_split16", k_gram_split16, dim3((unsigned)jobs.size()), dim3(256), 0,
              base, base + nel, base + 2 * nel, base + 3 * nel, base + 4 * nel, base + 5 * nel,
@@ -437,7 +432,6 @@ int tcc_build_device(klt_ctx* ctx, const float* src_host, int srcSize, float out
              ctx->tcc_part.as<float2>(), nks, ctx->spec.as<int2>(), T, 1.0 / (double)scale,
              tcc_dev);
   KLT_HIP(hipGetLastError());
-  KLT_HIP(hipStreamSynchronize(ctx->stream));
   return KLT_OK;
 }
 
