@@ -933,7 +933,7 @@ __global__ __launch_bounds__(256) void k_resist_coarse(const float2* __restrict_
 }
 
 // grid (kg, B): block (j, b) takes the j-th of kg groups of the kernels, from
-// the image's M^ (k_band_assemble) and g_c (k_resist_assemble); kg = 1 writes
+// the image's M^ (k_band_assemble) and g_c (k_resist_coarse); kg = 1 writes
 // the gradient spectrum, kg > 1 its partial Gam to
 // gam_part [B][kg][(2Ny+1)(2Nx+1)] for k_socs_mid_bwd_fin.
 __global__ __launch_bounds__(256) void k_socs_mid_bwd(
