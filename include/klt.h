@@ -141,6 +141,15 @@ int klt_tcc_build(klt_ctx* ctx, const float* src, int srcSize, float outerSigma,
  * as in LAPACK). */
 int klt_eig_topk(klt_ctx* ctx, const double* a, int n, int k, double* w, double* vecs,
                  unsigned flags);
+/* A batch of `count` Hermitian matrices of the same size in one call (no
+ * reference counterpart; the reference runs zheevr once per TCC,
+ * klitho_tcc.cpp:301-377): a [count][n][n] complex, w [count][k] and vecs
+ * [count][k][n] complex, each matrix's results equal to klt_eig_topk's.  Every
+ * kernel launch of the solve covers the whole batch, so the per-launch latency
+ * that bounds a single solve is shared by the kernel sets of several defocus
+ * or dose conditions. */
+int klt_eig_topk_batch(klt_ctx* ctx, const double* a, int count, int n, int k, double* w,
+                       double* vecs, unsigned flags);
 /* calcKernels: the nk (clamped to T) kernels of the largest eigenvalues in the
  * krn_i_{r,i}.bin layout, krn[q][j] = z_q[T-1-j] as float (klitho_tcc.cpp:
  * 358-362), and the eigenvalues as float (:344-347).  krn_r, krn_i: nk x T. */

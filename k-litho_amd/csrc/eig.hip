@@ -34,6 +34,18 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// Element strides between the matrices of a batch: the arrays of matrix mi
+// start at base + mi * stride.  mi is the kernel's last grid dimension.
+struct EigStride {
+  size_t a;     // A, doubles (2 n^2)
+  size_t v;     // V, complex (n^2)
+  size_t n;     // tau, diag, e, e2 (doubles) and sub (complex): n
+  size_t p;     // p, complex (2 n: two steps' sets)
+  size_t part;  // block partials, complex (two steps' sets)
+  size_t k;     // eigenvalues: k
+  size_t z;     // z (doubles) and output vectors (complex): k n; lu work: 5 k n
+};
+
 struct cd {
   double x, y;
 };
@@ -94,10 +106,20 @@ __global__ __launch_bounds__(kThreads) void k_hh_symv(double* __restrict__ A, in
                                                       cd* __restrict__ sub,
                                                       double* __restrict__ diag,
                                                       cd* __restrict__ p,
-                                                      cd* __restrict__ part) {
+                                                      cd* __restrict__ part, EigStride st) {
   extern __shared__ cd sv[];  // m entries
   __shared__ double red[8];
   const int m = n - j - 1;
+  {
+    const size_t mi = blockIdx.y;
+    A += mi * st.a;
+    V += mi * st.v;
+    tau += mi * st.n;
+    sub += mi * st.n;
+    diag += mi * st.n;
+    p += mi * st.p;
+    part += mi * st.part;
+  }
   const cd* Aj = reinterpret_cast<const cd*>(A) + (size_t)j * n + j + 1;
   double s2 = 0.0, dummy = 0.0;
   for (int i = threadIdx.x; i < m; i += kThreads) {
@@ -174,10 +196,22 @@ __global__ __launch_bounds__(kThreads) void k_hh_step(double* __restrict__ A, in
                                                       const cd* __restrict__ p_prev,
                                                       const cd* __restrict__ part_prev,
                                                       int nparts_prev, cd* __restrict__ p,
-                                                      cd* __restrict__ part) {
+                                                      cd* __restrict__ part, EigStride st) {
   extern __shared__ cd sh[];
   __shared__ double red[8];
   const int m = n - j - 1;
+  {
+    const size_t mi = blockIdx.y;
+    A += mi * st.a;
+    V += mi * st.v;
+    tau += mi * st.n;
+    sub += mi * st.n;
+    diag += mi * st.n;
+    p_prev += mi * st.p;
+    part_prev += mi * st.part;
+    p += mi * st.p;
+    part += mi * st.part;
+  }
   cd* sv = sh;              // row j (raw), then this step's reflector; m entries
   cd* svp = sh + m;         // v', m + 1 entries (global index j + i)
   cd* spp = svp + (m + 1);  // p'; w'_i = p'_i - K' v'_i is formed where used
@@ -302,10 +336,19 @@ __global__ __launch_bounds__(kThreads) void k_hh_her2(double* __restrict__ A, in
                                                       const cd* __restrict__ V,
                                                       const double* __restrict__ tau,
                                                       const cd* __restrict__ p,
-                                                      const cd* __restrict__ part, int nparts) {
+                                                      const cd* __restrict__ part, int nparts,
+                                                      EigStride st) {
   __shared__ cd sv_i[64], sw_i[64], sv_l[64], sw_l[64];
   __shared__ double red[8];
   const int m = n - j - 1;
+  {
+    const size_t mi = blockIdx.z;
+    A += mi * st.a;
+    V += mi * st.v;
+    tau += mi * st.n;
+    p += mi * st.p;
+    part += mi * st.part;
+  }
   const double t = tau[j];
   if (t == 0.0) return;  // no reflection this step (uniform over the grid)
   double kr = 0.0, ki = 0.0;
@@ -360,8 +403,11 @@ __global__ __launch_bounds__(kThreads) void k_hh_her2(double* __restrict__ A, in
 
 // The last 2 x 2 block after the reduction.
 __global__ void k_hh_finish(const double* __restrict__ A, int n, cd* __restrict__ sub,
-                            double* __restrict__ diag) {
+                            double* __restrict__ diag, EigStride st) {
   if (threadIdx.x != 0) return;
+  A += blockIdx.x * st.a;
+  sub += blockIdx.x * st.n;
+  diag += blockIdx.x * st.n;
   if (n >= 2) {
     const cd a = reinterpret_cast<const cd*>(A)[(size_t)(n - 2) * n + n - 1];
     sub[n - 2] = {a.x, -a.y};  // A[n-1][n-2] = conj(A[n-2][n-1])
@@ -402,13 +448,19 @@ __device__ int sturm_count(const double* d, const double* e2, int n, double x, d
 // lanes count the eigenvalues below 64 equally spaced interior points of the
 // bracket at once (Sturm sequences), and the bracket shrinks to the gap
 // between the two points where the count passes the target, 65x per pass.
+// bounds[mi] = (lower bound, upper bound, pivot threshold) of matrix mi.
 __global__ __launch_bounds__(64) void k_bisect(const double* __restrict__ d,
                                                const double* __restrict__ e2, int n, int k,
-                                               double lo0, double hi0, double pivmin,
-                                               double* __restrict__ w) {
+                                               const double* __restrict__ bounds,
+                                               double* __restrict__ w, EigStride st) {
   extern __shared__ double sde[];  // d, then e2 (n entries each)
   const int idx = blockIdx.x, lane = threadIdx.x;
   if (idx >= k) return;
+  const size_t mi = blockIdx.y;
+  d += mi * st.n;
+  e2 += mi * st.n;
+  w += mi * st.k;
+  const double lo0 = bounds[3 * mi], hi0 = bounds[3 * mi + 1], pivmin = bounds[3 * mi + 2];
   for (int i = lane; i < n; i += 64) {
     sde[i] = d[i];
     sde[n + i] = e2[i];
@@ -588,13 +640,25 @@ __device__ void tri_solve(const double* __restrict__ lu, int n, double* __restri
 // eigenvalue order (Gram-Schmidt applied twice, against kGs earlier vectors
 // at a time, parallel over the vector length) and normalises them.
 // Equal eigenvalues of a cluster are separated slightly in the shifts.
+// Matrix mi has ncl[mi] clusters starting at cl_begin[mi (k + 1) ..] and the
+// tridiagonal norm tnorms[mi]; blocks past its cluster count return.
 __global__ __launch_bounds__(256) void k_inverse_iteration(
     const double* __restrict__ d, const double* __restrict__ e, int n,
-    const double* __restrict__ w, const int* __restrict__ cl_begin, double tnorm,
-    double* __restrict__ z, double* __restrict__ work) {
+    const double* __restrict__ w, const int* __restrict__ cl_all,
+    const int* __restrict__ ncl, const double* __restrict__ tnorms,
+    double* __restrict__ z, double* __restrict__ work, EigStride st) {
   __shared__ double red[4][33];
   __shared__ double dots[32];
   const int c = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const size_t mi = blockIdx.y;
+  if (c >= ncl[mi]) return;
+  d += mi * st.n;
+  e += mi * st.n;
+  w += mi * st.k;
+  z += mi * st.z;
+  work += mi * 5 * st.z;
+  const int* cl_begin = cl_all + mi * (st.k + 1);
+  const double tnorm = tnorms[mi];
   const int first = cl_begin[c], last = cl_begin[c + 1];
   const double eps = 2.220446049250313e-16;
   for (int q = first + tid; q < last; q += 256) {
@@ -695,10 +759,19 @@ __global__ __launch_bounds__(kThreads) void k_back_transform(const double* __res
                                                              int n, const cd* __restrict__ V,
                                                              const double* __restrict__ tau,
                                                              const cd* __restrict__ sub,
-                                                             cd* __restrict__ out) {
+                                                             cd* __restrict__ out,
+                                                             EigStride st) {
   extern __shared__ cd sy[];  // n entries
   __shared__ double red[8];
   const int q = blockIdx.x;
+  {
+    const size_t mi = blockIdx.y;
+    zr += mi * st.z;
+    V += mi * st.v;
+    tau += mi * st.n;
+    sub += mi * st.n;
+    out += mi * st.z;
+  }
   const double* z = zr + (size_t)q * n;
   __shared__ cd scan[kThreads];
   phase_scale(z, n, sub, sy, scan);
@@ -733,11 +806,20 @@ __global__ __launch_bounds__(kThreads) void k_back_transform_reg(const double* _
                                                                  const cd* __restrict__ V,
                                                                  const double* __restrict__ tau,
                                                                  const cd* __restrict__ sub,
-                                                                 cd* __restrict__ out) {
+                                                                 cd* __restrict__ out,
+                                                                 EigStride st) {
   extern __shared__ cd sy[];  // n entries
   __shared__ double red[8];
   __shared__ cd scan[kThreads];
   const int q = blockIdx.x, tid = threadIdx.x;
+  {
+    const size_t mi = blockIdx.y;
+    zr += mi * st.z;
+    V += mi * st.v;
+    tau += mi * st.n;
+    sub += mi * st.n;
+    out += mi * st.z;
+  }
   phase_scale(zr + (size_t)q * n, n, sub, sy, scan);
   const cd zero = {0.0, 0.0};
   cd y[C], vc[C], vn[C];
@@ -812,24 +894,36 @@ namespace klt {
 // Top-k eigenpairs of the Hermitian n x n matrix a_dev (device, complex
 // double, full storage; not modified).  w_dev: k eigenvalues (descending);
 // vec_dev: k x n complex eigenvectors (rows, unit norm).
-int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_dev,
-                    double* vec_dev) {
+int eig_topk_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, int k,
+                          double* w_dev, double* vec_dev) {
   if (n <= 0 || k <= 0 || k > n) return set_error(KLT_ERR_ARG, "eig: bad n or k");
+  if (count <= 0 || count > 65535) return set_error(KLT_ERR_ARG, "eig: bad batch count");
   if (n > 8192) return set_error(KLT_ERR_ARG, "eig: n above 8192 is not supported");
   const size_t nn = (size_t)n * n;
-  // workspace: A copy, V, tau, sub, diag, p, parts, e, e2, z, lu work, clusters
+  // workspace per matrix: A copy, V, tau, sub, diag, p, parts, e, e2, z, lu
+  // work, clusters; each array holds the batch's matrices back to back
   const int nparts_max = 1024;
+  EigStride es;
+  es.a = 2 * nn;
+  es.v = nn;
+  es.n = (size_t)n;
+  es.p = 2 * (size_t)n;
+  es.part = 2 * (size_t)nparts_max;
+  es.k = (size_t)k;
+  es.z = (size_t)k * n;
+  const size_t B = (size_t)count;
   size_t off = 0;
   auto carve = [&](size_t bytes) {
     size_t o = off;
     off += (bytes + 255) / 256 * 256;
     return o;
   };
-  const size_t oA = carve(nn * 16), oV = carve(nn * 16), oTau = carve(n * 8),
-               oSub = carve(n * 16), oDiag = carve(n * 8), oP = carve(2 * (size_t)n * 16),
-               oPart = carve(2 * nparts_max * 16), oE = carve(n * 8), oE2 = carve(n * 8),
-               oZ = carve((size_t)k * n * 8), oWork = carve((size_t)k * 5 * n * 8),
-               oCl = carve((k + 1) * 4);
+  const size_t oA = carve(B * nn * 16), oV = carve(B * nn * 16), oTau = carve(B * n * 8),
+               oSub = carve(B * n * 16), oDiag = carve(B * n * 8), oP = carve(B * es.p * 16),
+               oPart = carve(B * es.part * 16), oE = carve(B * n * 8), oE2 = carve(B * n * 8),
+               oZ = carve(B * es.z * 8), oWork = carve(B * 5 * es.z * 8),
+               oCl = carve(B * (k + 1) * 4), oNcl = carve(B * 4), oBnd = carve(B * 3 * 8),
+               oTn = carve(B * 8);
   KLT_TRY(ctx->eig_ws.ensure(off));
   char* base = ctx->eig_ws.as<char>();
   double* A = reinterpret_cast<double*>(base + oA);
@@ -844,10 +938,13 @@ int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_d
   double* z = reinterpret_cast<double*>(base + oZ);
   double* work = reinterpret_cast<double*>(base + oWork);
   int* cl = reinterpret_cast<int*>(base + oCl);
+  int* ncl_dev = reinterpret_cast<int*>(base + oNcl);
+  double* bnd = reinterpret_cast<double*>(base + oBnd);
+  double* tn_dev = reinterpret_cast<double*>(base + oTn);
   hipStream_t st = ctx->stream;
-  KLT_HIP(hipMemcpyAsync(A, a_dev, nn * 16, hipMemcpyDeviceToDevice, st));
-  KLT_HIP(hipMemsetAsync(tau, 0, n * 8, st));
-  KLT_HIP(hipMemsetAsync(sub, 0, n * 16, st));
+  KLT_HIP(hipMemcpyAsync(A, a_dev, B * nn * 16, hipMemcpyDeviceToDevice, st));
+  KLT_HIP(hipMemsetAsync(tau, 0, B * n * 8, st));
+  KLT_HIP(hipMemsetAsync(sub, 0, B * n * 16, st));
   const size_t lds_v = std::max<size_t>(16, (size_t)n * 16);
   if (lds_v > 64 * 1024) {
     KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hh_symv),
@@ -874,87 +971,110 @@ int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_d
     if (fused) {
       const cd* pp = p + (size_t)((j + 1) & 1) * n;
       const cd* partp = part + (size_t)((j + 1) & 1) * nparts_max;
-      KLT_LAUNCH(ctx, "k_hh_step", k_hh_step, dim3(np), dim3(kThreads),
+      KLT_LAUNCH(ctx, "k_hh_step", k_hh_step, dim3(np, count), dim3(kThreads),
                  (size_t)(3 * m + 2) * 16, A, n, j, rows, V, tau, sub, diag, pp, partp, np_prev,
-                 pj, partj);
+                 pj, partj, es);
       if (j + 3 == n) {  // the last step's update, for the final 2 x 2 block
-        KLT_LAUNCH(ctx, "k_hh_her2", k_hh_her2, dim3(1, 1), dim3(kThreads), 0, A, n, j, V, tau,
-                   pj, partj, np);
+        KLT_LAUNCH(ctx, "k_hh_her2", k_hh_her2, dim3(1, 1, count), dim3(kThreads), 0, A, n, j,
+                   V, tau, pj, partj, np, es);
       }
     } else {
-      KLT_LAUNCH(ctx, "k_hh_symv", k_hh_symv, dim3(np), dim3(kThreads), (size_t)m * 16, A, n,
-                 j, rows, V, tau, sub, diag, pj, partj);
+      KLT_LAUNCH(ctx, "k_hh_symv", k_hh_symv, dim3(np, count), dim3(kThreads), (size_t)m * 16,
+                 A, n, j, rows, V, tau, sub, diag, pj, partj, es);
       const int tiles = (m + 63) / 64;
-      KLT_LAUNCH(ctx, "k_hh_her2", k_hh_her2, dim3(tiles, tiles), dim3(kThreads), 0, A, n, j,
-                 V, tau, pj, partj, np);
+      KLT_LAUNCH(ctx, "k_hh_her2", k_hh_her2, dim3(tiles, tiles, count), dim3(kThreads), 0, A,
+                 n, j, V, tau, pj, partj, np, es);
     }
     np_prev = np;
   }
-  KLT_LAUNCH(ctx, "k_hh_finish", k_hh_finish, dim3(1), dim3(64), 0, A, n, sub, diag);
+  KLT_LAUNCH(ctx, "k_hh_finish", k_hh_finish, dim3(count), dim3(64), 0, A, n, sub, diag, es);
   KLT_HIP(hipGetLastError());
-  // real tridiagonal, Gershgorin bounds, pivot threshold (host: O(n))
-  std::vector<double> hd(n);
-  std::vector<cd> hs(n);
-  KLT_HIP(hipMemcpyAsync(hd.data(), diag, n * 8, hipMemcpyDeviceToHost, st));
-  KLT_HIP(hipMemcpyAsync(hs.data(), sub, n * 16, hipMemcpyDeviceToHost, st));
+  // per matrix: real tridiagonal, Gershgorin bounds, pivot threshold (host: O(n))
+  std::vector<double> hd(B * n);
+  std::vector<cd> hs(B * n);
+  KLT_HIP(hipMemcpyAsync(hd.data(), diag, B * n * 8, hipMemcpyDeviceToHost, st));
+  KLT_HIP(hipMemcpyAsync(hs.data(), sub, B * n * 16, hipMemcpyDeviceToHost, st));
   KLT_HIP(hipStreamSynchronize(st));
-  std::vector<double> he(n, 0.0), he2(n, 0.0);
-  double lo = 1e300, hi = -1e300, tnorm = 0.0;
-  for (int i = 0; i < n; ++i) {
-    if (i + 1 < n) {
-      he[i] = std::hypot(hs[i].x, hs[i].y);
-      he2[i] = he[i] * he[i];
+  std::vector<double> he(B * n, 0.0), he2(B * n, 0.0), hb(B * 3), htn(B);
+  for (size_t mi = 0; mi < B; ++mi) {
+    const double* dd = hd.data() + mi * n;
+    const cd* ss = hs.data() + mi * n;
+    double* ee = he.data() + mi * n;
+    double* ee2 = he2.data() + mi * n;
+    double lo = 1e300, hi = -1e300, tnorm = 0.0;
+    for (int i = 0; i < n; ++i) {
+      if (i + 1 < n) {
+        ee[i] = std::hypot(ss[i].x, ss[i].y);
+        ee2[i] = ee[i] * ee[i];
+      }
+      const double r = (i > 0 ? ee[i - 1] : 0.0) + (i + 1 < n ? ee[i] : 0.0);
+      lo = std::min(lo, dd[i] - r);
+      hi = std::max(hi, dd[i] + r);
+      tnorm = std::max(tnorm, std::fabs(dd[i]) + r);
     }
-    const double r = (i > 0 ? he[i - 1] : 0.0) + (i + 1 < n ? he[i] : 0.0);
-    lo = std::min(lo, hd[i] - r);
-    hi = std::max(hi, hd[i] + r);
-    tnorm = std::max(tnorm, std::fabs(hd[i]) + r);
+    if (!std::isfinite(lo) || !std::isfinite(hi))
+      return set_error(KLT_ERR_EIG, "eig: non-finite tridiagonal (input has NaN/Inf)");
+    const double pivmin = std::max(1e-300, 2.2e-16 * 2.2e-16 * std::max(1.0, tnorm * tnorm));
+    hb[3 * mi] = lo - (2.2e-16 * tnorm + pivmin);
+    hb[3 * mi + 1] = hi + (2.2e-16 * tnorm + pivmin);
+    hb[3 * mi + 2] = pivmin;
+    htn[mi] = tnorm;
   }
-  if (!std::isfinite(lo) || !std::isfinite(hi))
-    return set_error(KLT_ERR_EIG, "eig: non-finite tridiagonal (input has NaN/Inf)");
-  const double pivmin = std::max(1e-300, 2.2e-16 * 2.2e-16 * std::max(1.0, tnorm * tnorm)) ;
-  lo -= 2.2e-16 * tnorm + pivmin;
-  hi += 2.2e-16 * tnorm + pivmin;
-  KLT_HIP(hipMemcpyAsync(e, he.data(), n * 8, hipMemcpyHostToDevice, st));
-  KLT_HIP(hipMemcpyAsync(e2, he2.data(), n * 8, hipMemcpyHostToDevice, st));
+  KLT_HIP(hipMemcpyAsync(e, he.data(), B * n * 8, hipMemcpyHostToDevice, st));
+  KLT_HIP(hipMemcpyAsync(e2, he2.data(), B * n * 8, hipMemcpyHostToDevice, st));
+  KLT_HIP(hipMemcpyAsync(bnd, hb.data(), B * 3 * 8, hipMemcpyHostToDevice, st));
+  KLT_HIP(hipMemcpyAsync(tn_dev, htn.data(), B * 8, hipMemcpyHostToDevice, st));
   const size_t lds_de = (size_t)2 * n * 8;
   if (lds_de > 64 * 1024)
     KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bisect),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_de));
-  KLT_LAUNCH(ctx, "k_bisect", k_bisect, dim3(k), dim3(64), lds_de, diag, e2, n, k, lo, hi,
-             pivmin, w_dev);
-  std::vector<double> hw(k);
-  KLT_HIP(hipMemcpyAsync(hw.data(), w_dev, k * 8, hipMemcpyDeviceToHost, st));
+  KLT_LAUNCH(ctx, "k_bisect", k_bisect, dim3(k, count), dim3(64), lds_de, diag, e2, n, k, bnd,
+             w_dev, es);
+  std::vector<double> hw(B * k);
+  KLT_HIP(hipMemcpyAsync(hw.data(), w_dev, B * k * 8, hipMemcpyDeviceToHost, st));
   KLT_HIP(hipStreamSynchronize(st));
   // clusters of close eigenvalues (descending order).  Inverse iteration
   // leaves an orthogonality error of about eps ||T|| / gap between two
   // eigenvectors; pairs closer than 1e-4 ||T|| are re-orthogonalised, which
   // bounds the error elsewhere by ~2e-12.
-  std::vector<int> hcl;
-  hcl.push_back(0);
-  for (int q = 1; q < k; ++q)
-    if (std::fabs(hw[q - 1] - hw[q]) > 1e-4 * tnorm) hcl.push_back(q);
-  hcl.push_back(k);
-  const int ncl = (int)hcl.size() - 1;
+  std::vector<int> hcl(B * (k + 1), k), hncl(B);
+  int ncl_max = 0;
+  for (size_t mi = 0; mi < B; ++mi) {
+    const double* ww = hw.data() + mi * k;
+    int* c = hcl.data() + mi * (k + 1);
+    int nc = 0;
+    c[nc++] = 0;
+    for (int q = 1; q < k; ++q)
+      if (std::fabs(ww[q - 1] - ww[q]) > 1e-4 * htn[mi]) c[nc++] = q;
+    c[nc] = k;
+    hncl[mi] = nc;
+    ncl_max = std::max(ncl_max, nc);
+  }
   KLT_HIP(hipMemcpyAsync(cl, hcl.data(), hcl.size() * 4, hipMemcpyHostToDevice, st));
-  KLT_LAUNCH(ctx, "k_inverse_iteration", k_inverse_iteration, dim3(ncl), dim3(256), 0, diag,
-             e, n, w_dev, cl, tnorm, z, work);
+  KLT_HIP(hipMemcpyAsync(ncl_dev, hncl.data(), B * 4, hipMemcpyHostToDevice, st));
+  KLT_LAUNCH(ctx, "k_inverse_iteration", k_inverse_iteration, dim3(ncl_max, count), dim3(256),
+             0, diag, e, n, w_dev, cl, ncl_dev, tn_dev, z, work, es);
   if (n <= kThreads * 5) {
-    KLT_LAUNCH(ctx, "k_back_transform", k_back_transform_reg<5>, dim3(k), dim3(kThreads), lds_v,
-               z, n, V, tau, sub, reinterpret_cast<cd*>(vec_dev));
+    KLT_LAUNCH(ctx, "k_back_transform", k_back_transform_reg<5>, dim3(k, count), dim3(kThreads),
+               lds_v, z, n, V, tau, sub, reinterpret_cast<cd*>(vec_dev), es);
   } else if (n <= kThreads * 8) {
     if (lds_v > 64 * 1024)
       KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_back_transform_reg<8>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_v));
-    KLT_LAUNCH(ctx, "k_back_transform", k_back_transform_reg<8>, dim3(k), dim3(kThreads), lds_v,
-               z, n, V, tau, sub, reinterpret_cast<cd*>(vec_dev));
+    KLT_LAUNCH(ctx, "k_back_transform", k_back_transform_reg<8>, dim3(k, count), dim3(kThreads),
+               lds_v, z, n, V, tau, sub, reinterpret_cast<cd*>(vec_dev), es);
   } else {
-    KLT_LAUNCH(ctx, "k_back_transform", k_back_transform, dim3(k), dim3(kThreads), lds_v, z, n,
-               V, tau, sub, reinterpret_cast<cd*>(vec_dev));
+    KLT_LAUNCH(ctx, "k_back_transform", k_back_transform, dim3(k, count), dim3(kThreads), lds_v,
+               z, n, V, tau, sub, reinterpret_cast<cd*>(vec_dev), es);
   }
   KLT_HIP(hipGetLastError());
   KLT_HIP(hipStreamSynchronize(st));  // host vectors above must outlive the copies
   return KLT_OK;
+}
+
+int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_dev,
+                    double* vec_dev) {
+  return eig_topk_batch_device(ctx, a_dev, 1, n, k, w_dev, vec_dev);
 }
 
 }  // namespace klt
@@ -983,6 +1103,36 @@ int klt_eig_topk(klt_ctx* ctx, const double* a, int n, int k, double* w, double*
   if (flags & KLT_HOST_PTRS) {
     KLT_HIP(hipMemcpyAsync(vecs, dv, (size_t)k * n * 16, hipMemcpyDeviceToHost, ctx->stream));
     KLT_HIP(hipMemcpyAsync(w, dw, (size_t)k * 8, hipMemcpyDeviceToHost, ctx->stream));
+    KLT_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return KLT_OK;
+}
+
+int klt_eig_topk_batch(klt_ctx* ctx, const double* a, int count, int n, int k, double* w,
+                       double* vecs, unsigned flags) {
+  if (!ctx || !a || !w || !vecs)
+    return klt::set_error(KLT_ERR_ARG, "klt_eig_topk_batch: NULL argument");
+  if (n <= 0 || k <= 0 || count <= 0)
+    return klt::set_error(KLT_ERR_ARG, "klt_eig_topk_batch: bad size");
+  if (k > n) k = n;
+  KLT_TRY(ctx->activate());
+  const size_t nn = (size_t)n * n, B = (size_t)count;
+  const double* da = a;
+  double* dw = w;
+  double* dv = vecs;
+  if (flags & KLT_HOST_PTRS) {
+    KLT_TRY(ctx->stage_in.ensure(B * nn * 16));
+    KLT_TRY(ctx->stage_out.ensure(B * ((size_t)k * n * 16 + (size_t)k * 8) + 256));
+    KLT_HIP(hipMemcpyAsync(ctx->stage_in.ptr, a, B * nn * 16, hipMemcpyHostToDevice,
+                           ctx->stream));
+    da = ctx->stage_in.as<double>();
+    dv = ctx->stage_out.as<double>();
+    dw = dv + B * k * n * 2;
+  }
+  KLT_TRY(klt::eig_topk_batch_device(ctx, da, count, n, k, dw, dv));
+  if (flags & KLT_HOST_PTRS) {
+    KLT_HIP(hipMemcpyAsync(vecs, dv, B * k * n * 16, hipMemcpyDeviceToHost, ctx->stream));
+    KLT_HIP(hipMemcpyAsync(w, dw, B * k * 8, hipMemcpyDeviceToHost, ctx->stream));
     KLT_HIP(hipStreamSynchronize(ctx->stream));
   }
   return KLT_OK;

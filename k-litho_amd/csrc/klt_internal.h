@@ -251,6 +251,10 @@ int abbe_grad_coarse(klt_ctx* ctx, const float2* Mfull, const float* gc, int B, 
 // Top-k eigenpairs of a Hermitian matrix on the device (eig.hip).
 int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_dev,
                     double* vec_dev);
+// The same for `count` matrices at once (a_dev [count][n][n] complex, w_dev
+// [count][k], vec_dev [count][k][n] complex): every launch covers the batch.
+int eig_topk_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, int k,
+                          double* w_dev, double* vec_dev);
 // Mixed-size, multi-set batch (socs_fused.hip): images[m * nbank + s] is mask
 // m under kernel set s.  Captured into a graph on first use, replayed after.
 int socs_forward_batch(klt_ctx* ctx, int nmask, const float* const* masks, const int* Lx,

@@ -78,6 +78,7 @@ def lib() -> C.CDLL:
         L.klt_read_kernel_info.argtypes = [cs, C.POINTER(i), C.POINTER(i), fp, i]
         L.klt_tcc_build.argtypes = [vp, fp, i, f, f, f, f, f, i, i, i, i, i, fp, u]
         L.klt_eig_topk.argtypes = [vp, fp, i, i, fp, fp, u]
+        L.klt_eig_topk_batch.argtypes = [vp, fp, i, i, i, fp, fp, u]
         L.klt_calc_kernels.argtypes = [vp, fp, i, i, fp, fp, fp, u]
         L.klt_hopkins_image.argtypes = [vp, fp, i, i, fp, i, i, fp, u]
         L.klt_socs_backward.argtypes = [vp, vp, fp, fp, i, i, i, C.c_float, C.c_float, fp, fp,
@@ -323,6 +324,25 @@ class Context:
         pv, hv = _ptr(v, np.float64)
         with _TorchOrder(self._h, a):
             check(lib().klt_eig_topk(self._h, pa, n, k, pw, pv, _flags(ha, hw, hv)))
+        return w, v
+
+    def eig_topk_batch(self, a, k: int):
+        """Top-k eigenpairs of each Hermitian matrix of a [count, n, n, 2] in one
+        call: (w [count, k], vecs [count, k, n, 2]) on the same side as a."""
+        count, n = int(a.shape[0]), int(a.shape[1])
+        k = min(k, n)
+        if _is_torch(a):
+            import torch
+            w = torch.empty((count, k), dtype=torch.float64, device=a.device)
+            v = torch.empty((count, k, n, 2), dtype=torch.float64, device=a.device)
+        else:
+            w = np.empty((count, k), np.float64)
+            v = np.empty((count, k, n, 2), np.float64)
+        pa, ha = _ptr(a, np.float64)
+        pw, hw = _ptr(w, np.float64)
+        pv, hv = _ptr(v, np.float64)
+        with _TorchOrder(self._h, a):
+            check(lib().klt_eig_topk_batch(self._h, pa, count, n, k, pw, pv, _flags(ha, hw, hv)))
         return w, v
 
     def calc_kernels(self, tcc, nk: int):

@@ -148,6 +148,33 @@ def test_eig_large_known_spectrum(ctx, n, k, seed):
         assert np.abs(P1 - P2).max() < 1e-9
 
 
+def test_eig_batch_equals_single_calls(ctx):
+    """klt_eig_topk_batch: each matrix's eigenpairs bit-identical to its own
+    klt_eig_topk call (the same kernels over a grid with one more dimension),
+    for three TCCs of T = 169 (annular source at three defocus values) and a
+    random Hermitian matrix with an exact pair and triple."""
+    print("case eig batch of 4", flush=True)
+    mats = [_tcc(512, inner=0.6, outer=0.9, df=df)[0] for df in (0.0, 0.15, -0.3)]
+    n = mats[0].shape[0]
+    rng = np.random.default_rng(11)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    lam = np.sort(rng.random(n))[::-1] * 10
+    lam[1] = lam[2] = lam[1]
+    lam[3] = lam[4] = lam[5] = lam[3]
+    r = (Q * lam) @ Q.conj().T
+    mats.append(0.5 * (r + r.conj().T))
+    k = 24
+    packed = np.stack([_pack(a) for a in mats])
+    wb, vb = ctx.eig_topk_batch(packed, k)
+    assert wb.shape == (4, k) and vb.shape == (4, k, n, 2)
+    for i, a in enumerate(mats):
+        w, v = ctx.eig_topk(_pack(a), k)
+        assert np.array_equal(wb[i], w), i
+        assert np.array_equal(vb[i], v), i
+    wo, vo = O.eigh_topk(mats[1], k)
+    _check(mats[1], wb[1], _c(vb[1]), wo, vo, k)
+
+
 def test_calc_kernels_layout_and_image(ctx):
     print("case calc_kernels", flush=True)
     a, Nx, Ny = _tcc(512, inner=0.6, outer=0.9)
