@@ -7,9 +7,12 @@
 //     operations in the reference's order (this file is compiled with
 //     -ffp-contract=off);
 //   a complex Gram, either
-//     k_gram_split16 -- split-precision FP16 MFMA: X = 2^e S P and Y = P are
-//       split into hi + lo halves, X^H Y ~ Xh^H Yh + Xh^H Yl + Xl^H Yh with FP32
-//       accumulation (v_mfma_f32_16x16x32_f16), upper-triangle 64 x 64 tiles;
+//     k_gram_split16 -- split-precision FP16 MFMA on one operand
+//       X = sqrt(2^e S) P (real and imaginary parts interleaved along K, split
+//       into hi + lo halves): Re TCC = X X^T and Im TCC = X X'^T, X' the
+//       lane-local pair swap (x_re, x_im) -> (-x_im, x_re) of X, each product
+//       ~ hh + hl + lh with FP32 accumulation (v_mfma_f32_16x16x32_f16),
+//       upper-triangle 128 x 128 tiles;
 //     k_gram_f64 -- the FP64 reference Gram, accumulated in double like the
 //       reference.
 // The TCC is written as T x T complex double, row-major, both triangles.
@@ -46,82 +49,74 @@ __global__ __launch_bounds__(256) void k_pupil(PupilGeom g, const SrcPoint* __re
   P[(size_t)s * g.T + i] = v;
 }
 
-// Split operands for the MFMA Gram, transposed to [Tpad][Kpad] (k contiguous):
-//   U  = [2^e S Pr ; 2^e S Pi],  V1 = [Pr ; Pi],  V2 = [-Pi ; Pr]
-// (k < ns: real parts, ns <= k < 2ns: imaginary parts), each as hi + lo fp16.
+// The split operand of the MFMA Gram, [Tpad][Kpad] (k contiguous):
+//   X[i][2 s] = sqrt(scale S_s) Re P(i, s),  X[i][2 s + 1] = sqrt(scale S_s) Im P(i, s)
+// as hi + lo fp16 (zero past T and past 2 ns).  S >= 0, so
+//   sum_k X[i][k] X[j][k] = scale Re sum_s S_s P(i,s) conj(P(j,s)).
 __global__ __launch_bounds__(256) void k_split_operands(
     const float2* __restrict__ P, const SrcPoint* __restrict__ pts, int ns, int T, int Kpad,
-    float scale, _Float16* __restrict__ Uh, _Float16* __restrict__ Ul,
-    _Float16* __restrict__ V1h, _Float16* __restrict__ V1l, _Float16* __restrict__ V2h,
-    _Float16* __restrict__ V2l) {
+    float scale, _Float16* __restrict__ Xh, _Float16* __restrict__ Xl) {
   // grid (Tpad / 64, Kpad / 64): a 64 x 64 tile of (row i, column k), read
   // along i from P[s][i] and written along k through LDS
-  __shared__ float2 sp[64][65];
+  __shared__ float2 sp[32][65];
   const int i0 = blockIdx.x * 64, kb = blockIdx.y * 64, tid = threadIdx.x;
-  for (int e = tid; e < 64 * 64; e += 256) {
-    const int kk = e >> 6, ii = e & 63;
-    const int i = i0 + ii, k = kb + kk;
+  for (int e = tid; e < 32 * 64; e += 256) {
+    const int ss = e >> 6, ii = e & 63;
+    const int i = i0 + ii, s = kb / 2 + ss;
     float2 p = make_float2(0.f, 0.f);
-    if (i < T && k < 2 * ns) p = P[(size_t)(k < ns ? k : k - ns) * T + i];
-    sp[kk][ii] = p;
+    if (i < T && s < ns) p = P[(size_t)s * T + i];
+    sp[ss][ii] = p;
   }
   __syncthreads();
   for (int e = tid; e < 64 * 64; e += 256) {
     const int ii = e >> 6, kk = e & 63;
-    const int i = i0 + ii, k = kb + kk;
-    float u = 0.f, v1 = 0.f, v2 = 0.f;
-    if (i < T && k < 2 * ns) {
-      const float2 p = sp[kk][ii];
-      const float su = scale * pts[k < ns ? k : k - ns].S;
-      if (k < ns) {
-        u = su * p.x;
-        v1 = p.x;
-        v2 = -p.y;
-      } else {
-        u = su * p.y;
-        v1 = p.y;
-        v2 = p.x;
-      }
+    const int i = i0 + ii, k = kb + kk, s = k >> 1;
+    float x = 0.f;
+    if (i < T && s < ns) {
+      const float2 p = sp[kk >> 1][ii];
+      x = sqrtf(scale * pts[s].S) * ((k & 1) ? p.y : p.x);
     }
     const size_t o = (size_t)i * Kpad + k;
-    const _Float16 uh = (_Float16)u, v1h = (_Float16)v1, v2h = (_Float16)v2;
-    Uh[o] = uh;
-    Ul[o] = (_Float16)(u - (float)uh);
-    V1h[o] = v1h;
-    V1l[o] = (_Float16)(v1 - (float)v1h);
-    V2h[o] = v2h;
-    V2l[o] = (_Float16)(v2 - (float)v2h);
+    const _Float16 xh = (_Float16)x;
+    Xh[o] = xh;
+    Xl[o] = (_Float16)(x - (float)xh);
   }
 }
 
-// Split-FP16 MFMA Gram, split over the source points.  Block (tile, ks) takes
-// one upper-triangle pair of 64 x 64 tiles and the ks-th chunk of kc columns
-// of K; each of the 4 waves owns a 32 x 32 quadrant (2 x 2 MFMA 16 x 16
-// blocks, real and imaginary parts) and writes its FP32 partial sums to
-// part[ks][tile][64][64] (re, im).  v_mfma_f32_16x16x32_f16 operand layout:
-// lane l holds A[m = l%16][k = 8(l/16) .. +7] and B[k = 8(l/16) .. +7][n = l%16];
-// the result D[m = 4(l/16) + r][n = l%16], r = 0..3.  Operands load two
-// steps ahead into two register sets (the loop runs two steps per pass; Kpad
-// and kc are multiples of 64, and the loads past the chunk's end re-read its
-// last step).
-// Split-precision Gram, one 128 x 128 output tile over one K range per block:
-// jobs[blockIdx.x] = (row panel, column panel, K range ks * kc .. + kc, tile
-// list index), or x < 0 for an idle block.  The host lays the jobs out so
-// that the blocks of one XCD (block ids equal mod 8) take consecutive jobs of
-// one K range: they share operand panels, which then come from that XCD's L2
-// instead of each block reading its own from the memory-side cache.
-// Waves 2 x 2, each a 64 x 64
-// sub-tile of 4 x 4 MFMA blocks (TCC_R and TCC_I).  Per 32-wide K step the
-// block stages the step's U, V1, V2 hi / lo rows (6 x 128 rows x 32 halves)
-// in LDS, and issues the next step's global loads before its products.
+// (x_0, x_1, ..., x_7) -> (-x_1, x_0, -x_3, x_2, ...): the operand of the
+// imaginary part, exact on both split halves.
+__device__ __forceinline__ half8_t swap_neg(half8_t b) {
+  half8_t r;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    r[2 * t] = -b[2 * t + 1];
+    r[2 * t + 1] = b[2 * t];
+  }
+  return r;
+}
+
+// Split-FP16 MFMA Gram, split over the source points: one 128 x 128 output
+// tile over one K range per block.  jobs[blockIdx.x] = (row panel, column
+// panel, K range ks * kc .. + kc, tile list index), or x < 0 for an idle
+// block.  The host lays the jobs out so that the blocks of one XCD (block ids
+// equal mod 8) take consecutive jobs of one K range: they share operand
+// panels, which then come from that XCD's L2 instead of each block reading its
+// own from the memory-side cache.  Waves 2 x 2, each a 64 x 64 sub-tile of
+// 4 x 4 MFMA blocks with real (X_r . X_c) and imaginary (X_r . X'_c)
+// accumulators; partial sums go to part[ks][tile][128][128] (re, im).
+// v_mfma_f32_16x16x32_f16 operand layout: lane l holds A[m = l%16][k = 8(l/16)
+// .. +7] and B[k = 8(l/16) .. +7][n = l%16], so the pair swap of X' stays in
+// the lane; the result D[m = 4(l/16) + r][n = l%16], r = 0..3.  Per 32-wide K
+// step the block stages the step's X rows of both panels (hi, lo: 4 x 128 rows
+// x 32 halves) in LDS and issues the next step's global loads before its
+// products.
 constexpr int kGT = 128;     // output tile
 constexpr int kGLd = 40;     // LDS row stride (halves): 80 bytes
 __global__ __launch_bounds__(256, 1) void k_gram_split16(
-    const _Float16* __restrict__ Uh, const _Float16* __restrict__ Ul,
-    const _Float16* __restrict__ V1h, const _Float16* __restrict__ V1l,
-    const _Float16* __restrict__ V2h, const _Float16* __restrict__ V2l, int Kpad, int kc,
+    const _Float16* __restrict__ Xh, const _Float16* __restrict__ Xl, int Kpad, int kc,
     const int4* __restrict__ jobs, int ntiles, float2* __restrict__ part) {
-  __shared__ __attribute__((aligned(16))) _Float16 sop[6][kGT * kGLd];
+  // X rows of the tile's row panel (hi, lo) and of its column panel (hi, lo)
+  __shared__ __attribute__((aligned(16))) _Float16 sop[4][kGT * kGLd];
   const int4 job = jobs[blockIdx.x];
   if (job.x < 0) return;
   const int2 t = make_int2(job.x, job.y);
@@ -136,21 +131,17 @@ __global__ __launch_bounds__(256, 1) void k_gram_split16(
   const int row0 = tid >> 2, col0 = (tid & 3) * 8;  // chunk tid + 256: row + 64
   const size_t ou0 = (size_t)(t.x * kGT + row0) * Kpad + col0, ou1 = ou0 + (size_t)64 * Kpad;
   const size_t ov0 = (size_t)(t.y * kGT + row0) * Kpad + col0, ov1 = ov0 + (size_t)64 * Kpad;
-  uint4 g0, g1, g2, g3, g4, g5, g6, g7, g8, g9, g10, g11;
+  uint4 g0, g1, g2, g3, g4, g5, g6, g7;
 #define KLT_GRAM_FETCH(k)                                                   \
   do {                                                                      \
-    g0 = *reinterpret_cast<const uint4*>(Uh + ou0 + (k));                   \
-    g1 = *reinterpret_cast<const uint4*>(Uh + ou1 + (k));                   \
-    g2 = *reinterpret_cast<const uint4*>(Ul + ou0 + (k));                   \
-    g3 = *reinterpret_cast<const uint4*>(Ul + ou1 + (k));                   \
-    g4 = *reinterpret_cast<const uint4*>(V1h + ov0 + (k));                  \
-    g5 = *reinterpret_cast<const uint4*>(V1h + ov1 + (k));                  \
-    g6 = *reinterpret_cast<const uint4*>(V1l + ov0 + (k));                  \
-    g7 = *reinterpret_cast<const uint4*>(V1l + ov1 + (k));                  \
-    g8 = *reinterpret_cast<const uint4*>(V2h + ov0 + (k));                  \
-    g9 = *reinterpret_cast<const uint4*>(V2h + ov1 + (k));                  \
-    g10 = *reinterpret_cast<const uint4*>(V2l + ov0 + (k));                 \
-    g11 = *reinterpret_cast<const uint4*>(V2l + ov1 + (k));                 \
+    g0 = *reinterpret_cast<const uint4*>(Xh + ou0 + (k));                   \
+    g1 = *reinterpret_cast<const uint4*>(Xh + ou1 + (k));                   \
+    g2 = *reinterpret_cast<const uint4*>(Xl + ou0 + (k));                   \
+    g3 = *reinterpret_cast<const uint4*>(Xl + ou1 + (k));                   \
+    g4 = *reinterpret_cast<const uint4*>(Xh + ov0 + (k));                   \
+    g5 = *reinterpret_cast<const uint4*>(Xh + ov1 + (k));                   \
+    g6 = *reinterpret_cast<const uint4*>(Xl + ov0 + (k));                   \
+    g7 = *reinterpret_cast<const uint4*>(Xl + ov1 + (k));                   \
   } while (0)
   float4_t accR[4][4], accI[4][4];
 #pragma unroll
@@ -172,10 +163,6 @@ __global__ __launch_bounds__(256, 1) void k_gram_split16(
     *reinterpret_cast<uint4*>(&sop[2][l1]) = g5;
     *reinterpret_cast<uint4*>(&sop[3][l0]) = g6;
     *reinterpret_cast<uint4*>(&sop[3][l1]) = g7;
-    *reinterpret_cast<uint4*>(&sop[4][l0]) = g8;
-    *reinterpret_cast<uint4*>(&sop[4][l1]) = g9;
-    *reinterpret_cast<uint4*>(&sop[5][l0]) = g10;
-    *reinterpret_cast<uint4*>(&sop[5][l1]) = g11;
     __syncthreads();
     if (k + 32 < k1) KLT_GRAM_FETCH(k + 32);
     // A fragments of the wave's 4 row blocks (hi, lo), kept over the columns
@@ -191,8 +178,7 @@ __global__ __launch_bounds__(256, 1) void k_gram_split16(
       const int col = wc * 64 + b * 16 + r16;
       const half8_t b1h = *reinterpret_cast<const half8_t*>(&sop[2][col * kGLd + kq]);
       const half8_t b1l = *reinterpret_cast<const half8_t*>(&sop[3][col * kGLd + kq]);
-      const half8_t b2h = *reinterpret_cast<const half8_t*>(&sop[4][col * kGLd + kq]);
-      const half8_t b2l = *reinterpret_cast<const half8_t*>(&sop[5][col * kGLd + kq]);
+      const half8_t b2h = swap_neg(b1h), b2l = swap_neg(b1l);
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
         accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], b1h, accR[a][b], 0, 0, 0);
@@ -394,11 +380,11 @@ int tcc_build_device(klt_ctx* ctx, const float* src_host, int srcSize, float out
   const float scale = std::ldexp(1.0f, -e);
   const int Kpad = round_up(2 * ns, 64), Tpad = round_up(T, kGT);
   const size_t nel = (size_t)Tpad * Kpad;
-  KLT_TRY(ctx->qpart.ensure(6 * nel * sizeof(_Float16)));
+  KLT_TRY(ctx->qpart.ensure(2 * nel * sizeof(_Float16)));
   _Float16* base = ctx->qpart.as<_Float16>();
   KLT_LAUNCH(ctx, "k_split_operands", k_split_operands, dim3(Tpad / 64, Kpad / 64), dim3(256), 0,
              ctx->misc.as<float2>(), ctx->tcc_pts.as<SrcPoint>(), ns, T, Kpad, scale, base,
-             base + nel, base + 2 * nel, base + 3 * nel, base + 4 * nel, base + 5 * nel);
+             base + nel);
   const int nt = Tpad / kGT;
   std::vector<int2> tiles;
   for (int a = 0; a < nt; ++a)
@@ -426,8 +412,8 @@ int tcc_build_device(klt_ctx* ctx, const float* src_host, int srcSize, float out
   KLT_TRY(ctx->upload(ctx->tcc_jobs.ptr, jobs.data(), sizeof(int4) * jobs.size()));
   KLT_TRY(ctx->tcc_part.ensure((size_t)nks * ntl * kGT * kGT * sizeof(float2)));
   KLT_LAUNCH(ctx, "k_gram_split16", k_gram_split16, dim3((unsigned)jobs.size()), dim3(256), 0,
-             base, base + nel, base + 2 * nel, base + 3 * nel, base + 4 * nel, base + 5 * nel,
-             Kpad, kc, ctx->tcc_jobs.as<int4>(), ntl, ctx->tcc_part.as<float2>());
+             base, base + nel, Kpad, kc, ctx->tcc_jobs.as<int4>(), ntl,
+             ctx->tcc_part.as<float2>());
   KLT_LAUNCH(ctx, "k_gram_reduce", k_gram_reduce, dim3((unsigned)ntl, kGT / 16), dim3(256), 0,
              ctx->tcc_part.as<float2>(), nks, ctx->spec.as<int2>(), T, 1.0 / (double)scale,
              tcc_dev);
