@@ -33,6 +33,8 @@
 namespace {
 
 constexpr int kThreads = 256;
+// most blocks of one Householder step (one partial sum of v^H p each)
+constexpr int kMaxParts = 1024;
 
 // Element strides between the matrices of a batch: the arrays of matrix mi
 // start at base + mi * stride.  mi is the kernel's last grid dimension.
@@ -232,22 +234,50 @@ __global__ __launch_bounds__(kThreads) void k_hh_step(double* __restrict__ A, in
       pre[c] = l < m ? Af[l] : cd{0.0, 0.0};
     }
   }
-  // every load of the prologue at once: step j-1's partial sums, v', p' and
-  // row j
+  // every load of the prologue at once, one memory round trip for m < 2048:
+  // step j-1's partial sums (at most kMaxParts, kPartsPer a thread), then
+  // v', p' and row j, kU entries a thread per pass, all issued before any is
+  // used (a loop that stores each entry to LDS as it arrives waits for one
+  // round trip per 256 entries)
+  constexpr int kPartsPer = kMaxParts / kThreads, kU = 8;
   const cd* Aj = reinterpret_cast<const cd*>(A) + (size_t)j * n + j + 1;
-  double kr = 0.0, ki = 0.0;
-  if (upd) {
-    for (int q = threadIdx.x; q < nparts_prev; q += kThreads) {
-      kr += part_prev[q].x;
-      ki += part_prev[q].y;
+  const cd* vp = V + (size_t)(j > 0 ? j - 1 : 0) * n + j;
+  // (the partial sums are loaded unconditionally, clamped to entry 0 of the
+  // allocated set, and selected where they are added: a conditional load
+  // would let the compiler start the sum, and wait for every load in
+  // flight, right after it)
+  cd pt[kPartsPer];
+#pragma unroll
+  for (int u = 0; u < kPartsPer; ++u) {
+    const int q = threadIdx.x + u * kThreads;
+    pt[u] = part_prev[q < nparts_prev ? q : 0];
+  }
+  for (int i0 = threadIdx.x; i0 <= m; i0 += kU * kThreads) {
+    cd ra[kU], rv[kU], rp[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int i = i0 + u * kThreads;
+      ra[u] = i < m ? Aj[i] : cd{0.0, 0.0};
+      rv[u] = upd && i <= m ? vp[i] : cd{0.0, 0.0};
+      rp[u] = upd && i <= m ? p_prev[i] : cd{0.0, 0.0};
     }
-    const cd* vp = V + (size_t)(j - 1) * n + j;
-    for (int i = threadIdx.x; i <= m; i += kThreads) {
-      svp[i] = vp[i];
-      spp[i] = p_prev[i];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int i = i0 + u * kThreads;
+      if (i < m) sv[i] = ra[u];
+      if (upd && i <= m) {
+        svp[i] = rv[u];
+        spp[i] = rp[u];
+      }
     }
   }
-  for (int i = threadIdx.x; i < m; i += kThreads) sv[i] = Aj[i];
+  double kr = 0.0, ki = 0.0;
+#pragma unroll
+  for (int u = 0; u < kPartsPer; ++u) {
+    const bool use = upd && threadIdx.x + u * kThreads < nparts_prev;
+    kr += use ? pt[u].x : 0.0;
+    ki += use ? pt[u].y : 0.0;
+  }
   block_sum2(kr, ki, red);  // (its barriers also publish the LDS writes)
   const cd K = {0.5 * tp * kr, 0.5 * tp * ki};
   auto wprev = [&](int i) {
@@ -902,7 +932,7 @@ int eig_topk_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, i
   const size_t nn = (size_t)n * n;
   // workspace per matrix: A copy, V, tau, sub, diag, p, parts, e, e2, z, lu
   // work, clusters; each array holds the batch's matrices back to back
-  const int nparts_max = 1024;
+  const int nparts_max = kMaxParts;
   EigStride es;
   es.a = 2 * nn;
   es.v = nn;
