@@ -46,6 +46,7 @@ struct EigStride {
   size_t part;  // block partials, complex (two steps' sets)
   size_t k;     // eigenvalues: k
   size_t z;     // z (doubles) and output vectors (complex): k n; lu work: 5 k n
+  size_t g;     // Gram entries of the back-transform's reflector groups, complex: 3 n
 };
 
 struct cd {
@@ -96,6 +97,23 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
   __syncthreads();
   a = red[0] + red[2] + red[4] + red[6];
   b = red[1] + red[3] + red[5] + red[7];
+}
+
+// Block sums of N values over 256 threads (red: 4 N doubles); valid in every
+// thread.
+template <int N>
+__device__ __forceinline__ void block_sum_n(double (&v)[N], double* red) {
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] = wave_sum(v[q]);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) red[w * N + q] = v[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] = red[q] + red[N + q] + red[2 * N + q] + red[3 * N + q];
 }
 
 // Step j of the reduction.  A: n x n complex (row-major, full Hermitian
@@ -827,19 +845,68 @@ __global__ __launch_bounds__(kThreads) void k_back_transform(const double* __res
   for (int i = threadIdx.x; i < n; i += kThreads) out[(size_t)q * n + i] = sy[i];
 }
 
-// The same for n <= 256 C: thread t keeps y[t + 256 c] (c < C) in registers
-// for the whole sweep (so a step needs no barrier beyond its reduction) and
-// loads reflector j - 1 while it applies reflector j.
-template <int C>
+// The back-transform applies the reflectors in groups of R: group r holds
+// u_a = row j0 - a of V (entries i > j0 - a), a < R, j0 = n - 3 - R r, in
+// that order.  With G_ab = u_a^H u_b (b < a) the group's product acts as
+//   s_a = tau_a (u_a^H y - sum_{b<a} G_ab s_b),   y <- y - sum_a u_a s_a,
+// which equals applying the R reflectors one by one but needs one block
+// reduction (of the R dot products u_a^H y) instead of R.  k_reflector_gram
+// computes G once for all vectors: G[r][a (a - 1) / 2 + b].  Rows past
+// j0 - a < 0 are zero (tau 0).
+template <int R>
+__global__ __launch_bounds__(kThreads) void k_reflector_gram(int n, const cd* __restrict__ V,
+                                                             cd* __restrict__ G, EigStride st) {
+  constexpr int P = R * (R - 1) / 2;
+  __shared__ double red[4 * 2 * P];
+  const int r = blockIdx.x;
+  {
+    const size_t mi = blockIdx.y;
+    V += mi * st.v;
+    G += mi * st.g;
+  }
+  const int j0 = n - 3 - R * r;
+  double acc[2 * P];
+#pragma unroll
+  for (int q = 0; q < 2 * P; ++q) acc[q] = 0.0;
+  for (int i = threadIdx.x; i < n; i += kThreads) {
+    cd u[R];
+#pragma unroll
+    for (int a = 0; a < R; ++a) {
+      const int j = j0 - a;
+      u[a] = (j >= 0 && i > j) ? V[(size_t)j * n + i] : cd{0.0, 0.0};
+    }
+#pragma unroll
+    for (int a = 1; a < R; ++a)
+#pragma unroll
+      for (int b = 0; b < a; ++b) {
+        const cd c = cmulc(u[a], u[b]);
+        acc[2 * (a * (a - 1) / 2 + b)] += c.x;
+        acc[2 * (a * (a - 1) / 2 + b) + 1] += c.y;
+      }
+  }
+  block_sum_n<2 * P>(acc, red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < P; ++q) G[(size_t)r * P + q] = {acc[2 * q], acc[2 * q + 1]};
+  }
+}
+
+// x = H_0 ... H_{n-3} D z for n <= 256 C: thread t keeps y[t + 256 c]
+// (c < C) in registers for the whole sweep, applies the reflectors in
+// groups of R (one block reduction a group) and loads group r + 1 while it
+// applies group r.
+template <int C, int R>
 __global__ __launch_bounds__(kThreads) void k_back_transform_reg(const double* __restrict__ zr,
                                                                  int n,
                                                                  const cd* __restrict__ V,
                                                                  const double* __restrict__ tau,
                                                                  const cd* __restrict__ sub,
+                                                                 const cd* __restrict__ G,
                                                                  cd* __restrict__ out,
                                                                  EigStride st) {
+  constexpr int P = R * (R - 1) / 2;
   extern __shared__ cd sy[];  // n entries
-  __shared__ double red[8];
+  __shared__ double red[4 * 2 * R];
   __shared__ cd scan[kThreads];
   const int q = blockIdx.x, tid = threadIdx.x;
   {
@@ -848,53 +915,102 @@ __global__ __launch_bounds__(kThreads) void k_back_transform_reg(const double* _
     V += mi * st.v;
     tau += mi * st.n;
     sub += mi * st.n;
+    G += mi * st.g;
     out += mi * st.z;
   }
   phase_scale(zr + (size_t)q * n, n, sub, sy, scan);
-  const cd zero = {0.0, 0.0};
-  cd y[C], vc[C], vn[C];
+  cd y[C];
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     const int i = tid + kThreads * c;
-    y[c] = i < n ? sy[i] : zero;
+    y[c] = i < n ? sy[i] : cd{0.0, 0.0};  // (a prvalue: an lvalue zero
+    // would make the compiler select between the two addresses and load
+    // through a flat pointer)
   }
-  // row j of V holds the reflector's entries i >= j + 1 (the rest is unset)
-  auto load_row = [&](int j, cd(&dst)[C]) {
+  const int ngroups = (n - 2 + R - 1) / R;  // reflectors n - 3 .. 0
+  // tau and G of a group are loaded one value a lane (lane a: tau_{j0-a}
+  // and G entry a) and read back with readlane where they are used: as
+  // uniform loads they would go through the scalar cache, whose counter the
+  // block reduction's LDS wait also drains, so the next group's values would
+  // be waited for inside this group
+  const int lq = tid & 63;
+  // Every load is unconditional, from an address clamped into the arrays,
+  // and the unused values are selected away: a load that a branch may skip
+  // leaves the compiler unable to count the loads in flight, and it then
+  // waits for all of them (the prefetched group's too) at the loop head.
+  auto load_group = [&](int r, cd(&u)[R][C], double& t, cd& g) {
+    const int j0 = n - 3 - R * r;
+    const int jt = j0 - lq;
+    const double tl = tau[min(max(jt, 0), n - 1)];
+    t = (lq < R && jt >= 0) ? tl : 0.0;
+    const cd gl = G[(size_t)min(r, ngroups - 1) * (P > 0 ? P : 1) + min(lq, P > 0 ? P - 1 : 0)];
+    g = lq < P ? gl : cd{0.0, 0.0};
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
-      const int i = tid + kThreads * c;
-      dst[c] = (i > j && i < n) ? V[(size_t)j * n + i] : zero;
+    for (int a = 0; a < R; ++a) {
+      const int j = j0 - a;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int i = tid + kThreads * c;
+        const cd v = V[(size_t)max(j, 0) * n + min(i, n - 1)];
+        u[a][c] = (j >= 0 && i > j && i < n) ? v : cd{0.0, 0.0};
+      }
     }
   };
-  int j = n - 3;
-  if (j >= 0) load_row(j, vc);
-  double t = j >= 0 ? tau[j] : 0.0;
-  for (; j >= 0; --j) {
-    double tn = 0.0;
-    if (j > 0) {
-      load_row(j - 1, vn);
-      tn = tau[j - 1];
-    }
-    if (t != 0.0) {  // (uniform)
+  auto lane_value = [](double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+  };
+  // group r from (u, t, g) while group r + 1 loads into (un, tn, gn); the
+  // loop alternates two buffer sets, so that no register copy waits for the
+  // loads in flight
+  auto apply_group = [&](int r, const cd(&u)[R][C], double t, cd g, cd(&un)[R][C], double& tn,
+                         cd& gn) {
+    load_group(r + 1, un, tn, gn);  // (past the last group: zeros)
+    double d[2 * R];
+#pragma unroll
+    for (int a = 0; a < R; ++a) {
       double sr = 0.0, si = 0.0;
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        const cd m = cmulc(vc[c], y[c]);
+        const cd m = cmulc(u[a][c], y[c]);
         sr += m.x;
         si += m.y;
       }
-      block_sum2(sr, si, red);
-      const cd s = {t * sr, t * si};
+      d[2 * a] = sr;
+      d[2 * a + 1] = si;
+    }
+    block_sum_n<2 * R>(d, red);
+    cd sa[R];
 #pragma unroll
-      for (int c = 0; c < C; ++c) {
-        const cd m = cmul(vc[c], s);
+    for (int a = 0; a < R; ++a) {
+      cd x = {d[2 * a], d[2 * a + 1]};
+#pragma unroll
+      for (int b = 0; b < a; ++b) {
+        const int pi = a * (a - 1) / 2 + b;
+        const cd gab = {lane_value(g.x, pi), lane_value(g.y, pi)};
+        const cd m = cmul(gab, sa[b]);
+        x.x -= m.x;
+        x.y -= m.y;
+      }
+      const double ta = lane_value(t, a);
+      sa[a] = {ta * x.x, ta * x.y};
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int a = 0; a < R; ++a) {
+        const cd m = cmul(u[a][c], sa[a]);
         y[c].x -= m.x;
         y[c].y -= m.y;
       }
-    }
-#pragma unroll
-    for (int c = 0; c < C; ++c) vc[c] = vn[c];
-    t = tn;
+  };
+  cd u0[R][C], u1[R][C];
+  double t0 = 0.0, t1 = 0.0;
+  cd g0 = {0.0, 0.0}, g1 = {0.0, 0.0};
+  if (ngroups > 0) load_group(0, u0, t0, g0);
+  for (int r = 0; r < ngroups; r += 2) {  // (an odd count ends with an empty group)
+    apply_group(r, u0, t0, g0, u1, t1, g1);
+    apply_group(r + 1, u1, t1, g1, u0, t0, g0);
   }
 #pragma unroll
   for (int c = 0; c < C; ++c) {
@@ -941,6 +1057,7 @@ int eig_topk_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, i
   es.part = 2 * (size_t)nparts_max;
   es.k = (size_t)k;
   es.z = (size_t)k * n;
+  es.g = 3 * (size_t)n;
   const size_t B = (size_t)count;
   size_t off = 0;
   auto carve = [&](size_t bytes) {
@@ -953,7 +1070,7 @@ int eig_topk_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, i
                oPart = carve(B * es.part * 16), oE = carve(B * n * 8), oE2 = carve(B * n * 8),
                oZ = carve(B * es.z * 8), oWork = carve(B * 5 * es.z * 8),
                oCl = carve(B * (k + 1) * 4), oNcl = carve(B * 4), oBnd = carve(B * 3 * 8),
-               oTn = carve(B * 8);
+               oTn = carve(B * 8), oG = carve(B * es.g * 16);
   KLT_TRY(ctx->eig_ws.ensure(off));
   char* base = ctx->eig_ws.as<char>();
   double* A = reinterpret_cast<double*>(base + oA);
@@ -971,6 +1088,7 @@ int eig_topk_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, i
   int* ncl_dev = reinterpret_cast<int*>(base + oNcl);
   double* bnd = reinterpret_cast<double*>(base + oBnd);
   double* tn_dev = reinterpret_cast<double*>(base + oTn);
+  cd* G = reinterpret_cast<cd*>(base + oG);
   hipStream_t st = ctx->stream;
   KLT_HIP(hipMemcpyAsync(A, a_dev, B * nn * 16, hipMemcpyDeviceToDevice, st));
   KLT_HIP(hipMemsetAsync(tau, 0, B * n * 8, st));
@@ -1084,15 +1202,22 @@ int eig_topk_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, i
   KLT_HIP(hipMemcpyAsync(ncl_dev, hncl.data(), B * 4, hipMemcpyHostToDevice, st));
   KLT_LAUNCH(ctx, "k_inverse_iteration", k_inverse_iteration, dim3(ncl_max, count), dim3(256),
              0, diag, e, n, w_dev, cl, ncl_dev, tn_dev, z, work, es);
+  // reflector groups of 4 (2 where 8 entries a thread leave fewer registers);
+  // n <= 2 has no reflectors
   if (n <= kThreads * 5) {
-    KLT_LAUNCH(ctx, "k_back_transform", k_back_transform_reg<5>, dim3(k, count), dim3(kThreads),
-               lds_v, z, n, V, tau, sub, reinterpret_cast<cd*>(vec_dev), es);
+    if (n > 2)
+      KLT_LAUNCH(ctx, "k_reflector_gram", k_reflector_gram<4>, dim3((n - 2 + 3) / 4, count),
+                 dim3(kThreads), 0, n, V, G, es);
+    KLT_LAUNCH(ctx, "k_back_transform", (k_back_transform_reg<5, 4>), dim3(k, count),
+               dim3(kThreads), lds_v, z, n, V, tau, sub, G, reinterpret_cast<cd*>(vec_dev), es);
   } else if (n <= kThreads * 8) {
     if (lds_v > 64 * 1024)
-      KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_back_transform_reg<8>),
+      KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_back_transform_reg<8, 2>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_v));
-    KLT_LAUNCH(ctx, "k_back_transform", k_back_transform_reg<8>, dim3(k, count), dim3(kThreads),
-               lds_v, z, n, V, tau, sub, reinterpret_cast<cd*>(vec_dev), es);
+    KLT_LAUNCH(ctx, "k_reflector_gram", k_reflector_gram<2>, dim3((n - 2 + 1) / 2, count),
+               dim3(kThreads), 0, n, V, G, es);  // (n > 1280 here)
+    KLT_LAUNCH(ctx, "k_back_transform", (k_back_transform_reg<8, 2>), dim3(k, count),
+               dim3(kThreads), lds_v, z, n, V, tau, sub, G, reinterpret_cast<cd*>(vec_dev), es);
   } else {
     KLT_LAUNCH(ctx, "k_back_transform", k_back_transform, dim3(k, count), dim3(kThreads), lds_v,
                z, n, V, tau, sub, reinterpret_cast<cd*>(vec_dev), es);
