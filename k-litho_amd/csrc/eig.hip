@@ -535,18 +535,21 @@ __global__ __launch_bounds__(64) void k_bisect(const double* __restrict__ d,
 }
 
 // LU of T_r - lam I with partial pivoting between rows i and i+1, into the
-// 5n doubles at lu: U diagonal, first and second superdiagonals, multipliers,
-// and the swap flags.  The sweeps below are serial in i; each runs in chunks
+// 5n doubles at lu: reciprocals of U's diagonal (so that the back substitution
+// multiplies: a division on its serial chain costs about ten dependent FP64
+// operations), U's first and second superdiagonals, multipliers, and the swap
+// flags.  The sweeps below are serial in i; each runs in chunks
 // of kSweep steps whose loads are issued together ahead of the chunk's
 // recurrence, so that the memory latency is paid once per chunk instead of
 // once per step.
-constexpr int kSweep = 8;
+constexpr int kSweep = 16;
 constexpr int kGs = 8;  // earlier vectors per Gram-Schmidt round (registers)
+constexpr int kInvIters = 3;  // inverse iteration rounds
 
 __device__ void tri_lu(const double* __restrict__ d, const double* __restrict__ e, int n,
                        double lam, double tnorm, double* __restrict__ lu) {
   const double eps = 2.220446049250313e-16;
-  double* lu_d = lu;
+  double* lu_rd = lu;
   double* lu_u1 = lu + n;
   double* lu_u2 = lu + 2 * n;
   double* lu_l = lu + 3 * n;
@@ -558,7 +561,7 @@ __device__ void tri_lu(const double* __restrict__ d, const double* __restrict__ 
     if (fabs(a) >= fabs(cdn)) {
       if (a == 0.0) a = eps * tnorm;
       const double l = cdn / a;
-      od = a;
+      od = 1.0 / a;
       ou1 = b;
       ou2 = 0.0;
       ol = l;
@@ -567,7 +570,7 @@ __device__ void tri_lu(const double* __restrict__ d, const double* __restrict__ 
       b = en;
     } else {
       const double l = a / cdn;
-      od = cdn;
+      od = 1.0 / cdn;
       ou1 = dn;
       ou2 = en;
       ol = l;
@@ -590,7 +593,7 @@ __device__ void tri_lu(const double* __restrict__ d, const double* __restrict__ 
     for (int u = 0; u < kSweep; ++u) step(cv[u], dv[u], ev[u], od[u], ou1[u], ou2[u], ol[u], op[u]);
 #pragma unroll
     for (int u = 0; u < kSweep; ++u) {
-      lu_d[i + u] = od[u];
+      lu_rd[i + u] = od[u];
       lu_u1[i + u] = ou1[u];
       lu_u2[i + u] = ou2[u];
       lu_l[i + u] = ol[u];
@@ -599,15 +602,15 @@ __device__ void tri_lu(const double* __restrict__ d, const double* __restrict__ 
   }
   for (; i < n - 1; ++i) {
     const double cdn = e[i], dn = d[i + 1] - lam, en = i + 1 < n - 1 ? e[i + 1] : 0.0;
-    step(cdn, dn, en, lu_d[i], lu_u1[i], lu_u2[i], lu_l[i], piv[i]);
+    step(cdn, dn, en, lu_rd[i], lu_u1[i], lu_u2[i], lu_l[i], piv[i]);
   }
   if (a == 0.0) a = eps * tnorm;
-  lu_d[n - 1] = a;
+  lu_rd[n - 1] = 1.0 / a;
 }
 
 // x <- (T_r - lam I)^{-1} x with the factors of tri_lu.
 __device__ void tri_solve(const double* __restrict__ lu, int n, double* __restrict__ x) {
-  const double* lu_d = lu;
+  const double* lu_rd = lu;
   const double* lu_u1 = lu + n;
   const double* lu_u2 = lu + 2 * n;
   const double* lu_l = lu + 3 * n;
@@ -648,10 +651,10 @@ __device__ void tri_solve(const double* __restrict__ lu, int n, double* __restri
   }
   x[n - 1] = xi;
   // U: backward
-  double x1 = x[n - 1] / lu_d[n - 1];
+  double x1 = x[n - 1] * lu_rd[n - 1];
   x[n - 1] = x1;
   if (n < 2) return;
-  double x0 = (x[n - 2] - lu_u1[n - 2] * x1) / lu_d[n - 2];
+  double x0 = (x[n - 2] - lu_u1[n - 2] * x1) * lu_rd[n - 2];
   x[n - 2] = x0;
   i = n - 3;
   for (; i - (kSweep - 1) >= 0; i -= kSweep) {
@@ -661,11 +664,11 @@ __device__ void tri_solve(const double* __restrict__ lu, int n, double* __restri
       xv[u] = x[i - u];
       u1v[u] = lu_u1[i - u];
       u2v[u] = lu_u2[i - u];
-      dv[u] = lu_d[i - u];
+      dv[u] = lu_rd[i - u];
     }
 #pragma unroll
     for (int u = 0; u < kSweep; ++u) {
-      const double v = (xv[u] - u1v[u] * x0 - u2v[u] * x1) / dv[u];
+      const double v = (xv[u] - u1v[u] * x0 - u2v[u] * x1) * dv[u];
       xv[u] = v;
       x1 = x0;
       x0 = v;
@@ -674,7 +677,7 @@ __device__ void tri_solve(const double* __restrict__ lu, int n, double* __restri
     for (int u = 0; u < kSweep; ++u) x[i - u] = xv[u];
   }
   for (; i >= 0; --i) {
-    const double v = (x[i] - lu_u1[i] * x0 - lu_u2[i] * x1) / lu_d[i];
+    const double v = (x[i] - lu_u1[i] * x0 - lu_u2[i] * x1) * lu_rd[i];
     x[i] = v;
     x1 = x0;
     x0 = v;
@@ -688,6 +691,11 @@ __device__ void tri_solve(const double* __restrict__ lu, int n, double* __restri
 // eigenvalue order (Gram-Schmidt applied twice, against kGs earlier vectors
 // at a time, parallel over the vector length) and normalises them.
 // Equal eigenvalues of a cluster are separated slightly in the shifts.
+// Three rounds: a shift within 10 eps ||T|| of lambda_q amplifies the wanted
+// component over any eigenvalue at least 1e-4 ||T|| away (every eigenvalue
+// outside the cluster) by more than 1e10 a round, so the random start
+// converges in two and the third leaves the error at rounding level; within a
+// cluster the re-orthogonalisation keeps the vectors apart.
 // Matrix mi has ncl[mi] clusters starting at cl_begin[mi (k + 1) ..] and the
 // tridiagonal norm tnorms[mi]; blocks past its cluster count return.
 __global__ __launch_bounds__(256) void k_inverse_iteration(
@@ -722,7 +730,7 @@ __global__ __launch_bounds__(256) void k_inverse_iteration(
       x[i] = ((double)(s & 0xffffff) / 16777216.0) - 0.5;
     }
   }
-  for (int it = 0; it < 6; ++it) {
+  for (int it = 0; it < kInvIters; ++it) {
     for (int q = first + tid; q < last; q += 256)
       tri_solve(work + (size_t)q * 5 * n, n, z + (size_t)q * n);
     __syncthreads();
