@@ -12,8 +12,9 @@
 //      The off-diagonal entries are complex; a diagonal unitary D maps the
 //      tridiagonal T to the real symmetric T_r = D^H T D with off-diagonal
 //      |e_j|.
-//   2. k_bisect: the k largest eigenvalues of T_r by multisection on Sturm
-//      counts (one wave per eigenvalue, 64 points per pass).
+//   2. k_bisect: the k largest eigenvalues of T_r by multisection on
+//      eigenvalue counts (64 points per pass, each count from a twisted
+//      factorization whose two halves run on two waves).
 //   3. k_inverse_iteration: eigenvectors of T_r by inverse iteration with a
 //      partially pivoted LU of T_r - lambda I (one thread per vector);
 //      eigenvalues closer than 1e-4 ||T|| form a cluster whose vectors are
@@ -464,21 +465,34 @@ __global__ void k_hh_finish(const double* __restrict__ A, int n, cd* __restrict_
   diag[n - 1] = A[2 * ((size_t)(n - 1) * n + n - 1)];
 }
 
-// Number of eigenvalues of T_r below x (Sturm sequence of the LDL^T pivots).
-// d and e2 are in LDS; the loads of 8 steps are issued ahead of their
-// recurrence steps.
-__device__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
-  int c = 0;
-  double q = d[0] - x;
+// Number of eigenvalues of T_r below x from a twisted factorization
+// T_r - x I = N Delta N^T with twist index r (Sylvester's law of inertia:
+// the count is the number of negative entries of Delta).  Delta holds the
+// top-down pivots D+_i, i < r, the bottom-up pivots D-_i, i > r, and
+//   gamma_r = d_r - x - e2_{r-1} / D+_{r-1} - e2_r / D-_{r+1},
+// so the two halves run as independent chains of n / 2 steps (one wave
+// each) instead of one chain of n.  Pivots smaller than pivmin count as
+// -pivmin.  d and e2 are in LDS; the loads of 8 steps are issued ahead of
+// their recurrence steps.
+__device__ __forceinline__ void pivot_step(double& q, int& c, double di, double ei, double x,
+                                           double pivmin) {
+  q = di - x - ei / q;
   if (fabs(q) < pivmin) q = -pivmin;
   if (q < 0) ++c;
-  auto step = [&](double di, double ei) {
-    q = di - x - ei / q;
+}
+
+// D+_0 .. D+_{r-1}: their negative count; qlast = D+_{r-1}.
+__device__ int sturm_top(const double* d, const double* e2, int r, double x, double pivmin,
+                         double& qlast) {
+  int c = 0;
+  double q = 1.0;
+  if (r > 0) {
+    q = d[0] - x;
     if (fabs(q) < pivmin) q = -pivmin;
     if (q < 0) ++c;
-  };
+  }
   int i = 1;
-  for (; i + 8 <= n; i += 8) {
+  for (; i + 8 <= r; i += 8) {
     double dv[8], ev[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -486,30 +500,63 @@ __device__ int sturm_count(const double* d, const double* e2, int n, double x, d
       ev[u] = e2[i - 1 + u];
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) step(dv[u], ev[u]);
+    for (int u = 0; u < 8; ++u) pivot_step(q, c, dv[u], ev[u], x, pivmin);
   }
-  for (; i < n; ++i) step(d[i], e2[i - 1]);
+  for (; i < r; ++i) pivot_step(q, c, d[i], e2[i - 1], x, pivmin);
+  qlast = q;
   return c;
 }
 
-// Wave idx finds the (idx+1)-th largest eigenvalue by multisection: its 64
-// lanes count the eigenvalues below 64 equally spaced interior points of the
-// bracket at once (Sturm sequences), and the bracket shrinks to the gap
-// between the two points where the count passes the target, 65x per pass.
-// bounds[mi] = (lower bound, upper bound, pivot threshold) of matrix mi.
-__global__ __launch_bounds__(64) void k_bisect(const double* __restrict__ d,
-                                               const double* __restrict__ e2, int n, int k,
-                                               const double* __restrict__ bounds,
-                                               double* __restrict__ w, EigStride st) {
+// D-_{n-1} .. D-_{r+1} (D-_i = d_i - x - e2_i / D-_{i+1}): their negative
+// count; qlast = D-_{r+1}.
+__device__ int sturm_bottom(const double* d, const double* e2, int n, int r, double x,
+                            double pivmin, double& qlast) {
+  int c = 0;
+  double q = 1.0;
+  if (r + 1 < n) {
+    q = d[n - 1] - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    if (q < 0) ++c;
+  }
+  int i = n - 2;
+  for (; i - 7 >= r + 1; i -= 8) {
+    double dv[8], ev[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      dv[u] = d[i - u];
+      ev[u] = e2[i - u];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) pivot_step(q, c, dv[u], ev[u], x, pivmin);
+  }
+  for (; i >= r + 1; --i) pivot_step(q, c, d[i], e2[i], x, pivmin);
+  qlast = q;
+  return c;
+}
+
+// The k largest eigenvalues of T_r: block idx finds eigenvalue idx
+// (descending) by multisection of [lo, hi] at 64 points a pass.  Wave 0 runs
+// the top halves of the 64 twisted factorizations, wave 1 the bottom halves;
+// wave 0 combines them, picks the subinterval and shares it through LDS, so
+// that both waves leave the loop together.  Matrix mi (grid y) takes its
+// Gershgorin interval and pivot threshold from bounds[3 mi ..].
+__global__ __launch_bounds__(128) void k_bisect(const double* __restrict__ d,
+                                                const double* __restrict__ e2, int n, int k,
+                                                const double* __restrict__ bounds,
+                                                double* __restrict__ w, EigStride st) {
   extern __shared__ double sde[];  // d, then e2 (n entries each)
-  const int idx = blockIdx.x, lane = threadIdx.x;
+  __shared__ double sq[2][64];
+  __shared__ int sc[2][64];
+  __shared__ double s_lohi[2];
+  __shared__ int s_go;
+  const int idx = blockIdx.x, tid = threadIdx.x, lane = tid & 63, half = tid >> 6;
   if (idx >= k) return;
   const size_t mi = blockIdx.y;
   d += mi * st.n;
   e2 += mi * st.n;
   w += mi * st.k;
   const double lo0 = bounds[3 * mi], hi0 = bounds[3 * mi + 1], pivmin = bounds[3 * mi + 2];
-  for (int i = lane; i < n; i += 64) {
+  for (int i = tid; i < n; i += 128) {
     sde[i] = d[i];
     sde[n + i] = e2[i];
   }
@@ -517,21 +564,41 @@ __global__ __launch_bounds__(64) void k_bisect(const double* __restrict__ d,
   const double* ld = sde;
   const double* le2 = sde + n;
   const int target = n - 1 - idx;  // ascending index
+  const int r = n / 2;
   double lo = lo0, hi = hi0;
   for (int it = 0; it < 64; ++it) {
-    if (hi - lo <= 2.2e-16 * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+    if (hi - lo <= 2.2e-16 * fmax(fabs(lo), fabs(hi)) + pivmin) break;  // (uniform)
     const double h = hi - lo;
     const double x = lo + h * (double)(lane + 1) / 65.0;
-    const bool above = sturm_count(ld, le2, n, x, pivmin) > target;
-    const unsigned long long mask = __ballot(above);
-    const int f = mask ? __ffsll(mask) - 1 : 64;  // first point past the eigenvalue
-    const double nlo = f == 0 ? lo : lo + h * (double)f / 65.0;
-    const double nhi = f == 64 ? hi : lo + h * (double)(f + 1) / 65.0;
-    if (!(nlo > lo || nhi < hi)) break;  // no progress left in double precision
-    lo = nlo;
-    hi = nhi;
+    double qe;
+    const int c = half == 0 ? sturm_top(ld, le2, r, x, pivmin, qe)
+                            : sturm_bottom(ld, le2, n, r, x, pivmin, qe);
+    sq[half][lane] = qe;
+    sc[half][lane] = c;
+    __syncthreads();
+    if (half == 0) {
+      double g = ld[r] - x;
+      if (r > 0) g -= le2[r - 1] / sq[0][lane];
+      if (r + 1 < n) g -= le2[r] / sq[1][lane];
+      if (fabs(g) < pivmin) g = -pivmin;
+      const int cnt = sc[0][lane] + sc[1][lane] + (g < 0 ? 1 : 0);
+      const unsigned long long mask = __ballot(cnt > target);
+      const int f = mask ? __ffsll(mask) - 1 : 64;  // first point past the eigenvalue
+      const double nlo = f == 0 ? lo : lo + h * (double)f / 65.0;
+      const double nhi = f == 64 ? hi : lo + h * (double)(f + 1) / 65.0;
+      const bool go = nlo > lo || nhi < hi;  // progress left in double precision
+      if (lane == 0) {
+        s_go = go;
+        s_lohi[0] = go ? nlo : lo;
+        s_lohi[1] = go ? nhi : hi;
+      }
+    }
+    __syncthreads();
+    lo = s_lohi[0];
+    hi = s_lohi[1];
+    if (!s_go) break;
   }
-  if (lane == 0) w[idx] = 0.5 * (lo + hi);
+  if (tid == 0) w[idx] = 0.5 * (lo + hi);
 }
 
 // LU of T_r - lam I with partial pivoting between rows i and i+1, into the
@@ -1184,7 +1251,7 @@ int eig_topk_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, i
   if (lds_de > 64 * 1024)
     KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bisect),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_de));
-  KLT_LAUNCH(ctx, "k_bisect", k_bisect, dim3(k, count), dim3(64), lds_de, diag, e2, n, k, bnd,
+  KLT_LAUNCH(ctx, "k_bisect", k_bisect, dim3(k, count), dim3(128), lds_de, diag, e2, n, k, bnd,
              w_dev, es);
   std::vector<double> hw(B * k);
   KLT_HIP(hipMemcpyAsync(hw.data(), w_dev, B * k * 8, hipMemcpyDeviceToHost, st));
