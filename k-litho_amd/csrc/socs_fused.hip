@@ -1612,7 +1612,7 @@ static bool split_ok(int units, int parts) {
 // Picks the split factors for one launch shape: ks column ranges per band
 // row block, rs row ranges per synthesis strip, kg kernel groups per image in
 // the coarse-grid stage.  With tuning on (the default) the first call of a
-// shape runs `run` once per candidate on the context stream, timed with HIP
+// shape runs `run` twice per candidate on the context stream, timed with HIP
 // events, one factor at a time, and keeps the fastest; with tuning off, or
 // while the stream is being captured or launches are being profiled, the
 // smallest factors that give each grid two blocks per CU (one for kg).
@@ -1650,12 +1650,20 @@ static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, in
     return KLT_OK;
   }
   hipEvent_t e0 = ctx->take_event(), e1 = ctx->take_event();
+  // each candidate runs twice and keeps the faster time, so that one slow
+  // run (a clock change, another process on the device) does not decide
   auto time_run = [&](const klt_ctx::Tuned& c, float* ms) -> int {
-    KLT_HIP(hipEventRecord(e0, ctx->stream));
-    KLT_TRY(run(c));
-    KLT_HIP(hipEventRecord(e1, ctx->stream));
-    KLT_HIP(hipEventSynchronize(e1));
-    KLT_HIP(hipEventElapsedTime(ms, e0, e1));
+    float best_ms = 0.f;
+    for (int rep = 0; rep < 2; ++rep) {
+      float t = 0.f;
+      KLT_HIP(hipEventRecord(e0, ctx->stream));
+      KLT_TRY(run(c));
+      KLT_HIP(hipEventRecord(e1, ctx->stream));
+      KLT_HIP(hipEventSynchronize(e1));
+      KLT_HIP(hipEventElapsedTime(&t, e0, e1));
+      best_ms = rep == 0 ? t : std::min(best_ms, t);
+    }
+    *ms = best_ms;
     return KLT_OK;
   };
   KLT_TRY(run(h));  // warm-up (first launches of this shape)
