@@ -8,7 +8,9 @@
 //        k_hh_her2 : A22 -= v w^H + w v^H, w = p - (tau/2)(v^H p) v.
 //      For n <= 3400 one launch per step (k_hh_step) does both: it applies
 //      step j-1's rank-2 update to row j (for the reflector) and to the
-//      block's rows of the trailing matrix while multiplying them by v.
+//      block's rows of the trailing matrix while multiplying them by v;
+//      once the trailing block has at most kTail rows, k_hh_tail runs the
+//      remaining steps in the LDS of one block per matrix.
 //      The off-diagonal entries are complex; a diagonal unitary D maps the
 //      tridiagonal T to the real symmetric T_r = D^H T D with off-diagonal
 //      |e_j|.
@@ -447,6 +449,167 @@ __global__ __launch_bounds__(kThreads) void k_hh_her2(double* __restrict__ A, in
     av.x -= ur;
     av.y -= ui;
     *a = av;
+  }
+}
+
+// The last steps of the reduction in one block per matrix, from step j0 on,
+// once the trailing block A[j0.., j0..] (s = n - j0 <= kTail rows) fits in
+// LDS: the launches of those steps would each cost the fixed latency of a
+// device-wide step for a few rows of work.  The block loads the trailing
+// block, applies step j0-1's pending rank-2 update (as k_hh_step would), then
+// runs steps j0 .. n-3 in LDS with the same reflector conventions (V rows,
+// tau, sub, diag) and finishes the last 2 x 2 block (as k_hh_finish).
+// Dynamic LDS: (s^2 + 3 s) complex values.
+constexpr int kTail = 96;
+constexpr int kTailThreads = 1024;
+
+// Block sum of (a, b) over kTailThreads threads (red: 32 doubles); valid in
+// every thread.
+__device__ __forceinline__ void tail_sum2(double& a, double& b, double* red) {
+  constexpr int NW = kTailThreads / 64;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    red[2 * w] = a;
+    red[2 * w + 1] = b;
+  }
+  __syncthreads();
+  a = 0.0;
+  b = 0.0;
+#pragma unroll
+  for (int q = 0; q < NW; ++q) {
+    a += red[2 * q];
+    b += red[2 * q + 1];
+  }
+}
+
+__global__ __launch_bounds__(kTailThreads) void k_hh_tail(
+    const double* __restrict__ A, int n, int j0, cd* __restrict__ V,
+    double* __restrict__ tau, cd* __restrict__ sub, double* __restrict__ diag,
+    const cd* __restrict__ p_prev, const cd* __restrict__ part_prev, int nparts_prev,
+    EigStride st) {
+  extern __shared__ cd tsh[];
+  __shared__ double red[2 * (kTailThreads / 64)];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int NW = kTailThreads / 64;
+  {
+    const size_t mi = blockIdx.y;
+    A += mi * st.a;
+    V += mi * st.v;
+    tau += mi * st.n;
+    sub += mi * st.n;
+    diag += mi * st.n;
+    p_prev += mi * st.p;
+    part_prev += mi * st.part;
+  }
+  const int s = n - j0;
+  cd* S = tsh;           // S[i][l] = A[j0 + i][j0 + l], s x s
+  cd* sv = S + s * s;    // reflector v (and v' of the pending update)
+  cd* sw = sv + s;       // w (and w')
+  cd* sp = sw + s;       // p
+  const cd* Ag = reinterpret_cast<const cd*>(A);
+  for (int q = tid; q < s * s; q += kTailThreads) {
+    const int i = q / s, l = q - i * s;
+    S[q] = Ag[(size_t)(j0 + i) * n + j0 + l];
+  }
+  // step j0-1's update S -= v' w'^H + w' v'^H, w' = p' - K' v'
+  const double tp = j0 > 0 ? tau[j0 - 1] : 0.0;
+  if (tp != 0.0) {  // (uniform)
+    double kr = 0.0, ki = 0.0;
+    for (int q = tid; q < nparts_prev; q += kTailThreads) {
+      kr += part_prev[q].x;
+      ki += part_prev[q].y;
+    }
+    tail_sum2(kr, ki, red);
+    const cd K = {0.5 * tp * kr, 0.5 * tp * ki};
+    if (tid < s) {
+      const cd v = V[(size_t)(j0 - 1) * n + j0 + tid], pp = p_prev[tid], kv = cmul(K, v);
+      sv[tid] = v;
+      sw[tid] = {pp.x - kv.x, pp.y - kv.y};
+    }
+    __syncthreads();
+    for (int q = tid; q < s * s; q += kTailThreads) {
+      const int i = q / s, l = q - i * s;
+      const cd vi = sv[i], wi = sw[i], vl = sv[l], wl = sw[l];
+      S[q].x -= (vi.x * wl.x + vi.y * wl.y) + (wi.x * vl.x + wi.y * vl.y);
+      S[q].y -= (vi.y * wl.x - vi.x * wl.y) + (wi.y * vl.x - wi.x * vl.y);
+    }
+  }
+  __syncthreads();
+  for (int t = 0; t + 2 < s; ++t) {
+    const int j = j0 + t, m = s - t - 1;
+    const cd* St = S + t * s + t + 1;  // row t, columns t + 1 ..
+    // the reflector of x = conj(row t)
+    cd xi = {0.0, 0.0};
+    if (tid < m) xi = {St[tid].x, -St[tid].y};
+    double s2 = xi.x * xi.x + xi.y * xi.y, dummy = 0.0;
+    tail_sum2(s2, dummy, red);
+    const cd x0 = {St[0].x, -St[0].y};
+    const double ax0 = hypot(x0.x, x0.y);
+    const double alpha = sqrt(s2);
+    const bool skip = (s2 - ax0 * ax0) <= 0.0;  // x already a multiple of e_1
+    cd ph = {1.0, 0.0};
+    if (ax0 > 0.0) ph = {x0.x / ax0, x0.y / ax0};
+    const double tt = skip ? 0.0 : 1.0 / (alpha * (alpha + ax0));  // 2 / (v^H v)
+    if (tid < m) {
+      cd v = xi;
+      if (tid == 0 && !skip) {
+        v.x += ph.x * alpha;
+        v.y += ph.y * alpha;
+      }
+      sv[tid] = v;
+      V[(size_t)j * n + j + 1 + tid] = skip ? cd{0.0, 0.0} : v;
+    }
+    if (tid == 0) {
+      tau[j] = tt;
+      sub[j] = skip ? x0 : cd{-ph.x * alpha, -ph.y * alpha};
+      diag[j] = S[t * s + t].x;
+    }
+    if (skip) continue;  // (uniform) the trailing block stays as it is
+    __syncthreads();
+    // p_i = tau sum_l S[t+1+i][t+1+l] v_l, one wave per row
+    const cd* S22 = S + (t + 1) * s + t + 1;
+    double pr = 0.0, pim = 0.0;  // this wave's partial conj(v) . p
+    for (int i = wave; i < m; i += NW) {
+      double sr = 0.0, si = 0.0;
+      for (int l = lane; l < m; l += 64) {
+        const cd a = S22[i * s + l], v = sv[l];
+        sr += a.x * v.x - a.y * v.y;
+        si += a.x * v.y + a.y * v.x;
+      }
+      sr = wave_sum(sr) * tt;
+      si = wave_sum(si) * tt;
+      if (lane == 0) {
+        sp[i] = {sr, si};
+        const cd v = sv[i];
+        pr += v.x * sr + v.y * si;
+        pim += v.x * si - v.y * sr;
+      }
+    }
+    tail_sum2(pr, pim, red);  // (its barriers also publish sp)
+    const cd K = {0.5 * tt * pr, 0.5 * tt * pim};
+    if (tid < m) {
+      const cd v = sv[tid], pp = sp[tid], kv = cmul(K, v);
+      sw[tid] = {pp.x - kv.x, pp.y - kv.y};
+    }
+    __syncthreads();
+    for (int q = tid; q < m * m; q += kTailThreads) {
+      const int i = q / m, l = q - i * m;
+      const cd vi = sv[i], wi = sw[i], vl = sv[l], wl = sw[l];
+      cd& a = S[(t + 1 + i) * s + t + 1 + l];
+      a.x -= (vi.x * wl.x + vi.y * wl.y) + (wi.x * vl.x + wi.y * vl.y);
+      a.y -= (vi.y * wl.x - vi.x * wl.y) + (wi.y * vl.x - wi.x * vl.y);
+    }
+    __syncthreads();
+  }
+  // the last 2 x 2 block (A[n-1][n-2] = conj(A[n-2][n-1]))
+  if (tid == 0) {
+    const cd a = S[(s - 2) * s + s - 1];
+    sub[n - 2] = {a.x, -a.y};
+    diag[n - 2] = S[(s - 2) * s + s - 2].x;
+    diag[n - 1] = S[(s - 1) * s + s - 1].x;
   }
 }
 
@@ -1181,9 +1344,24 @@ int eig_topk_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, i
   if (fused && lds_step > 64 * 1024)
     KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hh_step),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_step));
+  const size_t lds_tail = (size_t)(kTail * kTail + 3 * kTail) * 16;
+  if (fused)
+    KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hh_tail),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_tail));
   int np_prev = 0;
+  bool tail = false;
   for (int j = 0; j + 2 < n; ++j) {
     const int m = n - j - 1;
+    if (fused && n - j <= kTail) {  // the remaining steps in one block per matrix
+      const int sz = n - j;
+      const cd* pp = p + (size_t)((j + 1) & 1) * n;
+      const cd* partp = part + (size_t)((j + 1) & 1) * nparts_max;
+      KLT_LAUNCH(ctx, "k_hh_tail", k_hh_tail, dim3(1, count), dim3(kTailThreads),
+                 (size_t)(sz * sz + 3 * sz) * 16, A, n, j, V, tau, sub, diag, pp, partp,
+                 np_prev, es);
+      tail = true;
+      break;
+    }
     // the fused step: one row per wave; the two-kernel path: 16 rows a block
     const int nblk = std::min(nparts_max, fused ? (m + 3) / 4 : (m + 15) / 16);
     const int rows = (m + nblk - 1) / nblk;
@@ -1210,7 +1388,8 @@ int eig_topk_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, i
     }
     np_prev = np;
   }
-  KLT_LAUNCH(ctx, "k_hh_finish", k_hh_finish, dim3(count), dim3(64), 0, A, n, sub, diag, es);
+  if (!tail)
+    KLT_LAUNCH(ctx, "k_hh_finish", k_hh_finish, dim3(count), dim3(64), 0, A, n, sub, diag, es);
   KLT_HIP(hipGetLastError());
   // per matrix: real tridiagonal, Gershgorin bounds, pivot threshold (host: O(n))
   std::vector<double> hd(B * n);
