@@ -175,6 +175,27 @@ def test_eig_batch_equals_single_calls(ctx):
     _check(mats[1], wb[1], _c(vb[1]), wo, vo, k)
 
 
+@pytest.mark.parametrize("n,count", [(40, 3), (96, 2), (97, 2), (99, 1)])
+def test_eig_small_orders_and_tail(ctx, n, count):
+    """Orders around the last-columns block (the final 96 columns run in one
+    LDS-resident block per matrix): n = 40 and 96 run entirely there, 97 and
+    99 take one and three device-wide steps first.  Batched results equal
+    single calls bit for bit and match the oracle."""
+    print(f"case eig small n={n} count={count}", flush=True)
+    rng = np.random.default_rng(20 + n)
+    mats = []
+    for _ in range(count):
+        a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        mats.append(0.5 * (a + a.conj().T))
+    k = min(12, n)
+    wb, vb = ctx.eig_topk_batch(np.stack([_pack(a) for a in mats]), k)
+    for i, a in enumerate(mats):
+        w, v = ctx.eig_topk(_pack(a), k)
+        assert np.array_equal(wb[i], w) and np.array_equal(vb[i], v), i
+        wo, vo = O.eigh_topk(a, k)
+        _check(a, w, _c(v), wo, vo, k)
+
+
 def test_calc_kernels_layout_and_image(ctx):
     print("case calc_kernels", flush=True)
     a, Nx, Ny = _tcc(512, inner=0.6, outer=0.9)
