@@ -14,12 +14,12 @@
 // given (after createSource's normalisation), for every grid point inside the
 // unit disc (S_s = 0 included).
 //
-//   1. TCC of the source (split-FP16 matrix-core Gram: within 2e-7 of the
+//   1. mask bands by the matrix-core band pass, M^ (2N+1)^2 per mask
+//      (k_abbe_assemble), queued before the host reads a device source;
+//   2. TCC of the source (split-FP16 matrix-core Gram: within 2e-7 of the
 //      FP64 Gram in relative Frobenius norm, far inside the gradient's
-//      tolerance);
-//   2. mask bands by the matrix-core band pass, M^ (2N+1)^2 per mask
-//      (k_abbe_assemble), and the Hopkins image I_A of each mask from it (one
-//      wave per spectrum entry, then the matrix-core synthesis);
+//      tolerance), and the Hopkins image I_A of each mask from M^ (one wave
+//      per spectrum entry, then the matrix-core synthesis);
 //   3. resist bands (G, loss partials) by the band pass, then per mask the
 //      loss and the coarse-grid g_c of G^ (resist_coarse_grid);
 //   4. the double sum on the 64 x 64 coarse grid (abbe_grad_coarse in
@@ -97,12 +97,18 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
     KLT_HIP(hipMemcpyAsync(s + bytes, targets, bytes, hipMemcpyHostToDevice, st));
     dm = reinterpret_cast<const float*>(s);
     dt = reinterpret_cast<const float*>(s + bytes);
-  } else {
-    KLT_HIP(hipMemcpyAsync(hsrc.data(), src, ns2 * sizeof(float), hipMemcpyDeviceToHost, st));
-    KLT_HIP(hipStreamSynchronize(st));
   }
-  float sigma = klt_outer_sigma(hsrc.data(), srcSize);
-  if (sigma > 1.0f) sigma = 1.0f;
+  // a device source is read into pinned memory and waited for with an event
+  // after the mask band pass (which does not need it) has been queued, so that
+  // the band pass runs while the host prepares the TCC build
+  hipEvent_t src_read = nullptr;
+  void* src_pinned = nullptr;
+  if (!(flags & KLT_HOST_PTRS)) {
+    KLT_TRY(ctx->download_area(ns2 * sizeof(float), &src_pinned));
+    KLT_HIP(hipMemcpyAsync(src_pinned, src, ns2 * sizeof(float), hipMemcpyDeviceToHost, st));
+    src_read = ctx->take_event();
+    KLT_HIP(hipEventRecord(src_read, st));
+  }
   const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
   const int Gy = 4 * Ny + 1;
   // band partials: row blocks of 128 rows, each split into ks column ranges
@@ -121,12 +127,7 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
     KLT_TRY(ctx->bwd_img.ensure(bytes));
     I = ctx->bwd_img.as<float>();
   }
-  // 1. TCC of the source
-  KLT_TRY(ctx->abbe_tcc.ensure((size_t)T * T * 16));
-  double* tcc = ctx->abbe_tcc.as<double>();
-  KLT_TRY(klt::tcc_build_device(ctx, hsrc.data(), srcSize, sigma, lambda, NA, defocus, nmed, Lx,
-                                Ly, Nx, Ny, KLT_TCC_SPLIT_F16, tcc));
-  // 2. mask bands (matrix-core band pass), then the Hopkins images from them
+  // 1. mask bands (matrix-core band pass)
   KLT_TRY(ctx->qpart.ensure((size_t)nmask * nblk * Wy * 16 * sizeof(float2)));
   KLT_TRY(ctx->gpart.ensure((size_t)nmask * nblk * Gy * 32 * sizeof(float2)));
   KLT_TRY(ctx->lpart.ensure((size_t)nmask * nblk * sizeof(float)));
@@ -136,6 +137,19 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
   KLT_TRY(klt::band_mask_partials(ctx, Lx, Ly, Nx, Ny, dm, nmask, ks, ctx->qpart.as<float2>()));
   KLT_LAUNCH(ctx, "k_abbe_assemble", k_abbe_assemble, dim3(nmask), dim3(kThreads), 0,
              ctx->qpart.as<float2>(), nblk, Nx, Ny, inv_area, Mfull);
+  // 2. TCC of the source, then the Hopkins images from the bands
+  if (src_read) {
+    const hipError_t e = hipEventSynchronize(src_read);
+    ctx->event_pool.push_back(src_read);
+    KLT_HIP(e);
+    std::memcpy(hsrc.data(), src_pinned, ns2 * sizeof(float));
+  }
+  float sigma = klt_outer_sigma(hsrc.data(), srcSize);
+  if (sigma > 1.0f) sigma = 1.0f;
+  KLT_TRY(ctx->abbe_tcc.ensure((size_t)T * T * 16));
+  double* tcc = ctx->abbe_tcc.as<double>();
+  KLT_TRY(klt::tcc_build_device(ctx, hsrc.data(), srcSize, sigma, lambda, NA, defocus, nmed, Lx,
+                                Ly, Nx, Ny, KLT_TCC_SPLIT_F16, tcc));
   KLT_TRY(klt::hopkins_images_fused(ctx, Mfull, nmask, Lx, Ly, tcc, Nx, Ny, I));
   // 3. resist bands and loss partials
   KLT_TRY(klt::band_resist_partials(ctx, Lx, Ly, Nx, Ny, I, dt, nmask, ks, a_sig, thr,

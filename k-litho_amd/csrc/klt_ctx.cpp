@@ -143,6 +143,19 @@ int klt_ctx::upload(void* dev, const void* host, size_t bytes) {
   return KLT_OK;
 }
 
+int klt_ctx::download_area(size_t bytes, void** out) {
+  if (bytes > dl_cap) {  // (its previous reads have all been waited for)
+    if (dl_base) KLT_HIP(hipHostFree(dl_base));
+    dl_base = nullptr;
+    dl_cap = 0;
+    const size_t cap = std::max<size_t>(bytes, 64 << 10);
+    KLT_HIP(hipHostMalloc(&dl_base, cap, hipHostMallocDefault));
+    dl_cap = cap;
+  }
+  *out = dl_base;
+  return KLT_OK;
+}
+
 hipEvent_t klt_ctx::take_event() {
   if (!event_pool.empty()) {
     hipEvent_t e = event_pool.back();
@@ -208,6 +221,7 @@ void klt_ctx_destroy(klt_ctx* ctx) {
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
   if (ctx->up_last) (void)hipEventDestroy(ctx->up_last);
   if (ctx->up_base) (void)hipHostFree(ctx->up_base);
+  if (ctx->dl_base) (void)hipHostFree(ctx->dl_base);
   ctx->socs_plans.clear();
   ctx->fused_tables.clear();
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);

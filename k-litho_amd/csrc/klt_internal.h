@@ -137,6 +137,11 @@ struct klt_ctx {
   size_t up_cap = 0, up_used = 0;
   hipEvent_t up_last = nullptr;
   int upload(void* dev, const void* host, size_t bytes);
+  // pinned host area for small device-to-host reads that the host waits for
+  // with an event (so that kernels queued after the copy run meanwhile)
+  void* dl_base = nullptr;
+  size_t dl_cap = 0;
+  int download_area(size_t bytes, void** out);
   // scratch reused across calls (grown on demand)
   klt::DevBuf stage_in, stage_out, qpart, band, spec, ccols, misc, eig_ws;
   klt::DevBuf gpart, gspec, lpart, bwd_img;  // adjoint scratch
