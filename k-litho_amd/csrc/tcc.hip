@@ -438,10 +438,13 @@ int klt_tcc_build(klt_ctx* ctx, const float* src, int srcSize, float outerSigma,
   std::vector<float> hsrc((size_t)srcSize * srcSize);
   if (flags & KLT_HOST_PTRS) {
     std::memcpy(hsrc.data(), src, sizeof(float) * hsrc.size());
-  } else {
-    KLT_HIP(hipMemcpyAsync(hsrc.data(), src, sizeof(float) * hsrc.size(), hipMemcpyDeviceToHost,
+  } else {  // through pinned memory (a pageable read is staged and slower)
+    void* pinned = nullptr;
+    KLT_TRY(ctx->download_area(sizeof(float) * hsrc.size(), &pinned));
+    KLT_HIP(hipMemcpyAsync(pinned, src, sizeof(float) * hsrc.size(), hipMemcpyDeviceToHost,
                            ctx->stream));
     KLT_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(hsrc.data(), pinned, sizeof(float) * hsrc.size());
   }
   const int T = (2 * Nx + 1) * (2 * Ny + 1);
   const size_t bytes = sizeof(double) * 2 * (size_t)T * T;
