@@ -21,7 +21,8 @@
 //      partially pivoted LU of T_r - lambda I (one thread per vector);
 //      eigenvalues closer than 1e-4 ||T|| form a cluster whose vectors are
 //      re-orthogonalised in eigenvalue order at every iteration (Gram-Schmidt
-//      twice, parallel over the vector length), one block per cluster.
+//      parallel over the vector length, twice in the last of three rounds),
+//      one block per cluster.
 //   4. k_back_transform: x = H_0 ... H_{n-3} D z, one block per vector (D by a
 //      block scan of the unit phases).
 // Textbook algorithms (Golub & Van Loan, ch. 8; Wilkinson's inverse
@@ -918,8 +919,9 @@ __device__ void tri_solve(const double* __restrict__ lu, int n, double* __restri
 // eigenvalues (cl_begin[c] .. cl_begin[c+1]).  Thread t of the block factors
 // T_r - lam_q I and solves with it for vector q = first + t; after every
 // round of solves the block re-orthogonalises the cluster's vectors in
-// eigenvalue order (Gram-Schmidt applied twice, against kGs earlier vectors
-// at a time, parallel over the vector length) and normalises them.
+// eigenvalue order (Gram-Schmidt against kGs earlier vectors at a time,
+// parallel over the vector length; applied twice in the last round) and
+// normalises them.
 // Equal eigenvalues of a cluster are separated slightly in the shifts.
 // Three rounds: a shift within 10 eps ||T|| of lambda_q amplifies the wanted
 // component over any eigenvalue at least 1e-4 ||T|| away (every eigenvalue
@@ -966,7 +968,10 @@ __global__ __launch_bounds__(256) void k_inverse_iteration(
     __syncthreads();
     for (int q = first; q < last; ++q) {
       double* x = z + (size_t)q * n;
-      for (int pass = 0; pass < 2; ++pass)
+      // Gram-Schmidt once in the early rounds (the next solve amplifies the
+      // wanted directions again), twice in the last
+      const int passes = it + 1 == kInvIters ? 2 : 1;
+      for (int pass = 0; pass < passes; ++pass)
         for (int r0 = first; r0 < q; r0 += kGs) {
           const int nr = min(kGs, q - r0);
           double acc[kGs];
