@@ -915,6 +915,74 @@ __device__ void tri_solve(const double* __restrict__ lu, int n, double* __restri
   }
 }
 
+// Re-orthogonalises vector q of a cluster against the cluster's earlier
+// vectors (rows first .. q-1 of z; Gram-Schmidt `passes` times, kGs rows a
+// round) and normalises it, for n <= 256 C: the block keeps x and each
+// round's kGs rows in registers (C entries a thread), so that x is read and
+// written once and every round's loads are issued together (clamped,
+// unconditional loads: a load in a branch would be waited for there).
+template <int C>
+__device__ void orthonormalise_reg(double* __restrict__ x, const double* __restrict__ z, int n,
+                                   int first, int q, int passes, double (*red)[33],
+                                   double* dots) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double xr[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const int i = tid + 256 * c;
+    const double v = x[min(i, n - 1)];
+    xr[c] = i < n ? v : 0.0;
+  }
+  for (int pass = 0; pass < passes; ++pass)
+    for (int r0 = first; r0 < q; r0 += kGs) {
+      const int nr = min(kGs, q - r0);
+      double zr[kGs][C];
+#pragma unroll
+      for (int j = 0; j < kGs; ++j)
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const int i = tid + 256 * c;
+          const double v = z[(size_t)(r0 + min(j, nr - 1)) * n + min(i, n - 1)];
+          zr[j][c] = (j < nr && i < n) ? v : 0.0;
+        }
+      double acc[kGs];
+#pragma unroll
+      for (int j = 0; j < kGs; ++j) {
+        acc[j] = 0.0;
+#pragma unroll
+        for (int c = 0; c < C; ++c) acc[j] += zr[j][c] * xr[c];
+      }
+#pragma unroll
+      for (int j = 0; j < kGs; ++j) {
+        const double v = wave_sum(acc[j]);
+        if (lane == 0) red[wv][j] = v;
+      }
+      __syncthreads();
+      if (tid < nr) dots[tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < kGs; ++j) {
+        const double dj = dots[j];
+#pragma unroll
+        for (int c = 0; c < C; ++c) xr[c] -= dj * zr[j][c];  // (zr is 0 for j >= nr)
+      }
+      __syncthreads();
+    }
+  double s2 = 0.0;
+#pragma unroll
+  for (int c = 0; c < C; ++c) s2 += xr[c] * xr[c];
+  s2 = wave_sum(s2);
+  if (lane == 0) red[wv][32] = s2;
+  __syncthreads();
+  const double inv = 1.0 / sqrt(red[0][32] + red[1][32] + red[2][32] + red[3][32]);
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const int i = tid + 256 * c;
+    if (i < n) x[i] = xr[c] * inv;
+  }
+  __syncthreads();
+}
+
 // Eigenvectors of T_r by inverse iteration, one block per cluster of close
 // eigenvalues (cl_begin[c] .. cl_begin[c+1]).  Thread t of the block factors
 // T_r - lam_q I and solves with it for vector q = first + t; after every
@@ -971,6 +1039,14 @@ __global__ __launch_bounds__(256) void k_inverse_iteration(
       // Gram-Schmidt once in the early rounds (the next solve amplifies the
       // wanted directions again), twice in the last
       const int passes = it + 1 == kInvIters ? 2 : 1;
+      if (n <= 256 * 5) {
+        orthonormalise_reg<5>(x, z, n, first, q, passes, red, dots);
+        continue;
+      }
+      if (n <= 256 * 8) {
+        orthonormalise_reg<8>(x, z, n, first, q, passes, red, dots);
+        continue;
+      }
       for (int pass = 0; pass < passes; ++pass)
         for (int r0 = first; r0 < q; r0 += kGs) {
           const int nr = min(kGs, q - r0);
