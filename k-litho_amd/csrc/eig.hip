@@ -776,6 +776,7 @@ __global__ __launch_bounds__(128) void k_bisect(const double* __restrict__ d,
 constexpr int kSweep = 16;
 constexpr int kGs = 8;  // earlier vectors per Gram-Schmidt round (registers)
 constexpr int kInvIters = 3;  // inverse iteration rounds
+constexpr int kPipe = 8;      // steps a chunk of the pipelined solves
 
 __device__ void tri_lu(const double* __restrict__ d, const double* __restrict__ e, int n,
                        double lam, double tnorm, double* __restrict__ lu) {
@@ -846,19 +847,27 @@ __device__ void tri_solve(const double* __restrict__ lu, int n, double* __restri
   const double* lu_u2 = lu + 2 * n;
   const double* lu_l = lu + 3 * n;
   const double* piv = lu + 4 * n;
+  // Both sweeps run in chunks of kPipe steps, the loads of chunk c + 1 issued
+  // before chunk c's recurrence (two register sets, used alternately); a
+  // chunk's loads never touch the entries that the chunk before it writes.
+  // Past the last chunk the loads repeat it (in bounds, unused).
   // L (with the row swaps): forward
   double xi = x[0];
-  int i = 0;
-  for (; i + kSweep <= n - 1; i += kSweep) {
-    double pv[kSweep], lv[kSweep], xn[kSweep], xo[kSweep];
+  const int nf = (n - 1) / kPipe;  // full chunks: steps c kPipe .. + kPipe - 1
+  auto load_f = [&](int c, double(&pv)[kPipe], double(&lv)[kPipe], double(&xn)[kPipe]) {
+    const int b = min(c, nf - 1) * kPipe;
 #pragma unroll
-    for (int u = 0; u < kSweep; ++u) {
-      pv[u] = piv[i + u];
-      lv[u] = lu_l[i + u];
-      xn[u] = x[i + 1 + u];
+    for (int u = 0; u < kPipe; ++u) {
+      pv[u] = piv[b + u];
+      lv[u] = lu_l[b + u];
+      xn[u] = x[b + 1 + u];
     }
+  };
+  auto sweep_f = [&](int c, const double(&pv)[kPipe], const double(&lv)[kPipe],
+                     const double(&xn)[kPipe]) {
+    double xo[kPipe];
 #pragma unroll
-    for (int u = 0; u < kSweep; ++u) {
+    for (int u = 0; u < kPipe; ++u) {
       if (pv[u] != 0.0) {
         xo[u] = xn[u];
         xi = xi - lv[u] * xn[u];
@@ -868,9 +877,21 @@ __device__ void tri_solve(const double* __restrict__ lu, int n, double* __restri
       }
     }
 #pragma unroll
-    for (int u = 0; u < kSweep; ++u) x[i + u] = xo[u];
+    for (int u = 0; u < kPipe; ++u) x[c * kPipe + u] = xo[u];
+  };
+  {
+    double pa[kPipe], la[kPipe], xa[kPipe], pb[kPipe], lb[kPipe], xb[kPipe];
+    if (nf > 0) load_f(0, pa, la, xa);
+    int c = 0;
+    for (; c + 1 < nf; c += 2) {
+      load_f(c + 1, pb, lb, xb);
+      sweep_f(c, pa, la, xa);
+      load_f(c + 2, pa, la, xa);
+      sweep_f(c + 1, pb, lb, xb);
+    }
+    if (c < nf) sweep_f(c, pa, la, xa);
   }
-  for (; i < n - 1; ++i) {
+  for (int i = nf * kPipe; i < n - 1; ++i) {
     const double xn = x[i + 1];
     if (piv[i] != 0.0) {
       x[i] = xn;
@@ -887,27 +908,45 @@ __device__ void tri_solve(const double* __restrict__ lu, int n, double* __restri
   if (n < 2) return;
   double x0 = (x[n - 2] - lu_u1[n - 2] * x1) * lu_rd[n - 2];
   x[n - 2] = x0;
-  i = n - 3;
-  for (; i - (kSweep - 1) >= 0; i -= kSweep) {
-    double xv[kSweep], u1v[kSweep], u2v[kSweep], dv[kSweep];
+  const int nb = (n - 2) / kPipe;  // full chunks: steps n-3 - c kPipe - u, u < kPipe
+  auto load_b = [&](int c, double(&xv)[kPipe], double(&u1v)[kPipe], double(&u2v)[kPipe],
+                    double(&dv)[kPipe]) {
+    const int b = n - 3 - min(c, nb - 1) * kPipe;
 #pragma unroll
-    for (int u = 0; u < kSweep; ++u) {
-      xv[u] = x[i - u];
-      u1v[u] = lu_u1[i - u];
-      u2v[u] = lu_u2[i - u];
-      dv[u] = lu_rd[i - u];
+    for (int u = 0; u < kPipe; ++u) {
+      xv[u] = x[b - u];
+      u1v[u] = lu_u1[b - u];
+      u2v[u] = lu_u2[b - u];
+      dv[u] = lu_rd[b - u];
     }
+  };
+  auto sweep_b = [&](int c, double(&xv)[kPipe], const double(&u1v)[kPipe],
+                     const double(&u2v)[kPipe], const double(&dv)[kPipe]) {
 #pragma unroll
-    for (int u = 0; u < kSweep; ++u) {
+    for (int u = 0; u < kPipe; ++u) {
       const double v = (xv[u] - u1v[u] * x0 - u2v[u] * x1) * dv[u];
       xv[u] = v;
       x1 = x0;
       x0 = v;
     }
+    const int b = n - 3 - c * kPipe;
 #pragma unroll
-    for (int u = 0; u < kSweep; ++u) x[i - u] = xv[u];
+    for (int u = 0; u < kPipe; ++u) x[b - u] = xv[u];
+  };
+  {
+    double xa[kPipe], u1a[kPipe], u2a[kPipe], da[kPipe];
+    double xb[kPipe], u1b[kPipe], u2b[kPipe], db[kPipe];
+    if (nb > 0) load_b(0, xa, u1a, u2a, da);
+    int c = 0;
+    for (; c + 1 < nb; c += 2) {
+      load_b(c + 1, xb, u1b, u2b, db);
+      sweep_b(c, xa, u1a, u2a, da);
+      load_b(c + 2, xa, u1a, u2a, da);
+      sweep_b(c + 1, xb, u1b, u2b, db);
+    }
+    if (c < nb) sweep_b(c, xa, u1a, u2a, da);
   }
-  for (; i >= 0; --i) {
+  for (int i = n - 3 - nb * kPipe; i >= 0; --i) {
     const double v = (x[i] - lu_u1[i] * x0 - lu_u2[i] * x1) * lu_rd[i];
     x[i] = v;
     x1 = x0;
