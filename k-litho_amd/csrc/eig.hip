@@ -780,7 +780,7 @@ __global__ __launch_bounds__(128) void k_bisect(const double* __restrict__ d,
 // LU of T_r - lam I with partial pivoting between rows i and i+1, into the
 // 5n doubles at lu: reciprocals of U's diagonal (so that the back substitution
 // multiplies: a division on its serial chain costs about ten dependent FP64
-// operations), U's first and second superdiagonals, multipliers, and the swap
+// operations; the LU forms them with recip, which its multipliers share), U's first and second superdiagonals, multipliers, and the swap
 // flags.  The sweeps below are serial in i; each runs in chunks
 // of kSweep steps whose loads are issued together ahead of the chunk's
 // recurrence, so that the memory latency is paid once per chunk instead of
@@ -804,8 +804,9 @@ __device__ void tri_lu(const double* __restrict__ d, const double* __restrict__ 
                   double& ol, double& op) {
     if (fabs(a) >= fabs(cdn)) {
       if (a == 0.0) a = eps * tnorm;
-      const double l = cdn / a;
-      od = 1.0 / a;
+      const double r = recip(a);
+      const double l = cdn * r;
+      od = r;
       ou1 = b;
       ou2 = 0.0;
       ol = l;
@@ -813,8 +814,9 @@ __device__ void tri_lu(const double* __restrict__ d, const double* __restrict__ 
       a = dn - l * b;
       b = en;
     } else {
-      const double l = a / cdn;
-      od = 1.0 / cdn;
+      const double r = recip(cdn);
+      const double l = a * r;
+      od = r;
       ou1 = dn;
       ou2 = en;
       ol = l;
@@ -849,7 +851,7 @@ __device__ void tri_lu(const double* __restrict__ d, const double* __restrict__ 
     step(cdn, dn, en, lu_rd[i], lu_u1[i], lu_u2[i], lu_l[i], piv[i]);
   }
   if (a == 0.0) a = eps * tnorm;
-  lu_rd[n - 1] = 1.0 / a;
+  lu_rd[n - 1] = recip(a);
 }
 
 // x <- (T_r - lam I)^{-1} x with the factors of tri_lu.
