@@ -172,13 +172,7 @@ def bench_batch_uneven(ctx, timer, torch, src):
 def blurred_dipole():
     """101 x 101 binary dipole (sigma_c 0.7, radius 0.2, on x) blurred by a
     Gaussian of 1.5 cells (truncated at 4 sigma), normalised."""
-    d = klt.source_create(101, "Dipole", radius=0.2, offset=0.7, on_x_axis=1)
-    r = np.arange(-6, 7)
-    k = np.exp(-0.5 * (r / 1.5) ** 2)
-    k /= k.sum()
-    b = np.apply_along_axis(lambda v: np.convolve(v, k, mode="same"), 0, d.astype(np.float64))
-    b = np.apply_along_axis(lambda v: np.convolve(v, k, mode="same"), 1, b)
-    return np.ascontiguousarray((b / b.sum()).astype(np.float32))
+    return W.blurred(klt.source_create(101, "Dipole", radius=0.2, offset=0.7, on_x_axis=1))
 
 
 def bench_tcc_eig(ctx, timer, torch):

@@ -48,7 +48,11 @@ int klt_device_count(void);
 int klt_ctx_create(int device, klt_ctx** out);
 void klt_ctx_destroy(klt_ctx* ctx);
 /* Use `stream` (a hipStream_t) for all work of this context; NULL restores
- * the context's own stream.  The caller keeps ownership. */
+ * the context's own stream.  The caller keeps ownership.  Work already queued
+ * by the context on the previous stream comes first: the new stream waits for
+ * it (an event, no host wait), since both use the context's buffers.  The
+ * previous stream must still exist; if it is being captured, no such wait is
+ * added (the capture orders its own work). */
 int klt_ctx_set_stream(klt_ctx* ctx, void* stream);
 void* klt_ctx_stream(klt_ctx* ctx);
 int klt_ctx_sync(klt_ctx* ctx);
@@ -193,9 +197,13 @@ int klt_socs_forward(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
  * computed once and shared by its sets.  The call's launches are captured in
  * a hipGraph on first use and replayed when the same arguments (pointers,
  * sizes, sets) come again, so a caller that refills the same buffers pays one
- * graph launch per batch; a stream the caller is itself capturing receives
- * the launches directly.  Device pointers only (the pointer arrays are host
- * memory). */
+ * graph launch per batch.  A stream the caller is itself capturing receives
+ * the launches directly, for a batch whose arguments have been seen before
+ * (the first call of a batch uploads its job tables with synchronous copies,
+ * which a capture does not allow: it returns KLT_ERR_ARG under capture); the
+ * library then keeps that batch's buffers until the context is destroyed, as
+ * the caller's graph refers to them.  Device pointers only (the pointer
+ * arrays are host memory). */
 int klt_socs_forward_batch(klt_ctx* ctx, int nmask, const float* const* masks, const int* Lx,
                            const int* Ly, int nbank, const klt_kernels* const* banks,
                            float* const* images, unsigned flags);

@@ -799,11 +799,17 @@ __device__ void tri_lu(const double* __restrict__ d, const double* __restrict__ 
   double* lu_l = lu + 3 * n;
   double* piv = lu + 4 * n;
   double a = d[0] - lam, b = n > 1 ? e[0] : 0.0;
+  // Pivots smaller than eps ||T|| are replaced by +-eps ||T|| (LAPACK's
+  // dlagtf / dstein perturb them the same way); a zero tridiagonal (||T|| = 0)
+  // takes unit pivots, so that the solves leave the start vector unchanged.
+  // The floor also keeps subnormal pivots away from the reciprocal estimate,
+  // which is not accurate there.
+  const double pfloor = tnorm > 0.0 ? eps * tnorm : 1.0;
   // one elimination step: row i (a, b, 0) against row i + 1 (cdn, dn, en)
   auto step = [&](double cdn, double dn, double en, double& od, double& ou1, double& ou2,
                   double& ol, double& op) {
-    if (fabs(a) >= fabs(cdn)) {
-      if (a == 0.0) a = eps * tnorm;
+    if (fabs(a) >= fabs(cdn) || fabs(cdn) < pfloor) {
+      if (fabs(a) < pfloor) a = copysign(pfloor, a);
       const double r = recip(a);
       const double l = cdn * r;
       od = r;
@@ -850,7 +856,7 @@ __device__ void tri_lu(const double* __restrict__ d, const double* __restrict__ 
     const double cdn = e[i], dn = d[i + 1] - lam, en = i + 1 < n - 1 ? e[i + 1] : 0.0;
     step(cdn, dn, en, lu_rd[i], lu_u1[i], lu_u2[i], lu_l[i], piv[i]);
   }
-  if (a == 0.0) a = eps * tnorm;
+  if (fabs(a) < pfloor) a = copysign(pfloor, a);
   lu_rd[n - 1] = recip(a);
 }
 

@@ -230,7 +230,23 @@ void klt_ctx_destroy(klt_ctx* ctx) {
 
 int klt_ctx_set_stream(klt_ctx* ctx, void* stream) {
   if (!ctx) return klt::set_error(KLT_ERR_ARG, "klt_ctx_set_stream: ctx is NULL");
-  ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
+  const hipStream_t next = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
+  if (next == ctx->stream) return KLT_OK;
+  KLT_TRY(ctx->activate());
+  // The context's scratch buffers, batch graphs and pinned upload area are
+  // shared by all its calls: work on the new stream must not start before
+  // what is queued on the old one (this also keeps the upload area's reuse
+  // rule, which waits for the last copy recorded on the newest stream).
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  KLT_HIP(hipStreamIsCapturing(ctx->stream, &cap));
+  if (cap == hipStreamCaptureStatusNone) {
+    hipEvent_t e = ctx->take_event();
+    const hipError_t r = hipEventRecord(e, ctx->stream);
+    const hipError_t w = r == hipSuccess ? hipStreamWaitEvent(next, e, 0) : r;
+    ctx->event_pool.push_back(e);  // (a wait holds the event's state at the call)
+    KLT_HIP(w);
+  }
+  ctx->stream = next;
   return KLT_OK;
 }
 

@@ -71,6 +71,28 @@ struct FusedTables {
   DevBuf Wgf, twyg;
 };
 
+// Device pointers of one fused forward launch (socs_fused_launch): the tables
+// (layouts of FusedTables / SocsPlan: Wf fragment-ordered row twiddles with
+// Kpad columns, twy the [Ly][ldtw] column twiddles, TYh / TYl [Lypad][64], Ef
+// the synthesis column fragments, tw64 e^{+2 pi i j/64}) and the intermediates
+// of nb images: qpart [nb][nblk * ks][2Ny+1][16], spec [nb][32][4Nx+1], and
+// for kg > 1 ipart [nb][kg][64 * 64] and gsum [nb][64 * 64].
+struct FusedView {
+  int Kpad, Lypad, ldtw;
+  const float4* Wf;
+  const float2* twy;
+  const _Float16* TYh;
+  const _Float16* TYl;
+  const float4* Ef;
+  const float2* tw64;
+};
+struct FusedScratch {
+  float2* qpart;
+  float2* spec;
+  float* ipart;
+  float* gsum;
+};
+
 // Work descriptors of the mixed-size batch (klt_socs_forward_batch): one per
 // block of each stage, so that one launch covers masks of different sizes
 // and several kernel sets.
@@ -100,11 +122,15 @@ struct SynthJob {       // one 128-column strip of one image
 };
 
 // One captured mixed batch: its job tables, intermediates and the graph that
-// replays its three stages; `key` is the call's arguments.
+// replays its three stages; `key` is the call's arguments.  `held`: its
+// launches went into a graph that the caller was capturing, which keeps
+// pointers to these buffers, so the entry stays until the context is
+// destroyed instead of taking part in the cache's eviction.
 struct BatchGraph {
   std::vector<uint64_t> key;
   int nimg = 0, nband = 0, nmid = 0, nsynth = 0;
   bool full = true;
+  bool held = false;
   DevBuf band_jobs, mid_jobs, synth_jobs, mpart, spec, strips_done, done_at, t0;
   const float2* spec_tw = nullptr;  // e^{+2 pi i j/64} of the coarse FFTs
   hipGraph_t graph = nullptr;
@@ -213,6 +239,16 @@ int socs_mid_from_band(klt_ctx* ctx, const klt_kernels* kb, const float2* band, 
                        float2* spec, int G);
 int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int B, int Lx,
                        int Ly, float* images);
+// The launches of one fused forward pass over nb masks (kernel half-size
+// <= 15) with explicit tables and intermediates, split factors ks (band column
+// ranges, <= 8), rs (synthesis row ranges, <= 16) and kg (kernel groups).
+int socs_fused_launch(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int nb, int Lx,
+                      int Ly, float* images, const FusedView& v, const FusedScratch& w,
+                      int ks, int rs, int kg);
+// Intermediate bytes per image of socs_fused_launch, and the fixed split rule
+// (two blocks per CU) for nb images.
+size_t fused_scratch_bytes(int Lx, int Ly, int Nx, int Ny, int ks, int kg);
+void fused_fixed_splits(int nb, int Lx, int Ly, int nk, int* ks, int* rs, int* kg);
 // Band partial sums of the fused passes (socs_fused.hip), nblk = ceil(Ly/128)
 // row blocks split into ks column ranges (band_split: the fixed rule of the
 // batched passes, parts = nblk * ks):

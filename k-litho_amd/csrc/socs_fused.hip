@@ -1464,11 +1464,10 @@ namespace klt {
 template <int NCB, bool RESIST>
 static void launch_band(klt_ctx* ctx, const char* name, int nblk, int ks, int B,
                         const float* in0, const float* in1, float a_sig, float thr, int Lx,
-                        int Ly, int Kpad, const DevBuf& Wf, const float2* twy, int ldtw, int Bx,
+                        int Ly, int Kpad, const float4* w, const float2* twy, int ldtw, int Bx,
                         int By, int ldo, float2* out, float* lpart) {
   const bool full = Lx % 32 == 0 && (reinterpret_cast<uintptr_t>(in0) & 15) == 0 &&
                     (!RESIST || (reinterpret_cast<uintptr_t>(in1) & 15) == 0);
-  const float4* w = Wf.as<float4>();
   const int parts = nblk * ks;
   const dim3 grid(parts * B);
   if (full)
@@ -1627,10 +1626,8 @@ static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, in
     return KLT_OK;
   }
   klt_ctx::Tuned h{1, 1, 1, 0.f};
-  while (h.ks < 4 && nblk * h.ks * nb < 512 && split_ok(nchunk, 2 * h.ks)) h.ks *= 2;
-  while (h.rs < 4 && nstrips * h.rs * nb < 512 && split_ok(nrow32, 2 * h.rs)) h.rs *= 2;
+  fused_fixed_splits(nb, Lx, Ly, nk, &h.ks, &h.rs, &h.kg);
   auto kg_ok = [&](int v) { return v <= nk && (v == 1 || v * nb <= kGroupSlots); };
-  while (h.kg < 8 && h.kg * nb < 256 && kg_ok(2 * h.kg)) h.kg *= 2;
   if (ctx->pin_ks || ctx->pin_rs || ctx->pin_kg) {
     auto fit = [](int v, int hi, const std::function<bool(int)>& ok) {
       v = std::min(v, hi);
@@ -1698,6 +1695,52 @@ static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, in
   return KLT_OK;
 }
 
+int socs_fused_launch(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int nb, int Lx,
+                      int Ly, float* images, const FusedView& v, const FusedScratch& w,
+                      int ks, int rs, int kg) {
+  const int Nx = kb->Nx, Ny = kb->Ny;
+  const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
+  const int nstrips = (Lx + kColsSynth - 1) / kColsSynth;
+  const float inv_area = 1.0f / ((float)Lx * (float)Ly);
+  launch_band<2, false>(ctx, "k_band_mfma", nblk, ks, nb, masks, nullptr, 0.f, 0.f, Lx, Ly,
+                        v.Kpad, v.Wf, v.twy, v.ldtw, Nx, Ny, 16, w.qpart, nullptr);
+  KLT_LAUNCH(ctx, "k_socs_mid", k_socs_mid, dim3(kg, nb), dim3(256), 0, w.qpart, nblk * ks, Nx,
+             Ny, inv_area, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk, v.tw64,
+             w.spec, w.ipart);
+  if (kg > 1) {
+    KLT_LAUNCH(ctx, "k_sum_groups", k_sum_groups, dim3(kM * kM / 256, nb), dim3(256), 0, w.ipart,
+               kg, kM * kM, w.gsum);
+    KLT_LAUNCH(ctx, "k_socs_mid_fin", k_socs_mid_fin, dim3(nb), dim3(256), 0, w.gsum, 1, Nx, Ny,
+               v.tw64, w.spec);
+  }
+  KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma, dim3(nstrips * rs, nb), dim3(256), 0, w.spec, Nx,
+             Lx, Ly, v.Lypad, rs, v.Ef, v.TYh, v.TYl, images);
+  KLT_HIP(hipGetLastError());
+  return KLT_OK;
+}
+
+size_t fused_scratch_bytes(int Lx, int Ly, int Nx, int Ny, int ks, int kg) {
+  const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
+  size_t b = (size_t)nblk * ks * (2 * Ny + 1) * 16 * sizeof(float2);  // qpart
+  b += (size_t)32 * (4 * Nx + 1) * sizeof(float2);                     // spec
+  if (kg > 1) b += (size_t)(kg + 1) * kM * kM * sizeof(float);         // ipart, gsum
+  (void)Lx;
+  return b;
+}
+
+void fused_fixed_splits(int nb, int Lx, int Ly, int nk, int* ks, int* rs, int* kg) {
+  const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
+  const int nchunk = round_up(Lx, 32 * kBandChunk) / (32 * kBandChunk);
+  const int nstrips = (Lx + kColsSynth - 1) / kColsSynth, nrow32 = round_up(Ly, 32) / 32;
+  int k = 1, r = 1, g = 1;
+  while (k < 4 && nblk * k * nb < 512 && split_ok(nchunk, 2 * k)) k *= 2;
+  while (r < 4 && nstrips * r * nb < 512 && split_ok(nrow32, 2 * r)) r *= 2;
+  while (g < 8 && g * nb < 256 && 2 * g <= nk && 2 * g * nb <= kGroupSlots) g *= 2;
+  *ks = k;
+  *rs = r;
+  *kg = g;
+}
+
 int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int B, int Lx,
                        int Ly, float* images) {
   SocsPlan* p = nullptr;
@@ -1712,31 +1755,17 @@ int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, 
   KLT_TRY(ctx->spec.ensure((size_t)SB * 32 * Gx * sizeof(float2)));
   KLT_TRY(ctx->ipart.ensure((size_t)kGroupSlots * kM * kM * sizeof(float)));
   KLT_TRY(ctx->gsum.ensure((size_t)SB * kM * kM * sizeof(float)));
-  const float inv_area = 1.0f / ((float)Lx * (float)Ly);
+  const FusedView view{ft->Kpad, ft->Lypad, p->NP, ft->Wf.as<float4>(), p->twy_in.as<float2>(),
+                       ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), ft->Ef.as<float4>(),
+                       ft->tw64.as<float2>()};
+  const FusedScratch work{ctx->qpart.as<float2>(), ctx->spec.as<float2>(),
+                          ctx->ipart.as<float>(), ctx->gsum.as<float>()};
   for (int b0 = 0; b0 < B; b0 += SB) {
     const int nb = std::min(SB, B - b0);
     const float* m = masks + (size_t)b0 * Lx * Ly;
     float* out = images + (size_t)b0 * Lx * Ly;
     auto run = [&](const klt_ctx::Tuned& c) -> int {
-      launch_band<2, false>(ctx, "k_band_mfma", nblk, c.ks, nb, m, nullptr, 0.f, 0.f, Lx, Ly,
-                            ft->Kpad, ft->Wf, p->twy_in.as<float2>(), p->NP, Nx, Ny, 16,
-                            ctx->qpart.as<float2>(), nullptr);
-      KLT_LAUNCH(ctx, "k_socs_mid", k_socs_mid, dim3(c.kg, nb), dim3(256), 0,
-                 ctx->qpart.as<float2>(), nblk * c.ks, Nx, Ny, inv_area, kb->kern.as<float2>(),
-                 kb->scales.as<float>(), kb->nk, ft->tw64.as<float2>(), ctx->spec.as<float2>(),
-                 ctx->ipart.as<float>());
-      if (c.kg > 1) {
-        KLT_LAUNCH(ctx, "k_sum_groups", k_sum_groups, dim3(kM * kM / 256, nb), dim3(256), 0,
-                   ctx->ipart.as<float>(), c.kg, kM * kM, ctx->gsum.as<float>());
-        KLT_LAUNCH(ctx, "k_socs_mid_fin", k_socs_mid_fin, dim3(nb), dim3(256), 0,
-                   ctx->gsum.as<float>(), 1, Nx, Ny, ft->tw64.as<float2>(),
-                   ctx->spec.as<float2>());
-      }
-      KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma, dim3(nstrips * c.rs, nb), dim3(256), 0,
-                 ctx->spec.as<float2>(), Nx, Lx, Ly, ft->Lypad, c.rs, ft->Ef.as<float4>(),
-                 ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), out);
-      KLT_HIP(hipGetLastError());
-      return KLT_OK;
+      return socs_fused_launch(ctx, kb, m, nb, Lx, Ly, out, view, work, c.ks, c.rs, c.kg);
     };
     klt_ctx::Tuned c{};
     KLT_TRY(pick_split(ctx, 0, nb, Lx, Ly, Nx, Ny, kb->nk, ft->Kpad / (kBandChunk * 32), nblk,
@@ -1850,9 +1879,12 @@ int socs_forward_batch(klt_ctx* ctx, int nmask, const float* const* masks, const
   }
   for (int i = 0; i < nmask * nbank; ++i) key.push_back(reinterpret_cast<uintptr_t>(images[i]));
   // A stream that the caller is capturing records the launches into the
-  // caller's graph directly.
+  // caller's graph directly; that graph keeps pointers to the entry's
+  // buffers, so the entry is then held (never evicted) until the context is
+  // destroyed.
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   KLT_HIP(hipStreamIsCapturing(ctx->stream, &cap));
+  const bool capturing = cap != hipStreamCaptureStatusNone;
   for (size_t i = 0; i < ctx->batches.size(); ++i)
     if (ctx->batches[i]->key == key) {
       std::unique_ptr<BatchGraph> bg = std::move(ctx->batches[i]);
@@ -1860,7 +1892,11 @@ int socs_forward_batch(klt_ctx* ctx, int nmask, const float* const* masks, const
       ctx->batches.push_back(std::move(bg));
       BatchGraph* b = ctx->batches.back().get();
       ctx->last_batch = b;
-      if (cap != hipStreamCaptureStatusNone || !b->exec) {
+      if (capturing) {
+        b->held = true;
+        launch_batch(ctx, b);
+        KLT_HIP(hipGetLastError());
+      } else if (!b->exec) {
         launch_batch(ctx, b);
         KLT_HIP(hipGetLastError());
       } else {
@@ -1868,27 +1904,40 @@ int socs_forward_batch(klt_ctx* ctx, int nmask, const float* const* masks, const
       }
       return KLT_OK;
     }
+  // The job tables are uploaded with synchronous copies, which a stream
+  // capture does not allow: the first call of a batch comes before any
+  // capture of it.
+  if (capturing)
+    return set_error(KLT_ERR_ARG,
+                     "klt_socs_forward_batch: a batch's first call may not be captured "
+                     "(call it once outside the capture first)");
   auto bg = std::make_unique<BatchGraph>();
   bg->key = key;
   KLT_TRY(build_batch(ctx, bg.get(), nmask, masks, Lx, Ly, nbank, banks, images));
-  if (cap != hipStreamCaptureStatusNone) {
-    launch_batch(ctx, bg.get());
-    KLT_HIP(hipGetLastError());
-  } else {
-    KLT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    launch_batch(ctx, bg.get());
-    const hipError_t le = hipGetLastError();
-    const hipError_t ce = hipStreamEndCapture(ctx->stream, &bg->graph);
-    KLT_HIP(le);
-    KLT_HIP(ce);
-    KLT_HIP(hipGraphInstantiate(&bg->exec, bg->graph, nullptr, nullptr, 0));
-    KLT_HIP(hipGraphLaunch(bg->exec, ctx->stream));
-  }
+  KLT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  launch_batch(ctx, bg.get());
+  const hipError_t le = hipGetLastError();
+  const hipError_t ce = hipStreamEndCapture(ctx->stream, &bg->graph);
+  KLT_HIP(le);
+  KLT_HIP(ce);
+  KLT_HIP(hipGraphInstantiate(&bg->exec, bg->graph, nullptr, nullptr, 0));
+  KLT_HIP(hipGraphLaunch(bg->exec, ctx->stream));
   ctx->batches.push_back(std::move(bg));
-  if (ctx->batches.size() > 4) {
-    // the evicted batch's buffers may still be in use by its last launch
+  // keep at most 4 entries that are not held; an evicted entry's buffers may
+  // still be in use by its last launch
+  constexpr size_t kKeep = 4;
+  size_t unheld = 0;
+  for (auto& b : ctx->batches) unheld += b->held ? 0 : 1;
+  if (unheld > kKeep) {
     KLT_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->batches.erase(ctx->batches.begin());
+    for (size_t i = 0; i < ctx->batches.size() && unheld > kKeep;) {
+      if (!ctx->batches[i]->held) {
+        ctx->batches.erase(ctx->batches.begin() + i);
+        --unheld;
+      } else {
+        ++i;
+      }
+    }
   }
   ctx->last_batch = ctx->batches.back().get();
   return KLT_OK;
@@ -1989,7 +2038,7 @@ int band_mask_partials(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const float
   if (ks < 1 || !split_ok(ft->Kpad / (32 * kBandChunk), ks))
     return set_error(KLT_ERR_ARG, "band_mask_partials: column split does not fit");
   launch_band<2, false>(ctx, "k_band_mfma", nblk, ks, B, masks, nullptr, 0.f, 0.f, Lx, Ly,
-                        ft->Kpad, ft->Wf,
+                        ft->Kpad, ft->Wf.as<float4>(),
                         p->twy_in.as<float2>(), p->NP, Nx, Ny, 16, out, nullptr);
   KLT_HIP(hipGetLastError());
   return KLT_OK;
@@ -2006,7 +2055,7 @@ int band_resist_partials(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const flo
   if (ks < 1 || !split_ok(ft->Kpad / (32 * kBandChunk), ks))
     return set_error(KLT_ERR_ARG, "band_resist_partials: column split does not fit");
   launch_band<4, true>(ctx, "k_band_resist", nblk, ks, B, images, targets, a_sig, thr, Lx, Ly,
-                       ft->Kpad, ft->Wgf,
+                       ft->Kpad, ft->Wgf.as<float4>(),
                        ft->twyg.as<float2>(), 32, 2 * Nx, 2 * Ny, 32, out, lpart);
   KLT_HIP(hipGetLastError());
   return KLT_OK;
@@ -2046,7 +2095,7 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
     auto run = [&](const klt_ctx::Tuned& c) -> int {
       const int ks = c.ks, rs = c.rs, parts = nblk * ks;
       launch_band<2, false>(ctx, "k_band_mfma", nblk, ks, nb, masks + off, nullptr, 0.f, 0.f,
-                            Lx, Ly, ft->Kpad, ft->Wf, p->twy_in.as<float2>(), p->NP, Nx, Ny,
+                            Lx, Ly, ft->Kpad, ft->Wf.as<float4>(), p->twy_in.as<float2>(), p->NP, Nx, Ny,
                             16, ctx->qpart.as<float2>(), nullptr);
       // each image's M^ once, for the forward and the adjoint coarse stages
       KLT_LAUNCH(ctx, "k_band_assemble", k_band_assemble,
@@ -2066,7 +2115,7 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
                  ctx->spec.as<float2>(), Nx, Lx, Ly, ft->Lypad, rs, ft->Ef.as<float4>(),
                  ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), I);
       launch_band<4, true>(ctx, "k_band_resist", nblk, ks, nb, I, targets + off, a_sig, thr, Lx,
-                           Ly, ft->Kpad, ft->Wgf, ft->twyg.as<float2>(), 32, 2 * Nx, 2 * Ny, 32,
+                           Ly, ft->Kpad, ft->Wgf.as<float4>(), ft->twyg.as<float2>(), 32, 2 * Nx, 2 * Ny, 32,
                            ctx->gpart.as<float2>(), ctx->lpart.as<float>());
       // each image's G^, coarse-grid g_c and loss once, for all kernel groups
       KLT_LAUNCH(ctx, "k_resist_sum", k_resist_sum,

@@ -5,6 +5,10 @@
   sides uniform in 16..128 px.  Other sizes scale the side range by L/1024
   and the count range by (L/1024)^2 (minimum side 4 px, minimum count 4).
 * relaxed(mask, seed): clip(mask + U(-0.1, 0.1), 0, 1) (adjoint workload).
+* blurred(src): a source grid convolved with a Gaussian of 1.5 cells along
+  both axes (truncated at 4 sigma, zero outside the grid), normalised to sum 1:
+  applied to the 101 x 101 binary dipole (sigma_c 0.7, radius 0.2, on x) it is
+  the source of the T = 1225 TCC precision study.
 * Optics of the workloads: lambda 193 nm, NA 1.35, 51 x 51 annular source
   sigma 0.6 / 0.9, and the TCC periods that give the stated kernel supports
   (1088 nm -> 29 x 29 kernels, 1280 nm -> 35 x 35).
@@ -52,6 +56,16 @@ def relaxed(mask: np.ndarray, seed: int) -> np.ndarray:
     rng = np.random.Generator(np.random.PCG64(seed))
     noise = rng.uniform(-0.1, 0.1, size=mask.shape)
     return np.clip(mask + noise, 0.0, 1.0).astype(np.float32)
+
+
+def blurred(src: np.ndarray, sigma_cells: float = 1.5) -> np.ndarray:
+    r = np.arange(-int(4 * sigma_cells), int(4 * sigma_cells) + 1)
+    k = np.exp(-0.5 * (r / sigma_cells) ** 2)
+    k /= k.sum()
+    b = np.apply_along_axis(lambda v: np.convolve(v, k, mode="same"), 0,
+                            np.asarray(src, np.float64))
+    b = np.apply_along_axis(lambda v: np.convolve(v, k, mode="same"), 1, b)
+    return np.ascontiguousarray((b / b.sum()).astype(np.float32))
 
 
 def manhattan_batch(L: int, n: int, seed0: int = 1000) -> np.ndarray:

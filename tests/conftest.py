@@ -16,8 +16,13 @@ def pytest_configure(config):
 
 
 def relerr(a, b) -> float:
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
+    """Relative L2 (Frobenius) error of a against b.  Complex inputs keep their
+    imaginary parts (the difference is taken in complex128)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    dt = np.complex128 if (np.iscomplexobj(a) or np.iscomplexobj(b)) else np.float64
+    a = a.astype(dt)
+    b = b.astype(dt)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 

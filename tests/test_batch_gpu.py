@@ -109,3 +109,52 @@ def test_batch_errors(ctx):
     bp = (C.c_void_p * 1)(b6.handle.value)
     rc = klt.lib().klt_socs_forward_batch(ctx.handle, 1, mp, lx, lx, 1, bp, ip, klt.KLT_HOST_PTRS)
     assert rc == -1
+
+
+def test_batch_captured_by_caller_survives_later_batches(ctx):
+    """A batch launched into the caller's own graph capture (torch.cuda.graph)
+    keeps its buffers while later distinct batches fill and evict the
+    library's graph cache; each replay of the caller's graph, with the
+    program's eager refills between replays, equals the single calls."""
+    import torch
+    print("case batch captured by caller", flush=True)
+    banks = [_bank(ctx, 40 + s) for s in range(2)]
+    bl = [b for b, _ in banks]
+    sizes = [(128, 128), (96, 64)]
+    masks = _masks(torch, sizes, 5)
+    imgs = [torch.empty_like(m) for m in masks for _ in banks]
+    s = torch.cuda.Stream()
+    ctx.set_stream(s.cuda_stream)
+    try:
+        with torch.cuda.stream(s):
+            ctx.socs_forward_batch(bl, masks, imgs)  # first call: outside the capture
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            ctx.socs_forward_batch(bl, masks, imgs)
+        torch.cuda.synchronize()
+        # five other batches: more distinct entries than the cache keeps
+        for j in range(5):
+            other = _masks(torch, [(64 + 16 * j, 80)], 50 + j)
+            with torch.cuda.stream(s):
+                ctx.socs_forward_batch(bl, other)
+        torch.cuda.synchronize()
+        for rep in range(3):
+            rng = np.random.default_rng(200 + rep)
+            for mk in masks:
+                mk.copy_(torch.from_numpy(
+                    (rng.random(tuple(mk.shape)) > 0.5).astype(np.float32)).cuda())
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            for m, mk in enumerate(masks):
+                for si, bank in enumerate(bl):
+                    with torch.cuda.stream(s):
+                        single = ctx.socs_forward(bank, mk.clone())
+                    torch.cuda.synchronize()
+                    assert relerr(imgs[m * 2 + si].cpu().numpy(),
+                                  single.cpu().numpy()) < 1e-6, (rep, m, si)
+    finally:
+        torch.cuda.synchronize()
+        ctx.set_stream(None)
+        ctx.sync()

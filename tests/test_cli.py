@@ -121,3 +121,39 @@ def test_tcc_then_socs_end_to_end(workdir, NA, df, stype):
         assert r.returncode == 0, r.stderr + r.stdout
         simg = np.fromfile(workdir / "out" / "image.bin", np.float32).reshape(L, L)
         assert relerr(simg, O.socs_image(mask, kr, ki, sc, Nx, Ny)) < 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", ["default", "plain"])
+def test_socs_reference_kernels_text_mask(workdir, path):
+    """klitho_socs on the reference's own shipped kernel set (33 x 33, three
+    kernels: half-size 16, the size its test_tcc.sh sample gives at period
+    2048) with a 1024^2 Manhattan mask read from a whitespace text file
+    (Import, common/mask.cpp:45-51 -> file_io.cpp:204-230), centred in a
+    2048^2 period; image.bin against the oracle's SOCS image."""
+    import shutil
+    import workloads as W
+    print(f"case cli reference kernels 33x33, 2048^2, text mask, path {path}", flush=True)
+    kd = workdir / "ref_kernels"
+    os.makedirs(kd)
+    gold = os.path.join(os.path.dirname(__file__), "golden")
+    for f in os.listdir(gold):
+        if f.startswith("krn_") or f == "kernel_info.txt":
+            shutil.copy(os.path.join(gold, f), kd / f)
+    inner = W.manhattan(1024, 4242)
+    np.savetxt(workdir / "mask.txt", inner, fmt="%g")
+    env = {} if path == "default" else {"KLT_SOCS_PATH": "plain"}
+    r = _run(SOCS, [2048, 2048, 1024, 1024, "Import", 0, 0, 0, "mask.txt", str(kd) + "/"],
+             workdir, env)
+    assert r.returncode == 0, r.stderr + r.stdout
+    assert "Nx: 16,  Ny: 16" in r.stdout
+    sx, sy, sc = klt.read_kernel_info(str(kd / "kernel_info.txt"))
+    assert (sx, sy, len(sc)) == (33, 33, 3)
+    kr = np.stack([np.fromfile(kd / f"krn_{i}_r.bin", np.float32) for i in range(3)])
+    ki = np.stack([np.fromfile(kd / f"krn_{i}_i.bin", np.float32) for i in range(3)])
+    mask = np.zeros(2048 * 2048, np.float32)
+    O.lib().or_mask_place(np.ascontiguousarray(inner.ravel()), 1024, 1024, mask, 2048, 2048,
+                          1.0)
+    mask = mask.reshape(2048, 2048)
+    img = np.fromfile(workdir / "out" / "image.bin", np.float32).reshape(2048, 2048)
+    assert relerr(img, O.socs_image(mask, kr, ki, sc, 16, 16)) < 1e-5

@@ -215,3 +215,30 @@ def test_calc_kernels_layout_and_image(ctx):
     kr2, _, sc2 = ctx.calc_kernels(_pack(a[:9, :9]), 20)
     assert kr2.shape == (9, 9) and sc2.shape == (9,)
     assert T == 169
+
+
+@pytest.mark.parametrize("kind", ["zero", "diagonal", "diagonal_repeated"])
+def test_eig_zero_and_diagonal_inputs(ctx, kind):
+    """Tridiagonals with zero off-diagonals: an all-zero matrix (every pivot of
+    the inverse iteration's LU is zero), distinct and repeated diagonal
+    entries.  Finite results, exact eigenvalues, orthonormal eigenvectors."""
+    print(f"case eig {kind}", flush=True)
+    n, k = 96, 12
+    rng = np.random.default_rng(5)
+    if kind == "zero":
+        d = np.zeros(n)
+    elif kind == "diagonal":
+        d = rng.permutation(n).astype(np.float64) - 20.0
+    else:
+        d = np.repeat(np.arange(n // 4, dtype=np.float64), 4)
+        rng.shuffle(d)
+    a = np.diag(d).astype(np.complex128)
+    w, v = ctx.eig_topk(_pack(a), k)
+    v = _c(v)
+    assert np.all(np.isfinite(w)) and np.all(np.isfinite(v))
+    wo = np.sort(d)[::-1][:k]
+    scale = max(1.0, np.abs(d).max())
+    assert np.abs(w - wo).max() < 1e-12 * scale
+    for q in range(k):
+        assert np.linalg.norm(a @ v[q] - w[q] * v[q]) < 1e-11 * scale
+    assert np.abs(v @ v.conj().T - np.eye(k)).max() < 1e-11

@@ -209,6 +209,63 @@ def test_point_source_is_coherent():
     assert abs(w[-2]) < 1e-9 * npup
 
 
+@pytest.mark.parametrize("NA,nmed", [(0.9, 1.0), (1.35, 1.44)])
+@pytest.mark.parametrize("df", [0.2361, -0.2361, 0.4722, -0.4722])
+@pytest.mark.parametrize("pq", [(0, 0), (7, -4), (-16, 9)])
+def test_defocus_phase_is_angular_spectrum_propagation(NA, nmed, df, pq):
+    """The defocus phase of the pupil (klitho_tcc.cpp:191,197,206-221), pinned by
+    physics rather than by the reference's formula: a plane wave of spatial
+    frequency nu (cycles/nm) that propagates a distance z in a medium of index n
+    gains the phase 2 pi z sqrt(n^2/lambda^2 - |nu|^2) (angular-spectrum
+    transfer function).  Here z = defocus * lambda / NA^2 (the normalised
+    defocus of the tools' command line), and diffraction order (nx, ny) of a
+    period L lit from source cell (p, q) of the sigma-grid (sh cells per unit
+    sigma) has nu = (nx, ny)/L + (p, q)/sh * NA/lambda; the pupil passes
+    |nu| <= NA/lambda.  With a one-cell source the TCC is P P^H, so
+    arg TCC[i][c] = phi(nu_i) - phi(nu_c).  Everything below is float64 in
+    physical units; the oracle evaluates the phase in float32 like the
+    reference, hence the 1e-5 rad tolerance."""
+    lam, n, L = 193.0, 51, 2048
+    sh = (n - 1) // 2
+    p, q = pq
+    src = np.zeros((n, n), np.float32)
+    src[q + sh, p + sh] = 1.0
+    sig = O.outer_sigma(src)
+    Nx, Ny = O.sizing(NA, L, L, sig)
+    t = O.tcc(src, sig, NA, df, L, L, Nx, Ny, nmed=nmed)
+    W = 2 * Nx + 1
+    ny, nx = np.divmod(np.arange(t.shape[0]), W)
+    nx, ny = nx - Nx, ny - Ny
+    z = df * lam / NA ** 2
+    vx = nx / L + (p / sh) * NA / lam
+    vy = ny / L + (q / sh) * NA / lam
+    v2 = vx * vx + vy * vy
+    kz = np.sqrt(np.maximum(nmed ** 2 / lam ** 2 - v2, 0.0))
+    phi = 2.0 * np.pi * z * kz
+    rho = np.sqrt(v2) * lam / NA  # |nu| in units of NA / lambda
+    c = Ny * W + Nx  # nu_c = the source direction itself, always inside
+    col = t[:, c]
+    inside = np.abs(col) > 0
+    # membership: |nu| <= NA/lambda (decided in float32 by the reference, so
+    # samples within 1e-5 of the edge are left out)
+    clear = np.abs(rho - 1.0) > 1e-5
+    assert np.array_equal(inside[clear], rho[clear] <= 1.0)
+    assert inside.sum() > 200
+    d = np.angle(col[inside]) - (phi[inside] - phi[c])
+    d = (d + np.pi) % (2 * np.pi) - np.pi
+    assert np.abs(d).max() < 1e-5
+    # the phase varies over the pupil by much more than the tolerance
+    assert np.ptp(phi[inside]) > 0.1
+
+
+def test_relerr_sees_the_imaginary_part():
+    """A TCC whose imaginary part has the wrong sign must fail the comparison."""
+    rng = np.random.default_rng(0)
+    t = rng.standard_normal((20, 20)) + 1j * rng.standard_normal((20, 20))
+    assert relerr(t.conj(), t) > 0.5
+    assert relerr(t, t) == 0.0
+
+
 def test_pupil_float_decisions_vs_numpy_float32():
     """Membership decided in float32 exactly as klitho_tcc.cpp:206-219."""
     src = O.source("Annular", 51, inner=0.6, outer=0.9)

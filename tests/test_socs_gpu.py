@@ -57,6 +57,23 @@ def test_small_random_kernels(ctx, Lx, Ly, Nx, Ny, plain):
     assert relerr(img, O.socs_image(m, kr, ki, sc, Nx, Ny)) < TOL
 
 
+# kernel half-sizes above 15 (the reference's own 33 x 33 sample has N = 16):
+# odd and unequal periods, Ny != Nx
+LARGE_N = [(67, 70, 16, 16), (200, 75, 16, 18), (129, 100, 18, 17), (160, 160, 18, 18)]
+
+
+@pytest.mark.parametrize("plain", [True, False], ids=["plain", "default"])
+@pytest.mark.parametrize("Lx,Ly,Nx,Ny", LARGE_N)
+def test_large_half_size_random_kernels(ctx, Lx, Ly, Nx, Ny, plain):
+    print(f"case large N Lx={Lx} Ly={Ly} Nx={Nx} Ny={Ny} plain={plain}", flush=True)
+    rng = np.random.default_rng(Lx * 11 + Ly + Nx)
+    kr, ki, sc = _random_bank(rng, Nx, Ny, 4)
+    m = (rng.random((Ly, Lx)) > 0.5).astype(np.float32)
+    bank = ctx.kernels(kr, ki, sc, 2 * Nx + 1, 2 * Ny + 1)
+    img = ctx.socs_forward(bank, m, plain=plain)
+    assert relerr(img, O.socs_image(m, kr, ki, sc, Nx, Ny)) < TOL
+
+
 def test_continuous_mask_fused(ctx):
     """Non-binary transmittance exercises the lo half of the split mask."""
     print("case continuous mask", flush=True)

@@ -5,7 +5,10 @@
   defocus (device cosf/sinf vs glibc's differ by at most an ulp or two).
   The diagonal TCC[i][i] = sum_s S_s [i inside pupil(s)] pins every discrete
   pupil decision: one flipped sample moves it by S_s >= 1e-4.
-* split-FP16 MFMA Gram: relative Frobenius error <= 1e-5 against FP64.
+* split-FP16 MFMA Gram: relative Frobenius error <= 1e-5 against the FP64
+  oracle and the FP64 GPU Gram on complex entries; the imaginary part alone
+  within 1e-5 of the norm (annular and dipole sources, with and without
+  defocus).
 """
 import numpy as np
 import pytest
@@ -42,15 +45,32 @@ def test_tcc_fp64_vs_oracle(ctx, L, NA, df, nmed):
     assert np.abs(got - got.conj().T).max() == 0.0
 
 
-@pytest.mark.parametrize("L,NA,df,nmed", [(512, 1.35, 0.0, 1.44), (1088, 1.35, 0.0, 1.44),
-                                          (1280, 1.35, 0.4722, 1.44)])
-def test_tcc_split16_vs_fp64(ctx, L, NA, df, nmed):
-    print(f"case tcc split16 L={L} df={df}", flush=True)
-    src, sig = _annular()
-    Nx, Ny = O.sizing(NA, L, L, sig)
+SPLIT_CASES = [("Annular", 512, 1.35, 0.0, 1.44), ("Annular", 1088, 1.35, 0.0, 1.44),
+               ("Annular", 1280, 1.35, 0.4722, 1.44), ("Dipole", 700, 0.95, -0.25, 1.0),
+               ("Dipole", 1024, 1.35, 0.2361, 1.44)]
+
+
+@pytest.mark.parametrize("kind,L,NA,df,nmed", SPLIT_CASES)
+def test_tcc_split16_vs_oracle(ctx, kind, L, NA, df, nmed):
+    """split-FP16 MFMA Gram against the FP64 oracle and the FP64 GPU Gram, on the
+    complex entries.  With defocus the imaginary part (formed by the pair-swap
+    operand X') carries a large share of the norm; it is checked on its own."""
+    print(f"case tcc split16 {kind} L={L} NA={NA} df={df}", flush=True)
+    if kind == "Annular":
+        src, sig = _annular()
+    else:
+        src = O.source("Dipole", 101, radius=0.2, offset=0.7, onx=1)
+        sig = O.outer_sigma(src)
+    Nx, Ny = O.sizing(NA, L, L, min(sig, 1.0))
+    ref = O.tcc(src, sig, NA, df, L, L, Nx, Ny, nmed=nmed)
     a = _cplx(ctx.tcc_build(src, sig, NA, df, L, L, Nx, Ny, precision=0, nmed=nmed))
     b = _cplx(ctx.tcc_build(src, sig, NA, df, L, L, Nx, Ny, precision=1, nmed=nmed))
+    assert relerr(b, ref) < 1e-5
     assert relerr(b, a) < 1e-5
+    nref = np.linalg.norm(ref)
+    assert np.linalg.norm(b.imag - ref.imag) < 1e-5 * nref
+    if df != 0.0:
+        assert np.linalg.norm(ref.imag) > 0.1 * nref
     assert np.abs(b - b.conj().T).max() == 0.0
 
 
