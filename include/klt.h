@@ -190,6 +190,24 @@ int klt_socs_image(klt_ctx* ctx, const klt_kernels* kb, const float* mask, int L
 int klt_socs_forward(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
                      int nmask, int Lx, int Ly, float* images, unsigned flags);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the images of klt_socs_forward (same arithmetic and
+ * kernels, kernel half-size <= 15, device pointers only) with every table and
+ * intermediate in one fixed device array of KLT_NOALLOC_SCRATCH_BYTES per GPU,
+ * which the library holds until the process ends.  No allocation and no
+ * synchronous copy: the call only launches kernels on the context stream (the
+ * twiddle tables are generated on the device at the start of every call).
+ * Lx and Ly are limited to KLT_NOALLOC_MAX_SIDE (KLT_ERR_ARG beyond); larger
+ * batches run in sub-batches that fit the array.  All calls on one GPU share
+ * the array and it keeps the last call's contents: calls that overlap on one
+ * GPU (two streams, two host threads) race.  The split factors come from the
+ * fixed rule, not the tuner. */
+#define KLT_NOALLOC_SCRATCH_BYTES (32u << 20)
+#define KLT_NOALLOC_MAX_SIDE 4096
+int klt_socs_forward_noalloc(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
+                             int nmask, int Lx, int Ly, float* images);
+
 /* Masks of different sizes under several kernel sets in one call (no
  * reference counterpart: klitho_socs.cpp:338-343 images one mask per run).
  * masks[m] (Ly[m] x Lx[m]) under set s gives images[m * nbank + s].  Every

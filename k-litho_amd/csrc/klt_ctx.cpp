@@ -206,6 +206,12 @@ int klt_ctx_create(int device, klt_ctx** out) {
     return klt::set_error(KLT_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
   c->stream = c->own_stream;
+  rc = klt::noalloc_setup(device);
+  if (rc != KLT_OK) {
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return rc;
+  }
   *out = c;
   return KLT_OK;
 }

@@ -301,6 +301,9 @@ int eig_topk_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, i
 int socs_forward_batch(klt_ctx* ctx, int nmask, const float* const* masks, const int* Lx,
                        const int* Ly, int nbank, const klt_kernels* const* banks,
                        float* const* images);
+// Device address of socs_noalloc.hip's scratch array on `device` (the current
+// device), looked up once per device; called by klt_ctx_create.
+int noalloc_setup(int device);
 // Copy helpers honouring KLT_HOST_PTRS.
 int to_device(klt_ctx* ctx, const void* src, size_t bytes, unsigned flags, DevBuf& stage,
               const void** dptr);

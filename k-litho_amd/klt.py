@@ -90,6 +90,7 @@ def lib() -> C.CDLL:
         L.klt_kernels_destroy.restype = None
         L.klt_socs_image.argtypes = [vp, vp, fp, i, i, fp, u]
         L.klt_socs_forward.argtypes = [vp, vp, fp, i, i, i, fp, u]
+        L.klt_socs_forward_noalloc.argtypes = [vp, vp, fp, i, i, i, fp]
         L.klt_socs_forward_batch.argtypes = [vp, i, vp, vp, vp, i, vp, vp, u]
         L.klt_socs_batch_latency.argtypes = [vp, fp, i]
         _lib = L
