@@ -445,7 +445,8 @@ def main():
         "metric": "SOCS images/s at 1024^2 (K=24) + adjoint grads/s; rel-L2 err vs FP64 oracle",
         "value": value, "unit": "images/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "split-fp16/fp32",
+        "data": "synthetic",
         "config": {"workload": "SOCS imaging, 1024x1024 seeded Manhattan masks, K=24 kernels "
                                "29x29 (annular 0.6/0.9, NA 1.35, period 1088 nm)",
                    "batch": B, "L": L, "K": NK, "kernel_size": KSIZE,
