@@ -19,9 +19,10 @@ Also reported (same JSON line):
                p50/p99 per-image latency and the same work as grouped calls
   kernels      per-kernel HIP-event times of one step; `roofline` is the
                dominant kernel's algorithmic HBM bytes over its duration
-  cpu_baseline a numpy/pocketfft port of the reference's FFTW path
-               (FT_r2c + calcSOCS + FI) timed on a bounded sample on this
-               host, with the GPU image's (and a 512^2 adjoint sample's)
+  cpu_baseline the reference's FFTW3 path (FT_r2c + calcSOCS + FI) restated
+               in C++ on the FFTW3 interface of the host's MKL, 1 core and all
+               cores, calcTCC and zheevr_ top-32 at T = 1225, on bounded
+               samples, with the GPU image's (and a 512^2 adjoint sample's)
                relative L2 error vs the FP64 oracle
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--batch 512]
@@ -279,31 +280,95 @@ def bench_adjoint(ctx, bank, timer, torch, dev, src):
             "hbm_bytes_per_mask_grad": 4 * 5 * L * L}
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _host_threads():
+    """Worker threads for the all-core leg: the cores this process may use,
+    at most 16 (one GPU's share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def cpu_baseline(ctx, bank, fused_img, mask, kr, ki, sc, budget_s):
-    """Reference-path port (numpy) on one core, and FP64 oracle errors."""
+    """The reference's CPU path on this host, timed on bounded samples:
+    FT_r2c + calcSOCS + FI on FFTW3 (oracle/cpu_ref_path.cpp, the library
+    loaded at run time) on 1 core and on all cores (one worker per core, each
+    single threaded), calcTCC (the oracle's C restatement, 1 core) and
+    calcKernels' zheevr_ top-32 at T = 1225 on 1 and all cores; plus the FP64
+    oracle errors of the GPU image and mask gradient and the numpy port of the
+    FFTW path.  `value` is the all-core images/s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import cpu_ref as CR
     import oracle as O
     import ref_pipeline as RP
     N = (KSIZE - 1) // 2
-    t0 = time.perf_counter()
-    n = 0
-    while True:
-        RP.socs_pipeline(W.manhattan(L, 5000 + n), kr, ki, sc, N, N)
-        n += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
+    pool = np.stack([W.manhattan(L, 5000 + i) for i in range(16)])
     ref64 = O.socs_image(mask, kr, ki, sc, N, N)
     err = float(np.linalg.norm(fused_img - ref64) / np.linalg.norm(ref64))
-    # adjoint sample: one relaxed 512^2 mask
     tg = W.manhattan(512, 77)
     mk = W.relaxed(tg, 78)
     _, g, _ = klt.socs_backward(ctx, bank, mk[None].copy(), tg[None].copy())
     _, _, g64 = O.socs_loss_grad(mk.astype(np.float64), tg, kr, ki, sc, N, N)
     gerr = float(np.linalg.norm(g[0] - g64) / np.linalg.norm(g64))
-    cpu = {"value": n / dt, "unit": "images/s", "cores": 1, "kind": "port",
-           "sample": f"{n} seeded 1024^2 Manhattan masks, K=24 29x29, numpy pocketfft port "
-                     f"of FT_r2c+calcSOCS+FI, {dt:.1f} s on 1 core"}
+    nth = _host_threads()
+    info = {"nproc": os.cpu_count(), "cpu_model": _cpu_model(), "threads_all_cores": nth,
+            "library": CR.reason()}
+    # numpy port of the FFTW procedure (secondary figure), ~1/4 of the budget
+    t0, n = time.perf_counter(), 0
+    while time.perf_counter() - t0 < 0.25 * budget_s:
+        RP.socs_pipeline(pool[n % 16], kr, ki, sc, N, N)
+        n += 1
+    info["numpy_port_images_per_s_1core"] = n / (time.perf_counter() - t0)
+    if not CR.available():
+        cpu = {"value": info["numpy_port_images_per_s_1core"], "unit": "images/s", "cores": 1,
+               "kind": "port", "sample": f"FFTW3/LAPACK {CR.reason()}; numpy pocketfft port of "
+                                         f"FT_r2c+calcSOCS+FI, {n} seeded 1024^2 masks, 1 core",
+               **info}
+        return cpu, err, gerr
+    one = CR.socs_images(pool[0], kr, ki, sc, N, N, threads=1)[0]
+    info["image_rel_l2_vs_fp64_oracle"] = float(np.linalg.norm(one - ref64) / np.linalg.norm(ref64))
+    # 1 core
+    t0, n1 = time.perf_counter(), 0
+    while time.perf_counter() - t0 < 0.25 * budget_s:
+        CR.socs_images(pool[n1 % 16:n1 % 16 + 2], kr, ki, sc, N, N, threads=1)
+        n1 += 2
+    rate1 = n1 / (time.perf_counter() - t0)
+    # all cores: one call of 2 masks per worker, repeated within the budget
+    batch = np.concatenate([pool] * ((2 * nth + 15) // 16))[:2 * nth]
+    t0, na = time.perf_counter(), 0
+    while time.perf_counter() - t0 < 0.25 * budget_s:
+        CR.socs_images(batch, kr, ki, sc, N, N, threads=nth)
+        na += batch.shape[0]
+    rate_all = na / (time.perf_counter() - t0)
+    # calcTCC + zheevr_ top 32 at T = 1225 (blurred dipole, period 1280, 0 nm)
+    src = blurred_dipole()
+    sig = min(float(O.outer_sigma(src)), 1.0)
+    t0 = time.perf_counter()
+    t = O.tcc(src, sig, W.NA, 0.0, W.PERIOD_35, W.PERIOD_35, 17, 17, nmed=NMED)
+    info["calc_tcc_T1225_ms_1core"] = 1e3 * (time.perf_counter() - t0)
+    for th in (1, nth):
+        t0 = time.perf_counter()
+        CR.zheevr_topk(t, 32, threads=th)
+        info[f"zheevr_top32_T1225_ms_{th}threads"] = 1e3 * (time.perf_counter() - t0)
+    info["images_per_s_1core"] = rate1
+    cpu = {"value": rate_all, "unit": "images/s", "cores": nth, "kind": "port",
+           "sample": f"{na} (all cores) and {n1} (1 core) seeded 1024^2 Manhattan masks, K=24 "
+                     "29x29: the reference's FT_r2c+calcSOCS+FI restated on FFTW3 "
+                     "(FFTW_ESTIMATE, double) from the library below, one single-threaded "
+                     "worker per core",
+           **info}
     return cpu, err, gerr
 
 
@@ -313,7 +378,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=512)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=16.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline and plain path only")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -438,7 +503,8 @@ def main():
     if not args.no_cpu:
         cpu, err, gerr = cpu_baseline(ctx, bank, fused_img0, masks_h[0], kr, ki, sc,
                                       args.cpu_seconds)
-        log(f"cpu baseline: {cpu['value']:.2f} images/s; GPU image rel-L2 vs FP64 {err:.2e}, "
+        log(f"cpu baseline: {cpu['value']:.2f} images/s on {cpu['cores']} cores "
+            f"({cpu.get('images_per_s_1core', 0):.1f} on 1); GPU image rel-L2 vs FP64 {err:.2e}, "
             f"mask gradient rel-L2 vs FP64 {gerr:.2e}")
 
     out = {
