@@ -337,7 +337,7 @@ def cpu_baseline(ctx, bank, fused_img, mask, kr, ki, sc, budget_s):
                                          f"FT_r2c+calcSOCS+FI, {n} seeded 1024^2 masks, 1 core",
                **info}
         return cpu, err, gerr
-    one = CR.socs_images(pool[0], kr, ki, sc, N, N, threads=1)[0]
+    one = CR.socs_images(mask, kr, ki, sc, N, N, threads=1)[0]
     info["image_rel_l2_vs_fp64_oracle"] = float(np.linalg.norm(one - ref64) / np.linalg.norm(ref64))
     # 1 core
     t0, n1 = time.perf_counter(), 0

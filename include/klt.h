@@ -121,6 +121,22 @@ int klt_write_kernel_info(const char* path, int sizeX, int sizeY, int nk, const 
 /* common/file_io.cpp:282-307 (readKernelInfo): returns nk (>= 0) or an error. */
 int klt_read_kernel_info(const char* path, int* sizeX, int* sizeY, float* scales, int cap);
 
+/* ---- PNG views (common/file_io.cpp:7-135) --------------------------------
+ * The reference's three PNG writers, pixel for pixel: 8-bit grayscale of
+ * clamp((v - min) / (max - min), 0, 1) * 255 (:7-23, source.png and mask.png),
+ * the 7-colour partition of that value into [0, 1/6), ..., [5/6, 1), {1}
+ * (:25-80, image_s.png with min 0 and max TARGET_INTENSITY), and the
+ * continuous blue-aqua-green-yellow-orange-red map (:82-135, image.png,
+ * tcc_{r,i}.png, the kernel PNGs); min == max gives uniform green in the colour
+ * maps, as in the reference.  The files use stored (uncompressed) deflate
+ * blocks.  data: sy rows of sx values.  KLT_ERR_IO ("Error: Failed to write
+ * PNG: ...") when the file cannot be written. */
+int klt_write_png_gray(const char* path, int sx, int sy, const float* data, float min, float max);
+int klt_write_png_7colors(const char* path, int sx, int sy, const float* data, float min,
+                          float max);
+int klt_write_png_continuous(const char* path, int sx, int sy, const float* data, float min,
+                             float max);
+
 /* ---- TCC build (K-Litho-TCC/klitho_tcc.cpp:187-267, calcTCC) -----------
  * TCC[i][j] = sum_s S_s P(i,s) conj(P(j,s)) over the source points the
  * reference uses (S != 0, inside the unit disc, |p|,|q| <= ceil(sh*sigma)).
@@ -231,6 +247,17 @@ int klt_socs_forward_batch(klt_ctx* ctx, int nmask, const float* const* masks, c
  * image count n. */
 int klt_socs_batch_latency(klt_ctx* ctx, float* us, int cap);
 
+/* ---- resist step (klitho_tcc.cpp:442,547; klitho_socs.cpp:275,353;
+ * common/file_io.cpp:25-80) ------------------------------------------------
+ * The class of each pixel of nimg images (Ly x Lx) in the reference's 7-level
+ * view: v = clamp((I - min) / (max - min), 0, 1) computed in float, level 0..5
+ * for v in [k/6, (k+1)/6), level 6 for v = 1; min == max gives level 3.  With
+ * min = 0 and max = KLT_TARGET_INTENSITY, level 6 is the printed resist
+ * (I >= threshold).  levels: nimg * Ly * Lx bytes. */
+#define KLT_TARGET_INTENSITY 0.225f
+int klt_resist_levels(klt_ctx* ctx, const float* images, int nimg, int Lx, int Ly, float min,
+                      float max, unsigned char* levels, unsigned flags);
+
 /* ---- adjoint (no reference counterpart) ---------------------------------
  * Resist loss of each SOCS image, L = sum_x (sigmoid(a (I(x) - thr)) - T(x))^2,
  * and its gradient with respect to the mask, dL/dm (Ly x Lx per mask), through
@@ -252,6 +279,27 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
                          int Lx, int Ly, const float* src, int srcSize, float lambda, float NA,
                          float defocus, float nmed, int Nx, int Ny, float a, float thr,
                          float* grad_src, float* losses, float* images, unsigned flags);
+
+/* ---- source-mask optimisation (no reference counterpart) ----------------
+ * Projected, normalised gradient descent of the resist loss of
+ * klt_socs_backward over nmask masks and their shared source, `steps`
+ * iterations of: the TCC of the current source (klt_tcc_build, FP64 Gram, at
+ * the kernel half-size of a filled pupil, N = floor(NA L 2 / lambda), so that
+ * the source may grow anywhere inside the unit disc), its K = min(nk, T)
+ * kernels (klt_calc_kernels), the loss and mask gradient g_m through the SOCS
+ * model (klt_socs_backward), the source gradient g_S through the Abbe model
+ * (klt_abbe_source_grad, summed over the masks), then
+ *   m <- clip(m - step_mask g_m / max|g_m|, 0, 1)                    per mask,
+ *   S <- S' / sum S',  S' = max(S - step_src max(S) g_S / max|g_S|, 0)
+ * (grid points outside the unit disc stay zero).  masks: device, nmask x Ly x
+ * Lx, updated in place; targets: device; src: host srcSize^2, normalised,
+ * updated in place.  history[steps + 1]: the summed SOCS loss before each step
+ * and after the last; mismatch (optional, NULL) [steps + 1]: the count of
+ * pixels whose printed resist (I >= thr) differs from the target (T >= 0.5). */
+int klt_smo_run(klt_ctx* ctx, float* masks, const float* targets, int nmask, int Lx, int Ly,
+                float* src, int srcSize, float lambda, float NA, float defocus, float nmed,
+                int nk, float a, float thr, float step_mask, float step_src, int steps,
+                double* history, long long* mismatch);
 
 #ifdef __cplusplus
 }

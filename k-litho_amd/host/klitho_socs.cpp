@@ -3,8 +3,8 @@
 //
 // Same 10 positional arguments, messages and exit codes; reads
 // <krnDir>kernel_info.txt and <krnDir>krn_<i>_{r,i}.bin and writes
-// ./out/image.bin (float32 Ly x Lx).  The PNG visualisations of the reference
-// are not produced.  Environment:
+// ./out/image.bin (float32 Ly x Lx) and the reference's PNG views
+// ./out/{mask,image,image_s}.png (file_io.cpp:7-135).  Environment:
 //   KLT_SOCS_PATH  "fused" (default: split-precision matrix-core passes,
 //                  N <= 15) or "plain" (FP32 single-image path).
 //   KLT_DEVICE     GPU index (default 0).
@@ -27,6 +27,21 @@ namespace {
 
 void check(int rc) {
   if (rc != KLT_OK) die(klt_last_error());
+}
+
+// getMax / getMin (klitho_socs.cpp:16-35)
+float max_of(const vector<float>& v) {
+  float m = -1e+30f;
+  for (float x : v)
+    if (m < x) m = x;
+  return m;
+}
+
+float min_of(const vector<float>& v) {
+  float m = 1e+30f;
+  for (float x : v)
+    if (m > x) m = x;
+  return m;
 }
 
 }  // namespace
@@ -78,6 +93,7 @@ int main(int argc, char* argv[]) {
   if (maskType == "Import") cout << "Read Mask: " << maskInFile << endl;
   check(klt_mask_create(msk.data(), Lx, Ly, maskSizeX, maskSizeY, maskType.c_str(), lineWidth,
                         spaceWidth, isHorizontal, maskInFile.c_str(), dose));
+  check(klt_write_png_gray("./out/mask.png", Lx, Ly, msk.data(), min_of(msk), max_of(msk)));
   cout << endl << "Simulation Parameters:" << endl;
   cout << "-------------------" << endl;
   cout << "Mask Period: " << Lx << " x " << Ly << "\nMask Size: " << maskSizeX << " x "
@@ -112,11 +128,12 @@ int main(int argc, char* argv[]) {
     klt_kernels_destroy(kb);
   }
   klt_ctx_destroy(ctx);
-  float mx = -1e+30f;
-  for (float v : img)
-    if (mx < v) mx = v;
-  cout << "Max. Intensity: " << mx << endl;
+  cout << "Max. Intensity: " << max_of(img) << endl;
   check(klt_write_floats("./out/image.bin", img.data(), Lx * Ly));
+  check(klt_write_png_continuous("./out/image.png", Lx, Ly, img.data(), min_of(img),
+                                 max_of(img)));
+  check(klt_write_png_7colors("./out/image_s.png", Lx, Ly, img.data(), 0.0f,
+                              KLT_TARGET_INTENSITY));
 
   auto dur = chrono::system_clock::now() - start;
   auto us = chrono::duration_cast<chrono::microseconds>(dur).count();

@@ -2,11 +2,13 @@
 // running the TCC build, the Hopkins image and the eigensolver on the GPU
 // through libklt.
 //
-// Same 22 positional arguments, messages and exit codes; writes
+// Same 22 positional arguments, messages and exit codes; writes the
+// reference's output tree:
 //   ./out/image.bin                    (Hopkins image, float32 Ly x Lx)
 //   ./out/kernels/krn_<i>_{r,i}.bin    (float32 (2Ny+1) x (2Nx+1), index-reversed)
 //   ./out/kernels/kernel_info.txt
-// The PNG visualisations of the reference are not produced.
+//   ./out/{source,mask,tcc_r,tcc_i,image,image_s}.png, ./out/kernels/png/krn_<i>_{r,i}.png
+//                                      (the reference's PNG views, file_io.cpp:7-135)
 // Options that the reference's exact argument count cannot see come from the
 // environment:
 //   KLT_IMMERSION_INDEX  image-side medium index n (default 1): NA may be up to n;
@@ -28,10 +30,18 @@ using namespace std;
 
 namespace {
 
+// getMax / getMin (klitho_tcc.cpp:16-35)
 float max_of(const vector<float>& v) {
   float m = -1e+30f;
   for (float x : v)
     if (m < x) m = x;
+  return m;
+}
+
+float min_of(const vector<float>& v) {
+  float m = 1e+30f;
+  for (float x : v)
+    if (m > x) m = x;
   return m;
 }
 
@@ -111,6 +121,8 @@ int main(int argc, char* argv[]) {
     cout << "Read Source: " << sourceInFile << endl;
   check(klt_source_create(src.data(), srcSize, srcType.c_str(), innerR, outerR, radius, offset,
                           onXAxis ? 1 : 0, ptX, ptY, sourceInFile.c_str()));
+  check(klt_write_png_gray("./out/source.png", srcSize, srcSize, src.data(), min_of(src),
+                           max_of(src)));
 
   vector<float> msk((size_t)Lx * Ly, 0.0f);
   stringstream ss_msk;
@@ -122,6 +134,7 @@ int main(int argc, char* argv[]) {
     cout << "Read Mask: " << maskInFile << endl;
   check(klt_mask_create(msk.data(), Lx, Ly, maskSizeX, maskSizeY, maskType.c_str(), lineWidth,
                         spaceWidth, isHorizontal, maskInFile.c_str(), dose));
+  check(klt_write_png_gray("./out/mask.png", Lx, Ly, msk.data(), min_of(msk), max_of(msk)));
 
   float outerSigma = klt_outer_sigma(src.data(), srcSize);
   if (outerSigma > 1) {
@@ -160,15 +173,18 @@ int main(int argc, char* argv[]) {
   check(klt_tcc_build(ctx, src.data(), srcSize, outerSigma, lambda, (float)NA, defocus, nmed, Lx,
                       Ly, Nx, Ny, precision, tcc.data(), KLT_HOST_PTRS));
   {
-    // complexToFloat + getMax (klitho_tcc.cpp:518-528)
-    float mr = -1e+30f, mi = -1e+30f;
-    for (size_t i = 0; i < (size_t)tccSize * tccSize; ++i) {
-      const float r = (float)tcc[2 * i], im = (float)tcc[2 * i + 1];
-      if (mr < r) mr = r;
-      if (mi < im) mi = im;
+    // complexToFloat, getMax / getMin and the two PNG views (klitho_tcc.cpp:518-529)
+    vector<float> tr((size_t)tccSize * tccSize), ti((size_t)tccSize * tccSize);
+    for (size_t i = 0; i < tr.size(); ++i) {
+      tr[i] = (float)tcc[2 * i];
+      ti[i] = (float)tcc[2 * i + 1];
     }
-    cout << "Max. TCC Real Value: " << mr << endl;
-    cout << "Max. TCC Imag Value: " << mi << endl;
+    cout << "Max. TCC Real Value: " << max_of(tr) << endl;
+    check(klt_write_png_continuous("./out/tcc_r.png", tccSize, tccSize, tr.data(), min_of(tr),
+                                   max_of(tr)));
+    cout << "Max. TCC Imag Value: " << max_of(ti) << endl;
+    check(klt_write_png_continuous("./out/tcc_i.png", tccSize, tccSize, ti.data(), min_of(ti),
+                                   max_of(ti)));
   }
 
   cout << endl << "------------------------------------" << endl;
@@ -178,6 +194,10 @@ int main(int argc, char* argv[]) {
                           KLT_HOST_PTRS));
   cout << "Max. Intensity: " << max_of(img) << endl;
   check(klt_write_floats("./out/image.bin", img.data(), Lx * Ly));
+  check(klt_write_png_continuous("./out/image.png", Lx, Ly, img.data(), min_of(img),
+                                 max_of(img)));
+  check(klt_write_png_7colors("./out/image_s.png", Lx, Ly, img.data(), 0.0f,
+                              KLT_TARGET_INTENSITY));
 
   if (nk > 0) {
     cout << endl << "------------------------------------" << endl;
@@ -199,6 +219,12 @@ int main(int argc, char* argv[]) {
                              kr.data() + (size_t)i * tccSize, tccSize));
       check(klt_write_floats(("./out/kernels/krn_" + to_string(i) + "_i.bin").c_str(),
                              ki.data() + (size_t)i * tccSize, tccSize));
+      const vector<float> r(kr.begin() + (size_t)i * tccSize, kr.begin() + (size_t)(i + 1) * tccSize);
+      const vector<float> im(ki.begin() + (size_t)i * tccSize, ki.begin() + (size_t)(i + 1) * tccSize);
+      check(klt_write_png_continuous(("./out/kernels/png/krn_" + to_string(i) + "_r.png").c_str(),
+                                     2 * Nx + 1, 2 * Ny + 1, r.data(), min_of(r), max_of(r)));
+      check(klt_write_png_continuous(("./out/kernels/png/krn_" + to_string(i) + "_i.png").c_str(),
+                                     2 * Nx + 1, 2 * Ny + 1, im.data(), min_of(im), max_of(im)));
     }
     check(klt_write_kernel_info("./out/kernels/kernel_info.txt", 2 * Nx + 1, 2 * Ny + 1, nk,
                                 scales.data()));

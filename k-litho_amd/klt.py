@@ -93,6 +93,11 @@ def lib() -> C.CDLL:
         L.klt_socs_forward_noalloc.argtypes = [vp, vp, fp, i, i, i, fp]
         L.klt_socs_forward_batch.argtypes = [vp, i, vp, vp, vp, i, vp, vp, u]
         L.klt_socs_batch_latency.argtypes = [vp, fp, i]
+        for name in ("klt_write_png_gray", "klt_write_png_7colors", "klt_write_png_continuous"):
+            getattr(L, name).argtypes = [cs, i, i, fp, f, f]
+        L.klt_resist_levels.argtypes = [vp, fp, i, i, i, f, f, fp, u]
+        L.klt_smo_run.argtypes = [vp, vp, vp, i, i, i, fp, i, f, f, f, f, i, f, f, f, f, i, fp,
+                                  fp]
         _lib = L
     return _lib
 
@@ -192,6 +197,18 @@ def tcc_sizing(NA: float, Lx: int, Ly: int, sigma: float, lam: float = 193.0):
     nx, ny = C.c_int(), C.c_int()
     check(lib().klt_tcc_sizing(NA, Lx, Ly, sigma, lam, C.byref(nx), C.byref(ny)))
     return nx.value, ny.value
+
+
+TARGET_INTENSITY = 0.225  # klitho_tcc.cpp:442
+
+
+def write_png(kind: str, path: str, data: np.ndarray, vmin: float, vmax: float):
+    """The reference's PNG views (file_io.cpp:7-135): kind 'gray', '7colors' or
+    'continuous'; data [sy, sx] float32."""
+    d = np.ascontiguousarray(data, np.float32)
+    sy, sx = d.shape
+    fn = getattr(lib(), f"klt_write_png_{kind}")
+    check(fn(_cstr(path), sx, sy, d.ctypes.data_as(C.c_void_p), vmin, vmax))
 
 
 def read_floats(path: str, size: int) -> np.ndarray:
@@ -384,6 +401,25 @@ class Context:
                                           _flags(hm, ht, hi)))
         return img
 
+    def resist_levels(self, images, vmin: float = 0.0, vmax: float = TARGET_INTENSITY):
+        """Per-pixel class 0..6 of the reference's 7-level view (6 = I >= vmax
+        when vmin = 0: the printed resist), uint8 of the images' shape."""
+        shape = tuple(images.shape)
+        Ly, Lx = shape[-2:]
+        n = int(np.prod(shape[:-2])) if len(shape) > 2 else 1
+        if _is_torch(images):
+            import torch
+            out = torch.empty(shape, dtype=torch.uint8, device=images.device)
+            po, ho = C.c_void_p(out.data_ptr()), False
+        else:
+            out = np.empty(shape, np.uint8)
+            po, ho = out.ctypes.data_as(C.c_void_p), True
+        pi, hi = _ptr(images, np.float32)
+        with _TorchOrder(self._h, images):
+            check(lib().klt_resist_levels(self._h, pi, n, Lx, Ly, vmin, vmax, po,
+                                          _flags(hi, ho)))
+        return out
+
     def kernels(self, kr, ki, scales, size_x: int, size_y: int) -> "KernelBank":
         return KernelBank(self, kr, ki, scales, size_x, size_y)
 
@@ -534,3 +570,28 @@ class KernelBank:
             self.close()
         except Exception:
             pass
+
+
+def smo_run(ctx: "Context", masks, targets, src: np.ndarray, NA: float, steps: int,
+            step_mask: float = 0.05, step_src: float = 0.05, nk: int = 24, defocus: float = 0.0,
+            nmed: float = 1.0, a: float = 50.0, thr: float = 0.25, lam: float = 193.0):
+    """klt_smo_run: masks (torch GPU tensor [B, Ly, Lx], updated in place) and
+    src (numpy [n, n] float32, updated in place); returns (loss history
+    [steps + 1], printed-resist mismatch counts [steps + 1])."""
+    if not _is_torch(masks) or not _is_torch(targets):
+        raise TypeError("smo_run takes device (torch) masks and targets")
+    if not (isinstance(src, np.ndarray) and src.dtype == np.float32 and src.flags.c_contiguous):
+        raise TypeError("src must be a C-contiguous float32 numpy array")
+    shape = tuple(masks.shape)
+    Ly, Lx = shape[-2:]
+    B = int(np.prod(shape[:-2])) if len(shape) > 2 else 1
+    hist = np.zeros(steps + 1, np.float64)
+    mis = np.zeros(steps + 1, np.int64)
+    pm, _ = _ptr(masks, np.float32)
+    pt, _ = _ptr(targets, np.float32)
+    with _TorchOrder(ctx.handle, masks, targets):
+        check(lib().klt_smo_run(ctx.handle, pm, pt, B, Lx, Ly, src.ctypes.data_as(C.c_void_p),
+                                int(src.shape[0]), lam, NA, defocus, nmed, nk, a, thr, step_mask,
+                                step_src, steps, hist.ctypes.data_as(C.c_void_p),
+                                mis.ctypes.data_as(C.c_void_p)))
+    return hist, mis

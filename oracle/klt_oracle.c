@@ -1019,3 +1019,31 @@ void or_abbe_image(const double* mask, int Lx, int Ly, const float* src, int src
                     1.0, 0.0, img, NULL);
   free(zt);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Resist step: the 7-level view image_s.png (common/file_io.cpp:25-80,      */
+/* klitho_tcc.cpp:547 with min 0, max TARGET_INTENSITY = 0.225, :442)         */
+/* ------------------------------------------------------------------------ */
+
+/* lev[i] = the colour class 0..6 the reference gives img[i]: v =
+ * std::max(0.0f, std::min(1.0f, (img - mn) / (mx - mn))) in float, classes
+ * [k/6, (k+1)/6) compared in double, 6 for v >= 1; mn == mx gives class 3
+ * (the reference's uniform green). */
+void or_resist_levels(const float* img, int n, float mn, float mx, unsigned char* lev) {
+  for (int i = 0; i < n; ++i) {
+    if (mn == mx) {
+      lev[i] = 3;
+      continue;
+    }
+    float q = (img[i] - mn) / (mx - mn);
+    float t = (q < 1.0f) ? q : 1.0f;
+    double v = (0.0f < t) ? t : 0.0f;
+    int c = 6;
+    for (int k = 1; k <= 6; ++k)
+      if (v < (double)k / 6.0) {
+        c = k - 1;
+        break;
+      }
+    lev[i] = (unsigned char)c;
+  }
+}

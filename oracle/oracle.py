@@ -62,6 +62,7 @@ def lib():
             "or_abbe_loss_grad": (d, [_f64p, _f32p, i, i, _f32p, i, f, f, f, i, i, f, d, d,
                                       C.c_void_p, C.c_void_p]),
             "or_abbe_image": (None, [_f64p, i, i, _f32p, i, f, f, f, i, i, f, _f64p]),
+            "or_resist_levels": (None, [_f32p, i, f, f, np.ctypeslib.ndpointer(np.uint8)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -85,6 +86,8 @@ def ref():
         R.ref_read_float_bin.argtypes = [s, _f32p, i]
         R.ref_read_float_bin.restype = i
         R.ref_write_float_bin.argtypes = [s, _f32p, i]
+        for name in ("ref_png_gray", "ref_png_7colors", "ref_png_continuous"):
+            getattr(R, name).argtypes = [s, i, i, _f32p, f, f]
         _ref = R
     return _ref
 
@@ -245,3 +248,11 @@ def abbe_image(mask, src, NA, defocus, Nx, Ny, lam=193.0, nmed=1.0) -> np.ndarra
 def resist_loss(img: np.ndarray, target: np.ndarray, a=50.0, thr=0.25) -> float:
     s = 1.0 / (1.0 + np.exp(-a * (img.astype(np.float64) - thr)))
     return float(np.sum((s - target.astype(np.float64)) ** 2))
+
+
+def resist_levels(img: np.ndarray, vmin: float = 0.0, vmax: float = 0.225) -> np.ndarray:
+    """Class 0..6 of each pixel in the reference's 7-level view (file_io.cpp:25-80)."""
+    d = np.ascontiguousarray(img, np.float32)
+    out = np.empty(d.shape, np.uint8)
+    lib().or_resist_levels(d.ravel(), d.size, vmin, vmax, out.reshape(-1))
+    return out

@@ -76,4 +76,19 @@ void ref_write_float_bin(const char* path, const float* data, int size) {
   writeFloatArrayToBinary(path, d, size);
 }
 
+// file_io.cpp:7-135: the three PNG writers of the tools
+void ref_png_gray(const char* path, int sx, int sy, const float* data, float mn, float mx) {
+  std::vector<float> d(data, data + (size_t)sx * sy);
+  writeFloatArrayToPNGWithGrayscale(sx, sy, d, mn, mx, path);
+}
+void ref_png_7colors(const char* path, int sx, int sy, const float* data, float mn, float mx) {
+  std::vector<float> d(data, data + (size_t)sx * sy);
+  writeFloatArrayToPNGWith7Colors(sx, sy, d, path, mn, mx);
+}
+void ref_png_continuous(const char* path, int sx, int sy, const float* data, float mn,
+                        float mx) {
+  std::vector<float> d(data, data + (size_t)sx * sy);
+  writeFloatArrayToPNGWithContinuousColor(sx, sy, d, path, mn, mx);
+}
+
 }  // extern "C"

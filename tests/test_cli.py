@@ -35,7 +35,7 @@ def _tcc_args(L=512, ms=512, mtype="LineSpace", lw=32, sw=32, hz=0, mfile="-", n
 
 @pytest.fixture
 def workdir(tmp_path):
-    os.makedirs(tmp_path / "out" / "kernels")
+    os.makedirs(tmp_path / "out" / "kernels" / "png")
     return tmp_path
 
 
@@ -113,6 +113,21 @@ def test_tcc_then_socs_end_to_end(workdir, NA, df, stype):
     assert kr.shape == (12, T)
     assert np.allclose((kr.astype(np.float64) ** 2 + ki.astype(np.float64) ** 2).sum(1), 1,
                        atol=1e-5)
+    # the reference's PNG views: sizes, and image_s.png = the 7-level classes
+    # of image.bin against TARGET_INTENSITY (pinned to the reference's writer
+    # in test_png_resist.py)
+    from png_decode import read_png
+    seven = np.array([(0, 0, 255), (0, 127, 255), (0, 255, 127), (0, 255, 0), (127, 255, 0),
+                      (255, 127, 0), (255, 0, 0)], np.uint8)
+    assert read_png(workdir / "out" / "image_s.png").shape == (L, L, 3)
+    assert np.array_equal(read_png(workdir / "out" / "image_s.png"),
+                          seven[O.resist_levels(img)])
+    assert read_png(workdir / "out" / "image.png").shape == (L, L, 3)
+    assert read_png(workdir / "out" / "mask.png").shape == (L, L)
+    assert read_png(workdir / "out" / "source.png").shape == (51, 51)
+    assert read_png(workdir / "out" / "tcc_r.png").shape == (T, T, 3)
+    for i in range(12):
+        assert read_png(workdir / "out" / "kernels" / "png" / f"krn_{i}_i.png").shape == (sy, sx, 3)
     # klitho_socs on the written kernels (both paths)
     for path in ("fused", "plain"):
         os.remove(workdir / "out" / "image.bin")
