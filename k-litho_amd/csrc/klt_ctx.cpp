@@ -7,13 +7,6 @@
 
 namespace klt {
 
-static thread_local std::string g_last_error;
-
-int set_error(int code, const std::string& msg) {
-  g_last_error = msg;
-  return code;
-}
-
 int DevBuf::ensure(size_t need) {
   if (need <= bytes && ptr) return KLT_OK;
   release();
@@ -174,8 +167,6 @@ int klt_ctx::activate() const {
 }
 
 extern "C" {
-
-const char* klt_last_error(void) { return klt::g_last_error.c_str(); }
 
 int klt_version(void) { return 1; }
 

@@ -13,11 +13,9 @@
 #include <vector>
 
 #include "../../include/klt.h"
+#include "klt_error.h"
 
 namespace klt {
-
-// Records `msg` as the calling thread's last error and returns `code`.
-int set_error(int code, const std::string& msg);
 
 #define KLT_HIP(expr)                                                         \
   do {                                                                        \
@@ -25,12 +23,6 @@ int set_error(int code, const std::string& msg);
     if (e_ != hipSuccess)                                                     \
       return ::klt::set_error(KLT_ERR_HIP, std::string(#expr) + ": " +        \
                                                hipGetErrorString(e_));        \
-  } while (0)
-
-#define KLT_TRY(expr)          \
-  do {                         \
-    int rc_ = (expr);          \
-    if (rc_ != KLT_OK) return rc_; \
   } while (0)
 
 // Growable device buffer owned by a context.

@@ -10,7 +10,7 @@
 #include <string>
 #include <vector>
 
-#include "klt_internal.h"
+#include "klt_error.h"
 
 namespace {
 
