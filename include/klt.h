@@ -36,6 +36,7 @@ extern "C" {
 #define KLT_ERR_EIG -7      /* eigensolver did not converge (reference: zheevr info != 0) */
 
 #define KLT_HOST_PTRS 1u    /* array arguments are host memory */
+#define KLT_EIG_PSD 2u      /* eigensolver input is positive semidefinite (see klt_eig_topk) */
 
 typedef struct klt_ctx klt_ctx;
 typedef struct klt_kernels klt_kernels;
@@ -158,7 +159,14 @@ int klt_tcc_build(klt_ctx* ctx, const float* src, int srcSize, float outerSigma,
  * storage; like zheevr_ with uplo='U' the result depends only on the upper
  * triangle for an exactly Hermitian input).  w: k eigenvalues, descending;
  * vecs: k x n complex, row q the unit eigenvector of w[q] (phase arbitrary,
- * as in LAPACK). */
+ * as in LAPACK).  With KLT_EIG_PSD in flags the caller states that the matrix
+ * is positive semidefinite, as every TCC of a non-negative source is
+ * (TCC = P^H diag(S) P); the solver then takes the k <= 48 largest eigenpairs
+ * by block subspace iteration (64 columns, rounds of X <- orth(A^2 X) with
+ * CholQR2, Rayleigh-Ritz) to residuals of 5e-14 of the largest eigenvalue, and
+ * uses the Householder path for matrices it does not reach (n < 128, k > 48,
+ * no convergence in 24 rounds).  The flag applies to klt_eig_topk_batch and
+ * klt_calc_kernels as well. */
 int klt_eig_topk(klt_ctx* ctx, const double* a, int n, int k, double* w, double* vecs,
                  unsigned flags);
 /* A batch of `count` Hermitian matrices of the same size in one call (no

@@ -1626,6 +1626,17 @@ int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_d
   return eig_topk_batch_device(ctx, a_dev, 1, n, k, w_dev, vec_dev);
 }
 
+// KLT_EIG_PSD: the subspace solver first; the Householder path when it
+// reports that a matrix is outside its reach.
+static int eig_dispatch(klt_ctx* ctx, const double* a_dev, int count, int n, int k, double* w,
+                        double* v, unsigned flags) {
+  if (flags & KLT_EIG_PSD) {
+    const int rc = eig_topk_psd_batch_device(ctx, a_dev, count, n, k, w, v);
+    if (rc != KLT_ERR_EIG) return rc;
+  }
+  return eig_topk_batch_device(ctx, a_dev, count, n, k, w, v);
+}
+
 }  // namespace klt
 
 extern "C" {
@@ -1648,7 +1659,7 @@ int klt_eig_topk(klt_ctx* ctx, const double* a, int n, int k, double* w, double*
     dv = ctx->stage_out.as<double>();
     dw = dv + (size_t)k * n * 2;
   }
-  KLT_TRY(klt::eig_topk_device(ctx, da, n, k, dw, dv));
+  KLT_TRY(klt::eig_dispatch(ctx, da, 1, n, k, dw, dv, flags));
   if (flags & KLT_HOST_PTRS) {
     KLT_HIP(hipMemcpyAsync(vecs, dv, (size_t)k * n * 16, hipMemcpyDeviceToHost, ctx->stream));
     KLT_HIP(hipMemcpyAsync(w, dw, (size_t)k * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -1678,7 +1689,7 @@ int klt_eig_topk_batch(klt_ctx* ctx, const double* a, int count, int n, int k, d
     dv = ctx->stage_out.as<double>();
     dw = dv + B * k * n * 2;
   }
-  KLT_TRY(klt::eig_topk_batch_device(ctx, da, count, n, k, dw, dv));
+  KLT_TRY(klt::eig_dispatch(ctx, da, count, n, k, dw, dv, flags));
   if (flags & KLT_HOST_PTRS) {
     KLT_HIP(hipMemcpyAsync(vecs, dv, B * k * n * 16, hipMemcpyDeviceToHost, ctx->stream));
     KLT_HIP(hipMemcpyAsync(w, dw, B * k * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -1709,7 +1720,7 @@ int klt_calc_kernels(klt_ctx* ctx, const double* tcc, int T, int nk, float* krn_
   float* fr = reinterpret_cast<float*>(o + vb + wb);
   float* fi = fr + (size_t)nk * T;
   float* fs = fi + (size_t)nk * T;
-  KLT_TRY(klt::eig_topk_device(ctx, da, T, nk, dw, dv));
+  KLT_TRY(klt::eig_dispatch(ctx, da, 1, T, nk, dw, dv, flags));
   const int tot = nk * T;
   KLT_LAUNCH(ctx, "k_kernels_from_vectors", k_kernels_from_vectors, dim3((tot + 255) / 256),
              dim3(256), 0, reinterpret_cast<const cd*>(dv), dw, nk, T, fr, fi, fs);

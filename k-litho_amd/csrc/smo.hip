@@ -114,8 +114,9 @@ int klt_smo_run(klt_ctx* ctx, float* masks, const float* targets, int nmask, int
     KLT_HIP(hipMemcpyAsync(dsrc.ptr, src, (size_t)n2 * 4, hipMemcpyHostToDevice, ctx->stream));
     KLT_TRY(klt_tcc_build(ctx, dsrc.as<float>(), srcSize, 1.0f, lambda, NA, defocus, nmed, Lx,
                           Ly, Nx, Ny, KLT_TCC_FP64, tcc.as<double>(), 0));
+    // the source stays non-negative, so the TCC is positive semidefinite
     KLT_TRY(klt_calc_kernels(ctx, tcc.as<double>(), T, K, kr.as<float>(), ki.as<float>(),
-                             sc.as<float>(), 0));
+                             sc.as<float>(), KLT_EIG_PSD));
     klt_kernels* kb = nullptr;
     KLT_TRY(klt_kernels_create(ctx, kr.as<float>(), ki.as<float>(), sc.as<float>(), K,
                                2 * Nx + 1, 2 * Ny + 1, 0, &kb));

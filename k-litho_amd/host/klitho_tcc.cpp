@@ -204,8 +204,10 @@ int main(int argc, char* argv[]) {
     cout << "Starting calcKernels..." << endl;
     vector<float> kr((size_t)nk * tccSize), ki((size_t)nk * tccSize), scales(nk);
     cout << "Solving eigenproblem..." << endl;
+    // a non-negative source gives a positive semidefinite TCC (P^H diag(S) P)
+    const unsigned psd = min_of(src) >= 0.0f ? KLT_EIG_PSD : 0u;
     const int rc = klt_calc_kernels(ctx, tcc.data(), tccSize, nk, kr.data(), ki.data(),
-                                    scales.data(), KLT_HOST_PTRS);
+                                    scales.data(), KLT_HOST_PTRS | psd);
     if (rc != KLT_OK) {
       cerr << "Kernels derivation failed with info = " << rc << " (" << klt_last_error() << ")"
            << endl;

@@ -18,6 +18,7 @@ HEADER = os.path.join(os.path.dirname(PKG), "include", "klt.h")
 
 KLT_OK = 0
 KLT_HOST_PTRS = 1
+KLT_EIG_PSD = 2
 
 ERRORS = {-1: "KLT_ERR_ARG", -2: "KLT_ERR_SIZE", -3: "KLT_ERR_HIP", -4: "KLT_ERR_NODEV",
           -5: "KLT_ERR_IO", -6: "KLT_ERR_NOMEM", -7: "KLT_ERR_EIG"}
@@ -324,8 +325,9 @@ class Context:
                                       Ny, precision, po, _flags(hs, ho)))
         return out
 
-    def eig_topk(self, a, k: int):
-        """Top-k eigenpairs of a Hermitian matrix a [n, n, 2] (re, im), FP64 on the GPU.
+    def eig_topk(self, a, k: int, psd: bool = False):
+        """Top-k eigenpairs of a Hermitian matrix a [n, n, 2] (re, im), FP64 on the GPU;
+        psd=True: a is positive semidefinite (KLT_EIG_PSD).
 
         Returns (w [k], vecs [k, n, 2]) on the same side (host/device) as a."""
         n = int(a.shape[0])
@@ -341,10 +343,11 @@ class Context:
         pw, hw = _ptr(w, np.float64)
         pv, hv = _ptr(v, np.float64)
         with _TorchOrder(self._h, a):
-            check(lib().klt_eig_topk(self._h, pa, n, k, pw, pv, _flags(ha, hw, hv)))
+            check(lib().klt_eig_topk(self._h, pa, n, k, pw, pv,
+                                     _flags(ha, hw, hv) | (KLT_EIG_PSD if psd else 0)))
         return w, v
 
-    def eig_topk_batch(self, a, k: int):
+    def eig_topk_batch(self, a, k: int, psd: bool = False):
         """Top-k eigenpairs of each Hermitian matrix of a [count, n, n, 2] in one
         call: (w [count, k], vecs [count, k, n, 2]) on the same side as a."""
         count, n = int(a.shape[0]), int(a.shape[1])
@@ -360,11 +363,14 @@ class Context:
         pw, hw = _ptr(w, np.float64)
         pv, hv = _ptr(v, np.float64)
         with _TorchOrder(self._h, a):
-            check(lib().klt_eig_topk_batch(self._h, pa, count, n, k, pw, pv, _flags(ha, hw, hv)))
+            check(lib().klt_eig_topk_batch(self._h, pa, count, n, k, pw, pv,
+                                           _flags(ha, hw, hv) | (KLT_EIG_PSD if psd else 0)))
         return w, v
 
-    def calc_kernels(self, tcc, nk: int):
-        """calcKernels: (krn_r [nk, T], krn_i [nk, T], scales [nk]) float32."""
+    def calc_kernels(self, tcc, nk: int, psd: bool = True):
+        """calcKernels: (krn_r [nk, T], krn_i [nk, T], scales [nk]) float32; psd:
+        the TCC is positive semidefinite (KLT_EIG_PSD; true of every TCC of a
+        non-negative source)."""
         T = int(tcc.shape[0])
         nk = min(nk, T)
         if _is_torch(tcc):
@@ -382,7 +388,7 @@ class Context:
         p3, h3 = _ptr(sc, np.float32)
         with _TorchOrder(self._h, tcc):
             check(lib().klt_calc_kernels(self._h, pt, T, nk, p1, p2, p3,
-                                         _flags(ht, h1, h2, h3)))
+                                         _flags(ht, h1, h2, h3) | (KLT_EIG_PSD if psd else 0)))
         return kr, ki, sc
 
     def hopkins_image(self, mask, tcc, Nx: int, Ny: int):
