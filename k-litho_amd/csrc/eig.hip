@@ -454,47 +454,45 @@ __global__ __launch_bounds__(kThreads) void k_hh_her2(double* __restrict__ A, in
 }
 
 // The last steps of the reduction in one block per matrix, from step j0 on,
-// once the trailing block A[j0.., j0..] (s = n - j0 <= kTail rows) fits in
-// LDS: the launches of those steps would each cost the fixed latency of a
-// device-wide step for a few rows of work.  The block loads the trailing
-// block, applies step j0-1's pending rank-2 update (as k_hh_step would), then
-// runs steps j0 .. n-3 in LDS with the same reflector conventions (V rows,
-// tau, sub, diag) and finishes the last 2 x 2 block (as k_hh_finish).
-// Dynamic LDS: (s^2 + 3 s) complex values.
-constexpr int kTail = 96;
-constexpr int kTailThreads = 1024;
+// once the trailing block A[j0.., j0..] has s = n - j0 <= kTail rows: the
+// launches of those steps would each cost the fixed latency of a device-wide
+// step for a few rows of work.  The trailing block, padded with zeros to
+// 64 x 64, is held in registers: wave q, lane c keeps row 4 (jb + i) + q,
+// column c, in W[i] during group jb of four steps (the window moves down one
+// row per wave after each group, so that register indices stay static).  The
+// block applies step j0-1's pending rank-2 update (as k_hh_step would), then
+// runs steps j0 .. n-3 with the same reflector conventions (V rows, tau, sub,
+// diag) and records the last 2 x 2 block (as k_hh_finish).  Step t (row t of
+// the block): the wave holding row t forms the reflector of x = conj(row t)
+// right of the diagonal and publishes v and tau through LDS; every wave sums
+// its rows of p = tau S22 v as column sums (S22 is Hermitian, so
+// p_c = tau sum_r conj(S[r][c]) v_r), the four waves' sums are added in
+// order through LDS, each wave forms w = p - (tau / 2)(v^H p) v in its lanes
+// and updates its rows, S[r][c] -= v_r conj(w_c) + w_r conj(v_c), taking v_r
+// and w_r from lane r.  Outputs are staged in LDS and written at the end.
+constexpr int kTail = 64;
 
-// Block sum of (a, b) over kTailThreads threads (red: 32 doubles); valid in
-// every thread.
-__device__ __forceinline__ void tail_sum2(double& a, double& b, double* red) {
-  constexpr int NW = kTailThreads / 64;
-  a = wave_sum(a);
-  b = wave_sum(b);
-  const int w = threadIdx.x >> 6;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) {
-    red[2 * w] = a;
-    red[2 * w + 1] = b;
-  }
-  __syncthreads();
-  a = 0.0;
-  b = 0.0;
-#pragma unroll
-  for (int q = 0; q < NW; ++q) {
-    a += red[2 * q];
-    b += red[2 * q + 1];
-  }
+__device__ __forceinline__ double lane_d(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
 }
+__device__ __forceinline__ cd lane_cd(cd v, int l) { return {lane_d(v.x, l), lane_d(v.y, l)}; }
 
-__global__ __launch_bounds__(kTailThreads) void k_hh_tail(
+__global__ __launch_bounds__(256) void k_hh_tail(
     const double* __restrict__ A, int n, int j0, cd* __restrict__ V,
     double* __restrict__ tau, cd* __restrict__ sub, double* __restrict__ diag,
     const cd* __restrict__ p_prev, const cd* __restrict__ part_prev, int nparts_prev,
     EigStride st) {
-  extern __shared__ cd tsh[];
-  __shared__ double red[2 * (kTailThreads / 64)];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  constexpr int NW = kTailThreads / 64;
+  __shared__ cd vb[2][64];     // v of the step (double-buffered)
+  __shared__ double tb[2];     // tau of the step
+  __shared__ cd pb[2][4][64];  // the waves' column sums of S22 v
+  __shared__ cd Vs[64][64];    // row t: v of step t
+  __shared__ cd so[64];        // sub
+  __shared__ double to[64], dg[64];
+  __shared__ double red[8];
+  const int tid = threadIdx.x, c = tid & 63;
+  const int q = __builtin_amdgcn_readfirstlane(tid >> 6);  // the wave (uniform)
   {
     const size_t mi = blockIdx.y;
     A += mi * st.a;
@@ -506,111 +504,121 @@ __global__ __launch_bounds__(kTailThreads) void k_hh_tail(
     part_prev += mi * st.part;
   }
   const int s = n - j0;
-  cd* S = tsh;           // S[i][l] = A[j0 + i][j0 + l], s x s
-  cd* sv = S + s * s;    // reflector v (and v' of the pending update)
-  cd* sw = sv + s;       // w (and w')
-  cd* sp = sw + s;       // p
   const cd* Ag = reinterpret_cast<const cd*>(A);
-  for (int q = tid; q < s * s; q += kTailThreads) {
-    const int i = q / s, l = q - i * s;
-    S[q] = Ag[(size_t)(j0 + i) * n + j0 + l];
+  cd W[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int r = q + 4 * i;
+    W[i] = (r < s && c < s) ? Ag[(size_t)(j0 + r) * n + j0 + c] : cd{0.0, 0.0};
   }
   // step j0-1's update S -= v' w'^H + w' v'^H, w' = p' - K' v'
   const double tp = j0 > 0 ? tau[j0 - 1] : 0.0;
   if (tp != 0.0) {  // (uniform)
     double kr = 0.0, ki = 0.0;
-    for (int q = tid; q < nparts_prev; q += kTailThreads) {
-      kr += part_prev[q].x;
-      ki += part_prev[q].y;
+    for (int u = tid; u < nparts_prev; u += 256) {
+      kr += part_prev[u].x;
+      ki += part_prev[u].y;
     }
-    tail_sum2(kr, ki, red);
+    block_sum2(kr, ki, red);
     const cd K = {0.5 * tp * kr, 0.5 * tp * ki};
-    if (tid < s) {
-      const cd v = V[(size_t)(j0 - 1) * n + j0 + tid], pp = p_prev[tid], kv = cmul(K, v);
-      sv[tid] = v;
-      sw[tid] = {pp.x - kv.x, pp.y - kv.y};
+    const cd vc = c < s ? V[(size_t)(j0 - 1) * n + j0 + c] : cd{0.0, 0.0};
+    const cd pc = c < s ? p_prev[c] : cd{0.0, 0.0}, kv = cmul(K, vc);
+    const cd wc = {pc.x - kv.x, pc.y - kv.y};
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int r = q + 4 * i;
+      if (r >= s) continue;  // (uniform)
+      const cd vr = lane_cd(vc, r), wr = lane_cd(wc, r);
+      W[i].x -= (vr.x * wc.x + vr.y * wc.y) + (wr.x * vc.x + wr.y * vc.y);
+      W[i].y -= (vr.y * wc.x - vr.x * wc.y) + (wr.y * vc.x - wr.x * vc.y);
     }
-    __syncthreads();
-    for (int q = tid; q < s * s; q += kTailThreads) {
-      const int i = q / s, l = q - i * s;
-      const cd vi = sv[i], wi = sw[i], vl = sv[l], wl = sw[l];
-      S[q].x -= (vi.x * wl.x + vi.y * wl.y) + (wi.x * vl.x + wi.y * vl.y);
-      S[q].y -= (vi.y * wl.x - vi.x * wl.y) + (wi.y * vl.x - wi.x * vl.y);
+  }
+  for (int jb = 0; jb < 16; ++jb) {
+#pragma unroll
+    for (int jq = 0; jq < 4; ++jq) {
+      const int t = 4 * jb + jq, par = jq & 1;
+      if (t + 2 >= s) break;  // (uniform)
+      if (q == jq) {  // row t is this wave's W[0]
+        const cd g = W[0];
+        const cd x = c > t ? cd{g.x, -g.y} : cd{0.0, 0.0};
+        const double s2 = wave_sum(x.x * x.x + x.y * x.y);
+        const cd x0 = lane_cd(x, t + 1);
+        const double ax0 = hypot(x0.x, x0.y);
+        const double alpha = sqrt(s2);
+        const bool skip = (s2 - ax0 * ax0) <= 0.0;  // x already a multiple of e_1
+        cd ph = {1.0, 0.0};
+        if (ax0 > 0.0) ph = {x0.x / ax0, x0.y / ax0};
+        const double tt = skip ? 0.0 : 1.0 / (alpha * (alpha + ax0));  // 2 / (v^H v)
+        cd v = x;
+        if (c == t + 1 && !skip) {
+          v.x += ph.x * alpha;
+          v.y += ph.y * alpha;
+        }
+        vb[par][c] = v;
+        Vs[t][c] = skip ? cd{0.0, 0.0} : v;
+        if (c == 0) {
+          tb[par] = tt;
+          to[t] = tt;
+          so[t] = skip ? x0 : cd{-ph.x * alpha, -ph.y * alpha};
+        }
+      }
+      __syncthreads();
+      const double tt = tb[par];
+      if (tt == 0.0) continue;  // (uniform) the trailing block stays as it is
+      const cd vc = vb[par][c];
+      double ar = 0.0, ai = 0.0;  // this wave's rows of sum_r conj(S[r][c]) v_r
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int r = 4 * (jb + i) + q;
+        if (r <= t || r >= s) continue;  // (uniform)
+        const cd vr = lane_cd(vc, r);
+        ar += W[i].x * vr.x + W[i].y * vr.y;
+        ai += W[i].x * vr.y - W[i].y * vr.x;
+      }
+      pb[par][q][c] = {ar, ai};
+      __syncthreads();
+      cd pc = {0.0, 0.0};
+      if (c > t) {
+        const cd p0 = pb[par][0][c], p1 = pb[par][1][c], p2 = pb[par][2][c], p3 = pb[par][3][c];
+        pc = {tt * (((p0.x + p1.x) + p2.x) + p3.x), tt * (((p0.y + p1.y) + p2.y) + p3.y)};
+      }
+      const double kr = wave_sum(vc.x * pc.x + vc.y * pc.y);  // conj(v) . p
+      const double ki = wave_sum(vc.x * pc.y - vc.y * pc.x);
+      const cd K = {0.5 * tt * kr, 0.5 * tt * ki}, kv = cmul(K, vc);
+      const cd wc = {pc.x - kv.x, pc.y - kv.y};
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int r = 4 * (jb + i) + q;
+        if (r <= t || r >= s) continue;  // (uniform)
+        const cd vr = lane_cd(vc, r), wr = lane_cd(wc, r);
+        W[i].x -= (vr.x * wc.x + vr.y * wc.y) + (wr.x * vc.x + wr.y * vc.y);
+        W[i].y -= (vr.y * wc.x - vr.x * wc.y) + (wr.y * vc.x - wr.x * vc.y);
+      }
     }
+    // row 4 jb + q is final: its diagonal entry (and the last 2 x 2 block's
+    // off-diagonal one, A[n-1][n-2] = conj(A[n-2][n-1])); then shift the window
+    const int r = 4 * jb + q;
+    if (r < s) {
+      const double d = lane_d(W[0].x, r);
+      cd e = {0.0, 0.0};
+      if (r == s - 2) e = lane_cd(W[0], s - 1);
+      if (c == 0) {
+        dg[r] = d;
+        if (r == s - 2) so[r] = {e.x, -e.y};
+      }
+    }
+#pragma unroll
+    for (int i = 0; i + 1 < 16; ++i) W[i] = W[i + 1];
   }
   __syncthreads();
-  for (int t = 0; t + 2 < s; ++t) {
-    const int j = j0 + t, m = s - t - 1;
-    const cd* St = S + t * s + t + 1;  // row t, columns t + 1 ..
-    // the reflector of x = conj(row t)
-    cd xi = {0.0, 0.0};
-    if (tid < m) xi = {St[tid].x, -St[tid].y};
-    double s2 = xi.x * xi.x + xi.y * xi.y, dummy = 0.0;
-    tail_sum2(s2, dummy, red);
-    const cd x0 = {St[0].x, -St[0].y};
-    const double ax0 = hypot(x0.x, x0.y);
-    const double alpha = sqrt(s2);
-    const bool skip = (s2 - ax0 * ax0) <= 0.0;  // x already a multiple of e_1
-    cd ph = {1.0, 0.0};
-    if (ax0 > 0.0) ph = {x0.x / ax0, x0.y / ax0};
-    const double tt = skip ? 0.0 : 1.0 / (alpha * (alpha + ax0));  // 2 / (v^H v)
-    if (tid < m) {
-      cd v = xi;
-      if (tid == 0 && !skip) {
-        v.x += ph.x * alpha;
-        v.y += ph.y * alpha;
-      }
-      sv[tid] = v;
-      V[(size_t)j * n + j + 1 + tid] = skip ? cd{0.0, 0.0} : v;
-    }
-    if (tid == 0) {
-      tau[j] = tt;
-      sub[j] = skip ? x0 : cd{-ph.x * alpha, -ph.y * alpha};
-      diag[j] = S[t * s + t].x;
-    }
-    if (skip) continue;  // (uniform) the trailing block stays as it is
-    __syncthreads();
-    // p_i = tau sum_l S[t+1+i][t+1+l] v_l, one wave per row
-    const cd* S22 = S + (t + 1) * s + t + 1;
-    double pr = 0.0, pim = 0.0;  // this wave's partial conj(v) . p
-    for (int i = wave; i < m; i += NW) {
-      double sr = 0.0, si = 0.0;
-      for (int l = lane; l < m; l += 64) {
-        const cd a = S22[i * s + l], v = sv[l];
-        sr += a.x * v.x - a.y * v.y;
-        si += a.x * v.y + a.y * v.x;
-      }
-      sr = wave_sum(sr) * tt;
-      si = wave_sum(si) * tt;
-      if (lane == 0) {
-        sp[i] = {sr, si};
-        const cd v = sv[i];
-        pr += v.x * sr + v.y * si;
-        pim += v.x * si - v.y * sr;
-      }
-    }
-    tail_sum2(pr, pim, red);  // (its barriers also publish sp)
-    const cd K = {0.5 * tt * pr, 0.5 * tt * pim};
-    if (tid < m) {
-      const cd v = sv[tid], pp = sp[tid], kv = cmul(K, v);
-      sw[tid] = {pp.x - kv.x, pp.y - kv.y};
-    }
-    __syncthreads();
-    for (int q = tid; q < m * m; q += kTailThreads) {
-      const int i = q / m, l = q - i * m;
-      const cd vi = sv[i], wi = sw[i], vl = sv[l], wl = sw[l];
-      cd& a = S[(t + 1 + i) * s + t + 1 + l];
-      a.x -= (vi.x * wl.x + vi.y * wl.y) + (wi.x * vl.x + wi.y * vl.y);
-      a.y -= (vi.y * wl.x - vi.x * wl.y) + (wi.y * vl.x - wi.x * vl.y);
-    }
-    __syncthreads();
+  for (int o = tid; o < 64 * 64; o += 256) {
+    const int t = o >> 6, cc = o & 63;
+    if (t + 2 < s && cc > t && cc < s) V[(size_t)(j0 + t) * n + j0 + cc] = Vs[t][cc];
   }
-  // the last 2 x 2 block (A[n-1][n-2] = conj(A[n-2][n-1]))
-  if (tid == 0) {
-    const cd a = S[(s - 2) * s + s - 1];
-    sub[n - 2] = {a.x, -a.y};
-    diag[n - 2] = S[(s - 2) * s + s - 2].x;
-    diag[n - 1] = S[(s - 1) * s + s - 1].x;
+  for (int t = tid; t < s; t += 256) {
+    if (t + 2 < s) tau[j0 + t] = to[t];
+    if (t + 1 < s) sub[j0 + t] = so[t];
+    diag[j0 + t] = dg[t];
   }
 }
 
@@ -1064,6 +1072,7 @@ __global__ __launch_bounds__(256) void k_inverse_iteration(
     double* __restrict__ z, double* __restrict__ work, EigStride st) {
   __shared__ double red[4][33];
   __shared__ double dots[32];
+  __shared__ double zs[64 * 64];  // n <= 64: the cluster's vectors
   const int c = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const size_t mi = blockIdx.y;
   if (c >= ncl[mi]) return;
@@ -1093,11 +1102,44 @@ __global__ __launch_bounds__(256) void k_inverse_iteration(
     for (int q = first + tid; q < last; q += 256)
       tri_solve(work + (size_t)q * 5 * n, n, z + (size_t)q * n);
     __syncthreads();
+    // Gram-Schmidt once in the early rounds (the next solve amplifies the
+    // wanted directions again), twice in the last
+    const int passes = it + 1 == kInvIters ? 2 : 1;
+    if (n <= 64) {
+      // a vector fits one wave: wave 0 runs the whole cluster from LDS, lane
+      // i holding entry i, with no block barrier between vectors (the same
+      // kGs-row rounds: classical within a round, rounds in order)
+      const int m = last - first;
+      for (int u = tid; u < m * n; u += 256) zs[u] = z[(size_t)first * n + u];
+      __syncthreads();
+      if (wv == 0) {
+        for (int q = 0; q < m; ++q) {
+          double x = lane < n ? zs[q * n + lane] : 0.0;
+          for (int pass = 0; pass < passes; ++pass)
+            for (int r0 = 0; r0 < q; r0 += kGs) {
+              double zr[kGs], dd[kGs];
+#pragma unroll
+              for (int j = 0; j < kGs; ++j) {
+                const bool ok = r0 + j < q && lane < n;
+                const double v = zs[min(r0 + j, q - 1) * n + min(lane, n - 1)];
+                zr[j] = ok ? v : 0.0;
+              }
+#pragma unroll
+              for (int j = 0; j < kGs; ++j) dd[j] = wave_sum(zr[j] * x);
+#pragma unroll
+              for (int j = 0; j < kGs; ++j) x -= dd[j] * zr[j];
+            }
+          const double inv = 1.0 / sqrt(wave_sum(x * x));
+          if (lane < n) zs[q * n + lane] = x * inv;
+        }
+      }
+      __syncthreads();
+      for (int u = tid; u < m * n; u += 256) z[(size_t)first * n + u] = zs[u];
+      __syncthreads();
+      continue;
+    }
     for (int q = first; q < last; ++q) {
       double* x = z + (size_t)q * n;
-      // Gram-Schmidt once in the early rounds (the next solve amplifies the
-      // wanted directions again), twice in the last
-      const int passes = it + 1 == kInvIters ? 2 : 1;
       if (n <= 256 * 5) {
         orthonormalise_reg<5>(x, z, n, first, q, passes, red, dots);
         continue;
@@ -1484,21 +1526,15 @@ int eig_topk_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, i
   if (fused && lds_step > 64 * 1024)
     KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hh_step),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_step));
-  const size_t lds_tail = (size_t)(kTail * kTail + 3 * kTail) * 16;
-  if (fused)
-    KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hh_tail),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_tail));
   int np_prev = 0;
   bool tail = false;
   for (int j = 0; j + 2 < n; ++j) {
     const int m = n - j - 1;
     if (fused && n - j <= kTail) {  // the remaining steps in one block per matrix
-      const int sz = n - j;
       const cd* pp = p + (size_t)((j + 1) & 1) * n;
       const cd* partp = part + (size_t)((j + 1) & 1) * nparts_max;
-      KLT_LAUNCH(ctx, "k_hh_tail", k_hh_tail, dim3(1, count), dim3(kTailThreads),
-                 (size_t)(sz * sz + 3 * sz) * 16, A, n, j, V, tau, sub, diag, pp, partp,
-                 np_prev, es);
+      KLT_LAUNCH(ctx, "k_hh_tail", k_hh_tail, dim3(1, count), dim3(256), 0, A, n, j, V, tau,
+                 sub, diag, pp, partp, np_prev, es);
       tail = true;
       break;
     }

@@ -22,7 +22,9 @@
 // Householder path.  Every kernel takes the batch in its last grid dimension
 // and a per-matrix `active` flag, so each matrix runs the same arithmetic and
 // stops at the same round alone or in a batch; the split-K factor of the
-// GEMMs depends on n only.
+// GEMMs depends on n only.  The products (A X, the tall n x 64 times 64 x 64
+// ones and the Grams) run on the matrix cores in FP64 (k_zmm), the Cholesky
+// factor and its inverse in registers of one block (k_chol_inv).
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -34,90 +36,99 @@ namespace {
 typedef double2 cd;
 
 constexpr int kP = 64;          // block size (columns)
-constexpr int kFirstCheck = 9;  // rounds of X <- orth(A^2 X) before the first check
+constexpr int kFirstCheck = 11;  // rounds of X <- orth(A^2 X) before the first check
 constexpr int kCheckEvery = 3;  // rounds between later checks
 constexpr int kMaxRounds = 24;
 constexpr double kTol = 5e-14;  // residual, relative to theta_0
-constexpr int kT = 64;          // GEMM tile (rows, columns)
-constexpr int kK = 16;          // GEMM K step
 
-__device__ __forceinline__ cd cfma(cd a, cd b, cd c) {  // c + a b
-  c.x = fma(a.x, b.x, fma(-a.y, b.y, c.x));
-  c.y = fma(a.x, b.y, fma(a.y, b.x, c.y));
-  return c;
-}
+typedef double d4 __attribute__((ext_vector_type(4)));
 
-// C[b] = A[b] B[b] (complex, row-major; A: n x K with row stride lda, B: K x p,
-// or with TB its transpose stored p x K; C: n x p).  grid (ceil(n / 64),
-// nsplit, count); block = 256 threads, a 64 x 64 output tile, thread (ty, tx)
-// owns rows ty + 16 i, columns tx + 16 j.  nsplit > 1: the block's share of K
-// goes to part[b][ks] (n x p) and k_zgemm_reduce sums the shares.
-template <bool TB>
-__global__ __launch_bounds__(256) void k_zgemm(const cd* __restrict__ A, int lda, size_t sA,
-                                               const cd* __restrict__ B, int ldb, size_t sB,
-                                               cd* __restrict__ C, size_t sC, int n, int K,
-                                               int p, int kchunk, cd* __restrict__ part,
-                                               const int* __restrict__ active) {
+// C = op(A) op(B) (complex FP64) on the matrix cores, v_mfma_f64_16x16x4f64:
+// lane l supplies a = op(A)[l % 16][l / 16] and b = op(B)[l / 16][l % 16] and
+// holds D[l / 16 + 4 v][l % 16], v < 4.  A complex product takes four of
+// them: re += ar br, re' += ai bi, im += ar bi + ai br (re - re' at the end).
+// op(A) is M x K: A row-major (lda), or with AT the conjugate transpose of a
+// K x M row-major A.  op(B) is K x Q, Q <= 64: B row-major (ldb), or with BT
+// the transpose of a Q x K row-major B.  Block: 256 threads, a tile of 16 rows
+// x 64 columns; wave w takes k-steps (of 4) w, w + 4, ... of the block's share
+// [k0, k1) of K, and the four waves' sums are added in wave order through LDS.
+// grid (ceil(M / 16), nsplit, count), kchunk a multiple of 16; nsplit > 1:
+// block (x, y) writes part[b][y] (M x 64, row stride 64), else C (row stride
+// ldc).  Loads are clamped in bounds and zeroed outside.
+template <bool AT, bool BT>
+__global__ __launch_bounds__(256, 2) void k_zmm(const cd* __restrict__ A, int lda, size_t sA,
+                                             const cd* __restrict__ B, int ldb, size_t sB,
+                                             cd* __restrict__ C, int ldc, size_t sC, int M,
+                                             int K, int Q, int kchunk, cd* __restrict__ part,
+                                             const int* __restrict__ active) {
   const int b = blockIdx.z;
   if (active && !active[b]) return;
-  __shared__ cd sA_[kT][kK + 1];
-  __shared__ cd sB_[kK][kT + 1];
-  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
-  const int r0 = blockIdx.x * kT, ks = blockIdx.y, nsplit = gridDim.y;
-  const int k0 = ks * kchunk, k1 = min(K, k0 + kchunk);
+  __shared__ cd red[4][16][64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int r0 = blockIdx.x * 16, ys = blockIdx.y, nsplit = gridDim.y;
+  const int k0 = ys * kchunk, k1 = min(K, k0 + kchunk);
   A += (size_t)b * sA;
   B += (size_t)b * sB;
-  cd acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = make_double2(0.0, 0.0);
-  for (int kb = k0; kb < k1; kb += kK) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int e = tid + 256 * u;
-      const int row = e >> 4, kk = e & 15;
-      const int gr = r0 + row, gk = kb + kk;
-      sA_[row][kk] = (gr < n && gk < k1) ? A[(size_t)gr * lda + gk] : make_double2(0.0, 0.0);
-      if (TB) {
-        const int q = e >> 4, j = e & 15;
-        const int gj = kb + j;
-        sB_[j][q] = (q < p && gj < k1) ? B[(size_t)q * ldb + gj] : make_double2(0.0, 0.0);
-      } else {
-        const int kr = e >> 6, col = e & 63;
-        const int gk2 = kb + kr;
-        sB_[kr][col] = (col < p && gk2 < k1) ? B[(size_t)gk2 * ldb + col] : make_double2(0.0, 0.0);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < kK; ++kk) {
-      cd a[4], bb[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = sA_[ty + 16 * i][kk];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) bb[j] = sB_[kk][tx + 16 * j];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = cfma(a[i], bb[j], acc[i][j]);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int r = r0 + ty + 16 * i;
-    if (r >= n) continue;
+  const int row = r0 + lr, rowc = min(row, M - 1);
+  auto load = [&](int ks, cd& a, cd (&bv)[4]) {
+    const int k = ks + lk, kc = min(k, K - 1);
+    const bool kin = k < k1;
+    cd av = AT ? A[(size_t)kc * lda + rowc] : A[(size_t)rowc * lda + kc];
+    if (AT) av.y = -av.y;
+    a = (kin && row < M) ? av : make_double2(0.0, 0.0);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int c = tx + 16 * j;
-      if (c >= p) continue;
-      const size_t o = (size_t)r * p + c;
-      if (nsplit > 1)
-        part[((size_t)b * nsplit + ks) * n * p + o] = acc[i][j];
-      else
-        C[(size_t)b * sC + o] = acc[i][j];
+      const int col = 16 * j + lr, colc = min(col, Q - 1);
+      const cd bj = BT ? B[(size_t)colc * ldb + kc] : B[(size_t)kc * ldb + colc];
+      bv[j] = (kin && col < Q) ? bj : make_double2(0.0, 0.0);
     }
+  };
+  d4 re1[4], re2[4], im[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) re1[j] = re2[j] = im[j] = d4{0.0, 0.0, 0.0, 0.0};
+  auto mma = [&](const cd& a, const cd (&bv)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) re1[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bv[j].x, re1[j], 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) re2[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bv[j].y, re2[j], 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) im[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bv[j].y, im[j], 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) im[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bv[j].x, im[j], 0, 0, 0);
+  };
+  // this wave's steps i = 0 .. nst - 1 at k = k0 + 4 wave + 16 i (zeros past
+  // k1); operands of four steps in a register ring, loaded three steps ahead
+  const int nst = (k1 - k0 + 15) / 16, kw = k0 + 4 * wave;
+  cd a[4], bv[4][4];
+#pragma unroll
+  for (int u = 0; u < 3; ++u) load(kw + 16 * u, a[u], bv[u]);
+  int i = 0;
+  for (; i + 4 <= nst; i += 4) {  // sets 0, 1, 2 hold steps i, i + 1, i + 2 here
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      load(kw + 16 * (i + u + 3), a[(u + 3) & 3], bv[(u + 3) & 3]);
+      mma(a[u], bv[u]);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 3; ++u)
+    if (i + u < nst) mma(a[u], bv[u]);
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      red[wave][lk + 4 * v][16 * j + lr] = make_double2(re1[j][v] - re2[j][v], im[j][v]);
+  __syncthreads();
+  cd* out = nsplit > 1 ? part + ((size_t)b * nsplit + ys) * M * 64 : C + (size_t)b * sC;
+  const int ldo = nsplit > 1 ? 64 : ldc;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int e = tid + 256 * u, rr = e >> 6, col = e & 63;
+    const cd v0 = red[0][rr][col], v1 = red[1][rr][col], v2 = red[2][rr][col],
+             v3 = red[3][rr][col];
+    const cd v = make_double2(((v0.x + v1.x) + v2.x) + v3.x, ((v0.y + v1.y) + v2.y) + v3.y);
+    if (r0 + rr < M && col < Q) out[(size_t)(r0 + rr) * ldo + col] = v;
   }
 }
 
@@ -137,60 +148,6 @@ __global__ __launch_bounds__(256) void k_zgemm_reduce(const cd* __restrict__ par
     s.y += v.y;
   }
   C[(size_t)b * sC + o] = s;
-}
-
-// Partial Gram G_part[b][chunk] = X[chunk rows]^H Y[chunk rows] (p x p, p <= 64);
-// grid (ceil(n / 64), 1, count).
-__global__ __launch_bounds__(256) void k_gram(const cd* __restrict__ X, const cd* __restrict__ Y,
-                                              size_t s, int n, int p, cd* __restrict__ part,
-                                              const int* __restrict__ active) {
-  const int b = blockIdx.z;
-  if (active && !active[b]) return;
-  __shared__ cd sX[kK][kT + 1];
-  __shared__ cd sY[kK][kT + 1];
-  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
-  const int r0 = blockIdx.x * kT, r1 = min(n, r0 + kT);
-  X += (size_t)b * s;
-  Y += (size_t)b * s;
-  cd acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = make_double2(0.0, 0.0);
-  for (int rb = r0; rb < r1; rb += kK) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int e = tid + 256 * u, rr = e >> 6, col = e & 63, gr = rb + rr;
-      const bool ok = gr < r1 && col < p;
-      sX[rr][col] = ok ? X[(size_t)gr * p + col] : make_double2(0.0, 0.0);
-      sY[rr][col] = ok ? Y[(size_t)gr * p + col] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int rr = 0; rr < kK; ++rr) {
-      cd a[4], bb[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        a[i] = sX[rr][ty + 16 * i];
-        a[i].y = -a[i].y;  // conj(X)
-      }
-#pragma unroll
-      for (int j = 0; j < 4; ++j) bb[j] = sY[rr][tx + 16 * j];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = cfma(a[i], bb[j], acc[i][j]);
-    }
-    __syncthreads();
-  }
-  cd* out = part + ((size_t)b * gridDim.x + blockIdx.x) * p * p;
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int r = ty + 16 * i, c = tx + 16 * j;
-      if (r < p && c < p) out[r * p + c] = acc[i][j];
-    }
 }
 
 // G[b] = sum over chunks (fixed order); herm: G <- (G + G^H) / 2.
@@ -217,77 +174,97 @@ __global__ __launch_bounds__(256) void k_gram_reduce(const cd* __restrict__ part
   G[(size_t)b * p * p + o] = s;
 }
 
-// Per matrix (grid (1, 1, count), 256 threads): G = L L^H (left-looking
-// Cholesky in LDS: for column j, thread (i, part) sums a quarter of
-// sum_{l<j} L[i][l] conj(L[j][l]), then row i's entry is finished and scaled)
-// and Binv = L^{-H} (upper triangular, row-major p x p) for Y <- Y Binv, from
-// M = L^{-1} row by row (M[i][c] = (delta_ic - sum_{c<=l<i} L[i][l] M[l][c]) /
-// L[i][i], split the same way); fail[b] = 1 when a pivot is not positive.
-__global__ __launch_bounds__(256) void k_chol_inv(const cd* __restrict__ G, int p,
-                                                  cd* __restrict__ Binv, int* __restrict__ fail,
-                                                  const int* __restrict__ active) {
+// Per matrix (grid (1, 1, count), kCW waves): G = L L^H and Binv = L^{-H}
+// (upper triangular, row-major p x p, p <= 64) for Y <- Y Binv.  G is padded
+// with the identity to 64 x 64 and held in registers: wave q, lane c keeps
+// W[i][c] = row kCW (jb + i) + q, column c, during group jb of kCW steps.
+// Step j of the right-looking factorisation turns W into M = L^{-1} one row
+// and column at a time, in place: the wave holding row j (which is,
+// conjugated, the trailing column j of the Hermitian remainder) forms
+// l_jj = sqrt(W[j][j]) and u[c] = W[j][c] / l_jj (u[j] = 1 / l_jj), so that
+// u[c] = conj(L[c][j]) for c > j and u[c] = M[j][c] for c <= j; row j becomes
+// M[j][.] (zero right of the diagonal); then every row r > j takes
+// W[r][c] -= L[r][j] u[c], L[r][j] = conj(u[r]) (column j first cleared),
+// which is the trailing update G'[r][c] -= L[r][j] conj(L[c][j]) for c > j and
+// the forward substitution M[r][c] -= L[r][j] M[j][c] for c <= j.  After a
+// group its kCW rows are final: they go to LDS (Binv is written at the end)
+// and the register window moves down one row per wave (static register
+// indices throughout).  The wave holding row j + 1 updates that row first
+// and forms step j + 1's pivot before its other rows (look-ahead).  One
+// barrier a step (u double-buffered); fail[b] = 1 when a pivot is not
+// positive.  Sixteen waves, four a SIMD: the steps are chains of dependent
+// FP64 operations, and several waves a SIMD overlap them.
+constexpr int kCW = 16;        // waves
+constexpr int kCR = kP / kCW;  // rows a wave
+__global__ __launch_bounds__(kCW * 64) void k_chol_inv(const cd* __restrict__ G, int p,
+                                                       cd* __restrict__ Binv,
+                                                       int* __restrict__ fail,
+                                                       const int* __restrict__ active) {
   const int b = blockIdx.z;
   if (active && !active[b]) return;
-  __shared__ cd L[kP][kP + 1];
-  __shared__ cd M[kP][kP + 1];
-  __shared__ cd red[4][kP];
-  __shared__ int bad;
-  const int tid = threadIdx.x, r = tid & 63, part = tid >> 6;
-  for (int o = tid; o < p * p; o += 256) L[o / p][o % p] = G[(size_t)b * p * p + o];
-  if (tid == 0) bad = 0;
+  __shared__ cd ub[2][kP];
+  __shared__ cd mt[kP][kP + 1];  // mt[c][r] = conj(M[r][c]), written out at the end
+  const int tid = threadIdx.x, c = tid & 63;
+  const int q = __builtin_amdgcn_readfirstlane(tid >> 6);  // the wave (uniform)
+  const cd* Gb = G + (size_t)b * p * p;
+  cd* Bo = Binv + (size_t)b * p * p;
+  cd W[kCR];
+#pragma unroll
+  for (int i = 0; i < kCR; ++i) {
+    const int r = q + kCW * i;
+    W[i] = (r < p && c < p) ? Gb[(size_t)r * p + c] : make_double2(r == c ? 1.0 : 0.0, 0.0);
+  }
+  bool bad = false;
+  // step jj's pivot on row jj (register Wr of this wave): u and row jj of M
+  auto pivot = [&](cd& Wr, int jj, cd* u) {
+    const cd g = Wr;
+    const int lo = __builtin_amdgcn_readlane(__double2loint(g.x), jj);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(g.x), jj);
+    const double d = __hiloint2double(hi, lo);
+    if (!(d > 0.0)) bad = true;
+    const double inv = 1.0 / (d > 0.0 ? sqrt(d) : 1.0);
+    const cd gi = make_double2(g.x * inv, g.y * inv);
+    u[c] = c == jj ? make_double2(inv, 0.0) : gi;
+    Wr = c < jj ? gi : make_double2(c == jj ? inv : 0.0, 0.0);
+  };
+  auto update = [&](cd& Wr, const cd& ur, const cd& uc, int j) {  // L[r][j] = conj(ur)
+    cd w = c == j ? make_double2(0.0, 0.0) : Wr;
+    w.x -= ur.x * uc.x + ur.y * uc.y;
+    w.y -= ur.x * uc.y - ur.y * uc.x;
+    Wr = w;
+  };
+  if (q == 0) pivot(W[0], 0, ub[0]);
+  for (int jb = 0; jb < kCR; ++jb) {
+#pragma unroll
+    for (int jq = 0; jq < kCW; ++jq) {
+      const int j = kCW * jb + jq;
+      const cd* u = ub[jq & 1];
+      __syncthreads();  // u of step j is complete
+      const cd uc = u[c];
+      // the wave holding row j + 1 (W[in] of wave wn) updates it first and
+      // forms step j + 1's pivot, ahead of its other rows
+      const int wn = (jq + 1) % kCW, in = jq + 1 < kCW ? 0 : 1;
+      if (q == wn && j + 1 < kP) {
+        update(W[in], u[j + 1], uc, j);
+        pivot(W[in], j + 1, ub[(jq + 1) & 1]);
+      }
+#pragma unroll
+      for (int i = 0; i < kCR; ++i) {
+        // row r = kCW (jb + i) + q, updated when j < r < 64 (uniform)
+        if (i == 0 ? q <= jq : jb + i >= kCR) continue;
+        if (q == wn && i == in) continue;  // done above
+        update(W[i], u[kCW * (jb + i) + q], uc, j);
+      }
+    }
+    // row kCW jb + q is final (staged in LDS, so that no barrier waits for
+    // global stores); then shift the window
+    mt[c][kCW * jb + q] = make_double2(W[0].x, -W[0].y);
+#pragma unroll
+    for (int i = 0; i + 1 < kCR; ++i) W[i] = W[i + 1];
+  }
   __syncthreads();
-  for (int j = 0; j < p; ++j) {
-    cd s = make_double2(0.0, 0.0);
-    if (r >= j && r < p)
-      for (int l = part; l < j; l += 4) {
-        const cd x = L[r][l], y = L[j][l];  // x conj(y)
-        s.x += x.x * y.x + x.y * y.y;
-        s.y += x.y * y.x - x.x * y.y;
-      }
-    red[part][r] = s;
-    __syncthreads();
-    // every finishing thread forms the pivot itself (no barrier for it)
-    if (part == 0 && r >= j && r < p) {
-      const double d = L[j][j].x - red[0][j].x - red[1][j].x - red[2][j].x - red[3][j].x;
-      const double ljj = d > 0.0 ? sqrt(d) : 1.0;
-      if (r == j) {
-        if (!(d > 0.0)) bad = 1;
-        L[j][j] = make_double2(ljj, 0.0);
-      } else {
-        const cd g = L[r][j];
-        const double inv = 1.0 / ljj;
-        L[r][j] = make_double2((g.x - red[0][r].x - red[1][r].x - red[2][r].x - red[3][r].x) * inv,
-                               (g.y - red[0][r].y - red[1][r].y - red[2][r].y - red[3][r].y) * inv);
-      }
-    }
-    __syncthreads();
-  }
-  for (int i = 0; i < p; ++i) {
-    // M[i][c] for c = r <= i
-    cd s = make_double2(0.0, 0.0);
-    if (r < i)
-      for (int l = r + part; l < i; l += 4) s = cfma(L[i][l], M[l][r], s);
-    red[part][r] = s;
-    __syncthreads();
-    if (part == 0 && r <= i) {
-      const double inv = 1.0 / L[i][i].x;
-      if (r == i) {
-        M[i][i] = make_double2(inv, 0.0);
-      } else {
-        const cd t = make_double2(red[0][r].x + red[1][r].x + red[2][r].x + red[3][r].x,
-                                  red[0][r].y + red[1][r].y + red[2][r].y + red[3][r].y);
-        M[i][r] = make_double2(-t.x * inv, -t.y * inv);
-      }
-    }
-    if (part == 1 && r > i && r < p) M[i][r] = make_double2(0.0, 0.0);
-    __syncthreads();
-  }
-  for (int o = tid; o < p * p; o += 256) {  // Binv[r][q] = conj(M[q][r])
-    const int rr = o / p, q = o % p;
-    const cd v = M[q][rr];
-    Binv[(size_t)b * p * p + o] = make_double2(v.x, -v.y);
-  }
-  if (tid == 0 && bad) fail[b] = 1;
+  for (int o = tid; o < p * p; o += kCW * 64) Bo[o] = mt[o / p][o % p];  // Binv[c][r]
+  if (bad && c == 0) fail[b] = 1;
 }
 
 // res[b][q] = || Z[:, q] - theta[b][q] X[:, q] ||, q < k; grid (k, 1, count)
@@ -354,11 +331,18 @@ int eig_topk_psd_batch_device(klt_ctx* ctx, const double* a_dev, int count, int 
   const int p = kP;
   if (k + 16 > p || n < 2 * p) return set_error(KLT_ERR_EIG, "psd subspace: shape not covered");
   const size_t np = (size_t)n * p, nk = (size_t)n * k, B = (size_t)count, pp = (size_t)p * p;
-  const int rowtiles = (n + kT - 1) / kT;
-  // split-K from n alone (identical arithmetic in batched and single calls)
-  int nsplit = std::max(1, std::min(16, (512 + rowtiles - 1) / rowtiles));
-  const int kchunk = ((n + nsplit - 1) / nsplit + kK - 1) / kK * kK;
+  // 16-row tiles; split-K of A M from n alone (identical arithmetic in
+  // batched and single calls), about 512 blocks a product
+  const int rt = (n + 15) / 16;
+  // about 768 blocks a product (measured best between 256 and 1024 at
+  // n = 1225); chunks of 64 k, four steps of each wave
+  int nsplit = std::max(1, std::min(16, 768 / rt));
+  const int kchunk = ((n + nsplit - 1) / nsplit + 63) / 64 * 64;
   nsplit = (n + kchunk - 1) / kchunk;
+  int nsg = 1, kchg = n;
+  nsg = std::min(16, (n + 15) / 16);  // Gram: 16 shares of the n rows
+  kchg = ((n + nsg - 1) / nsg + 15) / 16 * 16;
+  nsg = (n + kchg - 1) / kchg;
   size_t off = 0;
   auto carve = [&](size_t bytes) {
     const size_t o = off;
@@ -367,7 +351,7 @@ int eig_topk_psd_batch_device(klt_ctx* ctx, const double* a_dev, int count, int 
   };
   const size_t oX = carve(B * np * 16), oY0 = carve(B * np * 16), oY1 = carve(B * np * 16),
                oZ = carve(B * np * 16), oPart = carve(B * nsplit * np * 16),
-               oGp = carve(B * rowtiles * pp * 16), oG = carve(B * pp * 16),
+               oGp = carve(B * nsg * pp * 16), oG = carve(B * pp * 16),
                oBi = carve(B * pp * 16), oU = carve(B * k * p * 16), oTh = carve(B * k * 8),
                oRes = carve(B * k * 8), oFail = carve(B * 4), oAct = carve(B * 4);
   KLT_TRY(ctx->sub_ws.ensure(off));
@@ -393,8 +377,8 @@ int eig_topk_psd_batch_device(klt_ctx* ctx, const double* a_dev, int count, int 
   const dim3 gblk(256);
   // C = A M (n x p blocks)
   auto gemm_A = [&](const cd* M, cd* C) {
-    KLT_LAUNCH(ctx, "k_zgemm", k_zgemm<false>, dim3(rowtiles, nsplit, count), gblk, 0, Ad, n,
-               (size_t)n * n, M, p, np, C, np, n, n, p, kchunk, part, act);
+    KLT_LAUNCH(ctx, "k_zgemm", (k_zmm<false, false>), dim3(rt, nsplit, count), gblk, 0, Ad, n,
+               (size_t)n * n, M, p, np, C, p, np, n, n, p, kchunk, part, act);
     if (nsplit > 1)
       KLT_LAUNCH(ctx, "k_zgemm_reduce", k_zgemm_reduce,
                  dim3((unsigned)((np + 255) / 256), 1, count), gblk, 0, part, nsplit, n, p, C,
@@ -403,22 +387,24 @@ int eig_topk_psd_batch_device(klt_ctx* ctx, const double* a_dev, int count, int 
   // C (n x q) = Yin (n x p) Bm, Bm p x q (tb: Bm holds its transpose, q x p)
   auto gemm_small = [&](const cd* Yin, const cd* Bm, bool tb, int q, cd* C) {
     if (tb)
-      KLT_LAUNCH(ctx, "k_zgemm", k_zgemm<true>, dim3(rowtiles, 1, count), gblk, 0, Yin, p, np,
-                 Bm, p, (size_t)q * p, C, (size_t)n * q, n, p, q, p, part, act);
+      KLT_LAUNCH(ctx, "k_zgemm_tall", (k_zmm<false, true>), dim3(rt, 1, count), gblk, 0, Yin, p,
+                 np, Bm, p, (size_t)q * p, C, q, (size_t)n * q, n, p, q, kP, part, act);
     else
-      KLT_LAUNCH(ctx, "k_zgemm", k_zgemm<false>, dim3(rowtiles, 1, count), gblk, 0, Yin, p, np,
-                 Bm, q, (size_t)p * q, C, (size_t)n * q, n, p, q, p, part, act);
+      KLT_LAUNCH(ctx, "k_zgemm_tall", (k_zmm<false, false>), dim3(rt, 1, count), gblk, 0, Yin,
+                 p, np, Bm, q, (size_t)p * q, C, q, (size_t)n * q, n, p, q, kP, part, act);
   };
+  // G = herm(P^H Q) (p x p)
   auto gram = [&](const cd* P, const cd* Q) {
-    KLT_LAUNCH(ctx, "k_gram", k_gram, dim3(rowtiles, 1, count), gblk, 0, P, Q, np, n, p, Gp, act);
+    KLT_LAUNCH(ctx, "k_gram", (k_zmm<true, false>), dim3(p / 16, nsg, count), gblk, 0, P, p, np,
+               Q, p, np, Gp, p, pp, p, n, p, kchg, Gp, act);
     KLT_LAUNCH(ctx, "k_gram_reduce", k_gram_reduce, dim3((unsigned)((pp + 255) / 256), 1, count),
-               gblk, 0, Gp, rowtiles, p, G, true, act);
+               gblk, 0, Gp, nsg, p, G, true, act);
   };
   // one CholQR pass: Xout = Yin L^{-H}, Yin^H Yin = L L^H (Xout distinct from
   // Yin); two passes (CholQR2) orthonormalise to rounding level
   auto cholqr = [&](const cd* Yin, cd* Xout) {
     gram(Yin, Yin);
-    KLT_LAUNCH(ctx, "k_chol_inv", k_chol_inv, dim3(1, 1, count), dim3(256), 0, G, p, Bi, fail,
+    KLT_LAUNCH(ctx, "k_chol_inv", k_chol_inv, dim3(1, 1, count), dim3(kCW * 64), 0, G, p, Bi, fail,
                act);
     gemm_small(Yin, Bi, false, p, Xout);
   };

@@ -175,12 +175,12 @@ def test_eig_batch_equals_single_calls(ctx):
     _check(mats[1], wb[1], _c(vb[1]), wo, vo, k)
 
 
-@pytest.mark.parametrize("n,count", [(40, 3), (96, 2), (97, 2), (99, 1)])
+@pytest.mark.parametrize("n,count", [(3, 2), (40, 3), (64, 2), (65, 2), (67, 1), (97, 2)])
 def test_eig_small_orders_and_tail(ctx, n, count):
-    """Orders around the last-columns block (the final 96 columns run in one
-    LDS-resident block per matrix): n = 40 and 96 run entirely there, 97 and
-    99 take one and three device-wide steps first.  Batched results equal
-    single calls bit for bit and match the oracle."""
+    """Orders around the last-columns block (the final 64 columns run in one
+    register-resident block per matrix): n = 3, 40 and 64 run entirely there,
+    65, 67 and 97 take 1, 3 and 33 device-wide steps first.  Batched results
+    equal single calls bit for bit and match the oracle."""
     print(f"case eig small n={n} count={count}", flush=True)
     rng = np.random.default_rng(20 + n)
     mats = []
