@@ -46,6 +46,7 @@ import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 MFMA_F16_PEAK_TFLOPS = 2500.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 L = 1024
 NK = 24
 KSIZE = 29
@@ -236,6 +237,14 @@ def bench_tcc_eig(ctx, timer, torch):
                          float(np.linalg.norm(b - a) / np.linalg.norm(a)),
                      "lambda_max_rel_err_split16": float(np.abs(w16 - w64).max() / w64[0]),
                      "lambda0": float(w64[0])})
+    # the PSD path's A X products (n x n times n x 64, complex: 8 flops a MAC)
+    zg = eig_kernels["subspace"]["k_zgemm"]
+    zflops = 8.0 * T * T * 64
+    out["eig_psd_products"] = zg["calls"]
+    out["eig_psd_zgemm_tflops"] = zflops * zg["calls"] / (zg["ms"] / 1e3) / 1e12
+    out["eig_psd_zgemm_frac_fp64_peak"] = out["eig_psd_zgemm_tflops"] / FP64_PEAK_TFLOPS
+    out["eig_psd_product_tflops_whole_solve"] = (zflops * zg["calls"] /
+                                                 (rows[2]["eig_top32_ms"] / 1e3) / 1e12)
     # the five FP64 TCCs in one batched solve (every launch covers the five)
     ctx.eig_topk_batch(t5, 32, psd=True)
     ms_batch = timer(lambda: ctx.eig_topk_batch(t5, 32, psd=True), 1)
@@ -251,12 +260,32 @@ def bench_tcc_eig(ctx, timer, torch):
                 "split16_mfma_utilisation": 3 * flops / (best / 1e3) / 1e12 / MFMA_F16_PEAK_TFLOPS,
                 "eig_top32_kernels_at_0nm": eig_kernels,
                 "eig_note": "eig_top32_ms: the positive-semidefinite path (KLT_EIG_PSD, "
-                            "Chebyshev-filtered subspace iteration) that klt_calc_kernels uses "
-                            "for TCCs; eig_top32_householder_ms: the general Hermitian path",
+                            "block subspace iteration on A^2 with CholQR and Rayleigh-Ritz) "
+                            "that the TCC tool and the source-mask loop use; "
+                            "eig_top32_householder_ms: the general Hermitian path; "
+                            "eig_psd_zgemm_tflops: the A X products (k_zgemm, FP64 MFMA) "
+                            "over their own kernel time, against the FP64 peak",
                 "note": "tcc_*_ms: whole klt_tcc_build calls (source set-up on the host, pupil, "
                         "operands, Gram) into device memory; gram_split16_kernel_ms: the Gram "
                         "and its split-K reduction kernels alone (HIP events)"})
     return out
+
+
+def section_eig_psd(ctx, timer, torch, steps, warmup):
+    """Top-32 FP64 eigensolve of the T = 1225 blurred-dipole TCC (no defocus) on
+    the positive-semidefinite path, alone: `steps` timed solves after `warmup`."""
+    src = blurred_dipole()
+    sig = min(klt.outer_sigma(src), 1.0)
+    Nx, Ny = klt.tcc_sizing(W.NA, W.PERIOD_35, W.PERIOD_35, sig)
+    T = (2 * Nx + 1) * (2 * Ny + 1)
+    dsrc = torch.from_numpy(src).cuda()
+    t64 = torch.empty((T, T, 2), dtype=torch.float64, device="cuda")
+    ctx.tcc_build(dsrc, sig, W.NA, 0.0, W.PERIOD_35, W.PERIOD_35, Nx, Ny, 0, NMED, out=t64)
+    for _ in range(warmup):
+        ctx.eig_topk(t64, 32, psd=True)
+    ms = timer(lambda: ctx.eig_topk(t64, 32, psd=True), steps)
+    log(f"eig_psd T={T}: {ms:.3f} ms per top-32 solve")
+    return {"section": "eig_psd", "T": T, "k": 32, "solves": steps, "ms_per_solve": ms}
 
 
 def bench_adjoint(ctx, bank, timer, torch, dev, src):
@@ -388,6 +417,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline and plain path only")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--section", choices=["all", "eig_psd"], default="all",
+                    help="eig_psd: only --steps top-32 solves of the T = 1225 blurred-dipole "
+                         "TCC on the positive-semidefinite path, their timing as a JSON line "
+                         "(for profiling; no bench line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the SOCS "
@@ -403,6 +436,9 @@ def main():
     ctx.set_stream(stream.cuda_stream)
     torch.cuda.set_stream(stream)
     timer = Timer(torch, stream)
+    if args.section == "eig_psd":
+        print(json.dumps(section_eig_psd(ctx, timer, torch, args.steps, args.warmup)))
+        return
 
     t0 = time.perf_counter()
     src, kr, ki, sc = gpu_kernels(ctx)
@@ -491,7 +527,8 @@ def main():
             f"{100 * tce['split16_mfma_utilisation']:.1f} %; eig top-32 "
             f"{tce['rows'][2]['eig_top32_ms']:.2f} ms (Householder path "
             f"{tce['rows'][2]['eig_top32_householder_ms']:.1f} ms), five in one batched call "
-            f"{tce['eig_top32_batch5_ms']:.1f} ms")
+            f"{tce['eig_top32_batch5_ms']:.1f} ms; A X products {tce['eig_psd_zgemm_tflops']:.1f} "
+            f"TF/s FP64 ({100 * tce['eig_psd_zgemm_frac_fp64_peak']:.0f} % of peak)")
         extras["tcc_eig"] = tce
         bu = bench_batch_uneven(ctx, timer, torch, src)
         log(f"uneven batch: {bu['images_per_s']:.0f} images/s ({bu['ms_per_batch']:.3f} ms per "

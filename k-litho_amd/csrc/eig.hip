@@ -457,20 +457,23 @@ __global__ __launch_bounds__(kThreads) void k_hh_her2(double* __restrict__ A, in
 // once the trailing block A[j0.., j0..] has s = n - j0 <= kTail rows: the
 // launches of those steps would each cost the fixed latency of a device-wide
 // step for a few rows of work.  The trailing block, padded with zeros to
-// 64 x 64, is held in registers: wave q, lane c keeps row 4 (jb + i) + q,
-// column c, in W[i] during group jb of four steps (the window moves down one
-// row per wave after each group, so that register indices stay static).  The
+// 64 x 64, is held in registers of kTW waves: wave q, lane c keeps row
+// kTW (jb + i) + q, column c, in W[i] during group jb of kTW steps (the window
+// moves down one row per wave after each group, so that register indices
+// stay static).  The
 // block applies step j0-1's pending rank-2 update (as k_hh_step would), then
 // runs steps j0 .. n-3 with the same reflector conventions (V rows, tau, sub,
 // diag) and records the last 2 x 2 block (as k_hh_finish).  Step t (row t of
 // the block): the wave holding row t forms the reflector of x = conj(row t)
 // right of the diagonal and publishes v and tau through LDS; every wave sums
 // its rows of p = tau S22 v as column sums (S22 is Hermitian, so
-// p_c = tau sum_r conj(S[r][c]) v_r), the four waves' sums are added in
-// order through LDS, each wave forms w = p - (tau / 2)(v^H p) v in its lanes
+// p_c = tau sum_r conj(S[r][c]) v_r), the waves' sums are added in order
+// through LDS, each wave forms w = p - (tau / 2)(v^H p) v in its lanes
 // and updates its rows, S[r][c] -= v_r conj(w_c) + w_r conj(v_c), taking v_r
 // and w_r from lane r.  Outputs are staged in LDS and written at the end.
 constexpr int kTail = 64;
+constexpr int kTW = 4;           // waves (measured: 16 is slower at n = 64)
+constexpr int kTR = 64 / kTW;    // rows a wave
 
 __device__ __forceinline__ double lane_d(double v, int l) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
@@ -479,18 +482,18 @@ __device__ __forceinline__ double lane_d(double v, int l) {
 }
 __device__ __forceinline__ cd lane_cd(cd v, int l) { return {lane_d(v.x, l), lane_d(v.y, l)}; }
 
-__global__ __launch_bounds__(256) void k_hh_tail(
+__global__ __launch_bounds__(kTW * 64) void k_hh_tail(
     const double* __restrict__ A, int n, int j0, cd* __restrict__ V,
     double* __restrict__ tau, cd* __restrict__ sub, double* __restrict__ diag,
     const cd* __restrict__ p_prev, const cd* __restrict__ part_prev, int nparts_prev,
     EigStride st) {
   __shared__ cd vb[2][64];     // v of the step (double-buffered)
   __shared__ double tb[2];     // tau of the step
-  __shared__ cd pb[2][4][64];  // the waves' column sums of S22 v
+  __shared__ cd pb[2][kTW][64];  // the waves' column sums of S22 v
   __shared__ cd Vs[64][64];    // row t: v of step t
   __shared__ cd so[64];        // sub
   __shared__ double to[64], dg[64];
-  __shared__ double red[8];
+  __shared__ double red[2 * kTW];
   const int tid = threadIdx.x, c = tid & 63;
   const int q = __builtin_amdgcn_readfirstlane(tid >> 6);  // the wave (uniform)
   {
@@ -505,38 +508,50 @@ __global__ __launch_bounds__(256) void k_hh_tail(
   }
   const int s = n - j0;
   const cd* Ag = reinterpret_cast<const cd*>(A);
-  cd W[16];
+  cd W[kTR];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const int r = q + 4 * i;
+  for (int i = 0; i < kTR; ++i) {
+    const int r = q + kTW * i;
     W[i] = (r < s && c < s) ? Ag[(size_t)(j0 + r) * n + j0 + c] : cd{0.0, 0.0};
   }
   // step j0-1's update S -= v' w'^H + w' v'^H, w' = p' - K' v'
   const double tp = j0 > 0 ? tau[j0 - 1] : 0.0;
   if (tp != 0.0) {  // (uniform)
     double kr = 0.0, ki = 0.0;
-    for (int u = tid; u < nparts_prev; u += 256) {
+    for (int u = tid; u < nparts_prev; u += kTW * 64) {
       kr += part_prev[u].x;
       ki += part_prev[u].y;
     }
-    block_sum2(kr, ki, red);
+    kr = wave_sum(kr);
+    ki = wave_sum(ki);
+    if (c == 0) {
+      red[2 * q] = kr;
+      red[2 * q + 1] = ki;
+    }
+    __syncthreads();
+    kr = 0.0;
+    ki = 0.0;
+    for (int w = 0; w < kTW; ++w) {
+      kr += red[2 * w];
+      ki += red[2 * w + 1];
+    }
     const cd K = {0.5 * tp * kr, 0.5 * tp * ki};
     const cd vc = c < s ? V[(size_t)(j0 - 1) * n + j0 + c] : cd{0.0, 0.0};
     const cd pc = c < s ? p_prev[c] : cd{0.0, 0.0}, kv = cmul(K, vc);
     const cd wc = {pc.x - kv.x, pc.y - kv.y};
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int r = q + 4 * i;
+    for (int i = 0; i < kTR; ++i) {
+      const int r = q + kTW * i;
       if (r >= s) continue;  // (uniform)
       const cd vr = lane_cd(vc, r), wr = lane_cd(wc, r);
       W[i].x -= (vr.x * wc.x + vr.y * wc.y) + (wr.x * vc.x + wr.y * vc.y);
       W[i].y -= (vr.y * wc.x - vr.x * wc.y) + (wr.y * vc.x - wr.x * vc.y);
     }
   }
-  for (int jb = 0; jb < 16; ++jb) {
+  for (int jb = 0; jb < kTR; ++jb) {
 #pragma unroll
-    for (int jq = 0; jq < 4; ++jq) {
-      const int t = 4 * jb + jq, par = jq & 1;
+    for (int jq = 0; jq < kTW; ++jq) {
+      const int t = kTW * jb + jq, par = jq & 1;
       if (t + 2 >= s) break;  // (uniform)
       if (q == jq) {  // row t is this wave's W[0]
         const cd g = W[0];
@@ -568,8 +583,8 @@ __global__ __launch_bounds__(256) void k_hh_tail(
       const cd vc = vb[par][c];
       double ar = 0.0, ai = 0.0;  // this wave's rows of sum_r conj(S[r][c]) v_r
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int r = 4 * (jb + i) + q;
+      for (int i = 0; i < kTR; ++i) {
+        const int r = kTW * (jb + i) + q;
         if (r <= t || r >= s) continue;  // (uniform)
         const cd vr = lane_cd(vc, r);
         ar += W[i].x * vr.x + W[i].y * vr.y;
@@ -579,25 +594,30 @@ __global__ __launch_bounds__(256) void k_hh_tail(
       __syncthreads();
       cd pc = {0.0, 0.0};
       if (c > t) {
-        const cd p0 = pb[par][0][c], p1 = pb[par][1][c], p2 = pb[par][2][c], p3 = pb[par][3][c];
-        pc = {tt * (((p0.x + p1.x) + p2.x) + p3.x), tt * (((p0.y + p1.y) + p2.y) + p3.y)};
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int w = 0; w < kTW; ++w) {
+          sr += pb[par][w][c].x;
+          si += pb[par][w][c].y;
+        }
+        pc = {tt * sr, tt * si};
       }
       const double kr = wave_sum(vc.x * pc.x + vc.y * pc.y);  // conj(v) . p
       const double ki = wave_sum(vc.x * pc.y - vc.y * pc.x);
       const cd K = {0.5 * tt * kr, 0.5 * tt * ki}, kv = cmul(K, vc);
       const cd wc = {pc.x - kv.x, pc.y - kv.y};
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int r = 4 * (jb + i) + q;
+      for (int i = 0; i < kTR; ++i) {
+        const int r = kTW * (jb + i) + q;
         if (r <= t || r >= s) continue;  // (uniform)
         const cd vr = lane_cd(vc, r), wr = lane_cd(wc, r);
         W[i].x -= (vr.x * wc.x + vr.y * wc.y) + (wr.x * vc.x + wr.y * vc.y);
         W[i].y -= (vr.y * wc.x - vr.x * wc.y) + (wr.y * vc.x - wr.x * vc.y);
       }
     }
-    // row 4 jb + q is final: its diagonal entry (and the last 2 x 2 block's
+    // row kTW jb + q is final: its diagonal entry (and the last 2 x 2 block's
     // off-diagonal one, A[n-1][n-2] = conj(A[n-2][n-1])); then shift the window
-    const int r = 4 * jb + q;
+    const int r = kTW * jb + q;
     if (r < s) {
       const double d = lane_d(W[0].x, r);
       cd e = {0.0, 0.0};
@@ -608,14 +628,14 @@ __global__ __launch_bounds__(256) void k_hh_tail(
       }
     }
 #pragma unroll
-    for (int i = 0; i + 1 < 16; ++i) W[i] = W[i + 1];
+    for (int i = 0; i + 1 < kTR; ++i) W[i] = W[i + 1];
   }
   __syncthreads();
-  for (int o = tid; o < 64 * 64; o += 256) {
+  for (int o = tid; o < 64 * 64; o += kTW * 64) {
     const int t = o >> 6, cc = o & 63;
     if (t + 2 < s && cc > t && cc < s) V[(size_t)(j0 + t) * n + j0 + cc] = Vs[t][cc];
   }
-  for (int t = tid; t < s; t += 256) {
+  for (int t = tid; t < s; t += kTW * 64) {
     if (t + 2 < s) tau[j0 + t] = to[t];
     if (t + 1 < s) sub[j0 + t] = so[t];
     diag[j0 + t] = dg[t];
@@ -1533,7 +1553,7 @@ int eig_topk_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, i
     if (fused && n - j <= kTail) {  // the remaining steps in one block per matrix
       const cd* pp = p + (size_t)((j + 1) & 1) * n;
       const cd* partp = part + (size_t)((j + 1) & 1) * nparts_max;
-      KLT_LAUNCH(ctx, "k_hh_tail", k_hh_tail, dim3(1, count), dim3(256), 0, A, n, j, V, tau,
+      KLT_LAUNCH(ctx, "k_hh_tail", k_hh_tail, dim3(1, count), dim3(kTW * 64), 0, A, n, j, V, tau,
                  sub, diag, pp, partp, np_prev, es);
       tail = true;
       break;
