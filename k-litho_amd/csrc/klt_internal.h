@@ -168,6 +168,19 @@ struct klt_ctx {
   klt::DevBuf tcc_pts;                        // source points of the TCC build
   klt::DevBuf tcc_part;                       // split-K partial sums of the TCC Gram
   klt::DevBuf tcc_jobs;                       // per-block jobs of the TCC Gram
+  klt::DevBuf tcc_tab;                        // split16 TCC: points, chunk masks, jobs
+  std::vector<char> tcc_tab_host;             // its last uploaded contents
+  // the split16 TCC plan of the last call: source values and the geometry
+  // that fixes pupil membership (defocus changes only the phase)
+  struct TccPlan {
+    std::vector<float> src;
+    std::array<float, 4> fp{};   // outerSigma, lambda, NA, nmed
+    std::array<int, 5> ip{};     // srcSize, Lx, Ly, Nx, Ny
+    int ns = 0, Kpad = 0, Tpad = 0, nc = 0, ntl = 0, njobs = 0, nblocks = 0;
+    float scale = 0.f;
+    size_t oP = 0, oL = 0, oT = 0, oTJ = 0, oJ = 0, oO = 0, oC = 0;
+    bool valid = false;
+  } tcc_plan;
   klt::DevBuf tw64;                           // e^{+2 pi i j/64}, j < 64
   klt::DevBuf ipart, gam_part;                // kernel-group partials of the coarse stages
   klt::DevBuf mhat, gfull, gcoarse;           // adjoint: per-image M^, G^ and coarse-grid g_c
