@@ -271,16 +271,38 @@ def bench_tcc_eig(ctx, timer, torch):
     return out
 
 
-def section_eig_psd(ctx, timer, torch, steps, warmup):
-    """Top-32 FP64 eigensolve of the T = 1225 blurred-dipole TCC (no defocus) on
-    the positive-semidefinite path, alone: `steps` timed solves after `warmup`."""
+def _dipole_tcc(ctx, torch, precision):
+    """The T = 1225 blurred-dipole TCC (no defocus) on the device, and a
+    function that rebuilds it."""
     src = blurred_dipole()
     sig = min(klt.outer_sigma(src), 1.0)
     Nx, Ny = klt.tcc_sizing(W.NA, W.PERIOD_35, W.PERIOD_35, sig)
     T = (2 * Nx + 1) * (2 * Ny + 1)
     dsrc = torch.from_numpy(src).cuda()
-    t64 = torch.empty((T, T, 2), dtype=torch.float64, device="cuda")
-    ctx.tcc_build(dsrc, sig, W.NA, 0.0, W.PERIOD_35, W.PERIOD_35, Nx, Ny, 0, NMED, out=t64)
+    t = torch.empty((T, T, 2), dtype=torch.float64, device="cuda")
+
+    def build():
+        ctx.tcc_build(dsrc, sig, W.NA, 0.0, W.PERIOD_35, W.PERIOD_35, Nx, Ny, precision, NMED,
+                      out=t)
+    build()
+    return T, t, build
+
+
+def section_tcc_split16(ctx, timer, torch, steps, warmup):
+    """Split-FP16 builds of the T = 1225 blurred-dipole TCC, alone: `steps`
+    timed builds after `warmup`."""
+    T, _, build = _dipole_tcc(ctx, torch, 1)
+    for _ in range(warmup):
+        build()
+    ms = timer(build, steps)
+    log(f"tcc_split16 T={T}: {ms:.3f} ms per build")
+    return {"section": "tcc_split16", "T": T, "builds": steps, "ms_per_build": ms}
+
+
+def section_eig_psd(ctx, timer, torch, steps, warmup):
+    """Top-32 FP64 eigensolve of the T = 1225 blurred-dipole TCC (no defocus) on
+    the positive-semidefinite path, alone: `steps` timed solves after `warmup`."""
+    T, t64, _ = _dipole_tcc(ctx, torch, 0)
     for _ in range(warmup):
         ctx.eig_topk(t64, 32, psd=True)
     ms = timer(lambda: ctx.eig_topk(t64, 32, psd=True), steps)
@@ -417,10 +439,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline and plain path only")
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--section", choices=["all", "eig_psd"], default="all",
+    ap.add_argument("--section", choices=["all", "eig_psd", "tcc_split16"], default="all",
                     help="eig_psd: only --steps top-32 solves of the T = 1225 blurred-dipole "
-                         "TCC on the positive-semidefinite path, their timing as a JSON line "
-                         "(for profiling; no bench line)")
+                         "TCC on the positive-semidefinite path; tcc_split16: only --steps "
+                         "split16 builds of that TCC (no defocus); their timing as a JSON "
+                         "line (for profiling; no bench line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the SOCS "
@@ -436,8 +459,9 @@ def main():
     ctx.set_stream(stream.cuda_stream)
     torch.cuda.set_stream(stream)
     timer = Timer(torch, stream)
-    if args.section == "eig_psd":
-        print(json.dumps(section_eig_psd(ctx, timer, torch, args.steps, args.warmup)))
+    if args.section != "all":
+        fn = section_eig_psd if args.section == "eig_psd" else section_tcc_split16
+        print(json.dumps(fn(ctx, timer, torch, args.steps, args.warmup)))
         return
 
     t0 = time.perf_counter()

@@ -116,6 +116,10 @@ __device__ __forceinline__ half8_t swap_neg(half8_t b) {
 // block stages the chunk's X rows of both panels (hi, lo: 4 x 128 rows x 32
 // halves) in LDS and issues the next chunk's global loads before its products.
 constexpr int kGT = 128;     // output tile
+// Gram blocks: one a CU.  Each writes a 128 x 128 partial (128 KB) that the
+// reduction reads back; at T = 1225 the whole build measured 0.078 ms with
+// 256, 0.086 ms with 128 (a slower Gram) and with 512 (a slower reduction).
+constexpr int kGramJobs = 256;
 constexpr int kGLd = 40;     // LDS row stride (halves): 80 bytes
 __global__ __launch_bounds__(256, 1) void k_gram_split16(
     const _Float16* __restrict__ Xh, const _Float16* __restrict__ Xl, int Kpad,
@@ -421,13 +425,13 @@ static int plan_split16(klt_ctx* ctx, const PupilGeom& g, const float* src_host,
     }
   const int ntl = (int)tiles.size();
   // runs of at most per_job chunks, the smallest length that needs no more
-  // than 256 blocks (one a CU, a single round); a tile's runs are consecutive
-  // jobs, summed in order by the reduction
-  int per_job = std::max<int>(1, (int)((work + 255) / 256));
+  // than kGramJobs blocks (at most one a CU, a single round); a tile's runs
+  // are consecutive jobs, summed in order by the reduction
+  int per_job = std::max<int>(1, (int)((work + kGramJobs - 1) / kGramJobs));
   for (;; ++per_job) {
     size_t nj = 0;
     for (auto& nd : need) nj += (nd.size() + per_job - 1) / per_job;
-    if (nj <= 256) break;
+    if (nj <= (size_t)kGramJobs) break;
   }
   std::vector<int2> tile_jobs(ntl);
   std::vector<int4> jobs;
