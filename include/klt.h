@@ -210,7 +210,8 @@ int klt_socs_image(klt_ctx* ctx, const klt_kernels* kb, const float* mask, int L
 /* nmask masks of one size, stored back to back.  For N <= 15 the full-field
  * DFT passes run as split hi+lo FP16 GEMMs on the matrix cores with FP32
  * accumulation (image relative L2 error ~1e-6 against FP64); the coarse-grid
- * FFTs are FP32. */
+ * FFTs are FP32.  Kernel half-sizes 16 .. 31 take the FP32 path of
+ * klt_socs_image; above 31 the call fails with KLT_ERR_ARG. */
 int klt_socs_forward(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
                      int nmask, int Lx, int Ly, float* images, unsigned flags);
 

@@ -168,6 +168,7 @@ struct klt_ctx {
   klt::DevBuf tcc_pts;                        // source points of the TCC build
   klt::DevBuf tcc_part;                       // split-K partial sums of the TCC Gram
   klt::DevBuf tcc_jobs;                       // per-block jobs of the TCC Gram
+  klt::DevBuf coarse_ws;                      // coarse-grid scratch of the N > 18 SOCS stage
   klt::DevBuf tcc_tab;                        // split16 TCC: points, chunk masks, jobs
   std::vector<char> tcc_tab_host;             // its last uploaded contents
   // the split16 TCC plan of the last call: source values and the geometry

@@ -12,7 +12,9 @@
 //      it, accumulates lambda_k |E_k|^2, and forward-transforms the coarse
 //      intensity to its |g| <= 2N spectrum: for N <= 15 with the 64-point
 //      LDS FFTs of socs_fused.hip (k_socs_mid_band), above that with direct
-//      transforms (k_socs_coarse);
+//      transforms, in one block per image while they fit in LDS
+//      (k_socs_coarse, N <= 18), else in four launches over device scratch
+//      (k_coarse_b / _int / _rx / _gy, N <= 31);
 //   3. k_synth_cols / k_synth_rows: evaluates the band-limited intensity at
 //      every pixel with two separable passes (one write of the image).
 // HBM traffic per image is one mask read and one image write plus small
@@ -235,6 +237,108 @@ __global__ __launch_bounds__(kThreads) void k_socs_coarse(
 }
 
 // ---------------------------------------------------------------------------
+// 2'. the same coarse-grid stage in four launches over device scratch, for
+// half-sizes whose single-block stage does not fit in LDS (N > 18; M = 128):
+//   k_coarse_b   B[b][k][r][ux] = sum_nx K_k[r][nx] band[b][r][nx] e^{2 pi i (nx-Nx) ux/M}
+//   k_coarse_int I[b][uy][ux]   = sum_k lambda_k |sum_r B[b][k][r][ux] e^{2 pi i (r-Ny) uy/M}|^2
+//   k_coarse_rx  R[b][uy][gi]   = sum_ux I[b][uy][ux] e^{-2 pi i gx ux/M}, gx = gi - 2Nx
+//   k_coarse_gy  spec[b][gy][gi] = 1/M^2 sum_uy R[b][uy][gi] e^{-2 pi i gy uy/M}
+// with the sums of k_socs_coarse in the same order (same FP32 results).  One
+// thread per output; ux (or gi) runs fastest, so a wave shares its other
+// indices and reads the band, kernel and twiddle entries as broadcasts.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_coarse_b(const float2* __restrict__ band,
+                                                       const float2* __restrict__ kern, int nk,
+                                                       int Nx, int Ny, int M,
+                                                       const float2* __restrict__ twM,
+                                                       float2* __restrict__ Bm, size_t n) {
+  const size_t o = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (o >= n) return;
+  const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy, mm = M - 1;
+  const int ux = (int)(o % M);
+  const size_t q = o / M;
+  const int r = (int)(q % Wy);
+  const size_t bk = q / Wy;
+  const int k = (int)(bk % nk), b = (int)(bk / nk);
+  const float2* K = kern + (size_t)k * T + r * Wx;
+  const float2* bd = band + (size_t)b * T + r * Wx;
+  float2 acc = make_float2(0.f, 0.f);
+  for (int nx = 0; nx < Wx; ++nx) {
+    const float2 a = cmul(K[nx], bd[nx]);
+    const float2 w = twM[((nx - Nx) * ux) & mm];
+    acc.x += a.x * w.x - a.y * w.y;
+    acc.y += a.x * w.y + a.y * w.x;
+  }
+  Bm[o] = acc;
+}
+
+__global__ __launch_bounds__(kThreads) void k_coarse_int(const float2* __restrict__ Bm,
+                                                         const float* __restrict__ scales,
+                                                         int nk, int Ny, int M,
+                                                         const float2* __restrict__ twM,
+                                                         float* __restrict__ I, size_t n) {
+  const size_t o = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (o >= n) return;
+  const int Wy = 2 * Ny + 1, mm = M - 1;
+  const int ux = (int)(o % M), uy = (int)((o / M) % M), b = (int)(o / ((size_t)M * M));
+  float s = 0.f;
+  for (int k = 0; k < nk; ++k) {
+    const float2* Bk = Bm + (((size_t)b * nk + k) * Wy) * M + ux;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int r = 0; r < Wy; ++r) {
+      const float2 v = Bk[(size_t)r * M];
+      const float2 w = twM[((r - Ny) * uy) & mm];
+      acc.x += v.x * w.x - v.y * w.y;
+      acc.y += v.x * w.y + v.y * w.x;
+    }
+    s += scales[k] * (acc.x * acc.x + acc.y * acc.y);
+  }
+  I[o] = s;
+}
+
+__global__ __launch_bounds__(kThreads) void k_coarse_rx(const float* __restrict__ I, int Nx,
+                                                        int M, const float2* __restrict__ twM,
+                                                        float2* __restrict__ R, size_t n) {
+  const size_t o = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (o >= n) return;
+  const int Gx = 4 * Nx + 1, mm = M - 1;
+  const int gi = (int)(o % Gx), gx = gi - 2 * Nx;
+  const size_t row = o / Gx;  // b M + uy
+  const float* Ir = I + row * M;
+  float2 acc = make_float2(0.f, 0.f);
+  for (int ux = 0; ux < M; ++ux) {
+    const float v = Ir[ux];
+    const float2 w = twM[(-gx * ux) & mm];
+    acc.x += v * w.x;
+    acc.y += v * w.y;
+  }
+  R[o] = acc;
+}
+
+__global__ __launch_bounds__(kThreads) void k_coarse_gy(const float2* __restrict__ R, int Nx,
+                                                        int Ny, int M,
+                                                        const float2* __restrict__ twM,
+                                                        float2* __restrict__ spec, int G,
+                                                        size_t n) {
+  const size_t o = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (o >= n) return;
+  const int Gx = 4 * Nx + 1, mm = M - 1;
+  const int gi = (int)(o % Gx), gy = (int)((o / Gx) % G), b = (int)(o / ((size_t)G * Gx));
+  float2 acc = make_float2(0.f, 0.f);
+  if (gy <= 2 * Ny) {
+    const float2* Rb = R + (size_t)b * M * Gx + gi;
+    for (int uy = 0; uy < M; ++uy) {
+      const float2 v = Rb[(size_t)uy * Gx];
+      const float2 w = twM[(-gy * uy) & mm];
+      acc.x += v.x * w.x - v.y * w.y;
+      acc.y += v.x * w.y + v.y * w.x;
+    }
+  }
+  const float inv = 1.0f / ((float)M * (float)M);
+  spec[o] = make_float2(acc.x * inv, acc.y * inv);
+}
+
+// ---------------------------------------------------------------------------
 // 3a. C[gy][x] = w_gy sum_gx J[gy][gx] e^{+2 pi i gx (x-cx)/Lx}, w_0 = 1,
 // w_gy = 2 (the gy < 0 half is the conjugate and is folded in by taking real
 // parts in 3b).  One thread per column.  ccols: [b][G][Lx].
@@ -439,13 +543,21 @@ int socs_forward_device(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
   const int nB = std::max((2 * Ny + 1) * M, M * Gx);
   const size_t lds_coarse =
       sizeof(float2) * (2 * (size_t)T + nB + M) + sizeof(float) * (size_t)M * M;
-  if (lds_coarse > 160 * 1024)
-    return set_error(KLT_ERR_ARG, "SOCS kernel support too large for the coarse grid");
   if (NP > 32 || G > 64)
     return set_error(KLT_ERR_ARG, "SOCS kernel half-size N above 31 is not supported");
-  if (lds_coarse > 64 * 1024)
+  // the single-block coarse stage while it fits in LDS (N <= 18), else the
+  // four-launch one over device scratch, in sub-batches of cb images
+  const bool global_coarse = !socs_fused_supported(kb) && lds_coarse > 160 * 1024;
+  const size_t szB = (size_t)kb->nk * (2 * Ny + 1) * M * sizeof(float2);
+  const size_t szI = (size_t)M * M * sizeof(float), szR = (size_t)M * Gx * sizeof(float2);
+  int cb = 1;
+  if (global_coarse) {
+    cb = (int)std::max<size_t>(1, std::min<size_t>(SB, (256u << 20) / (szB + szI + szR)));
+    KLT_TRY(ctx->coarse_ws.ensure((szB + szI + szR) * cb));
+  } else if (lds_coarse > 64 * 1024) {
     KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_socs_coarse),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_coarse));
+  }
   for (int b0 = 0; b0 < B; b0 += SB) {
     const int nb = std::min(SB, B - b0);
     const float* m = masks + (size_t)b0 * Lx * Ly;
@@ -460,10 +572,30 @@ int socs_forward_device(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
     }
     if (socs_fused_supported(kb)) {
       KLT_TRY(socs_mid_from_band(ctx, kb, band, nb, ctx->spec.as<float2>(), G));
-    } else {
+    } else if (!global_coarse) {
       KLT_LAUNCH(ctx, "k_socs_coarse", k_socs_coarse, dim3(nb), dim3(kThreads), lds_coarse,
                  band, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk, Nx, Ny, M,
                  p->twM.as<float2>(), ctx->spec.as<float2>(), G);
+    } else {
+      float2* Bm = ctx->coarse_ws.as<float2>();
+      float* Ic = reinterpret_cast<float*>(ctx->coarse_ws.as<char>() + szB * cb);
+      float2* R = reinterpret_cast<float2*>(ctx->coarse_ws.as<char>() + (szB + szI) * cb);
+      const float2* tw = p->twM.as<float2>();
+      auto grid = [](size_t n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); };
+      for (int c0 = 0; c0 < nb; c0 += cb) {
+        const int nc = std::min(cb, nb - c0);
+        const float2* bd = band + (size_t)c0 * T;
+        const size_t nbk = (size_t)nc * kb->nk * (2 * Ny + 1) * M, ni = (size_t)nc * M * M,
+                     nr = (size_t)nc * M * Gx, ns = (size_t)nc * G * Gx;
+        KLT_LAUNCH(ctx, "k_coarse_b", k_coarse_b, grid(nbk), dim3(kThreads), 0, bd,
+                   kb->kern.as<float2>(), kb->nk, Nx, Ny, M, tw, Bm, nbk);
+        KLT_LAUNCH(ctx, "k_coarse_int", k_coarse_int, grid(ni), dim3(kThreads), 0, Bm,
+                   kb->scales.as<float>(), kb->nk, Ny, M, tw, Ic, ni);
+        KLT_LAUNCH(ctx, "k_coarse_rx", k_coarse_rx, grid(nr), dim3(kThreads), 0, Ic, Nx, M, tw,
+                   R, nr);
+        KLT_LAUNCH(ctx, "k_coarse_gy", k_coarse_gy, grid(ns), dim3(kThreads), 0, R, Nx, Ny, M,
+                   tw, ctx->spec.as<float2>() + (size_t)c0 * G * Gx, G, ns);
+      }
     }
     KLT_LAUNCH(ctx, "k_synth_cols", k_synth_cols, dim3((Lx + kThreads - 1) / kThreads, nb),
                dim3(kThreads), sizeof(float2) * G * Gx, ctx->spec.as<float2>(), G, Nx, Ny, Lx,

@@ -59,7 +59,10 @@ def test_small_random_kernels(ctx, Lx, Ly, Nx, Ny, plain):
 
 # kernel half-sizes above 15 (the reference's own 33 x 33 sample has N = 16):
 # odd and unequal periods, Ny != Nx
-LARGE_N = [(67, 70, 16, 16), (200, 75, 16, 18), (129, 100, 18, 17), (160, 160, 18, 18)]
+# N 16-18: the single-block coarse stage; N 19-31: the four-launch one over
+# device scratch (4N + 1 <= L at the limit: 125 <= 127)
+LARGE_N = [(67, 70, 16, 16), (200, 75, 16, 18), (129, 100, 18, 17), (160, 160, 18, 18),
+           (150, 140, 19, 20), (256, 200, 24, 22), (130, 127, 31, 31), (300, 260, 27, 31)]
 
 
 @pytest.mark.parametrize("plain", [True, False], ids=["plain", "default"])
@@ -72,6 +75,19 @@ def test_large_half_size_random_kernels(ctx, Lx, Ly, Nx, Ny, plain):
     bank = ctx.kernels(kr, ki, sc, 2 * Nx + 1, 2 * Ny + 1)
     img = ctx.socs_forward(bank, m, plain=plain)
     assert relerr(img, O.socs_image(m, kr, ki, sc, Nx, Ny)) < TOL
+
+
+def test_large_half_size_batch(ctx):
+    """Three masks in one call at N = 25 (the coarse stage over device scratch
+    takes the images in sub-batches): each image against the oracle."""
+    print("case large N batch of 3, N=25", flush=True)
+    rng = np.random.default_rng(25)
+    kr, ki, sc = _random_bank(rng, 25, 25, 6)
+    ms = np.stack([(rng.random((256, 256)) > 0.5).astype(np.float32) for _ in range(3)])
+    bank = ctx.kernels(kr, ki, sc, 51, 51)
+    img = ctx.socs_forward(bank, ms)
+    for i in range(3):
+        assert relerr(img[i], O.socs_image(ms[i], kr, ki, sc, 25, 25)) < TOL, i
 
 
 def test_continuous_mask_fused(ctx):
