@@ -9,7 +9,8 @@ over the timed steps.
 
 Also reported (same JSON line):
   plain        the FP32 single-image path (klt_socs_image, one call per image)
-  adjoint      klt_socs_backward on 16 relaxed 1024^2 masks (mask gradients/s)
+  adjoint      klt_socs_backward on 16 (and 128) relaxed 1024^2 masks (mask
+               gradients/s, with its own roofline object)
                and klt_abbe_source_grad (source gradient over the 51^2 grid)
   tcc_eig      TCC T = 1225 (blurred dipole, period 1280, 5 defocus values),
                split-FP16 MFMA Gram vs FP64 Gram, top-32 FP64 eigensolve (one
@@ -310,31 +311,117 @@ def section_eig_psd(ctx, timer, torch, steps, warmup):
     return {"section": "eig_psd", "T": T, "k": 32, "solves": steps, "ms_per_solve": ms}
 
 
-def bench_adjoint(ctx, bank, timer, torch, dev, src):
-    B = 16
-    tg = np.stack([W.manhattan(L, 3000 + i) for i in range(B)])
+ADJ_KERNELS = ("k_band_mfma", "k_synth_resist", "k_synth_mfma")  # the adjoint's streams
+
+
+def adjoint_batch(ctx, bank, timer, torch, dev, B):
+    """klt_socs_backward on B relaxed 1024^2 masks (targets: the binary masks):
+    ms per call (mean of 20 after 3 warm-up calls, the first of which tunes
+    the shape) and per-kernel HIP-event times of one call."""
+    tg = np.stack([W.manhattan(L, 3000 + i % 64) for i in range(B)])
     mk = np.stack([W.relaxed(tg[i], 4000 + i) for i in range(B)])
     dm = torch.from_numpy(mk).to(dev)
     dt = torch.from_numpy(tg).to(dev)
-    ds = torch.from_numpy(np.ascontiguousarray(src)).to(dev)
-    for _ in range(3):  # the first call tunes the shape
+    for _ in range(3):
         klt.socs_backward(ctx, bank, dm, dt)
-    # 20 calls: a 0.15 ms call is short enough that the host's issue of the
+    # 20 calls: a 0.1 ms call is short enough that the host's issue of the
     # first one (after the synchronisation) would weigh on a few-call mean
     ms = timer(lambda: klt.socs_backward(ctx, bank, dm, dt), 20)
+    ctx.set_profiling(True)
+    klt.socs_backward(ctx, bank, dm, dt)
+    rep = ctx.profile_report()
+    ctx.set_profiling(False)
+    kern = {k: {"ms": v[1], "launches": v[0]} for k, v in rep.items()}
+    # each streaming kernel moves 4 L^2 B algorithmic bytes: the mask read
+    # (band), the target read (fused synthesis + resist band), the gradient
+    # write (synthesis of the gradient)
+    for k in ADJ_KERNELS:
+        if k in kern:
+            v = kern[k]
+            v["algorithmic_bytes_per_launch"] = 4.0 * L * L * B
+            v["achieved_gbs"] = 4.0 * L * L * B / (v["ms"] / v["launches"] / 1e3) / 1e9
+    return dm, dt, {"batch": B, "mask_grads_per_s": B / (ms / 1e3), "ms_per_batch": ms,
+                    "path_hbm_gbs": 12.0 * L * L * B / (ms / 1e3) / 1e9, "kernels": kern}
+
+
+ADJ_COUNTERS = os.path.join(ROOT, "profiles", "adjoint_1024_hbm_counters.json")
+
+
+def adjoint_roofline(res):
+    """Roofline object of the adjoint's dominant streaming kernel at one batch:
+    its algorithmic bytes over its HIP-event duration, and (when a counter
+    measurement of this kernel and batch is on file) its copy-calibrated
+    FETCH + WRITE bytes per launch."""
+    kern = res["kernels"]
+    dom = max((k for k in ADJ_KERNELS if k in kern),
+              key=lambda k: kern[k]["ms"] / kern[k]["launches"])
+    v = kern[dom]
+    traffic = None
+    try:
+        with open(ADJ_COUNTERS) as f:
+            c = json.load(f)
+        traffic = c["batches"][str(res["batch"])][dom]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        pass
+    return {"kernel": dom, "batch": res["batch"], "bound": "hbm",
+            "achieved": v["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": v["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic,
+            "algorithmic_bytes_per_launch": v["algorithmic_bytes_per_launch"]}
+
+
+def bench_adjoint(ctx, bank, timer, torch, dev, src):
+    """Config 4's batch (16 masks) with the source gradient, then 128 masks."""
+    dm, dt, r16 = adjoint_batch(ctx, bank, timer, torch, dev, 16)
+    ds = torch.from_numpy(np.ascontiguousarray(src)).to(dev)
     # first source-gradient call outside the profiled one (buffers, first launches)
     klt.abbe_source_grad(ctx, dm, dt, ds, W.NA, 0.0, 14, 14, nmed=NMED)
     ctx.set_profiling(True)
-    klt.socs_backward(ctx, bank, dm, dt)
-    kern = {k: v[1] for k, v in ctx.profile_report().items()}
     klt.abbe_source_grad(ctx, dm, dt, ds, W.NA, 0.0, 14, 14, nmed=NMED)
     kern_src = {k: v[1] for k, v in ctx.profile_report().items()}
     ctx.set_profiling(False)
     ms_src = timer(lambda: klt.abbe_source_grad(ctx, dm, dt, ds, W.NA, 0.0, 14, 14, nmed=NMED), 3)
-    return {"batch": B, "mask_grads_per_s": B / (ms / 1e3), "ms_per_batch": ms,
-            "kernels_ms": kern, "source_grad_ms_per_batch": ms_src,
-            "source_grad_kernels_ms": kern_src,
-            "hbm_bytes_per_mask_grad": 4 * 5 * L * L}
+    del dm, dt
+    _, _, r128 = adjoint_batch(ctx, bank, timer, torch, dev, 128)
+    out = dict(r16)
+    out.update({"roofline": adjoint_roofline(r16), "source_grad_ms_per_batch": ms_src,
+                "source_grad_kernels_ms": kern_src,
+                "hbm_bytes_per_mask_grad": 12 * L * L,
+                "batch128": dict(r128, roofline=adjoint_roofline(r128)),
+                "note": "algorithmic bytes per mask gradient: mask + target read, gradient "
+                        "written (12 MiB at 1024^2); the forward image is formed tile by tile "
+                        "inside k_synth_resist and never stored"})
+    return out
+
+
+def section_adjoint(ctx, timer, torch, steps, warmup, batch):
+    """klt_socs_backward alone on `batch` relaxed 1024^2 masks with the 24
+    kernels of 29 x 29: `steps` timed calls after `warmup` (for profiling)."""
+    _, kr, ki, sc = gpu_kernels(ctx)
+    bank = ctx.kernels(kr, ki, sc, KSIZE, KSIZE)
+    dev = torch.device("cuda:0")
+    tg = np.stack([W.manhattan(L, 3000 + i % 64) for i in range(batch)])
+    mk = np.stack([W.relaxed(tg[i], 4000 + i) for i in range(batch)])
+    dm = torch.from_numpy(mk).to(dev)
+    dt = torch.from_numpy(tg).to(dev)
+    for _ in range(warmup):
+        klt.socs_backward(ctx, bank, dm, dt)
+    ms = timer(lambda: klt.socs_backward(ctx, bank, dm, dt), steps)
+    log(f"adjoint batch {batch}: {ms:.3f} ms per call, {batch / (ms / 1e3):.0f} mask grads/s")
+    bank.close()
+    return {"section": "adjoint", "batch": batch, "calls": steps, "ms_per_call": ms,
+            "mask_grads_per_s": batch / (ms / 1e3)}
+
+
+def copy_calibration(torch, timer, dev):
+    """A torch device-to-device copy of 1 GiB (the runtime's copy kernel), whose
+    known bytes calibrate the FETCH_SIZE / WRITE_SIZE counters."""
+    a = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev)
+    bcopy = torch.empty_like(a)
+    cms = timer(lambda: bcopy.copy_(a), 5)
+    out = {"bytes_read": a.numel() * 4, "bytes_written": a.numel() * 4, "ms": cms,
+           "gbs": 2 * a.numel() * 4 / (cms / 1e3) / 1e9}
+    log(f"copy calibration: {out['gbs']:.0f} GB/s")
+    return out
 
 
 def _cpu_model():
@@ -439,11 +526,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline and plain path only")
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--section", choices=["all", "eig_psd", "tcc_split16"], default="all",
+    ap.add_argument("--section", choices=["all", "eig_psd", "tcc_split16", "adjoint"],
+                    default="all",
                     help="eig_psd: only --steps top-32 solves of the T = 1225 blurred-dipole "
                          "TCC on the positive-semidefinite path; tcc_split16: only --steps "
-                         "split16 builds of that TCC (no defocus); their timing as a JSON "
-                         "line (for profiling; no bench line)")
+                         "split16 builds of that TCC (no defocus); adjoint: only --steps "
+                         "klt_socs_backward calls on --batch 1024^2 masks; their timing as a "
+                         "JSON line (for profiling; no bench line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the SOCS "
@@ -459,6 +548,12 @@ def main():
     ctx.set_stream(stream.cuda_stream)
     torch.cuda.set_stream(stream)
     timer = Timer(torch, stream)
+    if args.section == "adjoint":
+        res = section_adjoint(ctx, timer, torch, args.steps, args.warmup, args.batch)
+        if args.copy_calibration:
+            res["copy_calibration"] = copy_calibration(torch, timer, dev)
+        print(json.dumps(res))
+        return
     if args.section != "all":
         fn = section_eig_psd if args.section == "eig_psd" else section_tcc_split16
         print(json.dumps(fn(ctx, timer, torch, args.steps, args.warmup)))
@@ -542,7 +637,9 @@ def main():
     extras = {}
     if not args.no_extras:
         adj = bench_adjoint(ctx, bank, timer, torch, dev, src)
-        log(f"adjoint: {adj['mask_grads_per_s']:.1f} mask grads/s, source grad "
+        log(f"adjoint: {adj['mask_grads_per_s']:.1f} mask grads/s at 16 "
+            f"({adj['roofline']['kernel']} {adj['roofline']['frac']:.3f} of HBM peak), "
+            f"{adj['batch128']['mask_grads_per_s']:.1f} at 128; source grad "
             f"{adj['source_grad_ms_per_batch']:.2f} ms / 16 masks")
         extras["adjoint"] = adj
         tce = bench_tcc_eig(ctx, timer, torch)
@@ -560,12 +657,7 @@ def main():
             f"{bu['p50_image_latency_us']:.0f} us, p99 {bu['p99_image_latency_us']:.0f} us")
         extras["batch_uneven"] = bu
     if args.copy_calibration:
-        a = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev)
-        bcopy = torch.empty_like(a)
-        cms = timer(lambda: bcopy.copy_(a), 5)
-        extras["copy_calibration"] = {"bytes_read": a.numel() * 4, "bytes_written": a.numel() * 4,
-                                      "ms": cms, "gbs": 2 * a.numel() * 4 / (cms / 1e3) / 1e9}
-        log(f"copy calibration: {extras['copy_calibration']['gbs']:.0f} GB/s")
+        extras["copy_calibration"] = copy_calibration(torch, timer, dev)
 
     cpu = err = gerr = None
     if not args.no_cpu:

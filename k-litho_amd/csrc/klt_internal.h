@@ -61,6 +61,10 @@ struct FusedTables {
   // adjoint: resist-gradient band rows Wgf (64 rows, centre (Lx+1)/2, fragment
   // order) and the column twiddles e^{-2 pi i gy (y - (Ly+1)/2)/Ly} (Ly x 32)
   DevBuf Wgf, twyg;
+  // fused adjoint synthesis (k_synth_resist): the column twiddles of the
+  // resist band as A fragments over 16-row tiles (TYgf) and the synthesis
+  // column twiddles E as B fragments over 32-column steps (Ebf)
+  DevBuf TYgf, Ebf;
 };
 
 // Device pointers of one fused forward launch (socs_fused_launch): the tables

@@ -1447,6 +1447,392 @@ __global__ void k_batch_begin(unsigned* __restrict__ strips_done, int n,
   if (blockIdx.x == 0 && threadIdx.x == 0) *t0 = wall_clock64();
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint stage 3 + resist band, fused.  grid (ceil(Lx/128) * rs, B), 256
+// threads; block x takes strip x / rs (128 columns at x0) and the (x % rs)-th
+// of rs row ranges, like k_synth_mfma.  The block synthesises its tile of the
+// image I exactly as k_synth_mfma does (P from the same split spectrum
+// operand, here kept in LDS), but instead of storing I it forms the resist
+// gradient G = dL/dI = 2 (s - T) a s (1 - s), s = sigmoid(a (I - thr)), with
+// the target T read in 4 rows x 256 B per load, and contracts G on the matrix
+// cores to the block's share of the centred resist band
+//   G^[gy][gx] = sum_{y, x} G[y][x] e^{-2 pi i (gx (x - cx)/Lx + gy (y - cy)/Ly)},
+//   cx = (Lx+1)/2, cy = (Ly+1)/2, |gx| <= 2Nx, |gy| <= 2Ny:
+//  - per 16-row tile, Q[j][x] += sum_y TYg[y][j] G[y][x] (K = 16 rows, j = 2 gy:
+//    cos, 2 gy + 1: -sin; the synthesis output tile is already the B operand);
+//  - once per block, C[j][k] = sum_x Q[j][x] E[x][k] (K = the block's 128
+//    columns, E the synthesis column twiddles cos / sin of gx, k = 2 (gx + 2Nx)
+//    + c), from which G^ = (C[2gy][2gi] + C[2gy+1][2gi+1]) +
+//    i (C[2gy+1][2gi] - C[2gy][2gi+1]) for gy >= 0, and its conjugate symmetry
+//    (G real) for gy < 0.
+// All products are hi + lo fp16 splits (three MFMAs, FP32 accumulation).  Out:
+// gpart[b][part][4Ny+1][32] (gy + 2Ny, gx = 0..2Nx, zero past 2Nx) and the
+// block's loss sum (s - T)^2 in lpart[b][part], the layouts k_resist_sum sums.
+// With img != nullptr the image is written as well.
+// TYgf: [Lypad/16][4 jt][hi, lo][64 lanes] x 4 halves, lane l holding
+//   TYg[16 yt + 4 (l/16) + t][16 jt + l % 16] (A operand, K = 16);
+// Ebf: [Lxp/32][8 kt][hi, lo][64 lanes] x 8 halves, lane l holding
+//   E[32 xs + 8 (l/16) + t][16 kt + l % 16] (B operand, K = 32).
+// ---------------------------------------------------------------------------
+constexpr int kQS = 132;  // LDS row stride (floats) of the block's Q and C
+
+// TVEC: the target rows are whole 16-byte vectors (Lx % 4 == 0, aligned).
+template <bool TVEC>
+__device__ __forceinline__ void synth_resist_strip(
+    int x0, int y0, int y1, int pidx, const float2* __restrict__ spec, int Nx, int Ny, int Lx,
+    int Ly, const float4* __restrict__ Ef, const _Float16* __restrict__ TYh,
+    const _Float16* __restrict__ TYl, const float* __restrict__ tgt, float a_sig, float thr,
+    const half4_t* __restrict__ TYgf, const float4* __restrict__ Ebf, float* __restrict__ img,
+    float2* __restrict__ gout, float* __restrict__ lout) {
+  constexpr int B_BYTES = 2 * 16 * 64 * 16, P_BYTES = 2 * kColsSynth * kPStride * 2;
+  constexpr int MAIN = B_BYTES > P_BYTES ? B_BYTES : P_BYTES;
+  constexpr int kStg = 68;
+  static_assert(64 * kQS * 4 <= MAIN, "Q and C fit in the operand LDS");
+  __shared__ __attribute__((aligned(16))) char smem[MAIN + 4 * 16 * kStg * 4];
+  __shared__ float red[4];
+  half8_t* sBh = reinterpret_cast<half8_t*>(smem);
+  half8_t* sBl = sBh + 16 * 64;
+  _Float16* sPh = reinterpret_cast<_Float16*>(smem);
+  _Float16* sPl = sPh + kColsSynth * kPStride;
+  const int tid = threadIdx.x;
+  const int Gx = 4 * Nx + 1;
+  // (the wave index as a scalar: everything derived from it, the rows of each
+  // step included, is then known to be uniform and branches on it are scalar)
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int r16 = lane & 15, kq = (lane >> 4) * 8, q4 = (lane >> 4) * 4;
+  float* stg = reinterpret_cast<float*>(smem + MAIN) + wave * 16 * kStg;
+  // --- P = E . B on the matrix cores, exactly as synth_strip -----------------
+  float part = 0.f;
+  for (int i = tid; i < 32 * Gx; i += 256) {
+    const float2 v = spec[i];
+    part += fabsf(v.x) + fabsf(v.y);
+  }
+  part = wave_sum_f(part);
+  if (lane == 0) red[wave] = part;
+  __syncthreads();
+  const float bound = red[0] + red[1] + red[2] + red[3];
+  int ex = 0;
+  (void)frexpf(bound > 0.f ? bound : 1.f, &ex);
+  const float pscale = ldexpf(1.0f, 12 - ex), inv_pscale = ldexpf(1.0f, ex - 12);
+  {
+    const int j = 16 * wave + r16, gy = j >> 1, d = j & 1;
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      float v[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int k = 32 * st + kq + t, gxi = k >> 1, c = k & 1;
+        const float2 J = gxi < Gx ? spec[gy * Gx + gxi] : make_float2(0.f, 0.f);
+        v[t] = pscale * (d == 0 ? (c == 0 ? J.x : -J.y) : (c == 0 ? J.y : J.x));
+      }
+      half8_t h, l;
+      split8(v, h, l);
+      sBh[(st * 4 + wave) * 64 + lane] = h;
+      sBl[(st * 4 + wave) * 64 + lane] = l;
+    }
+  }
+  __syncthreads();
+  {
+    float4_t pacc[2][4];
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) pacc[m][nt] = float4_t{0.f, 0.f, 0.f, 0.f};
+    const float4* ef = Ef + (size_t)(x0 / 16 + 2 * wave) * 512;
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      half8_t ah[2], al[2];
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        ah[m] = __builtin_bit_cast(half8_t, ef[((m * 4 + st) * 2 + 0) * 64 + lane]);
+        al[m] = __builtin_bit_cast(half8_t, ef[((m * 4 + st) * 2 + 1) * 64 + lane]);
+      }
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        const half8_t bh = sBh[(st * 4 + nt) * 64 + lane], bl = sBl[(st * 4 + nt) * 64 + lane];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) pacc[m][nt] = mfma3(ah[m], al[m], bh, bl, pacc[m][nt]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int c = 32 * wave + 16 * m + 4 * (lane >> 4) + r, j = 16 * nt + r16;
+          const float v = pacc[m][nt][r];
+          const _Float16 h = (_Float16)v;
+          sPh[c * kPStride + j] = h;
+          sPl[c * kPStride + j] = (_Float16)(v - (float)h);
+        }
+  }
+  __syncthreads();
+  // --- rows: synthesis tile, G, Q accumulation -------------------------------
+  const int ch = wave & 1, rp = wave >> 1;
+  const int xs0 = x0 + ch * 64;
+  const bool ivec = img && (Lx & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 15) == 0;
+  float4_t Q[4][4];
+#pragma unroll
+  for (int jt = 0; jt < 4; ++jt)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) Q[jt][cb] = float4_t{0.f, 0.f, 0.f, 0.f};
+  float lsum = 0.f;
+  // the target tile of the 16 rows at yb: lane holds rows 4 q + lane / 16,
+  // columns 4 (lane % 16) .. + 4 of the wave's 64 columns.  Branch-free: rows
+  // and columns past the image read clamped addresses, whose values the
+  // gradient below masks out (a conditional load here would make the compiler
+  // wait for every load in flight, the prefetched tiles included).
+  // (named registers, not an array: an array passed by reference stays in
+  // scratch memory, and every prefetch then waits for its load)
+  auto load_t1 = [&](int y, float4& t) {
+    y = min(y, Ly - 1);
+    const int x = xs0 + r16 * 4;
+    const float* row = tgt + (size_t)y * Lx;
+    if constexpr (TVEC) {
+      t = *reinterpret_cast<const float4*>(row + min(x, Lx - 4));
+    } else {
+      t = make_float4(row[min(x, Lx - 1)], row[min(x + 1, Lx - 1)], row[min(x + 2, Lx - 1)],
+                      row[min(x + 3, Lx - 1)]);
+    }
+  };
+  auto load_t = [&](int yb, float4& t0, float4& t1, float4& t2, float4& t3) {
+    const int y = yb + (lane >> 4);
+    load_t1(y, t0);
+    load_t1(y + 4, t1);
+    load_t1(y + 8, t2);
+    load_t1(y + 12, t3);
+  };
+  auto load_ty = [&](int yb, half8_t& h0, half8_t& h1, half8_t& l0, half8_t& l1) {
+    const _Float16* th = TYh + (size_t)(yb + r16) * 64 + kq;
+    const _Float16* tl = TYl + (size_t)(yb + r16) * 64 + kq;
+    h0 = *reinterpret_cast<const half8_t*>(th);
+    h1 = *reinterpret_cast<const half8_t*>(th + 32);
+    l0 = *reinterpret_cast<const half8_t*>(tl);
+    l1 = *reinterpret_cast<const half8_t*>(tl + 32);
+  };
+  // s = 1 / (1 + 2^(c1 d + c0)) for the unscaled synthesis output d (I = d / pscale)
+  const float c1 = -a_sig * 1.44269504f * inv_pscale, c0 = a_sig * thr * 1.44269504f;
+  const float two_a = 2.f * a_sig;
+  // one 16-row tile at yb: synthesis, G, G split into hi / lo halves in the
+  // B-operand layout (lane: rows 4 (lane/16) + r of column 16 cb + lane % 16),
+  // and its Q products.
+  // tc0..3: its target tile, refilled with the tile two steps on (yb + 64);
+  // (ah0..al1): its TY operand, (nh0..nl1) refilled with yb + 32's.
+  auto tile = [&](int yb, float4& tc0, float4& tc1, float4& tc2, float4& tc3, half8_t& ah0,
+                  half8_t& ah1, half8_t& al0, half8_t& al1, half8_t& nh0, half8_t& nh1,
+                  half8_t& nl0, half8_t& nl1) {
+    // this tile's TYg operand (A of the Q products), consumed after the synthesis
+    half4_t gah[4], gal[4];
+    const half4_t* tg = TYgf + (size_t)(yb >> 4) * 8 * 64 + lane;
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt) {
+      gah[jt] = tg[(jt * 2 + 0) * 64];
+      gal[jt] = tg[(jt * 2 + 1) * 64];
+    }
+    if (yb + 32 < y1) load_ty(yb + 32, nh0, nh1, nl0, nl1);
+    float4_t d[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+      const int c = ch * 64 + cb * 16 + r16;
+      const half8_t ph0 = *reinterpret_cast<const half8_t*>(sPh + c * kPStride + kq);
+      const half8_t pl0 = *reinterpret_cast<const half8_t*>(sPl + c * kPStride + kq);
+      const half8_t ph1 = *reinterpret_cast<const half8_t*>(sPh + c * kPStride + 32 + kq);
+      const half8_t pl1 = *reinterpret_cast<const half8_t*>(sPl + c * kPStride + 32 + kq);
+      d[cb] = float4_t{0.f, 0.f, 0.f, 0.f};
+      d[cb] = mfma3(ah0, al0, ph0, pl0, d[cb]);
+      d[cb] = mfma3(ah1, al1, ph1, pl1, d[cb]);
+    }
+    // target tile through the wave's staging rows into the MFMA output layout
+    {
+      float* st = stg + (lane >> 4) * kStg + r16 * 4;
+      *reinterpret_cast<float4*>(st) = tc0;
+      *reinterpret_cast<float4*>(st + 4 * kStg) = tc1;
+      *reinterpret_cast<float4*>(st + 8 * kStg) = tc2;
+      *reinterpret_cast<float4*>(st + 12 * kStg) = tc3;
+    }
+    if (yb + 64 < y1) load_t(yb + 64, tc0, tc1, tc2, tc3);
+    __builtin_amdgcn_wave_barrier();
+    // tiles wholly inside the image skip the per-entry bounds
+    const bool interior = yb + 16 <= Ly && xs0 + 64 <= Lx;
+    half4_t gh[4], gl[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+      float g[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float T = stg[(q4 + r) * kStg + cb * 16 + r16];
+        const float sg = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(fmaf(c1, d[cb][r], c0)));
+        const float ds = sg - T;
+        const float gv = ds * fmaf(-sg, sg, sg) * two_a;
+        if (interior) {
+          lsum = fmaf(ds, ds, lsum);
+          g[r] = gv;
+        } else {
+          const bool ok = yb + q4 + r < Ly && xs0 + cb * 16 + r16 < Lx;
+          lsum = ok ? fmaf(ds, ds, lsum) : lsum;
+          g[r] = ok ? gv : 0.f;
+        }
+      }
+      split4(g, gh[cb], gl[cb]);
+    }
+    if (img) {
+      // (each lane rewrites the entries it has just read, then the wave reads
+      // them back as rows: the wave's LDS accesses complete in order)
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) stg[(q4 + r) * kStg + cb * 16 + r16] = d[cb][r] * inv_pscale;
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = q * 4 + (lane >> 4), col = r16 * 4;
+        const float4 v = *reinterpret_cast<const float4*>(stg + row * kStg + col);
+        const int y = yb + row, x = xs0 + col;
+        if (y >= Ly) continue;
+        float* dst = img + (size_t)y * Lx + x;
+        if (ivec) {
+          if (x < Lx)
+            __builtin_nontemporal_store(float4_t{v.x, v.y, v.z, v.w},
+                                        reinterpret_cast<float4_t*>(dst));
+        } else {
+          if (x < Lx) dst[0] = v.x;
+          if (x + 1 < Lx) dst[1] = v.y;
+          if (x + 2 < Lx) dst[2] = v.z;
+          if (x + 3 < Lx) dst[3] = v.w;
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // Q += TYg^T . G on v_mfma_f32_16x16x16_f16 (K = the tile's 16 rows)
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt)
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb)
+        Q[jt][cb] = mfma3k16(gah[jt], gal[jt], gh[cb], gl[cb], Q[jt][cb]);
+  };
+  {
+    const int ys = y0 + rp * 16;
+    float4 ta0, ta1, ta2, ta3, tb0, tb1, tb2, tb3;
+    half8_t ah0, ah1, al0, al1, bh0, bh1, bl0, bl1;
+    if (ys < y1) {
+      load_t(ys, ta0, ta1, ta2, ta3);
+      load_ty(ys, ah0, ah1, al0, al1);
+    }
+    if (ys + 32 < y1) load_t(ys + 32, tb0, tb1, tb2, tb3);
+    for (int yb = ys; yb < y1; yb += 64) {
+      tile(yb, ta0, ta1, ta2, ta3, ah0, ah1, al0, al1, bh0, bh1, bl0, bl1);
+      if (yb + 32 < y1)
+        tile(yb + 32, tb0, tb1, tb2, tb3, bh0, bh1, bl0, bl1, ah0, ah1, al0, al1);
+    }
+  }
+  // --- the block's Q (64 j x 128 columns) in LDS, then C = Q . E ------------
+  lsum = wave_sum_f(lsum);
+  __syncthreads();  // every wave is done with the P operand
+  float* sQ = reinterpret_cast<float*>(smem);
+#pragma unroll
+  for (int pass = 0; pass < 2; ++pass) {
+    if (rp == pass) {
+#pragma unroll
+      for (int jt = 0; jt < 4; ++jt)
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            float* e = sQ + (16 * jt + q4 + r) * kQS + ch * 64 + cb * 16 + r16;
+            *e = pass ? *e + Q[jt][cb][r] : Q[jt][cb][r];
+          }
+    }
+    __syncthreads();
+  }
+  // largest |Q| of the block: Q 2^sh in [2^13, 2^14) keeps both fp16 halves of
+  // the split in range whatever the image's level
+  float mx = 0.f;
+  for (int i = tid; i < 64 * 128; i += 256) mx = fmaxf(mx, fabsf(sQ[(i >> 7) * kQS + (i & 127)]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+  __syncthreads();  // (red was last read before the first barrier of the pass loop)
+  if (lane == 0) red[wave] = mx;
+  __syncthreads();
+  mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  int eq = 0;
+  (void)frexpf(mx > 0.f ? mx : 1.f, &eq);
+  const int sh = 14 - eq;
+  // wave w: column tiles kt = 2 w, 2 w + 1 of C, all four j tiles
+  float4_t C[4][2];
+#pragma unroll
+  for (int jt = 0; jt < 4; ++jt)
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) C[jt][kk] = float4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    half8_t bh[2], bl[2];
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      const float4* eb = Ebf + ((size_t)((x0 >> 5) + s) * 8 + 2 * wave + kk) * 2 * 64 + lane;
+      bh[kk] = __builtin_bit_cast(half8_t, eb[0]);
+      bl[kk] = __builtin_bit_cast(half8_t, eb[64]);
+    }
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt) {
+      const float* row = sQ + (16 * jt + r16) * kQS + 32 * s + kq;
+      const float4 u0 = *reinterpret_cast<const float4*>(row);
+      const float4 u1 = *reinterpret_cast<const float4*>(row + 4);
+      float v[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+      for (int t = 0; t < 8; ++t) v[t] = ldexpf(v[t], sh);
+      half8_t ah, al;
+      split8(v, ah, al);
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk) C[jt][kk] = mfma3(ah, al, bh[kk], bl[kk], C[jt][kk]);
+    }
+  }
+  __syncthreads();  // every read of Q ends before C overwrites it
+#pragma unroll
+  for (int jt = 0; jt < 4; ++jt)
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        sQ[(16 * jt + q4 + r) * kQS + 16 * (2 * wave + kk) + r16] = ldexpf(C[jt][kk][r], -sh);
+  if (lane == 0) red[wave] = lsum;
+  __syncthreads();
+  if (tid == 0) lout[pidx] = red[0] + red[1] + red[2] + red[3];
+  const int Gy = 4 * Ny + 1;
+  float2* out = gout + (size_t)pidx * Gy * 32;
+  for (int o = tid; o < Gy * 32; o += 256) {
+    const int gyi = o >> 5, gx = o & 31, gy = gyi - 2 * Ny;
+    float2 val = make_float2(0.f, 0.f);
+    if (gx <= 2 * Nx) {
+      const int a = gy >= 0 ? gy : -gy, gi = (gy >= 0 ? gx : -gx) + 2 * Nx;
+      const float* c0 = sQ + (2 * a) * kQS + 2 * gi;
+      const float* c1 = c0 + kQS;
+      const float re = c0[0] + c1[1], im = c1[0] - c0[1];
+      val = make_float2(re, gy >= 0 ? im : -im);
+    }
+    out[o] = val;
+  }
+}
+
+template <bool TVEC>
+__global__ __launch_bounds__(256, 2) void k_synth_resist(
+    const float2* __restrict__ spec, int Nx, int Ny, int Lx, int Ly, int Lypad, int rs,
+    const float4* __restrict__ Ef, const _Float16* __restrict__ TYh,
+    const _Float16* __restrict__ TYl, const float* __restrict__ tgt, float a_sig, float thr,
+    const half4_t* __restrict__ TYgf, const float4* __restrict__ Ebf, float* __restrict__ image,
+    float2* __restrict__ gpart, float* __restrict__ lpart) {
+  const int b = blockIdx.y, strip = blockIdx.x / rs, rh = blockIdx.x - strip * rs;
+  const int parts = gridDim.x;
+  const int per = (Lypad / 32 + rs - 1) / rs * 32;
+  const int y0 = min(Lypad, rh * per), y1 = min(Lypad, y0 + per);
+  const size_t ib = (size_t)b * Lx * Ly;
+  synth_resist_strip<TVEC>(strip * kColsSynth, y0, y1, blockIdx.x, spec + (size_t)b * 32 * (4 * Nx + 1),
+                     Nx, Ny, Lx, Ly, Ef, TYh, TYl, tgt + ib, a_sig, thr, TYgf, Ebf,
+                     image ? image + ib : nullptr,
+                     gpart + (size_t)b * parts * (4 * Ny + 1) * 32, lpart + (size_t)b * parts);
+}
+
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 void split_half(double v, _Float16* hi, _Float16* lo) {
@@ -1587,6 +1973,58 @@ static int get_fused_tables(klt_ctx* ctx, SocsPlan* p, FusedTables** out) {
       }
     KLT_TRY(t->twyg.ensure(ty.size() * sizeof(float2)));
     KLT_HIP(hipMemcpy(t->twyg.ptr, ty.data(), ty.size() * sizeof(float2), hipMemcpyHostToDevice));
+    // fused adjoint synthesis: TYg[y][2 gy] = cos, TYg[y][2 gy + 1] = -sin of
+    // 2 pi gy (y - (Ly+1)/2)/Ly (gy <= 2Ny, y < Ly) in the A-fragment order of
+    // v_mfma_f32_16x16x16_f16 (tile yt, j tile jt, half h, lane l, t < 4:
+    // y = 16 yt + 4 (l/16) + t, j = 16 jt + l % 16)
+    std::vector<_Float16> tg((size_t)t->Lypad / 16 * 4 * 2 * 64 * 4, (_Float16)0.f);
+    for (int yt = 0; yt < t->Lypad / 16; ++yt)
+      for (int jt = 0; jt < 4; ++jt)
+        for (int ln = 0; ln < 64; ++ln)
+          for (int tt = 0; tt < 4; ++tt) {
+            const int y = 16 * yt + 4 * (ln / 16) + tt, j = 16 * jt + ln % 16, gy = j / 2;
+            double v = 0.0;
+            if (y < Ly && gy <= 2 * Ny) {
+              long long r = ((long long)gy * (y - coy)) % Ly;
+              if (r < 0) r += Ly;
+              const double a = 2.0 * M_PI * (double)r / (double)Ly;
+              v = (j & 1) ? -std::sin(a) : std::cos(a);
+            }
+            _Float16 vh, vl;
+            split_half(v, &vh, &vl);
+            const size_t o = ((((size_t)yt * 4 + jt) * 2) * 64 + ln) * 4 + tt;
+            tg[o] = vh;
+            tg[o + 64 * 4] = vl;
+          }
+    KLT_TRY(t->TYgf.ensure(tg.size() * sizeof(_Float16)));
+    KLT_HIP(hipMemcpy(t->TYgf.ptr, tg.data(), tg.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+    // E[x][k] (k = 2 gxi + c: cos / sin of 2 pi (gxi - 2Nx)(x - (Lx+1)/2)/Lx,
+    // zero past Lx and 4Nx+1) in the B-fragment order of
+    // v_mfma_f32_16x16x32_f16 (step xs, k tile kt, half h, lane l, t < 8:
+    // x = 32 xs + 8 (l/16) + t, k = 16 kt + l % 16)
+    const int Lxp = round_up(Lx, kColsSynth), Gx = 4 * Nx + 1;
+    std::vector<_Float16> eb((size_t)Lxp / 32 * 8 * 2 * 64 * 8);
+    for (int xs = 0; xs < Lxp / 32; ++xs)
+      for (int kt = 0; kt < 8; ++kt)
+        for (int ln = 0; ln < 64; ++ln)
+          for (int tt = 0; tt < 8; ++tt) {
+            const int x = 32 * xs + 8 * (ln / 16) + tt, k = 16 * kt + ln % 16;
+            const int gxi = k / 2, c = k % 2;
+            double v = 0.0;
+            if (x < Lx && gxi < Gx) {
+              long long r = ((long long)(gxi - 2 * Nx) * (x - cox)) % Lx;
+              if (r < 0) r += Lx;
+              const double ang = 2.0 * M_PI * (double)r / (double)Lx;
+              v = c ? std::sin(ang) : std::cos(ang);
+            }
+            _Float16 vh, vl;
+            split_half(v, &vh, &vl);
+            const size_t o = ((((size_t)xs * 8 + kt) * 2) * 64 + ln) * 8 + tt;
+            eb[o] = vh;
+            eb[o + 64 * 8] = vl;
+          }
+    KLT_TRY(t->Ebf.ensure(eb.size() * sizeof(_Float16)));
+    KLT_HIP(hipMemcpy(t->Ebf.ptr, eb.data(), eb.size() * sizeof(_Float16), hipMemcpyHostToDevice));
   }
   std::vector<float2> tw(kM);
   for (int j = 0; j < kM; ++j) {
@@ -1676,6 +2114,9 @@ static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, in
   }
   for (int v = 1; v <= kMaxRs; v *= 2) {
     if (v == best.rs || !split_ok(nrow32, v)) continue;
+    // the adjoint keeps one resist-band partial per synthesis block: at most
+    // 16 blocks per CU
+    if (kind == 1 && (size_t)nstrips * v * nb > 4096) continue;
     klt_ctx::Tuned c = best;
     c.rs = v;
     KLT_TRY(time_run(c, &ms));
@@ -2074,26 +2515,26 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
   const int SB = std::min(B, 512);
   KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk * kMaxKs * Wy * 16 * sizeof(float2)));
   KLT_TRY(ctx->spec.ensure((size_t)SB * 32 * Gx * sizeof(float2)));
-  KLT_TRY(ctx->gpart.ensure((size_t)SB * nblk * kMaxKs * Gy * 32 * sizeof(float2)));
   KLT_TRY(ctx->gspec.ensure((size_t)SB * 32 * Gx * sizeof(float2)));
-  KLT_TRY(ctx->lpart.ensure((size_t)SB * nblk * kMaxKs * sizeof(float)));
   KLT_TRY(ctx->ipart.ensure((size_t)kGroupSlots * kM * kM * sizeof(float)));
   KLT_TRY(ctx->gam_part.ensure((size_t)kGroupSlots * (2 * Nx + 1) * (2 * Ny + 1) * sizeof(float2)));
   KLT_TRY(ctx->mhat.ensure((size_t)SB * (2 * Nx + 1) * Wy * sizeof(float2)));
   KLT_TRY(ctx->gcoarse.ensure((size_t)SB * kM * kM * sizeof(float)));
   KLT_TRY(ctx->gfull.ensure((size_t)SB * Gx * Gy * sizeof(float2)));
   KLT_TRY(ctx->gsum.ensure((size_t)SB * std::max(kM * kM, 2 * (2 * Nx + 1) * Wy) * sizeof(float)));
-  if (!images) {
-    KLT_TRY(ctx->bwd_img.ensure((size_t)SB * Lx * Ly * sizeof(float)));
-  }
   const float inv_area = 1.0f / ((float)Lx * (float)Ly);
   const int dcx = (Lx + 1) / 2 - Lx / 2, dcy = (Ly + 1) / 2 - Ly / 2;
+  // whole 16-byte target vectors: every image of the batch starts aligned
+  const bool tvec = Lx % 4 == 0 && (reinterpret_cast<uintptr_t>(targets) & 15) == 0;
   for (int b0 = 0; b0 < B; b0 += SB) {
     const int nb = std::min(SB, B - b0);
     const size_t off = (size_t)b0 * Lx * Ly;
-    float* I = images ? images + off : ctx->bwd_img.as<float>();
+    float* I = images ? images + off : nullptr;
     auto run = [&](const klt_ctx::Tuned& c) -> int {
-      const int ks = c.ks, rs = c.rs, parts = nblk * ks;
+      const int ks = c.ks, rs = c.rs, parts = nblk * ks, sparts = nstrips * rs;
+      // resist-band partials of the fused synthesis, one per (strip, row range)
+      KLT_TRY(ctx->gpart.ensure((size_t)nb * sparts * Gy * 32 * sizeof(float2)));
+      KLT_TRY(ctx->lpart.ensure((size_t)nb * sparts * sizeof(float)));
       launch_band<2, false>(ctx, "k_band_mfma", nblk, ks, nb, masks + off, nullptr, 0.f, 0.f,
                             Lx, Ly, ft->Kpad, ft->Wf.as<float4>(), p->twy_in.as<float2>(), p->NP, Nx, Ny,
                             16, ctx->qpart.as<float2>(), nullptr);
@@ -2111,16 +2552,24 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
                    ctx->gsum.as<float>(), 1, Nx, Ny, ft->tw64.as<float2>(),
                    ctx->spec.as<float2>());
       }
-      KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma, dim3(nstrips * rs, nb), dim3(256), 0,
-                 ctx->spec.as<float2>(), Nx, Lx, Ly, ft->Lypad, rs, ft->Ef.as<float4>(),
-                 ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), I);
-      launch_band<4, true>(ctx, "k_band_resist", nblk, ks, nb, I, targets + off, a_sig, thr, Lx,
-                           Ly, ft->Kpad, ft->Wgf.as<float4>(), ft->twyg.as<float2>(), 32, 2 * Nx, 2 * Ny, 32,
-                           ctx->gpart.as<float2>(), ctx->lpart.as<float>());
+      // the image tile by tile, its resist gradient G and G's band partials;
+      // the image itself is written only when the caller asks for it
+      if (tvec)
+        KLT_LAUNCH(ctx, "k_synth_resist", k_synth_resist<true>, dim3(sparts, nb), dim3(256), 0,
+                   ctx->spec.as<float2>(), Nx, Ny, Lx, Ly, ft->Lypad, rs, ft->Ef.as<float4>(),
+                   ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), targets + off, a_sig, thr,
+                   ft->TYgf.as<half4_t>(), ft->Ebf.as<float4>(), I, ctx->gpart.as<float2>(),
+                   ctx->lpart.as<float>());
+      else
+        KLT_LAUNCH(ctx, "k_synth_resist", k_synth_resist<false>, dim3(sparts, nb), dim3(256), 0,
+                   ctx->spec.as<float2>(), Nx, Ny, Lx, Ly, ft->Lypad, rs, ft->Ef.as<float4>(),
+                   ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), targets + off, a_sig, thr,
+                   ft->TYgf.as<half4_t>(), ft->Ebf.as<float4>(), I, ctx->gpart.as<float2>(),
+                   ctx->lpart.as<float>());
       // each image's G^, coarse-grid g_c and loss once, for all kernel groups
       KLT_LAUNCH(ctx, "k_resist_sum", k_resist_sum,
                  dim3((unsigned)(Gx * Gy + 255) / 256, nb), dim3(256), 0,
-                 ctx->gpart.as<float2>(), ctx->lpart.as<float>(), parts, Nx, Ny, inv_area,
+                 ctx->gpart.as<float2>(), ctx->lpart.as<float>(), sparts, Nx, Ny, inv_area,
                  ctx->gfull.as<float2>(), losses + b0);
       KLT_LAUNCH(ctx, "k_resist_coarse", k_resist_coarse, dim3(nb), dim3(256), 0,
                  ctx->gfull.as<float2>(), Nx, Ny, ft->tw64.as<float2>(),
