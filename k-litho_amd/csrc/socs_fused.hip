@@ -433,6 +433,14 @@ __device__ __forceinline__ float wave_sum_f(float v) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
+// Sum over the 64 lanes in double (butterfly of lane exchanges), valid in
+// every lane.
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 // 8-point DFT, X[k] = sum_n x[n] e^{s 2 pi i nk/8}, s = +1 (inverse) or -1.
 template <int S>
 __device__ __forceinline__ void dft8(float2* x) {
@@ -1124,7 +1132,7 @@ __global__ __launch_bounds__(256) void k_abbe_grad_coarse(
   __shared__ float2 sM[31 * 31];
   __shared__ float2 sB[62 * kSB];      // two points' rows (31 each)
   __shared__ float2 sX[2 * 32 * kXS];  // FFT scratch of the two points
-  __shared__ float red[4][2];
+  __shared__ double red[4][2];
   const int b = blockIdx.y, grp = blockIdx.x, ngrp = gridDim.x, tid = threadIdx.x;
   const int per = (npts + ngrp - 1) / ngrp, s0 = min(npts, grp * per), s1 = min(npts, s0 + per);
   const int g = tid >> 3, t = tid & 7, wave = tid >> 6, lane = tid & 63;
@@ -1184,7 +1192,9 @@ __global__ __launch_bounds__(256) void k_abbe_grad_coarse(
       }
     }
     __syncthreads();
-    float parta = 0.f, partb = 0.f;
+    // the sum over the coarse grid cancels strongly (g_c takes both signs):
+    // products in FP32, sums in FP64
+    double parta = 0.0, partb = 0.0;
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int ux = g + 32 * q;
@@ -1200,12 +1210,12 @@ __global__ __launch_bounds__(256) void k_abbe_grad_coarse(
       fft64_group2<1>(va, vb, twi, xs, xs2, t, true);
 #pragma unroll
       for (int k1 = 0; k1 < 8; ++k1) {
-        parta += gc[q][k1] * (va[k1].x * va[k1].x + va[k1].y * va[k1].y);
-        partb += gc[q][k1] * (vb[k1].x * vb[k1].x + vb[k1].y * vb[k1].y);
+        parta += (double)(gc[q][k1] * (va[k1].x * va[k1].x + va[k1].y * va[k1].y));
+        partb += (double)(gc[q][k1] * (vb[k1].x * vb[k1].x + vb[k1].y * vb[k1].y));
       }
     }
-    parta = wave_sum_f(parta);
-    partb = wave_sum_f(partb);
+    parta = wave_sum_d(parta);
+    partb = wave_sum_d(partb);
     if (lane == 0) {
       red[wave][0] = parta;
       red[wave][1] = partb;
@@ -1214,8 +1224,7 @@ __global__ __launch_bounds__(256) void k_abbe_grad_coarse(
     __syncthreads();
     if (tid < 2 && (tid == 0 || two)) {
       const int2 p = tid == 0 ? pa : pb;
-      const double tot = (double)red[0][tid] + (double)red[1][tid] + (double)red[2][tid] +
-                         (double)red[3][tid];
+      const double tot = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
       grad[(size_t)b * srcSize * srcSize + (size_t)(p.y + pg.sh) * srcSize + p.x + pg.sh] =
           (float)(scale * tot);
     }
