@@ -689,6 +689,9 @@ def main():
         "kernels": kern,
         "hbm_stream_rates": stream_rates,
     }
+    out["tuner"] = [{"pass": r[0], "batch": r[1], "Lx": r[2], "Ly": r[3], "N": r[4], "K": r[6],
+                     "band_rows": r[10], "band_column_splits": r[7], "synth_row_splits": r[8],
+                     "kernel_groups": r[9], "ms": r[11]} for r in ctx.tune_report()]
     out.update(extras)
     print(json.dumps(out))
     bank.close()

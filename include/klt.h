@@ -89,9 +89,14 @@ int klt_ctx_set_autotune(klt_ctx* ctx, int on);
  * factor that does not fit a launch shape is lowered to the largest that
  * does.  Pinned shapes are not timed and not recorded in the tune report. */
 int klt_ctx_set_splits(klt_ctx* ctx, int ks, int rs, int kg);
+/* Pins the tile height of the mask band pass of later klt_socs_forward /
+ * klt_socs_backward calls: 64 or 128 rows per block (0: tuned, or 128 under
+ * the fixed rule).  Like klt_ctx_set_splits, it has no reference counterpart:
+ * it exposes a choice of the autotuner. */
+int klt_ctx_set_band_rows(klt_ctx* ctx, int rows);
 /* One line per tuned shape: "socs_forward|socs_backward B Lx Ly Nx Ny nk
- * column_splits row_splits kernel_groups ms"; returns the byte count or an
- * error. */
+ * column_splits row_splits kernel_groups band_rows ms"; returns the byte
+ * count or an error. */
 int klt_ctx_tune_report(klt_ctx* ctx, char* buf, int len);
 
 /* ---- host boundary: sources, masks, sizing, file formats ---------------

@@ -304,6 +304,14 @@ int klt_ctx_set_splits(klt_ctx* ctx, int ks, int rs, int kg) {
   return KLT_OK;
 }
 
+int klt_ctx_set_band_rows(klt_ctx* ctx, int rows) {
+  if (!ctx) return klt::set_error(KLT_ERR_ARG, "klt_ctx_set_band_rows: ctx is NULL");
+  if (rows != 0 && rows != 64 && rows != 128)
+    return klt::set_error(KLT_ERR_ARG, "klt_ctx_set_band_rows: rows must be 0, 64 or 128");
+  ctx->pin_rb = rows;
+  return KLT_OK;
+}
+
 int klt_ctx_tune_report(klt_ctx* ctx, char* buf, int len) {
   if (!ctx || !buf || len <= 0)
     return klt::set_error(KLT_ERR_ARG, "klt_ctx_tune_report: bad argument");
@@ -311,9 +319,9 @@ int klt_ctx_tune_report(klt_ctx* ctx, char* buf, int len) {
   for (auto& kv : ctx->fused_tune) {
     char line[256];
     const auto& k = kv.first;
-    snprintf(line, sizeof line, "%s %d %d %d %d %d %d %d %d %d %.6f\n",
+    snprintf(line, sizeof line, "%s %d %d %d %d %d %d %d %d %d %d %.6f\n",
              k[0] == 0 ? "socs_forward" : "socs_backward", k[1], k[2], k[3], k[4], k[5], k[6],
-             kv.second.ks, kv.second.rs, kv.second.kg, kv.second.ms);
+             kv.second.ks, kv.second.rs, kv.second.kg, kv.second.rb, kv.second.ms);
     out += line;
   }
   if ((int)out.size() + 1 > len)

@@ -198,12 +198,13 @@ struct klt_ctx {
   // Nx, Ny, nk), chosen by timing on first use while `autotune` is on
   struct Tuned {
     int ks, rs, kg;  // band column ranges, synthesis row ranges, kernel groups
+    int rb;          // rows per band block (64 or 128)
     float ms;
   };
   bool autotune = true;
   std::map<std::array<int, 7>, Tuned> fused_tune;
   // split factors pinned by klt_ctx_set_splits (0: tuned or fixed rule)
-  int pin_ks = 0, pin_rs = 0, pin_kg = 0;
+  int pin_ks = 0, pin_rs = 0, pin_kg = 0, pin_rb = 0;
   // captured mixed batches, most recent last (a few kept), and the last one run
   std::vector<std::unique_ptr<klt::BatchGraph>> batches;
   klt::BatchGraph* last_batch = nullptr;
@@ -252,10 +253,12 @@ int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, 
                        int Ly, float* images);
 // The launches of one fused forward pass over nb masks (kernel half-size
 // <= 15) with explicit tables and intermediates, split factors ks (band column
-// ranges, <= 8), rs (synthesis row ranges, <= 16) and kg (kernel groups).
+// ranges, <= 8), rs (synthesis row ranges, <= 16) and kg (kernel groups), and
+// rb rows per band block (64 or 128; qpart then holds ceil(Ly/rb) ks partials
+// per image).
 int socs_fused_launch(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int nb, int Lx,
                       int Ly, float* images, const FusedView& v, const FusedScratch& w,
-                      int ks, int rs, int kg);
+                      int ks, int rs, int kg, int rb = 128);
 // Intermediate bytes per image of socs_fused_launch, and the fixed split rule
 // (two blocks per CU) for nb images.
 size_t fused_scratch_bytes(int Lx, int Ly, int Nx, int Ny, int ks, int kg);

@@ -133,7 +133,10 @@ __device__ __forceinline__ void load8(const float* row, int x, int Lx, bool vali
 // different column ranges give partials that simply add).  in0 / in1: that
 // image; the block writes partial `pidx` of its partials (out_img:
 // [parts][2By+1][ldo]) and of its loss partials (lpart_img).
-template <int NCB, bool RESIST, bool FULL>
+// RT: 16-row MFMA tiles per wave, so 64 RT rows per block (the tile shape
+// the tuner picks for the forward and backward passes: 128 rows for large
+// batches, 64 where more blocks fill the GPU).
+template <int NCB, bool RESIST, bool FULL, int RT = 2>
 __device__ __forceinline__ void band_block(
     int blk, int ch0, int ch1, int pidx, const float* __restrict__ in0,
     const float* __restrict__ in1, float a_sig, float thr, int Lx, int Ly, int Kpad,
@@ -157,23 +160,27 @@ __device__ __forceinline__ void band_block(
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int r16 = lane & 15, kq = (lane >> 4) * 8;
-  const int ybase = blk * kRowsBand + wave * 32;
+  static_assert(RT == 1 || RT == 2, "one or two row tiles per wave");
+  const int ybase = blk * (64 * RT) + wave * (16 * RT);
   const size_t ib = 0;
   const int ya = ybase + r16, yb = ybase + 16 + r16;
-  const bool va = ya < Ly, vb = yb < Ly;
+  const bool va = ya < Ly, vb = RT == 2 && yb < Ly;
   // rows past the image read row 0: their products land in rows of R that
   // the epilogue never reads
   const size_t ra = ib + (size_t)(va ? ya : 0) * Lx, rb = ib + (size_t)(vb ? yb : 0) * Lx;
-  float4_t acc[2][NCB];
+  float4_t acc[RT][NCB];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < RT; ++a)
 #pragma unroll
     for (int c = 0; c < NCB; ++c) acc[a][c] = float4_t{0.f, 0.f, 0.f, 0.f};
   float lsum = 0.f;
   // Rows stream DEPTH steps ahead of their use.  Each step converts its rows
   // to split halves first, which frees their registers for the rows of step
   // k + 32 DEPTH, then issues those rows, then runs the products.
-  constexpr int DEPTH = RESIST ? 1 : 2;
+  // (one row tile a wave streams twice as many steps ahead: the same bytes in
+  // flight)
+  constexpr int DEPTH = (RESIST ? 1 : 2) * (RT == 1 ? 2 : 1);
+  static_assert(kBandChunk % DEPTH == 0, "prefetch slots cycle within a chunk");
   float pa[DEPTH][8], pb[DEPTH][8], pta[DEPTH][8], ptb[DEPTH][8];
   auto fetch_rows = [&](int k, float* a8, float* b8, float* ta8, float* tb8) {
     if constexpr (FULL) {
@@ -183,13 +190,16 @@ __device__ __forceinline__ void band_block(
       k = min(k, Lx - 32);
       const float4 a0 = *reinterpret_cast<const float4*>(in0 + ra + k + kq);
       const float4 a1 = *reinterpret_cast<const float4*>(in0 + ra + k + kq + 4);
-      const float4 b0 = *reinterpret_cast<const float4*>(in0 + rb + k + kq);
-      const float4 b1 = *reinterpret_cast<const float4*>(in0 + rb + k + kq + 4);
       a8[0] = a0.x; a8[1] = a0.y; a8[2] = a0.z; a8[3] = a0.w;
       a8[4] = a1.x; a8[5] = a1.y; a8[6] = a1.z; a8[7] = a1.w;
-      b8[0] = b0.x; b8[1] = b0.y; b8[2] = b0.z; b8[3] = b0.w;
-      b8[4] = b1.x; b8[5] = b1.y; b8[6] = b1.z; b8[7] = b1.w;
+      if (RT == 2) {
+        const float4 b0 = *reinterpret_cast<const float4*>(in0 + rb + k + kq);
+        const float4 b1 = *reinterpret_cast<const float4*>(in0 + rb + k + kq + 4);
+        b8[0] = b0.x; b8[1] = b0.y; b8[2] = b0.z; b8[3] = b0.w;
+        b8[4] = b1.x; b8[5] = b1.y; b8[6] = b1.z; b8[7] = b1.w;
+      }
       if (RESIST) {
+        static_assert(!RESIST || RT == 2, "the resist band runs two row tiles a wave");
         const float4 c0 = *reinterpret_cast<const float4*>(in1 + ra + k + kq);
         const float4 c1 = *reinterpret_cast<const float4*>(in1 + ra + k + kq + 4);
         const float4 d0 = *reinterpret_cast<const float4*>(in1 + rb + k + kq);
@@ -201,7 +211,7 @@ __device__ __forceinline__ void band_block(
       }
     } else {
       load8(in0 + ra, k + kq, Lx, true, false, a8);
-      load8(in0 + rb, k + kq, Lx, true, false, b8);
+      if (RT == 2) load8(in0 + rb, k + kq, Lx, true, false, b8);
       if (RESIST) {
         load8(in1 + ra, k + kq, Lx, true, false, ta8);
         load8(in1 + rb, k + kq, Lx, true, false, tb8);
@@ -236,7 +246,7 @@ __device__ __forceinline__ void band_block(
     float4 stp0, stp1;
 #pragma unroll
     for (int s = 0; s < C; ++s) {
-      const int k = (ch * C + s) * 32, sl = DEPTH == 2 ? (s & 1) : 0;
+      const int k = (ch * C + s) * 32, sl = s % DEPTH;
       half8_t wh[NCB], wl[NCB];
 #pragma unroll
       for (int c = 0; c < NCB; ++c) {
@@ -247,7 +257,7 @@ __device__ __forceinline__ void band_block(
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
         va8[t] = pa[sl][t];
-        vb8[t] = pb[sl][t];
+        vb8[t] = RT == 2 ? pb[sl][t] : 0.f;
       }
       if (RESIST) {
 #pragma unroll
@@ -263,7 +273,7 @@ __device__ __forceinline__ void band_block(
       }
       half8_t ah, al, bh, bl;
       split8(va8, ah, al);
-      split8(vb8, bh, bl);
+      if (RT == 2) split8(vb8, bh, bl);
       if (s > 0) {
         wdst[((s - 1) * PS) * 256 + tid] = stp0;
         if (PS == 2) wdst[((s - 1) * PS + 1) * 256 + tid] = stp1;
@@ -278,7 +288,7 @@ __device__ __forceinline__ void band_block(
 #pragma unroll
       for (int c = 0; c < NCB; ++c) {
         acc[0][c] = mfma3(ah, al, wh[c], wl[c], acc[0][c]);
-        acc[1][c] = mfma3(bh, bl, wh[c], wl[c], acc[1][c]);
+        if constexpr (RT == 2) acc[1][c] = mfma3(bh, bl, wh[c], wl[c], acc[1][c]);
       }
     }
     wdst[((C - 1) * PS) * 256 + tid] = stp0;
@@ -295,11 +305,11 @@ __device__ __forceinline__ void band_block(
   const int Wy = 2 * By + 1;
   // this lane's column twiddles: A[|ny| = 16 nt + r16][row 4 (lane / 16) + t]
   // of row tile a (zero for rows past the image and |ny| > By)
-  float2 tw[NT][2][4];
+  float2 tw[NT][RT][4];
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < RT; ++a)
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int y = ybase + a * 16 + (lane >> 4) * 4 + t, ny = nt * 16 + r16;
@@ -310,7 +320,7 @@ __device__ __forceinline__ void band_block(
   // what falls below its normal range is < 2^-36 of the largest value
   float mx = 0.f;
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < RT; ++a)
 #pragma unroll
     for (int c = 0; c < NCB; ++c)
 #pragma unroll
@@ -320,9 +330,9 @@ __device__ __forceinline__ void band_block(
   int ex;
   (void)frexpf(mx, &ex);
   const int sh = 14 - ex;
-  half4_t rh[2][NCB], rl[2][NCB];
+  half4_t rh[RT][NCB], rl[RT][NCB];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < RT; ++a)
 #pragma unroll
     for (int c = 0; c < NCB; ++c) {
       float v[4];
@@ -332,9 +342,9 @@ __device__ __forceinline__ void band_block(
     }
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) {
-    half4_t th[2][2], tl[2][2];  // [row tile][re / im]
+    half4_t th[RT][2], tl[RT][2];  // [row tile][re / im]
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
+    for (int a = 0; a < RT; ++a) {
       float re[4], im[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
@@ -348,7 +358,7 @@ __device__ __forceinline__ void band_block(
     for (int c = 0; c < NCB; ++c) {
       float4_t p0 = {0.f, 0.f, 0.f, 0.f}, p1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
+      for (int a = 0; a < RT; ++a) {
         p0 = mfma3k16(th[a][0], tl[a][0], rh[a][c], rl[a][c], p0);
         p1 = mfma3k16(th[a][1], tl[a][1], rh[a][c], rl[a][c], p1);
       }
@@ -388,7 +398,7 @@ __device__ __forceinline__ void band_block(
 }
 // grid (parts * nimg): block w takes part w % parts (= row block * ks +
 // column range) of image w / parts.
-template <int NCB, bool RESIST, bool FULL>
+template <int NCB, bool RESIST, bool FULL, int RT = 2>
 __global__ __launch_bounds__(256, (RESIST || !FULL) ? 2 : 4) void k_band_mfma(
     const float* __restrict__ in0, const float* __restrict__ in1, float a_sig, float thr,
     int Lx, int Ly, int Kpad, int ks, int parts, const float4* Wf,
@@ -399,7 +409,7 @@ __global__ __launch_bounds__(256, (RESIST || !FULL) ? 2 : 4) void k_band_mfma(
   const int blk = px / ks, kh = px - blk * ks;
   const int ch0 = min(nchunk, kh * per), ch1 = min(nchunk, ch0 + per);
   const size_t ib = (size_t)b * Lx * Ly;
-  band_block<NCB, RESIST, FULL>(blk, ch0, ch1, px, in0 + ib, RESIST ? in1 + ib : nullptr, a_sig,
+  band_block<NCB, RESIST, FULL, RT>(blk, ch0, ch1, px, in0 + ib, RESIST ? in1 + ib : nullptr, a_sig,
                                 thr, Lx, Ly, Kpad, Wf, twy, ldtw, Bx, By, ldo,
                                 out_part + (size_t)b * parts * (2 * By + 1) * ldo,
                                 RESIST ? lpart + (size_t)b * parts : nullptr);
@@ -1856,7 +1866,7 @@ namespace klt {
 
 // Band launch: the branch-free vector-load variant needs 32-column tiles and
 // 16-byte-aligned inputs.
-template <int NCB, bool RESIST>
+template <int NCB, bool RESIST, int RT = 2>
 static void launch_band(klt_ctx* ctx, const char* name, int nblk, int ks, int B,
                         const float* in0, const float* in1, float a_sig, float thr, int Lx,
                         int Ly, int Kpad, const float4* w, const float2* twy, int ldtw, int Bx,
@@ -1866,10 +1876,10 @@ static void launch_band(klt_ctx* ctx, const char* name, int nblk, int ks, int B,
   const int parts = nblk * ks;
   const dim3 grid(parts * B);
   if (full)
-    KLT_LAUNCH(ctx, name, (k_band_mfma<NCB, RESIST, true>), grid, dim3(256), 0, in0, in1, a_sig,
+    KLT_LAUNCH(ctx, name, (k_band_mfma<NCB, RESIST, true, RT>), grid, dim3(256), 0, in0, in1, a_sig,
                thr, Lx, Ly, Kpad, ks, parts, w, twy, ldtw, Bx, By, ldo, out, lpart);
   else
-    KLT_LAUNCH(ctx, name, (k_band_mfma<NCB, RESIST, false>), grid, dim3(256), 0, in0, in1, a_sig,
+    KLT_LAUNCH(ctx, name, (k_band_mfma<NCB, RESIST, false, RT>), grid, dim3(256), 0, in0, in1, a_sig,
                thr, Lx, Ly, Kpad, ks, parts, w, twy, ldtw, Bx, By, ldo, out, lpart);
 }
 
@@ -2055,13 +2065,15 @@ static bool split_ok(int units, int parts) {
   return parts >= 1 && (parts - 1) * ((units + parts - 1) / parts) < units;
 }
 
-// Picks the split factors for one launch shape: ks column ranges per band
-// row block, rs row ranges per synthesis strip, kg kernel groups per image in
-// the coarse-grid stage.  With tuning on (the default) the first call of a
-// shape runs `run` twice per candidate on the context stream, timed with HIP
-// events, one factor at a time, and keeps the fastest; with tuning off, or
-// while the stream is being captured or launches are being profiled, the
-// smallest factors that give each grid two blocks per CU (one for kg).
+// Picks the tile shape and split factors for one launch shape: rb rows per
+// band block (64 or 128), ks column ranges per band row block, rs row ranges
+// per synthesis strip, kg kernel groups per image in the coarse-grid stage
+// (nk / kg kernels per block).  With tuning on (the default) the first call of
+// a shape runs `run` twice per candidate on the context stream, timed with
+// HIP events, one factor at a time, and keeps the fastest; with tuning off,
+// or while the stream is being captured or launches are being profiled,
+// 128-row band blocks and the smallest factors that give each grid two
+// blocks per CU (one for kg).
 static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, int Ny, int nk,
                       int nchunk, int nblk, int nstrips, int nrow32,
                       const std::function<int(const klt_ctx::Tuned&)>& run,
@@ -2072,10 +2084,10 @@ static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, in
     *out = it->second;
     return KLT_OK;
   }
-  klt_ctx::Tuned h{1, 1, 1, 0.f};
+  klt_ctx::Tuned h{1, 1, 1, kRowsBand, 0.f};
   fused_fixed_splits(nb, Lx, Ly, nk, &h.ks, &h.rs, &h.kg);
   auto kg_ok = [&](int v) { return v <= nk && (v == 1 || v * nb <= kGroupSlots); };
-  if (ctx->pin_ks || ctx->pin_rs || ctx->pin_kg) {
+  if (ctx->pin_ks || ctx->pin_rs || ctx->pin_kg || ctx->pin_rb) {
     auto fit = [](int v, int hi, const std::function<bool(int)>& ok) {
       v = std::min(v, hi);
       while (v > 1 && !ok(v)) --v;
@@ -2084,6 +2096,7 @@ static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, in
     if (ctx->pin_ks) h.ks = fit(ctx->pin_ks, kMaxKs, [&](int v) { return split_ok(nchunk, v); });
     if (ctx->pin_rs) h.rs = fit(ctx->pin_rs, kMaxRs, [&](int v) { return split_ok(nrow32, v); });
     if (ctx->pin_kg) h.kg = fit(ctx->pin_kg, nk, kg_ok);
+    if (ctx->pin_rb) h.rb = ctx->pin_rb;
     *out = h;
     return KLT_OK;
   }
@@ -2094,11 +2107,11 @@ static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, in
     return KLT_OK;
   }
   hipEvent_t e0 = ctx->take_event(), e1 = ctx->take_event();
-  // each candidate runs twice and keeps the faster time, so that one slow
-  // run (a clock change, another process on the device) does not decide
+  // each candidate runs three times and keeps the fastest time, so that one
+  // slow run (a clock change, another process on the device) does not decide
   auto time_run = [&](const klt_ctx::Tuned& c, float* ms) -> int {
     float best_ms = 0.f;
-    for (int rep = 0; rep < 2; ++rep) {
+    for (int rep = 0; rep < 3; ++rep) {
       float t = 0.f;
       KLT_HIP(hipEventRecord(e0, ctx->stream));
       KLT_TRY(run(c));
@@ -2113,13 +2126,25 @@ static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, in
   KLT_TRY(run(h));  // warm-up (first launches of this shape)
   klt_ctx::Tuned best = h;
   KLT_TRY(time_run(best, &best.ms));
+  // a candidate replaces the best so far only when it is faster by more than
+  // the run-to-run spread of the streaming passes (about 1 %)
+  auto better = [&](float ms) { return ms < 0.99f * best.ms; };
   float ms = 0.f;
   for (int v = 1; v <= kMaxKs; v *= 2) {
     if (v == best.ks || !split_ok(nchunk, v)) continue;
     klt_ctx::Tuned c = best;
     c.ks = v;
     KLT_TRY(time_run(c, &ms));
-    if (ms < best.ms) best = c, best.ms = ms;
+    if (better(ms)) best = c, best.ms = ms;
+  }
+  // band tile height: 128 rows (two 16-row MFMA tiles a wave) or 64 (one,
+  // twice the blocks and prefetch depth)
+  for (int v : {kRowsBand, 64}) {
+    if (v == best.rb) continue;
+    klt_ctx::Tuned c = best;
+    c.rb = v;
+    KLT_TRY(time_run(c, &ms));
+    if (better(ms)) best = c, best.ms = ms;
   }
   for (int v = 1; v <= kMaxRs; v *= 2) {
     if (v == best.rs || !split_ok(nrow32, v)) continue;
@@ -2129,14 +2154,14 @@ static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, in
     klt_ctx::Tuned c = best;
     c.rs = v;
     KLT_TRY(time_run(c, &ms));
-    if (ms < best.ms) best = c, best.ms = ms;
+    if (better(ms)) best = c, best.ms = ms;
   }
   for (int v : {1, 2, 4, 8, 12, 16, 24}) {
     if (v == best.kg || !kg_ok(v)) continue;
     klt_ctx::Tuned c = best;
     c.kg = v;
     KLT_TRY(time_run(c, &ms));
-    if (ms < best.ms) best = c, best.ms = ms;
+    if (better(ms)) best = c, best.ms = ms;
   }
   ctx->event_pool.push_back(e0);
   ctx->event_pool.push_back(e1);
@@ -2145,15 +2170,29 @@ static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, in
   return KLT_OK;
 }
 
+// The mask band of nb images in row blocks of rb rows (64 or 128), each split
+// into ks column ranges: partials [nb][ceil(Ly/rb) ks][2Ny+1][16].
+static void launch_mask_band(klt_ctx* ctx, int rb, int ks, int nb, const float* masks, int Lx,
+                             int Ly, int Kpad, const float4* Wf, const float2* twy, int ldtw,
+                             int Nx, int Ny, float2* out) {
+  const int nblk = (Ly + rb - 1) / rb;
+  if (rb == 64)
+    launch_band<2, false, 1>(ctx, "k_band_mfma", nblk, ks, nb, masks, nullptr, 0.f, 0.f, Lx, Ly,
+                             Kpad, Wf, twy, ldtw, Nx, Ny, 16, out, nullptr);
+  else
+    launch_band<2, false, 2>(ctx, "k_band_mfma", nblk, ks, nb, masks, nullptr, 0.f, 0.f, Lx, Ly,
+                             Kpad, Wf, twy, ldtw, Nx, Ny, 16, out, nullptr);
+}
+
 int socs_fused_launch(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int nb, int Lx,
                       int Ly, float* images, const FusedView& v, const FusedScratch& w,
-                      int ks, int rs, int kg) {
+                      int ks, int rs, int kg, int rb) {
   const int Nx = kb->Nx, Ny = kb->Ny;
-  const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
+  if (rb != 64 && rb != kRowsBand) return set_error(KLT_ERR_ARG, "socs_fused_launch: band rows");
+  const int nblk = (Ly + rb - 1) / rb;
   const int nstrips = (Lx + kColsSynth - 1) / kColsSynth;
   const float inv_area = 1.0f / ((float)Lx * (float)Ly);
-  launch_band<2, false>(ctx, "k_band_mfma", nblk, ks, nb, masks, nullptr, 0.f, 0.f, Lx, Ly,
-                        v.Kpad, v.Wf, v.twy, v.ldtw, Nx, Ny, 16, w.qpart, nullptr);
+  launch_mask_band(ctx, rb, ks, nb, masks, Lx, Ly, v.Kpad, v.Wf, v.twy, v.ldtw, Nx, Ny, w.qpart);
   KLT_LAUNCH(ctx, "k_socs_mid", k_socs_mid, dim3(kg, nb), dim3(256), 0, w.qpart, nblk * ks, Nx,
              Ny, inv_area, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk, v.tw64,
              w.spec, w.ipart);
@@ -2199,9 +2238,10 @@ int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, 
   KLT_TRY(get_fused_tables(ctx, p, &ft));
   const int Nx = kb->Nx, Ny = kb->Ny, Wy = 2 * Ny + 1, Gx = 4 * Nx + 1;
   const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
+  const int nblk64 = (Ly + 63) / 64;  // row blocks at the smaller band tile
   const int nstrips = (Lx + kColsSynth - 1) / kColsSynth;
   const int SB = std::min(B, 1024);
-  KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk * kMaxKs * Wy * 16 * sizeof(float2)));
+  KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk64 * kMaxKs * Wy * 16 * sizeof(float2)));
   KLT_TRY(ctx->spec.ensure((size_t)SB * 32 * Gx * sizeof(float2)));
   KLT_TRY(ctx->ipart.ensure((size_t)kGroupSlots * kM * kM * sizeof(float)));
   KLT_TRY(ctx->gsum.ensure((size_t)SB * kM * kM * sizeof(float)));
@@ -2215,7 +2255,7 @@ int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, 
     const float* m = masks + (size_t)b0 * Lx * Ly;
     float* out = images + (size_t)b0 * Lx * Ly;
     auto run = [&](const klt_ctx::Tuned& c) -> int {
-      return socs_fused_launch(ctx, kb, m, nb, Lx, Ly, out, view, work, c.ks, c.rs, c.kg);
+      return socs_fused_launch(ctx, kb, m, nb, Lx, Ly, out, view, work, c.ks, c.rs, c.kg, c.rb);
     };
     klt_ctx::Tuned c{};
     KLT_TRY(pick_split(ctx, 0, nb, Lx, Ly, Nx, Ny, kb->nk, ft->Kpad / (kBandChunk * 32), nblk,
@@ -2520,9 +2560,10 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
   KLT_TRY(get_fused_tables(ctx, p, &ft));
   const int Nx = kb->Nx, Ny = kb->Ny, Wy = 2 * Ny + 1, Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
   const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
+  const int nblk64 = (Ly + 63) / 64;  // row blocks at the smaller band tile
   const int nstrips = (Lx + kColsSynth - 1) / kColsSynth;
   const int SB = std::min(B, 512);
-  KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk * kMaxKs * Wy * 16 * sizeof(float2)));
+  KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk64 * kMaxKs * Wy * 16 * sizeof(float2)));
   KLT_TRY(ctx->spec.ensure((size_t)SB * 32 * Gx * sizeof(float2)));
   KLT_TRY(ctx->gspec.ensure((size_t)SB * 32 * Gx * sizeof(float2)));
   KLT_TRY(ctx->ipart.ensure((size_t)kGroupSlots * kM * kM * sizeof(float)));
@@ -2540,13 +2581,13 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
     const size_t off = (size_t)b0 * Lx * Ly;
     float* I = images ? images + off : nullptr;
     auto run = [&](const klt_ctx::Tuned& c) -> int {
-      const int ks = c.ks, rs = c.rs, parts = nblk * ks, sparts = nstrips * rs;
+      const int ks = c.ks, rs = c.rs, parts = (Ly + c.rb - 1) / c.rb * ks;
+      const int sparts = nstrips * rs;
       // resist-band partials of the fused synthesis, one per (strip, row range)
       KLT_TRY(ctx->gpart.ensure((size_t)nb * sparts * Gy * 32 * sizeof(float2)));
       KLT_TRY(ctx->lpart.ensure((size_t)nb * sparts * sizeof(float)));
-      launch_band<2, false>(ctx, "k_band_mfma", nblk, ks, nb, masks + off, nullptr, 0.f, 0.f,
-                            Lx, Ly, ft->Kpad, ft->Wf.as<float4>(), p->twy_in.as<float2>(), p->NP, Nx, Ny,
-                            16, ctx->qpart.as<float2>(), nullptr);
+      launch_mask_band(ctx, c.rb, ks, nb, masks + off, Lx, Ly, ft->Kpad, ft->Wf.as<float4>(),
+                       p->twy_in.as<float2>(), p->NP, Nx, Ny, ctx->qpart.as<float2>());
       // each image's M^ once, for the forward and the adjoint coarse stages
       KLT_LAUNCH(ctx, "k_band_assemble", k_band_assemble,
                  dim3((unsigned)((2 * Nx + 1) * Wy + 255) / 256, nb), dim3(256), 0,

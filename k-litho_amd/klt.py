@@ -65,6 +65,7 @@ def lib() -> C.CDLL:
         L.klt_ctx_profile_report.argtypes = [vp, C.c_char_p, i]
         L.klt_ctx_set_autotune.argtypes = [vp, i]
         L.klt_ctx_set_splits.argtypes = [vp, i, i, i]
+        L.klt_ctx_set_band_rows.argtypes = [vp, i]
         L.klt_ctx_tune_report.argtypes = [vp, C.c_char_p, i]
         f = C.c_float
         cs = C.c_char_p
@@ -281,9 +282,14 @@ class Context:
         of later batched calls (0: automatic)."""
         check(lib().klt_ctx_set_splits(self._h, ks, rs, kg))
 
+    def set_band_rows(self, rows: int = 0):
+        """Pins the rows per block of the mask band pass of later batched calls
+        (64 or 128; 0: automatic)."""
+        check(lib().klt_ctx_set_band_rows(self._h, rows))
+
     def tune_report(self) -> list:
-        """[(kind, B, Lx, Ly, Nx, Ny, nk, column_splits, row_splits, kernel_groups, ms)]
-        of the tuned shapes."""
+        """[(kind, B, Lx, Ly, Nx, Ny, nk, column_splits, row_splits, kernel_groups,
+        band_rows, ms)] of the tuned shapes."""
         buf = C.create_string_buffer(1 << 16)
         n = lib().klt_ctx_tune_report(self._h, buf, len(buf))
         if n < 0:
@@ -291,7 +297,7 @@ class Context:
         out = []
         for line in buf.value.decode().splitlines():
             f = line.split()
-            out.append((f[0], *map(int, f[1:10]), float(f[10])))
+            out.append((f[0], *map(int, f[1:11]), float(f[11])))
         return out
 
     def profile_report(self) -> dict:

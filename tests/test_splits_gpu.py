@@ -1,16 +1,21 @@
-"""Pinned split factors (klt_ctx_set_splits) of the batched SOCS passes.
+"""Pinned split factors (klt_ctx_set_splits) and band tile heights
+(klt_ctx_set_band_rows) of the batched SOCS passes.
 
-The tuner only keeps the split that runs fastest, so the other candidates'
+The tuner only keeps the choice that runs fastest, so the other candidates'
 results are never read back. These tests pin every factor explicitly at
-1024 x 1024 (8 band row blocks, 8 column chunks, 32 synthesis row groups):
+1024 x 1024 (8 or 16 band row blocks, 8 column chunks, 32 synthesis row
+groups):
   * forward, 2 masks, 24 kernels: band column ranges ks in {1, 2, 4, 8},
     synthesis row ranges rs in {1, 4, 16}, kernel groups kg in {1, 5, 24}
-    (5 leaves groups of unequal size). Every combination gives the same
-    images as ks = rs = kg = 1 to relative L2 1e-6 (the splits only reorder
-    FP32 sums), and that image matches the FP64 oracle to 1e-5;
-  * backward, 2 relaxed masks: (ks, rs, kg) in {(1, 1, 1), (8, 16, 24),
-    (2, 4, 5)}: gradients agree to 1e-6 and match the FP64 oracle to 1e-4,
-    losses to 1e-5;
+    (5 leaves groups of unequal size), band blocks of 128 and 64 rows. Every
+    combination gives the same images as ks = rs = kg = 1 to relative L2 1e-6
+    (the splits only reorder FP32 sums), and that image matches the FP64
+    oracle to 1e-5;
+  * backward, 2 relaxed masks: (ks, rs, kg, rows) in {(1, 1, 1, 128),
+    (8, 16, 24, 64), (2, 4, 5, 64), (2, 4, 5, 128)}: gradients agree to 1e-6
+    and match the FP64 oracle to 1e-4, losses to 1e-5;
+  * 64-row band blocks at odd, unequal sizes (the last block partly past the
+    image), forward and backward, against the oracle;
   * factors beyond a shape's limits are lowered, not rejected: ks = 64 at
     256 x 256 (2 column chunks) runs and equals ks = 2.
 """
@@ -39,6 +44,7 @@ def _bank(ctx, seed):
 def pinned(ctx):
     yield ctx
     ctx.set_splits(0, 0, 0)
+    ctx.set_band_rows(0)
 
 
 def test_forward_pinned_splits(pinned):
@@ -49,9 +55,10 @@ def test_forward_pinned_splits(pinned):
     m = (rng.random((2, L, L)) > 0.55).astype(np.float32)
     dm = torch.from_numpy(m).cuda()
     base = None
-    for ks, rs, kg in itertools.product((1, 2, 4, 8), (1, 4, 16), (1, 5, 24)):
-        print(f"case forward ks={ks} rs={rs} kg={kg}", flush=True)
+    for rb, ks, rs, kg in itertools.product((128, 64), (1, 2, 4, 8), (1, 4, 16), (1, 5, 24)):
+        print(f"case forward rows={rb} ks={ks} rs={rs} kg={kg}", flush=True)
         ctx.set_splits(ks, rs, kg)
+        ctx.set_band_rows(rb)
         out = ctx.socs_forward(bank, dm)
         ctx.sync()
         img = out.cpu().numpy()
@@ -61,7 +68,7 @@ def test_forward_pinned_splits(pinned):
                 assert relerr(img[i], O.socs_image(m[i], kr, ki, sc, N, N)) < 1e-5
             continue
         for i in range(2):
-            assert relerr(img[i], base[i]) < 1e-6, (ks, rs, kg, i)
+            assert relerr(img[i], base[i]) < 1e-6, (rb, ks, rs, kg, i)
     bank.close()
 
 
@@ -78,20 +85,49 @@ def test_backward_pinned_splits(pinned):
     ref = [O.socs_loss_grad(m[i].astype(np.float64), t[i], kr, ki, sc, N, N, thr=0.02)
            for i in range(2)]
     base = None
-    for ks, rs, kg in ((1, 1, 1), (8, 16, 24), (2, 4, 5)):
-        print(f"case backward ks={ks} rs={rs} kg={kg}", flush=True)
+    for ks, rs, kg, rb in ((1, 1, 1, 128), (8, 16, 24, 64), (2, 4, 5, 64), (2, 4, 5, 128)):
+        print(f"case backward ks={ks} rs={rs} kg={kg} rows={rb}", flush=True)
         ctx.set_splits(ks, rs, kg)
+        ctx.set_band_rows(rb)
         lo, g, _ = klt.socs_backward(ctx, bank, dm, dt, thr=0.02)
         ctx.sync()
         g, lo = g.cpu().numpy(), lo.cpu().numpy()
         for i in range(2):
             Lr, _, gr = ref[i]
-            assert relerr(g[i], gr) < 1e-4, (ks, rs, kg, i)
-            assert abs(lo[i] - Lr) < 1e-5 * Lr, (ks, rs, kg, i)
+            assert relerr(g[i], gr) < 1e-4, (ks, rs, kg, rb, i)
+            assert abs(lo[i] - Lr) < 1e-5 * Lr, (ks, rs, kg, rb, i)
             if base is not None:
-                assert relerr(g[i], base[0][i]) < 1e-6, (ks, rs, kg, i)
+                assert relerr(g[i], base[0][i]) < 1e-6, (ks, rs, kg, rb, i)
         if base is None:
             base = (g, lo)
+    bank.close()
+
+
+@pytest.mark.parametrize("Lx,Ly", [(300, 255), (257, 130)])
+def test_band_rows_64_odd_sizes(pinned, Lx, Ly):
+    import klt
+    import torch
+    print(f"case band rows 64 at {Lx}x{Ly}", flush=True)
+    ctx = pinned
+    n = 6
+    rng = np.random.default_rng(Lx + Ly)
+    T = (2 * n + 1) ** 2
+    kr = (rng.standard_normal((8, T)) / np.sqrt(T)).astype(np.float32)
+    ki = (rng.standard_normal((8, T)) / np.sqrt(T)).astype(np.float32)
+    sc = np.sort(rng.uniform(0.2, 3.0, 8))[::-1].astype(np.float32).copy()
+    bank = ctx.kernels(kr, ki, sc, 2 * n + 1, 2 * n + 1)
+    t = (rng.random((2, Ly, Lx)) > 0.6).astype(np.float32)
+    m = np.clip(t + rng.uniform(-0.1, 0.1, t.shape), 0, 1).astype(np.float32)
+    ctx.set_band_rows(64)
+    img = ctx.socs_forward(bank, torch.from_numpy(m).cuda())
+    ctx.sync()
+    img = img.cpu().numpy()
+    lo, g, _ = klt.socs_backward(ctx, bank, m, t, thr=0.02)
+    for i in range(2):
+        assert relerr(img[i], O.socs_image(m[i], kr, ki, sc, n, n)) < 1e-5
+        Lr, _, gr = O.socs_loss_grad(m[i].astype(np.float64), t[i], kr, ki, sc, n, n, thr=0.02)
+        assert relerr(g[i], gr) < 1e-4
+        assert abs(lo[i] - Lr) < 1e-5 * Lr
     bank.close()
 
 
