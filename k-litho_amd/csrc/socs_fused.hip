@@ -1496,6 +1496,13 @@ __global__ void k_batch_begin(unsigned* __restrict__ strips_done, int n,
 constexpr int kQS = 132;  // LDS row stride (floats) of the block's Q and C
 
 // TVEC: the target rows are whole 16-byte vectors (Lx % 4 == 0, aligned).
+// Wave w owns the 32 columns x0 + 32 w .. + 32 over all the block's rows; the
+// four waves step through the rows together, one 16-row tile a step.  The
+// tile's TY (synthesis) and TYg (resist band) operands are staged in LDS once
+// for the block, double-buffered behind one barrier a step, each thread
+// copying 2 x 16 bytes fetched a step ahead; the target streams two steps
+// ahead in registers (8 rows x 128 B per load).  The wave's P operand (its 32
+// columns) and its 64 x 32 Q accumulators stay in registers: 3 waves per SIMD.
 template <bool TVEC>
 __device__ __forceinline__ void synth_resist_strip(
     int x0, int y0, int y1, int pidx, const float2* __restrict__ spec, int Nx, int Ny, int Lx,
@@ -1505,9 +1512,13 @@ __device__ __forceinline__ void synth_resist_strip(
     float2* __restrict__ gout, float* __restrict__ lout) {
   constexpr int B_BYTES = 2 * 16 * 64 * 16, P_BYTES = 2 * kColsSynth * kPStride * 2;
   constexpr int MAIN = B_BYTES > P_BYTES ? B_BYTES : P_BYTES;
-  constexpr int kStg = 68;
+  constexpr int kStg = 36;  // staging row stride (floats) of a wave's 16 x 32 tile
+  // after the prologue: [slot][ks][hi, lo][lane] TY fragments (2 x 4 KB),
+  // [slot][jt][hi, lo][lane] TYg fragments (2 x 4 KB), the waves' staging rows
+  constexpr int TY_F4 = 4 * 64, TG_F4 = 4 * 64;  // float4 per slot
+  static_assert(2 * (TY_F4 + TG_F4) * 16 + 4 * 16 * kStg * 4 <= MAIN, "staging fits");
   static_assert(64 * kQS * 4 <= MAIN, "Q and C fit in the operand LDS");
-  __shared__ __attribute__((aligned(16))) char smem[MAIN + 4 * 16 * kStg * 4];
+  __shared__ __attribute__((aligned(16))) char smem[MAIN];
   __shared__ float red[4];
   half8_t* sBh = reinterpret_cast<half8_t*>(smem);
   half8_t* sBl = sBh + 16 * 64;
@@ -1515,11 +1526,10 @@ __device__ __forceinline__ void synth_resist_strip(
   _Float16* sPl = sPh + kColsSynth * kPStride;
   const int tid = threadIdx.x;
   const int Gx = 4 * Nx + 1;
-  // (the wave index as a scalar: everything derived from it, the rows of each
-  // step included, is then known to be uniform and branches on it are scalar)
+  // (the wave index as a scalar: everything derived from it is then known to
+  // be uniform)
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int r16 = lane & 15, kq = (lane >> 4) * 8, q4 = (lane >> 4) * 4;
-  float* stg = reinterpret_cast<float*>(smem + MAIN) + wave * 16 * kStg;
   // --- P = E . B on the matrix cores, exactly as synth_strip -----------------
   float part = 0.f;
   for (int i = tid; i < 32 * Gx; i += 256) {
@@ -1588,26 +1598,47 @@ __device__ __forceinline__ void synth_resist_strip(
         }
   }
   __syncthreads();
-  // --- rows: synthesis tile, G, Q accumulation -------------------------------
-  const int ch = wave & 1, rp = wave >> 1;
-  const int xs0 = x0 + ch * 64;
+  // the wave's P operand: columns 32 wave + 16 cb + lane % 16, K = j
+  half8_t pbh[2][2], pbl[2][2];
+#pragma unroll
+  for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      const int c = 32 * wave + cb * 16 + r16;
+      pbh[cb][ks] = *reinterpret_cast<const half8_t*>(sPh + c * kPStride + ks * 32 + kq);
+      pbl[cb][ks] = *reinterpret_cast<const half8_t*>(sPl + c * kPStride + ks * 32 + kq);
+    }
+  __syncthreads();  // the operand LDS becomes the staging area
+  float4* sTY = reinterpret_cast<float4*>(smem);          // [2][TY_F4]
+  float4* sTG = sTY + 2 * TY_F4;                           // [2][TG_F4]
+  float* stg = reinterpret_cast<float*>(sTG + 2 * TG_F4) + wave * 16 * kStg;
+  // --- rows: synthesis tile, G, Q accumulation --------------------------------
+  const int xw = x0 + 32 * wave;
   const bool ivec = img && (Lx & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 15) == 0;
-  float4_t Q[4][4];
+  float4_t Q[4][2];
 #pragma unroll
   for (int jt = 0; jt < 4; ++jt)
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) Q[jt][cb] = float4_t{0.f, 0.f, 0.f, 0.f};
+    for (int cb = 0; cb < 2; ++cb) Q[jt][cb] = float4_t{0.f, 0.f, 0.f, 0.f};
   float lsum = 0.f;
-  // the target tile of the 16 rows at yb: lane holds rows 4 q + lane / 16,
-  // columns 4 (lane % 16) .. + 4 of the wave's 64 columns.  Branch-free: rows
-  // and columns past the image read clamped addresses, whose values the
-  // gradient below masks out (a conditional load here would make the compiler
-  // wait for every load in flight, the prefetched tiles included).
-  // (named registers, not an array: an array passed by reference stays in
-  // scratch memory, and every prefetch then waits for its load)
+  // this thread's share of a tile's staged operands: 16 bytes of the TY
+  // fragments (ks = tid / 128, half = tid / 64 % 2, lane = tid % 64) and 16
+  // bytes of the TYg fragments
+  auto fetch_ops = [&](int yb, float4& o_ty, float4& o_tg) {
+    const int ln = tid & 63;
+    const _Float16* src = ((tid >> 6) & 1 ? TYl : TYh) + (size_t)(yb + (ln & 15)) * 64 +
+                          (tid >> 7) * 32 + (ln >> 4) * 8;
+    o_ty = *reinterpret_cast<const float4*>(src);
+    o_tg = reinterpret_cast<const float4*>(TYgf + (size_t)(yb >> 4) * 512)[tid];
+  };
+  // the target tile of the 16 rows at yb for this wave: lane holds rows
+  // lane / 8 and 8 + lane / 8, columns 4 (lane % 8) .. + 4 of its 32.
+  // Branch-free: rows and columns past the image read clamped addresses,
+  // whose values the gradient masks out.  (Named registers: an array passed
+  // by reference would stay in scratch memory.)
   auto load_t1 = [&](int y, float4& t) {
     y = min(y, Ly - 1);
-    const int x = xs0 + r16 * 4;
+    const int x = xw + (lane & 7) * 4;
     const float* row = tgt + (size_t)y * Lx;
     if constexpr (TVEC) {
       t = *reinterpret_cast<const float4*>(row + min(x, Lx - 4));
@@ -1616,99 +1647,76 @@ __device__ __forceinline__ void synth_resist_strip(
                       row[min(x + 3, Lx - 1)]);
     }
   };
-  auto load_t = [&](int yb, float4& t0, float4& t1, float4& t2, float4& t3) {
-    const int y = yb + (lane >> 4);
-    load_t1(y, t0);
-    load_t1(y + 4, t1);
-    load_t1(y + 8, t2);
-    load_t1(y + 12, t3);
-  };
-  auto load_ty = [&](int yb, half8_t& h0, half8_t& h1, half8_t& l0, half8_t& l1) {
-    const _Float16* th = TYh + (size_t)(yb + r16) * 64 + kq;
-    const _Float16* tl = TYl + (size_t)(yb + r16) * 64 + kq;
-    h0 = *reinterpret_cast<const half8_t*>(th);
-    h1 = *reinterpret_cast<const half8_t*>(th + 32);
-    l0 = *reinterpret_cast<const half8_t*>(tl);
-    l1 = *reinterpret_cast<const half8_t*>(tl + 32);
+  auto load_t = [&](int yb, float4& t0, float4& t1) {
+    load_t1(yb + (lane >> 3), t0);
+    load_t1(yb + 8 + (lane >> 3), t1);
   };
   // s = 1 / (1 + 2^(c1 d + c0)) for the unscaled synthesis output d (I = d / pscale)
   const float c1 = -a_sig * 1.44269504f * inv_pscale, c0 = a_sig * thr * 1.44269504f;
   const float two_a = 2.f * a_sig;
-  // one 16-row tile at yb: synthesis, G, G split into hi / lo halves in the
-  // B-operand layout (lane: rows 4 (lane/16) + r of column 16 cb + lane % 16),
-  // and its Q products.
-  // tc0..3: its target tile, refilled with the tile two steps on (yb + 64);
-  // (ah0..al1): its TY operand, (nh0..nl1) refilled with yb + 32's.
-  auto tile = [&](int yb, float4& tc0, float4& tc1, float4& tc2, float4& tc3, half8_t& ah0,
-                  half8_t& ah1, half8_t& al0, half8_t& al1, half8_t& nh0, half8_t& nh1,
-                  half8_t& nl0, half8_t& nl1) {
-    // this tile's TYg operand (A of the Q products), consumed after the synthesis
-    half4_t gah[4], gal[4];
-    const half4_t* tg = TYgf + (size_t)(yb >> 4) * 8 * 64 + lane;
-#pragma unroll
-    for (int jt = 0; jt < 4; ++jt) {
-      gah[jt] = tg[(jt * 2 + 0) * 64];
-      gal[jt] = tg[(jt * 2 + 1) * 64];
+  // one step: tile yb from staging slot sl; tc0, tc1: its target rows,
+  // refilled with the tile two steps on; o_ty, o_tg: the next tile's operands,
+  // stored to the other slot and refilled with the operands two steps on
+  auto step = [&](int yb, int sl, float4& tc0, float4& tc1, float4& o_ty, float4& o_tg) {
+    if (yb + 16 < y1) {
+      sTY[(sl ^ 1) * TY_F4 + tid] = o_ty;
+      sTG[(sl ^ 1) * TG_F4 + tid] = o_tg;
+      if (yb + 32 < y1) fetch_ops(yb + 32, o_ty, o_tg);
     }
-    if (yb + 32 < y1) load_ty(yb + 32, nh0, nh1, nl0, nl1);
-    float4_t d[4];
+    const float4* ty = sTY + sl * TY_F4 + lane;
+    const half8_t ah0 = __builtin_bit_cast(half8_t, ty[0]);
+    const half8_t al0 = __builtin_bit_cast(half8_t, ty[64]);
+    const half8_t ah1 = __builtin_bit_cast(half8_t, ty[128]);
+    const half8_t al1 = __builtin_bit_cast(half8_t, ty[192]);
+    float4_t d[2];
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
-      const int c = ch * 64 + cb * 16 + r16;
-      const half8_t ph0 = *reinterpret_cast<const half8_t*>(sPh + c * kPStride + kq);
-      const half8_t pl0 = *reinterpret_cast<const half8_t*>(sPl + c * kPStride + kq);
-      const half8_t ph1 = *reinterpret_cast<const half8_t*>(sPh + c * kPStride + 32 + kq);
-      const half8_t pl1 = *reinterpret_cast<const half8_t*>(sPl + c * kPStride + 32 + kq);
+    for (int cb = 0; cb < 2; ++cb) {
       d[cb] = float4_t{0.f, 0.f, 0.f, 0.f};
-      d[cb] = mfma3(ah0, al0, ph0, pl0, d[cb]);
-      d[cb] = mfma3(ah1, al1, ph1, pl1, d[cb]);
+      d[cb] = mfma3(ah0, al0, pbh[cb][0], pbl[cb][0], d[cb]);
+      d[cb] = mfma3(ah1, al1, pbh[cb][1], pbl[cb][1], d[cb]);
     }
-    // target tile through the wave's staging rows into the MFMA output layout
-    {
-      float* st = stg + (lane >> 4) * kStg + r16 * 4;
-      *reinterpret_cast<float4*>(st) = tc0;
-      *reinterpret_cast<float4*>(st + 4 * kStg) = tc1;
-      *reinterpret_cast<float4*>(st + 8 * kStg) = tc2;
-      *reinterpret_cast<float4*>(st + 12 * kStg) = tc3;
-    }
-    if (yb + 64 < y1) load_t(yb + 64, tc0, tc1, tc2, tc3);
-    __builtin_amdgcn_wave_barrier();
-    // tiles wholly inside the image skip the per-entry bounds
-    const bool interior = yb + 16 <= Ly && xs0 + 64 <= Lx;
-    half4_t gh[4], gl[4];
+    // target rows through the wave's staging rows into the MFMA output layout
+    *reinterpret_cast<float4*>(stg + (lane >> 3) * kStg + (lane & 7) * 4) = tc0;
+    *reinterpret_cast<float4*>(stg + (8 + (lane >> 3)) * kStg + (lane & 7) * 4) = tc1;
+    if (yb + 32 < y1) load_t(yb + 32, tc0, tc1);
+    wave_sync();
+    // every target value first (one wait), then the eight entries as
+    // independent branch-free chains; entries past the image count zero
+    float tv[2][4];
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) tv[cb][r] = stg[(q4 + r) * kStg + cb * 16 + r16];
+    half4_t gh[2], gl[2];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      const bool colok = xw + cb * 16 + r16 < Lx;
       float g[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const float T = stg[(q4 + r) * kStg + cb * 16 + r16];
-        const float sg = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(fmaf(c1, d[cb][r], c0)));
-        const float ds = sg - T;
-        const float gv = ds * fmaf(-sg, sg, sg) * two_a;
-        if (interior) {
-          lsum = fmaf(ds, ds, lsum);
-          g[r] = gv;
-        } else {
-          const bool ok = yb + q4 + r < Ly && xs0 + cb * 16 + r16 < Lx;
-          lsum = ok ? fmaf(ds, ds, lsum) : lsum;
-          g[r] = ok ? gv : 0.f;
-        }
+        const bool ok = colok && yb + q4 + r < Ly;
+        const float sg =
+            __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(fmaf(c1, d[cb][r], c0)));
+        const float ds = ok ? sg - tv[cb][r] : 0.f;
+        lsum = fmaf(ds, ds, lsum);
+        g[r] = ds * fmaf(-sg, sg, sg) * two_a;
       }
       split4(g, gh[cb], gl[cb]);
     }
     if (img) {
       // (each lane rewrites the entries it has just read, then the wave reads
-      // them back as rows: the wave's LDS accesses complete in order)
+      // them back as rows)
+      wave_sync();
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb)
+      for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
         for (int r = 0; r < 4; ++r) stg[(q4 + r) * kStg + cb * 16 + r16] = d[cb][r] * inv_pscale;
-      __builtin_amdgcn_wave_barrier();
+      wave_sync();
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int row = q * 4 + (lane >> 4), col = r16 * 4;
+      for (int h = 0; h < 2; ++h) {
+        const int row = 8 * h + (lane >> 3), col = (lane & 7) * 4;
         const float4 v = *reinterpret_cast<const float4*>(stg + row * kStg + col);
-        const int y = yb + row, x = xs0 + col;
+        const int y = yb + row, x = xw + col;
         if (y >= Ly) continue;
         float* dst = img + (size_t)y * Lx + x;
         if (ivec) {
@@ -1723,55 +1731,51 @@ __device__ __forceinline__ void synth_resist_strip(
         }
       }
     }
-    __builtin_amdgcn_wave_barrier();
+    wave_sync();  // the staging rows' reads end before the next step's writes
     // Q += TYg^T . G on v_mfma_f32_16x16x16_f16 (K = the tile's 16 rows)
+    const half4_t* tg = reinterpret_cast<const half4_t*>(sTG + sl * TG_F4) + lane;
 #pragma unroll
-    for (int jt = 0; jt < 4; ++jt)
+    for (int jt = 0; jt < 4; ++jt) {
+      const half4_t gah = tg[(jt * 2 + 0) * 64], gal = tg[(jt * 2 + 1) * 64];
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb)
-        Q[jt][cb] = mfma3k16(gah[jt], gal[jt], gh[cb], gl[cb], Q[jt][cb]);
+      for (int cb = 0; cb < 2; ++cb) Q[jt][cb] = mfma3k16(gah, gal, gh[cb], gl[cb], Q[jt][cb]);
+    }
+    __syncthreads();  // slot sl is read, slot sl ^ 1 written, by every wave
   };
   {
-    const int ys = y0 + rp * 16;
-    float4 ta0, ta1, ta2, ta3, tb0, tb1, tb2, tb3;
-    half8_t ah0, ah1, al0, al1, bh0, bh1, bl0, bl1;
-    if (ys < y1) {
-      load_t(ys, ta0, ta1, ta2, ta3);
-      load_ty(ys, ah0, ah1, al0, al1);
+    float4 ta0, ta1, tb0, tb1, o_ty, o_tg;
+    if (y0 < y1) {
+      fetch_ops(y0, o_ty, o_tg);
+      sTY[tid] = o_ty;
+      sTG[tid] = o_tg;
+      if (y0 + 16 < y1) fetch_ops(y0 + 16, o_ty, o_tg);
+      load_t(y0, ta0, ta1);
+      if (y0 + 16 < y1) load_t(y0 + 16, tb0, tb1);
     }
-    if (ys + 32 < y1) load_t(ys + 32, tb0, tb1, tb2, tb3);
-    for (int yb = ys; yb < y1; yb += 64) {
-      tile(yb, ta0, ta1, ta2, ta3, ah0, ah1, al0, al1, bh0, bh1, bl0, bl1);
-      if (yb + 32 < y1)
-        tile(yb + 32, tb0, tb1, tb2, tb3, bh0, bh1, bl0, bl1, ah0, ah1, al0, al1);
+    __syncthreads();
+    // (y1 - y0 is a multiple of 32: the steps come in pairs)
+    for (int yb = y0; yb < y1; yb += 32) {
+      step(yb, 0, ta0, ta1, o_ty, o_tg);
+      step(yb + 16, 1, tb0, tb1, o_ty, o_tg);
     }
   }
   // --- the block's Q (64 j x 128 columns) in LDS, then C = Q . E ------------
   lsum = wave_sum_f(lsum);
-  __syncthreads();  // every wave is done with the P operand
   float* sQ = reinterpret_cast<float*>(smem);
 #pragma unroll
-  for (int pass = 0; pass < 2; ++pass) {
-    if (rp == pass) {
+  for (int jt = 0; jt < 4; ++jt)
 #pragma unroll
-      for (int jt = 0; jt < 4; ++jt)
+    for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            float* e = sQ + (16 * jt + q4 + r) * kQS + ch * 64 + cb * 16 + r16;
-            *e = pass ? *e + Q[jt][cb][r] : Q[jt][cb][r];
-          }
-    }
-    __syncthreads();
-  }
+      for (int r = 0; r < 4; ++r)
+        sQ[(16 * jt + q4 + r) * kQS + 32 * wave + cb * 16 + r16] = Q[jt][cb][r];
+  __syncthreads();
   // largest |Q| of the block: Q 2^sh in [2^13, 2^14) keeps both fp16 halves of
   // the split in range whatever the image's level
   float mx = 0.f;
   for (int i = tid; i < 64 * 128; i += 256) mx = fmaxf(mx, fabsf(sQ[(i >> 7) * kQS + (i & 127)]));
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-  __syncthreads();  // (red was last read before the first barrier of the pass loop)
   if (lane == 0) red[wave] = mx;
   __syncthreads();
   mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
@@ -1807,7 +1811,7 @@ __device__ __forceinline__ void synth_resist_strip(
       for (int kk = 0; kk < 2; ++kk) C[jt][kk] = mfma3(ah, al, bh[kk], bl[kk], C[jt][kk]);
     }
   }
-  __syncthreads();  // every read of Q ends before C overwrites it
+  __syncthreads();  // every read of Q (and of red) ends before C overwrites it
 #pragma unroll
   for (int jt = 0; jt < 4; ++jt)
 #pragma unroll
@@ -1825,9 +1829,9 @@ __device__ __forceinline__ void synth_resist_strip(
     float2 val = make_float2(0.f, 0.f);
     if (gx <= 2 * Nx) {
       const int a = gy >= 0 ? gy : -gy, gi = (gy >= 0 ? gx : -gx) + 2 * Nx;
-      const float* c0 = sQ + (2 * a) * kQS + 2 * gi;
-      const float* c1 = c0 + kQS;
-      const float re = c0[0] + c1[1], im = c1[0] - c0[1];
+      const float* cr0 = sQ + (2 * a) * kQS + 2 * gi;
+      const float* cr1 = cr0 + kQS;
+      const float re = cr0[0] + cr1[1], im = cr1[0] - cr0[1];
       val = make_float2(re, gy >= 0 ? im : -im);
     }
     out[o] = val;
@@ -1835,7 +1839,7 @@ __device__ __forceinline__ void synth_resist_strip(
 }
 
 template <bool TVEC>
-__global__ __launch_bounds__(256, 2) void k_synth_resist(
+__global__ __launch_bounds__(256, 3) void k_synth_resist(
     const float2* __restrict__ spec, int Nx, int Ny, int Lx, int Ly, int Lypad, int rs,
     const float4* __restrict__ Ef, const _Float16* __restrict__ TYh,
     const _Float16* __restrict__ TYl, const float* __restrict__ tgt, float a_sig, float thr,

@@ -8,9 +8,12 @@ Checks, at 64 x 64 with a random kernel bank (N = 3, 4 kernels):
     different split factors, which reorders FP32 sums), and the images on both
     sides of the sub-batch boundary (0, 1023, 1024, 1099) match the FP64
     oracle to 1e-5;
-  * 520 masks backward: the same for gradients and losses (1e-6 between the
-    calls; gradients 1e-4 and losses 1e-5 against the FP64 oracle at masks 0,
-    511, 512, 519).
+  * 520 masks backward: gradients 1e-4 and losses 1e-5 against the FP64
+    oracle at masks 0, 511, 512, 519 for both calls, and 1e-5 per image
+    between them (the fused synthesis scales each block's resist-band
+    accumulators by a power of two from the block's largest entry before
+    their fp16 split, so other row splits round them differently; losses
+    1e-6).
 """
 import numpy as np
 import pytest
@@ -61,11 +64,16 @@ def test_backward_over_512_masks(ctx):
             for b in range(0, 520, 100)]
     l_parts = np.concatenate([o[0] for o in outs])
     g_parts = np.concatenate([o[1] for o in outs])
-    assert _per_image(g_full, g_parts) < 1e-6
-    assert np.max(np.abs(l_full - l_parts) / np.abs(l_parts)) < 1e-6
     for i in (0, 511, 512, 519):
         L0, _, g = O.socs_loss_grad(masks[i].astype(np.float64), tgts[i], kr, ki, sc, N, N,
                                     thr=0.02)
+        print(f"mask {i}: gradient rel-L2 vs FP64 {relerr(g_full[i], g):.2e} (520-mask call), "
+              f"{relerr(g_parts[i], g):.2e} (100-mask calls)", flush=True)
         assert relerr(g_full[i], g) < 1e-4, i
+        assert relerr(g_parts[i], g) < 1e-4, i
         assert abs(l_full[i] - L0) < 1e-5 * L0, i
+    d = _per_image(g_full, g_parts)
+    print(f"largest per-image gradient difference between the calls {d:.2e}", flush=True)
+    assert d < 1e-5
+    assert np.max(np.abs(l_full - l_parts) / np.abs(l_parts)) < 1e-6
     bank.close()
