@@ -2130,9 +2130,17 @@ static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, in
   KLT_TRY(run(h));  // warm-up (first launches of this shape)
   klt_ctx::Tuned best = h;
   KLT_TRY(time_run(best, &best.ms));
-  // a candidate replaces the best so far only when it is faster by more than
-  // the run-to-run spread of the streaming passes (about 1 %)
-  auto better = [&](float ms) { return ms < 0.99f * best.ms; };
+  // A candidate replaces the best so far only when it is faster by more than
+  // the run-to-run spread of the streaming passes (about 1 %), against the
+  // best timed again right after it: the device's clocks and caches drift over
+  // the tuning runs, which would otherwise favour whichever candidate runs
+  // late.
+  auto better = [&](float ms) -> bool {
+    float again = 0.f;
+    if (time_run(best, &again) != KLT_OK) return false;
+    best.ms = std::min(best.ms, again);
+    return ms < 0.99f * again;
+  };
   float ms = 0.f;
   for (int v = 1; v <= kMaxKs; v *= 2) {
     if (v == best.ks || !split_ok(nchunk, v)) continue;
@@ -2150,8 +2158,11 @@ static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, in
     KLT_TRY(time_run(c, &ms));
     if (better(ms)) best = c, best.ms = ms;
   }
-  for (int v = 1; v <= kMaxRs; v *= 2) {
-    if (v == best.rs || !split_ok(nrow32, v)) continue;
+  // (3, 6 and 12 row ranges as well: the fused adjoint synthesis runs three
+  // blocks per CU, and a grid of a whole number of such rounds leaves no
+  // partly filled last round)
+  for (int v : {1, 2, 3, 4, 6, 8, 12, 16}) {
+    if (v == best.rs || v > kMaxRs || !split_ok(nrow32, v)) continue;
     // the adjoint keeps one resist-band partial per synthesis block: at most
     // 16 blocks per CU
     if (kind == 1 && (size_t)nstrips * v * nb > 4096) continue;

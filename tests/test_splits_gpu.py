@@ -6,14 +6,16 @@ results are never read back. These tests pin every factor explicitly at
 1024 x 1024 (8 or 16 band row blocks, 8 column chunks, 32 synthesis row
 groups):
   * forward, 2 masks, 24 kernels: band column ranges ks in {1, 2, 4, 8},
-    synthesis row ranges rs in {1, 4, 16}, kernel groups kg in {1, 5, 24}
-    (5 leaves groups of unequal size), band blocks of 128 and 64 rows. Every
+    synthesis row ranges rs in {1, 4, 6, 16} (6 leaves a shorter last range),
+    kernel groups kg in {1, 5, 24} (5 leaves groups of unequal size), band
+    blocks of 128 and 64 rows. Every
     combination gives the same images as ks = rs = kg = 1 to relative L2 1e-6
     (the splits only reorder FP32 sums), and that image matches the FP64
     oracle to 1e-5;
   * backward, 2 relaxed masks: (ks, rs, kg, rows) in {(1, 1, 1, 128),
-    (8, 16, 24, 64), (2, 4, 5, 64), (2, 4, 5, 128)}: gradients agree to 1e-6
-    and match the FP64 oracle to 1e-4, losses to 1e-5;
+    (8, 16, 24, 64), (2, 4, 5, 64), (2, 3, 5, 128), (4, 6, 12, 128)}:
+    gradients agree to 1e-5 and match the FP64 oracle to 1e-4, losses to
+    1e-5;
   * 64-row band blocks at odd, unequal sizes (the last block partly past the
     image), forward and backward, against the oracle;
   * factors beyond a shape's limits are lowered, not rejected: ks = 64 at
@@ -55,7 +57,7 @@ def test_forward_pinned_splits(pinned):
     m = (rng.random((2, L, L)) > 0.55).astype(np.float32)
     dm = torch.from_numpy(m).cuda()
     base = None
-    for rb, ks, rs, kg in itertools.product((128, 64), (1, 2, 4, 8), (1, 4, 16), (1, 5, 24)):
+    for rb, ks, rs, kg in itertools.product((128, 64), (1, 2, 4, 8), (1, 4, 6, 16), (1, 5, 24)):
         print(f"case forward rows={rb} ks={ks} rs={rs} kg={kg}", flush=True)
         ctx.set_splits(ks, rs, kg)
         ctx.set_band_rows(rb)
@@ -85,7 +87,8 @@ def test_backward_pinned_splits(pinned):
     ref = [O.socs_loss_grad(m[i].astype(np.float64), t[i], kr, ki, sc, N, N, thr=0.02)
            for i in range(2)]
     base = None
-    for ks, rs, kg, rb in ((1, 1, 1, 128), (8, 16, 24, 64), (2, 4, 5, 64), (2, 4, 5, 128)):
+    for ks, rs, kg, rb in ((1, 1, 1, 128), (8, 16, 24, 64), (2, 4, 5, 64), (2, 3, 5, 128),
+                           (4, 6, 12, 128)):
         print(f"case backward ks={ks} rs={rs} kg={kg} rows={rb}", flush=True)
         ctx.set_splits(ks, rs, kg)
         ctx.set_band_rows(rb)
@@ -97,7 +100,7 @@ def test_backward_pinned_splits(pinned):
             assert relerr(g[i], gr) < 1e-4, (ks, rs, kg, rb, i)
             assert abs(lo[i] - Lr) < 1e-5 * Lr, (ks, rs, kg, rb, i)
             if base is not None:
-                assert relerr(g[i], base[0][i]) < 1e-6, (ks, rs, kg, rb, i)
+                assert relerr(g[i], base[0][i]) < 1e-5, (ks, rs, kg, rb, i)
         if base is None:
             base = (g, lo)
     bank.close()
