@@ -65,6 +65,10 @@ struct FusedTables {
   // resist band as A fragments over 16-row tiles (TYgf) and the synthesis
   // column twiddles E as B fragments over 32-column steps (Ebf)
   DevBuf TYgf, Ebf;
+  // kernel half-sizes 16..31 (k_synth_wide): Wf then holds 64 twiddle rows;
+  // TYw = TY (Lypad x 128) as A fragments over 16-row tiles, Efw the
+  // synthesis column twiddles as A fragments over 8 steps of 32
+  DevBuf TYw, Efw;
 };
 
 // Device pointers of one fused forward launch (socs_fused_launch): the tables
@@ -249,6 +253,17 @@ bool socs_fused_supported(const klt_kernels* kb);
 // spectra spec [B][G][4Nx+1].
 int socs_mid_from_band(klt_ctx* ctx, const klt_kernels* kb, const float2* band, int B,
                        float2* spec, int G);
+// Coarse-grid stage of any half-size N <= 31 (socs.hip): band
+// [B][2Ny+1][2Nx+1] (normalised) to spec [B][G][4Nx+1]; N <= 15 through
+// socs_mid_from_band, above that by direct transforms on the 128-point grid.
+int socs_coarse_spec(klt_ctx* ctx, const klt_kernels* kb, const SocsPlan* p,
+                     const float2* band, int B, float2* spec, int G);
+// Batched SOCS forward for kernel half-sizes 16..31 (socs_fused.hip): the
+// band pass on the matrix cores (64 twiddle rows), the coarse stage of
+// socs_coarse_spec, and the matrix-core synthesis at K = 128.
+bool socs_wide_supported(const klt_kernels* kb);
+int socs_forward_wide(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int B, int Lx,
+                      int Ly, float* images);
 int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int B, int Lx,
                        int Ly, float* images);
 // The launches of one fused forward pass over nb masks (kernel half-size

@@ -520,13 +520,64 @@ static void launch_rows(klt_ctx* ctx, const float2* ccols, int B, int Lx, int Ly
              p->twy_out.as<float2>(), images);
 }
 
+// Coarse-grid stage of nb images: band [nb][2Ny+1][2Nx+1] (normalised,
+// centred) to the intensity spectra spec [nb][G][4Nx+1] (rows 2Ny < gy < G
+// zero).  N <= 15: the 64-point FFT stage of socs_fused.hip; above that the
+// direct transforms on the M = 128 grid, in one block per image while they
+// fit in LDS (N <= 18), else in four launches over device scratch.
+int socs_coarse_spec(klt_ctx* ctx, const klt_kernels* kb, const SocsPlan* p,
+                     const float2* band, int nb, float2* spec, int G) {
+  if (socs_fused_supported(kb)) return socs_mid_from_band(ctx, kb, band, nb, spec, G);
+  const int Nx = kb->Nx, Ny = kb->Ny, M = p->M;
+  const int T = (2 * Nx + 1) * (2 * Ny + 1), Gx = 4 * Nx + 1;
+  const int nB = std::max((2 * Ny + 1) * M, M * Gx);
+  const size_t lds_coarse =
+      sizeof(float2) * (2 * (size_t)T + nB + M) + sizeof(float) * (size_t)M * M;
+  if (lds_coarse <= 160 * 1024) {
+    if (lds_coarse > 64 * 1024)
+      KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_socs_coarse),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_coarse));
+    KLT_LAUNCH(ctx, "k_socs_coarse", k_socs_coarse, dim3(nb), dim3(kThreads), lds_coarse, band,
+               kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk, Nx, Ny, M,
+               p->twM.as<float2>(), spec, G);
+    KLT_HIP(hipGetLastError());
+    return KLT_OK;
+  }
+  // sub-batches of cb images bound the scratch to 256 MiB
+  const size_t szB = (size_t)kb->nk * (2 * Ny + 1) * M * sizeof(float2);
+  const size_t szI = (size_t)M * M * sizeof(float), szR = (size_t)M * Gx * sizeof(float2);
+  const int cb = (int)std::max<size_t>(1, std::min<size_t>(nb, (256u << 20) / (szB + szI + szR)));
+  KLT_TRY(ctx->coarse_ws.ensure((szB + szI + szR) * cb));
+  float2* Bm = ctx->coarse_ws.as<float2>();
+  float* Ic = reinterpret_cast<float*>(ctx->coarse_ws.as<char>() + szB * cb);
+  float2* R = reinterpret_cast<float2*>(ctx->coarse_ws.as<char>() + (szB + szI) * cb);
+  const float2* tw = p->twM.as<float2>();
+  auto grid = [](size_t n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); };
+  for (int c0 = 0; c0 < nb; c0 += cb) {
+    const int nc = std::min(cb, nb - c0);
+    const float2* bd = band + (size_t)c0 * T;
+    const size_t nbk = (size_t)nc * kb->nk * (2 * Ny + 1) * M, ni = (size_t)nc * M * M,
+                 nr = (size_t)nc * M * Gx, ns = (size_t)nc * G * Gx;
+    KLT_LAUNCH(ctx, "k_coarse_b", k_coarse_b, grid(nbk), dim3(kThreads), 0, bd,
+               kb->kern.as<float2>(), kb->nk, Nx, Ny, M, tw, Bm, nbk);
+    KLT_LAUNCH(ctx, "k_coarse_int", k_coarse_int, grid(ni), dim3(kThreads), 0, Bm,
+               kb->scales.as<float>(), kb->nk, Ny, M, tw, Ic, ni);
+    KLT_LAUNCH(ctx, "k_coarse_rx", k_coarse_rx, grid(nr), dim3(kThreads), 0, Ic, Nx, M, tw, R,
+               nr);
+    KLT_LAUNCH(ctx, "k_coarse_gy", k_coarse_gy, grid(ns), dim3(kThreads), 0, R, Nx, Ny, M, tw,
+               spec + (size_t)c0 * G * Gx, G, ns);
+  }
+  KLT_HIP(hipGetLastError());
+  return KLT_OK;
+}
+
 // Runs the SOCS forward for B masks of Lx x Ly already on the device.
 int socs_forward_device(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int B,
                         int Lx, int Ly, float* images) {
   SocsPlan* p = nullptr;
   KLT_TRY(get_socs_plan(ctx, Lx, Ly, kb->Nx, kb->Ny, &p));
   const int NP = p->NP, G = p->G;
-  const int Nx = kb->Nx, Ny = kb->Ny, M = p->M;
+  const int Nx = kb->Nx, Ny = kb->Ny;
   const int T = (2 * Nx + 1) * (2 * Ny + 1), Gx = 4 * Nx + 1;
   const int xblocks = (Lx + 2 * kThreads - 1) / (2 * kThreads);
   // enough blocks to fill 256 CUs several times over
@@ -534,30 +585,14 @@ int socs_forward_device(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
   int chunks = std::min(std::max(1, want), std::max(1, Ly / 8));
   int rows = (Ly + chunks - 1) / chunks;
   chunks = (Ly + rows - 1) / rows;
+  if (NP > 32 || G > 64)
+    return set_error(KLT_ERR_ARG, "SOCS kernel half-size N above 31 is not supported");
   // sub-batches bound the scratch footprint
   const int SB = std::min(B, 512);
   KLT_TRY(ctx->qpart.ensure((size_t)SB * chunks * NP * Lx * sizeof(float2)));
   KLT_TRY(ctx->band.ensure((size_t)SB * T * sizeof(float2)));
   KLT_TRY(ctx->spec.ensure((size_t)SB * G * Gx * sizeof(float2)));
   KLT_TRY(ctx->ccols.ensure((size_t)SB * G * Lx * sizeof(float2)));
-  const int nB = std::max((2 * Ny + 1) * M, M * Gx);
-  const size_t lds_coarse =
-      sizeof(float2) * (2 * (size_t)T + nB + M) + sizeof(float) * (size_t)M * M;
-  if (NP > 32 || G > 64)
-    return set_error(KLT_ERR_ARG, "SOCS kernel half-size N above 31 is not supported");
-  // the single-block coarse stage while it fits in LDS (N <= 18), else the
-  // four-launch one over device scratch, in sub-batches of cb images
-  const bool global_coarse = !socs_fused_supported(kb) && lds_coarse > 160 * 1024;
-  const size_t szB = (size_t)kb->nk * (2 * Ny + 1) * M * sizeof(float2);
-  const size_t szI = (size_t)M * M * sizeof(float), szR = (size_t)M * Gx * sizeof(float2);
-  int cb = 1;
-  if (global_coarse) {
-    cb = (int)std::max<size_t>(1, std::min<size_t>(SB, (256u << 20) / (szB + szI + szR)));
-    KLT_TRY(ctx->coarse_ws.ensure((szB + szI + szR) * cb));
-  } else if (lds_coarse > 64 * 1024) {
-    KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_socs_coarse),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_coarse));
-  }
   for (int b0 = 0; b0 < B; b0 += SB) {
     const int nb = std::min(SB, B - b0);
     const float* m = masks + (size_t)b0 * Lx * Ly;
@@ -570,33 +605,7 @@ int socs_forward_device(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
       case 24: launch_band<24>(ctx, m, nb, Lx, Ly, chunks, rows, p, qpart, band); break;
       default: launch_band<32>(ctx, m, nb, Lx, Ly, chunks, rows, p, qpart, band); break;
     }
-    if (socs_fused_supported(kb)) {
-      KLT_TRY(socs_mid_from_band(ctx, kb, band, nb, ctx->spec.as<float2>(), G));
-    } else if (!global_coarse) {
-      KLT_LAUNCH(ctx, "k_socs_coarse", k_socs_coarse, dim3(nb), dim3(kThreads), lds_coarse,
-                 band, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk, Nx, Ny, M,
-                 p->twM.as<float2>(), ctx->spec.as<float2>(), G);
-    } else {
-      float2* Bm = ctx->coarse_ws.as<float2>();
-      float* Ic = reinterpret_cast<float*>(ctx->coarse_ws.as<char>() + szB * cb);
-      float2* R = reinterpret_cast<float2*>(ctx->coarse_ws.as<char>() + (szB + szI) * cb);
-      const float2* tw = p->twM.as<float2>();
-      auto grid = [](size_t n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); };
-      for (int c0 = 0; c0 < nb; c0 += cb) {
-        const int nc = std::min(cb, nb - c0);
-        const float2* bd = band + (size_t)c0 * T;
-        const size_t nbk = (size_t)nc * kb->nk * (2 * Ny + 1) * M, ni = (size_t)nc * M * M,
-                     nr = (size_t)nc * M * Gx, ns = (size_t)nc * G * Gx;
-        KLT_LAUNCH(ctx, "k_coarse_b", k_coarse_b, grid(nbk), dim3(kThreads), 0, bd,
-                   kb->kern.as<float2>(), kb->nk, Nx, Ny, M, tw, Bm, nbk);
-        KLT_LAUNCH(ctx, "k_coarse_int", k_coarse_int, grid(ni), dim3(kThreads), 0, Bm,
-                   kb->scales.as<float>(), kb->nk, Ny, M, tw, Ic, ni);
-        KLT_LAUNCH(ctx, "k_coarse_rx", k_coarse_rx, grid(nr), dim3(kThreads), 0, Ic, Nx, M, tw,
-                   R, nr);
-        KLT_LAUNCH(ctx, "k_coarse_gy", k_coarse_gy, grid(ns), dim3(kThreads), 0, R, Nx, Ny, M,
-                   tw, ctx->spec.as<float2>() + (size_t)c0 * G * Gx, G, ns);
-      }
-    }
+    KLT_TRY(socs_coarse_spec(ctx, kb, p, band, nb, ctx->spec.as<float2>(), G));
     KLT_LAUNCH(ctx, "k_synth_cols", k_synth_cols, dim3((Lx + kThreads - 1) / kThreads, nb),
                dim3(kThreads), sizeof(float2) * G * Gx, ctx->spec.as<float2>(), G, Nx, Ny, Lx,
                p->twx_out.as<float2>(), ctx->ccols.as<float2>());
@@ -769,10 +778,12 @@ static int socs_entry(klt_ctx* ctx, const klt_kernels* kb, const float* masks, i
     KLT_TRY(ctx->stage_out.ensure(bytes));
     dout = ctx->stage_out.as<float>();
   }
-  if (plain || !klt::socs_fused_supported(kb))
-    KLT_TRY(klt::socs_forward_device(ctx, kb, static_cast<const float*>(dm), nmask, Lx, Ly, dout));
-  else
+  if (!plain && klt::socs_fused_supported(kb))
     KLT_TRY(klt::socs_forward_fused(ctx, kb, static_cast<const float*>(dm), nmask, Lx, Ly, dout));
+  else if (!plain && klt::socs_wide_supported(kb))
+    KLT_TRY(klt::socs_forward_wide(ctx, kb, static_cast<const float*>(dm), nmask, Lx, Ly, dout));
+  else
+    KLT_TRY(klt::socs_forward_device(ctx, kb, static_cast<const float*>(dm), nmask, Lx, Ly, dout));
   if (flags & KLT_HOST_PTRS) {
     KLT_HIP(hipMemcpyAsync(images, dout, bytes, hipMemcpyDeviceToHost, ctx->stream));
     KLT_HIP(hipStreamSynchronize(ctx->stream));

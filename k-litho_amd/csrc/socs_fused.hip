@@ -399,7 +399,7 @@ __device__ __forceinline__ void band_block(
 // grid (parts * nimg): block w takes part w % parts (= row block * ks +
 // column range) of image w / parts.
 template <int NCB, bool RESIST, bool FULL, int RT = 2>
-__global__ __launch_bounds__(256, (RESIST || !FULL) ? 2 : 4) void k_band_mfma(
+__global__ __launch_bounds__(256, (RESIST || !FULL || NCB > 2) ? 2 : 4) void k_band_mfma(
     const float* __restrict__ in0, const float* __restrict__ in1, float a_sig, float thr,
     int Lx, int Ly, int Kpad, int ks, int parts, const float4* Wf,
     const float2* __restrict__ twy, int ldtw, int Bx, int By, int ldo,
@@ -842,10 +842,10 @@ __device__ __forceinline__ void gam_to_spec(const float2* sGam, int Nx, int Ny, 
 }
 
 // grid (ceil(T / 256), B), one thread per entry: the normalised mask band M^
-// [B][(2Ny+1)(2Nx+1)] from the band partials (Hermitian for nx < 0), shared
-// by the coarse stages that follow.
+// [B][(2Ny+1)(2Nx+1)] from the band partials [B][nblk][2Ny+1][ldo]
+// (Hermitian for nx < 0), shared by the coarse stages that follow.
 __global__ __launch_bounds__(256) void k_band_assemble(const float2* __restrict__ mpart,
-                                                       int nblk, int Nx, int Ny,
+                                                       int nblk, int Nx, int Ny, int ldo,
                                                        float inv_area,
                                                        float2* __restrict__ Mfull) {
   const int b = blockIdx.y;
@@ -858,7 +858,7 @@ __global__ __launch_bounds__(256) void k_band_assemble(const float2* __restrict_
     float2 s = make_float2(0.f, 0.f);
 #pragma unroll 8
     for (int k = 0; k < nblk; ++k) {
-      const float2 p = mpart[(((size_t)b * nblk + k) * Wy + r) * 16 + c];
+      const float2 p = mpart[(((size_t)b * nblk + k) * Wy + r) * ldo + c];
       s.x += p.x;
       s.y += p.y;
     }
@@ -1441,6 +1441,225 @@ __global__ __launch_bounds__(256, 4) void k_synth_mfma(const float2* __restrict_
               Lypad, Ef, TYh, TYl, image + (size_t)b * Lx * Ly);
 }
 
+// ---------------------------------------------------------------------------
+// Stage 3 for kernel half-sizes 16..31.  grid (ceil(Lx/128) * rs, B), 256
+// threads, as k_synth_mfma.  The coarse spectrum has up to 63 rows
+// (gy <= 2Ny) and 125 columns (|gx| <= 2Nx), so both products are twice as
+// deep as synth_strip's:
+//   P[x][j] = sum_k E[x][k] B[k][j], K = 256 (k = 2 gxi + c), j < 128
+//     (j = 2 gy + d), once per 128-column strip;
+//   I[y][x] = sum_j TY[y][j] P[x][j], K = 128, per 16-row tile.
+// Wave w owns the columns x0 + 32 w .. + 32 over all the block's rows: its P
+// operand (two 16-column tiles x four 32-deep steps, hi / lo) stays in
+// registers.  The tile's TY operand (16 rows x 128, hi / lo: 8 KB in
+// A-fragment order) is staged in LDS once for the block, double-buffered
+// behind the step's one barrier; each thread copies 2 x 16 bytes fetched a
+// step ahead.  The four waves' outputs meet in an LDS tile, so that the image
+// is written 2 rows x 512 contiguous bytes per wave store.
+// spec: [64][4Nx+1] of this image (rows gy > 2Ny zero).
+// TYw: [Lypad/16][4 ks][hi, lo][64 lanes] x 8 halves, lane l holding
+//   TY[16 yt + l % 16][32 ks + 8 (l/16) + t] (TY[y][2gy] = w_gy cos,
+//   TY[y][2gy+1] = -w_gy sin, as for synth_strip);
+// Efw: [Lxp/16][8 st][hi, lo][64 lanes] x 8 halves, lane l holding
+//   E[16 xt + l % 16][32 st + 8 (l/16) + t].
+// ---------------------------------------------------------------------------
+constexpr int kWPS = 72;   // LDS row stride (halves) of one half of P's transpose
+constexpr int kWOS = 132;  // LDS row stride (floats) of the block's output tile
+
+__device__ __forceinline__ void synth_wide_strip(int x0, int y0, int y1,
+                                                 const float2* __restrict__ spec, int Nx,
+                                                 int Lx, int Ly, const float4* __restrict__ Efw,
+                                                 const float4* __restrict__ TYw,
+                                                 float* __restrict__ img) {
+  constexpr int B_BYTES = 2 * 16 * 64 * 16;           // a quarter of B, hi / lo
+  constexpr int P_BYTES = 2 * kColsSynth * kWPS * 2;  // half of P's transpose, hi / lo
+  constexpr int TY_F4 = 8 * 64;                       // float4 per staged TY tile
+  constexpr int S_BYTES = 2 * TY_F4 * 16 + 2 * 16 * kWOS * 4;
+  constexpr int PRO = B_BYTES > P_BYTES ? B_BYTES : P_BYTES;
+  constexpr int MAIN = PRO > S_BYTES ? PRO : S_BYTES;
+  __shared__ __attribute__((aligned(16))) char smem[MAIN];
+  __shared__ float red[4];
+  half8_t* sBh = reinterpret_cast<half8_t*>(smem);  // [4 steps][4 column tiles][64 lanes]
+  half8_t* sBl = sBh + 16 * 64;
+  _Float16* sPh = reinterpret_cast<_Float16*>(smem);  // [128 columns][kWPS]
+  _Float16* sPl = sPh + kColsSynth * kWPS;
+  const int tid = threadIdx.x;
+  const int Gx = 4 * Nx + 1;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int r16 = lane & 15, kq = (lane >> 4) * 8, q4 = (lane >> 4) * 4;
+  // power-of-two scale putting max |P| <= sum |J| near 2^12 (as synth_strip)
+  float part = 0.f;
+  for (int i = tid; i < 64 * Gx; i += 256) {
+    const float2 v = spec[i];
+    part += fabsf(v.x) + fabsf(v.y);
+  }
+  part = wave_sum_f(part);
+  if (lane == 0) red[wave] = part;
+  __syncthreads();
+  const float bound = red[0] + red[1] + red[2] + red[3];
+  int ex = 0;
+  (void)frexpf(bound > 0.f ? bound : 1.f, &ex);
+  const float pscale = ldexpf(1.0f, 12 - ex), inv_pscale = ldexpf(1.0f, ex - 12);
+  // --- P = E . B, B staged a quarter (128 k x 64 j) at a time ----------------
+  // B[2gxi][2gy] = Jr, B[2gxi+1][2gy] = -Ji, B[2gxi][2gy+1] = Ji,
+  // B[2gxi+1][2gy+1] = Jr; fragment (st, nt, lane) of a quarter holds rows
+  // 32 st + 8 (lane/16) + t of its column 16 nt + lane % 16.
+  float4_t pacc[2][8];
+#pragma unroll
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt) pacc[m][nt] = float4_t{0.f, 0.f, 0.f, 0.f};
+  const float4* ef = Efw + (size_t)(x0 / 16 + 2 * wave) * 1024;
+#pragma unroll
+  for (int kh = 0; kh < 2; ++kh)
+#pragma unroll
+    for (int jh = 0; jh < 2; ++jh) {
+      {
+        const int j = 64 * jh + 16 * wave + r16, gy = j >> 1, d = j & 1;
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+          float v[8];
+#pragma unroll
+          for (int t = 0; t < 8; ++t) {
+            const int k = 128 * kh + 32 * st + kq + t, gxi = k >> 1, c = k & 1;
+            const float2 J = gxi < Gx ? spec[gy * Gx + gxi] : make_float2(0.f, 0.f);
+            v[t] = pscale * (d == 0 ? (c == 0 ? J.x : -J.y) : (c == 0 ? J.y : J.x));
+          }
+          half8_t h, l;
+          split8(v, h, l);
+          sBh[(st * 4 + wave) * 64 + lane] = h;
+          sBl[(st * 4 + wave) * 64 + lane] = l;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int st = 0; st < 4; ++st) {
+        half8_t ah[2], al[2];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+          ah[m] = __builtin_bit_cast(half8_t, ef[((m * 8 + 4 * kh + st) * 2 + 0) * 64 + lane]);
+          al[m] = __builtin_bit_cast(half8_t, ef[((m * 8 + 4 * kh + st) * 2 + 1) * 64 + lane]);
+        }
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+          const half8_t bh = sBh[(st * 4 + nt) * 64 + lane], bl = sBl[(st * 4 + nt) * 64 + lane];
+#pragma unroll
+          for (int m = 0; m < 2; ++m)
+            pacc[m][4 * jh + nt] = mfma3(ah[m], al[m], bh, bl, pacc[m][4 * jh + nt]);
+        }
+      }
+      __syncthreads();  // the quarter's reads end before the next one overwrites it
+    }
+  // --- the wave's P operand (its 32 columns, K = j), through LDS one half of
+  // j at a time (each wave reads back only the columns it wrote)
+  half8_t pbh[2][4], pbl[2][4];
+#pragma unroll
+  for (int jh = 0; jh < 2; ++jh) {
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int c = 32 * wave + 16 * m + q4 + r, j = 16 * nt + r16;
+          const float v = pacc[m][4 * jh + nt][r];
+          const _Float16 h = (_Float16)v;
+          sPh[c * kWPS + j] = h;
+          sPl[c * kWPS + j] = (_Float16)(v - (float)h);
+        }
+    __syncthreads();
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) {
+        const int c = 32 * wave + cb * 16 + r16;
+        pbh[cb][2 * jh + ks] = *reinterpret_cast<const half8_t*>(sPh + c * kWPS + ks * 32 + kq);
+        pbl[cb][2 * jh + ks] = *reinterpret_cast<const half8_t*>(sPl + c * kWPS + ks * 32 + kq);
+      }
+    __syncthreads();  // (the second half and then the staging reuse this LDS)
+  }
+  // --- rows -----------------------------------------------------------------
+  float4* sTY = reinterpret_cast<float4*>(smem);                 // [2][TY_F4]
+  float* sO = reinterpret_cast<float*>(sTY + 2 * TY_F4);          // [2][16][kWOS]
+  const bool ivec = (Lx & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 15) == 0;
+  auto fetch = [&](int yb, float4& o0, float4& o1) {
+    const float4* src = TYw + (size_t)(yb >> 4) * TY_F4;
+    o0 = src[tid];
+    o1 = src[256 + tid];
+  };
+  // one step: tile yb from staging slot sl; o0, o1: the next tile's operands,
+  // stored to the other slot and refilled with the operands two steps on
+  auto step = [&](int yb, int sl, float4& o0, float4& o1) {
+    if (yb + 16 < y1) {
+      sTY[(sl ^ 1) * TY_F4 + tid] = o0;
+      sTY[(sl ^ 1) * TY_F4 + 256 + tid] = o1;
+      if (yb + 32 < y1) fetch(yb + 32, o0, o1);
+    }
+    const float4* ty = sTY + sl * TY_F4 + lane;
+    float4_t d[2];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) d[cb] = float4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const half8_t ah = __builtin_bit_cast(half8_t, ty[(ks * 2 + 0) * 64]);
+      const half8_t al = __builtin_bit_cast(half8_t, ty[(ks * 2 + 1) * 64]);
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) d[cb] = mfma3(ah, al, pbh[cb][ks], pbl[cb][ks], d[cb]);
+    }
+    // lane l holds rows 4 (l / 16) + r of column 16 cb + l % 16 of the wave's tile
+    float* so = sO + sl * 16 * kWOS;
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        so[(q4 + r) * kWOS + 32 * wave + 16 * cb + r16] = d[cb][r] * inv_pscale;
+    __syncthreads();  // the tile is whole; slot sl of sTY is read by every wave
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int row = 8 * h + (tid >> 5), col = (tid & 31) * 4;
+      const float4 v = *reinterpret_cast<const float4*>(so + row * kWOS + col);
+      const int y = yb + row, x = x0 + col;
+      if (y >= Ly) continue;
+      float* dst = img + (size_t)y * Lx + x;
+      if (ivec) {
+        if (x < Lx)
+          __builtin_nontemporal_store(float4_t{v.x, v.y, v.z, v.w},
+                                      reinterpret_cast<float4_t*>(dst));
+      } else {
+        if (x < Lx) dst[0] = v.x;
+        if (x + 1 < Lx) dst[1] = v.y;
+        if (x + 2 < Lx) dst[2] = v.z;
+        if (x + 3 < Lx) dst[3] = v.w;
+      }
+    }
+  };
+  float4 o0, o1;
+  if (y0 < y1) {
+    fetch(y0, o0, o1);
+    sTY[tid] = o0;
+    sTY[256 + tid] = o1;
+    if (y0 + 16 < y1) fetch(y0 + 16, o0, o1);
+  }
+  __syncthreads();
+  // (y1 - y0 is a multiple of 32: the steps come in pairs)
+  for (int yb = y0; yb < y1; yb += 32) {
+    step(yb, 0, o0, o1);
+    step(yb + 16, 1, o0, o1);
+  }
+}
+
+__global__ __launch_bounds__(256, 3) void k_synth_wide(const float2* __restrict__ spec, int Nx,
+                                                    int Lx, int Ly, int Lypad, int rs,
+                                                    const float4* __restrict__ Efw,
+                                                    const float4* __restrict__ TYw,
+                                                    float* __restrict__ image) {
+  const int b = blockIdx.y, strip = blockIdx.x / rs, rh = blockIdx.x - strip * rs;
+  const int per = (Lypad / 32 + rs - 1) / rs * 32;
+  const int y0 = min(Lypad, rh * per), y1 = min(Lypad, y0 + per);
+  synth_wide_strip(strip * kColsSynth, y0, y1, spec + (size_t)b * 64 * (4 * Nx + 1), Nx, Lx,
+                   Ly, Efw, TYw, image + (size_t)b * Lx * Ly);
+}
+
 // Mixed sizes and kernel sets: one block per SynthJob (one strip of one
 // image).  The block that completes an image's last strip records the
 // device wall clock in done_at[image].
@@ -1918,6 +2137,7 @@ static int get_fused_tables(klt_ctx* ctx, SocsPlan* p, FusedTables** out) {
   t->Kpad = round_up(Lx, 32 * kBandChunk);
   t->Lypad = round_up(Ly, 32);
   const int cix = Lx / 2, coy = (Ly + 1) / 2;
+  const bool wide = Nx > 15 || Ny > 15;
   // row twiddles of the mask band: n = 2 nx (cos), 2 nx + 1 (-sin), nx <= Nx,
   // centre Lx/2, zero past Lx
   auto row_twiddle = [Lx](int n, int x, int nmax, int centre) {
@@ -1929,9 +2149,79 @@ static int get_fused_tables(klt_ctx* ctx, SocsPlan* p, FusedTables** out) {
     return (n & 1) ? -std::sin(a) : std::cos(a);
   };
   {
-    const auto f = band_fragments(2, t->Kpad, [&](int n, int x) { return row_twiddle(n, x, Nx, cix); });
+    const auto f = band_fragments(wide ? 4 : 2, t->Kpad,
+                                  [&](int n, int x) { return row_twiddle(n, x, Nx, cix); });
     KLT_TRY(t->Wf.ensure(f.size() * sizeof(_Float16)));
     KLT_HIP(hipMemcpy(t->Wf.ptr, f.data(), f.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+  }
+  // synthesis column twiddles E[x][k] (k = 2 gxi + c: cos / sin of
+  // 2 pi gx (x - (Lx+1)/2)/Lx, gx = gxi - 2Nx; zero past Lx and 4Nx+1) as hi /
+  // lo fp16 in A-fragment order over nst steps of 32: tile xt of 16 columns,
+  // step s, half h, lane l: x = 16 xt + l % 16, k = 32 s + 8 (l/16) + t
+  auto synth_cols = [&](int nst) {
+    const int Lxp = round_up(Lx, kColsSynth), Gx = 4 * Nx + 1, cox = (Lx + 1) / 2;
+    std::vector<_Float16> ef((size_t)Lxp / 16 * nst * 128 * 8);
+    for (int xt = 0; xt < Lxp / 16; ++xt)
+      for (int st = 0; st < nst; ++st)
+        for (int ln = 0; ln < 64; ++ln)
+          for (int tt = 0; tt < 8; ++tt) {
+            const int x = 16 * xt + ln % 16, k = 32 * st + 8 * (ln / 16) + tt;
+            const int gxi = k / 2, c = k % 2;
+            double v = 0.0;
+            if (x < Lx && gxi < Gx) {
+              long long r = ((long long)(gxi - 2 * Nx) * (x - cox)) % Lx;
+              if (r < 0) r += Lx;
+              const double ang = 2.0 * M_PI * (double)r / (double)Lx;
+              v = c ? std::sin(ang) : std::cos(ang);
+            }
+            _Float16 vh, vl;
+            split_half(v, &vh, &vl);
+            const size_t o = ((((size_t)xt * nst + st) * 2) * 64 + ln) * 8 + tt;
+            ef[o] = vh;
+            ef[o + 64 * 8] = vl;
+          }
+    return ef;
+  };
+  std::vector<float2> tw(kM);
+  for (int j = 0; j < kM; ++j) {
+    const double a = 2.0 * M_PI * j / kM;
+    tw[j] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  KLT_TRY(t->tw64.ensure(kM * sizeof(float2)));
+  KLT_HIP(hipMemcpy(t->tw64.ptr, tw.data(), kM * sizeof(float2), hipMemcpyHostToDevice));
+  if (wide) {
+    // k_synth_wide: TY[y][j] (j = 2 gy: w cos, 2 gy + 1: -w sin of
+    // 2 pi gy (y - (Ly+1)/2)/Ly, w = 1 for gy = 0 else 2; gy <= 2Ny, y < Ly) in
+    // A-fragment order over 16-row tiles: tile yt, step ks, half h, lane l:
+    // y = 16 yt + l % 16, j = 32 ks + 8 (l/16) + t
+    std::vector<_Float16> ty((size_t)t->Lypad / 16 * 4 * 128 * 8, (_Float16)0.f);
+    for (int yt = 0; yt < t->Lypad / 16; ++yt)
+      for (int ks = 0; ks < 4; ++ks)
+        for (int ln = 0; ln < 64; ++ln)
+          for (int tt = 0; tt < 8; ++tt) {
+            const int y = 16 * yt + ln % 16, j = 32 * ks + 8 * (ln / 16) + tt, gy = j / 2;
+            double v = 0.0;
+            if (y < Ly && gy <= 2 * Ny) {
+              long long r = ((long long)gy * (y - coy)) % Ly;
+              if (r < 0) r += Ly;
+              const double a = 2.0 * M_PI * (double)r / (double)Ly;
+              const double w = gy == 0 ? 1.0 : 2.0;
+              v = (j & 1) ? -w * std::sin(a) : w * std::cos(a);
+            }
+            _Float16 vh, vl;
+            split_half(v, &vh, &vl);
+            const size_t o = ((((size_t)yt * 4 + ks) * 2) * 64 + ln) * 8 + tt;
+            ty[o] = vh;
+            ty[o + 64 * 8] = vl;
+          }
+    KLT_TRY(t->TYw.ensure(ty.size() * sizeof(_Float16)));
+    KLT_HIP(hipMemcpy(t->TYw.ptr, ty.data(), ty.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+    const auto ef = synth_cols(8);
+    KLT_TRY(t->Efw.ensure(ef.size() * sizeof(_Float16)));
+    KLT_HIP(hipMemcpy(t->Efw.ptr, ef.data(), ef.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+    *out = t.get();
+    ctx->fused_tables[key] = std::move(t);
+    return KLT_OK;
   }
   std::vector<_Float16> h((size_t)t->Lypad * 64, (_Float16)0.f), l((size_t)t->Lypad * 64, (_Float16)0.f);
   for (int y = 0; y < Ly; ++y)
@@ -1948,32 +2238,8 @@ static int get_fused_tables(klt_ctx* ctx, SocsPlan* p, FusedTables** out) {
   KLT_TRY(t->TYl.ensure(l.size() * sizeof(_Float16)));
   KLT_HIP(hipMemcpy(t->TYh.ptr, h.data(), h.size() * sizeof(_Float16), hipMemcpyHostToDevice));
   KLT_HIP(hipMemcpy(t->TYl.ptr, l.data(), l.size() * sizeof(_Float16), hipMemcpyHostToDevice));
-  // synthesis column twiddles E[x][k] (k = 2 gxi + c: cos / sin of
-  // 2 pi gx (x - (Lx+1)/2)/Lx, gx = gxi - 2Nx; zero past Lx and 4Nx+1) as hi /
-  // lo fp16 in the A-fragment order of synth_strip: tile xt of 16 columns,
-  // step s, half h, lane l: x = 16 xt + l % 16, k = 32 s + 8 (l/16) + t
   {
-    const int Lxp = round_up(Lx, kColsSynth), Gx = 4 * Nx + 1, cox = (Lx + 1) / 2;
-    std::vector<_Float16> ef((size_t)Lxp / 16 * 512 * 8);
-    for (int xt = 0; xt < Lxp / 16; ++xt)
-      for (int st = 0; st < 4; ++st)
-        for (int ln = 0; ln < 64; ++ln)
-          for (int tt = 0; tt < 8; ++tt) {
-            const int x = 16 * xt + ln % 16, k = 32 * st + 8 * (ln / 16) + tt;
-            const int gxi = k / 2, c = k % 2;
-            double v = 0.0;
-            if (x < Lx && gxi < Gx) {
-              long long r = ((long long)(gxi - 2 * Nx) * (x - cox)) % Lx;
-              if (r < 0) r += Lx;
-              const double ang = 2.0 * M_PI * (double)r / (double)Lx;
-              v = c ? std::sin(ang) : std::cos(ang);
-            }
-            _Float16 vh, vl;
-            split_half(v, &vh, &vl);
-            const size_t o = ((((size_t)xt * 4 + st) * 2) * 64 + ln) * 8 + tt;
-            ef[o] = vh;
-            ef[o + 64 * 8] = vl;
-          }
+    const auto ef = synth_cols(4);
     KLT_TRY(t->Ef.ensure(ef.size() * sizeof(_Float16)));
     KLT_HIP(hipMemcpy(t->Ef.ptr, ef.data(), ef.size() * sizeof(_Float16), hipMemcpyHostToDevice));
   }
@@ -2049,13 +2315,6 @@ static int get_fused_tables(klt_ctx* ctx, SocsPlan* p, FusedTables** out) {
     KLT_TRY(t->Ebf.ensure(eb.size() * sizeof(_Float16)));
     KLT_HIP(hipMemcpy(t->Ebf.ptr, eb.data(), eb.size() * sizeof(_Float16), hipMemcpyHostToDevice));
   }
-  std::vector<float2> tw(kM);
-  for (int j = 0; j < kM; ++j) {
-    const double a = 2.0 * M_PI * j / kM;
-    tw[j] = make_float2((float)std::cos(a), (float)std::sin(a));
-  }
-  KLT_TRY(t->tw64.ensure(kM * sizeof(float2)));
-  KLT_HIP(hipMemcpy(t->tw64.ptr, tw.data(), kM * sizeof(float2), hipMemcpyHostToDevice));
   *out = t.get();
   ctx->fused_tables[key] = std::move(t);
   return KLT_OK;
@@ -2081,7 +2340,7 @@ static bool split_ok(int units, int parts) {
 static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, int Ny, int nk,
                       int nchunk, int nblk, int nstrips, int nrow32,
                       const std::function<int(const klt_ctx::Tuned&)>& run,
-                      klt_ctx::Tuned* out) {
+                      klt_ctx::Tuned* out, int kg_cap = 1 << 30) {
   const std::array<int, 7> key{kind, nb, Lx, Ly, Nx, Ny, nk};
   auto it = ctx->fused_tune.find(key);
   if (it != ctx->fused_tune.end()) {
@@ -2090,7 +2349,11 @@ static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, in
   }
   klt_ctx::Tuned h{1, 1, 1, kRowsBand, 0.f};
   fused_fixed_splits(nb, Lx, Ly, nk, &h.ks, &h.rs, &h.kg);
-  auto kg_ok = [&](int v) { return v <= nk && (v == 1 || v * nb <= kGroupSlots); };
+  // (kg_cap 1: a pass whose coarse stage has no kernel groups)
+  h.kg = std::min(h.kg, kg_cap);
+  auto kg_ok = [&](int v) {
+    return v <= nk && v <= kg_cap && (v == 1 || v * nb <= kGroupSlots);
+  };
   if (ctx->pin_ks || ctx->pin_rs || ctx->pin_kg || ctx->pin_rb) {
     auto fit = [](int v, int hi, const std::function<bool(int)>& ok) {
       v = std::min(v, hi);
@@ -2275,6 +2538,74 @@ int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, 
     klt_ctx::Tuned c{};
     KLT_TRY(pick_split(ctx, 0, nb, Lx, Ly, Nx, Ny, kb->nk, ft->Kpad / (kBandChunk * 32), nblk,
                        nstrips, ft->Lypad / 32, run, &c));
+    KLT_TRY(run(c));
+  }
+  return KLT_OK;
+}
+
+bool socs_wide_supported(const klt_kernels* kb) {
+  return !socs_fused_supported(kb) && kb->Nx <= 31 && kb->Ny <= 31;
+}
+
+// The mask band of nb images for kernel half-sizes 16..31: 64 twiddle rows
+// (nx <= Nx <= 31), partials [nb][ceil(Ly/rb) ks][2Ny+1][32].
+static void launch_mask_band_wide(klt_ctx* ctx, int rb, int ks, int nb, const float* masks,
+                                  int Lx, int Ly, int Kpad, const float4* Wf, const float2* twy,
+                                  int ldtw, int Nx, int Ny, float2* out) {
+  const int nblk = (Ly + rb - 1) / rb;
+  if (rb == 64)
+    launch_band<4, false, 1>(ctx, "k_band_mfma", nblk, ks, nb, masks, nullptr, 0.f, 0.f, Lx, Ly,
+                             Kpad, Wf, twy, ldtw, Nx, Ny, 32, out, nullptr);
+  else
+    launch_band<4, false, 2>(ctx, "k_band_mfma", nblk, ks, nb, masks, nullptr, 0.f, 0.f, Lx, Ly,
+                             Kpad, Wf, twy, ldtw, Nx, Ny, 32, out, nullptr);
+}
+
+// SOCS forward for kernel half-sizes 16..31 (replaces klitho_socs.cpp:196-241
+// for those sizes): the band pass on the matrix cores, M^ assembled per
+// image, the coarse stage on the 128-point grid (socs_coarse_spec), and the
+// matrix-core synthesis at K = 128 (k_synth_wide).  Band tile height, column
+// ranges and synthesis row ranges come from the tuner (no kernel groups).
+int socs_forward_wide(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int B, int Lx,
+                      int Ly, float* images) {
+  if (!socs_wide_supported(kb))
+    return set_error(KLT_ERR_ARG, "socs_forward_wide: kernel half-size outside 16..31");
+  SocsPlan* p = nullptr;
+  KLT_TRY(get_socs_plan(ctx, Lx, Ly, kb->Nx, kb->Ny, &p));
+  FusedTables* ft = nullptr;
+  KLT_TRY(get_fused_tables(ctx, p, &ft));
+  const int Nx = kb->Nx, Ny = kb->Ny, Wy = 2 * Ny + 1, Gx = 4 * Nx + 1;
+  const int T = (2 * Nx + 1) * Wy;
+  const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
+  const int nblk64 = (Ly + 63) / 64;
+  const int nstrips = (Lx + kColsSynth - 1) / kColsSynth;
+  const float inv_area = 1.0f / ((float)Lx * (float)Ly);
+  const int SB = std::min(B, 512);
+  KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk64 * kMaxKs * Wy * 32 * sizeof(float2)));
+  KLT_TRY(ctx->band.ensure((size_t)SB * T * sizeof(float2)));
+  KLT_TRY(ctx->spec.ensure((size_t)SB * 64 * Gx * sizeof(float2)));
+  for (int b0 = 0; b0 < B; b0 += SB) {
+    const int nb = std::min(SB, B - b0);
+    const float* m = masks + (size_t)b0 * Lx * Ly;
+    float* out = images + (size_t)b0 * Lx * Ly;
+    auto run = [&](const klt_ctx::Tuned& c) -> int {
+      const int parts = (Ly + c.rb - 1) / c.rb * c.ks;
+      launch_mask_band_wide(ctx, c.rb, c.ks, nb, m, Lx, Ly, ft->Kpad, ft->Wf.as<float4>(),
+                            p->twy_in.as<float2>(), p->NP, Nx, Ny, ctx->qpart.as<float2>());
+      KLT_LAUNCH(ctx, "k_band_assemble", k_band_assemble,
+                 dim3((unsigned)(T + 255) / 256, nb), dim3(256), 0, ctx->qpart.as<float2>(),
+                 parts, Nx, Ny, 32, inv_area, ctx->band.as<float2>());
+      KLT_TRY(socs_coarse_spec(ctx, kb, p, ctx->band.as<float2>(), nb, ctx->spec.as<float2>(),
+                               64));
+      KLT_LAUNCH(ctx, "k_synth_wide", k_synth_wide, dim3(nstrips * c.rs, nb), dim3(256), 0,
+                 ctx->spec.as<float2>(), Nx, Lx, Ly, ft->Lypad, c.rs, ft->Efw.as<float4>(),
+                 ft->TYw.as<float4>(), out);
+      KLT_HIP(hipGetLastError());
+      return KLT_OK;
+    };
+    klt_ctx::Tuned c{};
+    KLT_TRY(pick_split(ctx, 0, nb, Lx, Ly, Nx, Ny, kb->nk, ft->Kpad / (kBandChunk * 32), nblk,
+                       nstrips, ft->Lypad / 32, run, &c, 1));
     KLT_TRY(run(c));
   }
   return KLT_OK;
@@ -2606,7 +2937,7 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
       // each image's M^ once, for the forward and the adjoint coarse stages
       KLT_LAUNCH(ctx, "k_band_assemble", k_band_assemble,
                  dim3((unsigned)((2 * Nx + 1) * Wy + 255) / 256, nb), dim3(256), 0,
-                 ctx->qpart.as<float2>(), parts, Nx, Ny, inv_area, ctx->mhat.as<float2>());
+                 ctx->qpart.as<float2>(), parts, Nx, Ny, 16, inv_area, ctx->mhat.as<float2>());
       KLT_LAUNCH(ctx, "k_socs_mid", k_socs_mid_full, dim3(c.kg, nb), dim3(256), 0,
                  ctx->mhat.as<float2>(), Nx, Ny, kb->kern.as<float2>(), kb->scales.as<float>(),
                  kb->nk, ft->tw64.as<float2>(), ctx->spec.as<float2>(), ctx->ipart.as<float>());

@@ -58,7 +58,8 @@ def test_small_random_kernels(ctx, Lx, Ly, Nx, Ny, plain):
 
 
 # kernel half-sizes above 15 (the reference's own 33 x 33 sample has N = 16):
-# odd and unequal periods, Ny != Nx
+# odd and unequal periods, Ny != Nx.  "default" is the matrix-core path of
+# klt_socs_forward (k_synth_wide), "plain" the FP32 single-image path.
 # N 16-18: the single-block coarse stage; N 19-31: the four-launch one over
 # device scratch (4N + 1 <= L at the limit: 125 <= 127)
 LARGE_N = [(67, 70, 16, 16), (200, 75, 16, 18), (129, 100, 18, 17), (160, 160, 18, 18),
