@@ -68,7 +68,7 @@ struct FusedTables {
   // kernel half-sizes 16..31 (k_synth_wide): Wf then holds 64 twiddle rows;
   // TYw = TY (Lypad x 128) as A fragments over 16-row tiles, Efw the
   // synthesis column twiddles as A fragments over 8 steps of 32
-  DevBuf TYw, Efw;
+  DevBuf TYw, Efw, tw128;  // (tw128: e^{+2 pi i j/128}, j < 128)
 };
 
 // Device pointers of one fused forward launch (socs_fused_launch): the tables
@@ -259,8 +259,8 @@ int socs_mid_from_band(klt_ctx* ctx, const klt_kernels* kb, const float2* band, 
 int socs_coarse_spec(klt_ctx* ctx, const klt_kernels* kb, const SocsPlan* p,
                      const float2* band, int B, float2* spec, int G);
 // Batched SOCS forward for kernel half-sizes 16..31 (socs_fused.hip): the
-// band pass on the matrix cores (64 twiddle rows), the coarse stage of
-// socs_coarse_spec, and the matrix-core synthesis at K = 128.
+// band pass on the matrix cores (64 twiddle rows), the coarse stage as
+// 128-point FFTs, and the matrix-core synthesis at K = 128.
 bool socs_wide_supported(const klt_kernels* kb);
 int socs_forward_wide(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int B, int Lx,
                       int Ly, float* images);

@@ -1442,6 +1442,268 @@ __global__ __launch_bounds__(256, 4) void k_synth_mfma(const float2* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Stage 2 for kernel half-sizes 16..31: the coarse grid has 128 points a side
+// (4N + 1 <= 125, so the |g| <= 2N intensity band does not alias).
+// ---------------------------------------------------------------------------
+// 16-point DFT in registers, natural order in and out: two 8-point DFTs of
+// the even and odd samples, combined with e^{S 2 pi i k/16}.
+template <int S>
+__device__ __forceinline__ void dft16(float2* x) {
+  constexpr float C1 = 0.92387953251128675613f, S1 = 0.38268343236508977173f;
+  constexpr float R = 0.70710678118654752440f;
+  float2 e[8], o[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    e[i] = x[2 * i];
+    o[i] = x[2 * i + 1];
+  }
+  dft8<S>(e);
+  dft8<S>(o);
+  // w_k = cos(pi k/8) + S i sin(pi k/8)
+  const float2 w[8] = {make_float2(1.f, 0.f),  make_float2(C1, S * S1),  make_float2(R, S * R),
+                       make_float2(S1, S * C1), make_float2(0.f, 1.f * S), make_float2(-S1, S * C1),
+                       make_float2(-R, S * R),  make_float2(-C1, S * S1)};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    float2 p;
+    if (k == 0)
+      p = o[0];
+    else if (k == 4)
+      p = make_float2(-S * o[4].y, S * o[4].x);
+    else
+      p = cmulf(o[k], w[k]);
+    x[k] = make_float2(e[k].x + p.x, e[k].y + p.y);
+    x[k + 8] = make_float2(e[k].x - p.x, e[k].y - p.y);
+  }
+}
+
+constexpr int kRS = 136;  // LDS row stride (float2) of the 128-point stages
+
+// One 128-point transform per group of 8 threads (128 = 8 x 16).  On entry
+// thread t holds x[t + 8 n2] in v[n2] (n2 < 16); on exit it holds
+// X[t + 8 h + 16 k1] in v[8 h + k1] (h < 2, k1 < 8):
+//   Y_t[k2] = sum_n2 x[t + 8 n2] e^{S 2 pi i n2 k2/16}      (in registers)
+//   X[k2 + 16 k1] = sum_t Y_t[k2] e^{S 2 pi i t k2/128} e^{S 2 pi i t k1/8}
+// with the exchange through the group's LDS xs (8 rows of stride 17, 136
+// float2).  tw[k2] = e^{+2 pi i t k2/128}, conjugated here for S = -1.  As
+// fft64_group, every thread of the wave calls it.
+template <int S>
+__device__ __forceinline__ void fft128_group(float2* v, const float2* tw, float2* xs, int t,
+                                             bool active) {
+  if (active) {
+    dft16<S>(v);
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) {
+      const float2 w = S > 0 ? tw[k2] : make_float2(tw[k2].x, -tw[k2].y);
+      xs[t * 17 + k2] = cmulf(v[k2], w);
+    }
+  }
+  wave_sync();
+  if (active) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int n1 = 0; n1 < 8; ++n1) v[8 * h + n1] = xs[n1 * 17 + t + 8 * h];
+    dft8<S>(v);
+    dft8<S>(v + 8);
+  }
+  wave_sync();
+}
+
+// Forward transform of a 128 x 128 coarse intensity held in registers (thread
+// (g, t) holds column ux = g + 32 q at rows uy = t + 8 h + 16 k1 in
+// acc[q][8 h + k1]) to its |g| <= 2N spectrum: spec [64][4Nx+1], rows gy in
+// [0, 2Ny] (rows above zero).  sB: 64 rows of kRS float2; sX: 32 groups' FFT
+// exchange, kRS float2 each.
+__device__ __forceinline__ void coarse128_to_spec(const float (&acc)[4][16], float2* sB,
+                                                  float2* sX, const float2 (&twi)[16], int Nx,
+                                                  int Ny, float2* __restrict__ spec) {
+  const int tid = threadIdx.x, g = tid >> 3, t = tid & 7;
+  const int Gx = 4 * Nx + 1, G2 = 2 * Ny + 1;
+  float2* xs = sX + g * kRS;
+  __syncthreads();  // every wave is done with the caller's use of sB and sX
+  // columns: C[gy][ux] = sum_uy I[uy][ux] e^{-2 pi i gy uy/128}, gy <= 2Ny
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int ux = g + 32 * q;
+    float2 v[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) v[n2] = make_float2(acc[q][8 * (n2 & 1) + (n2 >> 1)], 0.f);
+    fft128_group<-1>(v, twi, xs, t, true);
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) {
+        const int gy = t + 8 * h + 16 * k1;
+        if (gy < G2) sB[gy * kRS + ux] = v[8 * h + k1];
+      }
+  }
+  __syncthreads();
+  // rows: J[gy][gx] = 1/128^2 sum_ux C[gy][ux] e^{-2 pi i gx ux/128}
+  const float inv = 1.0f / (128.f * 128.f);
+#pragma unroll
+  for (int rr = 0; rr < 2; ++rr) {
+    const int gy = g + 32 * rr;
+    const bool act = gy < G2;
+    float2 v[16];
+    if (act) {
+#pragma unroll
+      for (int n2 = 0; n2 < 16; ++n2) v[n2] = sB[gy * kRS + t + 8 * n2];
+    }
+    fft128_group<-1>(v, twi, xs, t, act);
+    if (act) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int k1 = 0; k1 < 8; ++k1) {
+          const int kk = t + 8 * h + 16 * k1;
+          const int gx = kk < 64 ? kk : kk - 128;
+          if (gx >= -2 * Nx && gx <= 2 * Nx)
+            spec[gy * Gx + gx + 2 * Nx] =
+                make_float2(v[8 * h + k1].x * inv, v[8 * h + k1].y * inv);
+        }
+    }
+  }
+  for (int i = G2 * Gx + tid; i < 64 * Gx; i += 256) spec[i] = make_float2(0.f, 0.f);
+}
+
+// grid (kg, B), 256 threads = 32 groups of 8, dynamic LDS
+// (96 kRS + T) float2.  Block (j, b): the j-th of kg groups of the kernels
+// over image b's normalised band Mfull [B][2Ny+1][2Nx+1] (k_band_assemble):
+//   B_k[r][ux] = sum_nx K_k[r][nx] M^[r][nx] e^{+2 pi i nx ux/128}
+//     (rows r < 2Ny+1; the exchange of each row's FFT in place in its row),
+//   E_k[uy][ux] = sum_r B_k[r][ux] e^{+2 pi i (r - Ny) uy/128},
+//   I[uy][ux] += lambda_k |E_k[uy][ux]|^2   (registers, 64 a thread).
+// With ipart, the block stores its partial I there ([B][kg][256][64],
+// thread-major) for k_socs_mid128_fin; else spec [B][64][4Nx+1] receives the
+// intensity spectrum.  With |n| <= 31, inputs t + 8 n2 of the transforms are
+// zero for n2 = 4..11: written as constants so that the first stage drops
+// them.
+__global__ __launch_bounds__(256) void k_socs_mid128(const float2* __restrict__ Mfull, int Nx,
+                                                     int Ny, const float2* __restrict__ kern,
+                                                     const float* __restrict__ scales, int nk,
+                                                     const float2* __restrict__ tw128,
+                                                     float2* __restrict__ spec,
+                                                     float* __restrict__ ipart) {
+  extern __shared__ float4 smem4[];
+  float2* sB = reinterpret_cast<float2*>(smem4);  // [64][kRS]
+  float2* sX = sB + 64 * kRS;                     // [32][kRS]
+  float2* sM = sX + 32 * kRS;                     // [T]
+  const int b = blockIdx.y, kgs = gridDim.x, j = blockIdx.x;
+  const int tid = threadIdx.x, g = tid >> 3, t = tid & 7;
+  const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
+  const int per = (nk + kgs - 1) / kgs;
+  const int k0 = min(nk, j * per), k1 = min(nk, k0 + per);
+  for (int i = tid; i < T; i += 256) sM[i] = Mfull[(size_t)b * T + i];
+  float2 twi[16];
+#pragma unroll
+  for (int k2 = 0; k2 < 16; ++k2) twi[k2] = tw128[(t * k2) & 127];
+  float acc[4][16];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[q][i] = 0.f;
+  __syncthreads();
+  for (int k = k0; k < k1; ++k) {
+    const float2* K = kern + (size_t)k * T;
+    const float lam = scales[k];
+#pragma unroll
+    for (int rr = 0; rr < 2; ++rr) {
+      const int r = g + 32 * rr;
+      const bool act = r < Wy;
+      float2 v[16];
+#pragma unroll
+      for (int n2 = 0; n2 < 16; ++n2) v[n2] = make_float2(0.f, 0.f);
+      if (act) {
+#pragma unroll
+        for (int n2 = 0; n2 < 16; ++n2) {
+          if (n2 >= 4 && n2 < 12) continue;
+          const int n = t + 8 * n2;
+          const int nx = n < 64 ? n : n - 128;
+          if (nx >= -Nx && nx <= Nx) {
+            const int i = r * Wx + nx + Nx;
+            v[n2] = cmulf(K[i], sM[i]);
+          }
+        }
+      }
+      float2* xr = sB + (act ? r : 0) * kRS;
+      fft128_group<1>(v, twi, xr, t, act);
+      if (act) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int q1 = 0; q1 < 8; ++q1) xr[t + 8 * h + 16 * q1] = v[8 * h + q1];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int ux = g + 32 * q;
+      float2 v[16];
+#pragma unroll
+      for (int n2 = 0; n2 < 16; ++n2) {
+        v[n2] = make_float2(0.f, 0.f);
+        if (n2 >= 4 && n2 < 12) continue;
+        const int n = t + 8 * n2;
+        const int ny = n < 64 ? n : n - 128;
+        if (ny >= -Ny && ny <= Ny) v[n2] = sB[(ny + Ny) * kRS + ux];
+      }
+      fft128_group<1>(v, twi, sX + g * kRS, t, true);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[q][i] += lam * (v[i].x * v[i].x + v[i].y * v[i].y);
+    }
+    __syncthreads();  // the column reads end before the next kernel's rows
+  }
+  if (ipart) {
+    float4* dst = reinterpret_cast<float4*>(ipart + ((size_t)b * kgs + j) * 16384 + tid * 64);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        dst[q * 4 + i] = make_float4(acc[q][4 * i], acc[q][4 * i + 1], acc[q][4 * i + 2],
+                                     acc[q][4 * i + 3]);
+    return;
+  }
+  coarse128_to_spec(acc, sB, sX, twi, Nx, Ny, spec + (size_t)b * 64 * (4 * Nx + 1));
+}
+
+// grid (B), dynamic LDS 96 kRS float2: sums the kg partial intensities of
+// each image (k_socs_mid128 with ipart) and transforms them to the intensity
+// spectrum [B][64][4Nx+1].
+__global__ __launch_bounds__(256) void k_socs_mid128_fin(const float* __restrict__ ipart, int kg,
+                                                         int Nx, int Ny,
+                                                         const float2* __restrict__ tw128,
+                                                         float2* __restrict__ spec) {
+  extern __shared__ float4 smem4[];
+  float2* sB = reinterpret_cast<float2*>(smem4);
+  float2* sX = sB + 64 * kRS;
+  const int b = blockIdx.x, tid = threadIdx.x, t = tid & 7;
+  float acc[4][16];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[q][i] = 0.f;
+  for (int j = 0; j < kg; ++j) {
+    const float4* src =
+        reinterpret_cast<const float4*>(ipart + ((size_t)b * kg + j) * 16384 + tid * 64);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float4 u = src[q * 4 + i];
+        acc[q][4 * i] += u.x;
+        acc[q][4 * i + 1] += u.y;
+        acc[q][4 * i + 2] += u.z;
+        acc[q][4 * i + 3] += u.w;
+      }
+  }
+  float2 twi[16];
+#pragma unroll
+  for (int k2 = 0; k2 < 16; ++k2) twi[k2] = tw128[(t * k2) & 127];
+  coarse128_to_spec(acc, sB, sX, twi, Nx, Ny, spec + (size_t)b * 64 * (4 * Nx + 1));
+}
+
+// ---------------------------------------------------------------------------
 // Stage 3 for kernel half-sizes 16..31.  grid (ceil(Lx/128) * rs, B), 256
 // threads, as k_synth_mfma.  The coarse spectrum has up to 63 rows
 // (gy <= 2Ny) and 125 columns (|gx| <= 2Nx), so both products are twice as
@@ -2219,6 +2481,13 @@ static int get_fused_tables(klt_ctx* ctx, SocsPlan* p, FusedTables** out) {
     const auto ef = synth_cols(8);
     KLT_TRY(t->Efw.ensure(ef.size() * sizeof(_Float16)));
     KLT_HIP(hipMemcpy(t->Efw.ptr, ef.data(), ef.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+    std::vector<float2> t128(128);
+    for (int j = 0; j < 128; ++j) {
+      const double a = 2.0 * M_PI * j / 128;
+      t128[j] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    KLT_TRY(t->tw128.ensure(128 * sizeof(float2)));
+    KLT_HIP(hipMemcpy(t->tw128.ptr, t128.data(), 128 * sizeof(float2), hipMemcpyHostToDevice));
     *out = t.get();
     ctx->fused_tables[key] = std::move(t);
     return KLT_OK;
@@ -2340,7 +2609,7 @@ static bool split_ok(int units, int parts) {
 static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, int Ny, int nk,
                       int nchunk, int nblk, int nstrips, int nrow32,
                       const std::function<int(const klt_ctx::Tuned&)>& run,
-                      klt_ctx::Tuned* out, int kg_cap = 1 << 30) {
+                      klt_ctx::Tuned* out) {
   const std::array<int, 7> key{kind, nb, Lx, Ly, Nx, Ny, nk};
   auto it = ctx->fused_tune.find(key);
   if (it != ctx->fused_tune.end()) {
@@ -2349,11 +2618,7 @@ static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, in
   }
   klt_ctx::Tuned h{1, 1, 1, kRowsBand, 0.f};
   fused_fixed_splits(nb, Lx, Ly, nk, &h.ks, &h.rs, &h.kg);
-  // (kg_cap 1: a pass whose coarse stage has no kernel groups)
-  h.kg = std::min(h.kg, kg_cap);
-  auto kg_ok = [&](int v) {
-    return v <= nk && v <= kg_cap && (v == 1 || v * nb <= kGroupSlots);
-  };
+  auto kg_ok = [&](int v) { return v <= nk && (v == 1 || v * nb <= kGroupSlots); };
   if (ctx->pin_ks || ctx->pin_rs || ctx->pin_kg || ctx->pin_rb) {
     auto fit = [](int v, int hi, const std::function<bool(int)>& ok) {
       v = std::min(v, hi);
@@ -2563,9 +2828,10 @@ static void launch_mask_band_wide(klt_ctx* ctx, int rb, int ks, int nb, const fl
 
 // SOCS forward for kernel half-sizes 16..31 (replaces klitho_socs.cpp:196-241
 // for those sizes): the band pass on the matrix cores, M^ assembled per
-// image, the coarse stage on the 128-point grid (socs_coarse_spec), and the
+// image, the coarse stage on the 128-point grid (k_socs_mid128: 128-point
+// FFTs in registers and LDS, kg groups of kernels per image), and the
 // matrix-core synthesis at K = 128 (k_synth_wide).  Band tile height, column
-// ranges and synthesis row ranges come from the tuner (no kernel groups).
+// ranges, synthesis row ranges and kernel groups come from the tuner.
 int socs_forward_wide(klt_ctx* ctx, const klt_kernels* kb, const float* masks, int B, int Lx,
                       int Ly, float* images) {
   if (!socs_wide_supported(kb))
@@ -2584,6 +2850,15 @@ int socs_forward_wide(klt_ctx* ctx, const klt_kernels* kb, const float* masks, i
   KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk64 * kMaxKs * Wy * 32 * sizeof(float2)));
   KLT_TRY(ctx->band.ensure((size_t)SB * T * sizeof(float2)));
   KLT_TRY(ctx->spec.ensure((size_t)SB * 64 * Gx * sizeof(float2)));
+  KLT_TRY(ctx->ipart.ensure((size_t)kGroupSlots * 128 * 128 * sizeof(float)));
+  // the 128-point coarse stage keeps its row transforms, FFT exchange (and
+  // the band) in LDS
+  const int lds_fin = 96 * kRS * (int)sizeof(float2);
+  const int lds_mid = lds_fin + T * (int)sizeof(float2);
+  KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_socs_mid128),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_mid));
+  KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_socs_mid128_fin),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_fin));
   for (int b0 = 0; b0 < B; b0 += SB) {
     const int nb = std::min(SB, B - b0);
     const float* m = masks + (size_t)b0 * Lx * Ly;
@@ -2595,8 +2870,20 @@ int socs_forward_wide(klt_ctx* ctx, const klt_kernels* kb, const float* masks, i
       KLT_LAUNCH(ctx, "k_band_assemble", k_band_assemble,
                  dim3((unsigned)(T + 255) / 256, nb), dim3(256), 0, ctx->qpart.as<float2>(),
                  parts, Nx, Ny, 32, inv_area, ctx->band.as<float2>());
-      KLT_TRY(socs_coarse_spec(ctx, kb, p, ctx->band.as<float2>(), nb, ctx->spec.as<float2>(),
-                               64));
+      if (c.kg == 1) {
+        KLT_LAUNCH(ctx, "k_socs_mid128", k_socs_mid128, dim3(1, nb), dim3(256), lds_mid,
+                   ctx->band.as<float2>(), Nx, Ny, kb->kern.as<float2>(),
+                   kb->scales.as<float>(), kb->nk, ft->tw128.as<float2>(),
+                   ctx->spec.as<float2>(), nullptr);
+      } else {
+        KLT_LAUNCH(ctx, "k_socs_mid128", k_socs_mid128, dim3(c.kg, nb), dim3(256), lds_mid,
+                   ctx->band.as<float2>(), Nx, Ny, kb->kern.as<float2>(),
+                   kb->scales.as<float>(), kb->nk, ft->tw128.as<float2>(), nullptr,
+                   ctx->ipart.as<float>());
+        KLT_LAUNCH(ctx, "k_socs_mid128_fin", k_socs_mid128_fin, dim3(nb), dim3(256), lds_fin,
+                   ctx->ipart.as<float>(), c.kg, Nx, Ny, ft->tw128.as<float2>(),
+                   ctx->spec.as<float2>());
+      }
       KLT_LAUNCH(ctx, "k_synth_wide", k_synth_wide, dim3(nstrips * c.rs, nb), dim3(256), 0,
                  ctx->spec.as<float2>(), Nx, Lx, Ly, ft->Lypad, c.rs, ft->Efw.as<float4>(),
                  ft->TYw.as<float4>(), out);
@@ -2605,7 +2892,7 @@ int socs_forward_wide(klt_ctx* ctx, const klt_kernels* kb, const float* masks, i
     };
     klt_ctx::Tuned c{};
     KLT_TRY(pick_split(ctx, 0, nb, Lx, Ly, Nx, Ny, kb->nk, ft->Kpad / (kBandChunk * 32), nblk,
-                       nstrips, ft->Lypad / 32, run, &c, 1));
+                       nstrips, ft->Lypad / 32, run, &c));
     KLT_TRY(run(c));
   }
   return KLT_OK;
