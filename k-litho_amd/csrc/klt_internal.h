@@ -69,6 +69,10 @@ struct FusedTables {
   // TYw = TY (Lypad x 128) as A fragments over 16-row tiles, Efw the
   // synthesis column twiddles as A fragments over 8 steps of 32
   DevBuf TYw, Efw, tw128;  // (tw128: e^{+2 pi i j/128}, j < 128)
+  // fused adjoint synthesis of half-sizes 16..31 (k_synth_resist_wide): the
+  // resist band's column twiddles over 16-row tiles (TYgw, 128 j) and the
+  // synthesis column twiddles as B fragments over 32-column steps (Ebw, 256 k)
+  DevBuf TYgw, Ebw;
 };
 
 // Device pointers of one fused forward launch (socs_fused_launch): the tables

@@ -822,9 +822,10 @@ __global__ __launch_bounds__(256) void k_socs_mid_jobs(const klt::MidJob* __rest
 // Gradient spectrum J[n] = Gam[-n] + conj(Gam[n]) (times the odd-size phase)
 // in the synthesis layout [32][4Nx+1] (rows gy in [0, Ny]).
 __device__ __forceinline__ void gam_to_spec(const float2* sGam, int Nx, int Ny, int dcx, int dcy,
-                                            int Lx, int Ly, float2* __restrict__ out) {
+                                            int Lx, int Ly, float2* __restrict__ out,
+                                            int rows = 32) {
   const int Wx = 2 * Nx + 1, Gx = 4 * Nx + 1;
-  for (int i = threadIdx.x; i < 32 * Gx; i += 256) {
+  for (int i = threadIdx.x; i < rows * Gx; i += 256) {
     const int gy = i / Gx, gx = i - gy * Gx - 2 * Nx;
     float2 j = make_float2(0.f, 0.f);
     if (gy <= Ny && gx >= -Nx && gx <= Nx) {
@@ -869,10 +870,10 @@ __global__ __launch_bounds__(256) void k_band_assemble(const float2* __restrict_
 
 // grid (ceil(Gy Gx / 256), B), one thread per entry: the resist band G^
 // [B][4Ny+1][4Nx+1] (centred, Hermitian for gx < 0) from the resist-band
-// partials, and (block 0) the loss of each image.
+// partials [B][nblk][4Ny+1][ldo], and (block 0) the loss of each image.
 __global__ __launch_bounds__(256) void k_resist_sum(const float2* __restrict__ gpart,
                                                     const float* __restrict__ lpart, int nblk,
-                                                    int Nx, int Ny, float inv_area,
+                                                    int Nx, int Ny, int ldo, float inv_area,
                                                     float2* __restrict__ Gfull,
                                                     float* __restrict__ loss) {
   const int b = blockIdx.y;
@@ -889,7 +890,7 @@ __global__ __launch_bounds__(256) void k_resist_sum(const float2* __restrict__ g
   float2 s = make_float2(0.f, 0.f);
 #pragma unroll 8
   for (int k = 0; k < nblk; ++k) {
-    const float2 p = gpart[(((size_t)b * nblk + k) * Gy + row) * 32 + col];
+    const float2 p = gpart[(((size_t)b * nblk + k) * Gy + row) * ldo + col];
     s.x += p.x;
     s.y += p.y;
   }
@@ -1089,11 +1090,12 @@ __global__ __launch_bounds__(256) void k_socs_mid_full(const float2* __restrict_
 }
 
 // Sums the kg partial Gam of each image and writes its gradient spectrum.
+// (rows: the spectrum rows per image, 32 for half-sizes up to 15, 64 above)
 __global__ __launch_bounds__(256) void k_socs_mid_bwd_fin(const float2* __restrict__ gam_part,
                                                           int kg, int Nx, int Ny, int dcx,
                                                           int dcy, int Lx, int Ly,
-                                                          float2* __restrict__ gspec) {
-  __shared__ float2 sGam[31 * 31];
+                                                          float2* __restrict__ gspec, int rows) {
+  __shared__ float2 sGam[63 * 63];
   const int b = blockIdx.x, T = (2 * Nx + 1) * (2 * Ny + 1);
   for (int i = threadIdx.x; i < T; i += 256) {
     float2 v = make_float2(0.f, 0.f);
@@ -1105,7 +1107,7 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd_fin(const float2* __restri
     sGam[i] = v;
   }
   __syncthreads();
-  gam_to_spec(sGam, Nx, Ny, dcx, dcy, Lx, Ly, gspec + (size_t)b * 32 * (4 * Nx + 1));
+  gam_to_spec(sGam, Nx, Ny, dcx, dcy, Lx, Ly, gspec + (size_t)b * rows * (4 * Nx + 1), rows);
 }
 
 // ---------------------------------------------------------------------------
@@ -1728,19 +1730,16 @@ __global__ __launch_bounds__(256) void k_socs_mid128_fin(const float* __restrict
 constexpr int kWPS = 72;   // LDS row stride (halves) of one half of P's transpose
 constexpr int kWOS = 132;  // LDS row stride (floats) of the block's output tile
 
-__device__ __forceinline__ void synth_wide_strip(int x0, int y0, int y1,
-                                                 const float2* __restrict__ spec, int Nx,
-                                                 int Lx, int Ly, const float4* __restrict__ Efw,
-                                                 const float4* __restrict__ TYw,
-                                                 float* __restrict__ img) {
-  constexpr int B_BYTES = 2 * 16 * 64 * 16;           // a quarter of B, hi / lo
-  constexpr int P_BYTES = 2 * kColsSynth * kWPS * 2;  // half of P's transpose, hi / lo
-  constexpr int TY_F4 = 8 * 64;                       // float4 per staged TY tile
-  constexpr int S_BYTES = 2 * TY_F4 * 16 + 2 * 16 * kWOS * 4;
-  constexpr int PRO = B_BYTES > P_BYTES ? B_BYTES : P_BYTES;
-  constexpr int MAIN = PRO > S_BYTES ? PRO : S_BYTES;
-  __shared__ __attribute__((aligned(16))) char smem[MAIN];
-  __shared__ float red[4];
+// Prologue of the wide synthesis passes: the wave's P operand (its 32
+// columns x0 + 32 wave .., K = j) in pbh / pbl, through smem (at least
+// kWideProBytes); returns 1 / pscale.
+constexpr int kWideProBytes = 2 * kColsSynth * kWPS * 2;  // >= a quarter of B (32 KB)
+
+__device__ __forceinline__ float wide_p_operand(int x0, const float2* __restrict__ spec, int Nx,
+                                                const float4* __restrict__ Efw, char* smem,
+                                                float* red, half8_t (&pbh)[2][4],
+                                                half8_t (&pbl)[2][4]) {
+  static_assert(kWideProBytes >= 2 * 16 * 64 * 16, "a quarter of B fits");
   half8_t* sBh = reinterpret_cast<half8_t*>(smem);  // [4 steps][4 column tiles][64 lanes]
   half8_t* sBl = sBh + 16 * 64;
   _Float16* sPh = reinterpret_cast<_Float16*>(smem);  // [128 columns][kWPS]
@@ -1761,7 +1760,7 @@ __device__ __forceinline__ void synth_wide_strip(int x0, int y0, int y1,
   const float bound = red[0] + red[1] + red[2] + red[3];
   int ex = 0;
   (void)frexpf(bound > 0.f ? bound : 1.f, &ex);
-  const float pscale = ldexpf(1.0f, 12 - ex), inv_pscale = ldexpf(1.0f, ex - 12);
+  const float pscale = ldexpf(1.0f, 12 - ex);
   // --- P = E . B, B staged a quarter (128 k x 64 j) at a time ----------------
   // B[2gxi][2gy] = Jr, B[2gxi+1][2gy] = -Ji, B[2gxi][2gy+1] = Ji,
   // B[2gxi+1][2gy+1] = Jr; fragment (st, nt, lane) of a quarter holds rows
@@ -1814,7 +1813,6 @@ __device__ __forceinline__ void synth_wide_strip(int x0, int y0, int y1,
     }
   // --- the wave's P operand (its 32 columns, K = j), through LDS one half of
   // j at a time (each wave reads back only the columns it wrote)
-  half8_t pbh[2][4], pbl[2][4];
 #pragma unroll
   for (int jh = 0; jh < 2; ++jh) {
 #pragma unroll
@@ -1838,8 +1836,26 @@ __device__ __forceinline__ void synth_wide_strip(int x0, int y0, int y1,
         pbh[cb][2 * jh + ks] = *reinterpret_cast<const half8_t*>(sPh + c * kWPS + ks * 32 + kq);
         pbl[cb][2 * jh + ks] = *reinterpret_cast<const half8_t*>(sPl + c * kWPS + ks * 32 + kq);
       }
-    __syncthreads();  // (the second half and then the staging reuse this LDS)
+    __syncthreads();  // (the second half and then the caller reuse this LDS)
   }
+  return ldexpf(1.0f, ex - 12);
+}
+
+__device__ __forceinline__ void synth_wide_strip(int x0, int y0, int y1,
+                                                 const float2* __restrict__ spec, int Nx,
+                                                 int Lx, int Ly, const float4* __restrict__ Efw,
+                                                 const float4* __restrict__ TYw,
+                                                 float* __restrict__ img) {
+  constexpr int TY_F4 = 8 * 64;  // float4 per staged TY tile
+  constexpr int S_BYTES = 2 * TY_F4 * 16 + 2 * 16 * kWOS * 4;
+  constexpr int MAIN = kWideProBytes > S_BYTES ? kWideProBytes : S_BYTES;
+  __shared__ __attribute__((aligned(16))) char smem[MAIN];
+  __shared__ float red[4];
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int r16 = lane & 15, q4 = (lane >> 4) * 4;
+  half8_t pbh[2][4], pbl[2][4];
+  const float inv_pscale = wide_p_operand(x0, spec, Nx, Efw, smem, red, pbh, pbl);
   // --- rows -----------------------------------------------------------------
   float4* sTY = reinterpret_cast<float4*>(smem);                 // [2][TY_F4]
   float* sO = reinterpret_cast<float*>(sTY + 2 * TY_F4);          // [2][16][kWOS]
@@ -1920,6 +1936,507 @@ __global__ __launch_bounds__(256, 3) void k_synth_wide(const float2* __restrict_
   const int y0 = min(Lypad, rh * per), y1 = min(Lypad, y0 + per);
   synth_wide_strip(strip * kColsSynth, y0, y1, spec + (size_t)b * 64 * (4 * Nx + 1), Nx, Lx,
                    Ly, Efw, TYw, image + (size_t)b * Lx * Ly);
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint stage 3 + resist band, fused, for kernel half-sizes 16..31: as
+// k_synth_resist (the same sums, the same layouts of the partials, with 64
+// columns gx = 0..2Nx per row), on the wide synthesis of k_synth_wide.  grid
+// (ceil(Lx/128) * rs, B), 256 threads; block x takes strip x / rs and the
+// (x % rs)-th of rs row ranges.  Per 16-row tile the block stages the tile's
+// TY (8 KB) and TYg (8 KB) operands in LDS, double-buffered; wave w
+// synthesises its 32 columns (K = 128), forms G = 2 (s - T) a s (1 - s)
+// there and accumulates Q[j][x] += sum_y TYg[y][j] G[y][x] for all 128 j (64
+// VGPRs of accumulators).  Once per block, the 128 x 128 Q goes through LDS
+// and wave w forms the C[j][k] = sum_x Q[j][x] E[x][k] tiles of k = 64 w ..
+// + 64, each of whose 2 x 2 (Re / Im) blocks gives one entry of the band:
+//   G^[a][gi] = (C[2a][2gi] + C[2a+1][2gi+1]) + i (C[2a+1][2gi] - C[2a][2gi+1])
+// at (gy, gx) = (a, gi - 2Nx) for gi >= 2Nx and, conjugated, at
+// (-a, 2Nx - gi) for a > 0, gi <= 2Nx (G real).  The even lane of each
+// column pair takes its partner's two entries by one lane exchange and
+// writes the entry straight to gpart[b][part][4Ny+1][64].
+// TYgw: [Lypad/16][8 jt][hi, lo][64 lanes] x 4 halves, lane l holding
+//   TYg[16 yt + 4 (l/16) + t][16 jt + l % 16] (A operand, K = 16);
+// Ebw: [Lxp/32][16 kt][hi, lo][64 lanes] x 8 halves, lane l holding
+//   E[32 xs + 8 (l/16) + t][16 kt + l % 16] (B operand, K = 32).
+// ---------------------------------------------------------------------------
+constexpr int kQW = 132;  // LDS row stride (floats) of the block's Q
+
+template <bool TVEC>
+__device__ __forceinline__ void synth_resist_wide_strip(
+    int x0, int y0, int y1, int pidx, const float2* __restrict__ spec, int Nx, int Ny, int Lx,
+    int Ly, const float4* __restrict__ Efw, const float4* __restrict__ TYw,
+    const float* __restrict__ tgt, float a_sig, float thr, const float4* __restrict__ TYgw,
+    const float4* __restrict__ Ebw, float* __restrict__ img, float2* __restrict__ gout,
+    float* __restrict__ lout) {
+  constexpr int TY_F4 = 8 * 64, TG_F4 = 8 * 64;  // float4 per staged tile
+  constexpr int kStg = 36;  // staging row stride (floats) of a wave's 16 x 32 tile
+  constexpr int S_BYTES = 2 * (TY_F4 + TG_F4) * 16 + 4 * 16 * kStg * 4;
+  constexpr int E_BYTES = 128 * kQW * 4;  // the block's Q
+  constexpr int M1 = kWideProBytes > S_BYTES ? kWideProBytes : S_BYTES;
+  constexpr int MAIN = M1 > E_BYTES ? M1 : E_BYTES;
+  __shared__ __attribute__((aligned(16))) char smem[MAIN];
+  __shared__ float red[4], red2[4];
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int r16 = lane & 15, kq = (lane >> 4) * 8, q4 = (lane >> 4) * 4;
+  half8_t pbh[2][4], pbl[2][4];
+  const float inv_pscale = wide_p_operand(x0, spec, Nx, Efw, smem, red, pbh, pbl);
+  float4* sTY = reinterpret_cast<float4*>(smem);  // [2][TY_F4]
+  float4* sTG = sTY + 2 * TY_F4;                  // [2][TG_F4]
+  float* stg = reinterpret_cast<float*>(sTG + 2 * TG_F4) + wave * 16 * kStg;
+  const int xw = x0 + 32 * wave;
+  const bool ivec = img && (Lx & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 15) == 0;
+  float4_t Q[8][2];
+#pragma unroll
+  for (int jt = 0; jt < 8; ++jt)
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) Q[jt][cb] = float4_t{0.f, 0.f, 0.f, 0.f};
+  float lsum = 0.f;
+  // this thread's share of a tile's staged operands: 2 x 16 bytes of each table
+  auto fetch_ops = [&](int yb, float4& a0, float4& a1, float4& b0, float4& b1) {
+    const float4* ty = TYw + (size_t)(yb >> 4) * TY_F4;
+    const float4* tg = TYgw + (size_t)(yb >> 4) * TG_F4;
+    a0 = ty[tid];
+    a1 = ty[256 + tid];
+    b0 = tg[tid];
+    b1 = tg[256 + tid];
+  };
+  // the target tile of the 16 rows at yb for this wave, as k_synth_resist:
+  // lane holds rows lane / 8 and 8 + lane / 8, columns 4 (lane % 8) .. + 4,
+  // branch-free (clamped addresses; the gradient masks what lies outside)
+  auto load_t1 = [&](int y, float4& t) {
+    y = min(y, Ly - 1);
+    const int x = xw + (lane & 7) * 4;
+    const float* row = tgt + (size_t)y * Lx;
+    if constexpr (TVEC) {
+      t = *reinterpret_cast<const float4*>(row + min(x, Lx - 4));
+    } else {
+      t = make_float4(row[min(x, Lx - 1)], row[min(x + 1, Lx - 1)], row[min(x + 2, Lx - 1)],
+                      row[min(x + 3, Lx - 1)]);
+    }
+  };
+  auto load_t = [&](int yb, float4& t0, float4& t1) {
+    load_t1(yb + (lane >> 3), t0);
+    load_t1(yb + 8 + (lane >> 3), t1);
+  };
+  // s = 1 / (1 + 2^(c1 d + c0)) for the unscaled synthesis output d (I = d / pscale)
+  const float c1 = -a_sig * 1.44269504f * inv_pscale, c0 = a_sig * thr * 1.44269504f;
+  const float two_a = 2.f * a_sig;
+  auto step = [&](int yb, int sl, float4& tc0, float4& tc1, float4& a0, float4& a1, float4& b0,
+                  float4& b1) {
+    if (yb + 16 < y1) {
+      sTY[(sl ^ 1) * TY_F4 + tid] = a0;
+      sTY[(sl ^ 1) * TY_F4 + 256 + tid] = a1;
+      sTG[(sl ^ 1) * TG_F4 + tid] = b0;
+      sTG[(sl ^ 1) * TG_F4 + 256 + tid] = b1;
+      if (yb + 32 < y1) fetch_ops(yb + 32, a0, a1, b0, b1);
+    }
+    const float4* ty = sTY + sl * TY_F4 + lane;
+    float4_t d[2];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) d[cb] = float4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const half8_t ah = __builtin_bit_cast(half8_t, ty[(ks * 2 + 0) * 64]);
+      const half8_t al = __builtin_bit_cast(half8_t, ty[(ks * 2 + 1) * 64]);
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) d[cb] = mfma3(ah, al, pbh[cb][ks], pbl[cb][ks], d[cb]);
+    }
+    // target rows through the wave's staging rows into the MFMA output layout
+    *reinterpret_cast<float4*>(stg + (lane >> 3) * kStg + (lane & 7) * 4) = tc0;
+    *reinterpret_cast<float4*>(stg + (8 + (lane >> 3)) * kStg + (lane & 7) * 4) = tc1;
+    if (yb + 32 < y1) load_t(yb + 32, tc0, tc1);
+    wave_sync();
+    float tv[2][4];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) tv[cb][r] = stg[(q4 + r) * kStg + cb * 16 + r16];
+    half4_t gh[2], gl[2];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      const bool colok = xw + cb * 16 + r16 < Lx;
+      float g[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const bool ok = colok && yb + q4 + r < Ly;
+        const float sg =
+            __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(fmaf(c1, d[cb][r], c0)));
+        const float ds = ok ? sg - tv[cb][r] : 0.f;
+        lsum = fmaf(ds, ds, lsum);
+        g[r] = ds * fmaf(-sg, sg, sg) * two_a;
+      }
+      split4(g, gh[cb], gl[cb]);
+    }
+    if (img) {
+      wave_sync();
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) stg[(q4 + r) * kStg + cb * 16 + r16] = d[cb][r] * inv_pscale;
+      wave_sync();
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int row = 8 * h + (lane >> 3), col = (lane & 7) * 4;
+        const float4 v = *reinterpret_cast<const float4*>(stg + row * kStg + col);
+        const int y = yb + row, x = xw + col;
+        if (y >= Ly) continue;
+        float* dst = img + (size_t)y * Lx + x;
+        if (ivec) {
+          if (x < Lx)
+            __builtin_nontemporal_store(float4_t{v.x, v.y, v.z, v.w},
+                                        reinterpret_cast<float4_t*>(dst));
+        } else {
+          if (x < Lx) dst[0] = v.x;
+          if (x + 1 < Lx) dst[1] = v.y;
+          if (x + 2 < Lx) dst[2] = v.z;
+          if (x + 3 < Lx) dst[3] = v.w;
+        }
+      }
+    }
+    wave_sync();  // the staging rows' reads end before the next step's writes
+    // Q += TYg^T . G on v_mfma_f32_16x16x16_f16 (K = the tile's 16 rows)
+    const half4_t* tg = reinterpret_cast<const half4_t*>(sTG + sl * TG_F4) + lane;
+#pragma unroll
+    for (int jt = 0; jt < 8; ++jt) {
+      const half4_t gah = tg[(jt * 2 + 0) * 64], gal = tg[(jt * 2 + 1) * 64];
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) Q[jt][cb] = mfma3k16(gah, gal, gh[cb], gl[cb], Q[jt][cb]);
+    }
+    __syncthreads();  // slot sl is read, slot sl ^ 1 written, by every wave
+  };
+  {
+    float4 ta0, ta1, tb0, tb1, a0, a1, b0, b1;
+    if (y0 < y1) {
+      fetch_ops(y0, a0, a1, b0, b1);
+      sTY[tid] = a0;
+      sTY[256 + tid] = a1;
+      sTG[tid] = b0;
+      sTG[256 + tid] = b1;
+      if (y0 + 16 < y1) fetch_ops(y0 + 16, a0, a1, b0, b1);
+      load_t(y0, ta0, ta1);
+      if (y0 + 16 < y1) load_t(y0 + 16, tb0, tb1);
+    }
+    __syncthreads();
+    // (y1 - y0 is a multiple of 32: the steps come in pairs)
+    for (int yb = y0; yb < y1; yb += 32) {
+      step(yb, 0, ta0, ta1, a0, a1, b0, b1);
+      step(yb + 16, 1, tb0, tb1, a0, a1, b0, b1);
+    }
+  }
+  // --- the block's Q (128 j x 128 columns) in LDS, then the C tiles ----------
+  lsum = wave_sum_f(lsum);
+  if (lane == 0) red2[wave] = lsum;
+  float* sQ = reinterpret_cast<float*>(smem);
+#pragma unroll
+  for (int jt = 0; jt < 8; ++jt)
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        sQ[(16 * jt + q4 + r) * kQW + 32 * wave + cb * 16 + r16] = Q[jt][cb][r];
+  __syncthreads();
+  // largest |Q| of the block: Q 2^sh in [2^13, 2^14) keeps both fp16 halves of
+  // the split in range whatever the image's level
+  float mx = 0.f;
+  for (int i = tid; i < 128 * 128; i += 256)
+    mx = fmaxf(mx, fabsf(sQ[(i >> 7) * kQW + (i & 127)]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+  if (lane == 0) red[wave] = mx;
+  __syncthreads();
+  mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  int eq = 0;
+  (void)frexpf(mx > 0.f ? mx : 1.f, &eq);
+  const int sh = 14 - eq;
+  // wave w: column tiles kt = 4 w .. 4 w + 3 of C, all eight j tiles
+  float4_t C[8][4];
+#pragma unroll
+  for (int jt = 0; jt < 8; ++jt)
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) C[jt][kk] = float4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    half8_t bh[4], bl[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const float4* eb = Ebw + ((size_t)((x0 >> 5) + st) * 16 + 4 * wave + kk) * 2 * 64 + lane;
+      bh[kk] = __builtin_bit_cast(half8_t, eb[0]);
+      bl[kk] = __builtin_bit_cast(half8_t, eb[64]);
+    }
+#pragma unroll
+    for (int jt = 0; jt < 8; ++jt) {
+      const float* row = sQ + (16 * jt + r16) * kQW + 32 * st + kq;
+      const float4 u0 = *reinterpret_cast<const float4*>(row);
+      const float4 u1 = *reinterpret_cast<const float4*>(row + 4);
+      float v[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+      for (int t = 0; t < 8; ++t) v[t] = ldexpf(v[t], sh);
+      half8_t ah, al;
+      split8(v, ah, al);
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) C[jt][kk] = mfma3(ah, al, bh[kk], bl[kk], C[jt][kk]);
+    }
+  }
+  // lane l holds C[16 jt + 4 (l/16) + r][16 kt + l % 16]: rows 2a, 2a + 1
+  // (r = 2 rp, 2 rp + 1), columns 2 gi (even lane) and 2 gi + 1 (odd lane)
+  const int Gy = 4 * Ny + 1;
+  float2* out = gout + (size_t)pidx * Gy * 64;
+#pragma unroll
+  for (int jt = 0; jt < 8; ++jt)
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      float c[4], p[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        c[r] = ldexpf(C[jt][kk][r], -sh);
+        p[r] = __shfl_xor(c[r], 1, 64);
+      }
+      const int gi = 8 * (4 * wave + kk) + (r16 >> 1);
+      if ((lane & 1) == 0 && gi <= 4 * Nx) {
+#pragma unroll
+        for (int rp = 0; rp < 2; ++rp) {
+          const int a = 8 * jt + 2 * (lane >> 4) + rp;
+          if (a > 2 * Ny) continue;
+          const float re = c[2 * rp] + p[2 * rp + 1], im = c[2 * rp + 1] - p[2 * rp];
+          if (gi >= 2 * Nx) out[(a + 2 * Ny) * 64 + gi - 2 * Nx] = make_float2(re, im);
+          if (a > 0 && gi <= 2 * Nx) out[(2 * Ny - a) * 64 + 2 * Nx - gi] = make_float2(re, -im);
+        }
+      }
+    }
+  __syncthreads();
+  if (tid == 0) lout[pidx] = red2[0] + red2[1] + red2[2] + red2[3];
+}
+
+template <bool TVEC>
+__global__ __launch_bounds__(256, 2) void k_synth_resist_wide(
+    const float2* __restrict__ spec, int Nx, int Ny, int Lx, int Ly, int Lypad, int rs,
+    const float4* __restrict__ Efw, const float4* __restrict__ TYw, const float* __restrict__ tgt,
+    float a_sig, float thr, const float4* __restrict__ TYgw, const float4* __restrict__ Ebw,
+    float* __restrict__ image, float2* __restrict__ gpart, float* __restrict__ lpart) {
+  const int b = blockIdx.y, strip = blockIdx.x / rs, rh = blockIdx.x - strip * rs;
+  const int parts = gridDim.x;
+  const int per = (Lypad / 32 + rs - 1) / rs * 32;
+  const int y0 = min(Lypad, rh * per), y1 = min(Lypad, y0 + per);
+  const size_t ib = (size_t)b * Lx * Ly;
+  synth_resist_wide_strip<TVEC>(strip * kColsSynth, y0, y1, blockIdx.x,
+                                spec + (size_t)b * 64 * (4 * Nx + 1), Nx, Ny, Lx, Ly, Efw, TYw,
+                                tgt + ib, a_sig, thr, TYgw, Ebw, image ? image + ib : nullptr,
+                                gpart + (size_t)b * parts * (4 * Ny + 1) * 64,
+                                lpart + (size_t)b * parts);
+}
+
+// grid (B), dynamic LDS 96 kRS float2: g_c(u) = sum_g G^[g] e^{+2 pi i g.u/128}
+// on the 128 x 128 coarse grid (real: G^ is Hermitian), thread-major
+// [B][256][64] in the layout of k_socs_mid128's intensity (thread (g, t):
+// column ux = g + 32 q, rows uy = t + 8 h + 16 k1 at [q][8 h + k1]).  Rows
+// gy >= 0 of G^ are transformed along gx (in place in LDS); rows gy < 0 are
+// their conjugates.
+__global__ __launch_bounds__(256) void k_resist_coarse128(const float2* __restrict__ Gfull,
+                                                          int Nx, int Ny,
+                                                          const float2* __restrict__ tw128,
+                                                          float* __restrict__ gcout) {
+  extern __shared__ float4 smem4[];
+  float2* sB = reinterpret_cast<float2*>(smem4);  // [64][kRS]: rows gy = 0..2Ny
+  float2* sX = sB + 64 * kRS;                     // [32][kRS]
+  const int b = blockIdx.x, tid = threadIdx.x, g = tid >> 3, t = tid & 7;
+  const int Gx = 4 * Nx + 1, G2 = 2 * Ny + 1;
+  const float2* G = Gfull + (size_t)b * Gx * (4 * Ny + 1);
+  float2 twi[16];
+#pragma unroll
+  for (int k2 = 0; k2 < 16; ++k2) twi[k2] = tw128[(t * k2) & 127];
+#pragma unroll
+  for (int rr = 0; rr < 2; ++rr) {
+    const int gy = g + 32 * rr;
+    const bool act = gy < G2;
+    float2 v[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) {
+      const int n = t + 8 * n2;
+      const int gx = n < 64 ? n : n - 128;
+      v[n2] = (act && gx >= -2 * Nx && gx <= 2 * Nx) ? G[(gy + 2 * Ny) * Gx + gx + 2 * Nx]
+                                                     : make_float2(0.f, 0.f);
+    }
+    float2* xr = sB + (act ? gy : 0) * kRS;
+    fft128_group<1>(v, twi, xr, t, act);
+    if (act) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int k1 = 0; k1 < 8; ++k1) xr[t + 8 * h + 16 * k1] = v[8 * h + k1];
+    }
+  }
+  __syncthreads();
+  float4* out = reinterpret_cast<float4*>(gcout + (size_t)b * 16384 + tid * 64);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int ux = g + 32 * q;
+    float2 v[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) {
+      const int n = t + 8 * n2;
+      const int gy = n < 64 ? n : n - 128;
+      float2 val = make_float2(0.f, 0.f);
+      if (gy >= 0 && gy <= 2 * Ny) {
+        val = sB[gy * kRS + ux];
+      } else if (gy < 0 && -gy <= 2 * Ny) {
+        const float2 u = sB[-gy * kRS + ux];
+        val = make_float2(u.x, -u.y);
+      }
+      v[n2] = val;
+    }
+    fft128_group<1>(v, twi, sX + g * kRS, t, true);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      out[q * 4 + i] = make_float4(v[4 * i].x, v[4 * i + 1].x, v[4 * i + 2].x, v[4 * i + 3].x);
+  }
+}
+
+// grid (kg, B), dynamic LDS (96 kRS + T) float2: the adjoint coarse stage of
+// half-sizes 16..31, as k_socs_mid_bwd on the 128-point grid.  Block (j, b)
+// takes the j-th of kg groups of the kernels; per kernel, E_k = the inverse
+// transform of K_k M^ (as k_socs_mid128), then per column conj(E_k) g_c
+// transformed forward along uy (|qy| <= Ny kept, in place in LDS), then along
+// ux (|qx| <= Nx kept): F_k[q]; Gam[-q] += lambda_k K_k[-q] F_k[q] / 128^2 in
+// LDS (one owner thread per entry).  kg = 1 writes the gradient spectrum
+// gspec [B][64][4Nx+1]; kg > 1 the partial Gam to gam_part [B][kg][T] for
+// k_socs_mid_bwd_fin.
+__global__ __launch_bounds__(256) void k_socs_mid_bwd128(
+    const float2* __restrict__ Mfull, const float* __restrict__ gcin, int Nx, int Ny,
+    const float2* __restrict__ kern, const float* __restrict__ scales, int nk,
+    const float2* __restrict__ tw128, int dcx, int dcy, int Lx, int Ly,
+    float2* __restrict__ gspec, float2* __restrict__ gam_part) {
+  extern __shared__ float4 smem4[];
+  float2* sB = reinterpret_cast<float2*>(smem4);  // [64][kRS]
+  float2* sX = sB + 64 * kRS;                     // [32][kRS]
+  float2* sGam = sX + 32 * kRS;                   // [T]
+  const int b = blockIdx.y, grp = blockIdx.x, kg = gridDim.x, tid = threadIdx.x;
+  const int per = (nk + kg - 1) / kg, kbeg = min(nk, grp * per), kend = min(nk, kbeg + per);
+  const int g = tid >> 3, t = tid & 7;
+  const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
+  const float2* M = Mfull + (size_t)b * T;
+  for (int i = tid; i < T; i += 256) sGam[i] = make_float2(0.f, 0.f);
+  float2 twi[16];
+#pragma unroll
+  for (int k2 = 0; k2 < 16; ++k2) twi[k2] = tw128[(t * k2) & 127];
+  float gc[4][16];
+  {
+    const float4* src = reinterpret_cast<const float4*>(gcin + (size_t)b * 16384 + tid * 64);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float4 u = src[q * 4 + i];
+        gc[q][4 * i] = u.x;
+        gc[q][4 * i + 1] = u.y;
+        gc[q][4 * i + 2] = u.z;
+        gc[q][4 * i + 3] = u.w;
+      }
+  }
+  float2* xs = sX + g * kRS;
+  __syncthreads();
+  const float inv = 1.0f / (128.f * 128.f);
+  for (int k = kbeg; k < kend; ++k) {
+    const float2* K = kern + (size_t)k * T;
+    const float lam = scales[k];
+#pragma unroll
+    for (int rr = 0; rr < 2; ++rr) {
+      const int r = g + 32 * rr;
+      const bool act = r < Wy;
+      float2 v[16];
+#pragma unroll
+      for (int n2 = 0; n2 < 16; ++n2) v[n2] = make_float2(0.f, 0.f);
+      if (act) {
+#pragma unroll
+        for (int n2 = 0; n2 < 16; ++n2) {
+          if (n2 >= 4 && n2 < 12) continue;
+          const int n = t + 8 * n2;
+          const int nx = n < 64 ? n : n - 128;
+          if (nx >= -Nx && nx <= Nx) {
+            const int i = r * Wx + nx + Nx;
+            v[n2] = cmulf(K[i], M[i]);
+          }
+        }
+      }
+      float2* xr = sB + (act ? r : 0) * kRS;
+      fft128_group<1>(v, twi, xr, t, act);
+      if (act) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int q1 = 0; q1 < 8; ++q1) xr[t + 8 * h + 16 * q1] = v[8 * h + q1];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int ux = g + 32 * q;
+      float2 v[16];
+#pragma unroll
+      for (int n2 = 0; n2 < 16; ++n2) {
+        v[n2] = make_float2(0.f, 0.f);
+        if (n2 >= 4 && n2 < 12) continue;
+        const int n = t + 8 * n2;
+        const int ny = n < 64 ? n : n - 128;
+        if (ny >= -Ny && ny <= Ny) v[n2] = sB[(ny + Ny) * kRS + ux];
+      }
+      fft128_group<1>(v, twi, xs, t, true);
+      // conj(e) g_c (g_c real), reordered to the input layout (uy = t + 8 n2),
+      // then forward along uy
+      float2 w[16];
+#pragma unroll
+      for (int n2 = 0; n2 < 16; ++n2) {
+        const int i = 8 * (n2 & 1) + (n2 >> 1);
+        w[n2] = make_float2(v[i].x * gc[q][i], -v[i].y * gc[q][i]);
+      }
+      fft128_group<-1>(w, twi, xs, t, true);
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int q1 = 0; q1 < 8; ++q1) {
+          const int qq = t + 8 * h + 16 * q1;
+          const int qy = qq < 64 ? qq : qq - 128;
+          if (qy >= -Ny && qy <= Ny) sB[(qy + Ny) * kRS + ux] = w[8 * h + q1];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < 2; ++rr) {
+      const int row = g + 32 * rr;
+      const bool act = row < Wy;
+      const int qy = row - Ny;
+      float2 v[16];
+      if (act) {
+#pragma unroll
+        for (int n2 = 0; n2 < 16; ++n2) v[n2] = sB[row * kRS + t + 8 * n2];
+      }
+      fft128_group<-1>(v, twi, xs, t, act);
+      if (act) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int q1 = 0; q1 < 8; ++q1) {
+            const int qq = t + 8 * h + 16 * q1;
+            const int qx = qq < 64 ? qq : qq - 128;
+            if (qx >= -Nx && qx <= Nx) {
+              const int idx = (-qy + Ny) * Wx + (-qx + Nx);
+              const float2 f = make_float2(v[8 * h + q1].x * inv, v[8 * h + q1].y * inv);
+              const float2 kf = cmulf(K[idx], f);
+              sGam[idx].x += lam * kf.x;
+              sGam[idx].y += lam * kf.y;
+            }
+          }
+      }
+    }
+    __syncthreads();  // the rows' reads end before the next kernel's rows
+  }
+  if (kg > 1) {
+    float2* dst = gam_part + ((size_t)b * kg + grp) * T;
+    for (int i = tid; i < T; i += 256) dst[i] = sGam[i];
+    return;
+  }
+  gam_to_spec(sGam, Nx, Ny, dcx, dcy, Lx, Ly, gspec + (size_t)b * 64 * (4 * Nx + 1), 64);
 }
 
 // Mixed sizes and kernel sets: one block per SynthJob (one strip of one
@@ -2488,6 +3005,57 @@ static int get_fused_tables(klt_ctx* ctx, SocsPlan* p, FusedTables** out) {
     }
     KLT_TRY(t->tw128.ensure(128 * sizeof(float2)));
     KLT_HIP(hipMemcpy(t->tw128.ptr, t128.data(), 128 * sizeof(float2), hipMemcpyHostToDevice));
+    // k_synth_resist_wide: TYg[y][2 gy] = cos, TYg[y][2 gy + 1] = -sin of
+    // 2 pi gy (y - (Ly+1)/2)/Ly (gy <= 2Ny, y < Ly) as A fragments of
+    // v_mfma_f32_16x16x16_f16 over 16-row tiles (tile yt, j tile jt < 8, half
+    // h, lane l, t < 4: y = 16 yt + 4 (l/16) + t, j = 16 jt + l % 16), and the
+    // synthesis column twiddles E as B fragments over 32-column steps (step
+    // xs, k tile kt < 16, half h, lane l, t < 8: x = 32 xs + 8 (l/16) + t,
+    // k = 16 kt + l % 16)
+    std::vector<_Float16> tg((size_t)t->Lypad / 16 * 8 * 2 * 64 * 4, (_Float16)0.f);
+    for (int yt = 0; yt < t->Lypad / 16; ++yt)
+      for (int jt = 0; jt < 8; ++jt)
+        for (int ln = 0; ln < 64; ++ln)
+          for (int tt = 0; tt < 4; ++tt) {
+            const int y = 16 * yt + 4 * (ln / 16) + tt, j = 16 * jt + ln % 16, gy = j / 2;
+            double v = 0.0;
+            if (y < Ly && gy <= 2 * Ny) {
+              long long r = ((long long)gy * (y - coy)) % Ly;
+              if (r < 0) r += Ly;
+              const double a = 2.0 * M_PI * (double)r / (double)Ly;
+              v = (j & 1) ? -std::sin(a) : std::cos(a);
+            }
+            _Float16 vh, vl;
+            split_half(v, &vh, &vl);
+            const size_t o = ((((size_t)yt * 8 + jt) * 2) * 64 + ln) * 4 + tt;
+            tg[o] = vh;
+            tg[o + 64 * 4] = vl;
+          }
+    KLT_TRY(t->TYgw.ensure(tg.size() * sizeof(_Float16)));
+    KLT_HIP(hipMemcpy(t->TYgw.ptr, tg.data(), tg.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+    const int Lxp = round_up(Lx, kColsSynth), Gx = 4 * Nx + 1, cox = (Lx + 1) / 2;
+    std::vector<_Float16> eb((size_t)Lxp / 32 * 16 * 2 * 64 * 8);
+    for (int xs = 0; xs < Lxp / 32; ++xs)
+      for (int kt = 0; kt < 16; ++kt)
+        for (int ln = 0; ln < 64; ++ln)
+          for (int tt = 0; tt < 8; ++tt) {
+            const int x = 32 * xs + 8 * (ln / 16) + tt, k = 16 * kt + ln % 16;
+            const int gxi = k / 2, c = k % 2;
+            double v = 0.0;
+            if (x < Lx && gxi < Gx) {
+              long long r = ((long long)(gxi - 2 * Nx) * (x - cox)) % Lx;
+              if (r < 0) r += Lx;
+              const double ang = 2.0 * M_PI * (double)r / (double)Lx;
+              v = c ? std::sin(ang) : std::cos(ang);
+            }
+            _Float16 vh, vl;
+            split_half(v, &vh, &vl);
+            const size_t o = ((((size_t)xs * 16 + kt) * 2) * 64 + ln) * 8 + tt;
+            eb[o] = vh;
+            eb[o + 64 * 8] = vl;
+          }
+    KLT_TRY(t->Ebw.ensure(eb.size() * sizeof(_Float16)));
+    KLT_HIP(hipMemcpy(t->Ebw.ptr, eb.data(), eb.size() * sizeof(_Float16), hipMemcpyHostToDevice));
     *out = t.get();
     ctx->fused_tables[key] = std::move(t);
     return KLT_OK;
@@ -3097,7 +3665,8 @@ int resist_coarse_grid(klt_ctx* ctx, const float2* gpart, const float* lpart, in
   const int Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
   KLT_TRY(ctx->gfull.ensure((size_t)B * Gx * Gy * sizeof(float2)));
   KLT_LAUNCH(ctx, "k_resist_sum", k_resist_sum, dim3((unsigned)(Gx * Gy + 255) / 256, B),
-             dim3(256), 0, gpart, lpart, parts, Nx, Ny, inv_area, ctx->gfull.as<float2>(), loss);
+             dim3(256), 0, gpart, lpart, parts, Nx, Ny, 32, inv_area, ctx->gfull.as<float2>(),
+             loss);
   KLT_LAUNCH(ctx, "k_resist_coarse", k_resist_coarse, dim3(B), dim3(256), 0,
              ctx->gfull.as<float2>(), Nx, Ny, ctx->tw64.as<float2>(), gc);
   KLT_HIP(hipGetLastError());
@@ -3252,7 +3821,7 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
       // each image's G^, coarse-grid g_c and loss once, for all kernel groups
       KLT_LAUNCH(ctx, "k_resist_sum", k_resist_sum,
                  dim3((unsigned)(Gx * Gy + 255) / 256, nb), dim3(256), 0,
-                 ctx->gpart.as<float2>(), ctx->lpart.as<float>(), sparts, Nx, Ny, inv_area,
+                 ctx->gpart.as<float2>(), ctx->lpart.as<float>(), sparts, Nx, Ny, 32, inv_area,
                  ctx->gfull.as<float2>(), losses + b0);
       KLT_LAUNCH(ctx, "k_resist_coarse", k_resist_coarse, dim3(nb), dim3(256), 0,
                  ctx->gfull.as<float2>(), Nx, Ny, ft->tw64.as<float2>(),
@@ -3268,11 +3837,109 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
                    ctx->gsum.as<float>());
         KLT_LAUNCH(ctx, "k_socs_mid_bwd_fin", k_socs_mid_bwd_fin, dim3(nb), dim3(256), 0,
                    reinterpret_cast<const float2*>(ctx->gsum.as<float>()), 1, Nx, Ny, dcx,
-                   dcy, Lx, Ly, ctx->gspec.as<float2>());
+                   dcy, Lx, Ly, ctx->gspec.as<float2>(), 32);
       }
       KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma, dim3(nstrips * rs, nb), dim3(256), 0,
                  ctx->gspec.as<float2>(), Nx, Lx, Ly, ft->Lypad, rs, ft->Ef.as<float4>(),
                  ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), grads + off);
+      KLT_HIP(hipGetLastError());
+      return KLT_OK;
+    };
+    klt_ctx::Tuned c{};
+    KLT_TRY(pick_split(ctx, 1, nb, Lx, Ly, Nx, Ny, kb->nk, ft->Kpad / (kBandChunk * 32), nblk,
+                       nstrips, ft->Lypad / 32, run, &c));
+    KLT_TRY(run(c));
+  }
+  return KLT_OK;
+}
+
+// Adjoint of the SOCS image with the resist loss for kernel half-sizes
+// 16..31: the same sums as socs_backward_fused on the wide passes (band with
+// 64 twiddle rows, 128-point coarse stages, synthesis at K = 128 fused with
+// the resist band, k_synth_wide for the gradient image).
+int socs_backward_wide(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
+                       const float* targets, int B, int Lx, int Ly, float a_sig, float thr,
+                       float* grads, float* losses, float* images) {
+  SocsPlan* p = nullptr;
+  KLT_TRY(get_socs_plan(ctx, Lx, Ly, kb->Nx, kb->Ny, &p));
+  FusedTables* ft = nullptr;
+  KLT_TRY(get_fused_tables(ctx, p, &ft));
+  const int Nx = kb->Nx, Ny = kb->Ny, Wy = 2 * Ny + 1, Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
+  const int T = (2 * Nx + 1) * Wy;
+  const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
+  const int nblk64 = (Ly + 63) / 64;
+  const int nstrips = (Lx + kColsSynth - 1) / kColsSynth;
+  const int SB = std::min(B, 512);
+  KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk64 * kMaxKs * Wy * 32 * sizeof(float2)));
+  KLT_TRY(ctx->spec.ensure((size_t)SB * 64 * Gx * sizeof(float2)));
+  KLT_TRY(ctx->gspec.ensure((size_t)SB * 64 * Gx * sizeof(float2)));
+  KLT_TRY(ctx->ipart.ensure((size_t)kGroupSlots * 128 * 128 * sizeof(float)));
+  KLT_TRY(ctx->gam_part.ensure((size_t)kGroupSlots * T * sizeof(float2)));
+  KLT_TRY(ctx->mhat.ensure((size_t)SB * T * sizeof(float2)));
+  KLT_TRY(ctx->gcoarse.ensure((size_t)SB * 128 * 128 * sizeof(float)));
+  KLT_TRY(ctx->gfull.ensure((size_t)SB * Gx * Gy * sizeof(float2)));
+  const int lds_fin = 96 * kRS * (int)sizeof(float2);
+  const int lds_mid = lds_fin + T * (int)sizeof(float2);
+  for (const void* f : {reinterpret_cast<const void*>(&k_socs_mid128),
+                        reinterpret_cast<const void*>(&k_socs_mid_bwd128)})
+    KLT_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_mid));
+  for (const void* f : {reinterpret_cast<const void*>(&k_socs_mid128_fin),
+                        reinterpret_cast<const void*>(&k_resist_coarse128)})
+    KLT_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_fin));
+  const float inv_area = 1.0f / ((float)Lx * (float)Ly);
+  const int dcx = (Lx + 1) / 2 - Lx / 2, dcy = (Ly + 1) / 2 - Ly / 2;
+  const bool tvec = Lx % 4 == 0 && (reinterpret_cast<uintptr_t>(targets) & 15) == 0;
+  for (int b0 = 0; b0 < B; b0 += SB) {
+    const int nb = std::min(SB, B - b0);
+    const size_t off = (size_t)b0 * Lx * Ly;
+    float* I = images ? images + off : nullptr;
+    auto run = [&](const klt_ctx::Tuned& c) -> int {
+      const int parts = (Ly + c.rb - 1) / c.rb * c.ks, sparts = nstrips * c.rs;
+      KLT_TRY(ctx->gpart.ensure((size_t)nb * sparts * Gy * 64 * sizeof(float2)));
+      KLT_TRY(ctx->lpart.ensure((size_t)nb * sparts * sizeof(float)));
+      launch_mask_band_wide(ctx, c.rb, c.ks, nb, masks + off, Lx, Ly, ft->Kpad,
+                            ft->Wf.as<float4>(), p->twy_in.as<float2>(), p->NP, Nx, Ny,
+                            ctx->qpart.as<float2>());
+      KLT_LAUNCH(ctx, "k_band_assemble", k_band_assemble, dim3((unsigned)(T + 255) / 256, nb),
+                 dim3(256), 0, ctx->qpart.as<float2>(), parts, Nx, Ny, 32, inv_area,
+                 ctx->mhat.as<float2>());
+      KLT_LAUNCH(ctx, "k_socs_mid128", k_socs_mid128, dim3(c.kg, nb), dim3(256), lds_mid,
+                 ctx->mhat.as<float2>(), Nx, Ny, kb->kern.as<float2>(), kb->scales.as<float>(),
+                 kb->nk, ft->tw128.as<float2>(), ctx->spec.as<float2>(),
+                 c.kg > 1 ? ctx->ipart.as<float>() : nullptr);
+      if (c.kg > 1)
+        KLT_LAUNCH(ctx, "k_socs_mid128_fin", k_socs_mid128_fin, dim3(nb), dim3(256), lds_fin,
+                   ctx->ipart.as<float>(), c.kg, Nx, Ny, ft->tw128.as<float2>(),
+                   ctx->spec.as<float2>());
+      if (tvec)
+        KLT_LAUNCH(ctx, "k_synth_resist_wide", k_synth_resist_wide<true>, dim3(sparts, nb),
+                   dim3(256), 0, ctx->spec.as<float2>(), Nx, Ny, Lx, Ly, ft->Lypad, c.rs,
+                   ft->Efw.as<float4>(), ft->TYw.as<float4>(), targets + off, a_sig, thr,
+                   ft->TYgw.as<float4>(), ft->Ebw.as<float4>(), I, ctx->gpart.as<float2>(),
+                   ctx->lpart.as<float>());
+      else
+        KLT_LAUNCH(ctx, "k_synth_resist_wide", k_synth_resist_wide<false>, dim3(sparts, nb),
+                   dim3(256), 0, ctx->spec.as<float2>(), Nx, Ny, Lx, Ly, ft->Lypad, c.rs,
+                   ft->Efw.as<float4>(), ft->TYw.as<float4>(), targets + off, a_sig, thr,
+                   ft->TYgw.as<float4>(), ft->Ebw.as<float4>(), I, ctx->gpart.as<float2>(),
+                   ctx->lpart.as<float>());
+      KLT_LAUNCH(ctx, "k_resist_sum", k_resist_sum, dim3((unsigned)(Gx * Gy + 255) / 256, nb),
+                 dim3(256), 0, ctx->gpart.as<float2>(), ctx->lpart.as<float>(), sparts, Nx, Ny,
+                 64, inv_area, ctx->gfull.as<float2>(), losses + b0);
+      KLT_LAUNCH(ctx, "k_resist_coarse128", k_resist_coarse128, dim3(nb), dim3(256), lds_fin,
+                 ctx->gfull.as<float2>(), Nx, Ny, ft->tw128.as<float2>(),
+                 ctx->gcoarse.as<float>());
+      KLT_LAUNCH(ctx, "k_socs_mid_bwd128", k_socs_mid_bwd128, dim3(c.kg, nb), dim3(256),
+                 lds_mid, ctx->mhat.as<float2>(), ctx->gcoarse.as<float>(), Nx, Ny,
+                 kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk, ft->tw128.as<float2>(),
+                 dcx, dcy, Lx, Ly, ctx->gspec.as<float2>(), ctx->gam_part.as<float2>());
+      if (c.kg > 1)
+        KLT_LAUNCH(ctx, "k_socs_mid_bwd_fin", k_socs_mid_bwd_fin, dim3(nb), dim3(256), 0,
+                   ctx->gam_part.as<float2>(), c.kg, Nx, Ny, dcx, dcy, Lx, Ly,
+                   ctx->gspec.as<float2>(), 64);
+      KLT_LAUNCH(ctx, "k_synth_wide", k_synth_wide, dim3(nstrips * c.rs, nb), dim3(256), 0,
+                 ctx->gspec.as<float2>(), Nx, Lx, Ly, ft->Lypad, c.rs, ft->Efw.as<float4>(),
+                 ft->TYw.as<float4>(), grads + off);
       KLT_HIP(hipGetLastError());
       return KLT_OK;
     };
@@ -3297,8 +3964,9 @@ int klt_socs_backward(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
     return klt::set_error(KLT_ERR_ARG, "klt_socs_backward: bad size");
   if (4 * kb->Nx + 1 > Lx || 4 * kb->Ny + 1 > Ly)
     return klt::set_error(KLT_ERR_SIZE, "ERROR in FI()");
-  if (!klt::socs_fused_supported(kb))
-    return klt::set_error(KLT_ERR_ARG, "klt_socs_backward: kernel half-size above 15");
+  const bool wide = !klt::socs_fused_supported(kb);
+  if (wide && !klt::socs_wide_supported(kb))
+    return klt::set_error(KLT_ERR_ARG, "klt_socs_backward: kernel half-size above 31");
   KLT_TRY(ctx->activate());
   const size_t bytes = (size_t)nmask * Lx * Ly * sizeof(float);
   const float *dm = masks, *dt = targets;
@@ -3316,7 +3984,10 @@ int klt_socs_backward(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
     di = images ? reinterpret_cast<float*>(o + bytes) : nullptr;
     dl = reinterpret_cast<float*>(o + 2 * bytes);
   }
-  KLT_TRY(klt::socs_backward_fused(ctx, kb, dm, dt, nmask, Lx, Ly, a_sig, thr, dg, dl, di));
+  if (wide)
+    KLT_TRY(klt::socs_backward_wide(ctx, kb, dm, dt, nmask, Lx, Ly, a_sig, thr, dg, dl, di));
+  else
+    KLT_TRY(klt::socs_backward_fused(ctx, kb, dm, dt, nmask, Lx, Ly, a_sig, thr, dg, dl, di));
   if (flags & KLT_HOST_PTRS) {
     KLT_HIP(hipMemcpyAsync(grads, dg, bytes, hipMemcpyDeviceToHost, ctx->stream));
     KLT_HIP(hipMemcpyAsync(losses, dl, (size_t)nmask * 4, hipMemcpyDeviceToHost, ctx->stream));
