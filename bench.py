@@ -18,6 +18,9 @@ Also reported (same JSON line):
   batch_uneven 64 masks of sides {256..1024} x 3 kernel sets (defocus -25/0/25
                nm) = 192 images per klt_socs_forward_batch graph launch, with
                p50/p99 per-image latency and the same work as grouped calls
+  wide         kernel half-sizes above 15: the reference's own test_tcc.sh
+               imaging (2048^2, 3 kernels of 33 x 33) and N = 31 (24 kernels of
+               63 x 63 at 1024^2), forward images/s and adjoint mask grads/s
   kernels      per-kernel HIP-event times of one step; `roofline` is the
                dominant kernel's algorithmic HBM bytes over its duration
   cpu_baseline the reference's FFTW3 path (FT_r2c + calcSOCS + FI) restated
@@ -412,6 +415,69 @@ def section_adjoint(ctx, timer, torch, steps, warmup, batch):
             "mask_grads_per_s": batch / (ms / 1e3)}
 
 
+WIDE_KERNELS = ("k_band_mfma", "k_synth_wide")  # the wide forward's streams
+
+
+def _wide_row(ctx, timer, torch, dev, kr, ki, sc, N, side, B, B_adj):
+    """Forward images/s over B seeded side^2 Manhattan masks and adjoint mask
+    gradients/s over B_adj relaxed ones on the kernel half-size N > 15 path,
+    with per-kernel HIP-event times of one forward call."""
+    bank = ctx.kernels(kr, ki, sc, 2 * N + 1, 2 * N + 1)
+    mh = np.stack([W.manhattan(side, 7000 + i) for i in range(min(B, 16))])
+    masks = torch.from_numpy(mh).to(dev).repeat((B + len(mh) - 1) // len(mh), 1, 1)[:B]
+    masks = masks.contiguous()
+    images = torch.empty_like(masks)
+    for _ in range(3):
+        ctx.socs_forward(bank, masks, images)
+    ms = timer(lambda: ctx.socs_forward(bank, masks, images), 10)
+    ctx.set_profiling(True)
+    ctx.socs_forward(bank, masks, images)
+    rep = ctx.profile_report()
+    ctx.set_profiling(False)
+    kern = {k: {"ms": v[1], "launches": v[0]} for k, v in rep.items()}
+    for k in WIDE_KERNELS:
+        if k in kern:
+            kern[k]["achieved_gbs"] = 4.0 * side * side * B / (kern[k]["ms"] / 1e3) / 1e9
+    dm, dt = masks[:B_adj].clone(), masks[:B_adj].clone()
+    dm.add_(torch.rand_like(dm) * 0.2 - 0.1).clamp_(0.0, 1.0)
+    for _ in range(3):
+        klt.socs_backward(ctx, bank, dm, dt)
+    ms_adj = timer(lambda: klt.socs_backward(ctx, bank, dm, dt), 10)
+    bank.close()
+    return {"N": N, "K": int(len(sc)), "side": side, "batch": B,
+            "images_per_s": B / (ms / 1e3), "ms_per_batch": ms,
+            "path_hbm_gbs": 8.0 * side * side * B / (ms / 1e3) / 1e9, "kernels": kern,
+            "adjoint_batch": B_adj, "mask_grads_per_s": B_adj / (ms_adj / 1e3),
+            "adjoint_ms_per_batch": ms_adj}
+
+
+def bench_wide(ctx, timer, torch, dev):
+    """Kernel half-sizes above 15 (the matrix-core wide path: 64-row band, 128-point
+    coarse FFTs, K = 128 synthesis):
+      sample  the reference's own test_tcc.sh imaging: 2048^2 period, annular 101
+              grid 0.6 / 0.9, NA 0.8, defocus 0.2, 3 kernels of 33 x 33 (N = 16),
+              built on the GPU; 32 masks of 2048^2 (adjoint: 8);
+      n31     N = 31 (the largest half-size): 24 kernels of 63 x 63 from the annular
+              51 grid at NA 1.35, period 2400 nm; 128 masks of 1024^2 (adjoint:
+              128).
+    (Their images and gradients are checked against the FP64 oracle in the
+    tests.)"""
+    out = {}
+    src = klt.source_create(101, "Annular", 0.6, 0.9)
+    sig = min(klt.outer_sigma(src), 1.0)
+    Nx, Ny = klt.tcc_sizing(0.8, 2048, 2048, sig)
+    tcc = ctx.tcc_build(src, sig, 0.8, 0.2, 2048, 2048, Nx, Ny, precision=0)
+    kr, ki, sc = ctx.calc_kernels(tcc, 3)
+    out["sample"] = _wide_row(ctx, timer, torch, dev, kr, ki, sc, Nx, 2048, 32, 8)
+    src = annular_source()
+    sig = min(klt.outer_sigma(src), 1.0)
+    Nx, Ny = klt.tcc_sizing(W.NA, 2400, 2400, sig)
+    tcc = ctx.tcc_build(src, sig, W.NA, 0.0, 2400, 2400, Nx, Ny, precision=0, nmed=NMED)
+    kr, ki, sc = ctx.calc_kernels(tcc, 24)
+    out["n31"] = _wide_row(ctx, timer, torch, dev, kr, ki, sc, Nx, 1024, 128, 128)
+    return out
+
+
 def copy_calibration(torch, timer, dev):
     """A torch device-to-device copy of 1 GiB (the runtime's copy kernel), whose
     known bytes calibrate the FETCH_SIZE / WRITE_SIZE counters."""
@@ -656,6 +722,13 @@ def main():
             f"192 images; grouped calls {bu['ms_per_batch_grouped_calls']:.3f} ms), p50 "
             f"{bu['p50_image_latency_us']:.0f} us, p99 {bu['p99_image_latency_us']:.0f} us")
         extras["batch_uneven"] = bu
+        wd = bench_wide(ctx, timer, torch, dev)
+        log(f"wide path: reference sample N={wd['sample']['N']} K={wd['sample']['K']} "
+            f"2048^2: {wd['sample']['images_per_s']:.0f} images/s, "
+            f"{wd['sample']['mask_grads_per_s']:.0f} mask grads/s; N={wd['n31']['N']} K=24 "
+            f"1024^2: {wd['n31']['images_per_s']:.0f} images/s, "
+            f"{wd['n31']['mask_grads_per_s']:.0f} mask grads/s")
+        extras["wide"] = wd
     if args.copy_calibration:
         extras["copy_calibration"] = copy_calibration(torch, timer, dev)
 

@@ -1732,11 +1732,14 @@ constexpr int kWOS = 132;  // LDS row stride (floats) of the block's output tile
 
 // Prologue of the wide synthesis passes: the wave's P operand (its 32
 // columns x0 + 32 wave .., K = j) in pbh / pbl, through smem (at least
-// kWideProBytes); returns 1 / pscale.
+// kWideProBytes); returns 1 / pscale.  Only the 32-deep k steps that hold
+// some gx (k < 2 (4Nx+1)) and the 16-wide j tiles that hold some gy
+// (j < 2 (2Ny+1)) are formed; the rest of P stays zero, as the spectrum is
+// there.
 constexpr int kWideProBytes = 2 * kColsSynth * kWPS * 2;  // >= a quarter of B (32 KB)
 
 __device__ __forceinline__ float wide_p_operand(int x0, const float2* __restrict__ spec, int Nx,
-                                                const float4* __restrict__ Efw, char* smem,
+                                                int Ny, const float4* __restrict__ Efw, char* smem,
                                                 float* red, half8_t (&pbh)[2][4],
                                                 half8_t (&pbl)[2][4]) {
   static_assert(kWideProBytes >= 2 * 16 * 64 * 16, "a quarter of B fits");
@@ -1761,6 +1764,7 @@ __device__ __forceinline__ float wide_p_operand(int x0, const float2* __restrict
   int ex = 0;
   (void)frexpf(bound > 0.f ? bound : 1.f, &ex);
   const float pscale = ldexpf(1.0f, 12 - ex);
+  const int nst = min(8, (2 * Gx + 31) / 32), nnt = min(8, (4 * Ny + 2 + 15) / 16);
   // --- P = E . B, B staged a quarter (128 k x 64 j) at a time ----------------
   // B[2gxi][2gy] = Jr, B[2gxi+1][2gy] = -Ji, B[2gxi][2gy+1] = Ji,
   // B[2gxi+1][2gy+1] = Jr; fragment (st, nt, lane) of a quarter holds rows
@@ -1775,10 +1779,12 @@ __device__ __forceinline__ float wide_p_operand(int x0, const float2* __restrict
   for (int kh = 0; kh < 2; ++kh)
 #pragma unroll
     for (int jh = 0; jh < 2; ++jh) {
-      {
+      if (4 * kh >= nst || 4 * jh >= nnt) continue;  // (uniform over the block)
+      if (4 * jh + wave < nnt) {
         const int j = 64 * jh + 16 * wave + r16, gy = j >> 1, d = j & 1;
 #pragma unroll
         for (int st = 0; st < 4; ++st) {
+          if (4 * kh + st >= nst) break;
           float v[8];
 #pragma unroll
           for (int t = 0; t < 8; ++t) {
@@ -1795,6 +1801,7 @@ __device__ __forceinline__ float wide_p_operand(int x0, const float2* __restrict
       __syncthreads();
 #pragma unroll
       for (int st = 0; st < 4; ++st) {
+        if (4 * kh + st >= nst) break;
         half8_t ah[2], al[2];
 #pragma unroll
         for (int m = 0; m < 2; ++m) {
@@ -1803,6 +1810,7 @@ __device__ __forceinline__ float wide_p_operand(int x0, const float2* __restrict
         }
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) {
+          if (4 * jh + nt >= nnt) break;
           const half8_t bh = sBh[(st * 4 + nt) * 64 + lane], bl = sBl[(st * 4 + nt) * 64 + lane];
 #pragma unroll
           for (int m = 0; m < 2; ++m)
@@ -1842,7 +1850,7 @@ __device__ __forceinline__ float wide_p_operand(int x0, const float2* __restrict
 }
 
 __device__ __forceinline__ void synth_wide_strip(int x0, int y0, int y1,
-                                                 const float2* __restrict__ spec, int Nx,
+                                                 const float2* __restrict__ spec, int Nx, int Ny,
                                                  int Lx, int Ly, const float4* __restrict__ Efw,
                                                  const float4* __restrict__ TYw,
                                                  float* __restrict__ img) {
@@ -1855,7 +1863,9 @@ __device__ __forceinline__ void synth_wide_strip(int x0, int y0, int y1,
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int r16 = lane & 15, q4 = (lane >> 4) * 4;
   half8_t pbh[2][4], pbl[2][4];
-  const float inv_pscale = wide_p_operand(x0, spec, Nx, Efw, smem, red, pbh, pbl);
+  const float inv_pscale = wide_p_operand(x0, spec, Nx, Ny, Efw, smem, red, pbh, pbl);
+  // 32-deep steps of j that hold some gy (the rest of TY and P is zero)
+  const int nks = min(4, (4 * Ny + 2 + 31) / 32);
   // --- rows -----------------------------------------------------------------
   float4* sTY = reinterpret_cast<float4*>(smem);                 // [2][TY_F4]
   float* sO = reinterpret_cast<float*>(sTY + 2 * TY_F4);          // [2][16][kWOS]
@@ -1879,6 +1889,7 @@ __device__ __forceinline__ void synth_wide_strip(int x0, int y0, int y1,
     for (int cb = 0; cb < 2; ++cb) d[cb] = float4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
+      if (ks >= nks) break;
       const half8_t ah = __builtin_bit_cast(half8_t, ty[(ks * 2 + 0) * 64]);
       const half8_t al = __builtin_bit_cast(half8_t, ty[(ks * 2 + 1) * 64]);
 #pragma unroll
@@ -1926,16 +1937,16 @@ __device__ __forceinline__ void synth_wide_strip(int x0, int y0, int y1,
   }
 }
 
-__global__ __launch_bounds__(256, 3) void k_synth_wide(const float2* __restrict__ spec, int Nx,
-                                                    int Lx, int Ly, int Lypad, int rs,
+__global__ __launch_bounds__(256, 4) void k_synth_wide(const float2* __restrict__ spec, int Nx,
+                                                    int Ny, int Lx, int Ly, int Lypad, int rs,
                                                     const float4* __restrict__ Efw,
                                                     const float4* __restrict__ TYw,
                                                     float* __restrict__ image) {
   const int b = blockIdx.y, strip = blockIdx.x / rs, rh = blockIdx.x - strip * rs;
   const int per = (Lypad / 32 + rs - 1) / rs * 32;
   const int y0 = min(Lypad, rh * per), y1 = min(Lypad, y0 + per);
-  synth_wide_strip(strip * kColsSynth, y0, y1, spec + (size_t)b * 64 * (4 * Nx + 1), Nx, Lx,
-                   Ly, Efw, TYw, image + (size_t)b * Lx * Ly);
+  synth_wide_strip(strip * kColsSynth, y0, y1, spec + (size_t)b * 64 * (4 * Nx + 1), Nx, Ny,
+                   Lx, Ly, Efw, TYw, image + (size_t)b * Lx * Ly);
 }
 
 // ---------------------------------------------------------------------------
@@ -1981,7 +1992,9 @@ __device__ __forceinline__ void synth_resist_wide_strip(
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int r16 = lane & 15, kq = (lane >> 4) * 8, q4 = (lane >> 4) * 4;
   half8_t pbh[2][4], pbl[2][4];
-  const float inv_pscale = wide_p_operand(x0, spec, Nx, Efw, smem, red, pbh, pbl);
+  const float inv_pscale = wide_p_operand(x0, spec, Nx, Ny, Efw, smem, red, pbh, pbl);
+  // 32-deep steps of j that hold some gy (the rest of TY and P is zero)
+  const int nks = min(4, (4 * Ny + 2 + 31) / 32);
   float4* sTY = reinterpret_cast<float4*>(smem);  // [2][TY_F4]
   float4* sTG = sTY + 2 * TY_F4;                  // [2][TG_F4]
   float* stg = reinterpret_cast<float*>(sTG + 2 * TG_F4) + wave * 16 * kStg;
@@ -2023,6 +2036,9 @@ __device__ __forceinline__ void synth_resist_wide_strip(
   // s = 1 / (1 + 2^(c1 d + c0)) for the unscaled synthesis output d (I = d / pscale)
   const float c1 = -a_sig * 1.44269504f * inv_pscale, c0 = a_sig * thr * 1.44269504f;
   const float two_a = 2.f * a_sig;
+  // j tiles of Q and C that hold some gy <= 2Ny; k tiles of C that hold some
+  // gx (k < 2 (4Nx+1)): the others stay zero and are never written out
+  const int njt = min(8, (4 * Ny + 2 + 15) / 16), nkt = min(16, (2 * (4 * Nx + 1) + 15) / 16);
   auto step = [&](int yb, int sl, float4& tc0, float4& tc1, float4& a0, float4& a1, float4& b0,
                   float4& b1) {
     if (yb + 16 < y1) {
@@ -2038,6 +2054,7 @@ __device__ __forceinline__ void synth_resist_wide_strip(
     for (int cb = 0; cb < 2; ++cb) d[cb] = float4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
+      if (ks >= nks) break;
       const half8_t ah = __builtin_bit_cast(half8_t, ty[(ks * 2 + 0) * 64]);
       const half8_t al = __builtin_bit_cast(half8_t, ty[(ks * 2 + 1) * 64]);
 #pragma unroll
@@ -2100,6 +2117,7 @@ __device__ __forceinline__ void synth_resist_wide_strip(
     const half4_t* tg = reinterpret_cast<const half4_t*>(sTG + sl * TG_F4) + lane;
 #pragma unroll
     for (int jt = 0; jt < 8; ++jt) {
+      if (jt >= njt) break;
       const half4_t gah = tg[(jt * 2 + 0) * 64], gal = tg[(jt * 2 + 1) * 64];
 #pragma unroll
       for (int cb = 0; cb < 2; ++cb) Q[jt][cb] = mfma3k16(gah, gal, gh[cb], gl[cb], Q[jt][cb]);
@@ -2167,6 +2185,7 @@ __device__ __forceinline__ void synth_resist_wide_strip(
     }
 #pragma unroll
     for (int jt = 0; jt < 8; ++jt) {
+      if (jt >= njt) break;
       const float* row = sQ + (16 * jt + r16) * kQW + 32 * st + kq;
       const float4 u0 = *reinterpret_cast<const float4*>(row);
       const float4 u1 = *reinterpret_cast<const float4*>(row + 4);
@@ -2176,7 +2195,8 @@ __device__ __forceinline__ void synth_resist_wide_strip(
       half8_t ah, al;
       split8(v, ah, al);
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk) C[jt][kk] = mfma3(ah, al, bh[kk], bl[kk], C[jt][kk]);
+      for (int kk = 0; kk < 4; ++kk)
+        if (4 * wave + kk < nkt) C[jt][kk] = mfma3(ah, al, bh[kk], bl[kk], C[jt][kk]);
     }
   }
   // lane l holds C[16 jt + 4 (l/16) + r][16 kt + l % 16]: rows 2a, 2a + 1
@@ -3453,7 +3473,7 @@ int socs_forward_wide(klt_ctx* ctx, const klt_kernels* kb, const float* masks, i
                    ctx->spec.as<float2>());
       }
       KLT_LAUNCH(ctx, "k_synth_wide", k_synth_wide, dim3(nstrips * c.rs, nb), dim3(256), 0,
-                 ctx->spec.as<float2>(), Nx, Lx, Ly, ft->Lypad, c.rs, ft->Efw.as<float4>(),
+                 ctx->spec.as<float2>(), Nx, Ny, Lx, Ly, ft->Lypad, c.rs, ft->Efw.as<float4>(),
                  ft->TYw.as<float4>(), out);
       KLT_HIP(hipGetLastError());
       return KLT_OK;
@@ -3938,7 +3958,7 @@ int socs_backward_wide(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
                    ctx->gam_part.as<float2>(), c.kg, Nx, Ny, dcx, dcy, Lx, Ly,
                    ctx->gspec.as<float2>(), 64);
       KLT_LAUNCH(ctx, "k_synth_wide", k_synth_wide, dim3(nstrips * c.rs, nb), dim3(256), 0,
-                 ctx->gspec.as<float2>(), Nx, Lx, Ly, ft->Lypad, c.rs, ft->Efw.as<float4>(),
+                 ctx->gspec.as<float2>(), Nx, Ny, Lx, Ly, ft->Lypad, c.rs, ft->Efw.as<float4>(),
                  ft->TYw.as<float4>(), grads + off);
       KLT_HIP(hipGetLastError());
       return KLT_OK;
