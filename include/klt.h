@@ -212,11 +212,11 @@ void klt_kernels_destroy(klt_kernels* kb);
  * klt_socs_image is the plain single-image path: FP32 arithmetic throughout. */
 int klt_socs_image(klt_ctx* ctx, const klt_kernels* kb, const float* mask, int Lx,
                    int Ly, float* image, unsigned flags);
-/* nmask masks of one size, stored back to back.  For N <= 15 the full-field
- * DFT passes run as split hi+lo FP16 GEMMs on the matrix cores with FP32
- * accumulation (image relative L2 error ~1e-6 against FP64); the coarse-grid
- * FFTs are FP32.  Kernel half-sizes 16 .. 31 take the FP32 path of
- * klt_socs_image; above 31 the call fails with KLT_ERR_ARG. */
+/* nmask masks of one size, stored back to back.  For every N <= 31 the
+ * full-field DFT passes run as split hi+lo FP16 GEMMs on the matrix cores with
+ * FP32 accumulation (image relative L2 error ~1e-6 against FP64); the
+ * coarse-grid FFTs are FP32, on a 64-point grid for N <= 15 and a 128-point
+ * grid for N 16 .. 31.  Above 31 the call fails with KLT_ERR_ARG. */
 int klt_socs_forward(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
                      int nmask, int Lx, int Ly, float* images, unsigned flags);
 
@@ -275,7 +275,7 @@ int klt_resist_levels(klt_ctx* ctx, const float* images, int nimg, int Lx, int L
 /* ---- adjoint (no reference counterpart) ---------------------------------
  * Resist loss of each SOCS image, L = sum_x (sigmoid(a (I(x) - thr)) - T(x))^2,
  * and its gradient with respect to the mask, dL/dm (Ly x Lx per mask), through
- * the same band-limited passes as klt_socs_forward (kernel half-size <= 15).
+ * the same band-limited passes as klt_socs_forward (kernel half-size <= 31).
  * masks (continuous transmittance allowed), targets, grads: nmask x Ly x Lx;
  * losses: nmask; images: optional (NULL) output of the forward images. */
 int klt_socs_backward(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
