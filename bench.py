@@ -592,12 +592,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline and plain path only")
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--section", choices=["all", "eig_psd", "tcc_split16", "adjoint"],
+    ap.add_argument("--section", choices=["all", "eig_psd", "tcc_split16", "adjoint", "wide"],
                     default="all",
                     help="eig_psd: only --steps top-32 solves of the T = 1225 blurred-dipole "
                          "TCC on the positive-semidefinite path; tcc_split16: only --steps "
                          "split16 builds of that TCC (no defocus); adjoint: only --steps "
-                         "klt_socs_backward calls on --batch 1024^2 masks; their timing as a "
+                         "klt_socs_backward calls on --batch 1024^2 masks; wide: only the "
+                         "kernel half-size 16 and 31 rows of the wide path; their timing as a "
                          "JSON line (for profiling; no bench line)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -619,6 +620,9 @@ def main():
         if args.copy_calibration:
             res["copy_calibration"] = copy_calibration(torch, timer, dev)
         print(json.dumps(res))
+        return
+    if args.section == "wide":
+        print(json.dumps(bench_wide(ctx, timer, torch, dev)))
         return
     if args.section != "all":
         fn = section_eig_psd if args.section == "eig_psd" else section_tcc_split16
