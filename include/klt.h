@@ -287,8 +287,9 @@ int klt_socs_backward(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
  * (equal to the Hopkins image of its TCC), dL/dS_s = sum_x G(x) |E_s(x)|^2,
  * G = dL/dI_A, for every grid point inside the unit disc (zero elsewhere),
  * with respect to the weights as given.  The optics are those of
- * klt_tcc_build (FP64 Gram).  grad_src: nmask x srcSize^2; losses: nmask
- * (loss of I_A); images: optional I_A (device pointers only, else NULL). */
+ * klt_tcc_build (FP64 Gram).  Kernel half-size <= 31.  grad_src: nmask x
+ * srcSize^2; losses: nmask (loss of I_A); images: optional I_A (device
+ * pointers only, else NULL). */
 int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets, int nmask,
                          int Lx, int Ly, const float* src, int srcSize, float lambda, float NA,
                          float defocus, float nmed, int Nx, int Ny, float a, float thr,

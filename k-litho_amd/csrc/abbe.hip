@@ -27,6 +27,10 @@
 //      = Lx Ly / 64^2 sum_u g_c(u) |e_s(u)|^2 with e_s, g_c the 64-point
 //      transforms of A_s and G^, one 2D transform per source point (two
 //      points per pass, the pupil values from a table built once per call).
+// Kernel half-sizes 16..31 take the wide passes of socs_fused.hip: the band
+// with 64 twiddle rows, the Hopkins image formed tile by tile inside the fused
+// resist band (k_synth_resist_wide, stored only when asked for), and steps 3-4
+// on the 128 x 128 coarse grid (Lx Ly / 128^2 sum_u g_c |e_s|^2).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -77,8 +81,10 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
     return klt::set_error(KLT_ERR_ARG, "klt_abbe_source_grad: NULL argument");
   if (nmask <= 0 || Lx <= 0 || Ly <= 0 || srcSize <= 0 || Nx < 0 || Ny < 0)
     return klt::set_error(KLT_ERR_ARG, "klt_abbe_source_grad: bad size");
-  if (Nx > 15 || Ny > 15)
-    return klt::set_error(KLT_ERR_ARG, "klt_abbe_source_grad: half-size N above 15");
+  if (Nx > 31 || Ny > 31)
+    return klt::set_error(KLT_ERR_ARG, "klt_abbe_source_grad: half-size N above 31");
+  // half-sizes 16..31: the 128-point coarse grid and the wide passes
+  const bool wide = Nx > 15 || Ny > 15;
   if (4 * Nx + 1 > Lx || 4 * Ny + 1 > Ly) return klt::set_error(KLT_ERR_SIZE, "ERROR in FI()");
   if (NA < 0 || !(nmed >= 1.0f) || NA > nmed)
     return klt::set_error(KLT_ERR_ARG, "klt_abbe_source_grad: NA must not exceed the medium index");
@@ -122,21 +128,27 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
     dl = dg + (size_t)nmask * ns2 + 64;
     di = nullptr;
   }
+  // (the wide passes form the image inside the fused resist band and write it
+  // only when the caller asks for it)
   float* I = di;
-  if (!I) {
+  if (!I && (!wide || images)) {
     KLT_TRY(ctx->bwd_img.ensure(bytes));
     I = ctx->bwd_img.as<float>();
   }
   // 1. mask bands (matrix-core band pass)
-  KLT_TRY(ctx->qpart.ensure((size_t)nmask * nblk * Wy * 16 * sizeof(float2)));
-  KLT_TRY(ctx->gpart.ensure((size_t)nmask * nblk * Gy * 32 * sizeof(float2)));
-  KLT_TRY(ctx->lpart.ensure((size_t)nmask * nblk * sizeof(float)));
   KLT_TRY(ctx->abbe_band.ensure((size_t)nmask * T * sizeof(float2)));
   float2* Mfull = ctx->abbe_band.as<float2>();
   const float inv_area = 1.0f / ((float)Lx * (float)Ly);
-  KLT_TRY(klt::band_mask_partials(ctx, Lx, Ly, Nx, Ny, dm, nmask, ks, ctx->qpart.as<float2>()));
-  KLT_LAUNCH(ctx, "k_abbe_assemble", k_abbe_assemble, dim3(nmask), dim3(kThreads), 0,
-             ctx->qpart.as<float2>(), nblk, Nx, Ny, inv_area, Mfull);
+  if (wide) {
+    KLT_TRY(klt::band_mask_full_wide(ctx, Lx, Ly, Nx, Ny, dm, nmask, Mfull));
+  } else {
+    KLT_TRY(ctx->qpart.ensure((size_t)nmask * nblk * Wy * 16 * sizeof(float2)));
+    KLT_TRY(ctx->gpart.ensure((size_t)nmask * nblk * Gy * 32 * sizeof(float2)));
+    KLT_TRY(ctx->lpart.ensure((size_t)nmask * nblk * sizeof(float)));
+    KLT_TRY(klt::band_mask_partials(ctx, Lx, Ly, Nx, Ny, dm, nmask, ks, ctx->qpart.as<float2>()));
+    KLT_LAUNCH(ctx, "k_abbe_assemble", k_abbe_assemble, dim3(nmask), dim3(kThreads), 0,
+               ctx->qpart.as<float2>(), nblk, Nx, Ny, inv_area, Mfull);
+  }
   // 2. TCC of the source, then the Hopkins images from the bands
   if (src_read) {
     const hipError_t e = hipEventSynchronize(src_read);
@@ -150,10 +162,18 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
   double* tcc = ctx->abbe_tcc.as<double>();
   KLT_TRY(klt::tcc_build_device(ctx, hsrc.data(), srcSize, sigma, lambda, NA, defocus, nmed, Lx,
                                 Ly, Nx, Ny, KLT_TCC_SPLIT_F16, tcc));
-  KLT_TRY(klt::hopkins_images_fused(ctx, Mfull, nmask, Lx, Ly, tcc, Nx, Ny, I));
-  // 3. resist bands and loss partials
-  KLT_TRY(klt::band_resist_partials(ctx, Lx, Ly, Nx, Ny, I, dt, nmask, ks, a_sig, thr,
-                                    ctx->gpart.as<float2>(), ctx->lpart.as<float>()));
+  const int M = wide ? 128 : 64;  // coarse grid
+  KLT_TRY(ctx->gcoarse.ensure((size_t)nmask * M * M * sizeof(float)));
+  if (wide) {
+    // 2b-3. the Hopkins image tile by tile, its resist band, the losses and g_c
+    KLT_TRY(klt::abbe_resist_wide(ctx, Mfull, nmask, Lx, Ly, tcc, Nx, Ny, dt, a_sig, thr, I, dl,
+                                  ctx->gcoarse.as<float>()));
+  } else {
+    KLT_TRY(klt::hopkins_images_fused(ctx, Mfull, nmask, Lx, Ly, tcc, Nx, Ny, I));
+    // 3. resist bands and loss partials
+    KLT_TRY(klt::band_resist_partials(ctx, Lx, Ly, Nx, Ny, I, dt, nmask, ks, a_sig, thr,
+                                      ctx->gpart.as<float2>(), ctx->lpart.as<float>()));
+  }
   // 4. per source point
   const int sh = (int)((srcSize - 1) / 2.0);
   std::vector<int2> pts;
@@ -166,16 +186,19 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
   KLT_LAUNCH(ctx, "k_fill_zero", k_fill_zero, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0,
              dg, ng);
   const klt::PupilGeom g = klt::make_geom(srcSize, lambda, NA, defocus, nmed, Lx, Ly, Nx, Ny);
-  KLT_TRY(ctx->gcoarse.ensure((size_t)nmask * 64 * 64 * sizeof(float)));
-  KLT_TRY(klt::resist_coarse_grid(ctx, ctx->gpart.as<float2>(), ctx->lpart.as<float>(), nblk,
-                                  nmask, Nx, Ny, inv_area, dl, ctx->gcoarse.as<float>()));
-  KLT_TRY(klt::abbe_grad_coarse(ctx, Mfull, ctx->gcoarse.as<float>(), nmask, Lx, Ly, g,
-                                ctx->misc.as<int2>(),
-                                (int)pts.size(), srcSize, dg));
+  if (wide) {
+    KLT_TRY(klt::abbe_grad_coarse128(ctx, Mfull, ctx->gcoarse.as<float>(), nmask, Lx, Ly, g,
+                                     ctx->misc.as<int2>(), (int)pts.size(), srcSize, dg));
+  } else {
+    KLT_TRY(klt::resist_coarse_grid(ctx, ctx->gpart.as<float2>(), ctx->lpart.as<float>(), nblk,
+                                    nmask, Nx, Ny, inv_area, dl, ctx->gcoarse.as<float>()));
+    KLT_TRY(klt::abbe_grad_coarse(ctx, Mfull, ctx->gcoarse.as<float>(), nmask, Lx, Ly, g,
+                                  ctx->misc.as<int2>(), (int)pts.size(), srcSize, dg));
+  }
   if (flags & KLT_HOST_PTRS) {
     KLT_HIP(hipMemcpyAsync(grad_src, dg, ng * 4, hipMemcpyDeviceToHost, st));
     KLT_HIP(hipMemcpyAsync(losses, dl, (size_t)nmask * 4, hipMemcpyDeviceToHost, st));
-    if (images) KLT_HIP(hipMemcpyAsync(images, I, bytes, hipMemcpyDeviceToHost, st));
+    if (images && I) KLT_HIP(hipMemcpyAsync(images, I, bytes, hipMemcpyDeviceToHost, st));
   }
   if (flags & KLT_HOST_PTRS) KLT_HIP(hipStreamSynchronize(st));  // the host outputs are complete
   return KLT_OK;

@@ -304,6 +304,20 @@ int hopkins_image_device(klt_ctx* ctx, const float* mask, int Lx, int Ly, const 
 // (normalised, centred): spec [B][rows][4Nx+1] (socs.hip).
 int hopkins_spec_device(klt_ctx* ctx, const float2* band, int B, const double* tcc, int Nx,
                         int Ny, int rows, float2* spec);
+// The source gradient's stages for kernel half-sizes 16..31 (socs_fused.hip):
+// the normalised bands Mfull [B][(2Ny+1)(2Nx+1)] of B masks; from the
+// Hopkins spectra of a device TCC, the resist loss per mask, the 128-point
+// coarse-grid g_c [B][256][64] (thread-major) and, with images != NULL, the
+// Hopkins images; the gradient per source point on the 128-point grid.
+int band_mask_full_wide(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const float* masks, int B,
+                        float2* Mfull);
+int abbe_resist_wide(klt_ctx* ctx, const float2* Mfull, int B, int Lx, int Ly, const double* tcc,
+                     int Nx, int Ny, const float* targets, float a_sig, float thr, float* images,
+                     float* loss, float* gc);
+struct PupilGeom;
+int abbe_grad_coarse128(klt_ctx* ctx, const float2* Mfull, const float* gc, int B, int Lx,
+                        int Ly, const PupilGeom& geom, const int2* pts, int npts, int srcSize,
+                        float* grad);
 // Hopkins images of B masks of Lx x Ly from their bands, through the batched
 // matrix-core synthesis (kernel half-size <= 15; socs_fused.hip).
 int hopkins_images_fused(klt_ctx* ctx, const float2* band, int B, int Lx, int Ly,

@@ -2459,6 +2459,106 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd128(
   gam_to_spec(sGam, Nx, Ny, dcx, dcy, Lx, Ly, gspec + (size_t)b * 64 * (4 * Nx + 1), 64);
 }
 
+// grid (ngrp, B), dynamic LDS (96 kRS + T) float2: the source gradient of
+// half-sizes 16..31, as k_abbe_grad_coarse on the 128-point grid.  Block (j,
+// b) takes the j-th of ngrp groups of the source points pq[0, npts) for mask b;
+// per point, A_s = P(-n, s) M^ (pupil values from ptab) through the row and
+// column transforms of k_socs_mid128, then sum_u g_c(u) |e_s(u)|^2 with g_c
+// from k_resist_coarse128 (same thread layout), products in FP32 and sums in
+// FP64 (g_c takes both signs and the sum cancels strongly), reduced over the
+// block; grad[b][point] = scale times that sum (scale = Lx Ly / 128^2).
+__global__ __launch_bounds__(256) void k_abbe_grad_coarse128(
+    const float2* __restrict__ Mfull, const float* __restrict__ gcin, klt::PupilGeom pg,
+    const int2* __restrict__ pq, const float2* __restrict__ ptab, int npts, int srcSize,
+    double scale, const float2* __restrict__ tw128, float* __restrict__ grad) {
+  extern __shared__ float4 smem4[];
+  float2* sB = reinterpret_cast<float2*>(smem4);  // [64][kRS]
+  float2* sX = sB + 64 * kRS;                     // [32][kRS]
+  float2* sM = sX + 32 * kRS;                     // [T]
+  __shared__ double red[4];
+  const int b = blockIdx.y, grp = blockIdx.x, ngrp = gridDim.x, tid = threadIdx.x;
+  const int per = (npts + ngrp - 1) / ngrp, s0 = min(npts, grp * per), s1 = min(npts, s0 + per);
+  const int g = tid >> 3, t = tid & 7, wave = tid >> 6, lane = tid & 63;
+  const int Nx = pg.Nx, Ny = pg.Ny, Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
+  for (int i = tid; i < T; i += 256) sM[i] = Mfull[(size_t)b * T + i];
+  float2 twi[16];
+#pragma unroll
+  for (int k2 = 0; k2 < 16; ++k2) twi[k2] = tw128[(t * k2) & 127];
+  float gc[4][16];
+  {
+    const float4* src = reinterpret_cast<const float4*>(gcin + (size_t)b * 16384 + tid * 64);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float4 u = src[q * 4 + i];
+        gc[q][4 * i] = u.x;
+        gc[q][4 * i + 1] = u.y;
+        gc[q][4 * i + 2] = u.z;
+        gc[q][4 * i + 3] = u.w;
+      }
+  }
+  __syncthreads();  // sM is complete before the points' rows
+  for (int s = s0; s < s1; ++s) {
+    const int2 pa = pq[s];
+#pragma unroll
+    for (int rr = 0; rr < 2; ++rr) {
+      const int r = g + 32 * rr;
+      const bool act = r < Wy;
+      float2 v[16];
+#pragma unroll
+      for (int n2 = 0; n2 < 16; ++n2) v[n2] = make_float2(0.f, 0.f);
+      if (act) {
+        // P(-n, s): entry T - 1 - (index of n)
+        const float2* Pa = ptab + (size_t)s * T + (T - 1 - r * Wx - Nx);
+#pragma unroll
+        for (int n2 = 0; n2 < 16; ++n2) {
+          if (n2 >= 4 && n2 < 12) continue;
+          const int n = t + 8 * n2;
+          const int nx = n < 64 ? n : n - 128;
+          if (nx >= -Nx && nx <= Nx) v[n2] = cmulf(Pa[-nx], sM[r * Wx + nx + Nx]);
+        }
+      }
+      float2* xr = sB + (act ? r : 0) * kRS;
+      fft128_group<1>(v, twi, xr, t, act);
+      if (act) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int q1 = 0; q1 < 8; ++q1) xr[t + 8 * h + 16 * q1] = v[8 * h + q1];
+      }
+    }
+    __syncthreads();
+    double part = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int ux = g + 32 * q;
+      float2 v[16];
+#pragma unroll
+      for (int n2 = 0; n2 < 16; ++n2) {
+        v[n2] = make_float2(0.f, 0.f);
+        if (n2 >= 4 && n2 < 12) continue;
+        const int n = t + 8 * n2;
+        const int ny = n < 64 ? n : n - 128;
+        if (ny >= -Ny && ny <= Ny) v[n2] = sB[(ny + Ny) * kRS + ux];
+      }
+      fft128_group<1>(v, twi, sX + g * kRS, t, true);
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        part += (double)(gc[q][i] * (v[i].x * v[i].x + v[i].y * v[i].y));
+    }
+    part = wave_sum_d(part);
+    if (lane == 0) red[wave] = part;
+    // (also ends the column reads of sB before the next point's rows)
+    __syncthreads();
+    if (tid == 0) {
+      const double tot = red[0] + red[1] + red[2] + red[3];
+      grad[(size_t)b * srcSize * srcSize + (size_t)(pa.y + pg.sh) * srcSize + pa.x + pg.sh] =
+          (float)(scale * tot);
+    }
+  }
+}
+
 // Mixed sizes and kernel sets: one block per SynthJob (one strip of one
 // image).  The block that completes an image's last strip records the
 // device wall clock in done_at[image].
@@ -3708,6 +3808,92 @@ int abbe_grad_coarse(klt_ctx* ctx, const float2* Mfull, const float* gc, int B, 
   KLT_LAUNCH(ctx, "k_abbe_grad_coarse", k_abbe_grad_coarse, dim3(ngrp, B), dim3(256), 0, Mfull,
              gc, geom, pts, ctx->abbe_pupil.as<float2>(), npts, srcSize,
              (double)Lx * (double)Ly / (double)(kM * kM), ctx->tw64.as<float2>(), grad);
+  KLT_HIP(hipGetLastError());
+  return KLT_OK;
+}
+
+// The source gradient's stages for kernel half-sizes 16..31 (abbe.hip):
+// each mask's normalised band M^ (wide band pass + k_band_assemble); from
+// the Hopkins spectra of the source's TCC, the fused wide synthesis + resist
+// band (the image is written only when `images` is given), the losses and
+// the 128-point g_c; the per-point sums on the 128-point grid.
+int band_mask_full_wide(klt_ctx* ctx, int Lx, int Ly, int Nx, int Ny, const float* masks, int B,
+                        float2* Mfull) {
+  SocsPlan* p = nullptr;
+  KLT_TRY(get_socs_plan(ctx, Lx, Ly, Nx, Ny, &p));
+  FusedTables* ft = nullptr;
+  KLT_TRY(get_fused_tables(ctx, p, &ft));
+  const int Wy = 2 * Ny + 1, T = (2 * Nx + 1) * Wy;
+  const int ks = band_split(Lx, Ly, B), parts = (Ly + kRowsBand - 1) / kRowsBand * ks;
+  KLT_TRY(ctx->qpart.ensure((size_t)B * parts * Wy * 32 * sizeof(float2)));
+  launch_mask_band_wide(ctx, kRowsBand, ks, B, masks, Lx, Ly, ft->Kpad, ft->Wf.as<float4>(),
+                        p->twy_in.as<float2>(), p->NP, Nx, Ny, ctx->qpart.as<float2>());
+  KLT_LAUNCH(ctx, "k_band_assemble", k_band_assemble, dim3((unsigned)(T + 255) / 256, B),
+             dim3(256), 0, ctx->qpart.as<float2>(), parts, Nx, Ny, 32,
+             1.0f / ((float)Lx * (float)Ly), Mfull);
+  KLT_HIP(hipGetLastError());
+  return KLT_OK;
+}
+
+int abbe_resist_wide(klt_ctx* ctx, const float2* Mfull, int B, int Lx, int Ly, const double* tcc,
+                     int Nx, int Ny, const float* targets, float a_sig, float thr, float* images,
+                     float* loss, float* gc) {
+  SocsPlan* p = nullptr;
+  KLT_TRY(get_socs_plan(ctx, Lx, Ly, Nx, Ny, &p));
+  FusedTables* ft = nullptr;
+  KLT_TRY(get_fused_tables(ctx, p, &ft));
+  const int Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
+  KLT_TRY(ctx->hspec.ensure((size_t)B * 64 * Gx * sizeof(float2)));
+  KLT_TRY(hopkins_spec_device(ctx, Mfull, B, tcc, Nx, Ny, 64, ctx->hspec.as<float2>()));
+  // row ranges per strip by the fixed rule of the batched passes
+  const int nstrips = (Lx + kColsSynth - 1) / kColsSynth, nrow32 = ft->Lypad / 32;
+  int rs = 1;
+  while (rs < 4 && nstrips * rs * B < 512 && split_ok(nrow32, 2 * rs)) rs *= 2;
+  const int sparts = nstrips * rs;
+  KLT_TRY(ctx->gpart.ensure((size_t)B * sparts * Gy * 64 * sizeof(float2)));
+  KLT_TRY(ctx->lpart.ensure((size_t)B * sparts * sizeof(float)));
+  KLT_TRY(ctx->gfull.ensure((size_t)B * Gx * Gy * sizeof(float2)));
+  const bool tvec = Lx % 4 == 0 && (reinterpret_cast<uintptr_t>(targets) & 15) == 0;
+  if (tvec)
+    KLT_LAUNCH(ctx, "k_synth_resist_wide", k_synth_resist_wide<true>, dim3(sparts, B), dim3(256),
+               0, ctx->hspec.as<float2>(), Nx, Ny, Lx, Ly, ft->Lypad, rs, ft->Efw.as<float4>(),
+               ft->TYw.as<float4>(), targets, a_sig, thr, ft->TYgw.as<float4>(),
+               ft->Ebw.as<float4>(), images, ctx->gpart.as<float2>(), ctx->lpart.as<float>());
+  else
+    KLT_LAUNCH(ctx, "k_synth_resist_wide", k_synth_resist_wide<false>, dim3(sparts, B),
+               dim3(256), 0, ctx->hspec.as<float2>(), Nx, Ny, Lx, Ly, ft->Lypad, rs,
+               ft->Efw.as<float4>(), ft->TYw.as<float4>(), targets, a_sig, thr,
+               ft->TYgw.as<float4>(), ft->Ebw.as<float4>(), images, ctx->gpart.as<float2>(),
+               ctx->lpart.as<float>());
+  KLT_LAUNCH(ctx, "k_resist_sum", k_resist_sum, dim3((unsigned)(Gx * Gy + 255) / 256, B),
+             dim3(256), 0, ctx->gpart.as<float2>(), ctx->lpart.as<float>(), sparts, Nx, Ny, 64,
+             1.0f / ((float)Lx * (float)Ly), ctx->gfull.as<float2>(), loss);
+  const int lds_fin = 96 * kRS * (int)sizeof(float2);
+  KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_resist_coarse128),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_fin));
+  KLT_LAUNCH(ctx, "k_resist_coarse128", k_resist_coarse128, dim3(B), dim3(256), lds_fin,
+             ctx->gfull.as<float2>(), Nx, Ny, ft->tw128.as<float2>(), gc);
+  KLT_HIP(hipGetLastError());
+  return KLT_OK;
+}
+
+int abbe_grad_coarse128(klt_ctx* ctx, const float2* Mfull, const float* gc, int B, int Lx,
+                        int Ly, const PupilGeom& geom, const int2* pts, int npts, int srcSize,
+                        float* grad) {
+  SocsPlan* p = nullptr;
+  KLT_TRY(get_socs_plan(ctx, Lx, Ly, geom.Nx, geom.Ny, &p));
+  FusedTables* ft = nullptr;
+  KLT_TRY(get_fused_tables(ctx, p, &ft));
+  KLT_TRY(ctx->abbe_pupil.ensure(sizeof(float2) * (size_t)npts * geom.T));
+  KLT_LAUNCH(ctx, "k_abbe_pupil", k_abbe_pupil, dim3((geom.T + 255) / 256, npts), dim3(256), 0,
+             geom, pts, ctx->abbe_pupil.as<float2>());
+  const int lds = (96 * kRS + geom.T) * (int)sizeof(float2);
+  KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_abbe_grad_coarse128),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  const int ngrp = std::max(1, std::min(npts, (512 + B - 1) / B));
+  KLT_LAUNCH(ctx, "k_abbe_grad_coarse128", k_abbe_grad_coarse128, dim3(ngrp, B), dim3(256), lds,
+             Mfull, gc, geom, pts, ctx->abbe_pupil.as<float2>(), npts, srcSize,
+             (double)Lx * (double)Ly / (128.0 * 128.0), ft->tw128.as<float2>(), grad);
   KLT_HIP(hipGetLastError());
   return KLT_OK;
 }
