@@ -94,9 +94,9 @@ int klt_ctx_set_splits(klt_ctx* ctx, int ks, int rs, int kg);
  * the fixed rule).  Like klt_ctx_set_splits, it has no reference counterpart:
  * it exposes a choice of the autotuner. */
 int klt_ctx_set_band_rows(klt_ctx* ctx, int rows);
-/* One line per tuned shape: "socs_forward|socs_backward B Lx Ly Nx Ny nk
- * column_splits row_splits kernel_groups band_rows ms"; returns the byte
- * count or an error. */
+/* One line per tuned shape: "socs_forward|socs_backward|socs_vjp B Lx Ly Nx
+ * Ny nk column_splits row_splits kernel_groups band_rows ms"; returns the
+ * byte count or an error. */
 int klt_ctx_tune_report(klt_ctx* ctx, char* buf, int len);
 
 /* ---- host boundary: sources, masks, sizing, file formats ---------------
@@ -281,6 +281,14 @@ int klt_resist_levels(klt_ctx* ctx, const float* images, int nimg, int Lx, int L
 int klt_socs_backward(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
                       const float* targets, int nmask, int Lx, int Ly, float a, float thr,
                       float* grads, float* losses, float* images, unsigned flags);
+
+/* No reference counterpart.  The vector-Jacobian product of the SOCS image
+ * for any loss: given dL/dI (nmask x Ly x Lx, I the klt_socs_forward images of
+ * masks), writes dL/dm = sum_x dL/dI(x) dI(x)/dm (nmask x Ly x Lx) through the
+ * adjoint passes of klt_socs_backward (kernel half-size <= 31; dL/dI's band
+ * with |g| <= 2N is all that enters).  The forward image is not formed. */
+int klt_socs_vjp(klt_ctx* ctx, const klt_kernels* kb, const float* masks, const float* dL_dI,
+                 int nmask, int Lx, int Ly, float* grads, unsigned flags);
 
 /* Gradient of the same loss with respect to the source, in the Abbe form:
  * with I_A(x) = sum_s S_s |E_s(x)|^2 the untruncated image of the source src

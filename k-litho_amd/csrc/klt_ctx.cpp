@@ -320,7 +320,8 @@ int klt_ctx_tune_report(klt_ctx* ctx, char* buf, int len) {
     char line[256];
     const auto& k = kv.first;
     snprintf(line, sizeof line, "%s %d %d %d %d %d %d %d %d %d %d %.6f\n",
-             k[0] == 0 ? "socs_forward" : "socs_backward", k[1], k[2], k[3], k[4], k[5], k[6],
+             k[0] == 0 ? "socs_forward" : k[0] == 1 ? "socs_backward" : "socs_vjp", k[1], k[2],
+             k[3], k[4], k[5], k[6],
              kv.second.ks, kv.second.rs, kv.second.kg, kv.second.rb, kv.second.ms);
     out += line;
   }

@@ -175,6 +175,7 @@ struct klt_ctx {
   // scratch reused across calls (grown on demand)
   klt::DevBuf stage_in, stage_out, qpart, band, spec, ccols, misc, eig_ws;
   klt::DevBuf gpart, gspec, lpart, bwd_img;  // adjoint scratch
+  klt::DevBuf vjp_sum;                        // klt_socs_vjp: the (unused) sums of G^2
   klt::DevBuf abbe_tcc, abbe_band, abbe_in;   // source-gradient scratch / staging
   klt::DevBuf abbe_pupil;                     // pupil table of the source-gradient points
   klt::DevBuf tcc_pts;                        // source points of the TCC build

@@ -1973,7 +1973,7 @@ __global__ __launch_bounds__(256, 4) void k_synth_wide(const float2* __restrict_
 // ---------------------------------------------------------------------------
 constexpr int kQW = 132;  // LDS row stride (floats) of the block's Q
 
-template <bool TVEC>
+template <bool TVEC, bool GIN>
 __device__ __forceinline__ void synth_resist_wide_strip(
     int x0, int y0, int y1, int pidx, const float2* __restrict__ spec, int Nx, int Ny, int Lx,
     int Ly, const float4* __restrict__ Efw, const float4* __restrict__ TYw,
@@ -1992,7 +1992,8 @@ __device__ __forceinline__ void synth_resist_wide_strip(
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int r16 = lane & 15, kq = (lane >> 4) * 8, q4 = (lane >> 4) * 4;
   half8_t pbh[2][4], pbl[2][4];
-  const float inv_pscale = wide_p_operand(x0, spec, Nx, Ny, Efw, smem, red, pbh, pbl);
+  float inv_pscale = 1.f;  // (GIN: G = dL/dI is the input itself, no synthesis)
+  if constexpr (!GIN) inv_pscale = wide_p_operand(x0, spec, Nx, Ny, Efw, smem, red, pbh, pbl);
   // 32-deep steps of j that hold some gy (the rest of TY and P is zero)
   const int nks = min(4, (4 * Ny + 2 + 31) / 32);
   float4* sTY = reinterpret_cast<float4*>(smem);  // [2][TY_F4]
@@ -2010,8 +2011,10 @@ __device__ __forceinline__ void synth_resist_wide_strip(
   auto fetch_ops = [&](int yb, float4& a0, float4& a1, float4& b0, float4& b1) {
     const float4* ty = TYw + (size_t)(yb >> 4) * TY_F4;
     const float4* tg = TYgw + (size_t)(yb >> 4) * TG_F4;
-    a0 = ty[tid];
-    a1 = ty[256 + tid];
+    if constexpr (!GIN) {
+      a0 = ty[tid];
+      a1 = ty[256 + tid];
+    }
     b0 = tg[tid];
     b1 = tg[256 + tid];
   };
@@ -2042,8 +2045,10 @@ __device__ __forceinline__ void synth_resist_wide_strip(
   auto step = [&](int yb, int sl, float4& tc0, float4& tc1, float4& a0, float4& a1, float4& b0,
                   float4& b1) {
     if (yb + 16 < y1) {
-      sTY[(sl ^ 1) * TY_F4 + tid] = a0;
-      sTY[(sl ^ 1) * TY_F4 + 256 + tid] = a1;
+      if constexpr (!GIN) {
+        sTY[(sl ^ 1) * TY_F4 + tid] = a0;
+        sTY[(sl ^ 1) * TY_F4 + 256 + tid] = a1;
+      }
       sTG[(sl ^ 1) * TG_F4 + tid] = b0;
       sTG[(sl ^ 1) * TG_F4 + 256 + tid] = b1;
       if (yb + 32 < y1) fetch_ops(yb + 32, a0, a1, b0, b1);
@@ -2054,7 +2059,7 @@ __device__ __forceinline__ void synth_resist_wide_strip(
     for (int cb = 0; cb < 2; ++cb) d[cb] = float4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
-      if (ks >= nks) break;
+      if (GIN || ks >= nks) break;
       const half8_t ah = __builtin_bit_cast(half8_t, ty[(ks * 2 + 0) * 64]);
       const half8_t al = __builtin_bit_cast(half8_t, ty[(ks * 2 + 1) * 64]);
 #pragma unroll
@@ -2071,22 +2076,49 @@ __device__ __forceinline__ void synth_resist_wide_strip(
 #pragma unroll
       for (int r = 0; r < 4; ++r) tv[cb][r] = stg[(q4 + r) * kStg + cb * 16 + r16];
     half4_t gh[2], gl[2];
+    float inv_gs = 1.f;  // (GIN: the tile's power-of-two scale, undone after the products)
+    if constexpr (GIN) {
+      // G itself, scaled so that the wave's largest |G| of the tile lies in
+      // [2^12, 2^13): both fp16 halves of the split stay in range
+      float g[2][4], mx = 0.f;
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
-      const bool colok = xw + cb * 16 + r16 < Lx;
-      float g[4];
+      for (int cb = 0; cb < 2; ++cb) {
+        const bool colok = xw + cb * 16 + r16 < Lx;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const bool ok = colok && yb + q4 + r < Ly;
-        const float sg =
-            __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(fmaf(c1, d[cb][r], c0)));
-        const float ds = ok ? sg - tv[cb][r] : 0.f;
-        lsum = fmaf(ds, ds, lsum);
-        g[r] = ds * fmaf(-sg, sg, sg) * two_a;
+        for (int r = 0; r < 4; ++r) {
+          g[cb][r] = colok && yb + q4 + r < Ly ? tv[cb][r] : 0.f;
+          mx = fmaxf(mx, fabsf(g[cb][r]));
+        }
       }
-      split4(g, gh[cb], gl[cb]);
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+      int eg = 0;
+      (void)frexpf(mx > 0.f ? mx : 1.f, &eg);
+      inv_gs = ldexpf(1.f, eg - 13);
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) g[cb][r] = ldexpf(g[cb][r], 13 - eg);
+        split4(g[cb], gh[cb], gl[cb]);
+      }
+    } else {
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) {
+        const bool colok = xw + cb * 16 + r16 < Lx;
+        float g[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const bool ok = colok && yb + q4 + r < Ly;
+          const float sg =
+              __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(fmaf(c1, d[cb][r], c0)));
+          const float ds = ok ? sg - tv[cb][r] : 0.f;
+          lsum = fmaf(ds, ds, lsum);
+          g[r] = ds * fmaf(-sg, sg, sg) * two_a;
+        }
+        split4(g, gh[cb], gl[cb]);
+      }
     }
-    if (img) {
+    if (!GIN && img) {
       wave_sync();
 #pragma unroll
       for (int cb = 0; cb < 2; ++cb)
@@ -2120,7 +2152,14 @@ __device__ __forceinline__ void synth_resist_wide_strip(
       if (jt >= njt) break;
       const half4_t gah = tg[(jt * 2 + 0) * 64], gal = tg[(jt * 2 + 1) * 64];
 #pragma unroll
-      for (int cb = 0; cb < 2; ++cb) Q[jt][cb] = mfma3k16(gah, gal, gh[cb], gl[cb], Q[jt][cb]);
+      for (int cb = 0; cb < 2; ++cb) {
+        if constexpr (GIN) {
+          const float4_t z = {0.f, 0.f, 0.f, 0.f};
+          Q[jt][cb] += mfma3k16(gah, gal, gh[cb], gl[cb], z) * inv_gs;
+        } else {
+          Q[jt][cb] = mfma3k16(gah, gal, gh[cb], gl[cb], Q[jt][cb]);
+        }
+      }
     }
     __syncthreads();  // slot sl is read, slot sl ^ 1 written, by every wave
   };
@@ -2229,7 +2268,7 @@ __device__ __forceinline__ void synth_resist_wide_strip(
   if (tid == 0) lout[pidx] = red2[0] + red2[1] + red2[2] + red2[3];
 }
 
-template <bool TVEC>
+template <bool TVEC, bool GIN = false>
 __global__ __launch_bounds__(256, 2) void k_synth_resist_wide(
     const float2* __restrict__ spec, int Nx, int Ny, int Lx, int Ly, int Lypad, int rs,
     const float4* __restrict__ Efw, const float4* __restrict__ TYw, const float* __restrict__ tgt,
@@ -2240,8 +2279,9 @@ __global__ __launch_bounds__(256, 2) void k_synth_resist_wide(
   const int per = (Lypad / 32 + rs - 1) / rs * 32;
   const int y0 = min(Lypad, rh * per), y1 = min(Lypad, y0 + per);
   const size_t ib = (size_t)b * Lx * Ly;
-  synth_resist_wide_strip<TVEC>(strip * kColsSynth, y0, y1, blockIdx.x,
-                                spec + (size_t)b * 64 * (4 * Nx + 1), Nx, Ny, Lx, Ly, Efw, TYw,
+  synth_resist_wide_strip<TVEC, GIN>(strip * kColsSynth, y0, y1, blockIdx.x,
+                                     GIN ? nullptr : spec + (size_t)b * 64 * (4 * Nx + 1), Nx,
+                                     Ny, Lx, Ly, Efw, TYw,
                                 tgt + ib, a_sig, thr, TYgw, Ebw, image ? image + ib : nullptr,
                                 gpart + (size_t)b * parts * (4 * Ny + 1) * 64,
                                 lpart + (size_t)b * parts);
@@ -2613,42 +2653,22 @@ __global__ void k_batch_begin(unsigned* __restrict__ strips_done, int n,
 // ---------------------------------------------------------------------------
 constexpr int kQS = 132;  // LDS row stride (floats) of the block's Q and C
 
-// TVEC: the target rows are whole 16-byte vectors (Lx % 4 == 0, aligned).
-// Wave w owns the 32 columns x0 + 32 w .. + 32 over all the block's rows; the
-// four waves step through the rows together, one 16-row tile a step.  The
-// tile's TY (synthesis) and TYg (resist band) operands are staged in LDS once
-// for the block, double-buffered behind one barrier a step, each thread
-// copying 2 x 16 bytes fetched a step ahead; the target streams two steps
-// ahead in registers (8 rows x 128 B per load).  The wave's P operand (its 32
-// columns) and its 64 x 32 Q accumulators stay in registers: 3 waves per SIMD.
-template <bool TVEC>
-__device__ __forceinline__ void synth_resist_strip(
-    int x0, int y0, int y1, int pidx, const float2* __restrict__ spec, int Nx, int Ny, int Lx,
-    int Ly, const float4* __restrict__ Ef, const _Float16* __restrict__ TYh,
-    const _Float16* __restrict__ TYl, const float* __restrict__ tgt, float a_sig, float thr,
-    const half4_t* __restrict__ TYgf, const float4* __restrict__ Ebf, float* __restrict__ img,
-    float2* __restrict__ gout, float* __restrict__ lout) {
-  constexpr int B_BYTES = 2 * 16 * 64 * 16, P_BYTES = 2 * kColsSynth * kPStride * 2;
-  constexpr int MAIN = B_BYTES > P_BYTES ? B_BYTES : P_BYTES;
-  constexpr int kStg = 36;  // staging row stride (floats) of a wave's 16 x 32 tile
-  // after the prologue: [slot][ks][hi, lo][lane] TY fragments (2 x 4 KB),
-  // [slot][jt][hi, lo][lane] TYg fragments (2 x 4 KB), the waves' staging rows
-  constexpr int TY_F4 = 4 * 64, TG_F4 = 4 * 64;  // float4 per slot
-  static_assert(2 * (TY_F4 + TG_F4) * 16 + 4 * 16 * kStg * 4 <= MAIN, "staging fits");
-  static_assert(64 * kQS * 4 <= MAIN, "Q and C fit in the operand LDS");
-  __shared__ __attribute__((aligned(16))) char smem[MAIN];
-  __shared__ float red[4];
+// Prologue of the fused adjoint synthesis (k_synth_resist): P = E . B on the
+// matrix cores exactly as synth_strip, through smem (the operand LDS), and the
+// wave's P operand (its 32 columns x0 + 32 wave .., K = j) in pbh / pbl;
+// returns 1 / pscale.
+__device__ __forceinline__ float narrow_p_operand(int x0, const float2* __restrict__ spec, int Nx,
+                                                  const float4* __restrict__ Ef, char* smem,
+                                                  float* red, half8_t (&pbh)[2][2],
+                                                  half8_t (&pbl)[2][2]) {
   half8_t* sBh = reinterpret_cast<half8_t*>(smem);
   half8_t* sBl = sBh + 16 * 64;
   _Float16* sPh = reinterpret_cast<_Float16*>(smem);
   _Float16* sPl = sPh + kColsSynth * kPStride;
   const int tid = threadIdx.x;
   const int Gx = 4 * Nx + 1;
-  // (the wave index as a scalar: everything derived from it is then known to
-  // be uniform)
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-  const int r16 = lane & 15, kq = (lane >> 4) * 8, q4 = (lane >> 4) * 4;
-  // --- P = E . B on the matrix cores, exactly as synth_strip -----------------
+  const int r16 = lane & 15, kq = (lane >> 4) * 8;
   float part = 0.f;
   for (int i = tid; i < 32 * Gx; i += 256) {
     const float2 v = spec[i];
@@ -2660,7 +2680,7 @@ __device__ __forceinline__ void synth_resist_strip(
   const float bound = red[0] + red[1] + red[2] + red[3];
   int ex = 0;
   (void)frexpf(bound > 0.f ? bound : 1.f, &ex);
-  const float pscale = ldexpf(1.0f, 12 - ex), inv_pscale = ldexpf(1.0f, ex - 12);
+  const float pscale = ldexpf(1.0f, 12 - ex);
   {
     const int j = 16 * wave + r16, gy = j >> 1, d = j & 1;
 #pragma unroll
@@ -2717,7 +2737,6 @@ __device__ __forceinline__ void synth_resist_strip(
   }
   __syncthreads();
   // the wave's P operand: columns 32 wave + 16 cb + lane % 16, K = j
-  half8_t pbh[2][2], pbl[2][2];
 #pragma unroll
   for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
@@ -2726,7 +2745,45 @@ __device__ __forceinline__ void synth_resist_strip(
       pbh[cb][ks] = *reinterpret_cast<const half8_t*>(sPh + c * kPStride + ks * 32 + kq);
       pbl[cb][ks] = *reinterpret_cast<const half8_t*>(sPl + c * kPStride + ks * 32 + kq);
     }
-  __syncthreads();  // the operand LDS becomes the staging area
+  __syncthreads();  // the operand LDS becomes the caller's staging area
+  return ldexpf(1.0f, ex - 12);
+}
+
+// TVEC: the target rows are whole 16-byte vectors (Lx % 4 == 0, aligned).
+// Wave w owns the 32 columns x0 + 32 w .. + 32 over all the block's rows; the
+// four waves step through the rows together, one 16-row tile a step.  The
+// tile's TY (synthesis) and TYg (resist band) operands are staged in LDS once
+// for the block, double-buffered behind one barrier a step, each thread
+// copying 2 x 16 bytes fetched a step ahead; the target streams two steps
+// ahead in registers (8 rows x 128 B per load).  The wave's P operand (its 32
+// columns) and its 64 x 32 Q accumulators stay in registers: 3 waves per SIMD.
+template <bool TVEC, bool GIN>
+__device__ __forceinline__ void synth_resist_strip(
+    int x0, int y0, int y1, int pidx, const float2* __restrict__ spec, int Nx, int Ny, int Lx,
+    int Ly, const float4* __restrict__ Ef, const _Float16* __restrict__ TYh,
+    const _Float16* __restrict__ TYl, const float* __restrict__ tgt, float a_sig, float thr,
+    const half4_t* __restrict__ TYgf, const float4* __restrict__ Ebf, float* __restrict__ img,
+    float2* __restrict__ gout, float* __restrict__ lout) {
+  constexpr int B_BYTES = 2 * 16 * 64 * 16, P_BYTES = 2 * kColsSynth * kPStride * 2;
+  constexpr int MAIN = B_BYTES > P_BYTES ? B_BYTES : P_BYTES;
+  constexpr int kStg = 36;  // staging row stride (floats) of a wave's 16 x 32 tile
+  // after the prologue: [slot][ks][hi, lo][lane] TY fragments (2 x 4 KB),
+  // [slot][jt][hi, lo][lane] TYg fragments (2 x 4 KB), the waves' staging rows
+  constexpr int TY_F4 = 4 * 64, TG_F4 = 4 * 64;  // float4 per slot
+  static_assert(2 * (TY_F4 + TG_F4) * 16 + 4 * 16 * kStg * 4 <= MAIN, "staging fits");
+  static_assert(64 * kQS * 4 <= MAIN, "Q and C fit in the operand LDS");
+  __shared__ __attribute__((aligned(16))) char smem[MAIN];
+  __shared__ float red[4];
+  const int tid = threadIdx.x;
+  // (the wave index as a scalar: everything derived from it is then known to
+  // be uniform)
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int r16 = lane & 15, kq = (lane >> 4) * 8, q4 = (lane >> 4) * 4;
+  // --- P = E . B on the matrix cores, exactly as synth_strip (GIN: G = dL/dI
+  // is the input itself, no synthesis) ----------------------------------------
+  half8_t pbh[2][2], pbl[2][2];
+  float inv_pscale = 1.f;
+  if constexpr (!GIN) inv_pscale = narrow_p_operand(x0, spec, Nx, Ef, smem, red, pbh, pbl);
   float4* sTY = reinterpret_cast<float4*>(smem);          // [2][TY_F4]
   float4* sTG = sTY + 2 * TY_F4;                           // [2][TG_F4]
   float* stg = reinterpret_cast<float*>(sTG + 2 * TG_F4) + wave * 16 * kStg;
@@ -2746,7 +2803,7 @@ __device__ __forceinline__ void synth_resist_strip(
     const int ln = tid & 63;
     const _Float16* src = ((tid >> 6) & 1 ? TYl : TYh) + (size_t)(yb + (ln & 15)) * 64 +
                           (tid >> 7) * 32 + (ln >> 4) * 8;
-    o_ty = *reinterpret_cast<const float4*>(src);
+    if constexpr (!GIN) o_ty = *reinterpret_cast<const float4*>(src);
     o_tg = reinterpret_cast<const float4*>(TYgf + (size_t)(yb >> 4) * 512)[tid];
   };
   // the target tile of the 16 rows at yb for this wave: lane holds rows
@@ -2777,21 +2834,24 @@ __device__ __forceinline__ void synth_resist_strip(
   // stored to the other slot and refilled with the operands two steps on
   auto step = [&](int yb, int sl, float4& tc0, float4& tc1, float4& o_ty, float4& o_tg) {
     if (yb + 16 < y1) {
-      sTY[(sl ^ 1) * TY_F4 + tid] = o_ty;
+      if constexpr (!GIN) sTY[(sl ^ 1) * TY_F4 + tid] = o_ty;
       sTG[(sl ^ 1) * TG_F4 + tid] = o_tg;
       if (yb + 32 < y1) fetch_ops(yb + 32, o_ty, o_tg);
     }
-    const float4* ty = sTY + sl * TY_F4 + lane;
-    const half8_t ah0 = __builtin_bit_cast(half8_t, ty[0]);
-    const half8_t al0 = __builtin_bit_cast(half8_t, ty[64]);
-    const half8_t ah1 = __builtin_bit_cast(half8_t, ty[128]);
-    const half8_t al1 = __builtin_bit_cast(half8_t, ty[192]);
     float4_t d[2];
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
-      d[cb] = float4_t{0.f, 0.f, 0.f, 0.f};
-      d[cb] = mfma3(ah0, al0, pbh[cb][0], pbl[cb][0], d[cb]);
-      d[cb] = mfma3(ah1, al1, pbh[cb][1], pbl[cb][1], d[cb]);
+    for (int cb = 0; cb < 2; ++cb) d[cb] = float4_t{0.f, 0.f, 0.f, 0.f};
+    if constexpr (!GIN) {
+      const float4* ty = sTY + sl * TY_F4 + lane;
+      const half8_t ah0 = __builtin_bit_cast(half8_t, ty[0]);
+      const half8_t al0 = __builtin_bit_cast(half8_t, ty[64]);
+      const half8_t ah1 = __builtin_bit_cast(half8_t, ty[128]);
+      const half8_t al1 = __builtin_bit_cast(half8_t, ty[192]);
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) {
+        d[cb] = mfma3(ah0, al0, pbh[cb][0], pbl[cb][0], d[cb]);
+        d[cb] = mfma3(ah1, al1, pbh[cb][1], pbl[cb][1], d[cb]);
+      }
     }
     // target rows through the wave's staging rows into the MFMA output layout
     *reinterpret_cast<float4*>(stg + (lane >> 3) * kStg + (lane & 7) * 4) = tc0;
@@ -2806,22 +2866,49 @@ __device__ __forceinline__ void synth_resist_strip(
 #pragma unroll
       for (int r = 0; r < 4; ++r) tv[cb][r] = stg[(q4 + r) * kStg + cb * 16 + r16];
     half4_t gh[2], gl[2];
+    float inv_gs = 1.f;  // (GIN: the tile's power-of-two scale, undone after the products)
+    if constexpr (GIN) {
+      // G itself, scaled so that the wave's largest |G| of the tile lies in
+      // [2^12, 2^13): both fp16 halves of the split stay in range
+      float g[2][4], mx = 0.f;
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
-      const bool colok = xw + cb * 16 + r16 < Lx;
-      float g[4];
+      for (int cb = 0; cb < 2; ++cb) {
+        const bool colok = xw + cb * 16 + r16 < Lx;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const bool ok = colok && yb + q4 + r < Ly;
-        const float sg =
-            __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(fmaf(c1, d[cb][r], c0)));
-        const float ds = ok ? sg - tv[cb][r] : 0.f;
-        lsum = fmaf(ds, ds, lsum);
-        g[r] = ds * fmaf(-sg, sg, sg) * two_a;
+        for (int r = 0; r < 4; ++r) {
+          g[cb][r] = colok && yb + q4 + r < Ly ? tv[cb][r] : 0.f;
+          mx = fmaxf(mx, fabsf(g[cb][r]));
+        }
       }
-      split4(g, gh[cb], gl[cb]);
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+      int eg = 0;
+      (void)frexpf(mx > 0.f ? mx : 1.f, &eg);
+      inv_gs = ldexpf(1.f, eg - 13);
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) g[cb][r] = ldexpf(g[cb][r], 13 - eg);
+        split4(g[cb], gh[cb], gl[cb]);
+      }
+    } else {
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) {
+        const bool colok = xw + cb * 16 + r16 < Lx;
+        float g[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const bool ok = colok && yb + q4 + r < Ly;
+          const float sg =
+              __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(fmaf(c1, d[cb][r], c0)));
+          const float ds = ok ? sg - tv[cb][r] : 0.f;
+          lsum = fmaf(ds, ds, lsum);
+          g[r] = ds * fmaf(-sg, sg, sg) * two_a;
+        }
+        split4(g, gh[cb], gl[cb]);
+      }
     }
-    if (img) {
+    if (!GIN && img) {
       // (each lane rewrites the entries it has just read, then the wave reads
       // them back as rows)
       wave_sync();
@@ -2856,7 +2943,14 @@ __device__ __forceinline__ void synth_resist_strip(
     for (int jt = 0; jt < 4; ++jt) {
       const half4_t gah = tg[(jt * 2 + 0) * 64], gal = tg[(jt * 2 + 1) * 64];
 #pragma unroll
-      for (int cb = 0; cb < 2; ++cb) Q[jt][cb] = mfma3k16(gah, gal, gh[cb], gl[cb], Q[jt][cb]);
+      for (int cb = 0; cb < 2; ++cb) {
+        if constexpr (GIN) {
+          const float4_t z = {0.f, 0.f, 0.f, 0.f};
+          Q[jt][cb] += mfma3k16(gah, gal, gh[cb], gl[cb], z) * inv_gs;
+        } else {
+          Q[jt][cb] = mfma3k16(gah, gal, gh[cb], gl[cb], Q[jt][cb]);
+        }
+      }
     }
     __syncthreads();  // slot sl is read, slot sl ^ 1 written, by every wave
   };
@@ -2956,7 +3050,7 @@ __device__ __forceinline__ void synth_resist_strip(
   }
 }
 
-template <bool TVEC>
+template <bool TVEC, bool GIN = false>
 __global__ __launch_bounds__(256, 3) void k_synth_resist(
     const float2* __restrict__ spec, int Nx, int Ny, int Lx, int Ly, int Lypad, int rs,
     const float4* __restrict__ Ef, const _Float16* __restrict__ TYh,
@@ -2968,7 +3062,8 @@ __global__ __launch_bounds__(256, 3) void k_synth_resist(
   const int per = (Lypad / 32 + rs - 1) / rs * 32;
   const int y0 = min(Lypad, rh * per), y1 = min(Lypad, y0 + per);
   const size_t ib = (size_t)b * Lx * Ly;
-  synth_resist_strip<TVEC>(strip * kColsSynth, y0, y1, blockIdx.x, spec + (size_t)b * 32 * (4 * Nx + 1),
+  synth_resist_strip<TVEC, GIN>(strip * kColsSynth, y0, y1, blockIdx.x,
+                     GIN ? nullptr : spec + (size_t)b * 32 * (4 * Nx + 1),
                      Nx, Ny, Lx, Ly, Ef, TYh, TYl, tgt + ib, a_sig, thr, TYgf, Ebf,
                      image ? image + ib : nullptr,
                      gpart + (size_t)b * parts * (4 * Ny + 1) * 32, lpart + (size_t)b * parts);
@@ -3379,9 +3474,9 @@ static int pick_split(klt_ctx* ctx, int kind, int nb, int Lx, int Ly, int Nx, in
   // partly filled last round)
   for (int v : {1, 2, 3, 4, 6, 8, 12, 16}) {
     if (v == best.rs || v > kMaxRs || !split_ok(nrow32, v)) continue;
-    // the adjoint keeps one resist-band partial per synthesis block: at most
-    // 16 blocks per CU
-    if (kind == 1 && (size_t)nstrips * v * nb > 4096) continue;
+    // the adjoint (and the product with a given dL/dI) keeps one resist-band
+    // partial per synthesis block: at most 16 blocks per CU
+    if (kind >= 1 && (size_t)nstrips * v * nb > 4096) continue;
     klt_ctx::Tuned c = best;
     c.rs = v;
     KLT_TRY(time_run(c, &ms));
@@ -4157,6 +4252,139 @@ int socs_backward_wide(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
   return KLT_OK;
 }
 
+// dL/dm from a given G = dL/dI: the adjoint stages of socs_backward_fused /
+// socs_backward_wide without the forward coarse stage and synthesis.  G's
+// band comes from the fused resist-band kernel reading G where the adjoint
+// reads its target (k_synth_resist<., true>, k_synth_resist_wide<., true>).
+int socs_vjp_device(klt_ctx* ctx, const klt_kernels* kb, const float* masks, const float* gin,
+                    int B, int Lx, int Ly, float* grads) {
+  SocsPlan* p = nullptr;
+  KLT_TRY(get_socs_plan(ctx, Lx, Ly, kb->Nx, kb->Ny, &p));
+  FusedTables* ft = nullptr;
+  KLT_TRY(get_fused_tables(ctx, p, &ft));
+  const bool wide = !socs_fused_supported(kb);
+  const int Nx = kb->Nx, Ny = kb->Ny, Wy = 2 * Ny + 1, Gx = 4 * Nx + 1, Gy = 4 * Ny + 1;
+  const int T = (2 * Nx + 1) * Wy;
+  const int ldb = wide ? 32 : 16, ldg = wide ? 64 : 32, rows = wide ? 64 : 32;
+  const int M2 = wide ? 128 * 128 : kM * kM;  // coarse-grid points
+  const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
+  const int nblk64 = (Ly + 63) / 64;
+  const int nstrips = (Lx + kColsSynth - 1) / kColsSynth;
+  const int SB = std::min(B, 512);
+  KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk64 * kMaxKs * Wy * ldb * sizeof(float2)));
+  KLT_TRY(ctx->gspec.ensure((size_t)SB * rows * Gx * sizeof(float2)));
+  KLT_TRY(ctx->gam_part.ensure((size_t)kGroupSlots * T * sizeof(float2)));
+  KLT_TRY(ctx->mhat.ensure((size_t)SB * T * sizeof(float2)));
+  KLT_TRY(ctx->gcoarse.ensure((size_t)SB * M2 * sizeof(float)));
+  KLT_TRY(ctx->gfull.ensure((size_t)SB * Gx * Gy * sizeof(float2)));
+  KLT_TRY(ctx->gsum.ensure((size_t)SB * 2 * T * sizeof(float)));
+  KLT_TRY(ctx->vjp_sum.ensure((size_t)SB * sizeof(float)));
+  const int lds_fin = 96 * kRS * (int)sizeof(float2);
+  const int lds_mid = lds_fin + T * (int)sizeof(float2);
+  if (wide) {
+    KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_socs_mid_bwd128),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds_mid));
+    KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_resist_coarse128),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds_fin));
+  }
+  const float inv_area = 1.0f / ((float)Lx * (float)Ly);
+  const int dcx = (Lx + 1) / 2 - Lx / 2, dcy = (Ly + 1) / 2 - Ly / 2;
+  const bool tvec = Lx % 4 == 0 && (reinterpret_cast<uintptr_t>(gin) & 15) == 0;
+  for (int b0 = 0; b0 < B; b0 += SB) {
+    const int nb = std::min(SB, B - b0);
+    const size_t off = (size_t)b0 * Lx * Ly;
+    auto run = [&](const klt_ctx::Tuned& c) -> int {
+      const int parts = (Ly + c.rb - 1) / c.rb * c.ks, sparts = nstrips * c.rs;
+      KLT_TRY(ctx->gpart.ensure((size_t)nb * sparts * Gy * ldg * sizeof(float2)));
+      KLT_TRY(ctx->lpart.ensure((size_t)nb * sparts * sizeof(float)));
+      if (wide)
+        launch_mask_band_wide(ctx, c.rb, c.ks, nb, masks + off, Lx, Ly, ft->Kpad,
+                              ft->Wf.as<float4>(), p->twy_in.as<float2>(), p->NP, Nx, Ny,
+                              ctx->qpart.as<float2>());
+      else
+        launch_mask_band(ctx, c.rb, c.ks, nb, masks + off, Lx, Ly, ft->Kpad,
+                         ft->Wf.as<float4>(), p->twy_in.as<float2>(), p->NP, Nx, Ny,
+                         ctx->qpart.as<float2>());
+      KLT_LAUNCH(ctx, "k_band_assemble", k_band_assemble, dim3((unsigned)(T + 255) / 256, nb),
+                 dim3(256), 0, ctx->qpart.as<float2>(), parts, Nx, Ny, ldb, inv_area,
+                 ctx->mhat.as<float2>());
+      // G's band partials, one per (strip, row range) block
+      if (wide && tvec)
+        KLT_LAUNCH(ctx, "k_synth_resist_wide", (k_synth_resist_wide<true, true>),
+                   dim3(sparts, nb), dim3(256), 0, nullptr, Nx, Ny, Lx, Ly, ft->Lypad, c.rs,
+                   ft->Efw.as<float4>(), ft->TYw.as<float4>(), gin + off, 0.f, 0.f,
+                   ft->TYgw.as<float4>(), ft->Ebw.as<float4>(), nullptr, ctx->gpart.as<float2>(),
+                   ctx->lpart.as<float>());
+      else if (wide)
+        KLT_LAUNCH(ctx, "k_synth_resist_wide", (k_synth_resist_wide<false, true>),
+                   dim3(sparts, nb), dim3(256), 0, nullptr, Nx, Ny, Lx, Ly, ft->Lypad, c.rs,
+                   ft->Efw.as<float4>(), ft->TYw.as<float4>(), gin + off, 0.f, 0.f,
+                   ft->TYgw.as<float4>(), ft->Ebw.as<float4>(), nullptr, ctx->gpart.as<float2>(),
+                   ctx->lpart.as<float>());
+      else if (tvec)
+        KLT_LAUNCH(ctx, "k_synth_resist", (k_synth_resist<true, true>), dim3(sparts, nb),
+                   dim3(256), 0, nullptr, Nx, Ny, Lx, Ly, ft->Lypad, c.rs, ft->Ef.as<float4>(),
+                   ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), gin + off, 0.f, 0.f,
+                   ft->TYgf.as<half4_t>(), ft->Ebf.as<float4>(), nullptr,
+                   ctx->gpart.as<float2>(), ctx->lpart.as<float>());
+      else
+        KLT_LAUNCH(ctx, "k_synth_resist", (k_synth_resist<false, true>), dim3(sparts, nb),
+                   dim3(256), 0, nullptr, Nx, Ny, Lx, Ly, ft->Lypad, c.rs, ft->Ef.as<float4>(),
+                   ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), gin + off, 0.f, 0.f,
+                   ft->TYgf.as<half4_t>(), ft->Ebf.as<float4>(), nullptr,
+                   ctx->gpart.as<float2>(), ctx->lpart.as<float>());
+      KLT_LAUNCH(ctx, "k_resist_sum", k_resist_sum, dim3((unsigned)(Gx * Gy + 255) / 256, nb),
+                 dim3(256), 0, ctx->gpart.as<float2>(), ctx->lpart.as<float>(), sparts, Nx, Ny,
+                 ldg, inv_area, ctx->gfull.as<float2>(), ctx->vjp_sum.as<float>());
+      if (wide) {
+        KLT_LAUNCH(ctx, "k_resist_coarse128", k_resist_coarse128, dim3(nb), dim3(256), lds_fin,
+                   ctx->gfull.as<float2>(), Nx, Ny, ft->tw128.as<float2>(),
+                   ctx->gcoarse.as<float>());
+        KLT_LAUNCH(ctx, "k_socs_mid_bwd128", k_socs_mid_bwd128, dim3(c.kg, nb), dim3(256),
+                   lds_mid, ctx->mhat.as<float2>(), ctx->gcoarse.as<float>(), Nx, Ny,
+                   kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk,
+                   ft->tw128.as<float2>(), dcx, dcy, Lx, Ly, ctx->gspec.as<float2>(),
+                   ctx->gam_part.as<float2>());
+        if (c.kg > 1)
+          KLT_LAUNCH(ctx, "k_socs_mid_bwd_fin", k_socs_mid_bwd_fin, dim3(nb), dim3(256), 0,
+                     ctx->gam_part.as<float2>(), c.kg, Nx, Ny, dcx, dcy, Lx, Ly,
+                     ctx->gspec.as<float2>(), 64);
+        KLT_LAUNCH(ctx, "k_synth_wide", k_synth_wide, dim3(nstrips * c.rs, nb), dim3(256), 0,
+                   ctx->gspec.as<float2>(), Nx, Ny, Lx, Ly, ft->Lypad, c.rs,
+                   ft->Efw.as<float4>(), ft->TYw.as<float4>(), grads + off);
+      } else {
+        KLT_LAUNCH(ctx, "k_resist_coarse", k_resist_coarse, dim3(nb), dim3(256), 0,
+                   ctx->gfull.as<float2>(), Nx, Ny, ft->tw64.as<float2>(),
+                   ctx->gcoarse.as<float>());
+        KLT_LAUNCH(ctx, "k_socs_mid_bwd", k_socs_mid_bwd, dim3(c.kg, nb), dim3(256), 0,
+                   ctx->mhat.as<float2>(), ctx->gcoarse.as<float>(), Nx, Ny,
+                   kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk,
+                   ft->tw64.as<float2>(), dcx, dcy, Lx, Ly, ctx->gspec.as<float2>(),
+                   ctx->gam_part.as<float2>());
+        if (c.kg > 1) {
+          const int n2 = 2 * T;  // Gam entries as floats
+          KLT_LAUNCH(ctx, "k_sum_groups", k_sum_groups, dim3((n2 + 255) / 256, nb), dim3(256),
+                     0, reinterpret_cast<const float*>(ctx->gam_part.as<float2>()), c.kg, n2,
+                     ctx->gsum.as<float>());
+          KLT_LAUNCH(ctx, "k_socs_mid_bwd_fin", k_socs_mid_bwd_fin, dim3(nb), dim3(256), 0,
+                     reinterpret_cast<const float2*>(ctx->gsum.as<float>()), 1, Nx, Ny, dcx,
+                     dcy, Lx, Ly, ctx->gspec.as<float2>(), 32);
+        }
+        KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma, dim3(nstrips * c.rs, nb), dim3(256), 0,
+                   ctx->gspec.as<float2>(), Nx, Lx, Ly, ft->Lypad, c.rs, ft->Ef.as<float4>(),
+                   ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), grads + off);
+      }
+      KLT_HIP(hipGetLastError());
+      return KLT_OK;
+    };
+    klt_ctx::Tuned c{};
+    KLT_TRY(pick_split(ctx, 2, nb, Lx, Ly, Nx, Ny, kb->nk, ft->Kpad / (kBandChunk * 32), nblk,
+                       nstrips, ft->Lypad / 32, run, &c));
+    KLT_TRY(run(c));
+  }
+  return KLT_OK;
+}
+
 }  // namespace klt
 
 extern "C" {
@@ -4198,6 +4426,38 @@ int klt_socs_backward(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
     KLT_HIP(hipMemcpyAsync(grads, dg, bytes, hipMemcpyDeviceToHost, ctx->stream));
     KLT_HIP(hipMemcpyAsync(losses, dl, (size_t)nmask * 4, hipMemcpyDeviceToHost, ctx->stream));
     if (images) KLT_HIP(hipMemcpyAsync(images, di, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    KLT_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return KLT_OK;
+}
+
+int klt_socs_vjp(klt_ctx* ctx, const klt_kernels* kb, const float* masks, const float* dL_dI,
+                 int nmask, int Lx, int Ly, float* grads, unsigned flags) {
+  if (!ctx || !kb || !masks || !dL_dI || !grads)
+    return klt::set_error(KLT_ERR_ARG, "klt_socs_vjp: NULL argument");
+  if (nmask <= 0 || Lx <= 0 || Ly <= 0)
+    return klt::set_error(KLT_ERR_ARG, "klt_socs_vjp: bad size");
+  if (4 * kb->Nx + 1 > Lx || 4 * kb->Ny + 1 > Ly)
+    return klt::set_error(KLT_ERR_SIZE, "ERROR in FI()");
+  if (!klt::socs_fused_supported(kb) && !klt::socs_wide_supported(kb))
+    return klt::set_error(KLT_ERR_ARG, "klt_socs_vjp: kernel half-size above 31");
+  KLT_TRY(ctx->activate());
+  const size_t bytes = (size_t)nmask * Lx * Ly * sizeof(float);
+  const float *dm = masks, *dg = dL_dI;
+  float* out = grads;
+  if (flags & KLT_HOST_PTRS) {
+    KLT_TRY(ctx->stage_in.ensure(2 * bytes));
+    char* s = ctx->stage_in.as<char>();
+    KLT_HIP(hipMemcpyAsync(s, masks, bytes, hipMemcpyHostToDevice, ctx->stream));
+    KLT_HIP(hipMemcpyAsync(s + bytes, dL_dI, bytes, hipMemcpyHostToDevice, ctx->stream));
+    dm = reinterpret_cast<const float*>(s);
+    dg = reinterpret_cast<const float*>(s + bytes);
+    KLT_TRY(ctx->stage_out.ensure(bytes));
+    out = ctx->stage_out.as<float>();
+  }
+  KLT_TRY(klt::socs_vjp_device(ctx, kb, dm, dg, nmask, Lx, Ly, out));
+  if (flags & KLT_HOST_PTRS) {
+    KLT_HIP(hipMemcpyAsync(grads, out, bytes, hipMemcpyDeviceToHost, ctx->stream));
     KLT_HIP(hipStreamSynchronize(ctx->stream));
   }
   return KLT_OK;

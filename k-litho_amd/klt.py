@@ -85,6 +85,7 @@ def lib() -> C.CDLL:
         L.klt_hopkins_image.argtypes = [vp, fp, i, i, fp, i, i, fp, u]
         L.klt_socs_backward.argtypes = [vp, vp, fp, fp, i, i, i, C.c_float, C.c_float, fp, fp,
                                         fp, u]
+        L.klt_socs_vjp.argtypes = [vp, vp, fp, fp, i, i, i, fp, u]
         L.klt_abbe_source_grad.argtypes = [vp, fp, fp, i, i, i, fp, i, f, f, f, f, i, i, f, f,
                                            fp, fp, fp, u]
         L.klt_kernels_create.argtypes = [vp, fp, fp, fp, i, i, i, u, C.POINTER(vp)]
@@ -524,6 +525,22 @@ def socs_backward(ctx: "Context", bank: "KernelBank", masks, targets, a: float =
         check(lib().klt_socs_backward(ctx.handle, bank.handle, pm, pt, B, Lx, Ly, a, thr, pg,
                                       pl, pi, _flags(*hs)))
     return losses, grads, images
+
+
+def socs_vjp(ctx: "Context", bank: "KernelBank", masks, dl_di):
+    """dL/dm [B, Ly, Lx] of the SOCS images of masks for a given dL/dI [B, Ly, Lx]
+    (klt_socs_vjp)."""
+    shape = tuple(masks.shape)
+    Ly, Lx = shape[-2:]
+    B = int(np.prod(shape[:-2])) if len(shape) > 2 else 1
+    grads = _empty_like(masks)
+    pm, hm = _ptr(masks, np.float32)
+    pd, hd = _ptr(dl_di, np.float32)
+    pg, hg = _ptr(grads, np.float32)
+    with _TorchOrder(ctx.handle, masks, dl_di):
+        check(lib().klt_socs_vjp(ctx.handle, bank.handle, pm, pd, B, Lx, Ly, pg,
+                                 _flags(hm, hd, hg)))
+    return grads
 
 
 def abbe_source_grad(ctx: "Context", masks, targets, src, NA: float, defocus: float, Nx: int,
