@@ -10,7 +10,8 @@ over the timed steps.
 Also reported (same JSON line):
   plain        the FP32 single-image path (klt_socs_image, one call per image)
   adjoint      klt_socs_backward on 16 (and 128) relaxed 1024^2 masks (mask
-               gradients/s, with its own roofline object)
+               gradients/s, with its own roofline object), klt_socs_vjp (the
+               product with a given dL/dI) on the same masks
                and klt_abbe_source_grad (source gradient over the 51^2 grid)
   tcc_eig      TCC T = 1225 (blurred dipole, period 1280, 5 defocus values),
                split-FP16 MFMA Gram vs FP64 Gram, top-32 FP64 eigensolve (one
@@ -334,6 +335,11 @@ def adjoint_batch(ctx, bank, timer, torch, dev, B):
     klt.socs_backward(ctx, bank, dm, dt)
     rep = ctx.profile_report()
     ctx.set_profiling(False)
+    # the product with a given dL/dI (no forward passes), on the same masks
+    dg = torch.randn_like(dm)
+    for _ in range(3):
+        klt.socs_vjp(ctx, bank, dm, dg)
+    ms_vjp = timer(lambda: klt.socs_vjp(ctx, bank, dm, dg), 20)
     kern = {k: {"ms": v[1], "launches": v[0]} for k, v in rep.items()}
     # each streaming kernel moves 4 L^2 B algorithmic bytes: the mask read
     # (band), the target read (fused synthesis + resist band), the gradient
@@ -344,7 +350,8 @@ def adjoint_batch(ctx, bank, timer, torch, dev, B):
             v["algorithmic_bytes_per_launch"] = 4.0 * L * L * B
             v["achieved_gbs"] = 4.0 * L * L * B / (v["ms"] / v["launches"] / 1e3) / 1e9
     return dm, dt, {"batch": B, "mask_grads_per_s": B / (ms / 1e3), "ms_per_batch": ms,
-                    "path_hbm_gbs": 12.0 * L * L * B / (ms / 1e3) / 1e9, "kernels": kern}
+                    "path_hbm_gbs": 12.0 * L * L * B / (ms / 1e3) / 1e9, "kernels": kern,
+                    "vjp_mask_grads_per_s": B / (ms_vjp / 1e3), "vjp_ms_per_batch": ms_vjp}
 
 
 ADJ_COUNTERS = os.path.join(ROOT, "profiles", "adjoint_1024_hbm_counters.json")
@@ -707,6 +714,8 @@ def main():
     extras = {}
     if not args.no_extras:
         adj = bench_adjoint(ctx, bank, timer, torch, dev, src)
+        log(f"vjp (given dL/dI): {adj['vjp_mask_grads_per_s']:.1f} mask grads/s at 16, "
+            f"{adj['batch128']['vjp_mask_grads_per_s']:.1f} at 128")
         log(f"adjoint: {adj['mask_grads_per_s']:.1f} mask grads/s at 16 "
             f"({adj['roofline']['kernel']} {adj['roofline']['frac']:.3f} of HBM peak), "
             f"{adj['batch128']['mask_grads_per_s']:.1f} at 128; source grad "
