@@ -241,7 +241,7 @@ int klt_socs_forward_noalloc(klt_ctx* ctx, const klt_kernels* kb, const float* m
 /* Masks of different sizes under several kernel sets in one call (no
  * reference counterpart: klitho_socs.cpp:338-343 images one mask per run).
  * masks[m] (Ly[m] x Lx[m]) under set s gives images[m * nbank + s].  Every
- * set must have the same kernel size, half-size <= 15.  Each mask's band is
+ * set must have the same kernel size, half-size <= 31.  Each mask's band is
  * computed once and shared by its sets.  The call's launches are captured in
  * a hipGraph on first use and replayed when the same arguments (pointers,
  * sizes, sets) come again, so a caller that refills the same buffers pays one

@@ -104,25 +104,25 @@ struct BandJob {        // row block `blk` of one mask
   const float* in;      // the mask, Ly x Lx
   const float4* Wf;     // its row twiddles (fragment order)
   const float2* twy;    // its column twiddles, [Ly][ldtw]
-  float2* out;          // its band partials, [nblk][2Ny+1][16]
+  float2* out;          // its band partials, [nblk][2Ny+1][16] (32 for N > 15)
   int Lx, Ly, Kpad, ldtw, Nx, Ny, blk, pad_;
 };
 struct MidJob {         // one mask x one kernel set
   const float2* mpart;  // the mask's band partials
   const float2* kern;   // the set's kernels, [nk][2Ny+1][2Nx+1]
   const float* scales;  // and eigenvalues
-  float2* spec;         // the image's intensity spectrum, [32][4Nx+1]
+  float2* spec;         // the image's intensity spectrum, [32][4Nx+1] (64 rows for N > 15)
   int nblk, Nx, Ny, nk;
   float inv_area;
   int pad_[3];
 };
 struct SynthJob {       // one 128-column strip of one image
   const float2* spec;
-  const float4* Ef;     // synthesis column twiddles (fragment order)
-  const _Float16* TYh;
+  const float4* Ef;     // synthesis column twiddles (fragment order; Efw for N > 15)
+  const _Float16* TYh;  // (N > 15: the TYw fragment table)
   const _Float16* TYl;
   float* img;
-  int Nx, Lx, Ly, Lypad, x0, image, nstrips, pad_;
+  int Nx, Lx, Ly, Lypad, x0, image, nstrips, Ny;
 };
 
 // One captured mixed batch: its job tables, intermediates and the graph that
@@ -133,10 +133,12 @@ struct SynthJob {       // one 128-column strip of one image
 struct BatchGraph {
   std::vector<uint64_t> key;
   int nimg = 0, nband = 0, nmid = 0, nsynth = 0;
+  bool wide = false;  // kernel half-size above 15 (the wide passes)
+  int lds_mid = 0;    // dynamic LDS of the wide coarse stage
   bool full = true;
   bool held = false;
   DevBuf band_jobs, mid_jobs, synth_jobs, mpart, spec, strips_done, done_at, t0;
-  const float2* spec_tw = nullptr;  // e^{+2 pi i j/64} of the coarse FFTs
+  const float2* spec_tw = nullptr;  // e^{+2 pi i j/64} (j/128 for N > 15) of the coarse FFTs
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~BatchGraph();

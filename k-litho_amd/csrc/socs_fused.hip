@@ -425,6 +425,16 @@ __global__ __launch_bounds__(256, FULL ? 4 : 2) void k_band_jobs(
                              nullptr);
 }
 
+// The same for kernel half-sizes 16..31: 64 twiddle rows, 32 columns a row
+// of partials.
+template <bool FULL>
+__global__ __launch_bounds__(256, 2) void k_band_jobs_wide(const klt::BandJob* __restrict__ jobs) {
+  const klt::BandJob j = jobs[blockIdx.x];
+  band_block<4, false, FULL>(j.blk, 0, j.Kpad / (kBandChunk * 32), j.blk, j.in, nullptr, 0.f,
+                             0.f, j.Lx, j.Ly, j.Kpad, j.Wf, j.twy, j.ldtw, j.Nx, j.Ny, 32, j.out,
+                             nullptr);
+}
+
 // Sum over the 64 lanes, valid in every lane, by DPP moves: pairs and quads
 // (quad_perm), rows of 16 (row_shr 4, 8), rows into lane 63 (row_bcast 15,
 // 31), read back from lane 63.
@@ -1581,22 +1591,40 @@ __device__ __forceinline__ void coarse128_to_spec(const float (&acc)[4][16], flo
 // intensity spectrum.  With |n| <= 31, inputs t + 8 n2 of the transforms are
 // zero for n2 = 4..11: written as constants so that the first stage drops
 // them.
-__global__ __launch_bounds__(256) void k_socs_mid128(const float2* __restrict__ Mfull, int Nx,
-                                                     int Ny, const float2* __restrict__ kern,
-                                                     const float* __restrict__ scales, int nk,
-                                                     const float2* __restrict__ tw128,
-                                                     float2* __restrict__ spec,
-                                                     float* __restrict__ ipart) {
+// Body of one block of the 128-point coarse stage: image band from Mband
+// (PARTS = false: the normalised band [2Ny+1][2Nx+1]; true: its nblk band
+// partials [nblk][2Ny+1][32], summed here and scaled by inv_area), kernels
+// [k0, k1); with ipart the block's partial intensity goes there (thread-major
+// [256][64]), else spec [64][4Nx+1] receives the intensity spectrum.
+template <bool PARTS>
+__device__ __forceinline__ void mid128_image(const float2* __restrict__ Mband, int nblk,
+                                             float inv_area, int Nx, int Ny,
+                                             const float2* __restrict__ kern,
+                                             const float* __restrict__ scales, int k0, int k1,
+                                             const float2* __restrict__ tw128,
+                                             float2* __restrict__ spec,
+                                             float* __restrict__ ipart) {
   extern __shared__ float4 smem4[];
   float2* sB = reinterpret_cast<float2*>(smem4);  // [64][kRS]
   float2* sX = sB + 64 * kRS;                     // [32][kRS]
   float2* sM = sX + 32 * kRS;                     // [T]
-  const int b = blockIdx.y, kgs = gridDim.x, j = blockIdx.x;
   const int tid = threadIdx.x, g = tid >> 3, t = tid & 7;
   const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
-  const int per = (nk + kgs - 1) / kgs;
-  const int k0 = min(nk, j * per), k1 = min(nk, k0 + per);
-  for (int i = tid; i < T; i += 256) sM[i] = Mfull[(size_t)b * T + i];
+  for (int i = tid; i < T; i += 256) {
+    if (!PARTS) {
+      sM[i] = Mband[i];
+      continue;
+    }
+    const int nyi = i / Wx, nx = i - nyi * Wx - Nx, ny = nyi - Ny;
+    const int r = nx >= 0 ? nyi : (-ny + Ny), c = nx >= 0 ? nx : -nx;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int k = 0; k < nblk; ++k) {
+      const float2 q = Mband[((size_t)k * Wy + r) * 32 + c];
+      acc.x += q.x;
+      acc.y += q.y;
+    }
+    sM[i] = make_float2(acc.x * inv_area, (nx < 0 ? -acc.y : acc.y) * inv_area);
+  }
   float2 twi[16];
 #pragma unroll
   for (int k2 = 0; k2 < 16; ++k2) twi[k2] = tw128[(t * k2) & 127];
@@ -1657,7 +1685,7 @@ __global__ __launch_bounds__(256) void k_socs_mid128(const float2* __restrict__ 
     __syncthreads();  // the column reads end before the next kernel's rows
   }
   if (ipart) {
-    float4* dst = reinterpret_cast<float4*>(ipart + ((size_t)b * kgs + j) * 16384 + tid * 64);
+    float4* dst = reinterpret_cast<float4*>(ipart + tid * 64);
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -1666,7 +1694,30 @@ __global__ __launch_bounds__(256) void k_socs_mid128(const float2* __restrict__ 
                                      acc[q][4 * i + 3]);
     return;
   }
-  coarse128_to_spec(acc, sB, sX, twi, Nx, Ny, spec + (size_t)b * 64 * (4 * Nx + 1));
+  coarse128_to_spec(acc, sB, sX, twi, Nx, Ny, spec);
+}
+
+__global__ __launch_bounds__(256) void k_socs_mid128(const float2* __restrict__ Mfull, int Nx,
+                                                     int Ny, const float2* __restrict__ kern,
+                                                     const float* __restrict__ scales, int nk,
+                                                     const float2* __restrict__ tw128,
+                                                     float2* __restrict__ spec,
+                                                     float* __restrict__ ipart) {
+  const int b = blockIdx.y, kgs = gridDim.x, j = blockIdx.x;
+  const int T = (2 * Nx + 1) * (2 * Ny + 1), per = (nk + kgs - 1) / kgs;
+  const int k0 = min(nk, j * per), k1 = min(nk, k0 + per);
+  mid128_image<false>(Mfull + (size_t)b * T, 0, 1.f, Nx, Ny, kern, scales, k0, k1, tw128,
+                      spec ? spec + (size_t)b * 64 * (4 * Nx + 1) : nullptr,
+                      ipart ? ipart + ((size_t)b * kgs + j) * 16384 : nullptr);
+}
+
+// Mixed sizes and kernel sets, half-sizes 16..31: one block per MidJob (one
+// mask x one set; the mask's band partials have 32 columns a row).
+__global__ __launch_bounds__(256) void k_socs_mid128_jobs(const klt::MidJob* __restrict__ jobs,
+                                                          const float2* __restrict__ tw128) {
+  const klt::MidJob j = jobs[blockIdx.x];
+  mid128_image<true>(j.mpart, j.nblk, j.inv_area, j.Nx, j.Ny, j.kern, j.scales, 0, j.nk, tw128,
+                     j.spec, nullptr);
 }
 
 // grid (B), dynamic LDS 96 kRS float2: sums the kg partial intensities of
@@ -1947,6 +1998,22 @@ __global__ __launch_bounds__(256, 4) void k_synth_wide(const float2* __restrict_
   const int y0 = min(Lypad, rh * per), y1 = min(Lypad, y0 + per);
   synth_wide_strip(strip * kColsSynth, y0, y1, spec + (size_t)b * 64 * (4 * Nx + 1), Nx, Ny,
                    Lx, Ly, Efw, TYw, image + (size_t)b * Lx * Ly);
+}
+
+// Mixed sizes and kernel sets, half-sizes 16..31: one block per SynthJob, as
+// k_synth_jobs (the job's Ef and TYh carry the wide tables Efw and TYw).
+__global__ __launch_bounds__(256, 4) void k_synth_wide_jobs(
+    const klt::SynthJob* __restrict__ jobs, unsigned* __restrict__ strips_done,
+    unsigned long long* __restrict__ done_at) {
+  const klt::SynthJob j = jobs[blockIdx.x];
+  synth_wide_strip(j.x0, 0, j.Lypad, j.spec, j.Nx, j.Ny, j.Lx, j.Ly, j.Ef,
+                   reinterpret_cast<const float4*>(j.TYh), j.img);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&strips_done[j.image], 1u) + 1u == (unsigned)j.nstrips)
+      done_at[j.image] = wall_clock64();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -3692,15 +3759,19 @@ static int build_batch(klt_ctx* ctx, BatchGraph* bg, int nmask, const float* con
                        const int* Lx, const int* Ly, int nbank,
                        const klt_kernels* const* banks, float* const* images) {
   const int Nx = banks[0]->Nx, Ny = banks[0]->Ny, Wy = 2 * Ny + 1, Gx = 4 * Nx + 1;
+  // half-sizes 16..31: partials of 32 columns, spectra of 64 rows, the wide
+  // passes and the 128-point coarse stage
+  bg->wide = !socs_fused_supported(banks[0]);
+  const int ldb = bg->wide ? 32 : 16, rows = bg->wide ? 64 : 32;
   std::vector<size_t> moff(nmask);
   size_t mtot = 0;
   for (int m = 0; m < nmask; ++m) {
     moff[m] = mtot;
-    mtot += (size_t)((Ly[m] + kRowsBand - 1) / kRowsBand) * Wy * 16;
+    mtot += (size_t)((Ly[m] + kRowsBand - 1) / kRowsBand) * Wy * ldb;
   }
   bg->nimg = nmask * nbank;
   KLT_TRY(bg->mpart.ensure(mtot * sizeof(float2)));
-  KLT_TRY(bg->spec.ensure((size_t)bg->nimg * 32 * Gx * sizeof(float2)));
+  KLT_TRY(bg->spec.ensure((size_t)bg->nimg * rows * Gx * sizeof(float2)));
   KLT_TRY(bg->strips_done.ensure((size_t)bg->nimg * sizeof(unsigned)));
   KLT_TRY(bg->done_at.ensure((size_t)bg->nimg * sizeof(unsigned long long)));
   KLT_TRY(bg->t0.ensure(sizeof(unsigned long long)));
@@ -3714,7 +3785,7 @@ static int build_batch(klt_ctx* ctx, BatchGraph* bg, int nmask, const float* con
     KLT_TRY(get_socs_plan(ctx, Lx[m], Ly[m], Nx, Ny, &p));
     FusedTables* ft = nullptr;
     KLT_TRY(get_fused_tables(ctx, p, &ft));
-    tw64 = ft->tw64.as<float2>();
+    tw64 = bg->wide ? ft->tw128.as<float2>() : ft->tw64.as<float2>();
     bg->full = bg->full && Lx[m] % 32 == 0 && (reinterpret_cast<uintptr_t>(masks[m]) & 15) == 0;
     const int nblk = (Ly[m] + kRowsBand - 1) / kRowsBand;
     float2* mp = bg->mpart.as<float2>() + moff[m];
@@ -3724,14 +3795,18 @@ static int build_batch(klt_ctx* ctx, BatchGraph* bg, int nmask, const float* con
     const int nstrips = (Lx[m] + kColsSynth - 1) / kColsSynth;
     for (int s = 0; s < nbank; ++s) {
       const int img = m * nbank + s;
-      float2* sp = bg->spec.as<float2>() + (size_t)img * 32 * Gx;
+      float2* sp = bg->spec.as<float2>() + (size_t)img * rows * Gx;
       mj.push_back(MidJob{mp, banks[s]->kern.as<float2>(), banks[s]->scales.as<float>(), sp,
                           nblk, Nx, Ny, banks[s]->nk,
                           1.0f / ((float)Lx[m] * (float)Ly[m]), {0, 0, 0}});
       for (int st = 0; st < nstrips; ++st)
-        sj.push_back(SynthJob{sp, ft->Ef.as<float4>(), ft->TYh.as<_Float16>(),
-                              ft->TYl.as<_Float16>(), images[img], Nx, Lx[m], Ly[m], ft->Lypad,
-                              st * kColsSynth, img, nstrips, 0});
+        sj.push_back(bg->wide
+                         ? SynthJob{sp, ft->Efw.as<float4>(), ft->TYw.as<_Float16>(), nullptr,
+                                    images[img], Nx, Lx[m], Ly[m], ft->Lypad, st * kColsSynth,
+                                    img, nstrips, Ny}
+                         : SynthJob{sp, ft->Ef.as<float4>(), ft->TYh.as<_Float16>(),
+                                    ft->TYl.as<_Float16>(), images[img], Nx, Lx[m], Ly[m],
+                                    ft->Lypad, st * kColsSynth, img, nstrips, Ny});
     }
   }
   bg->nband = (int)bj.size();
@@ -3747,6 +3822,11 @@ static int build_batch(klt_ctx* ctx, BatchGraph* bg, int nmask, const float* con
   KLT_HIP(hipMemcpy(bg->synth_jobs.ptr, sj.data(), sj.size() * sizeof(SynthJob),
                     hipMemcpyHostToDevice));
   bg->spec_tw = tw64;
+  if (bg->wide) {
+    bg->lds_mid = (96 * kRS + (2 * Nx + 1) * Wy) * (int)sizeof(float2);
+    KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_socs_mid128_jobs),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, bg->lds_mid));
+  }
   return KLT_OK;
 }
 
@@ -3755,6 +3835,20 @@ static void launch_batch(klt_ctx* ctx, const BatchGraph* bg) {
   hipLaunchKernelGGL(k_batch_begin, dim3(std::max(1, std::min(64, (bg->nimg + 255) / 256))),
                      dim3(256), 0, ctx->stream, bg->strips_done.as<unsigned>(), bg->nimg,
                      bg->t0.as<unsigned long long>());
+  if (bg->wide) {
+    if (bg->full)
+      hipLaunchKernelGGL(k_band_jobs_wide<true>, dim3(bg->nband), dim3(256), 0, ctx->stream,
+                         bg->band_jobs.as<BandJob>());
+    else
+      hipLaunchKernelGGL(k_band_jobs_wide<false>, dim3(bg->nband), dim3(256), 0, ctx->stream,
+                         bg->band_jobs.as<BandJob>());
+    hipLaunchKernelGGL(k_socs_mid128_jobs, dim3(bg->nmid), dim3(256), bg->lds_mid, ctx->stream,
+                       bg->mid_jobs.as<MidJob>(), bg->spec_tw);
+    hipLaunchKernelGGL(k_synth_wide_jobs, dim3(bg->nsynth), dim3(256), 0, ctx->stream,
+                       bg->synth_jobs.as<SynthJob>(), bg->strips_done.as<unsigned>(),
+                       bg->done_at.as<unsigned long long>());
+    return;
+  }
   if (bg->full)
     hipLaunchKernelGGL(k_band_jobs<true>, dim3(bg->nband), dim3(256), 0, ctx->stream,
                        bg->band_jobs.as<BandJob>());
@@ -4477,8 +4571,8 @@ int klt_socs_forward_batch(klt_ctx* ctx, int nmask, const float* const* masks, c
       return klt::set_error(KLT_ERR_ARG,
                             "klt_socs_forward_batch: kernel sets of different sizes");
   }
-  if (!klt::socs_fused_supported(banks[0]))
-    return klt::set_error(KLT_ERR_ARG, "klt_socs_forward_batch: kernel half-size above 15");
+  if (!klt::socs_fused_supported(banks[0]) && !klt::socs_wide_supported(banks[0]))
+    return klt::set_error(KLT_ERR_ARG, "klt_socs_forward_batch: kernel half-size above 31");
   for (int m = 0; m < nmask; ++m) {
     if (!masks[m] || Lx[m] <= 0 || Ly[m] <= 0)
       return klt::set_error(KLT_ERR_ARG, "klt_socs_forward_batch: invalid mask " +

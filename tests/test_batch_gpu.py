@@ -158,3 +158,41 @@ def test_batch_captured_by_caller_survives_later_batches(ctx):
         torch.cuda.synchronize()
         ctx.set_stream(None)
         ctx.sync()
+
+
+WIDE_SIZES = [(256, 256), (384, 200), (130, 300), (512, 512)]
+
+
+def test_batch_wide_half_size(ctx):
+    """Kernel half-size 16 (the wide passes: 64 twiddle rows, 128-point coarse
+    stage, K = 128 synthesis) in a mixed batch of two kernel sets: every image
+    against the oracle (1e-5) and the single-size calls (1e-5: the single calls
+    may split the band pass over column ranges and the coarse stage over kernel
+    groups), then one graph replay over refilled buffers."""
+    import torch
+    print("case batch wide N=16 sizes", WIDE_SIZES, flush=True)
+    banks = [_bank(ctx, 40 + s, nk=5, n=16) for s in range(2)]
+    masks = _masks(torch, WIDE_SIZES, 7)
+    imgs = None
+    for rep in range(2):
+        if rep:
+            rng = np.random.default_rng(8)
+            for mk in masks:
+                mk.copy_(torch.from_numpy((rng.random(tuple(mk.shape)) > 0.4).astype(np.float32)))
+        # (the second call, same buffers, replays the recorded graph)
+        imgs = ctx.socs_forward_batch([b for b, _ in banks], masks, imgs)
+        torch.cuda.synchronize()
+        ctx.sync()
+        for m, mk in enumerate(masks):
+            mh = mk.cpu().numpy()
+            for s, (bank, (kr, ki, sc)) in enumerate(banks):
+                got = imgs[m * 2 + s].cpu().numpy()
+                single = ctx.socs_forward(bank, mk.clone())
+                ctx.sync()
+                assert relerr(got, single.cpu().numpy()) < 1e-5, (rep, WIDE_SIZES[m], s)
+                if s == 0:
+                    assert relerr(got, O.socs_image(mh, kr, ki, sc, 16, 16)) < 1e-5, (rep, m)
+    lat = ctx.batch_latency()
+    assert lat.shape == (len(WIDE_SIZES) * 2,) and np.all(lat > 0)
+    for b, _ in banks:
+        b.close()
