@@ -10,6 +10,8 @@ checks:
   * the reference's own shipped 33 x 33 kernels (N = 16, three kernels) on
     two 1024^2 Manhattan masks, each image against the oracle;
   * N = 31 (the largest half-size) on unequal, odd sizes, and Nx != Ny;
+  * the smallest accepted period (4N+1) for the forward, the adjoint and the
+    product with a given dL/dI; 600 masks in sub-batches of 512;
   * every pinned tile height / split factor gives the same image to 1e-6
     (the splits only reorder FP32 sums; the row ranges write disjoint rows),
     and the first matches the oracle.
@@ -130,4 +132,56 @@ def test_wide_pinned_splits(ctx):
     finally:
         ctx.set_splits(0, 0, 0)
         ctx.set_band_rows(0)
+    bank.close()
+
+
+@pytest.mark.parametrize("N", [16, 31])
+def test_wide_minimum_size(ctx, N):
+    """The smallest period the reference accepts, 4N+1 (klitho_socs.cpp:164-169):
+    one partly filled 128-column strip and band row block; forward, adjoint and
+    the product with a given dL/dI against the oracle."""
+    L = 4 * N + 1
+    print(f"case wide minimum size N={N} L={L}", flush=True)
+    rng = np.random.default_rng(L)
+    kr, ki, sc = _random_bank(rng, N, N, 4)
+    bank = ctx.kernels(kr, ki, sc, 2 * N + 1, 2 * N + 1)
+    t = (rng.random((2, L, L)) > 0.6).astype(np.float32)
+    m = np.clip(t + rng.uniform(-0.1, 0.1, t.shape), 0, 1).astype(np.float32)
+    img = ctx.socs_forward(bank, m)
+    a, thr = 4.0 / float(np.std(img[0])), float(np.median(img[0]))
+    lo, g, _ = klt.socs_backward(ctx, bank, m, t, a=a, thr=thr)
+    G = rng.standard_normal(m.shape).astype(np.float32)
+    gv = klt.socs_vjp(ctx, bank, m, G)
+    for i in range(2):
+        assert relerr(img[i], O.socs_image(m[i], kr, ki, sc, N, N)) < TOL
+        Lr, _, gr = O.socs_loss_grad(m[i].astype(np.float64), t[i], kr, ki, sc, N, N, a=a,
+                                     thr=thr)
+        assert relerr(g[i], gr) < 1e-4
+        assert abs(lo[i] - Lr) < 1e-5 * Lr
+        # <m, dL/dm> = 2 sum G I for the product with G (I quadratic in m)
+        I = O.socs_image_d(m[i].astype(np.float64), kr, ki, sc, N, N)
+        lhs, rhs = float(np.sum(m[i] * gv[i])), float(2 * np.sum(G[i] * I))
+        assert abs(lhs - rhs) < 1e-4 * float(np.sum(np.abs(G[i] * I)))
+    bank.close()
+
+
+def test_wide_sub_batches(ctx):
+    """600 masks of 72 x 80 at N = 16: the forward and the adjoint run in
+    sub-batches of 512 that share the context's scratch; images and gradients
+    on both sides of the boundary against the oracle."""
+    print("case wide sub-batches 600 masks", flush=True)
+    rng = np.random.default_rng(600)
+    kr, ki, sc = _random_bank(rng, 16, 16, 3)
+    bank = ctx.kernels(kr, ki, sc, 33, 33)
+    t = (rng.random((600, 72, 80)) > 0.6).astype(np.float32)
+    m = np.clip(t + rng.uniform(-0.1, 0.1, t.shape), 0, 1).astype(np.float32)
+    img = ctx.socs_forward(bank, m)
+    a, thr = 4.0 / float(np.std(img[0])), float(np.median(img[0]))
+    lo, g, _ = klt.socs_backward(ctx, bank, m, t, a=a, thr=thr)
+    for i in (0, 511, 512, 599):
+        assert relerr(img[i], O.socs_image(m[i], kr, ki, sc, 16, 16)) < TOL, i
+        Lr, _, gr = O.socs_loss_grad(m[i].astype(np.float64), t[i], kr, ki, sc, 16, 16, a=a,
+                                     thr=thr)
+        assert relerr(g[i], gr) < 1e-4, i
+        assert abs(lo[i] - Lr) < 1e-5 * Lr, i
     bank.close()
