@@ -139,6 +139,48 @@ def test_tcc_then_socs_end_to_end(workdir, NA, df, stype):
 
 
 @pytest.mark.gpu
+def test_reference_sample_workflow(workdir):
+    """The reference's own sample (K-Litho-TCC/test_tcc.sh:4-49): 2048^2
+    period, 1024^2 line/space mask 128/128 (the script's LineSpace values; its
+    Import file is not shipped), annular 101 grid 0.6 / 0.9, NA 0.8, defocus
+    0.2, 3 kernels: N = 16, 33 x 33.  klitho_tcc's Hopkins image and
+    eigenvalues against the oracle's, then klitho_socs on the written kernels
+    (the matrix-core path for N = 16, and the plain path) against the oracle's
+    SOCS image of the same kernels."""
+    print("case cli reference sample workflow 2048^2 N=16", flush=True)
+    L, ms = 2048, 1024
+    r = _run(TCC, _tcc_args(L=L, ms=ms, lw=128, sw=128, n=101, NA=0.8, df=0.2, nk=3), workdir)
+    assert r.returncode == 0, r.stderr + r.stdout
+    assert "Nx: 16, Ny: 16" in r.stdout
+    src = O.source("Annular", 101, inner=0.6, outer=0.9)
+    sig = min(O.outer_sigma(src), 1.0)
+    inner = np.zeros(ms * ms, np.float32)
+    O.lib().or_mask_linespace(inner, ms, ms, 128, 128, 0)
+    mask = np.zeros(L * L, np.float32)
+    O.lib().or_mask_place(inner, ms, ms, mask, L, L, 1.0)
+    mask = mask.reshape(L, L)
+    t = O.tcc(src, sig, 0.8, 0.2, L, L, 16, 16)
+    img = np.fromfile(workdir / "out" / "image.bin", np.float32).reshape(L, L)
+    assert relerr(img, O.hopkins_image(mask, t, 16, 16)) < 1e-5
+    sx, sy, sc = klt.read_kernel_info(str(workdir / "out" / "kernels" / "kernel_info.txt"))
+    assert (sx, sy, len(sc)) == (33, 33, 3)
+    wo, _ = O.eigh_topk(t, 3)
+    assert np.abs(sc - wo).max() < 1e-5 * wo[0]  # 6 printed digits
+    kr = np.stack([np.fromfile(workdir / "out" / "kernels" / f"krn_{i}_r.bin", np.float32)
+                   for i in range(3)])
+    ki = np.stack([np.fromfile(workdir / "out" / "kernels" / f"krn_{i}_i.bin", np.float32)
+                   for i in range(3)])
+    ref = O.socs_image(mask, kr, ki, sc, 16, 16)
+    for path in ("fused", "plain"):
+        os.remove(workdir / "out" / "image.bin")
+        r = _run(SOCS, [L, L, ms, ms, "LineSpace", 128, 128, 0, "-", "./out/kernels/"], workdir,
+                 {"KLT_SOCS_PATH": path})
+        assert r.returncode == 0, r.stderr + r.stdout
+        simg = np.fromfile(workdir / "out" / "image.bin", np.float32).reshape(L, L)
+        assert relerr(simg, ref) < 1e-5, path
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("path", ["default", "plain"])
 def test_socs_reference_kernels_text_mask(workdir, path):
     """klitho_socs on the reference's own shipped kernel set (33 x 33, three
