@@ -80,7 +80,7 @@ struct FusedTables {
 // Kpad columns, twy the [Ly][ldtw] column twiddles, TYh / TYl [Lypad][64], Ef
 // the synthesis column fragments, tw64 e^{+2 pi i j/64}) and the intermediates
 // of nb images: qpart [nb][nblk * ks][2Ny+1][16], spec [nb][32][4Nx+1], and
-// for kg > 1 ipart [nb][kg][64 * 64] and gsum [nb][64 * 64].
+// for kg > 1 ipart [nb][kg][64 * 64].
 struct FusedView {
   int Kpad, Lypad, ldtw;
   const float4* Wf;
@@ -94,7 +94,6 @@ struct FusedScratch {
   float2* qpart;
   float2* spec;
   float* ipart;
-  float* gsum;
 };
 
 // Work descriptors of the mixed-size batch (klt_socs_forward_batch): one per
@@ -200,7 +199,6 @@ struct klt_ctx {
   klt::DevBuf tw64;                           // e^{+2 pi i j/64}, j < 64
   klt::DevBuf ipart, gam_part;                // kernel-group partials of the coarse stages
   klt::DevBuf mhat, gfull, gcoarse;           // adjoint: per-image M^, G^ and coarse-grid g_c
-  klt::DevBuf gsum;                           // kernel-group partials summed per image
   klt::DevBuf hspec;                          // Hopkins spectra of the source gradient
   klt::DevBuf sub_ws;                         // blocks of the subspace eigensolver
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::SocsPlan>> socs_plans;

@@ -753,19 +753,6 @@ __global__ __launch_bounds__(256) void k_socs_mid(const float2* __restrict__ mpa
                    kg > 1 ? ipart + ((size_t)b * kg + grp) * kM * kM : nullptr);
 }
 
-// grid (ceil(n / 256), B), one thread per element: out[b][e] = sum over
-// j < kg (in order) of part[b][j][e], the kernel-group partials of the coarse
-// stages, ahead of their per-image finishing transforms.
-__global__ __launch_bounds__(256) void k_sum_groups(const float* __restrict__ part, int kg,
-                                                    int n, float* __restrict__ out) {
-  const int b = blockIdx.y, e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= n) return;
-  const float* p = part + (size_t)b * kg * n + e;
-  float s = 0.f;
-  for (int j = 0; j < kg; ++j) s += p[(size_t)j * n];
-  out[(size_t)b * n + e] = s;
-}
-
 // Sums the kg partial coarse intensities of each image and transforms them
 // to the intensity spectrum [B][32][4Nx+1].
 __global__ __launch_bounds__(256) void k_socs_mid_fin(const float* __restrict__ ipart, int kg,
@@ -780,6 +767,8 @@ __global__ __launch_bounds__(256) void k_socs_mid_fin(const float* __restrict__ 
   for (int q = 0; q < 2; ++q)
 #pragma unroll
     for (int k1 = 0; k1 < 8; ++k1) acc[q][k1] = 0.f;
+  // (unrolled: four groups' loads issue together, the sums keep their order)
+#pragma unroll 4
   for (int j = 0; j < kg; ++j) {
     const float* src = ipart + ((size_t)b * kg + j) * kM * kM + tid * 16;
 #pragma unroll
@@ -1109,6 +1098,7 @@ __global__ __launch_bounds__(256) void k_socs_mid_bwd_fin(const float2* __restri
   const int b = blockIdx.x, T = (2 * Nx + 1) * (2 * Ny + 1);
   for (int i = threadIdx.x; i < T; i += 256) {
     float2 v = make_float2(0.f, 0.f);
+#pragma unroll 8
     for (int j = 0; j < kg; ++j) {
       const float2 p = gam_part[((size_t)b * kg + j) * T + i];
       v.x += p.x;
@@ -3589,12 +3579,9 @@ int socs_fused_launch(klt_ctx* ctx, const klt_kernels* kb, const float* masks, i
   KLT_LAUNCH(ctx, "k_socs_mid", k_socs_mid, dim3(kg, nb), dim3(256), 0, w.qpart, nblk * ks, Nx,
              Ny, inv_area, kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk, v.tw64,
              w.spec, w.ipart);
-  if (kg > 1) {
-    KLT_LAUNCH(ctx, "k_sum_groups", k_sum_groups, dim3(kM * kM / 256, nb), dim3(256), 0, w.ipart,
-               kg, kM * kM, w.gsum);
-    KLT_LAUNCH(ctx, "k_socs_mid_fin", k_socs_mid_fin, dim3(nb), dim3(256), 0, w.gsum, 1, Nx, Ny,
+  if (kg > 1)
+    KLT_LAUNCH(ctx, "k_socs_mid_fin", k_socs_mid_fin, dim3(nb), dim3(256), 0, w.ipart, kg, Nx, Ny,
                v.tw64, w.spec);
-  }
   KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma, dim3(nstrips * rs, nb), dim3(256), 0, w.spec, Nx,
              Lx, Ly, v.Lypad, rs, v.Ef, v.TYh, v.TYl, images);
   KLT_HIP(hipGetLastError());
@@ -3605,7 +3592,7 @@ size_t fused_scratch_bytes(int Lx, int Ly, int Nx, int Ny, int ks, int kg) {
   const int nblk = (Ly + kRowsBand - 1) / kRowsBand;
   size_t b = (size_t)nblk * ks * (2 * Ny + 1) * 16 * sizeof(float2);  // qpart
   b += (size_t)32 * (4 * Nx + 1) * sizeof(float2);                     // spec
-  if (kg > 1) b += (size_t)(kg + 1) * kM * kM * sizeof(float);         // ipart, gsum
+  if (kg > 1) b += (size_t)kg * kM * kM * sizeof(float);               // ipart
   (void)Lx;
   return b;
 }
@@ -3637,12 +3624,11 @@ int socs_forward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks, 
   KLT_TRY(ctx->qpart.ensure((size_t)SB * nblk64 * kMaxKs * Wy * 16 * sizeof(float2)));
   KLT_TRY(ctx->spec.ensure((size_t)SB * 32 * Gx * sizeof(float2)));
   KLT_TRY(ctx->ipart.ensure((size_t)kGroupSlots * kM * kM * sizeof(float)));
-  KLT_TRY(ctx->gsum.ensure((size_t)SB * kM * kM * sizeof(float)));
   const FusedView view{ft->Kpad, ft->Lypad, p->NP, ft->Wf.as<float4>(), p->twy_in.as<float2>(),
                        ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), ft->Ef.as<float4>(),
                        ft->tw64.as<float2>()};
   const FusedScratch work{ctx->qpart.as<float2>(), ctx->spec.as<float2>(),
-                          ctx->ipart.as<float>(), ctx->gsum.as<float>()};
+                          ctx->ipart.as<float>()};
   for (int b0 = 0; b0 < B; b0 += SB) {
     const int nb = std::min(SB, B - b0);
     const float* m = masks + (size_t)b0 * Lx * Ly;
@@ -4168,7 +4154,6 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
   KLT_TRY(ctx->mhat.ensure((size_t)SB * (2 * Nx + 1) * Wy * sizeof(float2)));
   KLT_TRY(ctx->gcoarse.ensure((size_t)SB * kM * kM * sizeof(float)));
   KLT_TRY(ctx->gfull.ensure((size_t)SB * Gx * Gy * sizeof(float2)));
-  KLT_TRY(ctx->gsum.ensure((size_t)SB * std::max(kM * kM, 2 * (2 * Nx + 1) * Wy) * sizeof(float)));
   const float inv_area = 1.0f / ((float)Lx * (float)Ly);
   const int dcx = (Lx + 1) / 2 - Lx / 2, dcy = (Ly + 1) / 2 - Ly / 2;
   // whole 16-byte target vectors: every image of the batch starts aligned
@@ -4192,13 +4177,10 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
       KLT_LAUNCH(ctx, "k_socs_mid", k_socs_mid_full, dim3(c.kg, nb), dim3(256), 0,
                  ctx->mhat.as<float2>(), Nx, Ny, kb->kern.as<float2>(), kb->scales.as<float>(),
                  kb->nk, ft->tw64.as<float2>(), ctx->spec.as<float2>(), ctx->ipart.as<float>());
-      if (c.kg > 1) {
-        KLT_LAUNCH(ctx, "k_sum_groups", k_sum_groups, dim3(kM * kM / 256, nb), dim3(256), 0,
-                   ctx->ipart.as<float>(), c.kg, kM * kM, ctx->gsum.as<float>());
+      if (c.kg > 1)
         KLT_LAUNCH(ctx, "k_socs_mid_fin", k_socs_mid_fin, dim3(nb), dim3(256), 0,
-                   ctx->gsum.as<float>(), 1, Nx, Ny, ft->tw64.as<float2>(),
+                   ctx->ipart.as<float>(), c.kg, Nx, Ny, ft->tw64.as<float2>(),
                    ctx->spec.as<float2>());
-      }
       // the image tile by tile, its resist gradient G and G's band partials;
       // the image itself is written only when the caller asks for it
       if (tvec)
@@ -4225,15 +4207,10 @@ int socs_backward_fused(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
                  ctx->mhat.as<float2>(), ctx->gcoarse.as<float>(), Nx, Ny,
                  kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk, ft->tw64.as<float2>(),
                  dcx, dcy, Lx, Ly, ctx->gspec.as<float2>(), ctx->gam_part.as<float2>());
-      if (c.kg > 1) {
-        const int n2 = 2 * (2 * Nx + 1) * Wy;  // Gam entries as floats
-        KLT_LAUNCH(ctx, "k_sum_groups", k_sum_groups, dim3((n2 + 255) / 256, nb), dim3(256), 0,
-                   reinterpret_cast<const float*>(ctx->gam_part.as<float2>()), c.kg, n2,
-                   ctx->gsum.as<float>());
+      if (c.kg > 1)
         KLT_LAUNCH(ctx, "k_socs_mid_bwd_fin", k_socs_mid_bwd_fin, dim3(nb), dim3(256), 0,
-                   reinterpret_cast<const float2*>(ctx->gsum.as<float>()), 1, Nx, Ny, dcx,
-                   dcy, Lx, Ly, ctx->gspec.as<float2>(), 32);
-      }
+                   ctx->gam_part.as<float2>(), c.kg, Nx, Ny, dcx, dcy, Lx, Ly,
+                   ctx->gspec.as<float2>(), 32);
       KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma, dim3(nstrips * rs, nb), dim3(256), 0,
                  ctx->gspec.as<float2>(), Nx, Lx, Ly, ft->Lypad, rs, ft->Ef.as<float4>(),
                  ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), grads + off);
@@ -4371,7 +4348,6 @@ int socs_vjp_device(klt_ctx* ctx, const klt_kernels* kb, const float* masks, con
   KLT_TRY(ctx->mhat.ensure((size_t)SB * T * sizeof(float2)));
   KLT_TRY(ctx->gcoarse.ensure((size_t)SB * M2 * sizeof(float)));
   KLT_TRY(ctx->gfull.ensure((size_t)SB * Gx * Gy * sizeof(float2)));
-  KLT_TRY(ctx->gsum.ensure((size_t)SB * 2 * T * sizeof(float)));
   KLT_TRY(ctx->vjp_sum.ensure((size_t)SB * sizeof(float)));
   const int lds_fin = 96 * kRS * (int)sizeof(float2);
   const int lds_mid = lds_fin + T * (int)sizeof(float2);
@@ -4455,15 +4431,10 @@ int socs_vjp_device(klt_ctx* ctx, const klt_kernels* kb, const float* masks, con
                    kb->kern.as<float2>(), kb->scales.as<float>(), kb->nk,
                    ft->tw64.as<float2>(), dcx, dcy, Lx, Ly, ctx->gspec.as<float2>(),
                    ctx->gam_part.as<float2>());
-        if (c.kg > 1) {
-          const int n2 = 2 * T;  // Gam entries as floats
-          KLT_LAUNCH(ctx, "k_sum_groups", k_sum_groups, dim3((n2 + 255) / 256, nb), dim3(256),
-                     0, reinterpret_cast<const float*>(ctx->gam_part.as<float2>()), c.kg, n2,
-                     ctx->gsum.as<float>());
+        if (c.kg > 1)
           KLT_LAUNCH(ctx, "k_socs_mid_bwd_fin", k_socs_mid_bwd_fin, dim3(nb), dim3(256), 0,
-                     reinterpret_cast<const float2*>(ctx->gsum.as<float>()), 1, Nx, Ny, dcx,
-                     dcy, Lx, Ly, ctx->gspec.as<float2>(), 32);
-        }
+                     ctx->gam_part.as<float2>(), c.kg, Nx, Ny, dcx, dcy, Lx, Ly,
+                     ctx->gspec.as<float2>(), 32);
         KLT_LAUNCH(ctx, "k_synth_mfma", k_synth_mfma, dim3(nstrips * c.rs, nb), dim3(256), 0,
                    ctx->gspec.as<float2>(), Nx, Lx, Ly, ft->Lypad, c.rs, ft->Ef.as<float4>(),
                    ft->TYh.as<_Float16>(), ft->TYl.as<_Float16>(), grads + off);

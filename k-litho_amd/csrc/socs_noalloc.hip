@@ -225,10 +225,7 @@ int klt_socs_forward_noalloc(klt_ctx* ctx, const klt_kernels* kb, const float* m
   w.qpart = reinterpret_cast<float2*>(
       carve((size_t)SB * nblk * ks * (2 * Ny + 1) * 16 * sizeof(float2)));
   w.spec = reinterpret_cast<float2*>(carve((size_t)SB * 32 * (4 * Nx + 1) * sizeof(float2)));
-  if (kg > 1) {
-    w.ipart = reinterpret_cast<float*>(carve((size_t)SB * kg * 64 * 64 * sizeof(float)));
-    w.gsum = reinterpret_cast<float*>(carve((size_t)SB * 64 * 64 * sizeof(float)));
-  }
+  if (kg > 1) w.ipart = reinterpret_cast<float*>(carve((size_t)SB * kg * 64 * 64 * sizeof(float)));
   if (off > kScratchBytes)  // (SB is chosen so that this cannot happen)
     return klt::set_error(KLT_ERR_ARG, "klt_socs_forward_noalloc: intermediates exceed the area");
   KLT_LAUNCH(ctx, "k_noalloc_tables", k_noalloc_tables, dim3(512), dim3(256), 0, t);
