@@ -39,7 +39,11 @@ constexpr int kM = 64;        // coarse grid
 constexpr int kRowsBand = 128;  // mask rows per band block
 constexpr int kColsSynth = 128; // image columns per synthesis block
 constexpr int kPStride = 72;    // LDS row stride (halves) of the P operand
-constexpr int kBandChunk = 4;   // 32-column steps per LDS twiddle chunk
+constexpr int kBandChunk = 4;   // 32-column steps per column range unit (128 columns)
+// 32-column steps per LDS twiddle chunk of the band kernel: four with 32
+// twiddle rows (32 KB double-buffered), two with 64 (the same 32 KB, so that
+// the wide band, too, fits beside three or four blocks on a CU)
+__host__ __device__ constexpr int band_chunk(int ncb) { return ncb > 2 ? 2 : kBandChunk; }
 constexpr int kGroupSlots = 2048;  // (image, kernel group) partials of one launch
 constexpr int kMaxKs = 8;        // band column ranges per row block, at most
 constexpr int kMaxRs = 16;       // synthesis row ranges per strip, at most
@@ -142,20 +146,20 @@ __device__ __forceinline__ void band_block(
     const float* __restrict__ in1, float a_sig, float thr, int Lx, int Ly, int Kpad,
     const float4* Wf, const float2* __restrict__ twy, int ldtw, int Bx, int By, int ldo,
     float2* __restrict__ out_img, float* __restrict__ lpart_img) {
-  constexpr int C = kBandChunk;
+  constexpr int C = band_chunk(NCB);
   constexpr int STEP_F4 = NCB * 2 * 64;    // 16-byte fragments per step
   constexpr int CHUNK_F4 = C * STEP_F4;    // per chunk
   constexpr int PER_T = CHUNK_F4 / 256;    // per thread per chunk
   constexpr int PS = PER_T / C;            // per thread per step
   constexpr int NT = NCB / 2;        // 16-row tiles of |ny| in the epilogue
   constexpr int PST = 16 * NCB + 1;  // LDS row stride of the epilogue's P tiles
-  constexpr int EPI_BYTES = 4 * 2 * 16 * NT * PST * 4;
+  constexpr int EPI_BYTES = 2 * 16 * NT * PST * 4;
   constexpr int W_BYTES = 2 * CHUNK_F4 * 16;
   // the twiddle chunks (main loop) and the epilogue's P tiles share the LDS
   __shared__ __attribute__((aligned(16))) char smem[EPI_BYTES > W_BYTES ? EPI_BYTES : W_BYTES];
   __shared__ float red[4];
-  // [wave][re / im twiddle part][|ny|][n]
-  auto sP = reinterpret_cast<float(*)[2][16 * NT][PST]>(smem);
+  // [re / im twiddle part][|ny|][n], summed over the waves
+  auto sP = reinterpret_cast<float(*)[16 * NT][PST]>(smem);
   float4* sW = reinterpret_cast<float4*>(smem);
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -179,8 +183,9 @@ __device__ __forceinline__ void band_block(
   // k + 32 DEPTH, then issues those rows, then runs the products.
   // (one row tile a wave streams twice as many steps ahead: the same bytes in
   // flight)
-  constexpr int DEPTH = (RESIST ? 1 : 2) * (RT == 1 ? 2 : 1);
-  static_assert(kBandChunk % DEPTH == 0, "prefetch slots cycle within a chunk");
+  constexpr int DEPTH_RT = (RESIST ? 1 : 2) * (RT == 1 ? 2 : 1);
+  constexpr int DEPTH = DEPTH_RT < C ? DEPTH_RT : C;
+  static_assert(C % DEPTH == 0, "prefetch slots cycle within a chunk");
   float pa[DEPTH][8], pb[DEPTH][8], pta[DEPTH][8], ptb[DEPTH][8];
   auto fetch_rows = [&](int k, float* a8, float* b8, float* ta8, float* tb8) {
     if constexpr (FULL) {
@@ -340,6 +345,7 @@ __device__ __forceinline__ void band_block(
       for (int r = 0; r < 4; ++r) v[r] = ldexpf(acc[a][c][r], sh);
       split4(v, rh[a][c], rl[a][c]);
     }
+  float4_t pq[NT][NCB][2];  // [|ny| tile][column block][re / im twiddle part]
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) {
     half4_t th[RT][2], tl[RT][2];  // [row tile][re / im]
@@ -364,15 +370,34 @@ __device__ __forceinline__ void band_block(
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = nt * 16 + (lane >> 4) * 4 + r;
-        sP[wave][0][row][c * 16 + r16] = ldexpf(p0[r], -sh - 12);
-        sP[wave][1][row][c * 16 + r16] = ldexpf(p1[r], -sh - 12);
+        pq[nt][c][0][r] = ldexpf(p0[r], -sh - 12);
+        pq[nt][c][1][r] = ldexpf(p1[r], -sh - 12);
       }
     }
   }
-  __syncthreads();
+  // the waves add their tiles into one LDS copy in turn (wave 0 first: the
+  // sums' order, and so their rounding, is fixed)
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    if (wave == w) {
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int c = 0; c < NCB; ++c)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = nt * 16 + (lane >> 4) * 4 + r;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              float& d = sP[h][row][c * 16 + r16];
+              d = w == 0 ? pq[nt][c][h][r] : d + pq[nt][c][h][r];
+            }
+          }
+    }
+    __syncthreads();
+  }
   if (RESIST && tid == 0) lpart_img[pidx] = red[0] + red[1] + red[2] + red[3];
-  // sum the waves (fixed order): with q0 = sum Re(tw) R and q1 = sum Im(tw) R,
+  // with q0 = sum Re(tw) R and q1 = sum Im(tw) R,
   // tw R = (q0.re - q1.im) + i (q0.im + q1.re) for ny >= 0, and the conjugate
   // twiddle of -ny gives (q0.re + q1.im) + i (q0.im - q1.re).  Whole rows of
   // ldo = 8 NCB entries are written (zero past Bx): a partly written 128-byte
@@ -383,14 +408,8 @@ __device__ __forceinline__ void band_block(
     const int ny = nyi - By, any = ny < 0 ? -ny : ny;
     float2 val = make_float2(0.f, 0.f);
     if (nx <= Bx) {
-      float q0r = 0.f, q0i = 0.f, q1r = 0.f, q1i = 0.f;
-#pragma unroll
-      for (int w = 0; w < 4; ++w) {
-        q0r += sP[w][0][any][2 * nx];
-        q0i += sP[w][0][any][2 * nx + 1];
-        q1r += sP[w][1][any][2 * nx];
-        q1i += sP[w][1][any][2 * nx + 1];
-      }
+      const float q0r = sP[0][any][2 * nx], q0i = sP[0][any][2 * nx + 1];
+      const float q1r = sP[1][any][2 * nx], q1i = sP[1][any][2 * nx + 1];
       val = ny >= 0 ? make_float2(q0r - q1i, q0i + q1r) : make_float2(q0r + q1i, q0i - q1r);
     }
     out[o] = val;
@@ -399,7 +418,7 @@ __device__ __forceinline__ void band_block(
 // grid (parts * nimg): block w takes part w % parts (= row block * ks +
 // column range) of image w / parts.
 template <int NCB, bool RESIST, bool FULL, int RT = 2>
-__global__ __launch_bounds__(256, (RESIST || !FULL || NCB > 2) ? 2 : 4) void k_band_mfma(
+__global__ __launch_bounds__(256, !FULL ? 2 : (RESIST || NCB > 2) ? 3 : 4) void k_band_mfma(
     const float* __restrict__ in0, const float* __restrict__ in1, float a_sig, float thr,
     int Lx, int Ly, int Kpad, int ks, int parts, const float4* Wf,
     const float2* __restrict__ twy, int ldtw, int Bx, int By, int ldo,
@@ -408,8 +427,9 @@ __global__ __launch_bounds__(256, (RESIST || !FULL || NCB > 2) ? 2 : 4) void k_b
   const int b = blockIdx.x / parts, px = blockIdx.x - b * parts;
   const int blk = px / ks, kh = px - blk * ks;
   const int ch0 = min(nchunk, kh * per), ch1 = min(nchunk, ch0 + per);
+  constexpr int CU = kBandChunk / band_chunk(NCB);  // twiddle chunks per range unit
   const size_t ib = (size_t)b * Lx * Ly;
-  band_block<NCB, RESIST, FULL, RT>(blk, ch0, ch1, px, in0 + ib, RESIST ? in1 + ib : nullptr, a_sig,
+  band_block<NCB, RESIST, FULL, RT>(blk, CU * ch0, CU * ch1, px, in0 + ib, RESIST ? in1 + ib : nullptr, a_sig,
                                 thr, Lx, Ly, Kpad, Wf, twy, ldtw, Bx, By, ldo,
                                 out_part + (size_t)b * parts * (2 * By + 1) * ldo,
                                 RESIST ? lpart + (size_t)b * parts : nullptr);
@@ -428,9 +448,9 @@ __global__ __launch_bounds__(256, FULL ? 4 : 2) void k_band_jobs(
 // The same for kernel half-sizes 16..31: 64 twiddle rows, 32 columns a row
 // of partials.
 template <bool FULL>
-__global__ __launch_bounds__(256, 2) void k_band_jobs_wide(const klt::BandJob* __restrict__ jobs) {
+__global__ __launch_bounds__(256, 3) void k_band_jobs_wide(const klt::BandJob* __restrict__ jobs) {
   const klt::BandJob j = jobs[blockIdx.x];
-  band_block<4, false, FULL>(j.blk, 0, j.Kpad / (kBandChunk * 32), j.blk, j.in, nullptr, 0.f,
+  band_block<4, false, FULL>(j.blk, 0, j.Kpad / (band_chunk(4) * 32), j.blk, j.in, nullptr, 0.f,
                              0.f, j.Lx, j.Ly, j.Kpad, j.Wf, j.twy, j.ldtw, j.Nx, j.Ny, 32, j.out,
                              nullptr);
 }
