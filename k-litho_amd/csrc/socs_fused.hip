@@ -1532,6 +1532,33 @@ __device__ __forceinline__ void fft128_group(float2* v, const float2* tw, float2
   wave_sync();
 }
 
+// The same transform with the twiddles read from a table in LDS where they
+// are used (tab[m] = e^{+2 pi i m/128}), instead of 16 registers held across
+// calls.
+template <int S>
+__device__ __forceinline__ void fft128_group_tab(float2* v, const float2* tab, float2* xs, int t,
+                                                 bool active) {
+  if (active) {
+    dft16<S>(v);
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) {
+      const float2 tw = tab[(t * k2) & 127];
+      const float2 w = S > 0 ? tw : make_float2(tw.x, -tw.y);
+      xs[t * 17 + k2] = cmulf(v[k2], w);
+    }
+  }
+  wave_sync();
+  if (active) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int n1 = 0; n1 < 8; ++n1) v[8 * h + n1] = xs[n1 * 17 + t + 8 * h];
+    dft8<S>(v);
+    dft8<S>(v + 8);
+  }
+  wave_sync();
+}
+
 // Forward transform of a 128 x 128 coarse intensity held in registers (thread
 // (g, t) holds column ux = g + 32 q at rows uy = t + 8 h + 16 k1 in
 // acc[q][8 h + k1]) to its |g| <= 2N spectrum: spec [64][4Nx+1], rows gy in
@@ -1589,8 +1616,8 @@ __device__ __forceinline__ void coarse128_to_spec(const float (&acc)[4][16], flo
   for (int i = G2 * Gx + tid; i < 64 * Gx; i += 256) spec[i] = make_float2(0.f, 0.f);
 }
 
-// grid (kg, B), 256 threads = 32 groups of 8, dynamic LDS
-// (96 kRS + T) float2.  Block (j, b): the j-th of kg groups of the kernels
+// grid (kg, B), 256 threads = 32 groups of 8, dynamic LDS 64 kRS + 128
+// float2 with ipart, 96 kRS + 128 without.  Block (j, b): the j-th of kg groups of the kernels
 // over image b's normalised band Mfull [B][2Ny+1][2Nx+1] (k_band_assemble):
 //   B_k[r][ux] = sum_nx K_k[r][nx] M^[r][nx] e^{+2 pi i nx ux/128}
 //     (rows r < 2Ny+1; the exchange of each row's FFT in place in its row),
@@ -1602,10 +1629,15 @@ __device__ __forceinline__ void coarse128_to_spec(const float (&acc)[4][16], flo
 // zero for n2 = 4..11: written as constants so that the first stage drops
 // them.
 // Body of one block of the 128-point coarse stage: image band from Mband
-// (PARTS = false: the normalised band [2Ny+1][2Nx+1]; true: its nblk band
-// partials [nblk][2Ny+1][32], summed here and scaled by inv_area), kernels
-// [k0, k1); with ipart the block's partial intensity goes there (thread-major
-// [256][64]), else spec [64][4Nx+1] receives the intensity spectrum.
+// (PARTS = false: the normalised band [2Ny+1][2Nx+1], read where used; true:
+// its nblk band partials [nblk][2Ny+1][32], summed here into LDS and scaled by
+// inv_area), kernels [k0, k1); with ipart the block's partial intensity goes
+// there (thread-major [256][64]), else spec [64][4Nx+1] receives the intensity
+// spectrum.  Dynamic LDS: sB [64][kRS] (the row transforms; the column
+// transforms, once their inputs are in registers, exchange through its rows),
+// the twiddle table [128], then without ipart sX [32][kRS] for the final
+// transform, then with PARTS the band [T].  With ipart and !PARTS the block
+// needs 70 KB: two blocks a CU.
 template <bool PARTS>
 __device__ __forceinline__ void mid128_image(const float2* __restrict__ Mband, int nblk,
                                              float inv_area, int Nx, int Ny,
@@ -1616,15 +1648,13 @@ __device__ __forceinline__ void mid128_image(const float2* __restrict__ Mband, i
                                              float* __restrict__ ipart) {
   extern __shared__ float4 smem4[];
   float2* sB = reinterpret_cast<float2*>(smem4);  // [64][kRS]
-  float2* sX = sB + 64 * kRS;                     // [32][kRS]
-  float2* sM = sX + 32 * kRS;                     // [T]
+  float2* sTw = sB + 64 * kRS;                    // [128]
+  float2* sX = sTw + 128;                         // [32][kRS] (without ipart)
+  float2* sM = sX + 32 * kRS;                     // [T] (PARTS)
   const int tid = threadIdx.x, g = tid >> 3, t = tid & 7;
   const int Wx = 2 * Nx + 1, Wy = 2 * Ny + 1, T = Wx * Wy;
-  for (int i = tid; i < T; i += 256) {
-    if (!PARTS) {
-      sM[i] = Mband[i];
-      continue;
-    }
+  if (tid < 128) sTw[tid] = tw128[tid];
+  for (int i = tid; PARTS && i < T; i += 256) {
     const int nyi = i / Wx, nx = i - nyi * Wx - Nx, ny = nyi - Ny;
     const int r = nx >= 0 ? nyi : (-ny + Ny), c = nx >= 0 ? nx : -nx;
     float2 acc = make_float2(0.f, 0.f);
@@ -1635,9 +1665,6 @@ __device__ __forceinline__ void mid128_image(const float2* __restrict__ Mband, i
     }
     sM[i] = make_float2(acc.x * inv_area, (nx < 0 ? -acc.y : acc.y) * inv_area);
   }
-  float2 twi[16];
-#pragma unroll
-  for (int k2 = 0; k2 < 16; ++k2) twi[k2] = tw128[(t * k2) & 127];
   float acc[4][16];
 #pragma unroll
   for (int q = 0; q < 4; ++q)
@@ -1662,12 +1689,12 @@ __device__ __forceinline__ void mid128_image(const float2* __restrict__ Mband, i
           const int nx = n < 64 ? n : n - 128;
           if (nx >= -Nx && nx <= Nx) {
             const int i = r * Wx + nx + Nx;
-            v[n2] = cmulf(K[i], sM[i]);
+            v[n2] = cmulf(K[i], PARTS ? sM[i] : Mband[i]);
           }
         }
       }
       float2* xr = sB + (act ? r : 0) * kRS;
-      fft128_group<1>(v, twi, xr, t, act);
+      fft128_group_tab<1>(v, sTw, xr, t, act);
       if (act) {
 #pragma unroll
         for (int h = 0; h < 2; ++h)
@@ -1676,23 +1703,30 @@ __device__ __forceinline__ void mid128_image(const float2* __restrict__ Mband, i
       }
     }
     __syncthreads();
+    // the inputs of the group's four column transforms (rows |ny| <= Ny: n2 < 4
+    // or n2 >= 12) into registers, so that sB's rows can serve as the groups'
+    // exchange scratch
+    float2 cin[4][8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int n = t + 8 * (j < 4 ? j : j + 8);
+        const int ny = n < 64 ? n : n - 128;
+        cin[q][j] = ny >= -Ny && ny <= Ny ? sB[(ny + Ny) * kRS + g + 32 * q] : make_float2(0.f, 0.f);
+      }
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int ux = g + 32 * q;
       float2 v[16];
 #pragma unroll
-      for (int n2 = 0; n2 < 16; ++n2) {
-        v[n2] = make_float2(0.f, 0.f);
-        if (n2 >= 4 && n2 < 12) continue;
-        const int n = t + 8 * n2;
-        const int ny = n < 64 ? n : n - 128;
-        if (ny >= -Ny && ny <= Ny) v[n2] = sB[(ny + Ny) * kRS + ux];
-      }
-      fft128_group<1>(v, twi, sX + g * kRS, t, true);
+      for (int n2 = 0; n2 < 16; ++n2)
+        v[n2] = n2 < 4 ? cin[q][n2] : n2 >= 12 ? cin[q][n2 - 8] : make_float2(0.f, 0.f);
+      fft128_group_tab<1>(v, sTw, sB + g * kRS, t, true);
 #pragma unroll
       for (int i = 0; i < 16; ++i) acc[q][i] += lam * (v[i].x * v[i].x + v[i].y * v[i].y);
     }
-    __syncthreads();  // the column reads end before the next kernel's rows
+    __syncthreads();  // the exchanges end before the next kernel's rows
   }
   if (ipart) {
     float4* dst = reinterpret_cast<float4*>(ipart + tid * 64);
@@ -1704,10 +1738,13 @@ __device__ __forceinline__ void mid128_image(const float2* __restrict__ Mband, i
                                      acc[q][4 * i + 3]);
     return;
   }
+  float2 twi[16];
+#pragma unroll
+  for (int k2 = 0; k2 < 16; ++k2) twi[k2] = sTw[(t * k2) & 127];
   coarse128_to_spec(acc, sB, sX, twi, Nx, Ny, spec);
 }
 
-__global__ __launch_bounds__(256) void k_socs_mid128(const float2* __restrict__ Mfull, int Nx,
+__global__ __launch_bounds__(256, 2) void k_socs_mid128(const float2* __restrict__ Mfull, int Nx,
                                                      int Ny, const float2* __restrict__ kern,
                                                      const float* __restrict__ scales, int nk,
                                                      const float2* __restrict__ tw128,
@@ -3707,12 +3744,12 @@ int socs_forward_wide(klt_ctx* ctx, const klt_kernels* kb, const float* masks, i
   KLT_TRY(ctx->band.ensure((size_t)SB * T * sizeof(float2)));
   KLT_TRY(ctx->spec.ensure((size_t)SB * 64 * Gx * sizeof(float2)));
   KLT_TRY(ctx->ipart.ensure((size_t)kGroupSlots * 128 * 128 * sizeof(float)));
-  // the 128-point coarse stage keeps its row transforms, FFT exchange (and
-  // the band) in LDS
+  // the 128-point coarse stage keeps its row transforms in LDS, and the
+  // exchange of the final transform when it writes the spectrum itself
   const int lds_fin = 96 * kRS * (int)sizeof(float2);
-  const int lds_mid = lds_fin + T * (int)sizeof(float2);
+  const int lds_mid = (64 * kRS + 128) * (int)sizeof(float2), lds_mid1 = lds_fin + 128 * 8;
   KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_socs_mid128),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_mid));
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_mid1));
   KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_socs_mid128_fin),
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds_fin));
   for (int b0 = 0; b0 < B; b0 += SB) {
@@ -3727,7 +3764,7 @@ int socs_forward_wide(klt_ctx* ctx, const klt_kernels* kb, const float* masks, i
                  dim3((unsigned)(T + 255) / 256, nb), dim3(256), 0, ctx->qpart.as<float2>(),
                  parts, Nx, Ny, 32, inv_area, ctx->band.as<float2>());
       if (c.kg == 1) {
-        KLT_LAUNCH(ctx, "k_socs_mid128", k_socs_mid128, dim3(1, nb), dim3(256), lds_mid,
+        KLT_LAUNCH(ctx, "k_socs_mid128", k_socs_mid128, dim3(1, nb), dim3(256), lds_mid1,
                    ctx->band.as<float2>(), Nx, Ny, kb->kern.as<float2>(),
                    kb->scales.as<float>(), kb->nk, ft->tw128.as<float2>(),
                    ctx->spec.as<float2>(), nullptr);
@@ -3829,7 +3866,7 @@ static int build_batch(klt_ctx* ctx, BatchGraph* bg, int nmask, const float* con
                     hipMemcpyHostToDevice));
   bg->spec_tw = tw64;
   if (bg->wide) {
-    bg->lds_mid = (96 * kRS + (2 * Nx + 1) * Wy) * (int)sizeof(float2);
+    bg->lds_mid = (96 * kRS + 128 + (2 * Nx + 1) * Wy) * (int)sizeof(float2);
     KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_socs_mid128_jobs),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, bg->lds_mid));
   }
@@ -4272,9 +4309,10 @@ int socs_backward_wide(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
   KLT_TRY(ctx->gfull.ensure((size_t)SB * Gx * Gy * sizeof(float2)));
   const int lds_fin = 96 * kRS * (int)sizeof(float2);
   const int lds_mid = lds_fin + T * (int)sizeof(float2);
-  for (const void* f : {reinterpret_cast<const void*>(&k_socs_mid128),
-                        reinterpret_cast<const void*>(&k_socs_mid_bwd128)})
-    KLT_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_mid));
+  KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_socs_mid_bwd128),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_mid));
+  KLT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_socs_mid128),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_fin + 128 * 8));
   for (const void* f : {reinterpret_cast<const void*>(&k_socs_mid128_fin),
                         reinterpret_cast<const void*>(&k_resist_coarse128)})
     KLT_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_fin));
@@ -4295,7 +4333,8 @@ int socs_backward_wide(klt_ctx* ctx, const klt_kernels* kb, const float* masks,
       KLT_LAUNCH(ctx, "k_band_assemble", k_band_assemble, dim3((unsigned)(T + 255) / 256, nb),
                  dim3(256), 0, ctx->qpart.as<float2>(), parts, Nx, Ny, 32, inv_area,
                  ctx->mhat.as<float2>());
-      KLT_LAUNCH(ctx, "k_socs_mid128", k_socs_mid128, dim3(c.kg, nb), dim3(256), lds_mid,
+      KLT_LAUNCH(ctx, "k_socs_mid128", k_socs_mid128, dim3(c.kg, nb), dim3(256),
+                 (c.kg > 1 ? 64 * kRS + 128 : 96 * kRS + 128) * (int)sizeof(float2),
                  ctx->mhat.as<float2>(), Nx, Ny, kb->kern.as<float2>(), kb->scales.as<float>(),
                  kb->nk, ft->tw128.as<float2>(), ctx->spec.as<float2>(),
                  c.kg > 1 ? ctx->ipart.as<float>() : nullptr);
