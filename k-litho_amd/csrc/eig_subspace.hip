@@ -177,32 +177,54 @@ __global__ __launch_bounds__(256) void k_gram_reduce(const cd* __restrict__ part
 // Per matrix (grid (1, 1, count), kCW waves): G = L L^H and Binv = L^{-H}
 // (upper triangular, row-major p x p, p <= 64) for Y <- Y Binv.  G is padded
 // with the identity to 64 x 64 and held in registers: wave q, lane c keeps
-// W[i][c] = row kCW (jb + i) + q, column c, during group jb of kCW steps.
-// Step j of the right-looking factorisation turns W into M = L^{-1} one row
-// and column at a time, in place: the wave holding row j (which is,
-// conjugated, the trailing column j of the Hermitian remainder) forms
-// l_jj = sqrt(W[j][j]) and u[c] = W[j][c] / l_jj (u[j] = 1 / l_jj), so that
-// u[c] = conj(L[c][j]) for c > j and u[c] = M[j][c] for c <= j; row j becomes
-// M[j][.] (zero right of the diagonal); then every row r > j takes
-// W[r][c] -= L[r][j] u[c], L[r][j] = conj(u[r]) (column j first cleared),
-// which is the trailing update G'[r][c] -= L[r][j] conj(L[c][j]) for c > j and
-// the forward substitution M[r][c] -= L[r][j] M[j][c] for c <= j.  After a
-// group its kCW rows are final: they go to LDS (Binv is written at the end)
-// and the register window moves down one row per wave (static register
-// indices throughout).  The wave holding row j + 1 updates that row first
-// and forms step j + 1's pivot before its other rows (look-ahead).  One
-// barrier a step (u double-buffered); fail[b] = 1 when a pivot is not
-// positive.  Sixteen waves, four a SIMD: the steps are chains of dependent
-// FP64 operations, and several waves a SIMD overlap them.
+// W[i] = row 4q + i, column c.  Step j of the right-looking factorisation
+// turns W into M = L^{-1} one row and column at a time, in place: the wave
+// holding row j (which is, conjugated, the trailing column j of the Hermitian
+// remainder) forms l_jj = sqrt(W[j][j]) and u[c] = W[j][c] / l_jj
+// (u[j] = 1 / l_jj), so that u[c] = conj(L[c][j]) for c > j and
+// u[c] = M[j][c] for c <= j; row j becomes M[j][.] (zero right of the
+// diagonal); then every row r > j takes W[r][c] -= L[r][j] u[c],
+// L[r][j] = conj(u[r]) (column j first cleared), which is the trailing update
+// G'[r][c] -= L[r][j] conj(L[c][j]) for c > j and the forward substitution
+// M[r][c] -= L[r][j] M[j][c] for c <= j.  The steps go in groups of four,
+// the rows of one wave: that wave runs the group's pivots and the updates
+// among its own rows (u[r] read from lane r), and publishes the four u
+// vectors through LDS; after one barrier every later wave applies them to
+// its rows in step order (the operations of the step-by-step order, in that
+// order), the wave of the next group first among its work.  16 barriers
+// (one a step took 28 us at p = 64); fail[b] = 1 when a pivot is not
+// positive.
 constexpr int kCW = 16;        // waves
-constexpr int kCR = kP / kCW;  // rows a wave
+constexpr int kCR = kP / kCW;  // rows a wave: one group of steps
+static_assert(kCR == 4, "four rows a wave");
+
+// 1 / sqrt(d) for a normal d > 0: the hardware estimate and two Newton steps
+// y <- y + y (1 - d y^2) / 2 (each squares the relative error; the estimate's
+// is far below 2^-14, so the result is within rounding of the exact value).
+// The IEEE square root and division it replaces are long dependent sequences
+// on the critical path of every factorisation step.
+__device__ __forceinline__ double rsqrt_nr(double d) {
+  double y = __builtin_amdgcn_rsq(d);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double e = fma(-d * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
+__device__ __forceinline__ double lane_f64(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
 __global__ __launch_bounds__(kCW * 64) void k_chol_inv(const cd* __restrict__ G, int p,
                                                        cd* __restrict__ Binv,
                                                        int* __restrict__ fail,
                                                        const int* __restrict__ active) {
   const int b = blockIdx.z;
   if (active && !active[b]) return;
-  __shared__ cd ub[2][kP];
+  __shared__ cd ub[2][kCR][kP];  // a group's u vectors (double-buffered)
   __shared__ cd mt[kP][kP + 1];  // mt[c][r] = conj(M[r][c]), written out at the end
   const int tid = threadIdx.x, c = tid & 63;
   const int q = __builtin_amdgcn_readfirstlane(tid >> 6);  // the wave (uniform)
@@ -211,57 +233,51 @@ __global__ __launch_bounds__(kCW * 64) void k_chol_inv(const cd* __restrict__ G,
   cd W[kCR];
 #pragma unroll
   for (int i = 0; i < kCR; ++i) {
-    const int r = q + kCW * i;
+    const int r = kCR * q + i;
     W[i] = (r < p && c < p) ? Gb[(size_t)r * p + c] : make_double2(r == c ? 1.0 : 0.0, 0.0);
   }
   bool bad = false;
-  // step jj's pivot on row jj (register Wr of this wave): u and row jj of M
-  auto pivot = [&](cd& Wr, int jj, cd* u) {
-    const cd g = Wr;
-    const int lo = __builtin_amdgcn_readlane(__double2loint(g.x), jj);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(g.x), jj);
-    const double d = __hiloint2double(hi, lo);
-    if (!(d > 0.0)) bad = true;
-    const double inv = 1.0 / (d > 0.0 ? sqrt(d) : 1.0);
-    const cd gi = make_double2(g.x * inv, g.y * inv);
-    u[c] = c == jj ? make_double2(inv, 0.0) : gi;
-    Wr = c < jj ? gi : make_double2(c == jj ? inv : 0.0, 0.0);
-  };
   auto update = [&](cd& Wr, const cd& ur, const cd& uc, int j) {  // L[r][j] = conj(ur)
     cd w = c == j ? make_double2(0.0, 0.0) : Wr;
     w.x -= ur.x * uc.x + ur.y * uc.y;
     w.y -= ur.x * uc.y - ur.y * uc.x;
     Wr = w;
   };
-  if (q == 0) pivot(W[0], 0, ub[0]);
-  for (int jb = 0; jb < kCR; ++jb) {
-#pragma unroll
-    for (int jq = 0; jq < kCW; ++jq) {
-      const int j = kCW * jb + jq;
-      const cd* u = ub[jq & 1];
-      __syncthreads();  // u of step j is complete
-      const cd uc = u[c];
-      // the wave holding row j + 1 (W[in] of wave wn) updates it first and
-      // forms step j + 1's pivot, ahead of its other rows
-      const int wn = (jq + 1) % kCW, in = jq + 1 < kCW ? 0 : 1;
-      if (q == wn && j + 1 < kP) {
-        update(W[in], u[j + 1], uc, j);
-        pivot(W[in], j + 1, ub[(jq + 1) & 1]);
-      }
+  for (int jb = 0; jb < kCW; ++jb) {
+    if (q == jb) {  // (uniform) the group's pivots
 #pragma unroll
       for (int i = 0; i < kCR; ++i) {
-        // row r = kCW (jb + i) + q, updated when j < r < 64 (uniform)
-        if (i == 0 ? q <= jq : jb + i >= kCR) continue;
-        if (q == wn && i == in) continue;  // done above
-        update(W[i], u[kCW * (jb + i) + q], uc, j);
+        const int j = kCR * jb + i;
+        const cd g = W[i];
+        const double d = lane_f64(g.x, j);
+        if (!(d > 0.0)) bad = true;
+        // (d is uniform; pivots below the normal range, which a Gram of this
+        // iteration does not reach, keep the exact operations)
+        const double inv = d > 1e-300 ? rsqrt_nr(d) : 1.0 / (d > 0.0 ? sqrt(d) : 1.0);
+        const cd gi = make_double2(g.x * inv, g.y * inv);
+        const cd u = c == j ? make_double2(inv, 0.0) : gi;
+        ub[jb & 1][i][c] = u;
+        W[i] = c < j ? gi : make_double2(c == j ? inv : 0.0, 0.0);
+#pragma unroll
+        for (int i2 = i + 1; i2 < kCR; ++i2) {
+          const int r = kCR * jb + i2;
+          update(W[i2], make_double2(lane_f64(u.x, r), lane_f64(u.y, r)), u, j);
+        }
       }
     }
-    // row kCW jb + q is final (staged in LDS, so that no barrier waits for
-    // global stores); then shift the window
-    mt[c][kCW * jb + q] = make_double2(W[0].x, -W[0].y);
+    __syncthreads();  // the group's u vectors are complete
+    if (q > jb) {
 #pragma unroll
-    for (int i = 0; i + 1 < kCR; ++i) W[i] = W[i + 1];
+      for (int i = 0; i < kCR; ++i) {
+        const cd* u = ub[jb & 1][i];
+        const cd uc = u[c];
+#pragma unroll
+        for (int i2 = 0; i2 < kCR; ++i2) update(W[i2], u[kCR * q + i2], uc, kCR * jb + i);
+      }
+    }
   }
+#pragma unroll
+  for (int i = 0; i < kCR; ++i) mt[c][kCR * q + i] = make_double2(W[i].x, -W[i].y);
   __syncthreads();
   for (int o = tid; o < p * p; o += kCW * 64) Bo[o] = mt[o / p][o % p];  // Binv[c][r]
   if (bad && c == 0) fail[b] = 1;
