@@ -489,6 +489,48 @@ def bench_wide(ctx, timer, torch, dev):
     return out
 
 
+def bench_widen(ctx, timer, torch, dev, masks, images, src):
+    """The rows either side of the imaging path, on the workload's own shapes:
+      hopkins  klt_hopkins_image (calcImage + FT_c2r) of one 1024^2 mask from the
+               FP64 TCC of the headline optics (T = 841);
+      resist   klt_resist_levels over the headline batch's images (512 of
+               1024^2: 4 bytes read and 1 written a pixel);
+      smo      klt_smo_run on 16 relaxed 1024^2 Manhattan masks, 51^2 annular
+               source, K = 24: each step builds the TCC, solves its top 24
+               eigenpairs, images the masks, and forms the mask adjoint and the
+               source gradient before the projected updates."""
+    out = {}
+    sig = min(klt.outer_sigma(src), 1.0)
+    dsrc = torch.from_numpy(src).to(dev)
+    tcc = ctx.tcc_build(dsrc, sig, W.NA, 0.0, W.PERIOD_29, W.PERIOD_29, 14, 14, precision=0,
+                        nmed=NMED)
+    m1 = masks[0]
+    ctx.hopkins_image(m1, tcc, 14, 14)
+    hms = timer(lambda: ctx.hopkins_image(m1, tcc, 14, 14), 5)
+    out["hopkins"] = {"T": int(tcc.shape[0]), "side": int(m1.shape[-1]), "ms_per_image": hms}
+    ctx.resist_levels(images)
+    rms = timer(lambda: ctx.resist_levels(images), 5)
+    nbytes = 5.0 * images.numel()
+    out["resist_levels"] = {"images": int(images.shape[0]), "side": int(images.shape[-1]),
+                            "ms": rms, "hbm_gbs": nbytes / (rms / 1e3) / 1e9}
+    T = W.manhattan_batch(L, 16, 3000)
+    m0 = np.stack([W.relaxed(T[i], 4000 + i) for i in range(16)])
+    dm = torch.from_numpy(m0).to(dev)
+    dt = torch.from_numpy(T).to(dev)
+    S = klt.source_create(W.SRC_SIZE, "Annular", W.SIGMA_IN, W.SIGMA_OUT)
+    S /= S.sum()
+    klt.smo_run(ctx, dm, dt, S, W.NA, 1, nmed=NMED)  # (first call: tuning of its shapes)
+    steps = 5
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist, _ = klt.smo_run(ctx, dm, dt, S, W.NA, steps, nmed=NMED)
+    torch.cuda.synchronize()
+    sms = (time.perf_counter() - t0) * 1e3 / steps
+    out["smo"] = {"masks": 16, "side": L, "K": NK, "source": W.SRC_SIZE, "steps": steps,
+                  "ms_per_step": sms, "loss_first": float(hist[0]), "loss_last": float(hist[-1])}
+    return out
+
+
 def copy_calibration(torch, timer, dev):
     """A torch device-to-device copy of 1 GiB (the runtime's copy kernel), whose
     known bytes calibrate the FETCH_SIZE / WRITE_SIZE counters."""
@@ -746,6 +788,12 @@ def main():
             f"1024^2: {wd['n31']['images_per_s']:.0f} images/s, "
             f"{wd['n31']['mask_grads_per_s']:.0f} mask grads/s")
         extras["wide"] = wd
+        wn = bench_widen(ctx, timer, torch, dev, masks, images, src)
+        log(f"Hopkins image {wn['hopkins']['ms_per_image']:.2f} ms (T={wn['hopkins']['T']}, "
+            f"1024^2); resist levels {wn['resist_levels']['ms']:.3f} ms for 512 images "
+            f"({wn['resist_levels']['hbm_gbs']:.0f} GB/s); SMO step "
+            f"{wn['smo']['ms_per_step']:.2f} ms (16 masks of 1024^2, K=24)")
+        extras["widen"] = wn
     if args.copy_calibration:
         extras["copy_calibration"] = copy_calibration(torch, timer, dev)
 
