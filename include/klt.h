@@ -37,6 +37,7 @@ extern "C" {
 
 #define KLT_HOST_PTRS 1u    /* array arguments are host memory */
 #define KLT_EIG_PSD 2u      /* eigensolver input is positive semidefinite (see klt_eig_topk) */
+#define KLT_EIG_WARM 4u     /* with KLT_EIG_PSD: start from the previous solve's block (see klt_eig_topk) */
 
 typedef struct klt_ctx klt_ctx;
 typedef struct klt_kernels klt_kernels;
@@ -171,7 +172,13 @@ int klt_tcc_build(klt_ctx* ctx, const float* src, int srcSize, float outerSigma,
  * CholQR2, Rayleigh-Ritz) to residuals of 5e-14 of the largest eigenvalue, and
  * uses the Householder path for matrices it does not reach (n < 128, k > 48,
  * no convergence in 24 rounds).  The flag applies to klt_eig_topk_batch and
- * klt_calc_kernels as well. */
+ * klt_calc_kernels as well.  With KLT_EIG_WARM besides (klt_eig_topk and
+ * klt_calc_kernels), the iteration starts from the 64-column block that the
+ * previous positive-semidefinite solve of a matrix of the same order left on
+ * this context, and tests convergence from its fifth round on: for a
+ * sequence of slowly changing matrices, such as the TCCs of a source
+ * optimisation.  The result meets the same residual test; without such a
+ * previous solve the flag is ignored. */
 int klt_eig_topk(klt_ctx* ctx, const double* a, int n, int k, double* w, double* vecs,
                  unsigned flags);
 /* A batch of `count` Hermitian matrices of the same size in one call (no

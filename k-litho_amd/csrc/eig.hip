@@ -1687,7 +1687,8 @@ int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_d
 static int eig_dispatch(klt_ctx* ctx, const double* a_dev, int count, int n, int k, double* w,
                         double* v, unsigned flags) {
   if (flags & KLT_EIG_PSD) {
-    const int rc = eig_topk_psd_batch_device(ctx, a_dev, count, n, k, w, v);
+    const int rc =
+        eig_topk_psd_batch_device(ctx, a_dev, count, n, k, w, v, (flags & KLT_EIG_WARM) != 0);
     if (rc != KLT_ERR_EIG) return rc;
   }
   return eig_topk_batch_device(ctx, a_dev, count, n, k, w, v);

@@ -37,6 +37,7 @@ typedef double2 cd;
 
 constexpr int kP = 64;          // block size (columns)
 constexpr int kFirstCheck = 11;  // rounds of X <- orth(A^2 X) before the first check
+constexpr int kWarmFirstCheck = 5;  // the same from the previous solve's block
 constexpr int kCheckEvery = 3;  // rounds between later checks
 constexpr int kMaxRounds = 24;
 constexpr double kTol = 5e-14;  // residual, relative to theta_0
@@ -343,7 +344,7 @@ namespace klt {
 // Returns KLT_OK with every matrix solved, or KLT_ERR_EIG when some matrix
 // is outside the method's reach (the caller then uses the Householder path).
 int eig_topk_psd_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, int k,
-                              double* w_dev, double* vec_dev) {
+                              double* w_dev, double* vec_dev, bool warm) {
   const int p = kP;
   if (k + 16 > p || n < 2 * p) return set_error(KLT_ERR_EIG, "psd subspace: shape not covered");
   const size_t np = (size_t)n * p, nk = (size_t)n * k, B = (size_t)count, pp = (size_t)p * p;
@@ -370,6 +371,10 @@ int eig_topk_psd_batch_device(klt_ctx* ctx, const double* a_dev, int count, int 
                oGp = carve(B * nsg * pp * 16), oG = carve(B * pp * 16),
                oBi = carve(B * pp * 16), oU = carve(B * k * p * 16), oTh = carve(B * k * 8),
                oRes = carve(B * k * 8), oFail = carve(B * 4), oAct = carve(B * 4);
+  // a warm start needs the block of a previous solve of this order (the
+  // workspace layout depends on count and n only, so X is still in place)
+  warm = warm && count == 1 && ctx->sub_warm_n == n && ctx->sub_ws.bytes >= off;
+  ctx->sub_warm_n = 0;  // (set again only by a solve that converges)
   KLT_TRY(ctx->sub_ws.ensure(off));
   char* base = ctx->sub_ws.as<char>();
   cd* X = reinterpret_cast<cd*>(base + oX);
@@ -445,17 +450,22 @@ int eig_topk_psd_batch_device(klt_ctx* ctx, const double* a_dev, int count, int 
   };
   // rounds of X <- orth(A^2 X); Rayleigh-Ritz after kFirstCheck rounds and
   // every kCheckEvery after that; a matrix stops at its first converged check
-  KLT_LAUNCH(ctx, "k_start_block", k_start_block, dim3((unsigned)((np + 255) / 256), 1, count),
-             gblk, 0, Y0, np, n, p);
-  cholqr(Y0, Y1);
-  cholqr(Y1, X);
+  // X: the previous solve's orthonormal block (warm) or the orthonormalised
+  // start block
+  if (!warm) {
+    KLT_LAUNCH(ctx, "k_start_block", k_start_block, dim3((unsigned)((np + 255) / 256), 1, count),
+               gblk, 0, Y0, np, n, p);
+    cholqr(Y0, Y1);
+    cholqr(Y1, X);
+  }
+  const int first = warm ? kWarmFirstCheck : kFirstCheck;
   for (int round = 1;; ++round) {
     gemm_A(X, Y0);
     gemm_A(Y0, Y1);
     // one pass keeps the span (to rounding relative to the largest column)
     // and a basis close enough to orthonormal for the next products; two
     // before the Rayleigh-Ritz step, which needs an orthonormal X
-    const bool check = round >= kFirstCheck && (round - kFirstCheck) % kCheckEvery == 0;
+    const bool check = round >= first && (round - first) % kCheckEvery == 0;
     if (check) {
       cholqr(Y1, Y0);
       cholqr(Y0, X);
@@ -483,6 +493,7 @@ int eig_topk_psd_batch_device(klt_ctx* ctx, const double* a_dev, int count, int 
   KLT_LAUNCH(ctx, "k_sub_output", k_sub_output, dim3((unsigned)((nk + 255) / 256), 1, count),
              gblk, 0, Y0, nk, n, k, k, th, w_dev, reinterpret_cast<cd*>(vec_dev));
   KLT_HIP(hipGetLastError());
+  if (count == 1) ctx->sub_warm_n = n;  // X spans the converged block
   return KLT_OK;
 }
 

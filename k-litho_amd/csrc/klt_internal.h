@@ -201,6 +201,7 @@ struct klt_ctx {
   klt::DevBuf mhat, gfull, gcoarse;           // adjoint: per-image M^, G^ and coarse-grid g_c
   klt::DevBuf hspec;                          // Hopkins spectra of the source gradient
   klt::DevBuf sub_ws;                         // blocks of the subspace eigensolver
+  int sub_warm_n = 0;  // order of the last single-matrix PSD solve whose block sub_ws holds
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::SocsPlan>> socs_plans;
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<klt::FusedTables>> fused_tables;
   // split factors of the fused passes per launch shape (kind, batch, Lx, Ly,
@@ -347,8 +348,9 @@ int eig_topk_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, i
 // The same for positive semidefinite matrices by block subspace iteration
 // (eig_subspace.hip); KLT_ERR_EIG when a matrix is outside its
 // reach (k + 16 > 64, n < 128, no convergence), for the caller to fall back.
+// warm (count 1): start from the block of the previous solve of order n.
 int eig_topk_psd_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, int k,
-                              double* w_dev, double* vec_dev);
+                              double* w_dev, double* vec_dev, bool warm = false);
 // Mixed-size, multi-set batch (socs_fused.hip): images[m * nbank + s] is mask
 // m under kernel set s.  Captured into a graph on first use, replayed after.
 int socs_forward_batch(klt_ctx* ctx, int nmask, const float* const* masks, const int* Lx,

@@ -19,6 +19,7 @@ HEADER = os.path.join(os.path.dirname(PKG), "include", "klt.h")
 KLT_OK = 0
 KLT_HOST_PTRS = 1
 KLT_EIG_PSD = 2
+KLT_EIG_WARM = 4
 
 ERRORS = {-1: "KLT_ERR_ARG", -2: "KLT_ERR_SIZE", -3: "KLT_ERR_HIP", -4: "KLT_ERR_NODEV",
           -5: "KLT_ERR_IO", -6: "KLT_ERR_NOMEM", -7: "KLT_ERR_EIG"}
@@ -332,9 +333,10 @@ class Context:
                                       Ny, precision, po, _flags(hs, ho)))
         return out
 
-    def eig_topk(self, a, k: int, psd: bool = False):
+    def eig_topk(self, a, k: int, psd: bool = False, warm: bool = False):
         """Top-k eigenpairs of a Hermitian matrix a [n, n, 2] (re, im), FP64 on the GPU;
-        psd=True: a is positive semidefinite (KLT_EIG_PSD).
+        psd=True: a is positive semidefinite (KLT_EIG_PSD); warm: start from the
+        previous solve's block (KLT_EIG_WARM).
 
         Returns (w [k], vecs [k, n, 2]) on the same side (host/device) as a."""
         n = int(a.shape[0])
@@ -351,7 +353,8 @@ class Context:
         pv, hv = _ptr(v, np.float64)
         with _TorchOrder(self._h, a):
             check(lib().klt_eig_topk(self._h, pa, n, k, pw, pv,
-                                     _flags(ha, hw, hv) | (KLT_EIG_PSD if psd else 0)))
+                                     _flags(ha, hw, hv) | (KLT_EIG_PSD if psd else 0)
+                                     | (KLT_EIG_WARM if psd and warm else 0)))
         return w, v
 
     def eig_topk_batch(self, a, k: int, psd: bool = False):
@@ -374,10 +377,11 @@ class Context:
                                            _flags(ha, hw, hv) | (KLT_EIG_PSD if psd else 0)))
         return w, v
 
-    def calc_kernels(self, tcc, nk: int, psd: bool = True):
+    def calc_kernels(self, tcc, nk: int, psd: bool = True, warm: bool = False):
         """calcKernels: (krn_r [nk, T], krn_i [nk, T], scales [nk]) float32; psd:
         the TCC is positive semidefinite (KLT_EIG_PSD; true of every TCC of a
-        non-negative source)."""
+        non-negative source); warm: start from the previous solve's block
+        (KLT_EIG_WARM)."""
         T = int(tcc.shape[0])
         nk = min(nk, T)
         if _is_torch(tcc):
@@ -395,7 +399,8 @@ class Context:
         p3, h3 = _ptr(sc, np.float32)
         with _TorchOrder(self._h, tcc):
             check(lib().klt_calc_kernels(self._h, pt, T, nk, p1, p2, p3,
-                                         _flags(ht, h1, h2, h3) | (KLT_EIG_PSD if psd else 0)))
+                                         _flags(ht, h1, h2, h3) | (KLT_EIG_PSD if psd else 0)
+                                         | (KLT_EIG_WARM if psd and warm else 0)))
         return kr, ki, sc
 
     def hopkins_image(self, mask, tcc, Nx: int, Ny: int):

@@ -11,7 +11,11 @@ eigen-decomposes (K-Litho-TCC/klitho_tcc.cpp:301-377):
 * a batch equals its single calls bit for bit;
 * the Householder path takes over, with correct results, for k above the
   block's reach and for a flat spectrum the filter cannot separate; the
-  kernel names recorded by the profiler show which path ran.
+  kernel names recorded by the profiler show which path ran;
+* KLT_EIG_WARM: after a small change of the source, a solve that starts from
+  the previous solve's block meets the same checks in fewer A X products; one
+  that starts from the block of an unrelated matrix of the same order meets
+  them too.
 """
 import numpy as np
 import pytest
@@ -129,3 +133,40 @@ def test_psd_falls_back_to_householder(ctx, kind):
     vv = _c(v)
     for q in range(k):
         assert np.linalg.norm(a @ vv[q] - w[q] * vv[q]) < 1e-11 * nrm
+
+
+def test_psd_warm_start(ctx):
+    print("case eig psd warm start (annular 1088, T = 841)", flush=True)
+    rng = np.random.default_rng(23)
+    src = O.source("Annular", 51, inner=0.6, outer=0.9)
+    sig = O.outer_sigma(src)
+    Nx, Ny = O.sizing(1.35, W.PERIOD_29, W.PERIOD_29, sig)
+    a1 = O.tcc(src, sig, 1.35, 0.0, W.PERIOD_29, W.PERIOD_29, Nx, Ny, nmed=1.44)
+    src2 = src.copy()
+    on = np.flatnonzero(src2 > 0)
+    pick = rng.choice(on, max(1, len(on) // 20), replace=False)
+    src2.reshape(-1)[pick] *= 0.9
+    a2 = O.tcc(src2, sig, 1.35, 0.0, W.PERIOD_29, W.PERIOD_29, Nx, Ny, nmed=1.44)
+    n = a2.shape[0]
+
+    def products(a, warm):
+        ctx.set_profiling(True)
+        w, v = ctx.eig_topk(_pack(a), 24, psd=True, warm=warm)
+        rep = ctx.profile_report()
+        ctx.set_profiling(False)
+        return w, _c(v), rep["k_zgemm"][0]
+
+    w, v, cold = products(a2, False)
+    _check(a2, w, v, 24)
+    products(a1, False)
+    w, v, warm = products(a2, True)
+    print(f"A X products: cold {cold}, warm {warm}", flush=True)
+    assert warm < cold
+    _check(a2, w, v, 24)
+    # a block of an unrelated matrix of the same order (decaying spectrum)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    d = 10.0 ** -np.linspace(0, 8, n)
+    b = (q * d) @ q.conj().T
+    products(0.5 * (b + b.conj().T), False)
+    w, v, _ = products(a2, True)
+    _check(a2, w, v, 24)
