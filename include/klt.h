@@ -313,10 +313,11 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
 /* ---- source-mask optimisation (no reference counterpart) ----------------
  * Projected, normalised gradient descent of the resist loss of
  * klt_socs_backward over nmask masks and their shared source, `steps`
- * iterations of: the TCC of the current source (klt_tcc_build, FP64 Gram, at
- * the kernel half-size of a filled pupil, N = floor(NA L 2 / lambda), so that
- * the source may grow anywhere inside the unit disc), its K = min(nk, T)
- * kernels (klt_calc_kernels), the loss and mask gradient g_m through the SOCS
+ * iterations of: the TCC of the current source (klt_tcc_build, split-FP16
+ * Gram, at the kernel half-size of a filled pupil, N = floor(NA L 2 / lambda),
+ * so that the source may grow anywhere inside the unit disc), its
+ * K = min(nk, T) kernels (klt_calc_kernels, KLT_EIG_PSD, and KLT_EIG_WARM
+ * from the second iteration on), the loss and mask gradient g_m through the SOCS
  * model (klt_socs_backward), the source gradient g_S through the Abbe model
  * (klt_abbe_source_grad, summed over the masks), then
  *   m <- clip(m - step_mask g_m / max|g_m|, 0, 1)                    per mask,
