@@ -564,6 +564,59 @@ def _host_threads():
     return max(1, min(16, n))
 
 
+def host_io_timing(O, img):
+    """The host-side rows beside the reference's own code (oracle/_ref: K-Litho's
+    mask.cpp and file_io.cpp compiled from its sources): a 1024^2 whitespace-text
+    mask read through createMask's Import, and the three PNG views of one 1024^2
+    image; milliseconds, the median of 5 runs each."""
+    import shutil
+    import tempfile
+    R = O.ref()
+    out = {}
+    d = tempfile.mkdtemp(prefix="klt_hostio_")
+
+    def med(fn):
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(1e3 * (time.perf_counter() - t0))
+        return float(np.median(ts))
+
+    # (the Import path prints its file name, as the reference's tools do:
+    # bench.py's stdout carries only the JSON line, so fd 1 is silenced here)
+    sys.stdout.flush()
+    saved, devnull = os.dup(1), os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+    try:
+        path = os.path.join(d, "mask.txt")
+        np.savetxt(path, W.manhattan(L, 4242), fmt="%g")
+        out["text_mask_import_ms"] = med(
+            lambda: klt.mask_create(L, L, L, L, "Import", mask_file=path))
+        if R is not None:
+            buf = np.zeros(L * L, np.float32)
+            out["text_mask_import_ms_reference"] = med(
+                lambda: R.ref_create_mask(buf, L, L, L, L, b"Import", 1, 1, 0, path.encode(),
+                                          1.0))
+        im = np.ascontiguousarray(img, np.float32)
+        for kind in ("gray", "7colors", "continuous"):
+            p = os.path.join(d, f"{kind}.png")
+            out[f"png_{kind}_ms"] = med(
+                lambda: klt.write_png(kind, p, im, 0.0, klt.TARGET_INTENSITY))
+            if R is not None:
+                out[f"png_{kind}_ms_reference"] = med(
+                    lambda: getattr(R, f"ref_png_{kind}")(p.encode(), L, L, im.ravel(), 0.0,
+                                                          klt.TARGET_INTENSITY))
+    finally:
+        import ctypes
+        ctypes.CDLL(None).fflush(None)
+        os.dup2(saved, 1)
+        os.close(devnull)
+        os.close(saved)
+        shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
 def cpu_baseline(ctx, bank, fused_img, mask, kr, ki, sc, budget_s):
     """The reference's CPU path on this host, timed on bounded samples:
     FT_r2c + calcSOCS + FI on FFTW3 (oracle/cpu_ref_path.cpp, the library
@@ -587,7 +640,7 @@ def cpu_baseline(ctx, bank, fused_img, mask, kr, ki, sc, budget_s):
     gerr = float(np.linalg.norm(g[0] - g64) / np.linalg.norm(g64))
     nth = _host_threads()
     info = {"nproc": os.cpu_count(), "cpu_model": _cpu_model(), "threads_all_cores": nth,
-            "library": CR.reason()}
+            "library": CR.reason(), "host_io": host_io_timing(O, fused_img)}
     # numpy port of the FFTW procedure (secondary figure), ~1/4 of the budget
     t0, n = time.perf_counter(), 0
     while time.perf_counter() - t0 < 0.25 * budget_s:
@@ -801,6 +854,14 @@ def main():
     if not args.no_cpu:
         cpu, err, gerr = cpu_baseline(ctx, bank, fused_img0, masks_h[0], kr, ki, sc,
                                       args.cpu_seconds)
+        hio = cpu["host_io"]
+        log(f"host I/O: text mask import {hio['text_mask_import_ms']:.1f} ms (reference "
+            f"{hio.get('text_mask_import_ms_reference', float('nan')):.1f}), PNG views "
+            f"{hio['png_gray_ms']:.1f} / {hio['png_7colors_ms']:.1f} / "
+            f"{hio['png_continuous_ms']:.1f} ms (reference "
+            f"{hio.get('png_gray_ms_reference', float('nan')):.1f} / "
+            f"{hio.get('png_7colors_ms_reference', float('nan')):.1f} / "
+            f"{hio.get('png_continuous_ms_reference', float('nan')):.1f})")
         log(f"cpu baseline: {cpu['value']:.2f} images/s on {cpu['cores']} cores "
             f"({cpu.get('images_per_s_1core', 0):.1f} on 1); GPU image rel-L2 vs FP64 {err:.2e}, "
             f"mask gradient rel-L2 vs FP64 {gerr:.2e}")
