@@ -566,9 +566,11 @@ def _host_threads():
 
 def host_io_timing(O, img):
     """The host-side rows beside the reference's own code (oracle/_ref: K-Litho's
-    mask.cpp and file_io.cpp compiled from its sources): a 1024^2 whitespace-text
-    mask read through createMask's Import, and the three PNG views of one 1024^2
-    image; milliseconds, the median of 5 runs each."""
+    source.cpp, mask.cpp and file_io.cpp compiled from its sources): a 1024^2
+    whitespace-text mask read through createMask's Import, a generated
+    LineSpace mask, the 101^2 annular source grid, image.bin written and read,
+    the 24 kernel files and kernel_info.txt of klitho_tcc, and the three PNG
+    views of one 1024^2 image; milliseconds, the median of 5 runs each."""
     import shutil
     import tempfile
     R = O.ref()
@@ -598,7 +600,47 @@ def host_io_timing(O, img):
             out["text_mask_import_ms_reference"] = med(
                 lambda: R.ref_create_mask(buf, L, L, L, L, b"Import", 1, 1, 0, path.encode(),
                                           1.0))
+        # createMask's generated LineSpace mask, createSource's annular grid of
+        # the TCC study (101^2), image.bin written and read back
+        out["linespace_mask_ms"] = med(
+            lambda: klt.mask_create(L, L, L, L, "LineSpace", 128, 128, 0))
+        out["annular_source_101_ms"] = med(lambda: klt.source_create(101, "Annular", 0.6, 0.9))
         im = np.ascontiguousarray(img, np.float32)
+        fb = os.path.join(d, "image.bin")
+        out["image_bin_write_ms"] = med(lambda: klt.write_floats(fb, im))
+        out["image_bin_read_ms"] = med(lambda: klt.read_floats(fb, L * L))
+        if R is not None:
+            out["linespace_mask_ms_reference"] = med(
+                lambda: R.ref_create_mask(buf, L, L, L, L, b"LineSpace", 128, 128, 0, b"", 1.0))
+            sbuf = np.zeros(101 * 101, np.float32)
+            out["annular_source_101_ms_reference"] = med(
+                lambda: R.ref_create_source(sbuf, 101, b"Annular", 0.6, 0.9, 0.1, 0.5, 1, 0.6,
+                                            0.2, b""))
+            out["image_bin_write_ms_reference"] = med(
+                lambda: R.ref_write_float_bin(fb.encode(), im.ravel(), L * L))
+            out["image_bin_read_ms_reference"] = med(
+                lambda: R.ref_read_float_bin(fb.encode(), buf, L * L))
+        # the kernel files klitho_tcc writes: 24 kernels of 29 x 29 (krn_*_r/i.bin)
+        # and kernel_info.txt
+        kk = np.random.default_rng(1).standard_normal((48, 29 * 29)).astype(np.float32)
+        scl = np.linspace(1.0, 0.01, 24).astype(np.float32)
+
+        def kernel_files(ref):
+            for q in range(48):
+                f = os.path.join(d, f"krn_{q // 2}_{'ri'[q % 2]}.bin")
+                if ref:
+                    R.ref_write_float_bin(f.encode(), kk[q], 29 * 29)
+                else:
+                    klt.write_floats(f, kk[q])
+            f = os.path.join(d, "kernel_info.txt")
+            if ref:
+                R.ref_write_kernel_info(f.encode(), 29, 29, 24, scl)
+            else:
+                klt.write_kernel_info(f, 29, 29, scl)
+
+        out["kernel_files_write_ms"] = med(lambda: kernel_files(False))
+        if R is not None:
+            out["kernel_files_write_ms_reference"] = med(lambda: kernel_files(True))
         for kind in ("gray", "7colors", "continuous"):
             p = os.path.join(d, f"{kind}.png")
             out[f"png_{kind}_ms"] = med(
