@@ -1685,13 +1685,50 @@ int eig_topk_device(klt_ctx* ctx, const double* a_dev, int n, int k, double* w_d
 // KLT_EIG_PSD: the subspace solver first; the Householder path when it
 // reports that a matrix is outside its reach.
 static int eig_dispatch(klt_ctx* ctx, const double* a_dev, int count, int n, int k, double* w,
-                        double* v, unsigned flags) {
+                        double* v, unsigned flags, double tol = 0.0) {
   if (flags & KLT_EIG_PSD) {
-    const int rc =
-        eig_topk_psd_batch_device(ctx, a_dev, count, n, k, w, v, (flags & KLT_EIG_WARM) != 0);
+    const int rc = eig_topk_psd_batch_device(ctx, a_dev, count, n, k, w, v,
+                                             (flags & KLT_EIG_WARM) != 0, tol);
     if (rc != KLT_ERR_EIG) return rc;
   }
   return eig_topk_batch_device(ctx, a_dev, count, n, k, w, v);
+}
+
+// klt_calc_kernels' body (klitho_tcc.cpp:301-377), with the subspace solver's
+// residual target tol (0: 5e-14).
+int calc_kernels_device(klt_ctx* ctx, const double* tcc, int T, int nk, float* krn_r,
+                        float* krn_i, float* scales, unsigned flags, double tol) {
+  if (!ctx || !tcc || !krn_r || !krn_i || !scales)
+    return set_error(KLT_ERR_ARG, "klt_calc_kernels: NULL argument");
+  if (T <= 0 || nk <= 0) return set_error(KLT_ERR_ARG, "klt_calc_kernels: bad size");
+  if (nk > T) nk = T;  // klitho_tcc.cpp:488-491
+  KLT_TRY(ctx->activate());
+  const size_t nn = (size_t)T * T;
+  const double* da = tcc;
+  if (flags & KLT_HOST_PTRS) {
+    KLT_TRY(ctx->stage_in.ensure(nn * 16));
+    KLT_HIP(hipMemcpyAsync(ctx->stage_in.ptr, tcc, nn * 16, hipMemcpyHostToDevice, ctx->stream));
+    da = ctx->stage_in.as<double>();
+  }
+  const size_t vb = (size_t)nk * T * 16, wb = (size_t)nk * 8, kb = (size_t)nk * T * 4;
+  KLT_TRY(ctx->stage_out.ensure(vb + wb + 2 * kb + nk * 4 + 1024));
+  char* o = ctx->stage_out.as<char>();
+  double* dv = reinterpret_cast<double*>(o);
+  double* dw = reinterpret_cast<double*>(o + vb);
+  float* fr = reinterpret_cast<float*>(o + vb + wb);
+  float* fi = fr + (size_t)nk * T;
+  float* fs = fi + (size_t)nk * T;
+  KLT_TRY(eig_dispatch(ctx, da, 1, T, nk, dw, dv, flags, tol));
+  const int tot = nk * T;
+  KLT_LAUNCH(ctx, "k_kernels_from_vectors", k_kernels_from_vectors, dim3((tot + 255) / 256),
+             dim3(256), 0, reinterpret_cast<const cd*>(dv), dw, nk, T, fr, fi, fs);
+  const hipMemcpyKind kind =
+      (flags & KLT_HOST_PTRS) ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  KLT_HIP(hipMemcpyAsync(krn_r, fr, kb, kind, ctx->stream));
+  KLT_HIP(hipMemcpyAsync(krn_i, fi, kb, kind, ctx->stream));
+  KLT_HIP(hipMemcpyAsync(scales, fs, nk * 4, kind, ctx->stream));
+  KLT_HIP(hipStreamSynchronize(ctx->stream));
+  return KLT_OK;
 }
 
 }  // namespace klt
@@ -1757,37 +1794,7 @@ int klt_eig_topk_batch(klt_ctx* ctx, const double* a, int count, int n, int k, d
 
 int klt_calc_kernels(klt_ctx* ctx, const double* tcc, int T, int nk, float* krn_r,
                      float* krn_i, float* scales, unsigned flags) {
-  if (!ctx || !tcc || !krn_r || !krn_i || !scales)
-    return klt::set_error(KLT_ERR_ARG, "klt_calc_kernels: NULL argument");
-  if (T <= 0 || nk <= 0) return klt::set_error(KLT_ERR_ARG, "klt_calc_kernels: bad size");
-  if (nk > T) nk = T;  // klitho_tcc.cpp:488-491
-  KLT_TRY(ctx->activate());
-  const size_t nn = (size_t)T * T;
-  const double* da = tcc;
-  if (flags & KLT_HOST_PTRS) {
-    KLT_TRY(ctx->stage_in.ensure(nn * 16));
-    KLT_HIP(hipMemcpyAsync(ctx->stage_in.ptr, tcc, nn * 16, hipMemcpyHostToDevice, ctx->stream));
-    da = ctx->stage_in.as<double>();
-  }
-  const size_t vb = (size_t)nk * T * 16, wb = (size_t)nk * 8, kb = (size_t)nk * T * 4;
-  KLT_TRY(ctx->stage_out.ensure(vb + wb + 2 * kb + nk * 4 + 1024));
-  char* o = ctx->stage_out.as<char>();
-  double* dv = reinterpret_cast<double*>(o);
-  double* dw = reinterpret_cast<double*>(o + vb);
-  float* fr = reinterpret_cast<float*>(o + vb + wb);
-  float* fi = fr + (size_t)nk * T;
-  float* fs = fi + (size_t)nk * T;
-  KLT_TRY(klt::eig_dispatch(ctx, da, 1, T, nk, dw, dv, flags));
-  const int tot = nk * T;
-  KLT_LAUNCH(ctx, "k_kernels_from_vectors", k_kernels_from_vectors, dim3((tot + 255) / 256),
-             dim3(256), 0, reinterpret_cast<const cd*>(dv), dw, nk, T, fr, fi, fs);
-  const hipMemcpyKind kind =
-      (flags & KLT_HOST_PTRS) ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
-  KLT_HIP(hipMemcpyAsync(krn_r, fr, kb, kind, ctx->stream));
-  KLT_HIP(hipMemcpyAsync(krn_i, fi, kb, kind, ctx->stream));
-  KLT_HIP(hipMemcpyAsync(scales, fs, nk * 4, kind, ctx->stream));
-  KLT_HIP(hipStreamSynchronize(ctx->stream));
-  return KLT_OK;
+  return klt::calc_kernels_device(ctx, tcc, T, nk, krn_r, krn_i, scales, flags, 0.0);
 }
 
 }  // extern "C"

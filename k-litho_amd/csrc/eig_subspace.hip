@@ -344,7 +344,8 @@ namespace klt {
 // Returns KLT_OK with every matrix solved, or KLT_ERR_EIG when some matrix
 // is outside the method's reach (the caller then uses the Householder path).
 int eig_topk_psd_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, int k,
-                              double* w_dev, double* vec_dev, bool warm) {
+                              double* w_dev, double* vec_dev, bool warm, double tol) {
+  if (!(tol > 0.0)) tol = kTol;
   const int p = kP;
   if (k + 16 > p || n < 2 * p) return set_error(KLT_ERR_EIG, "psd subspace: shape not covered");
   const size_t np = (size_t)n * p, nk = (size_t)n * k, B = (size_t)count, pp = (size_t)p * p;
@@ -480,7 +481,7 @@ int eig_topk_psd_batch_device(klt_ctx* ctx, const double* a_dev, int count, int 
       if (!h_act[b]) continue;
       double mx = 0.0;
       for (int q = 0; q < k; ++q) mx = std::max(mx, h_res[(size_t)b * k + q]);
-      if (mx <= kTol * std::fabs(h_th[(size_t)b * k])) {
+      if (mx <= tol * std::fabs(h_th[(size_t)b * k])) {
         h_act[b] = 0;  // Y0 / Y1 of this matrix hold its result from here on
       } else {
         all = false;

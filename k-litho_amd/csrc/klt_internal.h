@@ -348,9 +348,16 @@ int eig_topk_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, i
 // The same for positive semidefinite matrices by block subspace iteration
 // (eig_subspace.hip); KLT_ERR_EIG when a matrix is outside its
 // reach (k + 16 > 64, n < 128, no convergence), for the caller to fall back.
-// warm (count 1): start from the block of the previous solve of order n.
+// warm (count 1): start from the block of the previous solve of order n;
+// tol: the residual target relative to the largest eigenvalue (0: 5e-14).
 int eig_topk_psd_batch_device(klt_ctx* ctx, const double* a_dev, int count, int n, int k,
-                              double* w_dev, double* vec_dev, bool warm = false);
+                              double* w_dev, double* vec_dev, bool warm = false,
+                              double tol = 0.0);
+// klt_calc_kernels on device pointers with the subspace solver's residual
+// target tol (0: the public 5e-14), for callers that need less, such as the
+// source-mask optimisation loop.
+int calc_kernels_device(klt_ctx* ctx, const double* tcc, int T, int nk, float* krn_r,
+                        float* krn_i, float* scales, unsigned flags, double tol);
 // Mixed-size, multi-set batch (socs_fused.hip): images[m * nbank + s] is mask
 // m under kernel set s.  Captured into a graph on first use, replayed after.
 int socs_forward_batch(klt_ctx* ctx, int nmask, const float* const* masks, const int* Lx,

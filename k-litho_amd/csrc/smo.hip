@@ -135,8 +135,11 @@ int klt_smo_run(klt_ctx* ctx, float* masks, const float* targets, int nmask, int
                           Ly, Nx, Ny, KLT_TCC_SPLIT_F16, tcc.as<double>(), 0));
     // the source stays non-negative, so the TCC is positive semidefinite; from
     // the second iteration on the previous eigensolve's block starts this one
-    KLT_TRY(klt_calc_kernels(ctx, tcc.as<double>(), T, K, kr.as<float>(), ki.as<float>(),
-                             sc.as<float>(), KLT_EIG_PSD | (it > 0 ? KLT_EIG_WARM : 0u)));
+    // (residuals to 1e-10 of the largest eigenvalue: the loop's gradients are
+    // good to 1e-4, and the public calls' 5e-14 costs about three more rounds)
+    KLT_TRY(klt::calc_kernels_device(ctx, tcc.as<double>(), T, K, kr.as<float>(),
+                                     ki.as<float>(), sc.as<float>(),
+                                     KLT_EIG_PSD | (it > 0 ? KLT_EIG_WARM : 0u), 1e-10));
     klt_kernels* kb = nullptr;
     KLT_TRY(klt_kernels_create(ctx, kr.as<float>(), ki.as<float>(), sc.as<float>(), K,
                                2 * Nx + 1, 2 * Ny + 1, 0, &kb));
