@@ -316,8 +316,9 @@ int klt_abbe_source_grad(klt_ctx* ctx, const float* masks, const float* targets,
  * iterations of: the TCC of the current source (klt_tcc_build, split-FP16
  * Gram, at the kernel half-size of a filled pupil, N = floor(NA L 2 / lambda),
  * so that the source may grow anywhere inside the unit disc), its
- * K = min(nk, T) kernels (klt_calc_kernels, KLT_EIG_PSD, and KLT_EIG_WARM
- * from the second iteration on), the loss and mask gradient g_m through the SOCS
+ * K = min(nk, T) kernels (as klt_calc_kernels with KLT_EIG_PSD, and
+ * KLT_EIG_WARM from the second iteration on, to residuals of 1e-10 of the
+ * largest eigenvalue), the loss and mask gradient g_m through the SOCS
  * model (klt_socs_backward), the source gradient g_S through the Abbe model
  * (klt_abbe_source_grad, summed over the masks), then
  *   m <- clip(m - step_mask g_m / max|g_m|, 0, 1)                    per mask,
