@@ -1697,7 +1697,7 @@ static int eig_dispatch(klt_ctx* ctx, const double* a_dev, int count, int n, int
 // klt_calc_kernels' body (klitho_tcc.cpp:301-377), with the subspace solver's
 // residual target tol (0: 5e-14).
 int calc_kernels_device(klt_ctx* ctx, const double* tcc, int T, int nk, float* krn_r,
-                        float* krn_i, float* scales, unsigned flags, double tol) {
+                        float* krn_i, float* scales, unsigned flags, double tol, bool sync) {
   if (!ctx || !tcc || !krn_r || !krn_i || !scales)
     return set_error(KLT_ERR_ARG, "klt_calc_kernels: NULL argument");
   if (T <= 0 || nk <= 0) return set_error(KLT_ERR_ARG, "klt_calc_kernels: bad size");
@@ -1727,7 +1727,8 @@ int calc_kernels_device(klt_ctx* ctx, const double* tcc, int T, int nk, float* k
   KLT_HIP(hipMemcpyAsync(krn_r, fr, kb, kind, ctx->stream));
   KLT_HIP(hipMemcpyAsync(krn_i, fi, kb, kind, ctx->stream));
   KLT_HIP(hipMemcpyAsync(scales, fs, nk * 4, kind, ctx->stream));
-  KLT_HIP(hipStreamSynchronize(ctx->stream));
+  // (sync false, device pointers: the results are ready in stream order)
+  if (sync || (flags & KLT_HOST_PTRS)) KLT_HIP(hipStreamSynchronize(ctx->stream));
   return KLT_OK;
 }
 

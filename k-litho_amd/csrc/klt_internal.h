@@ -357,7 +357,12 @@ int eig_topk_psd_batch_device(klt_ctx* ctx, const double* a_dev, int count, int 
 // target tol (0: the public 5e-14), for callers that need less, such as the
 // source-mask optimisation loop.
 int calc_kernels_device(klt_ctx* ctx, const double* tcc, int T, int nk, float* krn_r,
-                        float* krn_i, float* scales, unsigned flags, double tol);
+                        float* krn_i, float* scales, unsigned flags, double tol,
+                        bool sync = true);
+// New kernels and scales (device pointers, the bank's count and size) into an
+// existing bank without allocating (socs.hip).
+int kernels_refill(klt_kernels* kb, const float* krn_r, const float* krn_i,
+                   const float* scales);
 // Mixed-size, multi-set batch (socs_fused.hip): images[m * nbank + s] is mask
 // m under kernel set s.  Captured into a graph on first use, replayed after.
 int socs_forward_batch(klt_ctx* ctx, int nmask, const float* const* masks, const int* Lx,

@@ -129,6 +129,11 @@ int klt_smo_run(klt_ctx* ctx, float* masks, const float* targets, int nmask, int
   std::vector<float> h_loss(nmask), h_g((size_t)nmask * n2), gs(n2);
   std::vector<unsigned> h_cnt(cblocks);
   const unsigned blocks = (unsigned)std::min<size_t>((px + 255) / 256, 512);
+  struct Bank {  // destroyed on every way out of the loop
+    klt_kernels* kb = nullptr;
+    explicit operator bool() const { return kb != nullptr; }
+    ~Bank() { klt_kernels_destroy(kb); }
+  } bank;
   for (int it = 0; it <= steps; ++it) {
     KLT_HIP(hipMemcpyAsync(dsrc.ptr, src, (size_t)n2 * 4, hipMemcpyHostToDevice, ctx->stream));
     KLT_TRY(klt_tcc_build(ctx, dsrc.as<float>(), srcSize, 1.0f, lambda, NA, defocus, nmed, Lx,
@@ -139,14 +144,15 @@ int klt_smo_run(klt_ctx* ctx, float* masks, const float* targets, int nmask, int
     // good to 1e-4, and the public calls' 5e-14 costs about three more rounds)
     KLT_TRY(klt::calc_kernels_device(ctx, tcc.as<double>(), T, K, kr.as<float>(),
                                      ki.as<float>(), sc.as<float>(),
-                                     KLT_EIG_PSD | (it > 0 ? KLT_EIG_WARM : 0u), 1e-10));
-    klt_kernels* kb = nullptr;
-    KLT_TRY(klt_kernels_create(ctx, kr.as<float>(), ki.as<float>(), sc.as<float>(), K,
-                               2 * Nx + 1, 2 * Ny + 1, 0, &kb));
-    const int rc = klt_socs_backward(ctx, kb, masks, targets, nmask, Lx, Ly, a_sig, thr,
-                                     grad.as<float>(), loss.as<float>(), img.as<float>(), 0);
-    klt_kernels_destroy(kb);
-    KLT_TRY(rc);
+                                     KLT_EIG_PSD | (it > 0 ? KLT_EIG_WARM : 0u), 1e-10, false));
+    // one kernel bank for the loop, refilled in place after the first step
+    if (!bank)
+      KLT_TRY(klt_kernels_create(ctx, kr.as<float>(), ki.as<float>(), sc.as<float>(), K,
+                                 2 * Nx + 1, 2 * Ny + 1, 0, &bank.kb));
+    else
+      KLT_TRY(klt::kernels_refill(bank.kb, kr.as<float>(), ki.as<float>(), sc.as<float>()));
+    KLT_TRY(klt_socs_backward(ctx, bank.kb, masks, targets, nmask, Lx, Ly, a_sig, thr,
+                              grad.as<float>(), loss.as<float>(), img.as<float>(), 0));
     KLT_LAUNCH(ctx, "k_smo_mismatch", k_smo_mismatch, dim3(cblocks), dim3(256), 0,
                img.as<float>(), targets, tot, thr, cnt.as<unsigned>());
     KLT_HIP(hipMemcpyAsync(h_loss.data(), loss.ptr, (size_t)nmask * 4, hipMemcpyDeviceToHost,

@@ -802,3 +802,22 @@ int klt_socs_image(klt_ctx* ctx, const klt_kernels* kb, const float* mask, int L
 }
 
 }  // extern "C"
+
+namespace klt {
+
+// New kernels and scales (device pointers, same count and size) into an
+// existing bank, on the context stream, without allocating (the bank's host
+// copy of the scales is left as it was: no path reads it).
+int kernels_refill(klt_kernels* kb, const float* krn_r, const float* krn_i,
+                   const float* scales) {
+  klt_ctx* ctx = kb->ctx;
+  const size_t n = (size_t)kb->nk * (2 * kb->Nx + 1) * (2 * kb->Ny + 1);
+  hipLaunchKernelGGL(k_pack_kernels, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     ctx->stream, krn_r, krn_i, (int)n, kb->kern.as<float2>());
+  KLT_HIP(hipGetLastError());
+  KLT_HIP(hipMemcpyAsync(kb->scales.ptr, scales, kb->nk * sizeof(float),
+                         hipMemcpyDeviceToDevice, ctx->stream));
+  return KLT_OK;
+}
+
+}  // namespace klt
