@@ -740,14 +740,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline and plain path only")
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--section", choices=["all", "eig_psd", "tcc_split16", "adjoint", "wide"],
+    ap.add_argument("--section", choices=["all", "eig_psd", "tcc_split16", "adjoint", "wide",
+                                         "batch_uneven"],
                     default="all",
                     help="eig_psd: only --steps top-32 solves of the T = 1225 blurred-dipole "
                          "TCC on the positive-semidefinite path; tcc_split16: only --steps "
                          "split16 builds of that TCC (no defocus); adjoint: only --steps "
                          "klt_socs_backward calls on --batch 1024^2 masks; wide: only the "
-                         "kernel half-size 16 and 31 rows of the wide path; their timing as a "
-                         "JSON line (for profiling; no bench line)")
+                         "kernel half-size 16 and 31 rows of the wide path; batch_uneven: only "
+                         "the 192-image mixed batch; their timing as a JSON line (for "
+                         "profiling; no bench line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the SOCS "
@@ -771,6 +773,9 @@ def main():
         return
     if args.section == "wide":
         print(json.dumps(bench_wide(ctx, timer, torch, dev)))
+        return
+    if args.section == "batch_uneven":
+        print(json.dumps(bench_batch_uneven(ctx, timer, torch, gpu_kernels(ctx)[0])))
         return
     if args.section != "all":
         fn = section_eig_psd if args.section == "eig_psd" else section_tcc_split16

@@ -111,9 +111,11 @@ struct MidJob {         // one mask x one kernel set
   const float2* kern;   // the set's kernels, [nk][2Ny+1][2Nx+1]
   const float* scales;  // and eigenvalues
   float2* spec;         // the image's intensity spectrum, [32][4Nx+1] (64 rows for N > 15)
+  float* ipart;         // kg > 1: the image's partial coarse intensities, [kg][64 * 64]
   int nblk, Nx, Ny, nk;
   float inv_area;
-  int pad_[3];
+  int k0, k1, grp, kg;  // this block's kernels [k0, k1), group grp of kg
+  int img;              // the image's index in the batch
 };
 struct SynthJob {       // one 128-column strip of one image
   const float2* spec;
@@ -132,11 +134,12 @@ struct SynthJob {       // one 128-column strip of one image
 struct BatchGraph {
   std::vector<uint64_t> key;
   int nimg = 0, nband = 0, nmid = 0, nsynth = 0;
+  int kg = 1;         // kernel groups per image in the narrow coarse stage
   bool wide = false;  // kernel half-size above 15 (the wide passes)
   int lds_mid = 0;    // dynamic LDS of the wide coarse stage
   bool full = true;
   bool held = false;
-  DevBuf band_jobs, mid_jobs, synth_jobs, mpart, spec, strips_done, done_at, t0;
+  DevBuf band_jobs, mid_jobs, synth_jobs, mpart, spec, strips_done, done_at, t0, ipart;
   const float2* spec_tw = nullptr;  // e^{+2 pi i j/64} (j/128 for N > 15) of the coarse FFTs
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;

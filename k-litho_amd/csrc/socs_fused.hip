@@ -429,16 +429,19 @@ __global__ __launch_bounds__(256, !FULL ? 2 : (RESIST || NCB > 2) ? 3 : 4) void 
   const int ch0 = min(nchunk, kh * per), ch1 = min(nchunk, ch0 + per);
   constexpr int CU = kBandChunk / band_chunk(NCB);  // twiddle chunks per range unit
   const size_t ib = (size_t)b * Lx * Ly;
-  band_block<NCB, RESIST, FULL, RT>(blk, CU * ch0, CU * ch1, px, in0 + ib, RESIST ? in1 + ib : nullptr, a_sig,
-                                thr, Lx, Ly, Kpad, Wf, twy, ldtw, Bx, By, ldo,
-                                out_part + (size_t)b * parts * (2 * By + 1) * ldo,
-                                RESIST ? lpart + (size_t)b * parts : nullptr);
+  band_block<NCB, RESIST, FULL, RT>(blk, CU * ch0, CU * ch1, px, in0 + ib,
+                                    RESIST ? in1 + ib : nullptr, a_sig, thr, Lx, Ly, Kpad, Wf,
+                                    twy, ldtw, Bx, By, ldo,
+                                    out_part + (size_t)b * parts * (2 * By + 1) * ldo,
+                                    RESIST ? lpart + (size_t)b * parts : nullptr);
 }
 
-// Mixed sizes: one block per BandJob (a row block of one mask).
+// Mixed sizes: one block per BandJob (a row block of one mask).  Block 0
+// stamps the batch's start on the device wall clock.
 template <bool FULL>
 __global__ __launch_bounds__(256, FULL ? 4 : 2) void k_band_jobs(
-    const klt::BandJob* __restrict__ jobs) {
+    const klt::BandJob* __restrict__ jobs, unsigned long long* __restrict__ t0) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *t0 = wall_clock64();
   const klt::BandJob j = jobs[blockIdx.x];
   band_block<2, false, FULL>(j.blk, 0, j.Kpad / (kBandChunk * 32), j.blk, j.in, nullptr, 0.f,
                              0.f, j.Lx, j.Ly, j.Kpad, j.Wf, j.twy, j.ldtw, j.Nx, j.Ny, 16, j.out,
@@ -448,7 +451,9 @@ __global__ __launch_bounds__(256, FULL ? 4 : 2) void k_band_jobs(
 // The same for kernel half-sizes 16..31: 64 twiddle rows, 32 columns a row
 // of partials.
 template <bool FULL>
-__global__ __launch_bounds__(256, 3) void k_band_jobs_wide(const klt::BandJob* __restrict__ jobs) {
+__global__ __launch_bounds__(256, 3) void k_band_jobs_wide(const klt::BandJob* __restrict__ jobs,
+                                                             unsigned long long* __restrict__ t0) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *t0 = wall_clock64();
   const klt::BandJob j = jobs[blockIdx.x];
   band_block<4, false, FULL>(j.blk, 0, j.Kpad / (band_chunk(4) * 32), j.blk, j.in, nullptr, 0.f,
                              0.f, j.Lx, j.Ly, j.Kpad, j.Wf, j.twy, j.ldtw, j.Nx, j.Ny, 32, j.out,
@@ -775,13 +780,13 @@ __global__ __launch_bounds__(256) void k_socs_mid(const float2* __restrict__ mpa
 
 // Sums the kg partial coarse intensities of each image and transforms them
 // to the intensity spectrum [B][32][4Nx+1].
-__global__ __launch_bounds__(256) void k_socs_mid_fin(const float* __restrict__ ipart, int kg,
-                                                      int Nx, int Ny,
-                                                      const float2* __restrict__ twM,
-                                                      float2* __restrict__ spec) {
+// One image: ipart [kg][64 * 64] -> spec [32][4Nx+1].
+__device__ __forceinline__ void mid_fin(const float* __restrict__ ipart, int kg, int Nx, int Ny,
+                                        const float2* __restrict__ twM,
+                                        float2* __restrict__ spec) {
   __shared__ float2 sB[31 * kSB];
   __shared__ float2 sX[32 * kXS];
-  const int b = blockIdx.x, tid = threadIdx.x, g = tid >> 3, t = tid & 7;
+  const int tid = threadIdx.x, g = tid >> 3, t = tid & 7;
   float acc[2][8];
 #pragma unroll
   for (int q = 0; q < 2; ++q)
@@ -790,7 +795,7 @@ __global__ __launch_bounds__(256) void k_socs_mid_fin(const float* __restrict__ 
   // (unrolled: four groups' loads issue together, the sums keep their order)
 #pragma unroll 4
   for (int j = 0; j < kg; ++j) {
-    const float* src = ipart + ((size_t)b * kg + j) * kM * kM + tid * 16;
+    const float* src = ipart + (size_t)j * kM * kM + tid * 16;
 #pragma unroll
     for (int q = 0; q < 2; ++q)
 #pragma unroll
@@ -802,7 +807,15 @@ __global__ __launch_bounds__(256) void k_socs_mid_fin(const float* __restrict__ 
     const float2 w = twM[(t * k2) & 63];
     twf[k2] = make_float2(w.x, -w.y);
   }
-  coarse_to_spec(acc, sB, sX + g * kXS, twf, Nx, Ny, spec + (size_t)b * 32 * (4 * Nx + 1), 32);
+  coarse_to_spec(acc, sB, sX + g * kXS, twf, Nx, Ny, spec, 32);
+}
+
+__global__ __launch_bounds__(256) void k_socs_mid_fin(const float* __restrict__ ipart, int kg,
+                                                      int Nx, int Ny,
+                                                      const float2* __restrict__ twM,
+                                                      float2* __restrict__ spec) {
+  const int b = blockIdx.x;
+  mid_fin(ipart + (size_t)b * kg * kM * kM, kg, Nx, Ny, twM, spec + (size_t)b * 32 * (4 * Nx + 1));
 }
 
 // FP32 single-image path: one block per image from its whole band
@@ -817,12 +830,26 @@ __global__ __launch_bounds__(256) void k_socs_mid_band(const float2* __restrict_
                   0, nk, twM, spec + (size_t)b * G * (4 * Nx + 1), G, nullptr);
 }
 
-// Mixed sizes and kernel sets: one block per MidJob (one mask x one set).
+// Mixed sizes and kernel sets: one block per MidJob (kernels [k0, k1) of one
+// mask x one set).  With kg > 1 groups per image the block writes its
+// partial coarse intensity and k_socs_mid_jobs_fin finishes the image.
+// Group 0 clears the image's strip counter for the synthesis stage.
 __global__ __launch_bounds__(256) void k_socs_mid_jobs(const klt::MidJob* __restrict__ jobs,
-                                                       const float2* __restrict__ twM) {
+                                                       const float2* __restrict__ twM,
+                                                       unsigned* __restrict__ strips_done) {
   const klt::MidJob j = jobs[blockIdx.x];
-  mid_image<false>(j.mpart, j.nblk, j.Nx, j.Ny, j.inv_area, j.kern, j.scales, 0, j.nk, twM,
-                   j.spec, 32, nullptr);
+  if (j.grp == 0 && threadIdx.x == 0) strips_done[j.img] = 0u;
+  mid_image<false>(j.mpart, j.nblk, j.Nx, j.Ny, j.inv_area, j.kern, j.scales, j.k0, j.k1, twM,
+                   j.spec, 32, j.kg > 1 ? j.ipart + (size_t)j.grp * kM * kM : nullptr);
+}
+
+// grid (images): sums image b's kg partials (jobs b kg .. b kg + kg - 1) in
+// group order and transforms them to its spectrum.
+__global__ __launch_bounds__(256) void k_socs_mid_jobs_fin(const klt::MidJob* __restrict__ jobs,
+                                                           int kg,
+                                                           const float2* __restrict__ twM) {
+  const klt::MidJob j = jobs[(size_t)blockIdx.x * kg];
+  mid_fin(j.ipart, kg, j.Nx, j.Ny, twM, j.spec);
 }
 
 // ---------------------------------------------------------------------------
@@ -1761,8 +1788,10 @@ __global__ __launch_bounds__(256, 2) void k_socs_mid128(const float2* __restrict
 // Mixed sizes and kernel sets, half-sizes 16..31: one block per MidJob (one
 // mask x one set; the mask's band partials have 32 columns a row).
 __global__ __launch_bounds__(256) void k_socs_mid128_jobs(const klt::MidJob* __restrict__ jobs,
-                                                          const float2* __restrict__ tw128) {
+                                                          const float2* __restrict__ tw128,
+                                                          unsigned* __restrict__ strips_done) {
   const klt::MidJob j = jobs[blockIdx.x];
+  if (threadIdx.x == 0) strips_done[j.img] = 0u;  // for the synthesis stage
   mid128_image<true>(j.mpart, j.nblk, j.inv_area, j.Nx, j.Ny, j.kern, j.scales, 0, j.nk, tw128,
                      j.spec, nullptr);
 }
@@ -2727,15 +2756,6 @@ __global__ __launch_bounds__(256) void k_synth_jobs(const klt::SynthJob* __restr
     if (atomicAdd(&strips_done[j.image], 1u) + 1u == (unsigned)j.nstrips)
       done_at[j.image] = wall_clock64();
   }
-}
-
-// Graph head of a mixed batch: clears the strip counters (a fill kernel, so
-// that graph replays start from zero) and records the start time.
-__global__ void k_batch_begin(unsigned* __restrict__ strips_done, int n,
-                              unsigned long long* __restrict__ t0) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    strips_done[i] = 0u;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *t0 = wall_clock64();
 }
 
 // ---------------------------------------------------------------------------
@@ -3794,8 +3814,9 @@ BatchGraph::~BatchGraph() {
   if (graph) (void)hipGraphDestroy(graph);
 }
 
-// Builds the job tables of a mixed batch and records its four launches on the
-// context stream (into a graph while capturing).
+// Builds the job tables of a mixed batch; launch_batch records its three
+// stages (four launches when the coarse stage splits an image's kernels) on
+// the context stream (into a graph while capturing).
 static int build_batch(klt_ctx* ctx, BatchGraph* bg, int nmask, const float* const* masks,
                        const int* Lx, const int* Ly, int nbank,
                        const klt_kernels* const* banks, float* const* images) {
@@ -3813,7 +3834,21 @@ static int build_batch(klt_ctx* ctx, BatchGraph* bg, int nmask, const float* con
   bg->nimg = nmask * nbank;
   KLT_TRY(bg->mpart.ensure(mtot * sizeof(float2)));
   KLT_TRY(bg->spec.ensure((size_t)bg->nimg * rows * Gx * sizeof(float2)));
+  // Narrow coarse stage: an image's kernels split over kg groups so that the
+  // batch's coarse blocks fill about two per CU (its LDS admits two).
+  int ncu = 256;
+  KLT_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  const int nk_min = [&] {
+    int v = banks[0]->nk;
+    for (int s = 1; s < nbank; ++s) v = std::min(v, banks[s]->nk);
+    return v;
+  }();
+  const int kg = bg->wide ? 1
+                          : std::max(1, std::min((nk_min + 3) / 4,
+                                                 (2 * ncu + bg->nimg - 1) / bg->nimg));
+  bg->kg = kg;
   KLT_TRY(bg->strips_done.ensure((size_t)bg->nimg * sizeof(unsigned)));
+  if (kg > 1) KLT_TRY(bg->ipart.ensure((size_t)bg->nimg * kg * kM * kM * sizeof(float)));
   KLT_TRY(bg->done_at.ensure((size_t)bg->nimg * sizeof(unsigned long long)));
   KLT_TRY(bg->t0.ensure(sizeof(unsigned long long)));
   std::vector<BandJob> bj;
@@ -3837,9 +3872,14 @@ static int build_batch(klt_ctx* ctx, BatchGraph* bg, int nmask, const float* con
     for (int s = 0; s < nbank; ++s) {
       const int img = m * nbank + s;
       float2* sp = bg->spec.as<float2>() + (size_t)img * rows * Gx;
-      mj.push_back(MidJob{mp, banks[s]->kern.as<float2>(), banks[s]->scales.as<float>(), sp,
-                          nblk, Nx, Ny, banks[s]->nk,
-                          1.0f / ((float)Lx[m] * (float)Ly[m]), {0, 0, 0}});
+      const int nk = banks[s]->nk, per = (nk + kg - 1) / kg;
+      for (int gi = 0; gi < kg; ++gi)
+        mj.push_back(MidJob{mp, banks[s]->kern.as<float2>(), banks[s]->scales.as<float>(), sp,
+                            kg > 1 ? bg->ipart.as<float>() + (size_t)img * kg * kM * kM
+                                   : nullptr,
+                            nblk, Nx, Ny, nk,
+                            1.0f / ((float)Lx[m] * (float)Ly[m]), std::min(nk, gi * per),
+                            std::min(nk, (gi + 1) * per), gi, kg, img});
       for (int st = 0; st < nstrips; ++st)
         sj.push_back(bg->wide
                          ? SynthJob{sp, ft->Efw.as<float4>(), ft->TYw.as<_Float16>(), nullptr,
@@ -3873,18 +3913,15 @@ static int build_batch(klt_ctx* ctx, BatchGraph* bg, int nmask, const float* con
 
 // The batch's launches, in stream order.
 static void launch_batch(klt_ctx* ctx, const BatchGraph* bg) {
-  hipLaunchKernelGGL(k_batch_begin, dim3(std::max(1, std::min(64, (bg->nimg + 255) / 256))),
-                     dim3(256), 0, ctx->stream, bg->strips_done.as<unsigned>(), bg->nimg,
-                     bg->t0.as<unsigned long long>());
   if (bg->wide) {
     if (bg->full)
       hipLaunchKernelGGL(k_band_jobs_wide<true>, dim3(bg->nband), dim3(256), 0, ctx->stream,
-                         bg->band_jobs.as<BandJob>());
+                         bg->band_jobs.as<BandJob>(), bg->t0.as<unsigned long long>());
     else
       hipLaunchKernelGGL(k_band_jobs_wide<false>, dim3(bg->nband), dim3(256), 0, ctx->stream,
-                         bg->band_jobs.as<BandJob>());
+                         bg->band_jobs.as<BandJob>(), bg->t0.as<unsigned long long>());
     hipLaunchKernelGGL(k_socs_mid128_jobs, dim3(bg->nmid), dim3(256), bg->lds_mid, ctx->stream,
-                       bg->mid_jobs.as<MidJob>(), bg->spec_tw);
+                       bg->mid_jobs.as<MidJob>(), bg->spec_tw, bg->strips_done.as<unsigned>());
     hipLaunchKernelGGL(k_synth_wide_jobs, dim3(bg->nsynth), dim3(256), 0, ctx->stream,
                        bg->synth_jobs.as<SynthJob>(), bg->strips_done.as<unsigned>(),
                        bg->done_at.as<unsigned long long>());
@@ -3892,12 +3929,15 @@ static void launch_batch(klt_ctx* ctx, const BatchGraph* bg) {
   }
   if (bg->full)
     hipLaunchKernelGGL(k_band_jobs<true>, dim3(bg->nband), dim3(256), 0, ctx->stream,
-                       bg->band_jobs.as<BandJob>());
+                       bg->band_jobs.as<BandJob>(), bg->t0.as<unsigned long long>());
   else
     hipLaunchKernelGGL(k_band_jobs<false>, dim3(bg->nband), dim3(256), 0, ctx->stream,
-                       bg->band_jobs.as<BandJob>());
+                       bg->band_jobs.as<BandJob>(), bg->t0.as<unsigned long long>());
   hipLaunchKernelGGL(k_socs_mid_jobs, dim3(bg->nmid), dim3(256), 0, ctx->stream,
-                     bg->mid_jobs.as<MidJob>(), bg->spec_tw);
+                     bg->mid_jobs.as<MidJob>(), bg->spec_tw, bg->strips_done.as<unsigned>());
+  if (bg->kg > 1)
+    hipLaunchKernelGGL(k_socs_mid_jobs_fin, dim3(bg->nimg), dim3(256), 0, ctx->stream,
+                       bg->mid_jobs.as<MidJob>(), bg->kg, bg->spec_tw);
   hipLaunchKernelGGL(k_synth_jobs, dim3(bg->nsynth), dim3(256), 0, ctx->stream,
                      bg->synth_jobs.as<SynthJob>(), bg->strips_done.as<unsigned>(),
                      bg->done_at.as<unsigned long long>());
